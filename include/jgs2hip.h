@@ -1,0 +1,242 @@
+/*
+ * libjgs2hip -- MI355X (gfx950) implementation of the JGS2 per-frame step.
+ *
+ * This C ABI is the drop-in boundary for the reference's WebGPU solver class
+ * `JGS2GpuSolver` (src/simulation/gpu/jgs2-solver.ts:905-3051). Each entry
+ * point names the reference member it replaces. Plain pointers and sizes
+ * only; every function returns a jgs2_status and never throws across the
+ * boundary. The calling thread's last error message is available from
+ * jgs2_last_error().
+ *
+ * Ownership: jgs2_create() deep-copies every input array into device memory
+ * the handle owns (jgs2-solver.ts:389-408). Read functions copy into
+ * caller-owned host buffers (jgs2-solver.ts:2996-3027). jgs2_destroy() frees
+ * everything; any call on a destroyed handle is an error
+ * (jgs2-solver.ts:3046-3050).
+ *
+ * Threading: a handle owns one HIP stream. Step and objective calls only
+ * enqueue work; reads and jgs2_sync() wait for it. Calls on one handle must
+ * not overlap from several threads.
+ */
+#ifndef JGS2HIP_H
+#define JGS2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference throws RangeError/TypeError/Error
+ * synchronously before any GPU work (layout.ts:223-416,
+ * jgs2-solver.ts:429-529) and rejects asynchronously on an infeasible
+ * source pose (jgs2-solver.ts:873-889); the codes mirror those classes. */
+typedef enum {
+  JGS2_OK = 0,
+  JGS2_ERROR_RANGE = 1,      /* RangeError: invalid sizes, values, limits */
+  JGS2_ERROR_TYPE = 2,       /* TypeError: malformed arguments, NULL */
+  JGS2_ERROR_STATE = 3,      /* use after destroy, unsupported mode */
+  JGS2_ERROR_DEVICE = 4,     /* HIP runtime failure */
+  JGS2_ERROR_INFEASIBLE = 5, /* initial pose fails the f32 feasibility gate */
+  JGS2_ERROR_MEMORY = 6      /* device or host allocation failed */
+} jgs2_status;
+
+/* Material tags stored in tet_meta[4*t+3] (layout.ts:28-29). */
+#define JGS2_MATERIAL_COROTATED_LINEAR 0
+#define JGS2_MATERIAL_STABLE_NEO_HOOKEAN 1
+
+/* Record widths of the reference ABI (layout.ts:16-27). */
+#define JGS2_CUBATURE_BASIS_FLOATS 36
+#define JGS2_REST_STIFFNESS_FLOATS 144
+#define JGS2_GLOBALIZATION_HISTORY_CAPACITY 64
+#define JGS2_MAX_BATCH_FRAMES 240
+#define JGS2_MAX_GLOBALIZED_ITERATIONS_PER_SUBMISSION 2048
+
+/*
+ * Scene input: the arrays of `JGS2GpuInput` (layout.ts:41-90), same element
+ * order and meaning. Optional arrays may be NULL.
+ */
+typedef struct {
+  uint32_t vertex_count;
+  uint32_t tet_count;
+  uint32_t cubature_k;
+  const float* positions;          /* 4 per vertex, xyz position, w = 1 */
+  const float* velocities;         /* optional, 4 per vertex */
+  const float* vertex_rest;        /* 4 per vertex: rest xyz, mass */
+  const uint32_t* vertex_info;     /* 4 per vertex: adjStart, adjCount, pinned, body */
+  const uint32_t* tet_indices;     /* 4 per tet */
+  const float* tet_inverse_dm;     /* 12 per tet: three padded columns of Dm^-1 */
+  const float* tet_meta;           /* 4 per tet: volume, lambda, mu, material tag */
+  const float* tet_rest_stiffness; /* optional for Neo-Hookean scenes; 144 per tet row-major */
+  const uint32_t* adjacency;       /* canonical vertex->tet CSR payload */
+  uint64_t adjacency_length;
+  const uint32_t* cubature_tet_ids; /* vertex_count * cubature_k, 0xffffffff = empty */
+  const float* cubature_weights;    /* vertex_count * cubature_k */
+  const float* cubature_basis;      /* vertex_count * cubature_k * 36, row-major Ubar blocks */
+  const float* external_forces;     /* optional, 3 per vertex */
+  const float* target_positions;    /* optional, 3 per vertex (with target_stiffnesses) */
+  const float* target_stiffnesses;  /* optional, 1 per vertex */
+} jgs2_scene_input;
+
+/* Schedules (jgs2-solver.ts:62). */
+#define JGS2_SCHEDULE_JACOBI 0
+#define JGS2_SCHEDULE_GRAPH_COLORED_GAUSS_SEIDEL 1
+
+/*
+ * `JGS2StepSettings` (jgs2-solver.ts:64-100). Values are doubles on the
+ * boundary and rounded to f32 when packed, as the reference's uniform
+ * packer does (jgs2-solver.ts:604-730).
+ */
+typedef struct {
+  int32_t schedule;
+  double timestep;
+  double gravity[3];
+  int32_t iterations;
+  double floor_height;
+  double floor_stiffness;
+  double velocity_damping;
+  double regularization;
+  double rotation_epsilon;
+  double max_step;
+  double residual_tolerance;
+  double normalized_update_tolerance;
+  double contact_tangential_damping;
+  double contact_margin;
+  double ipc_activation_distance;
+  double ipc_minimum_distance;
+  double ipc_barrier_stiffness;
+  double ipc_friction_coefficient;
+  double ipc_friction_velocity_epsilon;
+  double ipc_step_safety;
+  int32_t horizontal_body_correction;
+  int32_t parity_mode;
+} jgs2_settings;
+
+/* Override mask bits: a set bit means "use this field of the passed struct",
+ * the equivalent of a key present in the reference's Partial<> overrides. */
+enum {
+  JGS2_SET_SCHEDULE = 1u << 0,
+  JGS2_SET_TIMESTEP = 1u << 1,
+  JGS2_SET_GRAVITY = 1u << 2,
+  JGS2_SET_ITERATIONS = 1u << 3,
+  JGS2_SET_FLOOR_HEIGHT = 1u << 4,
+  JGS2_SET_FLOOR_STIFFNESS = 1u << 5,
+  JGS2_SET_VELOCITY_DAMPING = 1u << 6,
+  JGS2_SET_REGULARIZATION = 1u << 7,
+  JGS2_SET_ROTATION_EPSILON = 1u << 8,
+  JGS2_SET_MAX_STEP = 1u << 9,
+  JGS2_SET_RESIDUAL_TOLERANCE = 1u << 10,
+  JGS2_SET_NORMALIZED_UPDATE_TOLERANCE = 1u << 11,
+  JGS2_SET_CONTACT_TANGENTIAL_DAMPING = 1u << 12,
+  JGS2_SET_CONTACT_MARGIN = 1u << 13,
+  JGS2_SET_IPC_ACTIVATION_DISTANCE = 1u << 14,
+  JGS2_SET_IPC_MINIMUM_DISTANCE = 1u << 15,
+  JGS2_SET_IPC_BARRIER_STIFFNESS = 1u << 16,
+  JGS2_SET_IPC_FRICTION_COEFFICIENT = 1u << 17,
+  JGS2_SET_IPC_FRICTION_VELOCITY_EPSILON = 1u << 18,
+  JGS2_SET_IPC_STEP_SAFETY = 1u << 19,
+  JGS2_SET_HORIZONTAL_BODY_CORRECTION = 1u << 20,
+  JGS2_SET_PARITY_MODE = 1u << 21
+};
+
+typedef struct jgs2_solver jgs2_solver;
+
+/* DEFAULT_JGS2_STEP_SETTINGS (jgs2-solver.ts:164-187). */
+void jgs2_default_settings(jgs2_settings* out);
+
+/* Message of the calling thread's last failed call ("" after success). */
+size_t jgs2_last_error(char* buffer, size_t capacity);
+
+/* Number of visible HIP devices (0 without a GPU). */
+int jgs2_device_count(void);
+
+/* JGS2GpuSolver.create (jgs2-solver.ts:971-1502): validate, color, pack,
+ * upload and run the f32 source-feasibility preflight. `overrides` may be
+ * NULL (mask ignored). */
+jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* overrides, uint64_t override_mask,
+                        jgs2_solver** out);
+
+/* JGS2GpuSolver.destroy (jgs2-solver.ts:3029-3044). NULL is a no-op. */
+jgs2_status jgs2_destroy(jgs2_solver* solver);
+
+/* step / stepFrames (jgs2-solver.ts:1767-1812): odd-normalized iteration
+ * count and the convergence stop latch. */
+jgs2_status jgs2_step(jgs2_solver* solver, const jgs2_settings* overrides, uint64_t override_mask);
+jgs2_status jgs2_step_frames(jgs2_solver* solver, uint32_t frame_count, const jgs2_settings* overrides,
+                             uint64_t override_mask);
+
+/* stepExactIterations / stepFramesExactIterations (jgs2-solver.ts:1814-1862):
+ * exactly `iterations` sweeps per frame, even counts kept, no stop latch. */
+jgs2_status jgs2_step_exact_iterations(jgs2_solver* solver, uint32_t iterations, const jgs2_settings* overrides,
+                                       uint64_t override_mask);
+jgs2_status jgs2_step_frames_exact_iterations(jgs2_solver* solver, uint32_t frame_count, uint32_t iterations,
+                                              const jgs2_settings* overrides, uint64_t override_mask);
+
+/* Objective mutation API (jgs2-solver.ts:1576-1760). */
+jgs2_status jgs2_set_external_force(jgs2_solver* solver, uint32_t vertex, const float force[3]);
+jgs2_status jgs2_replace_external_forces(jgs2_solver* solver, const float* forces, size_t count);
+jgs2_status jgs2_clear_external_forces(jgs2_solver* solver);
+jgs2_status jgs2_set_quadratic_target(jgs2_solver* solver, uint32_t vertex, const float target[3],
+                                      float stiffness);
+jgs2_status jgs2_replace_quadratic_targets(jgs2_solver* solver, const float* positions, size_t position_count,
+                                           const float* stiffnesses, size_t stiffness_count);
+jgs2_status jgs2_release_quadratic_target(jgs2_solver* solver, uint32_t vertex);
+jgs2_status jgs2_release_all_quadratic_targets(jgs2_solver* solver);
+
+/* Reads (jgs2-solver.ts:2501-2604). `count` is the float capacity of `dst`;
+ * it must equal the exact size stated. All wait for queued work. */
+jgs2_status jgs2_read_positions(jgs2_solver* solver, float* dst, size_t count);        /* 4 * V */
+jgs2_status jgs2_read_velocities(jgs2_solver* solver, float* dst, size_t count);       /* 4 * V */
+jgs2_status jgs2_read_predicted_positions(jgs2_solver* solver, float* dst, size_t count); /* 4 * V */
+jgs2_status jgs2_read_final_iteration_updates(jgs2_solver* solver, float* dst, size_t count); /* 4 * V */
+jgs2_status jgs2_read_vertex_rotations(jgs2_solver* solver, float* dst, size_t count); /* 12 * V */
+jgs2_status jgs2_read_local_diagnostics(jgs2_solver* solver, float* dst, size_t count); /* 16 * V */
+jgs2_status jgs2_read_globalization_control(jgs2_solver* solver, float* dst, size_t count); /* 16 */
+jgs2_status jgs2_read_globalization_history(jgs2_solver* solver, float* dst, size_t count); /* 64 * 32 */
+
+/* awaitIdle (jgs2-solver.ts:2496-2499). */
+jgs2_status jgs2_sync(jgs2_solver* solver);
+
+/* Scene facts the reference exposes as readonly members
+ * (jgs2-solver.ts:906-916). */
+typedef struct {
+  uint32_t vertex_count;
+  uint32_t tet_count;
+  uint32_t cubature_k;
+  uint32_t body_count;
+  uint32_t schedule_color_count;
+  uint32_t globalization_enabled;
+  uint32_t free_vertex_count;
+  float scene_scale;
+} jgs2_solver_info;
+jgs2_status jgs2_get_info(jgs2_solver* solver, jgs2_solver_info* out);
+
+/* Greedy coloring of the schedule's conflict graph
+ * (buildJGS2InputVertexColoring, jgs2-solver.ts:308-367, over
+ * buildGreedyVertexColoring, src/simulation/cpu/vertex-coloring.ts:20-110).
+ * `colors` receives V entries. */
+jgs2_status jgs2_read_schedule_colors(jgs2_solver* solver, uint32_t* colors, size_t count);
+
+/*
+ * Test hook in the spirit of readOracleDiagnostics (jgs2-solver.ts:2882-2907):
+ * evaluates the local solve's gradient (3 per vertex) and Hessian (9 per
+ * vertex, row-major) at the current accepted pose with vertex frames
+ * refreshed there, using the given settings. Pinned vertices report zeros.
+ */
+jgs2_status jgs2_debug_local_systems(jgs2_solver* solver, const jgs2_settings* overrides, uint64_t override_mask,
+                                     float* gradients, size_t gradient_count, float* hessians,
+                                     size_t hessian_count);
+
+/* Average device time per launch of each kernel over the last
+ * `jgs2_profile_*` window, measured with HIP events on the solver's stream.
+ * Used by the benchmark to report the local-solve kernel's bandwidth. */
+jgs2_status jgs2_profile_begin(jgs2_solver* solver);
+jgs2_status jgs2_profile_end(jgs2_solver* solver, double* local_solve_ms_per_launch, uint64_t* local_solve_launches,
+                             double* frame_ms_total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JGS2HIP_H */

@@ -1,0 +1,76 @@
+/*
+ * libjgs2hip host scene setup: the reference's CPU scene path
+ * (src/scenes/index.ts buildScene/toJGS2GpuInput over src/simulation/cpu
+ * mesh.ts and fem.ts) for the synthetic workloads the step is measured on.
+ * Pure host code; usable without a GPU.
+ */
+#ifndef JGS2HIP_SCENE_H
+#define JGS2HIP_SCENE_H
+
+#include "jgs2hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jgs2_scene jgs2_scene;
+
+/*
+ * A regular cuboid split into six Kuhn tetrahedra per hexahedron
+ * (generateRegularCuboidMesh, src/simulation/cpu/mesh.ts:101-200), with
+ * optional seeded vertex jitter, alternating material layers and one pinned
+ * face, packed like packJGS2GpuInput (src/scenes/index.ts:975-1109).
+ */
+typedef struct {
+  uint32_t cells[3];
+  double origin[3];
+  double size[3];
+  /* Each coordinate moves by U(-f*h_axis, f*h_axis) drawn from `seed`
+   * (0 = no jitter). */
+  double jitter_fraction;
+  uint32_t seed;
+  /* Vertices with grid index 0 along this axis are pinned (3 = none). */
+  uint32_t pin_axis;
+  int32_t material_model; /* JGS2_MATERIAL_* */
+  double density;
+  double poisson_ratio;
+  /* Young's modulus of even and odd layers; layers are `layer_cells` cells
+   * thick along `layer_axis` (layer_cells = 0: one material). */
+  double young_modulus[2];
+  uint32_t layer_axis;
+  uint32_t layer_cells;
+  /* Cubature records per vertex (0 = none). */
+  uint32_t cubature_k;
+  /* Pack the 12x12 co-rotated rest stiffness (needed by the co-rotated
+   * path only; 576 bytes per tetrahedron). */
+  int32_t pack_rest_stiffness;
+} jgs2_cuboid_desc;
+
+jgs2_status jgs2_scene_build_cuboid(const jgs2_cuboid_desc* desc, jgs2_scene** out);
+
+/* Pointers into the scene's arrays, valid until jgs2_scene_destroy(). */
+jgs2_status jgs2_scene_get_input(const jgs2_scene* scene, jgs2_scene_input* out);
+
+/* Attach per-vertex objectives (3 floats per vertex forces, 3 per vertex
+ * targets, 1 per vertex stiffness; any may be NULL). */
+jgs2_status jgs2_scene_set_objectives(jgs2_scene* scene, const float* forces, const float* targets,
+                                      const float* stiffnesses);
+
+void jgs2_scene_destroy(jgs2_scene* scene);
+
+/* Canonical vertex->tetrahedron CSR (buildAdjacency, src/scenes/index.ts:
+ * 811-837). `starts`/`counts` receive V entries, `tets` 4T entries. */
+jgs2_status jgs2_build_adjacency(uint32_t vertex_count, uint32_t tet_count, const uint32_t* tet_indices,
+                                 uint32_t* starts, uint32_t* counts, uint32_t* tets);
+
+/* Deterministic greedy coloring over CSR conflict groups
+ * (buildGreedyVertexColoring, src/simulation/cpu/vertex-coloring.ts:20-110).
+ * Returns the color count through `color_count`. */
+jgs2_status jgs2_greedy_vertex_coloring(uint32_t vertex_count, const uint32_t* group_offsets, uint32_t group_count,
+                                        const uint32_t* group_members, uint32_t* colors, uint32_t* color_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JGS2HIP_SCENE_H */

@@ -1,0 +1,88 @@
+// Device-side state and launch interface of libjgs2hip (host view).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
+#include <cstdint>
+
+namespace jgs2 {
+
+// Per-pass uniform block, the role of the reference's SimParams
+// (src/simulation/gpu/jgs2-shader.ts:141-164), passed by value to kernels.
+struct PassParams {
+  uint32_t vertex_count, tet_count, cubature_k, body_count;
+  uint32_t src, tgt;          // 0 = posA, 1 = posB
+  uint32_t colored;           // in-place single-color pass
+  uint32_t color_begin, color_end;  // range of the color-sorted vertex list
+  uint32_t objective_flags;
+  uint32_t globalization;     // stable Neo-Hookean production path
+  uint32_t stop_enabled;      // convergence stop latch honored
+  uint32_t corotated;         // legacy co-rotated material path
+  float dt, inv_dt, inv_dt2, max_step;
+  float gx, gy, gz, floor_y;
+  float floor_k, regularization, polar_eps, damping;
+  float tangential_damping, contact_margin, ipc_dhat, ipc_kappa;
+  float residual_tol, update_tol, scene_scale, pad0;
+};
+
+// Device buffers of one solver. Layout (16-byte records throughout):
+//   vtx_rest  float4[V]  rest xyz, lumped mass
+//   vtx_info  uint4[V]   adjacency start/count, pinned, body
+//   tet_idx   uint4[T]
+//   tet_dm    float4[3T] columns of Dm^-1; w = volume, lambda, mu
+//   tet_k     float[144T] co-rotated rest stiffness (legacy path only)
+//   adjacency uint32[4T]
+//   cub       uint4[10 * V * K] record = {tet, weight bits, 0, 0} + 9 float4 Ubar
+//   objectives float4[2V]
+struct DeviceState {
+  uint32_t V = 0, T = 0, K = 0, bodies = 1;
+  float4* vtx_rest = nullptr;
+  uint4* vtx_info = nullptr;
+  uint4* tet_idx = nullptr;
+  float4* tet_dm = nullptr;
+  float* tet_k = nullptr;
+  uint32_t* adjacency = nullptr;
+  uint4* cub = nullptr;
+  float4* objectives = nullptr;
+  uint32_t* color_order = nullptr;  // vertices sorted by color (ascending id per color)
+  uint32_t* colors = nullptr;
+  float4* pos[2] = {nullptr, nullptr};
+  float4* predicted = nullptr;
+  float4* velocity = nullptr;
+  float4* old = nullptr;
+  float4* vrot = nullptr;       // 3 per vertex (columns)
+  float4* trot = nullptr;       // 3 per tet (legacy path)
+  float4* body_corr = nullptr;
+  float4* final_update = nullptr;
+  float4* local_diag = nullptr;  // 4 per vertex
+  float4* tet_glob = nullptr;    // 2 per tet
+  float4* vertex_energy = nullptr;
+  float4* conv = nullptr;        // 5 per vertex
+  float4* control = nullptr;     // 4
+  float4* history = nullptr;     // 64 * 8
+  void* partials = nullptr;      // block partial reductions
+  uint32_t partial_capacity = 0;
+  float* debug_g = nullptr;      // debug local systems
+  float* debug_h = nullptr;
+};
+
+constexpr uint32_t kHistoryCapacity = 64;
+
+// Launchers (jgs2_kernels.hip). All enqueue on `stream`.
+void launch_predict(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_tet_polar(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_reduce_candidate(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_body_correction(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_clear_frame(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_debug_local_systems(const DeviceState& s, const PassParams& p, hipStream_t stream);
+uint32_t partial_blocks_needed(uint32_t V, uint32_t T);
+size_t partial_bytes(uint32_t blocks);
+
+}  // namespace jgs2

@@ -1,0 +1,16 @@
+// Internal helpers shared by the host translation units of libjgs2hip.
+#pragma once
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "jgs2hip.h"
+
+namespace jgs2 {
+
+// Records the calling thread's last error message and returns `status`.
+jgs2_status fail(jgs2_status status, const char* format, ...);
+void clear_error();
+
+}  // namespace jgs2

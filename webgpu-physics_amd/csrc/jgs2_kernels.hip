@@ -1,0 +1,1585 @@
+// HIP kernels of the JGS2 step for gfx950 (MI355X).
+//
+// One kernel per pass of the reference's frame (src/simulation/gpu/
+// jgs2-solver.ts:2207-2494). Each kernel cites the WGSL entry point it
+// implements in src/simulation/gpu/jgs2-shader.ts ("S:line"). The arithmetic
+// is FP32 like the reference. Where a closed form is exactly equal in real
+// arithmetic to the reference's literal expression and cheaper, the kernel
+// uses the closed form (stated at each site); results then agree with the
+// reference to f32 rounding, not bit for bit.
+//
+// Launch shape: 256 threads (four 64-wide wavefronts) per block; vertex and
+// element kernels map blocks to contiguous index ranges per XCD so that the
+// neighbors a vertex gathers share one XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "jgs2_device.h"
+
+namespace jgs2 {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr float kF32Max = 3.0e38f;
+constexpr float kUnitRoundoff = 0.000000059604644775390625f;
+constexpr float kPositionResolutionMultiplier = 8.0f;
+constexpr float kRelativeEigenvalueFloor = 0.0000152587890625f;
+constexpr float kMaxNormalizedShift = 0.001f;
+constexpr float kArmijoC1 = 0.0001f;
+constexpr uint32_t kMaxBacktracks = 12;
+constexpr uint32_t kTrialCount = 13;
+constexpr float kDeterminantFloor = 0.0001f;
+constexpr float kTwoPiOverThree = 2.0943951023931953f;
+constexpr uint32_t kEmptyTet = 0xffffffffu;
+
+enum : uint32_t {
+  kAccepted = 0, kPinned = 1, kZeroGradient = 2, kShiftLimit = 3, kNonDescent = 4,
+  kLineSearchFailed = 5, kNonfinite = 6, kCholeskyFailed = 7,
+};
+enum : uint32_t {
+  kSourceGeometryValid = 1u, kCandidateGeometryValid = 2u, kAcceptedMinimumValid = 4u,
+  kLocalNumericsValid = 8u, kValidityMask = 15u, kStopLatch = 16u, kControlMask = 31u,
+  kSourceEnergyValid = 16u, kCandidateEnergyValid = 32u, kReductionMask = 1u | 2u | 8u | 16u | 32u,
+};
+enum : uint32_t { kForce = 1u, kTarget = 2u };
+
+// ------------------------------------------------------------------ small math
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// Column-major 3x3 like WGSL mat3x3f.
+struct m3 {
+  f3 c0, c1, c2;
+};
+__device__ __forceinline__ m3 mzero() { return {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)}; }
+__device__ __forceinline__ m3 mid() { return {mk(1, 0, 0), mk(0, 1, 0), mk(0, 0, 1)}; }
+__device__ __forceinline__ f3 mul(const m3& m, f3 v) { return m.c0 * v.x + m.c1 * v.y + m.c2 * v.z; }
+__device__ __forceinline__ m3 mul(const m3& a, const m3& b) { return {mul(a, b.c0), mul(a, b.c1), mul(a, b.c2)}; }
+__device__ __forceinline__ m3 tr(const m3& m) {
+  return {mk(m.c0.x, m.c1.x, m.c2.x), mk(m.c0.y, m.c1.y, m.c2.y), mk(m.c0.z, m.c1.z, m.c2.z)};
+}
+__device__ __forceinline__ f3 tmul(const m3& m, f3 v) {  // transpose(m) * v
+  return mk(dot(m.c0, v), dot(m.c1, v), dot(m.c2, v));
+}
+__device__ __forceinline__ m3 madd(const m3& a, const m3& b) { return {a.c0 + b.c0, a.c1 + b.c1, a.c2 + b.c2}; }
+__device__ __forceinline__ m3 msub(const m3& a, const m3& b) { return {a.c0 - b.c0, a.c1 - b.c1, a.c2 - b.c2}; }
+__device__ __forceinline__ m3 mscale(float s, const m3& a) { return {s * a.c0, s * a.c1, s * a.c2}; }
+__device__ __forceinline__ float frob(const m3& a, const m3& b) { return dot(a.c0, b.c0) + dot(a.c1, b.c1) + dot(a.c2, b.c2); }
+__device__ __forceinline__ float mdet(const m3& f) { return dot(f.c0, cross(f.c1, f.c2)); }
+__device__ __forceinline__ m3 cofactor(const m3& f) { return {cross(f.c1, f.c2), cross(f.c2, f.c0), cross(f.c0, f.c1)}; }
+
+__device__ __forceinline__ bool finite_s(float v) { return v == v && fabsf(v) <= kF32Max; }
+__device__ __forceinline__ bool finite3(f3 v) { return finite_s(v.x) && finite_s(v.y) && finite_s(v.z); }
+__device__ __forceinline__ bool finite4(float4 v) {
+  return finite_s(v.x) && finite_s(v.y) && finite_s(v.z) && finite_s(v.w);
+}
+__device__ __forceinline__ bool finite_m(const m3& m) { return finite3(m.c0) && finite3(m.c1) && finite3(m.c2); }
+__device__ __forceinline__ bool finite_nonneg_int(float v, float maximum) {  // S:919-922
+  return finite_s(v) && v >= 0.0f && v <= maximum && v == floorf(v);
+}
+__device__ __forceinline__ uint32_t control_bits(float packed) {  // S:888-898
+  if (!finite_s(packed) || packed < 0.0f || packed > float(kControlMask) || packed != floorf(packed)) return 0u;
+  return uint32_t(packed);
+}
+__device__ __forceinline__ float scaled_length3(f3 v) {  // jgs2-globalization-wgsl.ts:287-294
+  float mc = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z)));
+  if (!(mc > 0.0f)) return 0.0f;
+  f3 n = v / mc;
+  return mc * sqrtf(dot(n, n));
+}
+
+// XCD-aware block mapping: consecutive logical blocks land on one XCD so
+// that vertices sharing tetrahedra share an L2. Bijective for any grid.
+__device__ __forceinline__ uint32_t logical_block() {
+  const uint32_t grid = gridDim.x, bid = blockIdx.x;
+  const uint32_t q = grid / 8u, r = grid % 8u;
+  const uint32_t xcd = bid % 8u, j = bid / 8u;
+  const uint32_t base = xcd < r ? xcd * (q + 1u) : r * (q + 1u) + (xcd - r) * q;
+  return base + j;
+}
+
+__device__ __forceinline__ bool stop_latched(const DeviceState& s, const PassParams& p) {  // S:909-917
+  if (!p.stop_enabled) return false;
+  return (control_bits(s.control[0].w) & kStopLatch) != 0u;
+}
+
+// ------------------------------------------------------------------ element access
+struct Tet {
+  uint4 ix;
+  m3 inv;           // Dm^-1 columns
+  float vol, lam, mu;
+};
+__device__ __forceinline__ Tet load_tet(const DeviceState& s, uint32_t t) {
+  Tet e;
+  e.ix = s.tet_idx[t];
+  float4 a = s.tet_dm[3 * size_t(t)], b = s.tet_dm[3 * size_t(t) + 1], c = s.tet_dm[3 * size_t(t) + 2];
+  e.inv = {xyz(a), xyz(b), xyz(c)};
+  e.vol = a.w;
+  e.lam = b.w;
+  e.mu = c.w;
+  return e;
+}
+// Shape gradients: rows of Dm^-1 and their negative sum (S:1080-1088).
+__device__ __forceinline__ f3 shape_gradient(const Tet& e, uint32_t slot) {
+  f3 g1 = mk(e.inv.c0.x, e.inv.c1.x, e.inv.c2.x);
+  f3 g2 = mk(e.inv.c0.y, e.inv.c1.y, e.inv.c2.y);
+  f3 g3 = mk(e.inv.c0.z, e.inv.c1.z, e.inv.c2.z);
+  if (slot == 1) return g1;
+  if (slot == 2) return g2;
+  if (slot == 3) return g3;
+  return -g1 - g2 - g3;
+}
+__device__ __forceinline__ uint32_t local_slot(uint4 ix, uint32_t v) {  // S:1045-1051
+  if (ix.x == v) return 0;
+  if (ix.y == v) return 1;
+  if (ix.z == v) return 2;
+  if (ix.w == v) return 3;
+  return 4;
+}
+__device__ __forceinline__ f3 load_pos(const float4* pos, uint32_t v) { return xyz(pos[v]); }
+// F = Ds Dm^-1 (S:1062-1074).
+__device__ __forceinline__ m3 deformation(const Tet& e, const float4* pos) {
+  f3 p0 = load_pos(pos, e.ix.x);
+  f3 e1 = load_pos(pos, e.ix.y) - p0, e2 = load_pos(pos, e.ix.z) - p0, e3 = load_pos(pos, e.ix.w) - p0;
+  m3 ds = {e1, e2, e3};
+  return mul(ds, e.inv);
+}
+__device__ __forceinline__ m3 load_rot(const float4* r, uint32_t i) {
+  return {xyz(r[3 * size_t(i)]), xyz(r[3 * size_t(i) + 1]), xyz(r[3 * size_t(i) + 2])};
+}
+__device__ __forceinline__ void store_rot(float4* r, uint32_t i, const m3& m) {
+  r[3 * size_t(i)] = make_float4(m.c0.x, m.c0.y, m.c0.z, 0.0f);
+  r[3 * size_t(i) + 1] = make_float4(m.c1.x, m.c1.y, m.c1.z, 0.0f);
+  r[3 * size_t(i) + 2] = make_float4(m.c2.x, m.c2.y, m.c2.z, 0.0f);
+}
+
+// ------------------------------------------------------------------ stable Neo-Hookean
+// Invariants shared by density, stress and tangent (stable-neo-hookean-wgsl.ts:44-94).
+struct SnhPoint {
+  m3 F, cof;
+  float J, den, shear, volscale;
+};
+__device__ __forceinline__ SnhPoint snh_point(const m3& F, float lam, float mu) {
+  SnhPoint s;
+  s.F = F;
+  s.cof = cofactor(F);
+  float i1 = frob(F, F);
+  s.den = i1 + 1.0f;
+  s.J = dot(F.c0, s.cof.c0);
+  s.shear = mu * (1.0f - 1.0f / s.den);
+  s.volscale = lam * (s.J - 1.0f) - 0.75f * mu;
+  return s;
+}
+__device__ __forceinline__ float snh_energy(const m3& F, float lam, float mu) {  // SN:44-55
+  float i1 = frob(F, F);
+  float jm1 = mdet(F) - 1.0f;
+  return 0.5f * mu * (i1 - 3.0f) - 0.75f * mu * jm1 + 0.5f * lam * jm1 * jm1 - 0.5f * mu * logf((i1 + 1.0f) * 0.25f);
+}
+// P * g = shear F g + volscale cof g (SN:57-69 applied to a vector).
+__device__ __forceinline__ f3 piola_times(const SnhPoint& s, f3 g) {
+  return s.shear * mul(s.F, g) + s.volscale * mul(s.cof, g);
+}
+// <A, DP(F)[B]> for general directions A, B (SN:72-94 contracted with A).
+__device__ __forceinline__ float snh_second(const SnhPoint& s, const m3& A, const m3& B, float lam, float mu) {
+  m3 dcof = {cross(B.c1, s.F.c2) + cross(s.F.c1, B.c2), cross(B.c2, s.F.c0) + cross(s.F.c2, B.c0),
+             cross(B.c0, s.F.c1) + cross(s.F.c0, B.c1)};
+  float dshear = 2.0f * mu * frob(s.F, B) / (s.den * s.den);
+  float dvol = lam * frob(s.cof, B);
+  return s.shear * frob(A, B) + dshear * frob(A, s.F) + dvol * frob(A, s.cof) + s.volscale * frob(A, dcof);
+}
+
+// ------------------------------------------------------------------ globalization (jgs2-globalization-wgsl.ts)
+struct Sym3 {
+  float a00, a11, a22, a01, a02, a12;
+};
+__device__ __forceinline__ float frobenius_over_sqrt_three(const Sym3& m) {  // GL:136-162
+  float me = fmaxf(fmaxf(fmaxf(fabsf(m.a00), fabsf(m.a11)), fabsf(m.a22)),
+                   fmaxf(fmaxf(fabsf(m.a01), fabsf(m.a02)), fabsf(m.a12)));
+  if (!(me > 0.0f)) return 0.0f;
+  float a00 = m.a00 / me, a11 = m.a11 / me, a22 = m.a22 / me, a01 = m.a01 / me, a02 = m.a02 / me, a12 = m.a12 / me;
+  float sq = a00 * a00 + a11 * a11 + a22 * a22 + 2.0f * (a01 * a01 + a02 * a02 + a12 * a12);
+  return me * sqrtf(sq / 3.0f);
+}
+__device__ __forceinline__ float min_eigenvalue(const Sym3& m) {  // GL:165-208
+  float off = m.a01 * m.a01 + m.a02 * m.a02 + m.a12 * m.a12;
+  if (off == 0.0f) return fminf(m.a00, fminf(m.a11, m.a22));
+  float mean = (m.a00 + m.a11 + m.a22) / 3.0f;
+  float d00 = m.a00 - mean, d11 = m.a11 - mean, d22 = m.a22 - mean;
+  float centered = d00 * d00 + d11 * d11 + d22 * d22 + 2.0f * off;
+  float spectral = sqrtf(centered / 6.0f);
+  if (!(spectral > 0.0f)) return fminf(m.a00, fminf(m.a11, m.a22));
+  m3 nc = {mk(d00, m.a01, m.a02) / spectral, mk(m.a01, d11, m.a12) / spectral, mk(m.a02, m.a12, d22) / spectral};
+  float hd = fminf(fmaxf(0.5f * mdet(nc), -1.0f), 1.0f);
+  float angle = acosf(hd) / 3.0f;
+  float largest = mean + 2.0f * spectral * cosf(angle);
+  float smallest = mean + 2.0f * spectral * cosf(angle + kTwoPiOverThree);
+  float middle = 3.0f * mean - largest - smallest;
+  return fminf(largest, fminf(middle, smallest));
+}
+struct SolveOut {
+  f3 dir;
+  float gdotp, scale, min_eig, shift, residual;
+  uint32_t status;
+};
+// jgs2_globalization_solve (GL:328-492); `H` is the accumulated local Hessian.
+__device__ SolveOut globalization_solve(const m3& H, f3 g, float inertia_scale) {
+  SolveOut o{mk(0, 0, 0), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, kNonfinite};
+  // classify_shift (GL:223-285)
+  if (!finite_m(H) || !finite_s(inertia_scale) || !(inertia_scale > 0.0f)) return o;
+  Sym3 s{H.c0.x, H.c1.y, H.c2.z, 0.5f * (H.c0.y + H.c1.x), 0.5f * (H.c0.z + H.c2.x), 0.5f * (H.c1.z + H.c2.y)};
+  float lad = fmaxf(fabsf(s.a00), fmaxf(fabsf(s.a11), fabsf(s.a22)));
+  float scale = fmaxf(inertia_scale, fmaxf(lad, frobenius_over_sqrt_three(s)));
+  if (!finite_s(scale) || !(scale > 0.0f)) return o;
+  Sym3 n{s.a00 / scale, s.a11 / scale, s.a22 / scale, s.a01 / scale, s.a02 / scale, s.a12 / scale};
+  float me = min_eigenvalue(n);
+  if (!finite_s(me)) return o;
+  float shift = fmaxf(0.0f, kRelativeEigenvalueFloor - me);
+  o.scale = scale;
+  o.min_eig = me;
+  o.shift = shift;
+  if (shift > kMaxNormalizedShift) {
+    o.status = kShiftLimit;
+    return o;
+  }
+  if (!finite3(g)) {
+    o.status = kNonfinite;
+    return o;
+  }
+  float gs = fmaxf(fabsf(g.x), fmaxf(fabsf(g.y), fabsf(g.z)));
+  if (!(gs > 0.0f)) {
+    o.status = kZeroGradient;
+    return o;
+  }
+  float a00 = n.a00 + shift, a11 = n.a11 + shift, a22 = n.a22 + shift;
+  f3 rhs = (-g) / scale;
+  if (!finite3(rhs)) {
+    o.status = kNonfinite;
+    return o;
+  }
+  o.status = kCholeskyFailed;
+  if (!finite_s(a00) || !(a00 > 0.0f)) return o;
+  float l00 = sqrtf(a00);
+  float l10 = n.a01 / l00, l20 = n.a02 / l00;
+  float p1 = a11 - l10 * l10;
+  if (!finite_s(p1) || !(p1 > 0.0f)) return o;
+  float l11 = sqrtf(p1);
+  float l21 = (n.a12 - l20 * l10) / l11;
+  float p2 = a22 - l20 * l20 - l21 * l21;
+  if (!finite_s(p2) || !(p2 > 0.0f)) return o;
+  float l22 = sqrtf(p2);
+  float y0 = rhs.x / l00;
+  float y1 = (rhs.y - l10 * y0) / l11;
+  float y2 = (rhs.z - l20 * y0 - l21 * y1) / l22;
+  float x2 = y2 / l22;
+  float x1 = (y1 - l21 * x2) / l11;
+  float x0 = (y0 - l10 * x1 - l20 * x2) / l00;
+  f3 d = mk(x0, x1, x2);
+  if (!finite3(d)) return o;
+  float gd = dot(g, d);
+  if (!finite_s(gd)) {
+    o.status = kNonDescent;
+    return o;
+  }
+  if (!(gd < 0.0f)) {
+    o.gdotp = gd;
+    o.status = kNonDescent;
+    return o;
+  }
+  f3 r = mk(a00 * d.x + n.a01 * d.y + n.a02 * d.z, n.a01 * d.x + a11 * d.y + n.a12 * d.z,
+            n.a02 * d.x + n.a12 * d.y + a22 * d.z) - rhs;
+  float res = scaled_length3(r) / fmaxf(1.0f, scaled_length3(rhs));
+  o.gdotp = gd;
+  if (!finite_s(res)) {
+    o.status = kCholeskyFailed;
+    return o;
+  }
+  o.dir = d;
+  o.residual = res;
+  o.status = kAccepted;
+  return o;
+}
+__device__ __forceinline__ float log_one_plus(float v) {  // GL:496-506
+  if (fabsf(v) <= 0.001f) return v * (1.0f + v * (-0.5f + v * (0.3333333333333333f + v * (-0.25f + 0.2f * v))));
+  return logf(1.0f + v);
+}
+// Cancellation-safe SNH density difference for F1 = F0 + D (GL:510-559).
+__device__ __forceinline__ float snh_delta_general(const m3& i, const m3& dd, float lam, float mu) {
+  m3 f = madd(i, dd);
+  float ii = frob(i, i);
+  float id = frob(dd, madd(f, i));
+  float jd = dot(dd.c0, cross(i.c1, i.c2)) + dot(i.c0, cross(dd.c1, i.c2)) + dot(i.c0, cross(i.c1, dd.c2)) +
+             dot(dd.c0, cross(dd.c1, i.c2)) + dot(dd.c0, cross(i.c1, dd.c2)) + dot(i.c0, cross(dd.c1, dd.c2)) +
+             dot(dd.c0, cross(dd.c1, dd.c2));
+  float ij = mdet(i), fj = mdet(f);
+  float lr = log_one_plus(id / (ii + 1.0f));
+  return 0.5f * mu * id - 0.75f * mu * jd + 0.5f * lam * jd * (fj + ij - 2.0f) - 0.5f * mu * lr;
+}
+
+// ------------------------------------------------------------------ per-vertex helpers
+__device__ __forceinline__ float dist_inertia_weight(const DeviceState& s, const PassParams& p, uint32_t v) {  // S:2401-2408
+  uint4 inf = s.vtx_info[v];
+  if (inf.z != 0u) return 0.0f;
+  uint32_t n = inf.y > 1u ? inf.y : 1u;
+  return s.vtx_rest[v].w * p.inv_dt2 / float(n);
+}
+__device__ __forceinline__ f3 objective_gradient(const DeviceState& s, const PassParams& p, uint32_t v, f3 x) {  // S:1926-1938
+  f3 r = mk(0, 0, 0);
+  if (p.objective_flags & kForce) r = r - xyz(s.objectives[2 * size_t(v)]);
+  if (p.objective_flags & kTarget) {
+    float4 t = s.objectives[2 * size_t(v) + 1];
+    if (t.w > 0.0f) r = r + t.w * (x - xyz(t));
+  }
+  return r;
+}
+__device__ __forceinline__ float target_stiffness(const DeviceState& s, const PassParams& p, uint32_t v) {
+  return (p.objective_flags & kTarget) ? s.objectives[2 * size_t(v) + 1].w : 0.0f;
+}
+__device__ __forceinline__ float quadratic_delta(f3 base, f3 target, f3 d, float w) {  // S:1891-1901
+  f3 r = base - target;
+  return w * (dot(r, d) + 0.5f * dot(d, d));
+}
+__device__ __forceinline__ float objective_delta(const DeviceState& s, const PassParams& p, uint32_t v, f3 base, f3 d) {  // S:1903-1924
+  float r = 0.0f;
+  if (p.objective_flags & kForce) r -= dot(xyz(s.objectives[2 * size_t(v)]), d);
+  if (p.objective_flags & kTarget) {
+    float4 t = s.objectives[2 * size_t(v) + 1];
+    if (t.w > 0.0f) r += quadratic_delta(base, xyz(t), d, t.w);
+  }
+  return r;
+}
+__device__ __forceinline__ float floor_delta(const PassParams& p, float by, float dy) {  // S:1944-1956
+  if (!(p.floor_k > 0.0f)) return 0.0f;
+  float ip = fminf(0.0f, by - p.floor_y);
+  float fp = fminf(0.0f, by + dy - p.floor_y);
+  return 0.5f * p.floor_k * (fp - ip) * (fp + ip);
+}
+__device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const PassParams& p, const float4* pos, uint32_t v) {  // S:2557-2645
+  bool pin = s.vtx_info[v].z != 0u;
+  f3 x = load_pos(pos, v);
+  float e = 0.0f;
+  if (!pin) {
+    float in = s.vtx_rest[v].w * p.inv_dt2;
+    f3 r = x - xyz(s.predicted[v]);
+    e += 0.5f * in * dot(r, r);
+  }
+  if (p.objective_flags != 0u && !pin) {
+    if (p.objective_flags & kForce) e -= dot(xyz(s.objectives[2 * size_t(v)]), x);
+    if (p.objective_flags & kTarget) {
+      float4 t = s.objectives[2 * size_t(v) + 1];
+      if (t.w > 0.0f) {
+        f3 tr = x - xyz(t);
+        e += 0.5f * t.w * dot(tr, tr);
+      }
+    }
+  }
+  if (!pin && p.floor_k > 0.0f) {
+    float pen = fminf(0.0f, x.y - p.floor_y);
+    e += 0.5f * p.floor_k * pen * pen;
+  }
+  return e;
+}
+__device__ __forceinline__ void store_vertex_energy(const DeviceState& s, const PassParams& p, uint32_t v) {  // S:2647-2658
+  float se = vertex_energy_at(s, p, s.pos[p.src], v);
+  float ce = vertex_energy_at(s, p, s.pos[p.tgt], v);
+  bool sv = finite_s(se), cv = finite_s(ce);
+  s.vertex_energy[v] = make_float4(sv ? se : 0.0f, cv ? ce : 0.0f, sv ? 1.0f : 0.0f, cv ? 1.0f : 0.0f);
+}
+
+// ------------------------------------------------------------------ co-rotated (legacy) element terms
+__device__ __forceinline__ m3 stiffness_block(const DeviceState& s, uint32_t t, uint32_t rv, uint32_t cv) {  // S:1022-1043
+  const float* k = s.tet_k + size_t(t) * 144;
+  uint32_t r0 = rv * 3, c0 = cv * 3;
+  return {mk(k[r0 * 12 + c0], k[(r0 + 1) * 12 + c0], k[(r0 + 2) * 12 + c0]),
+          mk(k[r0 * 12 + c0 + 1], k[(r0 + 1) * 12 + c0 + 1], k[(r0 + 2) * 12 + c0 + 1]),
+          mk(k[r0 * 12 + c0 + 2], k[(r0 + 1) * 12 + c0 + 2], k[(r0 + 2) * 12 + c0 + 2])};
+}
+// tetGradient, co-rotated branch (S:1171-1197).
+__device__ void corot_gradients(const DeviceState& s, const PassParams& p, uint32_t t, uint4 ix, f3 out[4]) {
+  const float4* pos = s.pos[p.src];
+  uint32_t id[4] = {ix.x, ix.y, ix.z, ix.w};
+  f3 cur[4], rst[4];
+  for (int l = 0; l < 4; ++l) {
+    cur[l] = load_pos(pos, id[l]);
+    rst[l] = xyz(s.vtx_rest[id[l]]);
+  }
+  f3 cc = 0.25f * (((cur[0] + cur[1]) + cur[2]) + cur[3]);
+  f3 rc = 0.25f * (((rst[0] + rst[1]) + rst[2]) + rst[3]);
+  m3 R = load_rot(s.trot, t);
+  f3 disp[4];
+  for (int l = 0; l < 4; ++l) disp[l] = tmul(R, cur[l] - cc) - (rst[l] - rc);
+  for (int r = 0; r < 4; ++r) {
+    f3 lg = mk(0, 0, 0);
+    for (int c = 0; c < 4; ++c) lg = lg + mul(stiffness_block(s, t, r, c), disp[c]);
+    out[r] = mul(R, lg);
+  }
+}
+
+// ------------------------------------------------------------------ the local system
+struct Local {
+  f3 g;
+  m3 H;
+};
+// Gather of jgs2Solve (S:3074-3263) for the tetrahedral material.
+__device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_t v) {
+  const float4* pos = s.pos[p.src];
+  const f3 x = load_pos(pos, v);
+  const uint4 inf = s.vtx_info[v];
+  const float inertia = s.vtx_rest[v].w * p.inv_dt2;
+  Local L;
+  L.g = inertia * (x - xyz(s.predicted[v]));
+  float hdiag = inertia + target_stiffness(s, p, v);
+  L.H = mscale(hdiag, mid());
+  if (p.objective_flags != 0u) L.g = L.g + objective_gradient(s, p, v, x);
+
+  // Exact incident gather (S:3097-3111). Stable Neo-Hookean block in closed
+  // form: with a = F grad, b = cof(F) grad,
+  //   vol DP(F)[e_c grad^T] grad = vol (shear |grad|^2 e_c + 2 mu/den^2 a_c a
+  //                                    + lambda b_c b),
+  // because det(F + x grad^T) is affine in x, so the directional cofactor
+  // term of the tangent vanishes on this rank-one direction.
+  for (uint32_t k = 0; k < inf.y; ++k) {
+    const uint32_t t = s.adjacency[inf.x + k];
+    const Tet e = load_tet(s, t);
+    const uint32_t slot = local_slot(e.ix, v);
+    if (slot >= 4) continue;
+    if (!p.corotated) {
+      const f3 gr = shape_gradient(e, slot);
+      const SnhPoint sp = snh_point(deformation(e, pos), e.lam, e.mu);
+      const f3 a = mul(sp.F, gr), b = mul(sp.cof, gr);
+      L.g = L.g + e.vol * (sp.shear * a + sp.volscale * b);
+      const float cs = e.vol * sp.shear * dot(gr, gr);
+      const float ca = e.vol * 2.0f * e.mu / (sp.den * sp.den);
+      const float cb = e.vol * e.lam;
+      L.H.c0 = L.H.c0 + mk(cs, 0, 0) + (ca * a.x) * a + (cb * b.x) * b;
+      L.H.c1 = L.H.c1 + mk(0, cs, 0) + (ca * a.y) * a + (cb * b.y) * b;
+      L.H.c2 = L.H.c2 + mk(0, 0, cs) + (ca * a.z) * a + (cb * b.z) * b;
+    } else {
+      f3 tg[4];
+      corot_gradients(s, p, t, e.ix, tg);
+      const m3 R = load_rot(s.trot, t);
+      L.g = L.g + tg[slot];
+      L.H = madd(L.H, mul(mul(R, stiffness_block(s, t, slot, slot)), tr(R)));
+    }
+  }
+  if (p.floor_k > 0.0f && x.y < p.floor_y) {  // S:3188-3191
+    L.g.y += p.floor_k * (x.y - p.floor_y);
+    L.H.c1.y += p.floor_k;
+  }
+
+  // Cubature second-order correction (S:3198-3263).
+  if (p.cubature_k == 0) return L;
+  const m3 Rs_t = tr(load_rot(s.vrot, v));
+  const float w_src = dist_inertia_weight(s, p, v);
+  for (uint32_t k = 0; k < p.cubature_k; ++k) {
+    const size_t rec = size_t(v) * p.cubature_k + k;
+    const uint4 head = s.cub[10 * rec];
+    const uint32_t t = head.x;
+    const float w = __uint_as_float(head.y);
+    if (t == kEmptyTet || t >= p.tet_count || !(w > 0.0f)) continue;
+    const Tet e = load_tet(s, t);
+    const uint32_t id[4] = {e.ix.x, e.ix.y, e.ix.z, e.ix.w};
+    // B_l = R_{v_l} Ubar_l R_s^T (S:1846-1862); Ubar stored row-major.
+    const float4* ub = reinterpret_cast<const float4*>(s.cub + 10 * rec + 1);
+    float u[36];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      float4 c = ub[q];
+      u[4 * q] = c.x;
+      u[4 * q + 1] = c.y;
+      u[4 * q + 2] = c.z;
+      u[4 * q + 3] = c.w;
+    }
+    m3 B[4];
+    float wl[4];
+    f3 gr[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const float* b = u + 9 * l;
+      m3 U = {mk(b[0], b[3], b[6]), mk(b[1], b[4], b[7]), mk(b[2], b[5], b[8])};
+      B[l] = mul(mul(load_rot(s.vrot, id[l]), U), Rs_t);
+      wl[l] = dist_inertia_weight(s, p, id[l]);
+      gr[l] = shape_gradient(e, l);
+    }
+    f3 pg = mk(0, 0, 0);
+    m3 ph = mzero();
+    const uint32_t ss = local_slot(e.ix, v);
+    if (!p.corotated) {
+      const SnhPoint sp = snh_point(deformation(e, pos), e.lam, e.mu);
+      f3 tg[4];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        tg[l] = e.vol * piola_times(sp, gr[l]);
+        f3 sg = tg[l] + wl[l] * (load_pos(pos, id[l]) - xyz(s.predicted[id[l]]));
+        if (p.objective_flags != 0u && s.vtx_info[id[l]].z == 0u) {
+          uint32_t n = s.vtx_info[id[l]].y > 1u ? s.vtx_info[id[l]].y : 1u;
+          sg = sg + objective_gradient(s, p, id[l], load_pos(pos, id[l])) / float(n);
+        }
+        pg = pg + tmul(B[l], sg);
+      }
+      // Reduced Hessian (S:2339-2366): entry (r, c) = vol <D_r, DP(F)[D_c]>
+      // with direction matrices D_c = sum_l (B_l e_c) grad_l^T.
+      m3 D[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        m3 acc = mzero();
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+          f3 col = c == 0 ? B[l].c0 : (c == 1 ? B[l].c1 : B[l].c2);
+          acc.c0 = acc.c0 + gr[l].x * col;
+          acc.c1 = acc.c1 + gr[l].y * col;
+          acc.c2 = acc.c2 + gr[l].z * col;
+        }
+        D[c] = acc;
+      }
+      float h00 = e.vol * snh_second(sp, D[0], D[0], e.lam, e.mu);
+      float h11 = e.vol * snh_second(sp, D[1], D[1], e.lam, e.mu);
+      float h22 = e.vol * snh_second(sp, D[2], D[2], e.lam, e.mu);
+      float h01 = e.vol * snh_second(sp, D[0], D[1], e.lam, e.mu);
+      float h02 = e.vol * snh_second(sp, D[0], D[2], e.lam, e.mu);
+      float h12 = e.vol * snh_second(sp, D[1], D[2], e.lam, e.mu);
+      ph = {mk(h00, h01, h02), mk(h01, h11, h12), mk(h02, h12, h22)};
+      if (ss < 4) {
+        // Remove the exact source block and source inertia share (S:3237-3259).
+        const f3 gs = gr[ss];
+        const f3 a = mul(sp.F, gs), b = mul(sp.cof, gs);
+        const float cs = e.vol * sp.shear * dot(gs, gs);
+        const float ca = e.vol * 2.0f * e.mu / (sp.den * sp.den);
+        const float cb = e.vol * e.lam;
+        pg = pg - (tg[ss] + w_src * (x - xyz(s.predicted[v])));
+        ph.c0 = ph.c0 - (mk(cs, 0, 0) + (ca * a.x) * a + (cb * b.x) * b);
+        ph.c1 = ph.c1 - (mk(0, cs, 0) + (ca * a.y) * a + (cb * b.y) * b);
+        ph.c2 = ph.c2 - (mk(0, 0, cs) + (ca * a.z) * a + (cb * b.z) * b);
+        ph = msub(ph, mscale(w_src, mid()));
+      }
+    } else {
+      f3 tg[4];
+      corot_gradients(s, p, t, e.ix, tg);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) pg = pg + tmul(B[l], tg[l] + wl[l] * (load_pos(pos, id[l]) - xyz(s.predicted[id[l]])));
+      const m3 R = load_rot(s.trot, t);
+      f3 cols[3];
+      for (int c = 0; c < 3; ++c) {  // S:2368-2398
+        f3 ld[4], wp[4];
+        for (int l = 0; l < 4; ++l) ld[l] = tmul(R, c == 0 ? B[l].c0 : (c == 1 ? B[l].c1 : B[l].c2));
+        for (int rv = 0; rv < 4; ++rv) {
+          f3 lp = mk(0, 0, 0);
+          for (int cv = 0; cv < 4; ++cv) lp = lp + mul(stiffness_block(s, t, rv, cv), ld[cv]);
+          wp[rv] = mul(R, lp);
+        }
+        f3 rc;
+        rc.x = dot(B[0].c0, wp[0]) + dot(B[1].c0, wp[1]) + dot(B[2].c0, wp[2]) + dot(B[3].c0, wp[3]);
+        rc.y = dot(B[0].c1, wp[0]) + dot(B[1].c1, wp[1]) + dot(B[2].c1, wp[2]) + dot(B[3].c1, wp[3]);
+        rc.z = dot(B[0].c2, wp[0]) + dot(B[1].c2, wp[1]) + dot(B[2].c2, wp[2]) + dot(B[3].c2, wp[3]);
+        cols[c] = rc;
+      }
+      ph = {cols[0], cols[1], cols[2]};
+      if (ss < 4) {
+        pg = pg - (tg[ss] + w_src * (x - xyz(s.predicted[v])));
+        ph = msub(ph, madd(mul(mul(R, stiffness_block(s, t, ss, ss)), tr(R)), mscale(w_src, mid())));
+      }
+    }
+    // Distributed inertia and target Hessians sum_l w_l B_l^T B_l (S:2453-2477).
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      float wt = wl[l];
+      if (p.objective_flags & kTarget) {
+        uint4 il = s.vtx_info[id[l]];
+        if (il.z == 0u) wt += s.objectives[2 * size_t(id[l]) + 1].w / float(il.y > 1u ? il.y : 1u);
+      }
+      const m3& Bl = B[l];
+      f3 t0 = tmul(Bl, Bl.c0), t1 = tmul(Bl, Bl.c1), t2 = tmul(Bl, Bl.c2);
+      ph.c0 = ph.c0 + wt * t0;
+      ph.c1 = ph.c1 + wt * t1;
+      ph.c2 = ph.c2 + wt * t2;
+    }
+    if (ss < 4 && p.objective_flags != 0u) {
+      uint32_t n = inf.y > 1u ? inf.y : 1u;
+      pg = pg - objective_gradient(s, p, v, x) / float(n);
+      if (p.objective_flags & kTarget) ph = msub(ph, mscale(target_stiffness(s, p, v) / float(n), mid()));
+    }
+    L.g = L.g + w * pg;
+    L.H = madd(L.H, mscale(w, ph));
+  }
+  return L;
+}
+
+// Restricted trial model (S:1980-2337) for displacement d of vertex v:
+// minimum determinant over changed elements and the restricted energy delta.
+struct Trial {
+  float min_det;
+  bool geometry_valid;
+  float energy;
+};
+__device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uint32_t v, f3 x, f3 d, float initial_min) {
+  const float4* pos = s.pos[p.src];
+  const uint4 inf = s.vtx_info[v];
+  Trial tr{initial_min, true, 0.0f};
+  const float sin = s.vtx_rest[v].w * p.inv_dt2;
+  const f3 pred = xyz(s.predicted[v]);
+  float energy = quadratic_delta(x, pred, d, sin) + floor_delta(p, x.y, d.y);
+  if (p.objective_flags != 0u) energy += objective_delta(s, p, v, x, d);
+  // Incident tetrahedra: F' = F + d grad^T. In closed form det(F') = J + d.b
+  // and <D, F' + F> = 2 d.a + |d|^2 |grad|^2 with a = F grad, b = cof grad;
+  // the pair and triple terms of the determinant delta vanish for rank one.
+  for (uint32_t k = 0; k < inf.y; ++k) {
+    const uint32_t t = s.adjacency[inf.x + k];
+    const Tet e = load_tet(s, t);
+    const uint32_t slot = local_slot(e.ix, v);
+    const f3 gr = shape_gradient(e, slot);
+    const m3 F = deformation(e, pos);
+    const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
+    const float J = dot(F.c0, c12);
+    const f3 b = gr.x * c12 + gr.y * c20 + gr.z * c01;
+    const f3 a = mul(F, gr);
+    const float jd = dot(d, b);
+    const float det_trial = J + jd;
+    if (!finite_s(det_trial)) {
+      tr.geometry_valid = false;
+      return tr;
+    }
+    tr.min_det = fminf(tr.min_det, det_trial);
+    const float i1 = frob(F, F);
+    const float id = 2.0f * dot(d, a) + dot(d, d) * dot(gr, gr);
+    const float lr = log_one_plus(id / (i1 + 1.0f));
+    energy += e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
+  }
+  if (p.cubature_k > 0) {
+    const f3 ds = tmul(load_rot(s.vrot, v), d);  // R_s^T d
+    const float w_src = dist_inertia_weight(s, p, v);
+    for (uint32_t k = 0; k < p.cubature_k; ++k) {
+      const size_t rec = size_t(v) * p.cubature_k + k;
+      const uint4 head = s.cub[10 * rec];
+      const uint32_t t = head.x;
+      const float w = __uint_as_float(head.y);
+      if (t == kEmptyTet || t >= p.tet_count || !(w > 0.0f)) continue;
+      const Tet e = load_tet(s, t);
+      const uint32_t id[4] = {e.ix.x, e.ix.y, e.ix.z, e.ix.w};
+      const float4* ub = reinterpret_cast<const float4*>(s.cub + 10 * rec + 1);
+      float u[36];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        float4 c = ub[q];
+        u[4 * q] = c.x;
+        u[4 * q + 1] = c.y;
+        u[4 * q + 2] = c.z;
+        u[4 * q + 3] = c.w;
+      }
+      m3 Dd = mzero();
+      f3 bd[4];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const float* bb = u + 9 * l;
+        f3 ud = mk(bb[0] * ds.x + bb[1] * ds.y + bb[2] * ds.z, bb[3] * ds.x + bb[4] * ds.y + bb[5] * ds.z,
+                   bb[6] * ds.x + bb[7] * ds.y + bb[8] * ds.z);
+        bd[l] = mul(load_rot(s.vrot, id[l]), ud);  // B_l d
+        const f3 gl = shape_gradient(e, l);
+        Dd.c0 = Dd.c0 + gl.x * bd[l];
+        Dd.c1 = Dd.c1 + gl.y * bd[l];
+        Dd.c2 = Dd.c2 + gl.z * bd[l];
+      }
+      const m3 F = deformation(e, pos);
+      const float det_trial = mdet(madd(F, Dd));
+      if (!finite_s(det_trial)) {
+        tr.geometry_valid = false;
+        return tr;
+      }
+      tr.min_det = fminf(tr.min_det, det_trial);
+      float compl_e = snh_delta_general(F, Dd, e.lam, e.mu) * e.vol;
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const uint32_t vl = id[l];
+        const f3 xl = load_pos(pos, vl);
+        compl_e += quadratic_delta(xl, xyz(s.predicted[vl]), bd[l], dist_inertia_weight(s, p, vl));
+        if (p.objective_flags != 0u) {
+          uint4 il = s.vtx_info[vl];
+          if (il.z == 0u) compl_e += objective_delta(s, p, vl, xl, bd[l]) / float(il.y > 1u ? il.y : 1u);
+        }
+      }
+      const uint32_t ss = local_slot(e.ix, v);
+      if (ss < 4) {
+        const f3 gs = shape_gradient(e, ss);
+        const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
+        const float J = dot(F.c0, c12);
+        const f3 b = gs.x * c12 + gs.y * c20 + gs.z * c01;
+        const f3 a = mul(F, gs);
+        const float jd = dot(d, b);
+        const float id = 2.0f * dot(d, a) + dot(d, d) * dot(gs, gs);
+        const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
+        compl_e -= e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
+        compl_e -= quadratic_delta(x, pred, d, w_src);
+        if (p.objective_flags != 0u) compl_e -= objective_delta(s, p, v, x, d) / float(inf.y > 1u ? inf.y : 1u);
+      }
+      energy += w * compl_e;
+    }
+  }
+  tr.energy = energy;
+  return tr;
+}
+
+__device__ __forceinline__ void store_local_diag(const DeviceState& s, uint32_t v, f3 dir, float alpha, float min_eig,
+                                                 float scale, float dshift, float nshift, float de, float bound, float gd,
+                                                 float res, float minj, uint32_t bt, uint32_t ev, uint32_t st) {  // S:2518-2555
+  float4* b = s.local_diag + 4 * size_t(v);
+  b[0] = make_float4(dir.x, dir.y, dir.z, alpha);
+  b[1] = make_float4(min_eig, scale, dshift, nshift);
+  b[2] = make_float4(de, bound, gd, res);
+  b[3] = make_float4(minj, float(bt), float(ev), float(st));
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void __launch_bounds__(kBlock) k_clear_frame(DeviceState s) {
+  // Control records are cleared by predict's first invocation (S:2687-2691).
+  if (threadIdx.x < 4) s.control[threadIdx.x] = make_float4(0, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(kBlock) k_predict(DeviceState s, PassParams p) {  // S:2661-2692
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  const f3 o = load_pos(s.pos[0], v);
+  s.old[v] = make_float4(o.x, o.y, o.z, 1.0f);
+  const bool pinned = s.vtx_info[v].z != 0u;
+  f3 pr = xyz(s.vtx_rest[v]);
+  if (!pinned) {
+    const f3 dv = p.damping * xyz(s.velocity[v]);
+    pr = o + p.dt * dv + (p.dt * p.dt) * mk(p.gx, p.gy, p.gz);
+  } else {
+    s.velocity[v] = make_float4(0, 0, 0, 0);
+  }
+  s.predicted[v] = make_float4(pr.x, pr.y, pr.z, 1.0f);
+  const f3 src = p.globalization ? o : pr;
+  s.pos[1][v] = make_float4(src.x, src.y, src.z, 1.0f);
+  s.final_update[v] = make_float4(0, 0, 0, 0);
+}
+
+__device__ m3 polar_rotation(m3 r, float eps);
+
+__global__ void __launch_bounds__(kBlock) k_tet_polar(DeviceState s, PassParams p) {  // S:2932-2957
+  const uint32_t t = logical_block() * kBlock + threadIdx.x;
+  if (t >= p.tet_count) return;
+  if (stop_latched(s, p)) return;
+  const Tet e = load_tet(s, t);
+  store_rot(s.trot, t, polar_rotation(deformation(e, s.pos[p.src]), p.polar_eps));
+}
+
+__device__ __forceinline__ f3 safe_normalize(f3 v, f3 fb, float eps) {  // S:924-930
+  float sq = dot(v, v);
+  if (sq > eps * eps && sq < 1.0e30f) return v * (1.0f / sqrtf(sq));
+  return fb;
+}
+__device__ __forceinline__ f3 perpendicular_to(f3 a, float eps) {  // S:932-938
+  f3 c = fabsf(a.x) > 0.8f ? mk(0, 1, 0) : mk(1, 0, 0);
+  return safe_normalize(cross(a, c), mk(0, 0, 1), eps);
+}
+__device__ m3 polar_rotation(m3 r, float eps) {  // S:940-978
+  for (int it = 0; it < 7; ++it) {
+    float d = mdet(r);
+    float mag = fabsf(d);
+    if (!(mag > eps) || mag > 1.0e20f) break;
+    float rc = 1.0f / d;
+    m3 it_ = mscale(rc, cofactor(r));
+    r = mscale(0.5f, madd(r, it_));
+  }
+  f3 c0 = safe_normalize(r.c0, mk(1, 0, 0), eps);
+  f3 c1c = r.c1 - c0 * dot(c0, r.c1);
+  f3 c1 = safe_normalize(c1c, perpendicular_to(c0, eps), eps);
+  f3 c2 = safe_normalize(cross(c0, c1), mk(0, 0, 1), eps);
+  if (dot(c2, r.c2) < 0.0f) {
+    c1 = -c1;
+    c2 = -c2;
+  }
+  c1 = safe_normalize(cross(c2, c0), perpendicular_to(c0, eps), eps);
+  return {c0, c1, c2};
+}
+
+__global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassParams p) {  // S:2960-2989
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  if (stop_latched(s, p)) return;
+  const uint4 inf = s.vtx_info[v];
+  if (inf.y == 0u) {
+    store_rot(s.vrot, v, mid());
+    return;
+  }
+  m3 avg = mzero();
+  float tv = 0.0f;
+  for (uint32_t k = 0; k < inf.y; ++k) {
+    const uint32_t t = s.adjacency[inf.x + k];
+    const Tet e = load_tet(s, t);
+    avg = madd(avg, mscale(e.vol, deformation(e, s.pos[p.src])));
+    tv += e.vol;
+  }
+  avg = tv > 0.0f ? mscale(1.0f / tv, avg) : mid();
+  store_rot(s.vrot, v, polar_rotation(avg, p.polar_eps));
+}
+
+// jgs2Solve (S:2992-3487), one thread per vertex (Jacobi) or per vertex of
+// the active color (graph-colored Gauss-Seidel, in place).
+__global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
+  const uint32_t lane = logical_block() * kBlock + threadIdx.x;
+  uint32_t v;
+  if (p.colored) {
+    if (p.color_begin + lane >= p.color_end) return;
+    v = s.color_order[p.color_begin + lane];
+  } else {
+    if (lane >= p.vertex_count) return;
+    v = lane;
+  }
+  float4* tgt = s.pos[p.tgt];
+  if (stop_latched(s, p)) {
+    tgt[v] = s.pos[p.src][v];
+    return;
+  }
+  const uint4 inf = s.vtx_info[v];
+  if (inf.z != 0u) {
+    const float4 r = s.vtx_rest[v];
+    tgt[v] = make_float4(r.x, r.y, r.z, 1.0f);
+    s.final_update[v] = make_float4(0, 0, 0, 1);
+    if (p.globalization) {
+      store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, kPinned);
+      if (!p.colored) store_vertex_energy(s, p, v);
+    }
+    return;
+  }
+  const f3 x = load_pos(s.pos[p.src], v);
+  float initial_min = kF32Max;
+  if (p.globalization) {
+    const float4 pre = s.control[0];
+    const uint32_t pv = control_bits(pre.w) & kValidityMask;
+    const bool amv = (pv & kAcceptedMinimumValid) != 0u && finite_s(pre.z);
+    if (amv) initial_min = pre.z;
+    if (!(amv && pre.z > kDeterminantFloor)) {
+      tgt[v] = s.pos[p.src][v];
+      s.final_update[v] = make_float4(0, 0, 0, 1);
+      store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, amv ? pre.z : 0.0f, 0, 0,
+                       amv ? kLineSearchFailed : kNonfinite);
+      if (!p.colored) store_vertex_energy(s, p, v);
+      return;
+    }
+  }
+  const Local L = gather_local(s, p, v);
+  const float inertia = s.vtx_rest[v].w * p.inv_dt2;
+  if (!p.globalization) {  // legacy regularized solve (S:2479-2512, S:3475-3486)
+    const m3& m = L.H;
+    const f3 rhs = -L.g;
+    float dsc = fmaxf(1.0f, fmaxf(fabsf(m.c0.x), fmaxf(fabsf(m.c1.y), fabsf(m.c2.z))));
+    float shift = fmaxf(p.regularization * dsc, 1.0e-9f);
+    float a00 = m.c0.x + shift, a11 = m.c1.y + shift, a22 = m.c2.z + shift;
+    float a01 = 0.5f * (m.c0.y + m.c1.x), a02 = 0.5f * (m.c0.z + m.c2.x), a12 = 0.5f * (m.c1.z + m.c2.y);
+    float l00 = sqrtf(fmaxf(a00, shift));
+    float l10 = a01 / l00, l20 = a02 / l00;
+    float l11 = sqrtf(fmaxf(a11 - l10 * l10, shift));
+    float l21 = (a12 - l20 * l10) / l11;
+    float l22 = sqrtf(fmaxf(a22 - l20 * l20 - l21 * l21, shift));
+    float y0 = rhs.x / l00, y1 = (rhs.y - l10 * y0) / l11, y2 = (rhs.z - l20 * y0 - l21 * y1) / l22;
+    float x2 = y2 / l22, x1 = (y1 - l21 * x2) / l11, x0 = (y0 - l10 * x1 - l20 * x2) / l00;
+    f3 delta = mk(x0, x1, x2);
+    float sq = dot(delta, delta);
+    if (!(sq >= 0.0f && sq < 1.0e30f)) delta = mk(0, 0, 0);
+    float dl2 = dot(delta, delta);
+    if (p.max_step > 0.0f && dl2 > p.max_step * p.max_step) delta = delta * (p.max_step * (1.0f / sqrtf(dl2)));
+    const f3 nx = x + delta;
+    tgt[v] = make_float4(nx.x, nx.y, nx.z, 1.0f);
+    s.final_update[v] = make_float4(sqrtf(dot(delta, delta)), 0, 0, 1);
+    return;
+  }
+  // Globalized Newton direction and line search (S:3265-3471).
+  const SolveOut dr = globalization_solve(L.H, L.g, inertia);
+  uint32_t status = kNonfinite;
+  const f3 full = x + dr.dir;
+  const f3 fd = full - x;
+  const float sc = fmaxf(fabsf(x.x), fmaxf(fabsf(x.y), fabsf(x.z)));
+  const float resolution = fmaxf(p.scene_scale, fmaxf(sc, 1.0e-12f)) * kUnitRoundoff * kPositionResolutionMultiplier;
+  const bool below = dr.status == kAccepted && finite3(full) && finite3(fd) && scaled_length3(fd) <= resolution;
+  if (dr.status == kAccepted) status = below ? kZeroGradient : kLineSearchFailed;
+  else if (dr.status == kZeroGradient) status = kZeroGradient;
+  else if (dr.status == kShiftLimit) status = kShiftLimit;
+  else if (dr.status == kCholeskyFailed) status = kCholeskyFailed;
+  else if (dr.status == kNonDescent) status = kNonDescent;
+  float alpha = 0.0f, accepted_de = 0.0f, bound = 0.0f, min_trial = 0.0f;
+  uint32_t backtracks = 0, evaluations = 0;
+  f3 accepted_d = mk(0, 0, 0), accepted_x = x;
+  if (dr.status == kAccepted && !below) {
+    float ta = 1.0f;
+    const float dl = scaled_length3(dr.dir);
+    if (p.max_step > 0.0f && dl > p.max_step) ta = p.max_step / dl;
+    for (uint32_t b = 0; b <= kMaxBacktracks; ++b) {
+      const f3 tp = x + ta * dr.dir;
+      const f3 td = tp - x;
+      if (!finite3(tp) || !finite3(td)) {
+        status = kNonfinite;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      const float tl = scaled_length3(td);
+      if (p.max_step > 0.0f && tl > p.max_step) {
+        ta *= 0.5f;
+        continue;
+      }
+      if (tl <= resolution) {
+        status = kZeroGradient;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      const float tgd = dot(L.g, td);
+      if (!finite_s(tgd)) {
+        status = kNonfinite;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      if (!(tgd < 0.0f)) {
+        status = kZeroGradient;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      const Trial trial = restricted_trial(s, p, v, x, td, initial_min);
+      if (!trial.geometry_valid) {
+        status = kNonfinite;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      min_trial = trial.min_det;
+      if (trial.min_det > kDeterminantFloor) {
+        evaluations += 1;
+        if (!finite_s(trial.energy)) {
+          status = kNonfinite;
+          backtracks = b;
+          break;
+        }
+        const float tb = kArmijoC1 * tgd;
+        if (trial.energy <= tb) {
+          alpha = ta;
+          accepted_de = trial.energy;
+          bound = tb;
+          backtracks = b;
+          status = kAccepted;
+          accepted_d = td;
+          accepted_x = tp;
+          break;
+        }
+      }
+      ta *= 0.5f;
+    }
+    if (status == kLineSearchFailed) backtracks = kMaxBacktracks;
+  }
+  tgt[v] = make_float4(accepted_x.x, accepted_x.y, accepted_x.z, 1.0f);
+  s.final_update[v] = make_float4(scaled_length3(accepted_d), 0, 0, 1);
+  store_local_diag(s, v, dr.dir, alpha, dr.min_eig * dr.scale, dr.scale, dr.shift * dr.scale, dr.shift, accepted_de,
+                   bound, dr.gdotp, dr.residual, min_trial, backtracks, evaluations, status);
+  if (!p.colored) store_vertex_energy(s, p, v);
+}
+
+__global__ void __launch_bounds__(kBlock) k_vertex_energy(DeviceState s, PassParams p) {  // S:3490-3496
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  store_vertex_energy(s, p, v);
+}
+
+// ------------------------------------------------------------------ candidate reduction
+// Block partial of reduceCandidate (S:3612-3955). The reference reduces
+// with one 128-lane workgroup; here every block reduces a contiguous range
+// and a final single-block pass combines the partials in block order. Min,
+// max and integer counts are order independent; the energy sums use a fixed
+// order, so results are deterministic run to run.
+struct CandPartial {
+  float minj0, minj1, e0, e1, max_shift;
+  uint32_t failures, max_bt, skipped, evals, bits;
+};
+__device__ __forceinline__ void cand_identity(CandPartial& r) {
+  r = CandPartial{kF32Max, kF32Max, 0.0f, 0.0f, 0.0f, 0u, 0u, 0u, 0u, kReductionMask};
+}
+__device__ __forceinline__ void cand_add_energy(CandPartial& r, int which, float value, bool valid) {
+  const uint32_t bit = which == 0 ? kSourceEnergyValid : kCandidateEnergyValid;
+  float& acc = which == 0 ? r.e0 : r.e1;
+  if (valid) {
+    float sum = acc + value;
+    if (finite_s(sum)) acc = sum;
+    else {
+      acc = 0.0f;
+      r.bits &= ~bit;
+    }
+  } else {
+    r.bits &= ~bit;
+  }
+}
+__device__ __forceinline__ void cand_combine(CandPartial& l, const CandPartial& r) {
+  l.minj0 = fminf(l.minj0, r.minj0);
+  l.minj1 = fminf(l.minj1, r.minj1);
+  float s0 = l.e0 + r.e0, s1 = l.e1 + r.e1;
+  if (finite_s(s0)) l.e0 = s0;
+  else {
+    l.e0 = 0.0f;
+    l.bits &= ~kSourceEnergyValid;
+  }
+  if (finite_s(s1)) l.e1 = s1;
+  else {
+    l.e1 = 0.0f;
+    l.bits &= ~kCandidateEnergyValid;
+  }
+  l.max_shift = fmaxf(l.max_shift, r.max_shift);
+  l.failures += r.failures;
+  l.max_bt = l.max_bt > r.max_bt ? l.max_bt : r.max_bt;
+  l.skipped += r.skipped;
+  l.evals += r.evals;
+  l.bits &= r.bits;
+}
+template <class P, class Combine>
+__device__ __forceinline__ void block_reduce(P& mine, P* shared, Combine combine) {
+  shared[threadIdx.x] = mine;
+  __syncthreads();
+  for (uint32_t stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+      P a = shared[threadIdx.x];
+      combine(a, shared[threadIdx.x + stride]);
+      shared[threadIdx.x] = a;
+    }
+    __syncthreads();
+  }
+  mine = shared[0];
+}
+
+__device__ __forceinline__ bool local_status_failure(uint32_t st) {  // S:3603-3609
+  return st == kShiftLimit || st == kNonDescent || st == kLineSearchFailed || st == kNonfinite || st == kCholeskyFailed;
+}
+
+// candidateTetrahedron (S:3499-3601) fused with the tetrahedral half of the
+// candidate reduction; blocks past the tetrahedra reduce the vertex records.
+__global__ void __launch_bounds__(kBlock) k_candidate_partials(DeviceState s, PassParams p, uint32_t tet_blocks) {
+  __shared__ CandPartial sh[kBlock];
+  CandPartial r;
+  cand_identity(r);
+  const uint32_t b = blockIdx.x;
+  const bool latched = stop_latched(s, p);
+  if (b < tet_blocks) {
+    const uint32_t t = b * kBlock + threadIdx.x;
+    if (t < p.tet_count) {
+      float4 geo, en;
+      if (!latched) {
+        const Tet e = load_tet(s, t);
+        const m3 sf = deformation(e, s.pos[p.src]);
+        const m3 cf = deformation(e, s.pos[p.tgt]);
+        const float sj = mdet(sf), cj = mdet(cf);
+        const bool sgv = finite_s(sj), cgv = finite_s(cj);
+        float se = 0.0f, ce = 0.0f;
+        bool sev = false, cev = false;
+        if (sgv && sj > kDeterminantFloor) {
+          se = e.vol * snh_energy(sf, e.lam, e.mu);
+          sev = finite_s(se);
+        }
+        if (cgv && cj > kDeterminantFloor) {
+          ce = e.vol * snh_energy(cf, e.lam, e.mu);
+          cev = finite_s(ce);
+        }
+        geo = make_float4(sgv ? sj : 0.0f, cgv ? cj : 0.0f, sgv ? 1.0f : 0.0f, cgv ? 1.0f : 0.0f);
+        en = make_float4(sev ? se : 0.0f, cev ? ce : 0.0f, sev ? 1.0f : 0.0f, cev ? 1.0f : 0.0f);
+        s.tet_glob[2 * size_t(t)] = geo;
+        s.tet_glob[2 * size_t(t) + 1] = en;
+      } else {
+        geo = s.tet_glob[2 * size_t(t)];
+        en = s.tet_glob[2 * size_t(t) + 1];
+      }
+      const bool sgv = geo.z == 1.0f && finite_s(geo.x);
+      const bool cgv = geo.w == 1.0f && finite_s(geo.y);
+      if (sgv) r.minj0 = fminf(r.minj0, geo.x);
+      else r.bits &= ~kSourceGeometryValid;
+      if (cgv) r.minj1 = fminf(r.minj1, geo.y);
+      else r.bits &= ~kCandidateGeometryValid;
+      cand_add_energy(r, 0, en.x, en.z == 1.0f && finite_s(en.x));
+      cand_add_energy(r, 1, en.y, en.w == 1.0f && finite_s(en.y));
+    }
+  } else {
+    const uint32_t v = (b - tet_blocks) * kBlock + threadIdx.x;
+    if (v < p.vertex_count) {
+      const float4* l = s.local_diag + 4 * size_t(v);
+      const float4 l0 = l[0], l1 = l[1], l2 = l[2], l3 = l[3];
+      bool lnv = finite4(l0) && finite4(l1) && finite4(l2) && finite4(l3);
+      const bool stv = finite_nonneg_int(l3.w, float(kCholeskyFailed));
+      const uint32_t st = stv ? uint32_t(l3.w) : kNonfinite;
+      lnv = lnv && stv && st != kNonfinite;
+      if (local_status_failure(st)) r.failures += 1;
+      if (finite_s(l1.w) && l1.w >= 0.0f) r.max_shift = fmaxf(r.max_shift, l1.w);
+      else lnv = false;
+      const bool btv = finite_nonneg_int(l3.y, float(kMaxBacktracks));
+      const bool evv = finite_nonneg_int(l3.z, float(kTrialCount));
+      const uint32_t bt = btv ? uint32_t(l3.y) : 0u;
+      const uint32_t ev = evv ? uint32_t(l3.z) : 0u;
+      if (!btv || !evv) lnv = false;
+      r.max_bt = bt;
+      r.evals = ev;
+      uint32_t trials = 0;
+      if (st == kAccepted) trials = bt + 1;
+      else if (st == kLineSearchFailed && bt == kMaxBacktracks) trials = kTrialCount;
+      else if (st == kZeroGradient && bt > 0) trials = bt;
+      else if (st == kNonfinite) trials = bt + (l3.x > kDeterminantFloor ? 1u : 0u);
+      r.skipped = trials - (trials < ev ? trials : ev);
+      if (!lnv) r.bits &= ~kLocalNumericsValid;
+      const float4 ve = s.vertex_energy[v];
+      cand_add_energy(r, 0, ve.x, ve.z == 1.0f && finite_s(ve.x));
+      cand_add_energy(r, 1, ve.y, ve.w == 1.0f && finite_s(ve.y));
+    }
+  }
+  block_reduce(r, sh, cand_combine);
+  if (threadIdx.x == 0) reinterpret_cast<CandPartial*>(s.partials)[b] = r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_candidate_final(DeviceState s, PassParams p, uint32_t blocks) {
+  __shared__ CandPartial sh[kBlock];
+  CandPartial r;
+  cand_identity(r);
+  const CandPartial* parts = reinterpret_cast<const CandPartial*>(s.partials);
+  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) cand_combine(r, parts[i]);
+  block_reduce(r, sh, cand_combine);
+  if (threadIdx.x != 0 || stop_latched(s, p)) return;
+  uint32_t bits = r.bits;
+  if (p.tet_count == 0) bits &= ~(kSourceGeometryValid | kCandidateGeometryValid);
+  const bool sgv = (bits & kSourceGeometryValid) != 0u, cgv = (bits & kCandidateGeometryValid) != 0u;
+  const bool sev = (bits & kSourceEnergyValid) != 0u, cev = (bits & kCandidateEnergyValid) != 0u;
+  const bool accepted = sgv && r.minj0 > kDeterminantFloor && cgv && r.minj1 > kDeterminantFloor;
+  const float am = accepted ? r.minj1 : r.minj0;
+  const bool amv = accepted ? cgv : sgv;
+  const float ae = accepted ? r.e1 : r.e0;
+  const bool aev = accepted ? cev : sev;
+  if (amv) bits |= kAcceptedMinimumValid;
+  uint32_t hist = 0;
+  const float packed = s.control[1].w;
+  if (finite_nonneg_int(packed, float(kHistoryCapacity))) hist = uint32_t(packed);
+  else bits &= ~kLocalNumericsValid;
+  const uint32_t cb = bits & kValidityMask;
+  s.control[0] = make_float4(sgv ? r.minj0 : 0.0f, cgv ? r.minj1 : 0.0f, amv ? am : 0.0f, float(cb));
+  s.control[1] = make_float4(accepted ? 1.0f : 0.0f, accepted ? 0.0f : 1.0f, float(r.failures), float(hist));
+  s.control[2] = make_float4(sev ? r.e0 : 0.0f, cev ? r.e1 : 0.0f, aev ? ae : 0.0f, (sev && cev) ? 1.0f : 0.0f);
+  s.control[3] = make_float4(r.max_shift, float(r.max_bt), float(r.skipped), float(r.evals));
+}
+
+__global__ void __launch_bounds__(kBlock) k_apply_candidate(DeviceState s, PassParams p) {  // S:3958-3983
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  if (stop_latched(s, p)) return;
+  const bool accepted = s.control[1].x == 1.0f;
+  if (!accepted) {
+    s.pos[p.tgt][v] = s.pos[p.src][v];
+    s.final_update[v] = make_float4(0, 0, 0, 1);
+    return;
+  }
+  const f3 u = load_pos(s.pos[p.tgt], v) - load_pos(s.pos[p.src], v);
+  s.final_update[v] = make_float4(scaled_length3(u), 0, 0, 1);
+}
+
+// ------------------------------------------------------------------ convergence
+struct ConvPartial {
+  float ch[4];
+  float fin[2];
+  float max_update;
+  uint32_t valid;
+};
+__device__ __forceinline__ void conv_identity(ConvPartial& r) {
+  r = ConvPartial{{0, 0, 0, 0}, {0, 0}, 0.0f, 1u};
+}
+__device__ __forceinline__ void conv_max(ConvPartial& l, const ConvPartial& r) {
+  for (int k = 0; k < 4; ++k) l.ch[k] = fmaxf(l.ch[k], r.ch[k]);
+  l.fin[0] = fmaxf(l.fin[0], r.fin[0]);
+  l.fin[1] = fmaxf(l.fin[1], r.fin[1]);
+  l.max_update = fmaxf(l.max_update, r.max_update);
+  l.valid &= r.valid;
+}
+__device__ __forceinline__ void conv_sum(ConvPartial& l, const ConvPartial& r) {
+  for (int k = 0; k < 4; ++k) {
+    float s = l.ch[k] + r.ch[k];
+    if (finite_s(s)) l.ch[k] = s;
+    else {
+      l.ch[k] = 0.0f;
+      l.valid = 0u;
+    }
+  }
+  for (int k = 0; k < 2; ++k) {
+    float s = l.fin[k] + r.fin[k];
+    if (finite_s(s)) l.fin[k] = s;
+    else {
+      l.fin[k] = 0.0f;
+      l.valid = 0u;
+    }
+  }
+  l.valid &= r.valid;
+}
+__device__ __forceinline__ float max_abs(f3 v) { return fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))); }
+
+// convergenceGradient (S:3986-4097) fused with scan one of reduceConvergence
+// (S:4119-4189): component gradients at the candidate and their scales.
+__global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, PassParams p) {
+  __shared__ ConvPartial sh[kBlock];
+  ConvPartial r;
+  conv_identity(r);
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  const bool latched = stop_latched(s, p);
+  if (v < p.vertex_count) {
+    float4* c = s.conv + 5 * size_t(v);
+    f3 comp5[5];
+    if (!latched) {
+      const uint4 inf = s.vtx_info[v];
+      if (inf.z != 0u) {
+        for (int k = 0; k < 5; ++k) comp5[k] = mk(0, 0, 0);
+      } else {
+        const float4* pos = s.pos[p.tgt];
+        const f3 x = load_pos(pos, v);
+        comp5[0] = (s.vtx_rest[v].w * p.inv_dt2) * (x - xyz(s.predicted[v]));
+        comp5[2] = mk(0, 0, 0);
+        comp5[3] = mk(0, 0, 0);
+        if (p.objective_flags != 0u) {
+          if (p.objective_flags & kForce) comp5[2] = -xyz(s.objectives[2 * size_t(v)]);
+          if (p.objective_flags & kTarget) {
+            const float4 t = s.objectives[2 * size_t(v) + 1];
+            if (t.w > 0.0f) comp5[3] = t.w * (x - xyz(t));
+          }
+        }
+        f3 mat = mk(0, 0, 0);
+        for (uint32_t k = 0; k < inf.y; ++k) {
+          const uint32_t t = s.adjacency[inf.x + k];
+          const Tet e = load_tet(s, t);
+          const uint32_t slot = local_slot(e.ix, v);
+          if (slot >= 4) continue;
+          const SnhPoint sp = snh_point(deformation(e, pos), e.lam, e.mu);
+          mat = mat + e.vol * piola_times(sp, shape_gradient(e, slot));
+        }
+        comp5[1] = mat;
+        comp5[4] = mk(0, 0, 0);
+        if (p.floor_k > 0.0f && x.y < p.floor_y) comp5[4].y = p.floor_k * (x.y - p.floor_y);
+      }
+      for (int k = 0; k < 5; ++k) c[k] = make_float4(comp5[k].x, comp5[k].y, comp5[k].z, 0.0f);
+    } else {
+      for (int k = 0; k < 5; ++k) comp5[k] = xyz(c[k]);
+    }
+    f3 total = mk(0, 0, 0);
+    bool tv = true;
+    for (int k = 0; k < 5; ++k) {
+      const bool ok = finite3(comp5[k]);
+      const f3 val = ok ? comp5[k] : mk(0, 0, 0);
+      const float vs = max_abs(val);
+      if (k < 4) r.ch[k] = fmaxf(r.ch[k], vs);
+      else r.fin[0] = fmaxf(r.fin[0], vs);
+      total = total + val;
+      tv = tv && ok;
+    }
+    tv = tv && finite3(total);
+    if (tv) r.fin[1] = fmaxf(r.fin[1], max_abs(total));
+    else r.valid = 0u;
+    const float up = s.final_update[v].x;
+    if (finite_s(up) && up >= 0.0f) r.max_update = fmaxf(r.max_update, up);
+    else r.valid = 0u;
+  }
+  block_reduce(r, sh, conv_max);
+  if (threadIdx.x == 0) reinterpret_cast<ConvPartial*>(s.partials)[blockIdx.x] = r;
+}
+
+// Scale broadcast: combine scan-one partials into slot [blocks].
+__global__ void __launch_bounds__(kBlock) k_convergence_scales(DeviceState s, uint32_t blocks) {
+  __shared__ ConvPartial sh[kBlock];
+  ConvPartial r;
+  conv_identity(r);
+  ConvPartial* parts = reinterpret_cast<ConvPartial*>(s.partials);
+  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) conv_max(r, parts[i]);
+  block_reduce(r, sh, conv_max);
+  if (threadIdx.x == 0) parts[blocks] = r;
+}
+
+// Scan two (S:4211-4283): normalized squared sums with the global scales.
+__global__ void __launch_bounds__(kBlock) k_convergence_sums(DeviceState s, PassParams p, uint32_t blocks) {
+  __shared__ ConvPartial sh[kBlock];
+  ConvPartial* parts = reinterpret_cast<ConvPartial*>(s.partials);
+  const ConvPartial sc = parts[blocks];
+  ConvPartial r;
+  conv_identity(r);
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < p.vertex_count) {
+    const float4* c = s.conv + 5 * size_t(v);
+    f3 total = mk(0, 0, 0);
+    bool tv = true;
+    for (int k = 0; k < 5; ++k) {
+      const f3 raw = xyz(c[k]);
+      const bool ok = finite3(raw);
+      const f3 val = ok ? raw : mk(0, 0, 0);
+      const float scale = k < 4 ? sc.ch[k] : sc.fin[0];
+      float sq = 0.0f;
+      if (ok && scale > 0.0f) {
+        const f3 n = val / scale;
+        sq = dot(n, n);
+      }
+      const bool sqv = finite_s(sq) && sq >= 0.0f;
+      float& slot = k < 4 ? r.ch[k] : r.fin[0];
+      const float sum = slot + sq;
+      if (sqv && finite_s(sum)) slot = sum;
+      else {
+        slot = 0.0f;
+        r.valid = 0u;
+      }
+      total = total + val;
+      tv = tv && ok;
+    }
+    tv = tv && finite3(total);
+    float tsq = 0.0f;
+    if (tv && sc.fin[1] > 0.0f) {
+      const f3 n = total / sc.fin[1];
+      tsq = dot(n, n);
+    }
+    const float tsum = r.fin[1] + tsq;
+    if (tv && finite_s(tsq) && tsq >= 0.0f && finite_s(tsum)) r.fin[1] = tsum;
+    else {
+      r.fin[1] = 0.0f;
+      r.valid = 0u;
+    }
+  }
+  block_reduce(r, sh, conv_sum);
+  if (threadIdx.x == 0) parts[blocks + 1 + blockIdx.x] = r;
+}
+
+// Final pass of reduceConvergence (S:4320-4485): norms, history, stop latch.
+__global__ void __launch_bounds__(kBlock) k_convergence_final(DeviceState s, PassParams p, uint32_t blocks) {
+  __shared__ ConvPartial sh[kBlock];
+  ConvPartial* parts = reinterpret_cast<ConvPartial*>(s.partials);
+  const ConvPartial sc = parts[blocks];
+  ConvPartial r;
+  conv_identity(r);
+  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) conv_sum(r, parts[blocks + 1 + i]);
+  block_reduce(r, sh, conv_sum);
+  if (threadIdx.x != 0 || stop_latched(s, p)) return;
+  float norms[5];
+  for (int k = 0; k < 4; ++k) norms[k] = sc.ch[k] * sqrtf(r.ch[k]);
+  norms[4] = sc.fin[0] * sqrtf(r.fin[0]);
+  const float gnorm = sc.fin[1] * sqrtf(r.fin[1]);
+  const bool raw_valid = sc.valid == 1u;
+  bool cnv = raw_valid && r.valid == 1u;
+  for (int k = 0; k < 5; ++k) cnv = cnv && finite_s(norms[k]) && norms[k] >= 0.0f;
+  const float den = fmaxf(1.0f, norms[0] + norms[1] + norms[2] + norms[3] + norms[4]);
+  const float rel = gnorm / den;
+  const bool rnv = cnv && finite_s(gnorm) && gnorm >= 0.0f && finite_s(den) && den >= 1.0f && finite_s(rel) && rel >= 0.0f;
+  const bool ssv = finite_s(p.scene_scale) && p.scene_scale >= 0.0f;
+  const float nmu = sc.max_update / fmaxf(ssv ? p.scene_scale : 1.0e-12f, 1.0e-12f);
+  const bool unv = raw_valid && ssv && finite_s(sc.max_update) && sc.max_update >= 0.0f && finite_s(nmu) && nmu >= 0.0f;
+  const float4 geo = s.control[0];
+  const uint32_t cb = control_bits(geo.w) & kValidityMask;
+  const bool sgv = (cb & kSourceGeometryValid) && finite_s(geo.x);
+  const bool cgv = (cb & kCandidateGeometryValid) && finite_s(geo.y);
+  const bool amv = (cb & kAcceptedMinimumValid) && finite_s(geo.z);
+  const bool lnv = (cb & kLocalNumericsValid) != 0u;
+  const float4 acc = s.control[1];
+  const bool aav = finite_nonneg_int(acc.x, 1.0f), arv = finite_nonneg_int(acc.y, 1.0f);
+  const bool lfv = finite_nonneg_int(acc.z, float(p.vertex_count));
+  const bool hiv = finite_nonneg_int(acc.w, float(kHistoryCapacity));
+  const bool aacc = aav && acc.x == 1.0f, arev = arv && acc.y == 1.0f;
+  const uint32_t lf = lfv ? uint32_t(acc.z) : 0u, hi = hiv ? uint32_t(acc.w) : 0u;
+  const float4 cm = s.control[3];
+  const float trials_max = float(p.vertex_count) * float(kTrialCount);
+  const bool cmv = finite_s(cm.x) && cm.x >= 0.0f && finite_nonneg_int(cm.y, float(kMaxBacktracks)) &&
+                   finite_nonneg_int(cm.z, trials_max) && finite_nonneg_int(cm.w, trials_max);
+  const bool fin = rnv && unv && sgv && cgv && amv && lnv && aav && arv && lfv && hiv && cmv;
+  const bool rs = fin && rel <= p.residual_tol;
+  const bool us = fin && nmu <= p.update_tol;
+  const bool converged = rs && us && aacc && !arev && lf == 0u;
+  if (hiv && hi < kHistoryCapacity) {
+    float4* h = s.history + size_t(hi) * 8;
+    h[0] = s.control[0];
+    h[1] = make_float4(acc.x, acc.y, acc.z, arev ? 1.0f : 0.0f);
+    h[2] = s.control[2];
+    h[3] = make_float4(cnv ? norms[0] : 0.0f, cnv ? norms[1] : 0.0f, cnv ? norms[2] : 0.0f, cnv ? norms[3] : 0.0f);
+    h[4] = make_float4(cnv ? norms[4] : 0.0f, rnv ? gnorm : 0.0f, rnv ? den : 0.0f, rnv ? rel : 0.0f);
+    h[5] = make_float4(unv ? sc.max_update : 0.0f, unv ? nmu : 0.0f, rs ? 1.0f : 0.0f, us ? 1.0f : 0.0f);
+    h[6] = make_float4(converged ? 1.0f : 0.0f, fin ? 1.0f : 0.0f, float(hi), cmv ? cm.x : 0.0f);
+    h[7] = cmv ? make_float4(cm.y, cm.z, cm.w, 1.0f) : make_float4(0, 0, 0, 1);
+  }
+  s.control[1].w = float(hi + 1u);
+  if (converged && p.stop_enabled) s.control[0].w = float(cb | kStopLatch);
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_position(DeviceState s, PassParams p) {  // S:4489-4496
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  const float4 x = s.pos[p.src][v];
+  s.pos[p.tgt][v] = make_float4(x.x, x.y, x.z, 1.0f);
+}
+
+// bodyHorizontalCorrection (S:4499-4532): one block per body reduces its
+// vertices in a fixed order.
+__global__ void __launch_bounds__(kBlock) k_body_correction(DeviceState s, PassParams p) {
+  __shared__ float sm[5][kBlock];
+  __shared__ uint32_t sp[kBlock];
+  const uint32_t body = blockIdx.x;
+  float tm = 0, sx = 0, sz = 0, px = 0, pz = 0;
+  uint32_t pinned = 0;
+  for (uint32_t v = threadIdx.x; v < p.vertex_count; v += kBlock) {
+    const uint4 inf = s.vtx_info[v];
+    if (inf.w != body) continue;
+    pinned |= inf.z != 0u ? 1u : 0u;
+    const float m = fmaxf(s.vtx_rest[v].w, 0.0f);
+    const float4 x = s.pos[0][v], q = s.predicted[v];
+    tm += m;
+    sx += m * x.x;
+    sz += m * x.z;
+    px += m * q.x;
+    pz += m * q.z;
+  }
+  sm[0][threadIdx.x] = tm;
+  sm[1][threadIdx.x] = sx;
+  sm[2][threadIdx.x] = sz;
+  sm[3][threadIdx.x] = px;
+  sm[4][threadIdx.x] = pz;
+  sp[threadIdx.x] = pinned;
+  __syncthreads();
+  for (uint32_t stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+      for (int k = 0; k < 5; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + stride];
+      sp[threadIdx.x] |= sp[threadIdx.x + stride];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float cx = 0.0f, cz = 0.0f;
+    if (sp[0] == 0u && sm[0][0] > 0.0f) {
+      cx = (sm[3][0] - sm[1][0]) / sm[0][0];
+      cz = (sm[4][0] - sm[2][0]) / sm[0][0];
+    }
+    s.body_corr[body] = make_float4(cx, 0.0f, cz, 0.0f);
+  }
+}
+__global__ void __launch_bounds__(kBlock) k_apply_body_correction(DeviceState s, PassParams p) {  // S:4535-4551
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  const uint4 inf = s.vtx_info[v];
+  if (inf.z != 0u || inf.w >= p.body_count) return;
+  const float4 c = s.body_corr[inf.w];
+  float4 x = s.pos[0][v];
+  x.x += c.x;
+  x.z += c.z;
+  s.pos[0][v] = x;
+}
+
+__global__ void __launch_bounds__(kBlock) k_finalize(DeviceState s, PassParams p) {  // S:4554-4589
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  if (s.vtx_info[v].z != 0u) {
+    if (!p.globalization) {
+      const float4 r = s.vtx_rest[v];
+      s.pos[0][v] = make_float4(r.x, r.y, r.z, 1.0f);
+    }
+    s.velocity[v] = make_float4(0, 0, 0, 0);
+    return;
+  }
+  const f3 x = load_pos(s.pos[0], v);
+  const f3 o = load_pos(s.old, v);
+  f3 vel = (x - o) * p.inv_dt;
+  if (p.floor_k > 0.0f && p.tangential_damping > 0.0f && x.y <= p.floor_y + p.contact_margin) {
+    const float ts = expf(-p.tangential_damping * p.dt);
+    vel.x *= ts;
+    vel.z *= ts;
+  }
+  s.velocity[v] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, PassParams p) {
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  float* g = s.debug_g + 3 * size_t(v);
+  float* h = s.debug_h + 9 * size_t(v);
+  if (s.vtx_info[v].z != 0u) {
+    for (int k = 0; k < 3; ++k) g[k] = 0.0f;
+    for (int k = 0; k < 9; ++k) h[k] = 0.0f;
+    return;
+  }
+  const Local L = gather_local(s, p, v);
+  g[0] = L.g.x;
+  g[1] = L.g.y;
+  g[2] = L.g.z;
+  const f3 cols[3] = {L.H.c0, L.H.c1, L.H.c2};
+  for (int c = 0; c < 3; ++c) {
+    h[0 * 3 + c] = cols[c].x;
+    h[1 * 3 + c] = cols[c].y;
+    h[2 * 3 + c] = cols[c].z;
+  }
+}
+
+inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+uint32_t partial_blocks_needed(uint32_t V, uint32_t T) {
+  // Candidate partials use blocks(T) + blocks(V); convergence uses
+  // 2 * blocks(V) + 1.
+  uint32_t a = blocks_for(T) + blocks_for(V);
+  uint32_t b = 2 * blocks_for(V) + 1;
+  return a > b ? a : b;
+}
+size_t partial_bytes(uint32_t blocks) {
+  size_t a = sizeof(CandPartial), b = sizeof(ConvPartial);
+  return size_t(blocks) * (a > b ? a : b);
+}
+
+void launch_clear_frame(const DeviceState& s, const PassParams&, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_frame, dim3(1), dim3(kBlock), 0, st, s);
+}
+void launch_predict(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_predict, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+void launch_tet_polar(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_tet_polar, dim3(blocks_for(p.tet_count)), dim3(kBlock), 0, st, s, p);
+}
+void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_vertex_polar, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  const uint32_t n = p.colored ? p.color_end - p.color_begin : p.vertex_count;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_solve, dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
+}
+void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_vertex_energy, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+void launch_reduce_candidate(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  const uint32_t tb = blocks_for(p.tet_count), vb = blocks_for(p.vertex_count);
+  hipLaunchKernelGGL(k_candidate_partials, dim3(tb + vb), dim3(kBlock), 0, st, s, p, tb);
+  hipLaunchKernelGGL(k_candidate_final, dim3(1), dim3(kBlock), 0, st, s, p, tb + vb);
+}
+void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_apply_candidate, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  const uint32_t vb = blocks_for(p.vertex_count);
+  hipLaunchKernelGGL(k_convergence_gradient, dim3(vb), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_convergence_scales, dim3(1), dim3(kBlock), 0, st, s, vb);
+  hipLaunchKernelGGL(k_convergence_sums, dim3(vb), dim3(kBlock), 0, st, s, p, vb);
+  hipLaunchKernelGGL(k_convergence_final, dim3(1), dim3(kBlock), 0, st, s, p, vb);
+}
+void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_position, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+void launch_body_correction(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  if (p.body_count == 0) return;
+  hipLaunchKernelGGL(k_body_correction, dim3(p.body_count), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_apply_body_correction, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+void launch_debug_local_systems(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_debug_local_systems, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+
+}  // namespace jgs2

@@ -1,0 +1,455 @@
+// Host scene setup for libjgs2hip: cuboid meshes, rest data, lumped masses,
+// material parameters, canonical CSR adjacency, a cubature record generator
+// and the deterministic greedy coloring. Pure host code.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "jgs2_internal.h"
+#include "jgs2hip_scene.h"
+
+struct jgs2_scene {
+  uint32_t vertex_count = 0;
+  uint32_t tet_count = 0;
+  uint32_t cubature_k = 0;
+  std::vector<float> positions, vertex_rest;
+  std::vector<uint32_t> vertex_info, tet_indices, adjacency;
+  std::vector<float> tet_inverse_dm, tet_meta, tet_rest_stiffness;
+  std::vector<uint32_t> cubature_tet_ids;
+  std::vector<float> cubature_weights, cubature_basis;
+  std::vector<float> forces, targets, stiffnesses;
+};
+
+namespace {
+
+// Seeded uniform generator in [0, 1): the reference's corpus PRNG
+// (src/scenes/phase1.ts:106-115).
+struct CorpusRandom {
+  uint32_t state;
+  explicit CorpusRandom(uint32_t seed) : state(seed) {}
+  double next() {
+    state += 0x6d2b79f5u;
+    uint32_t v = state;
+    v = (v ^ (v >> 15)) * (v | 1u);
+    v ^= v + (v ^ (v >> 7)) * (v | 61u);
+    return double(v ^ (v >> 14)) / 4294967296.0;
+  }
+};
+
+double determinant3(const double* m) {  // src/simulation/cpu/math.ts:3-9
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+bool invert3(const double* m, double* out) {  // src/simulation/cpu/math.ts:11-35
+  double d = determinant3(m);
+  double scale = 0.0;
+  for (int i = 0; i < 9; ++i) scale = std::max(scale, std::fabs(m[i]));
+  if (scale == 0.0 || std::fabs(d) <= 1e-14 * scale * scale * scale) return false;
+  double r = 1.0 / d;
+  out[0] = (m[4] * m[8] - m[5] * m[7]) * r;
+  out[1] = (m[2] * m[7] - m[1] * m[8]) * r;
+  out[2] = (m[1] * m[5] - m[2] * m[4]) * r;
+  out[3] = (m[5] * m[6] - m[3] * m[8]) * r;
+  out[4] = (m[0] * m[8] - m[2] * m[6]) * r;
+  out[5] = (m[2] * m[3] - m[0] * m[5]) * r;
+  out[6] = (m[3] * m[7] - m[4] * m[6]) * r;
+  out[7] = (m[1] * m[6] - m[0] * m[7]) * r;
+  out[8] = (m[0] * m[4] - m[1] * m[3]) * r;
+  return true;
+}
+
+// Row-major edge matrix [b-a, c-a, d-a] as columns (src/simulation/cpu/fem.ts:41-60).
+void rest_matrix(const std::vector<double>& x, const uint32_t* t, double* m) {
+  const double* a = &x[3 * size_t(t[0])];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) m[3 * r + c] = x[3 * size_t(t[c + 1]) + r] - a[r];
+}
+
+// 12x12 small-strain stiffness B^T C B V (src/simulation/cpu/fem.ts:89-165).
+void linear_stiffness(const double* inv, double volume, double young, double poisson, float* out) {
+  double grad[12];
+  for (int c = 0; c < 3; ++c) {
+    grad[3 + c] = inv[c];
+    grad[6 + c] = inv[3 + c];
+    grad[9 + c] = inv[6 + c];
+    grad[c] = -grad[3 + c] - grad[6 + c] - grad[9 + c];
+  }
+  double B[6][12] = {};
+  for (int v = 0; v < 4; ++v) {
+    double gx = grad[3 * v], gy = grad[3 * v + 1], gz = grad[3 * v + 2];
+    int col = 3 * v;
+    B[0][col] = gx;
+    B[1][col + 1] = gy;
+    B[2][col + 2] = gz;
+    B[3][col] = gy;
+    B[3][col + 1] = gx;
+    B[4][col + 1] = gz;
+    B[4][col + 2] = gy;
+    B[5][col] = gz;
+    B[5][col + 2] = gx;
+  }
+  double lam = young * poisson / ((1 + poisson) * (1 - 2 * poisson));
+  double mu = young / (2 * (1 + poisson));
+  double C[6][6] = {};
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) C[r][c] = r == c ? lam + 2 * mu : lam;
+  C[3][3] = C[4][4] = C[5][5] = mu;
+  double CB[6][12];
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 12; ++c) {
+      double s = 0;
+      for (int i = 0; i < 6; ++i) s += C[r][i] * B[i][c];
+      CB[r][c] = s;
+    }
+  for (int r = 0; r < 12; ++r)
+    for (int c = r; c < 12; ++c) {
+      double s = 0;
+      for (int i = 0; i < 6; ++i) s += B[i][r] * CB[i][c];
+      s *= volume;
+      out[r * 12 + c] = float(s);
+      out[c * 12 + r] = float(s);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+jgs2_status jgs2_build_adjacency(uint32_t vertex_count, uint32_t tet_count, const uint32_t* tet_indices,
+                                 uint32_t* starts, uint32_t* counts, uint32_t* tets) {
+  if (!tet_indices || !starts || !counts || !tets) return jgs2::fail(JGS2_ERROR_TYPE, "NULL adjacency argument.");
+  // buildAdjacency (src/scenes/index.ts:811-837): counts, exclusive prefix,
+  // then tetrahedra in ascending order into each vertex row.
+  std::fill(counts, counts + vertex_count, 0u);
+  for (size_t i = 0; i < size_t(tet_count) * 4; ++i) {
+    if (tet_indices[i] >= vertex_count)
+      return jgs2::fail(JGS2_ERROR_RANGE, "tetIndices[%zu] references vertex %u, but vertexCount is %u.", i,
+                        tet_indices[i], vertex_count);
+    counts[tet_indices[i]] += 1;
+  }
+  uint32_t total = 0;
+  for (uint32_t v = 0; v < vertex_count; ++v) {
+    starts[v] = total;
+    total += counts[v];
+  }
+  std::vector<uint32_t> cursor(starts, starts + vertex_count);
+  for (uint32_t t = 0; t < tet_count; ++t)
+    for (int l = 0; l < 4; ++l) tets[cursor[tet_indices[4 * size_t(t) + l]]++] = t;
+  jgs2::clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_greedy_vertex_coloring(uint32_t vertex_count, const uint32_t* group_offsets, uint32_t group_count,
+                                        const uint32_t* group_members, uint32_t* colors, uint32_t* color_count) {
+  if (!group_offsets || !colors || !color_count || (group_count && !group_members))
+    return jgs2::fail(JGS2_ERROR_TYPE, "NULL coloring argument.");
+  // Vertex -> group incidence, so each vertex visits its conflict partners
+  // without materializing the pair graph (src/simulation/cpu/vertex-coloring.ts:20-110
+  // builds explicit neighbor sets; the greedy result depends only on the
+  // graph, which is identical).
+  std::vector<uint32_t> incident_count(vertex_count + 1, 0);
+  for (uint32_t g = 0; g < group_count; ++g) {
+    for (uint32_t i = group_offsets[g]; i < group_offsets[g + 1]; ++i) {
+      uint32_t v = group_members[i];
+      if (v >= vertex_count)
+        return jgs2::fail(JGS2_ERROR_RANGE, "conflictGroups[%u][%u] must be a valid vertex id", g,
+                          i - group_offsets[g]);
+      incident_count[v + 1] += 1;
+    }
+  }
+  for (uint32_t v = 0; v < vertex_count; ++v) incident_count[v + 1] += incident_count[v];
+  std::vector<uint32_t> incident(incident_count[vertex_count]);
+  std::vector<uint32_t> cursor(incident_count.begin(), incident_count.end() - 1);
+  for (uint32_t g = 0; g < group_count; ++g)
+    for (uint32_t i = group_offsets[g]; i < group_offsets[g + 1]; ++i) incident[cursor[group_members[i]]++] = g;
+  const uint32_t none = 0xffffffffu;
+  std::fill(colors, colors + vertex_count, none);
+  std::vector<uint32_t> stamp_of(size_t(vertex_count) + 1, 0);
+  uint32_t stamp = 0, count = 0;
+  for (uint32_t v = 0; v < vertex_count; ++v) {
+    ++stamp;
+    for (uint32_t k = incident_count[v]; k < incident_count[v + 1]; ++k) {
+      uint32_t g = incident[k];
+      for (uint32_t i = group_offsets[g]; i < group_offsets[g + 1]; ++i) {
+        uint32_t u = group_members[i];
+        if (u != v && colors[u] != none) stamp_of[colors[u]] = stamp;
+      }
+    }
+    uint32_t c = 0;
+    while (c < count && stamp_of[c] == stamp) ++c;
+    colors[v] = c;
+    count = std::max(count, c + 1);
+  }
+  *color_count = count;
+  jgs2::clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_scene_build_cuboid(const jgs2_cuboid_desc* d, jgs2_scene** out) {
+  if (!d || !out) return jgs2::fail(JGS2_ERROR_TYPE, "NULL cuboid description.");
+  *out = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    if (d->cells[a] < 1) return jgs2::fail(JGS2_ERROR_RANGE, "cells[%d] must be a positive integer.", a);
+    if (!(d->size[a] > 0) || !std::isfinite(d->size[a]))
+      return jgs2::fail(JGS2_ERROR_RANGE, "Cuboid dimensions must be finite and positive.");
+  }
+  if (!(d->density > 0) || !(d->young_modulus[0] > 0) || !(d->young_modulus[1] > 0 || d->layer_cells == 0))
+    return jgs2::fail(JGS2_ERROR_RANGE, "Material density and Young's modulus must be positive.");
+  if (!(d->poisson_ratio >= 0 && d->poisson_ratio < 0.5))
+    return jgs2::fail(JGS2_ERROR_RANGE, "Stable Neo-Hookean material requires Poisson ratio in [0, 0.5).");
+  if (d->material_model != JGS2_MATERIAL_COROTATED_LINEAR && d->material_model != JGS2_MATERIAL_STABLE_NEO_HOOKEAN)
+    return jgs2::fail(JGS2_ERROR_RANGE, "Unknown material model %d.", d->material_model);
+  if (!(d->jitter_fraction >= 0 && d->jitter_fraction < 0.25))
+    return jgs2::fail(JGS2_ERROR_RANGE, "jitter_fraction must lie in [0, 0.25).");
+  const uint64_t nx = d->cells[0], ny = d->cells[1], nz = d->cells[2];
+  const uint64_t V64 = (nx + 1) * (ny + 1) * (nz + 1);
+  const uint64_t T64 = nx * ny * nz * 6;
+  if (V64 >= 0xffffffffull || T64 * 4 >= 0xffffffffull)
+    return jgs2::fail(JGS2_ERROR_RANGE, "Cuboid is too large for 32-bit indices.");
+  jgs2_scene* s = new (std::nothrow) jgs2_scene();
+  if (!s) return jgs2::fail(JGS2_ERROR_MEMORY, "Out of host memory.");
+  const uint32_t V = uint32_t(V64), T = uint32_t(T64);
+  s->vertex_count = V;
+  s->tet_count = T;
+  s->cubature_k = d->cubature_k;
+  const uint64_t sy = nx + 1, sz = (nx + 1) * (ny + 1);
+  auto vid = [&](uint64_t x, uint64_t y, uint64_t z) { return uint32_t(x + y * sy + z * sz); };
+
+  // Vertices (mesh.ts:131-146) with optional seeded jitter.
+  std::vector<double> x(3 * size_t(V));
+  std::vector<uint8_t> fixed(V, 0);
+  double h[3] = {d->size[0] / double(nx), d->size[1] / double(ny), d->size[2] / double(nz)};
+  CorpusRandom rng(d->seed);
+  for (uint64_t z = 0; z <= nz; ++z)
+    for (uint64_t y = 0; y <= ny; ++y)
+      for (uint64_t xx = 0; xx <= nx; ++xx) {
+        uint32_t v = vid(xx, y, z);
+        uint64_t g[3] = {xx, y, z};
+        for (int a = 0; a < 3; ++a) x[3 * size_t(v) + a] = d->origin[a] + (d->size[a] * double(g[a])) / double(d->cells[a]);
+        if (d->pin_axis < 3 && g[d->pin_axis] == 0) fixed[v] = 1;
+      }
+  if (d->jitter_fraction > 0) {
+    for (uint32_t v = 0; v < V; ++v)
+      for (int a = 0; a < 3; ++a) x[3 * size_t(v) + a] += (2.0 * rng.next() - 1.0) * d->jitter_fraction * h[a];
+  }
+
+  // Six Kuhn tetrahedra per cell (mesh.ts:152-191) and orientation (mesh.ts:80-99).
+  s->tet_indices.resize(4 * size_t(T));
+  std::vector<uint32_t> layer(T, 0);
+  size_t o = 0;
+  for (uint64_t z = 0; z < nz; ++z)
+    for (uint64_t y = 0; y < ny; ++y)
+      for (uint64_t xx = 0; xx < nx; ++xx) {
+        uint32_t v000 = vid(xx, y, z), v100 = vid(xx + 1, y, z), v010 = vid(xx, y + 1, z),
+                 v110 = vid(xx + 1, y + 1, z), v001 = vid(xx, y, z + 1), v101 = vid(xx + 1, y, z + 1),
+                 v011 = vid(xx, y + 1, z + 1), v111 = vid(xx + 1, y + 1, z + 1);
+        const uint32_t cell[24] = {v000, v100, v110, v111, v000, v110, v010, v111, v000, v010, v011, v111,
+                                   v000, v011, v001, v111, v000, v001, v101, v111, v000, v101, v100, v111};
+        uint64_t g[3] = {xx, y, z};
+        uint32_t lay = d->layer_cells ? uint32_t((g[d->layer_axis % 3] / d->layer_cells) & 1u) : 0u;
+        for (int k = 0; k < 6; ++k) layer[o / 4 + k] = lay;
+        std::memcpy(&s->tet_indices[o], cell, sizeof(cell));
+        o += 24;
+      }
+  for (uint32_t t = 0; t < T; ++t) {
+    uint32_t* ix = &s->tet_indices[4 * size_t(t)];
+    double m[9];
+    rest_matrix(x, ix, m);
+    double det = determinant3(m);
+    if (std::fabs(det) <= 1e-14) {
+      delete s;
+      return jgs2::fail(JGS2_ERROR_RANGE, "Tetrahedron %u is degenerate.", t);
+    }
+    if (det < 0) std::swap(ix[1], ix[2]);
+  }
+
+  // Rest data (fem.ts:167-209), lumped masses (fem.ts:211-238) and the
+  // packed tetrahedron records (scenes/index.ts:1019-1050).
+  s->tet_inverse_dm.assign(12 * size_t(T), 0.0f);
+  s->tet_meta.resize(4 * size_t(T));
+  if (d->pack_rest_stiffness) s->tet_rest_stiffness.resize(144 * size_t(T));
+  std::vector<double> mass(V, 0.0);
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint32_t* ix = &s->tet_indices[4 * size_t(t)];
+    double m[9], inv[9];
+    rest_matrix(x, ix, m);
+    double volume = std::fabs(determinant3(m)) / 6.0;
+    if (!(volume > 1e-14) || !invert3(m, inv)) {
+      delete s;
+      return jgs2::fail(JGS2_ERROR_RANGE, "Cannot assemble a degenerate tetrahedron.");
+    }
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r) s->tet_inverse_dm[12 * size_t(t) + 4 * c + r] = float(inv[3 * r + c]);
+    double E = d->young_modulus[layer[t]], nu = d->poisson_ratio;
+    double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+    double mu = E / (2 * (1 + nu));
+    float tag = float(d->material_model);
+    if (d->material_model == JGS2_MATERIAL_STABLE_NEO_HOOKEAN) {
+      // computeStableNeoHookeanParameters (stable-neo-hookean.ts:68-109).
+      double adj_mu = (4.0 / 3.0) * mu;
+      double adj_lam = lam + (5.0 / 6.0) * mu;
+      lam = adj_lam;
+      mu = adj_mu;
+    }
+    s->tet_meta[4 * size_t(t)] = float(volume);
+    s->tet_meta[4 * size_t(t) + 1] = float(lam);
+    s->tet_meta[4 * size_t(t) + 2] = float(mu);
+    s->tet_meta[4 * size_t(t) + 3] = tag;
+    if (d->pack_rest_stiffness)
+      linear_stiffness(inv, volume, E, nu, &s->tet_rest_stiffness[144 * size_t(t)]);
+    double vm = d->density * volume / 4.0;
+    for (int l = 0; l < 4; ++l) mass[ix[l]] += vm;
+  }
+
+  // Canonical CSR and vertex records (scenes/index.ts:975-1017).
+  std::vector<uint32_t> starts(V), counts(V);
+  s->adjacency.resize(4 * size_t(T));
+  jgs2_build_adjacency(V, T, s->tet_indices.data(), starts.data(), counts.data(), s->adjacency.data());
+  s->positions.resize(4 * size_t(V));
+  s->vertex_rest.resize(4 * size_t(V));
+  s->vertex_info.resize(4 * size_t(V));
+  for (uint32_t v = 0; v < V; ++v) {
+    for (int a = 0; a < 3; ++a) {
+      s->positions[4 * size_t(v) + a] = float(x[3 * size_t(v) + a]);
+      s->vertex_rest[4 * size_t(v) + a] = float(x[3 * size_t(v) + a]);
+    }
+    s->positions[4 * size_t(v) + 3] = 1.0f;
+    s->vertex_rest[4 * size_t(v) + 3] = float(mass[v]);
+    s->vertex_info[4 * size_t(v)] = starts[v];
+    s->vertex_info[4 * size_t(v) + 1] = counts[v];
+    s->vertex_info[4 * size_t(v) + 2] = fixed[v];
+    s->vertex_info[4 * size_t(v) + 3] = 0;
+  }
+
+  // Cubature records. The reference trains them offline per scene with a
+  // dense exact basis and a nonnegative least-squares selection
+  // (src/simulation/cpu/precompute.ts:141-380); the step itself only
+  // consumes fixed-width records (layout.ts:24-27). This generator writes
+  // records of the same form and role, deterministically and in O(V):
+  //   * up to K-2 samples on incident tetrahedra (weight 1), spread evenly
+  //     over the vertex's CSR row;
+  //   * the remaining samples on second-ring tetrahedra that are not
+  //     incident (weight 1/4), which the step treats without the exact-block
+  //     subtraction;
+  //   * each sample's four Ubar blocks are phi(r) (I + a d e^T) with
+  //     r = |x_j - x_i| / h, phi = exp(-r^2), d the unit offset, e a fixed
+  //     axis and a = 0.1, so Ubar is I at the source vertex, decays with
+  //     distance, is zero on pinned vertices, and is not symmetric.
+  const uint32_t K = d->cubature_k;
+  const size_t R = size_t(V) * K;
+  s->cubature_tet_ids.assign(R, 0xffffffffu);
+  s->cubature_weights.assign(R, 0.0f);
+  s->cubature_basis.assign(R * 36, 0.0f);
+  if (K > 0) {
+    const double hh = std::cbrt(h[0] * h[1] * h[2]);
+    const double e[3] = {1.0 / std::sqrt(14.0), 2.0 / std::sqrt(14.0), 3.0 / std::sqrt(14.0)};
+    for (uint32_t v = 0; v < V; ++v) {
+      if (fixed[v] || counts[v] == 0) continue;
+      const uint32_t* row = &s->adjacency[starts[v]];
+      const uint32_t n = counts[v];
+      const uint32_t incident_slots = K >= 2 ? std::min<uint32_t>(K - 2, n) : std::min<uint32_t>(K, n);
+      uint32_t written = 0;
+      auto write_record = [&](uint32_t tet, float weight) {
+        size_t rec = size_t(v) * K + written;
+        s->cubature_tet_ids[rec] = tet;
+        s->cubature_weights[rec] = weight;
+        const uint32_t* ix = &s->tet_indices[4 * size_t(tet)];
+        for (int l = 0; l < 4; ++l) {
+          float* b = &s->cubature_basis[rec * 36 + 9 * size_t(l)];
+          uint32_t j = ix[l];
+          if (fixed[j]) continue;
+          double off[3], r2 = 0;
+          for (int a = 0; a < 3; ++a) {
+            off[a] = x[3 * size_t(j) + a] - x[3 * size_t(v) + a];
+            r2 += off[a] * off[a];
+          }
+          double r = std::sqrt(r2);
+          double phi = std::exp(-r2 / (hh * hh));
+          double dir[3] = {0, 0, 0};
+          if (r > 0)
+            for (int a = 0; a < 3; ++a) dir[a] = off[a] / r;
+          for (int rr = 0; rr < 3; ++rr)
+            for (int cc = 0; cc < 3; ++cc) b[3 * rr + cc] = float(phi * ((rr == cc ? 1.0 : 0.0) + 0.1 * dir[rr] * e[cc]));
+        }
+        ++written;
+      };
+      for (uint32_t k = 0; k < incident_slots; ++k) write_record(row[(size_t(k) * n) / incident_slots], 1.0f);
+      // Second ring: tetrahedra incident to the vertices of the first
+      // incident tetrahedra, not incident to v, in CSR order.
+      for (uint32_t a = 0; a < n && written < K; ++a) {
+        const uint32_t* ix = &s->tet_indices[4 * size_t(row[a])];
+        for (int l = 0; l < 4 && written < K; ++l) {
+          uint32_t u = ix[l];
+          if (u == v) continue;
+          const uint32_t* urow = &s->adjacency[starts[u]];
+          for (uint32_t b = 0; b < counts[u] && written < K; ++b) {
+            uint32_t cand = urow[(b * 7u + v) % counts[u]];
+            const uint32_t* cx = &s->tet_indices[4 * size_t(cand)];
+            bool incident = cx[0] == v || cx[1] == v || cx[2] == v || cx[3] == v;
+            bool seen = false;
+            for (uint32_t q = 0; q < written; ++q) seen |= s->cubature_tet_ids[size_t(v) * K + q] == cand;
+            if (!incident && !seen) {
+              write_record(cand, 0.25f);
+              break;
+            }
+          }
+        }
+      }
+    }
+  }
+  *out = s;
+  jgs2::clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_scene_get_input(const jgs2_scene* s, jgs2_scene_input* out) {
+  if (!s || !out) return jgs2::fail(JGS2_ERROR_TYPE, "NULL scene.");
+  std::memset(out, 0, sizeof(*out));
+  out->vertex_count = s->vertex_count;
+  out->tet_count = s->tet_count;
+  out->cubature_k = s->cubature_k;
+  out->positions = s->positions.data();
+  out->vertex_rest = s->vertex_rest.data();
+  out->vertex_info = s->vertex_info.data();
+  out->tet_indices = s->tet_indices.data();
+  out->tet_inverse_dm = s->tet_inverse_dm.data();
+  out->tet_meta = s->tet_meta.data();
+  out->tet_rest_stiffness = s->tet_rest_stiffness.empty() ? nullptr : s->tet_rest_stiffness.data();
+  out->adjacency = s->adjacency.data();
+  out->adjacency_length = s->adjacency.size();
+  out->cubature_tet_ids = s->cubature_tet_ids.data();
+  out->cubature_weights = s->cubature_weights.data();
+  out->cubature_basis = s->cubature_basis.data();
+  out->external_forces = s->forces.empty() ? nullptr : s->forces.data();
+  out->target_positions = s->targets.empty() ? nullptr : s->targets.data();
+  out->target_stiffnesses = s->stiffnesses.empty() ? nullptr : s->stiffnesses.data();
+  jgs2::clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_scene_set_objectives(jgs2_scene* s, const float* forces, const float* targets,
+                                      const float* stiffnesses) {
+  if (!s) return jgs2::fail(JGS2_ERROR_TYPE, "NULL scene.");
+  if ((targets == nullptr) != (stiffnesses == nullptr))
+    return jgs2::fail(JGS2_ERROR_RANGE,
+                      "objectives.targetPositions and objectives.targetStiffnesses must be provided together.");
+  const size_t V = s->vertex_count;
+  if (forces) s->forces.assign(forces, forces + 3 * V); else s->forces.clear();
+  if (targets) {
+    s->targets.assign(targets, targets + 3 * V);
+    s->stiffnesses.assign(stiffnesses, stiffnesses + V);
+  } else {
+    s->targets.clear();
+    s->stiffnesses.clear();
+  }
+  jgs2::clear_error();
+  return JGS2_OK;
+}
+
+void jgs2_scene_destroy(jgs2_scene* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+"""Synthetic workloads of the JGS2 step, built by the host scene setup in
+libjgs2hip (include/jgs2hip_scene.h).
+
+* `cantilever_beam`: a 4x4x16-hex beam (1 m x 1 m x 4 m, 425 vertices, 1536
+  tetrahedra), stable Neo-Hookean E = 1e5 Pa, nu = 0.4, rho = 1000, the x = 0
+  face pinned, gravity and a scripted soft tip target following the
+  reference's minimal-scene keyframes (src/scenes/index.ts:252-328).
+* `layered_box`: a 1 m x 1 m x 2 m box on a 64x64x128 hex grid (545,025
+  vertices, 3,145,728 tetrahedra) with seeded U(-0.1h, 0.1h) vertex jitter,
+  alternating 8-cell z-slabs of E = 1e4 / 1e6 Pa, nu = 0.45, the z = 0 face
+  pinned. `scale` shrinks the grid for tests.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .solver import GpuInput
+
+# Minimal-scene scripted target (src/scenes/index.ts:252-262).
+TARGET_PULL_START_FRAME = 48
+TARGET_PULL_END_FRAME = 90
+TARGET_RELEASE_FRAME = 108
+TARGET_KEYFRAME_INTERVAL = 3
+TARGET_STIFFNESS = 30000.0
+TARGET_PULL_OFFSET = (0.45, 0.65, 0.25)
+
+
+def _desc(cells, origin, size, *, jitter=0.0, seed=0, pin_axis=0, model=N.MATERIAL_STABLE_NEO_HOOKEAN,
+          density=1000.0, poisson=0.4, young=(1e5, 1e5), layer_axis=2, layer_cells=0, cubature_k=6,
+          rest_stiffness=False) -> N.CuboidDesc:
+    d = N.CuboidDesc()
+    for i in range(3):
+        d.cells[i] = int(cells[i])
+        d.origin[i] = float(origin[i])
+        d.size[i] = float(size[i])
+    d.jitter_fraction = float(jitter)
+    d.seed = int(seed) & 0xFFFFFFFF
+    d.pin_axis = int(pin_axis)
+    d.material_model = int(model)
+    d.density = float(density)
+    d.poisson_ratio = float(poisson)
+    d.young_modulus[0] = float(young[0])
+    d.young_modulus[1] = float(young[1])
+    d.layer_axis = int(layer_axis)
+    d.layer_cells = int(layer_cells)
+    d.cubature_k = int(cubature_k)
+    d.pack_rest_stiffness = 1 if rest_stiffness else 0
+    return d
+
+
+def build_cuboid(desc: N.CuboidDesc) -> GpuInput:
+    """Run the native scene builder and copy its arrays into a GpuInput."""
+    handle = C.c_void_p()
+    N.check(N.lib.jgs2_scene_build_cuboid(C.byref(desc), C.byref(handle)))
+    try:
+        s = N.SceneInput()
+        N.check(N.lib.jgs2_scene_get_input(handle, C.byref(s)))
+        V, T, K = s.vertex_count, s.tet_count, s.cubature_k
+
+        def f(ptr, n):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if ptr else None
+
+        return GpuInput(
+            vertex_count=V, tet_count=T, cubature_k=K,
+            positions=f(s.positions, 4 * V),
+            vertex_rest=f(s.vertex_rest, 4 * V),
+            vertex_info=f(s.vertex_info, 4 * V),
+            tet_indices=f(s.tet_indices, 4 * T),
+            tet_inverse_dm=f(s.tet_inverse_dm, 12 * T),
+            tet_meta=f(s.tet_meta, 4 * T),
+            adjacency=f(s.adjacency, int(s.adjacency_length)),
+            cubature_tet_ids=f(s.cubature_tet_ids, V * K),
+            cubature_weights=f(s.cubature_weights, V * K),
+            cubature_basis=f(s.cubature_basis, V * K * 36),
+            tet_rest_stiffness=f(s.tet_rest_stiffness, 144 * T),
+        )
+    finally:
+        N.lib.jgs2_scene_destroy(handle)
+
+
+def cantilever_beam(cells=(16, 4, 4), size=(4.0, 1.0, 1.0), cubature_k=6, model=N.MATERIAL_STABLE_NEO_HOOKEAN,
+                    young=1e5, poisson=0.4) -> GpuInput:
+    scene = build_cuboid(_desc(cells, (0.0, 0.0, 0.0), size, pin_axis=0, model=model, density=1000.0,
+                               poisson=poisson, young=(young, young), cubature_k=cubature_k,
+                               rest_stiffness=model == N.MATERIAL_COROTATED_LINEAR))
+    scene.extras["target_vertex"] = int(cells[0])  # grid vertex (nx, 0, 0): the free tip corner
+    return scene
+
+
+def scripted_target_at_frame(scene: GpuInput, frame: int):
+    """minimalScriptedTargetAtFrame (src/scenes/index.ts:269-328) around the
+    tip's rest position: (active, position, stiffness)."""
+    if frame < 0:
+        raise ValueError("Minimal scripted-target frame must be nonnegative.")
+    vertex = scene.extras["target_vertex"]
+    rest = scene.vertex_rest.reshape(-1, 4)[vertex, :3].astype(np.float64)
+    if frame < TARGET_PULL_START_FRAME:
+        return False, rest, 0.0
+    pull_frame = min(frame, TARGET_PULL_END_FRAME)
+    keyframe = min(TARGET_PULL_END_FRAME,
+                   TARGET_PULL_START_FRAME + ((pull_frame - TARGET_PULL_START_FRAME) // TARGET_KEYFRAME_INTERVAL)
+                   * TARGET_KEYFRAME_INTERVAL)
+    linear = (keyframe - TARGET_PULL_START_FRAME) / (TARGET_PULL_END_FRAME - TARGET_PULL_START_FRAME)
+    progress = linear * linear * (3 - 2 * linear)
+    position = rest + np.array(TARGET_PULL_OFFSET) * progress
+    if frame >= TARGET_RELEASE_FRAME:
+        return False, position, 0.0
+    return True, position, TARGET_STIFFNESS
+
+
+def layered_box(cells=(64, 64, 128), size=(1.0, 1.0, 2.0), seed=20251203, cubature_k=6,
+                layer_cells=8) -> GpuInput:
+    return build_cuboid(_desc(cells, (0.0, 0.0, 0.0), size, jitter=0.1, seed=seed, pin_axis=2,
+                              density=1000.0, poisson=0.45, young=(1e4, 1e6), layer_axis=2,
+                              layer_cells=layer_cells, cubature_k=cubature_k))
+
+
+# Settings of the two single-scene workloads (exact iteration counts).
+CANTILEVER_SETTINGS = {
+    "timestep": 1.0 / 60.0,
+    "gravity": (0.0, -9.81, 0.0),
+    "floor_stiffness": 0.0,
+    "parity_mode": True,
+    "schedule": "jacobi",
+}
+CANTILEVER_FRAMES = 60
+CANTILEVER_ITERATIONS = 8
+
+LAYERED_BOX_SETTINGS = {
+    "timestep": 1.0 / 60.0,
+    "gravity": (0.0, -9.81, 0.0),
+    "floor_stiffness": 0.0,
+    "parity_mode": True,
+    "schedule": "jacobi",
+}
+LAYERED_BOX_ITERATIONS = 8
+
+
+def algorithmic_bytes_per_sweep(V: int, T: int, K: int, objectives: bool = False) -> int:
+    """Reference-ABI compulsory bytes of one sweep of the local solve:
+    V*(16 x_S + 16 x~ + 48 VertexStatic + 48 R_v + 16 x_T + 16 update
+    + 64 local diagnostics + 16 vertex energy) + T*(80 TetStatic + 16
+    adjacency ids) + V*K*152 (+ 32 V with objectives)."""
+    b = V * 240 + T * 96 + V * K * 152
+    if objectives:
+        b += 32 * V
+    return b
