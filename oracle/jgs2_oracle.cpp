@@ -37,6 +37,7 @@
 #include <vector>
 #include <algorithm>
 #include <limits>
+#include <type_traits>
 
 namespace {
 
@@ -386,6 +387,7 @@ typedef struct {
 namespace {
 
 template <class T> struct Sim {
+  using Scalar = T;
   // ---- static scene (S:120-140, layout.ts:41-90)
   uint32_t V = 0, TT = 0, K = 0, bodies = 1;
   std::vector<V4<T>> rest;               // restMass
@@ -1807,6 +1809,27 @@ template <class T> void local_systems(Sim<T>& s, const jgs2o_settings& st, doubl
   }
 }
 
+
+// Restricted trial model of vertex `v` at displacement d (S:1980-2337):
+// out[0] = minimum determinant, out[1] = geometry valid, out[2] = energy
+// delta. Evaluated at the accepted pose with refreshed frames and the
+// accepted-minimum seed taken from a fresh source reduction.
+template <class T> void restricted_model(Sim<T>& s, const jgs2o_settings& st, uint32_t v, const double* d, double* out) {
+  s.set_params(st, false);
+  s.p.src = 0;
+  s.p.tgt = 0;
+  s.p.colored = false;
+  s.k_tet_polar();
+  s.k_vertex_polar();
+  s.k_candidate_tet();
+  s.k_reduce_candidate();
+  V3<T> dd = v3(T(d[0]), T(d[1]), T(d[2]));
+  typename Sim<T>::Geometry g = s.restricted_min_det(v, dd);
+  out[0] = double(g.minimum);
+  out[1] = g.valid ? 1.0 : 0.0;
+  out[2] = double(s.restricted_energy_delta(v, dd));
+}
+
 }  // namespace
 
 // ============================================================== exported API
@@ -1991,6 +2014,26 @@ double jgs2o_min_eigenvalue(int precision, const double* h_rm) {
     return double(min_sym_eigenvalue_normalized(H));
   };
   return precision == 64 ? run(0.0) : run(0.0f);
+}
+
+void jgs2o_restricted_model(void* hv, const jgs2o_settings* s, uint32_t vertex, const double* d, double* out) {
+  Handle* h = static_cast<Handle*>(hv);
+  if (h->d) restricted_model(*h->d, *s, vertex, d, out); else restricted_model(*h->f, *s, vertex, d, out);
+}
+
+// Sets positions (4 floats per vertex, xyz used) of both iteration buffers
+// and the predicted positions, for evaluating local models at chosen poses.
+void jgs2o_set_pose(void* hv, const float* positions, const float* predicted) {
+  Handle* h = static_cast<Handle*>(hv);
+  auto run = [&](auto& sim) {
+    using T = typename std::remove_reference<decltype(sim)>::type::Scalar;
+    for (uint32_t v = 0; v < sim.V; ++v) {
+      V3<T> x = v3(T(positions[4 * v]), T(positions[4 * v + 1]), T(positions[4 * v + 2]));
+      sim.pos[0][v] = sim.pos[1][v] = x;
+      if (predicted) sim.predicted[v] = v3(T(predicted[4 * v]), T(predicted[4 * v + 1]), T(predicted[4 * v + 2]));
+    }
+  };
+  if (h->d) run(*h->d); else run(*h->f);
 }
 
 // Deterministic greedy coloring (src/simulation/cpu/vertex-coloring.ts:20-110)

@@ -1,0 +1,342 @@
+"""ctypes wrapper of the CPU oracle (oracle/jgs2_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg. The product (webgpu-physics_amd/) never loads it.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIBRARY_PATH = ORACLE_DIR / "_build" / "libjgs2_oracle.so"
+
+f32p = C.POINTER(C.c_float)
+u32p = C.POINTER(C.c_uint32)
+f64p = C.POINTER(C.c_double)
+
+
+class OracleInput(C.Structure):
+    _fields_ = [
+        ("vertex_count", C.c_uint32),
+        ("tet_count", C.c_uint32),
+        ("cubature_k", C.c_uint32),
+        ("body_count", C.c_uint32),
+        ("positions", f32p),
+        ("velocities", f32p),
+        ("vertex_rest", f32p),
+        ("vertex_info", u32p),
+        ("tet_indices", u32p),
+        ("tet_inverse_dm", f32p),
+        ("tet_meta", f32p),
+        ("tet_rest_stiffness", f32p),
+        ("adjacency", u32p),
+        ("adjacency_length", C.c_uint64),
+        ("cubature_tet_ids", u32p),
+        ("cubature_weights", f32p),
+        ("cubature_basis", f32p),
+        ("objectives", f32p),
+        ("colors", u32p),
+        ("color_count", C.c_uint32),
+    ]
+
+
+class OracleSettings(C.Structure):
+    _fields_ = [
+        ("schedule", C.c_int32),
+        ("timestep", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("floor_height", C.c_double),
+        ("floor_stiffness", C.c_double),
+        ("velocity_damping", C.c_double),
+        ("regularization", C.c_double),
+        ("rotation_epsilon", C.c_double),
+        ("max_step", C.c_double),
+        ("residual_tolerance", C.c_double),
+        ("normalized_update_tolerance", C.c_double),
+        ("contact_tangential_damping", C.c_double),
+        ("contact_margin", C.c_double),
+        ("ipc_activation_distance", C.c_double),
+        ("ipc_barrier_stiffness", C.c_double),
+        ("horizontal_body_correction", C.c_int32),
+    ]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-C", str(ORACLE_DIR)], check=True, stdout=subprocess.DEVNULL)
+    return LIBRARY_PATH
+
+
+def _load() -> C.CDLL:
+    if not LIBRARY_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIBRARY_PATH))
+    sig = {
+        "jgs2o_create": (C.c_void_p, [C.POINTER(OracleInput), C.c_int]),
+        "jgs2o_destroy": (None, [C.c_void_p]),
+        "jgs2o_globalization_enabled": (C.c_int, [C.c_void_p]),
+        "jgs2o_step_frames": (None, [C.c_void_p, C.POINTER(OracleSettings), C.c_uint32, C.c_uint32, C.c_int]),
+        "jgs2o_set_objectives": (None, [C.c_void_p, f32p]),
+        "jgs2o_read_positions": (None, [C.c_void_p, f64p]),
+        "jgs2o_read_velocities": (None, [C.c_void_p, f64p]),
+        "jgs2o_read_predicted": (None, [C.c_void_p, f64p]),
+        "jgs2o_read_final_updates": (None, [C.c_void_p, f64p]),
+        "jgs2o_read_local_diagnostics": (None, [C.c_void_p, f64p]),
+        "jgs2o_read_control": (None, [C.c_void_p, f64p]),
+        "jgs2o_read_history": (None, [C.c_void_p, f64p]),
+        "jgs2o_read_vertex_rotations": (None, [C.c_void_p, f64p]),
+        "jgs2o_local_systems": (None, [C.c_void_p, C.POINTER(OracleSettings), f64p, f64p]),
+        "jgs2o_restricted_model": (None, [C.c_void_p, C.POINTER(OracleSettings), C.c_uint32, f64p, f64p]),
+        "jgs2o_set_pose": (None, [C.c_void_p, f32p, f32p]),
+        "jgs2o_snh": (None, [C.c_int, f64p, f64p, C.c_double, C.c_double, f64p, f64p, f64p]),
+        "jgs2o_snh_energy_delta": (C.c_double, [C.c_int, f64p, f64p, C.c_double, C.c_double]),
+        "jgs2o_globalization_solve": (C.c_int, [C.c_int, f64p, f64p, C.c_double, f64p]),
+        "jgs2o_min_eigenvalue": (C.c_double, [C.c_int, f64p]),
+        "jgs2o_greedy_coloring": (C.c_uint32, [C.c_uint32, u32p, C.c_uint32, u32p, u32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+# DEFAULT_JGS2_STEP_SETTINGS (src/simulation/gpu/jgs2-solver.ts:164-187).
+DEFAULTS = {
+    "schedule": "jacobi",
+    "timestep": 1.0 / 60.0,
+    "gravity": (0.0, -9.81, 0.0),
+    "iterations": 9,
+    "floor_height": 0.0,
+    "floor_stiffness": 20000.0,
+    "velocity_damping": 0.999,
+    "regularization": 1e-6,
+    "rotation_epsilon": 1e-7,
+    "max_step": 0.1,
+    "residual_tolerance": 1e-3,
+    "normalized_update_tolerance": 1e-3,
+    "contact_tangential_damping": 8.0,
+    "contact_margin": 0.01,
+    "ipc_activation_distance": 0.05,
+    "ipc_barrier_stiffness": 0.0,
+    "horizontal_body_correction": True,
+    "parity_mode": False,
+}
+
+
+def resolve(overrides=None) -> dict:
+    """resolveJGS2StepSettings (jgs2-solver.ts:531-550) without validation."""
+    s = dict(DEFAULTS)
+    s.update(overrides or {})
+    if s.get("parity_mode"):
+        s["velocity_damping"] = 1.0
+        s["contact_tangential_damping"] = 0.0
+        s["horizontal_body_correction"] = False
+    return s
+
+
+def settings_struct(resolved: dict) -> OracleSettings:
+    o = OracleSettings()
+    o.schedule = 1 if resolved["schedule"] == "graph-colored-gauss-seidel" else 0
+    for key in ("timestep", "floor_height", "floor_stiffness", "velocity_damping", "regularization",
+                "rotation_epsilon", "max_step", "residual_tolerance", "normalized_update_tolerance",
+                "contact_tangential_damping", "contact_margin", "ipc_activation_distance", "ipc_barrier_stiffness"):
+        setattr(o, key, float(resolved[key]))
+    for i in range(3):
+        o.gravity[i] = float(resolved["gravity"][i])
+    o.horizontal_body_correction = 1 if resolved["horizontal_body_correction"] else 0
+    return o
+
+
+def greedy_coloring(vertex_count: int, groups) -> np.ndarray:
+    offsets = [0]
+    members = []
+    for g in groups:
+        members.extend(int(v) for v in g)
+        offsets.append(len(members))
+    off = np.asarray(offsets, dtype=np.uint32)
+    mem = np.asarray(members if members else [0], dtype=np.uint32)
+    colors = np.empty(vertex_count, dtype=np.uint32)
+    lib.jgs2o_greedy_coloring(vertex_count, off.ctypes.data_as(u32p), len(offsets) - 1, mem.ctypes.data_as(u32p),
+                              colors.ctypes.data_as(u32p))
+    return colors
+
+
+def conflict_groups(scene) -> list:
+    """buildJGS2InputVertexColoring conflict groups (jgs2-solver.ts:308-367)."""
+    T, K, V = scene.tet_count, scene.cubature_k, scene.vertex_count
+    tets = scene.tet_indices.reshape(T, 4)
+    groups = [list(t) for t in tets]
+    if K:
+        ids = scene.cubature_tet_ids.reshape(V, K)
+        w = scene.cubature_weights.reshape(V, K)
+        for v in range(V):
+            for k in range(K):
+                t = int(ids[v, k])
+                if t == 0xFFFFFFFF or not (w[v, k] > 0):
+                    continue
+                groups.append([v, *tets[t]])
+    return groups
+
+
+class Oracle:
+    """The reference step restated on the CPU, FP32 (precision=32) or FP64."""
+
+    def __init__(self, scene, precision: int = 32, colors=None, objectives=None):
+        self.scene = scene
+        self.V = scene.vertex_count
+        self.precision = precision
+        info = scene.vertex_info.reshape(-1, 4)
+        self._keep = []
+        inp = OracleInput()
+        inp.vertex_count = scene.vertex_count
+        inp.tet_count = scene.tet_count
+        inp.cubature_k = scene.cubature_k
+        inp.body_count = int(info[:, 3].max()) + 1
+
+        def fp(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            self._keep.append(a)
+            return a.ctypes.data_as(f32p)
+
+        def up(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.uint32)
+            self._keep.append(a)
+            return a.ctypes.data_as(u32p)
+
+        inp.positions = fp(scene.positions)
+        inp.velocities = fp(scene.velocities)
+        inp.vertex_rest = fp(scene.vertex_rest)
+        inp.vertex_info = up(scene.vertex_info)
+        inp.tet_indices = up(scene.tet_indices)
+        inp.tet_inverse_dm = fp(scene.tet_inverse_dm)
+        inp.tet_meta = fp(scene.tet_meta)
+        inp.tet_rest_stiffness = fp(scene.tet_rest_stiffness)
+        inp.adjacency = up(scene.adjacency)
+        inp.adjacency_length = int(scene.adjacency.size)
+        inp.cubature_tet_ids = up(scene.cubature_tet_ids)
+        inp.cubature_weights = fp(scene.cubature_weights)
+        inp.cubature_basis = fp(scene.cubature_basis)
+        inp.objectives = fp(objectives if objectives is not None else scene.packed_objectives())
+        if colors is not None:
+            inp.colors = up(colors)
+            inp.color_count = int(np.max(colors)) + 1
+        self._handle = lib.jgs2o_create(C.byref(inp), precision)
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            lib.jgs2o_destroy(self._handle)
+            self._handle = None
+
+    @property
+    def globalization(self) -> bool:
+        return bool(lib.jgs2o_globalization_enabled(self._handle))
+
+    def step_frames(self, frames: int, iterations: int, settings=None, stop: bool = False) -> None:
+        st = settings_struct(resolve(settings))
+        lib.jgs2o_step_frames(self._handle, C.byref(st), int(frames), int(iterations), 1 if stop else 0)
+
+    def step(self, settings=None) -> None:
+        """step(): odd-normalized iterations with the stop latch (layout.ts:742-748)."""
+        it = int(resolve(settings)["iterations"])
+        it = max(1, it)
+        it = it + 1 if it % 2 == 0 else it
+        self.step_frames(1, it, settings, stop=True)
+
+    def set_objectives(self, packed: np.ndarray) -> None:
+        a = np.ascontiguousarray(packed, dtype=np.float32)
+        lib.jgs2o_set_objectives(self._handle, a.ctypes.data_as(f32p))
+
+    def set_pose(self, positions: np.ndarray, predicted: np.ndarray = None) -> None:
+        p = np.ascontiguousarray(positions, dtype=np.float32)
+        q = None if predicted is None else np.ascontiguousarray(predicted, dtype=np.float32)
+        lib.jgs2o_set_pose(self._handle, p.ctypes.data_as(f32p), None if q is None else q.ctypes.data_as(f32p))
+
+    def _read(self, fn, n: int) -> np.ndarray:
+        out = np.empty(n, dtype=np.float64)
+        fn(self._handle, out.ctypes.data_as(f64p))
+        return out
+
+    def positions(self) -> np.ndarray:
+        return self._read(lib.jgs2o_read_positions, 3 * self.V).reshape(-1, 3)
+
+    def velocities(self) -> np.ndarray:
+        return self._read(lib.jgs2o_read_velocities, 3 * self.V).reshape(-1, 3)
+
+    def predicted(self) -> np.ndarray:
+        return self._read(lib.jgs2o_read_predicted, 3 * self.V).reshape(-1, 3)
+
+    def final_updates(self) -> np.ndarray:
+        return self._read(lib.jgs2o_read_final_updates, 4 * self.V).reshape(-1, 4)
+
+    def local_diagnostics(self) -> np.ndarray:
+        return self._read(lib.jgs2o_read_local_diagnostics, 16 * self.V).reshape(-1, 4, 4)
+
+    def control(self) -> np.ndarray:
+        return self._read(lib.jgs2o_read_control, 16).reshape(4, 4)
+
+    def history(self) -> np.ndarray:
+        return self._read(lib.jgs2o_read_history, 64 * 32).reshape(64, 8, 4)
+
+    def vertex_rotations(self) -> np.ndarray:
+        return self._read(lib.jgs2o_read_vertex_rotations, 9 * self.V).reshape(-1, 3, 3)
+
+    def local_systems(self, settings=None):
+        st = settings_struct(resolve(settings))
+        g = np.empty(3 * self.V)
+        h = np.empty(9 * self.V)
+        lib.jgs2o_local_systems(self._handle, C.byref(st), g.ctypes.data_as(f64p), h.ctypes.data_as(f64p))
+        return g.reshape(-1, 3), h.reshape(-1, 3, 3)
+
+    def restricted_model(self, vertex: int, d, settings=None):
+        st = settings_struct(resolve(settings))
+        dd = np.ascontiguousarray(d, dtype=np.float64)
+        out = np.empty(3)
+        lib.jgs2o_restricted_model(self._handle, C.byref(st), int(vertex), dd.ctypes.data_as(f64p),
+                                   out.ctypes.data_as(f64p))
+        return out[0], bool(out[1]), out[2]
+
+
+def snh(F, H, lam, mu, precision=64):
+    f = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    h = np.ascontiguousarray(H, dtype=np.float64).reshape(9)
+    e = C.c_double()
+    p = np.empty(9)
+    t = np.empty(9)
+    lib.jgs2o_snh(precision, f.ctypes.data_as(f64p), h.ctypes.data_as(f64p), lam, mu, C.byref(e),
+                  p.ctypes.data_as(f64p), t.ctypes.data_as(f64p))
+    return e.value, p.reshape(3, 3), t.reshape(3, 3)
+
+
+def snh_energy_delta(F0, F1, lam, mu, precision=64) -> float:
+    a = np.ascontiguousarray(F0, dtype=np.float64).reshape(9)
+    b = np.ascontiguousarray(F1, dtype=np.float64).reshape(9)
+    return lib.jgs2o_snh_energy_delta(precision, a.ctypes.data_as(f64p), b.ctypes.data_as(f64p), lam, mu)
+
+
+def globalization_solve(H, g, inertia_scale, precision=64):
+    h = np.ascontiguousarray(H, dtype=np.float64).reshape(9)
+    gg = np.ascontiguousarray(g, dtype=np.float64).reshape(3)
+    out = np.empty(9)
+    status = lib.jgs2o_globalization_solve(precision, h.ctypes.data_as(f64p), gg.ctypes.data_as(f64p),
+                                           float(inertia_scale), out.ctypes.data_as(f64p))
+    return status, {
+        "direction": out[0:3].copy(), "gradient_dot_direction": out[3], "scale": out[4],
+        "minimum_eigenvalue_normalized": out[5], "normalized_shift": out[6], "maximum_relative_asymmetry": out[7],
+        "shifted_relative_residual": out[8],
+    }
+
+
+def min_eigenvalue(H, precision=64) -> float:
+    h = np.ascontiguousarray(H, dtype=np.float64).reshape(9)
+    return lib.jgs2o_min_eigenvalue(precision, h.ctypes.data_as(f64p))

@@ -61,8 +61,10 @@ def build_cuboid(desc: N.CuboidDesc) -> GpuInput:
         N.check(N.lib.jgs2_scene_get_input(handle, C.byref(s)))
         V, T, K = s.vertex_count, s.tet_count, s.cubature_k
 
-        def f(ptr, n):
-            return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if ptr else None
+        def f(ptr, n, optional=False):
+            if n == 0 or not ptr:
+                return None if optional else np.zeros(0, dtype=np.float32)
+            return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
 
         return GpuInput(
             vertex_count=V, tet_count=T, cubature_k=K,
@@ -76,7 +78,7 @@ def build_cuboid(desc: N.CuboidDesc) -> GpuInput:
             cubature_tet_ids=f(s.cubature_tet_ids, V * K),
             cubature_weights=f(s.cubature_weights, V * K),
             cubature_basis=f(s.cubature_basis, V * K * 36),
-            tet_rest_stiffness=f(s.tet_rest_stiffness, 144 * T),
+            tet_rest_stiffness=f(s.tet_rest_stiffness, 144 * T, optional=True),
         )
     finally:
         N.lib.jgs2_scene_destroy(handle)
