@@ -55,6 +55,7 @@ def main(argv=None):
     ap.add_argument("--cells", default="64,64,128")
     ap.add_argument("--iterations", type=int, default=8)
     ap.add_argument("--latency-frames", type=int, default=10)
+    ap.add_argument("--other-schedule-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args(argv)
@@ -114,7 +115,16 @@ def main(argv=None):
     achieved = bytes_per_sweep / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     x = solver.read_positions().reshape(-1, 4)[:, :3]
     finite = bool(np.isfinite(x).all())
+    hist = solver.read_globalization_history().reshape(64, 8, 4)
+    dynamics = {"reverted_iterations_last_frame": int(hist[:it, 1, 3].sum()),
+                "max_backtracks_last_frame": int(hist[:it, 7, 0].max()),
+                "relative_residual_last_iteration": float(hist[it - 1, 4, 3])}
     solver.destroy()
+
+    other = None
+    if args.other_schedule_steps > 0:
+        other_name = "graph-colored-gauss-seidel" if args.schedule == "jacobi" else "jacobi"
+        other = measure_schedule(scene, dict(settings, schedule=other_name), it, args.other_schedule_steps)
 
     result = {
         "metric": METRIC,
@@ -153,7 +163,10 @@ def main(argv=None):
             "launches_profiled": launches,
         },
         "positions_finite": finite,
+        "dynamics": dynamics,
     }
+    if other is not None:
+        result["other_schedule"] = other
 
     if not args.no_accuracy:
         result["accuracy"] = accuracy_check(settings, it)
@@ -161,6 +174,28 @@ def main(argv=None):
         result["cpu_baseline"] = cpu_baseline(settings, it)
     print(json.dumps(result), flush=True)
     return 0
+
+
+def measure_schedule(scene, settings, iterations, steps):
+    """Throughput of the second schedule on the same scene (fresh solver)."""
+    import jgs2
+
+    solver = jgs2.JGS2GpuSolver.create(scene, settings)
+    solver.step_exact_iterations(iterations, settings)
+    solver.await_idle()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.step_exact_iterations(iterations, settings)
+    solver.await_idle()
+    dt = time.perf_counter() - t0
+    hist = solver.read_globalization_history().reshape(64, 8, 4)
+    out = {"schedule": settings["schedule"], "colors": solver.schedule_color_count, "steps": steps,
+           "ms_per_step": dt / steps * 1e3,
+           "vertex_updates_per_s": steps * iterations * solver.free_vertex_count / dt,
+           "reverted_iterations_last_frame": int(hist[:iterations, 1, 3].sum())}
+    solver.destroy()
+    log(f"{settings['schedule']}: {out['ms_per_step']:.2f} ms/frame")
+    return out
 
 
 def _small_scene():
