@@ -18,6 +18,9 @@ struct PassParams {
   uint32_t globalization;     // stable Neo-Hookean production path
   uint32_t stop_enabled;      // convergence stop latch honored
   uint32_t corotated;         // legacy co-rotated material path
+  uint32_t fused_rot;         // solve reads frames from vrot_next and adopts them
+  uint32_t compute_next_rot;  // convergence pass also computes the next frames
+  uint32_t pad1, pad2;
   float dt, inv_dt, inv_dt2, max_step;
   float gx, gy, gz, floor_y;
   float floor_k, regularization, polar_eps, damping;
@@ -51,6 +54,7 @@ struct DeviceState {
   float4* velocity = nullptr;
   float4* old = nullptr;
   float4* vrot = nullptr;       // 3 per vertex (columns)
+  float4* vrot_next = nullptr;  // frames for the next sweep, 3 per vertex
   float4* trot = nullptr;       // 3 per tet (legacy path)
   float4* body_corr = nullptr;
   float4* final_update = nullptr;

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -64,8 +65,12 @@ struct jgs2_solver {
   bool globalization = true;
   bool corotated = false;
   uint32_t V = 0, T = 0, K = 0, bodies = 1, color_count = 1, free_vertices = 0;
-  std::vector<uint32_t> colors;
+  std::vector<uint32_t> colors;         // by caller vertex id
   std::vector<uint32_t> color_offsets;  // color_count + 1
+  // Device order: vertices and tetrahedra are stored along a Morton curve of
+  // their rest positions so that neighbors share cache lines and L2; the
+  // boundary maps ids both ways.
+  std::vector<uint32_t> vnew, vold, tnew, told;
   std::vector<float> objectives;        // 8 per vertex (host mirror)
   std::vector<uint8_t> pinned;
   uint32_t objective_flags = 0;
@@ -82,7 +87,7 @@ namespace {
 
 void free_device(DeviceState& d) {
   void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_idx, d.tet_dm, d.tet_k, d.adjacency, d.cub, d.objectives,
-                  d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.trot,
+                  d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h};
   for (void* p : ptrs)
@@ -382,6 +387,73 @@ jgs2_status color_vertices(const jgs2_scene_input& in, std::vector<uint32_t>& co
                                      &count);
 }
 
+// ---------------------------------------------------------------- device order
+uint64_t spread_bits(uint64_t x) {  // 21 bits -> every third of 63
+  x &= 0x1fffffull;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+
+// Device order of vertices and tetrahedra: caller order, or (on request)
+// Morton order of rest positions and rest centroids with ties in caller
+// order. Returns both directions of each map.
+void morton_orders(const jgs2_scene_input& in, std::vector<uint32_t>& vnew, std::vector<uint32_t>& vold,
+                   std::vector<uint32_t>& tnew, std::vector<uint32_t>& told) {
+  const uint32_t V = in.vertex_count, T = in.tet_count;
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < V; ++v)
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = std::min(mn[a], double(in.vertex_rest[4 * size_t(v) + a]));
+      mx[a] = std::max(mx[a], double(in.vertex_rest[4 * size_t(v) + a]));
+    }
+  const double extent = std::max({mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2], 1e-30});
+  auto code = [&](double x, double y, double z) {
+    const double q = double((1u << 21) - 1) / extent;
+    uint64_t ix = uint64_t(std::max(0.0, (x - mn[0]) * q));
+    uint64_t iy = uint64_t(std::max(0.0, (y - mn[1]) * q));
+    uint64_t iz = uint64_t(std::max(0.0, (z - mn[2]) * q));
+    return spread_bits(ix) | spread_bits(iy) << 1 | spread_bits(iz) << 2;
+  };
+  std::vector<std::pair<uint64_t, uint32_t>> keys(V);
+  for (uint32_t v = 0; v < V; ++v) {
+    const float* r = &in.vertex_rest[4 * size_t(v)];
+    keys[v] = {code(r[0], r[1], r[2]), v};
+  }
+  // Caller order by default (builders already emit grid-coherent ids);
+  // JGS2HIP_DEVICE_ORDER=morton reorders unstructured inputs.
+  const char* order = std::getenv("JGS2HIP_DEVICE_ORDER");
+  const bool identity = !(order && std::strcmp(order, "morton") == 0);
+  if (identity)
+    for (uint32_t v = 0; v < V; ++v) keys[v].first = 0;
+  std::sort(keys.begin(), keys.end());
+  vold.resize(V);
+  vnew.resize(V);
+  for (uint32_t i = 0; i < V; ++i) {
+    vold[i] = keys[i].second;
+    vnew[keys[i].second] = i;
+  }
+  std::vector<std::pair<uint64_t, uint32_t>> tkeys(T);
+  for (uint32_t t = 0; t < T; ++t) {
+    double c[3] = {0, 0, 0};
+    for (int l = 0; l < 4; ++l)
+      for (int a = 0; a < 3; ++a) c[a] += 0.25 * in.vertex_rest[4 * size_t(in.tet_indices[4 * size_t(t) + l]) + a];
+    tkeys[t] = {code(c[0], c[1], c[2]), t};
+  }
+  if (identity)
+    for (uint32_t t = 0; t < T; ++t) tkeys[t].first = 0;
+  std::sort(tkeys.begin(), tkeys.end());
+  told.resize(T);
+  tnew.resize(T);
+  for (uint32_t i = 0; i < T; ++i) {
+    told[i] = tkeys[i].second;
+    tnew[tkeys[i].second] = i;
+  }
+}
+
 // ---------------------------------------------------------------- pass parameters
 PassParams make_params(const jgs2_solver* s, const jgs2_settings& st, bool stop) {
   PassParams p{};
@@ -457,14 +529,21 @@ void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations,
       }
       if (s->globalization) launch_vertex_energy(d, p, q);
     } else {
+      // Sweeps after the first read the frames the previous convergence pass
+      // computed at this sweep's source pose.
+      const bool fused = s->globalization && it > 0;
       if (s->corotated) launch_tet_polar(d, p, q);
-      launch_vertex_polar(d, p, q);
-      record_solve(s, p);
+      if (!fused) launch_vertex_polar(d, p, q);
+      PassParams ps = p;
+      ps.fused_rot = fused ? 1u : 0u;
+      record_solve(s, ps);
     }
     if (s->globalization) {
       launch_reduce_candidate(d, p, q);
       launch_apply_candidate(d, p, q);
-      launch_convergence(d, p, q);
+      PassParams pc = p;
+      pc.compute_next_rot = (!gs && it + 1 < iterations) ? 1u : 0u;
+      launch_convergence(d, pc, q);
     }
   }
   if (iterations % 2 == 0) {
@@ -543,8 +622,19 @@ void recount_objectives(jgs2_solver* s) {  // computePackedObjectiveActivity (jg
 }
 
 jgs2_status upload_objectives(jgs2_solver* s, size_t first_vertex, size_t vertex_count) {
-  HIP_CHECK(hipMemcpyAsync(s->dev.objectives + 2 * first_vertex, &s->objectives[8 * first_vertex],
-                           vertex_count * 32, hipMemcpyHostToDevice, s->stream));
+  // The host mirror is in caller vertex ids; device records are in device order.
+  std::vector<float> staged(8 * vertex_count);
+  if (vertex_count == s->V) {
+    for (uint32_t nv = 0; nv < s->V; ++nv)
+      std::memcpy(&staged[8 * size_t(nv)], &s->objectives[8 * size_t(s->vold[nv])], 32);
+    HIP_CHECK(hipMemcpyAsync(s->dev.objectives, staged.data(), 32 * size_t(s->V), hipMemcpyHostToDevice, s->stream));
+  } else {
+    for (size_t i = 0; i < vertex_count; ++i) {
+      const size_t ov = first_vertex + i;
+      HIP_CHECK(hipMemcpyAsync(s->dev.objectives + 2 * size_t(s->vnew[ov]), &s->objectives[8 * ov], 32,
+                               hipMemcpyHostToDevice, s->stream));
+    }
+  }
   HIP_CHECK(hipStreamSynchronize(s->stream));
   recount_objectives(s);
   return JGS2_OK;
@@ -558,13 +648,23 @@ jgs2_status objective_supported(jgs2_solver* s) {
   return JGS2_OK;
 }
 
-jgs2_status read_buffer(jgs2_solver* s, const void* src, float* dst, size_t count, size_t expected, const char* what) {
+// Copies a device array to the caller. Per-vertex arrays (`stride` floats
+// per vertex) are returned in caller vertex ids.
+jgs2_status read_buffer(jgs2_solver* s, const void* src, float* dst, size_t count, size_t expected, const char* what,
+                        size_t stride = 0) {
   jgs2_status st = check_usable(s);
   if (st != JGS2_OK) return st;
   if (!dst) return fail(JGS2_ERROR_TYPE, "NULL destination for %s.", what);
   if (count != expected) return fail(JGS2_ERROR_RANGE, "%s needs exactly %zu floats; got %zu.", what, expected, count);
   HIP_CHECK(hipStreamSynchronize(s->stream));
-  HIP_CHECK(hipMemcpy(dst, src, expected * sizeof(float), hipMemcpyDeviceToHost));
+  if (stride == 0) {
+    HIP_CHECK(hipMemcpy(dst, src, expected * sizeof(float), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> staged(expected);
+    HIP_CHECK(hipMemcpy(staged.data(), src, expected * sizeof(float), hipMemcpyDeviceToHost));
+    for (uint32_t ov = 0; ov < s->V; ++ov)
+      std::memcpy(dst + stride * ov, &staged[stride * size_t(s->vnew[ov])], stride * sizeof(float));
+  }
   clear_error();
   return JGS2_OK;
 }
@@ -714,14 +814,21 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
     double diag = dyn ? std::hypot(std::hypot(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]) : 0.0;
     s->scene_scale = float((dyn > 0 && std::isfinite(diag) && diag > 0) ? diag : 1e-12);
   }
-  // Color-sorted vertex order (vertex-coloring.ts:92-107).
+  // Space-filling device order of vertices and tetrahedra.
+  morton_orders(in, s->vnew, s->vold, s->tnew, s->told);
+  // Color-sorted vertex order (vertex-coloring.ts:92-107), in device ids.
   s->color_offsets.assign(color_count + 1, 0);
   for (uint32_t v = 0; v < V; ++v) s->color_offsets[colors[v] + 1] += 1;
   for (uint32_t c = 0; c < color_count; ++c) s->color_offsets[c + 1] += s->color_offsets[c];
   std::vector<uint32_t> order(V);
+  std::vector<uint32_t> dev_colors(V);
   {
     std::vector<uint32_t> cursor(s->color_offsets.begin(), s->color_offsets.end() - 1);
-    for (uint32_t v = 0; v < V; ++v) order[cursor[colors[v]]++] = v;
+    for (uint32_t nv = 0; nv < V; ++nv) {
+      const uint32_t c = colors[s->vold[nv]];
+      order[cursor[c]++] = nv;
+      dev_colors[nv] = c;
+    }
   }
 
   auto cleanup = [&](jgs2_status code) {
@@ -747,45 +854,73 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
   d.T = T;
   d.K = K;
   d.bodies = s->bodies;
-  // Static records.
+  // Static records in device order.
+  const std::vector<uint32_t>& vnew = s->vnew;
+  const std::vector<uint32_t>& vold = s->vold;
+  const std::vector<uint32_t>& tnew = s->tnew;
+  const std::vector<uint32_t>& told = s->told;
+  std::vector<uint32_t> adjacency(in.adjacency_length);
   std::vector<float4> vtx_rest(V);
   std::vector<uint4> vtx_info(V);
-  for (uint32_t v = 0; v < V; ++v) {
-    const float* r = &in.vertex_rest[4 * size_t(v)];
-    const uint32_t* i = &in.vertex_info[4 * size_t(v)];
-    vtx_rest[v] = make_float4(r[0], r[1], r[2], r[3]);
-    vtx_info[v] = make_uint4(i[0], i[1], i[2], i[3]);
+  {
+    uint32_t cursor = 0;
+    for (uint32_t nv = 0; nv < V; ++nv) {
+      const uint32_t ov = vold[nv];
+      const float* r = &in.vertex_rest[4 * size_t(ov)];
+      const uint32_t* i = &in.vertex_info[4 * size_t(ov)];
+      vtx_rest[nv] = make_float4(r[0], r[1], r[2], r[3]);
+      vtx_info[nv] = make_uint4(cursor, i[1], i[2], i[3]);
+      // Row order stays ascending in caller tet ids (the reference's
+      // summation order); entries are device tet ids.
+      for (uint32_t k = 0; k < i[1]; ++k) adjacency[cursor + k] = tnew[in.adjacency[i[0] + k]];
+      cursor += i[1];
+    }
   }
   std::vector<uint4> tet_idx(T);
   std::vector<float4> tet_dm(3 * size_t(T));
-  for (uint32_t t = 0; t < T; ++t) {
-    const uint32_t* i = &in.tet_indices[4 * size_t(t)];
-    const float* m = &in.tet_inverse_dm[12 * size_t(t)];
-    const float* a = &in.tet_meta[4 * size_t(t)];
-    tet_idx[t] = make_uint4(i[0], i[1], i[2], i[3]);
-    tet_dm[3 * size_t(t)] = make_float4(m[0], m[1], m[2], a[0]);
-    tet_dm[3 * size_t(t) + 1] = make_float4(m[4], m[5], m[6], a[1]);
-    tet_dm[3 * size_t(t) + 2] = make_float4(m[8], m[9], m[10], a[2]);
+  std::vector<float> tet_k;
+  if (corotated) tet_k.resize(144 * size_t(T));
+  for (uint32_t nt = 0; nt < T; ++nt) {
+    const uint32_t ot = told[nt];
+    const uint32_t* i = &in.tet_indices[4 * size_t(ot)];
+    const float* m = &in.tet_inverse_dm[12 * size_t(ot)];
+    const float* a = &in.tet_meta[4 * size_t(ot)];
+    tet_idx[nt] = make_uint4(vnew[i[0]], vnew[i[1]], vnew[i[2]], vnew[i[3]]);
+    tet_dm[3 * size_t(nt)] = make_float4(m[0], m[1], m[2], a[0]);
+    tet_dm[3 * size_t(nt) + 1] = make_float4(m[4], m[5], m[6], a[1]);
+    tet_dm[3 * size_t(nt) + 2] = make_float4(m[8], m[9], m[10], a[2]);
+    if (corotated) std::memcpy(&tet_k[144 * size_t(nt)], &in.tet_rest_stiffness[144 * size_t(ot)], 144 * sizeof(float));
   }
   const size_t R = size_t(V) * K;
   std::vector<uint4> cub(10 * std::max<size_t>(R, 1));
-  for (size_t r = 0; r < R; ++r) {
-    uint32_t wbits;
-    std::memcpy(&wbits, &in.cubature_weights[r], 4);
-    cub[10 * r] = make_uint4(in.cubature_tet_ids[r], wbits, 0, 0);
-    std::memcpy(&cub[10 * r + 1], &in.cubature_basis[36 * r], 36 * sizeof(float));
+  for (uint32_t nv = 0; nv < V; ++nv) {
+    for (uint32_t k = 0; k < K; ++k) {
+      const size_t ro = size_t(vold[nv]) * K + k, rn = size_t(nv) * K + k;
+      uint32_t wbits;
+      std::memcpy(&wbits, &in.cubature_weights[ro], 4);
+      const uint32_t t = in.cubature_tet_ids[ro];
+      cub[10 * rn] = make_uint4(t < T ? tnew[t] : 0xffffffffu, wbits, 0, 0);
+      std::memcpy(&cub[10 * rn + 1], &in.cubature_basis[36 * ro], 36 * sizeof(float));
+    }
+  }
+  std::vector<float4> dev_objectives(2 * size_t(V));
+  for (uint32_t nv = 0; nv < V; ++nv) {
+    const float* o = &s->objectives[8 * size_t(vold[nv])];
+    dev_objectives[2 * size_t(nv)] = make_float4(o[0], o[1], o[2], o[3]);
+    dev_objectives[2 * size_t(nv) + 1] = make_float4(o[4], o[5], o[6], o[7]);
   }
   std::vector<float4> positions(V), velocities(V, make_float4(0, 0, 0, 0)), rot_v(3 * size_t(V)),
       rot_t(3 * size_t(T));
-  for (uint32_t v = 0; v < V; ++v) {
-    const float* x = &in.positions[4 * size_t(v)];
-    positions[v] = make_float4(x[0], x[1], x[2], 1.0f);
+  for (uint32_t nv = 0; nv < V; ++nv) {
+    const uint32_t ov = vold[nv];
+    const float* x = &in.positions[4 * size_t(ov)];
+    positions[nv] = make_float4(x[0], x[1], x[2], 1.0f);
     if (in.velocities)
-      velocities[v] = make_float4(in.velocities[4 * size_t(v)], in.velocities[4 * size_t(v) + 1],
-                                  in.velocities[4 * size_t(v) + 2], 0.0f);
-    rot_v[3 * size_t(v)] = make_float4(1, 0, 0, 0);
-    rot_v[3 * size_t(v) + 1] = make_float4(0, 1, 0, 0);
-    rot_v[3 * size_t(v) + 2] = make_float4(0, 0, 1, 0);
+      velocities[nv] = make_float4(in.velocities[4 * size_t(ov)], in.velocities[4 * size_t(ov) + 1],
+                                   in.velocities[4 * size_t(ov) + 2], 0.0f);
+    rot_v[3 * size_t(nv)] = make_float4(1, 0, 0, 0);
+    rot_v[3 * size_t(nv) + 1] = make_float4(0, 1, 0, 0);
+    rot_v[3 * size_t(nv) + 2] = make_float4(0, 0, 1, 0);
   }
   for (uint32_t t = 0; t < T; ++t) {
     rot_t[3 * size_t(t)] = make_float4(1, 0, 0, 0);
@@ -808,6 +943,7 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
   TRY(dalloc(&d.velocity, V));
   TRY(dalloc(&d.old, V));
   TRY(dalloc(&d.vrot, 3 * size_t(V)));
+  TRY(dalloc(&d.vrot_next, 3 * size_t(V)));
   TRY(dalloc(&d.trot, 3 * size_t(T)));
   TRY(dalloc(&d.body_corr, s->bodies));
   TRY(dalloc(&d.final_update, V));
@@ -825,17 +961,18 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
   TRY_HIP(hipMemcpyAsync(d.tet_idx, tet_idx.data(), T * sizeof(uint4), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.tet_dm, tet_dm.data(), tet_dm.size() * sizeof(float4), hipMemcpyHostToDevice, q));
   if (corotated)
-    TRY_HIP(hipMemcpyAsync(d.tet_k, in.tet_rest_stiffness, 144 * size_t(T) * sizeof(float), hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.tet_k, tet_k.data(), 144 * size_t(T) * sizeof(float), hipMemcpyHostToDevice, q));
   if (in.adjacency_length)
-    TRY_HIP(hipMemcpyAsync(d.adjacency, in.adjacency, in.adjacency_length * 4, hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.adjacency, adjacency.data(), in.adjacency_length * 4, hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.cub, cub.data(), cub.size() * sizeof(uint4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.objectives, s->objectives.data(), 32 * size_t(V), hipMemcpyHostToDevice, q));
+  TRY_HIP(hipMemcpyAsync(d.objectives, dev_objectives.data(), 32 * size_t(V), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.color_order, order.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.colors, colors.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
+  TRY_HIP(hipMemcpyAsync(d.colors, dev_colors.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
   for (float4* region : {d.pos[0], d.pos[1], d.predicted, d.old})
     TRY_HIP(hipMemcpyAsync(region, positions.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.velocity, velocities.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.vrot, rot_v.data(), rot_v.size() * sizeof(float4), hipMemcpyHostToDevice, q));
+  TRY_HIP(hipMemcpyAsync(d.vrot_next, rot_v.data(), rot_v.size() * sizeof(float4), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.trot, rot_t.data(), rot_t.size() * sizeof(float4), hipMemcpyHostToDevice, q));
   {
     std::vector<float4> zeros(std::max<size_t>({size_t(5) * V, size_t(4) * V, size_t(2) * T, size_t(kHistoryCapacity) * 8,
@@ -1049,28 +1186,28 @@ jgs2_status jgs2_release_all_quadratic_targets(jgs2_solver* s) {
 
 jgs2_status jgs2_read_positions(jgs2_solver* s, float* dst, size_t count) {
   if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.pos[0], dst, count, 4 * size_t(s->V), "readPositions");
+  return read_buffer(s, s->dev.pos[0], dst, count, 4 * size_t(s->V), "readPositions", 4);
 }
 jgs2_status jgs2_read_velocities(jgs2_solver* s, float* dst, size_t count) {
   if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.velocity, dst, count, 4 * size_t(s->V), "readVelocities");
+  return read_buffer(s, s->dev.velocity, dst, count, 4 * size_t(s->V), "readVelocities", 4);
 }
 jgs2_status jgs2_read_predicted_positions(jgs2_solver* s, float* dst, size_t count) {
   if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.predicted, dst, count, 4 * size_t(s->V), "readPredictedPositions");
+  return read_buffer(s, s->dev.predicted, dst, count, 4 * size_t(s->V), "readPredictedPositions", 4);
 }
 jgs2_status jgs2_read_final_iteration_updates(jgs2_solver* s, float* dst, size_t count) {
   if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.final_update, dst, count, 4 * size_t(s->V), "readFinalIterationUpdates");
+  return read_buffer(s, s->dev.final_update, dst, count, 4 * size_t(s->V), "readFinalIterationUpdates", 4);
 }
 jgs2_status jgs2_read_vertex_rotations(jgs2_solver* s, float* dst, size_t count) {
   if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.vrot, dst, count, 12 * size_t(s->V), "readVertexRotations");
+  return read_buffer(s, s->dev.vrot, dst, count, 12 * size_t(s->V), "readVertexRotations", 12);
 }
 jgs2_status jgs2_read_local_diagnostics(jgs2_solver* s, float* dst, size_t count) {
   if (jgs2_status st = check_usable(s)) return st;
   if (!s->globalization) return fail(JGS2_ERROR_STATE, "Globalization diagnostics require the stable material path.");
-  return read_buffer(s, s->dev.local_diag, dst, count, 16 * size_t(s->V), "readLocalDiagnostics");
+  return read_buffer(s, s->dev.local_diag, dst, count, 16 * size_t(s->V), "readLocalDiagnostics", 16);
 }
 jgs2_status jgs2_read_globalization_control(jgs2_solver* s, float* dst, size_t count) {
   if (jgs2_status st = check_usable(s)) return st;
@@ -1131,8 +1268,13 @@ jgs2_status jgs2_debug_local_systems(jgs2_solver* s, const jgs2_settings* ov, ui
   launch_debug_local_systems(d, p, s->stream);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s->stream));
-  HIP_CHECK(hipMemcpy(g, d.debug_g, gc * sizeof(float), hipMemcpyDeviceToHost));
-  HIP_CHECK(hipMemcpy(h, d.debug_h, hc * sizeof(float), hipMemcpyDeviceToHost));
+  std::vector<float> gs(gc), hs(hc);
+  HIP_CHECK(hipMemcpy(gs.data(), d.debug_g, gc * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(hs.data(), d.debug_h, hc * sizeof(float), hipMemcpyDeviceToHost));
+  for (uint32_t ov = 0; ov < s->V; ++ov) {
+    std::memcpy(g + 3 * size_t(ov), &gs[3 * size_t(s->vnew[ov])], 3 * sizeof(float));
+    std::memcpy(h + 9 * size_t(ov), &hs[9 * size_t(s->vnew[ov])], 9 * sizeof(float));
+  }
   clear_error();
   return JGS2_OK;
 }

@@ -62,7 +62,6 @@ __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y 
 __device__ __forceinline__ f3 cross(f3 a, f3 b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
 // Column-major 3x3 like WGSL mat3x3f.
 struct m3 {
@@ -113,6 +112,13 @@ __device__ __forceinline__ uint32_t logical_block() {
   const uint32_t xcd = bid % 8u, j = bid / 8u;
   const uint32_t base = xcd < r ? xcd * (q + 1u) : r * (q + 1u) + (xcd - r) * q;
   return base + j;
+}
+
+// Vertex frames the local solve reads: those of this iteration's polar
+// pass, or the ones the previous iteration's convergence pass computed at
+// the same positions.
+__device__ __forceinline__ const float4* rot_source(const DeviceState& s, const PassParams& p) {
+  return p.fused_rot ? s.vrot_next : s.vrot;
 }
 
 __device__ __forceinline__ bool stop_latched(const DeviceState& s, const PassParams& p) {  // S:909-917
@@ -196,15 +202,6 @@ __device__ __forceinline__ float snh_energy(const m3& F, float lam, float mu) { 
 __device__ __forceinline__ f3 piola_times(const SnhPoint& s, f3 g) {
   return s.shear * mul(s.F, g) + s.volscale * mul(s.cof, g);
 }
-// <A, DP(F)[B]> for general directions A, B (SN:72-94 contracted with A).
-__device__ __forceinline__ float snh_second(const SnhPoint& s, const m3& A, const m3& B, float lam, float mu) {
-  m3 dcof = {cross(B.c1, s.F.c2) + cross(s.F.c1, B.c2), cross(B.c2, s.F.c0) + cross(s.F.c2, B.c0),
-             cross(B.c0, s.F.c1) + cross(s.F.c0, B.c1)};
-  float dshear = 2.0f * mu * frob(s.F, B) / (s.den * s.den);
-  float dvol = lam * frob(s.cof, B);
-  return s.shear * frob(A, B) + dshear * frob(A, s.F) + dvol * frob(A, s.cof) + s.volscale * frob(A, dcof);
-}
-
 // ------------------------------------------------------------------ globalization (jgs2-globalization-wgsl.ts)
 struct Sym3 {
   float a00, a11, a22, a01, a02, a12;
@@ -436,7 +433,174 @@ struct Local {
   f3 g;
   m3 H;
 };
-// Gather of jgs2Solve (S:3074-3263) for the tetrahedral material.
+__device__ __forceinline__ uint32_t tet_vertex(uint4 ix, int l) {
+  return l == 0 ? ix.x : (l == 1 ? ix.y : (l == 2 ? ix.z : ix.w));
+}
+// Row-major Ubar block l of a cubature record (S:1204-1212), as columns.
+__device__ __forceinline__ m3 ubar_block(const float* ub, int l) {
+  const float* b = ub + 9 * l;
+  return {mk(b[0], b[3], b[6]), mk(b[1], b[4], b[7]), mk(b[2], b[5], b[8])};
+}
+// Ubar_l q for a row-major block.
+__device__ __forceinline__ f3 ubar_times(const float* ub, int l, f3 q) {
+  const float* b = ub + 9 * l;
+  return mk(b[0] * q.x + b[1] * q.y + b[2] * q.z, b[3] * q.x + b[4] * q.y + b[5] * q.z,
+            b[6] * q.x + b[7] * q.y + b[8] * q.z);
+}
+// Incident SNH block in closed form: with a = F g, b = cof(F) g,
+//   vol DP(F)[e_c g^T] g = vol (shear |g|^2 e_c + 2 mu/den^2 a_c a + lambda b_c b),
+// because det(F + x g^T) is affine in x, so the directional-cofactor term of
+// the tangent (stable-neo-hookean-wgsl.ts:72-94) vanishes on this direction.
+__device__ __forceinline__ void add_incident_block(const SnhPoint& sp, const Tet& e, f3 gr, float sign, f3& g, m3& H) {
+  const f3 a = mul(sp.F, gr), b = mul(sp.cof, gr);
+  g = g + (sign * e.vol) * (sp.shear * a + sp.volscale * b);
+  const float cs = sign * e.vol * sp.shear * dot(gr, gr);
+  const float ca = sign * e.vol * 2.0f * e.mu / (sp.den * sp.den);
+  const float cb = sign * e.vol * e.lam;
+  H.c0 = H.c0 + mk(cs, 0, 0) + (ca * a.x) * a + (cb * b.x) * b;
+  H.c1 = H.c1 + mk(0, cs, 0) + (ca * a.y) * a + (cb * b.y) * b;
+  H.c2 = H.c2 + mk(0, 0, cs) + (ca * a.z) * a + (cb * b.z) * b;
+}
+// <A, d cof(F)[B]>: the second derivative of det F in directions A, B.
+__device__ __forceinline__ float det_second(const m3& F, const m3& A, const m3& B) {
+  return dot(A.c0, cross(B.c1, F.c2) + cross(F.c1, B.c2)) + dot(A.c1, cross(B.c2, F.c0) + cross(F.c2, B.c0)) +
+         dot(A.c2, cross(B.c0, F.c1) + cross(F.c0, B.c1));
+}
+
+// One cubature record's contribution to (g, H) for stable Neo-Hookean
+// (S:3198-3263 with cubatureHessian S:2339-2366 and the distributed inertia
+// and objective terms S:2401-2477). The reduced Hessian entry (r, c) is
+// vol <D_r, DP(F)[D_c]> with D_c = sum_l (B_l e_c) grad_l^T, B_l = R_{v_l}
+// Ubar_l R_s^T; blocks are streamed so only the three D_c stay live.
+__device__ __forceinline__ void cubature_record_snh(const DeviceState& s, const PassParams& p, uint32_t v, f3 x,
+                                                    uint32_t t, const float* ub, const m3& Rs_t, float w, Local& L) {
+  const float4* pos = s.pos[p.src];
+  const Tet e = load_tet(s, t);
+  const SnhPoint sp = snh_point(deformation(e, pos), e.lam, e.mu);
+  f3 pg = mk(0, 0, 0);
+  float i00 = 0, i11 = 0, i22 = 0, i01 = 0, i02 = 0, i12 = 0;
+  m3 D0 = mzero(), D1 = mzero(), D2 = mzero();
+  int ss = 4;
+#pragma unroll 1
+  for (int l = 0; l < 4; ++l) {
+    const uint32_t vl = tet_vertex(e.ix, l);
+    if (vl == v) ss = l;
+    const m3 B = mul(mul(load_rot(rot_source(s, p), vl), ubar_block(ub, l)), Rs_t);
+    const f3 gr = shape_gradient(e, l);
+    const uint4 il = s.vtx_info[vl];
+    const f3 xl = load_pos(pos, vl);
+    const float n = float(il.y > 1u ? il.y : 1u);
+    const float wl = il.z == 0u ? s.vtx_rest[vl].w * p.inv_dt2 / n : 0.0f;
+    f3 sg = e.vol * piola_times(sp, gr) + wl * (xl - xyz(s.predicted[vl]));
+    float wt = wl;
+    if (p.objective_flags != 0u && il.z == 0u) {
+      sg = sg + objective_gradient(s, p, vl, xl) / n;
+      if (p.objective_flags & kTarget) wt += s.objectives[2 * size_t(vl) + 1].w / n;
+    }
+    pg = pg + tmul(B, sg);
+    D0.c0 = D0.c0 + gr.x * B.c0;
+    D0.c1 = D0.c1 + gr.y * B.c0;
+    D0.c2 = D0.c2 + gr.z * B.c0;
+    D1.c0 = D1.c0 + gr.x * B.c1;
+    D1.c1 = D1.c1 + gr.y * B.c1;
+    D1.c2 = D1.c2 + gr.z * B.c1;
+    D2.c0 = D2.c0 + gr.x * B.c2;
+    D2.c1 = D2.c1 + gr.y * B.c2;
+    D2.c2 = D2.c2 + gr.z * B.c2;
+    i00 += wt * dot(B.c0, B.c0);
+    i11 += wt * dot(B.c1, B.c1);
+    i22 += wt * dot(B.c2, B.c2);
+    i01 += wt * dot(B.c0, B.c1);
+    i02 += wt * dot(B.c0, B.c2);
+    i12 += wt * dot(B.c1, B.c2);
+  }
+  const float ca = 2.0f * e.mu / (sp.den * sp.den);
+  const float f0 = frob(sp.F, D0), f1 = frob(sp.F, D1), f2 = frob(sp.F, D2);
+  const float c0 = frob(sp.cof, D0), c1 = frob(sp.cof, D1), c2 = frob(sp.cof, D2);
+  auto entry = [&](const m3& A, const m3& B, float fa, float fb, float ga, float gb) {
+    return e.vol * (sp.shear * frob(A, B) + ca * fa * fb + e.lam * ga * gb + sp.volscale * det_second(sp.F, A, B));
+  };
+  const float h00 = entry(D0, D0, f0, f0, c0, c0) + i00;
+  const float h11 = entry(D1, D1, f1, f1, c1, c1) + i11;
+  const float h22 = entry(D2, D2, f2, f2, c2, c2) + i22;
+  const float h01 = entry(D0, D1, f0, f1, c0, c1) + i01;
+  const float h02 = entry(D0, D2, f0, f2, c0, c2) + i02;
+  const float h12 = entry(D1, D2, f1, f2, c1, c2) + i12;
+  m3 ph = {mk(h00, h01, h02), mk(h01, h11, h12), mk(h02, h12, h22)};
+  if (ss < 4) {
+    // Remove the exact source block and the source's distributed inertia and
+    // objective shares (S:3237-3259).
+    const uint4 inf = s.vtx_info[v];
+    const float n = float(inf.y > 1u ? inf.y : 1u);
+    const float w_src = s.vtx_rest[v].w * p.inv_dt2 / n;
+    add_incident_block(sp, e, shape_gradient(e, ss), -1.0f, pg, ph);
+    pg = pg - w_src * (x - xyz(s.predicted[v]));
+    float diag = w_src;
+    if (p.objective_flags != 0u) {
+      pg = pg - objective_gradient(s, p, v, x) / n;
+      if (p.objective_flags & kTarget) diag += target_stiffness(s, p, v) / n;
+    }
+    ph.c0.x -= diag;
+    ph.c1.y -= diag;
+    ph.c2.z -= diag;
+  }
+  L.g = L.g + w * pg;
+  L.H = madd(L.H, mscale(w, ph));
+}
+
+// Legacy co-rotated record (S:2368-2398 with S:3226-3259).
+__device__ void cubature_record_corot(const DeviceState& s, const PassParams& p, uint32_t v, f3 x, uint32_t t,
+                                      const float* ub, const m3& Rs_t, float w, Local& L) {
+  const float4* pos = s.pos[p.src];
+  const Tet e = load_tet(s, t);
+  m3 B[4];
+  float wl[4];
+  for (int l = 0; l < 4; ++l) {
+    const uint32_t vl = tet_vertex(e.ix, l);
+    B[l] = mul(mul(load_rot(rot_source(s, p), vl), ubar_block(ub, l)), Rs_t);
+    wl[l] = dist_inertia_weight(s, p, vl);
+  }
+  f3 tg[4];
+  corot_gradients(s, p, t, e.ix, tg);
+  f3 pg = mk(0, 0, 0);
+  for (int l = 0; l < 4; ++l) {
+    const uint32_t vl = tet_vertex(e.ix, l);
+    pg = pg + tmul(B[l], tg[l] + wl[l] * (load_pos(pos, vl) - xyz(s.predicted[vl])));
+  }
+  const m3 R = load_rot(s.trot, t);
+  f3 cols[3];
+  for (int c = 0; c < 3; ++c) {
+    f3 ld[4], wp[4];
+    for (int l = 0; l < 4; ++l) ld[l] = tmul(R, c == 0 ? B[l].c0 : (c == 1 ? B[l].c1 : B[l].c2));
+    for (int rv = 0; rv < 4; ++rv) {
+      f3 lp = mk(0, 0, 0);
+      for (int cv = 0; cv < 4; ++cv) lp = lp + mul(stiffness_block(s, t, rv, cv), ld[cv]);
+      wp[rv] = mul(R, lp);
+    }
+    cols[c].x = dot(B[0].c0, wp[0]) + dot(B[1].c0, wp[1]) + dot(B[2].c0, wp[2]) + dot(B[3].c0, wp[3]);
+    cols[c].y = dot(B[0].c1, wp[0]) + dot(B[1].c1, wp[1]) + dot(B[2].c1, wp[2]) + dot(B[3].c1, wp[3]);
+    cols[c].z = dot(B[0].c2, wp[0]) + dot(B[1].c2, wp[1]) + dot(B[2].c2, wp[2]) + dot(B[3].c2, wp[3]);
+  }
+  m3 ph = {cols[0], cols[1], cols[2]};
+  for (int l = 0; l < 4; ++l) {
+    const m3& Bl = B[l];
+    ph.c0 = ph.c0 + wl[l] * tmul(Bl, Bl.c0);
+    ph.c1 = ph.c1 + wl[l] * tmul(Bl, Bl.c1);
+    ph.c2 = ph.c2 + wl[l] * tmul(Bl, Bl.c2);
+  }
+  const uint32_t ss = local_slot(e.ix, v);
+  if (ss < 4) {
+    const float w_src = dist_inertia_weight(s, p, v);
+    pg = pg - (tg[ss] + w_src * (x - xyz(s.predicted[v])));
+    ph = msub(ph, madd(mul(mul(R, stiffness_block(s, t, ss, ss)), tr(R)), mscale(w_src, mid())));
+  }
+  L.g = L.g + w * pg;
+  L.H = madd(L.H, mscale(w, ph));
+}
+
+// Gather of jgs2Solve (S:3074-3263): inertia, objectives, the exact
+// incident elements, the floor penalty and the cubature correction.
+template <bool kCorot>
 __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_t v) {
   const float4* pos = s.pos[p.src];
   const f3 x = load_pos(pos, v);
@@ -444,32 +608,16 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
   const float inertia = s.vtx_rest[v].w * p.inv_dt2;
   Local L;
   L.g = inertia * (x - xyz(s.predicted[v]));
-  float hdiag = inertia + target_stiffness(s, p, v);
-  L.H = mscale(hdiag, mid());
+  L.H = mscale(inertia + target_stiffness(s, p, v), mid());
   if (p.objective_flags != 0u) L.g = L.g + objective_gradient(s, p, v, x);
-
-  // Exact incident gather (S:3097-3111). Stable Neo-Hookean block in closed
-  // form: with a = F grad, b = cof(F) grad,
-  //   vol DP(F)[e_c grad^T] grad = vol (shear |grad|^2 e_c + 2 mu/den^2 a_c a
-  //                                    + lambda b_c b),
-  // because det(F + x grad^T) is affine in x, so the directional cofactor
-  // term of the tangent vanishes on this rank-one direction.
-  for (uint32_t k = 0; k < inf.y; ++k) {
+#pragma unroll 1
+  for (uint32_t k = 0; k < inf.y; ++k) {  // S:3097-3111
     const uint32_t t = s.adjacency[inf.x + k];
     const Tet e = load_tet(s, t);
     const uint32_t slot = local_slot(e.ix, v);
     if (slot >= 4) continue;
-    if (!p.corotated) {
-      const f3 gr = shape_gradient(e, slot);
-      const SnhPoint sp = snh_point(deformation(e, pos), e.lam, e.mu);
-      const f3 a = mul(sp.F, gr), b = mul(sp.cof, gr);
-      L.g = L.g + e.vol * (sp.shear * a + sp.volscale * b);
-      const float cs = e.vol * sp.shear * dot(gr, gr);
-      const float ca = e.vol * 2.0f * e.mu / (sp.den * sp.den);
-      const float cb = e.vol * e.lam;
-      L.H.c0 = L.H.c0 + mk(cs, 0, 0) + (ca * a.x) * a + (cb * b.x) * b;
-      L.H.c1 = L.H.c1 + mk(0, cs, 0) + (ca * a.y) * a + (cb * b.y) * b;
-      L.H.c2 = L.H.c2 + mk(0, 0, cs) + (ca * a.z) * a + (cb * b.z) * b;
+    if constexpr (!kCorot) {
+      add_incident_block(snh_point(deformation(e, pos), e.lam, e.mu), e, shape_gradient(e, slot), 1.0f, L.g, L.H);
     } else {
       f3 tg[4];
       corot_gradients(s, p, t, e.ix, tg);
@@ -482,140 +630,19 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
     L.g.y += p.floor_k * (x.y - p.floor_y);
     L.H.c1.y += p.floor_k;
   }
-
-  // Cubature second-order correction (S:3198-3263).
   if (p.cubature_k == 0) return L;
-  const m3 Rs_t = tr(load_rot(s.vrot, v));
-  const float w_src = dist_inertia_weight(s, p, v);
-  for (uint32_t k = 0; k < p.cubature_k; ++k) {
+  const m3 Rs_t = tr(load_rot(rot_source(s, p), v));
+#pragma unroll 1
+  for (uint32_t k = 0; k < p.cubature_k; ++k) {  // S:3198-3263
     const size_t rec = size_t(v) * p.cubature_k + k;
     const uint4 head = s.cub[10 * rec];
-    const uint32_t t = head.x;
     const float w = __uint_as_float(head.y);
-    if (t == kEmptyTet || t >= p.tet_count || !(w > 0.0f)) continue;
-    const Tet e = load_tet(s, t);
-    const uint32_t id[4] = {e.ix.x, e.ix.y, e.ix.z, e.ix.w};
-    // B_l = R_{v_l} Ubar_l R_s^T (S:1846-1862); Ubar stored row-major.
-    const float4* ub = reinterpret_cast<const float4*>(s.cub + 10 * rec + 1);
-    float u[36];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      float4 c = ub[q];
-      u[4 * q] = c.x;
-      u[4 * q + 1] = c.y;
-      u[4 * q + 2] = c.z;
-      u[4 * q + 3] = c.w;
-    }
-    m3 B[4];
-    float wl[4];
-    f3 gr[4];
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const float* b = u + 9 * l;
-      m3 U = {mk(b[0], b[3], b[6]), mk(b[1], b[4], b[7]), mk(b[2], b[5], b[8])};
-      B[l] = mul(mul(load_rot(s.vrot, id[l]), U), Rs_t);
-      wl[l] = dist_inertia_weight(s, p, id[l]);
-      gr[l] = shape_gradient(e, l);
-    }
-    f3 pg = mk(0, 0, 0);
-    m3 ph = mzero();
-    const uint32_t ss = local_slot(e.ix, v);
-    if (!p.corotated) {
-      const SnhPoint sp = snh_point(deformation(e, pos), e.lam, e.mu);
-      f3 tg[4];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        tg[l] = e.vol * piola_times(sp, gr[l]);
-        f3 sg = tg[l] + wl[l] * (load_pos(pos, id[l]) - xyz(s.predicted[id[l]]));
-        if (p.objective_flags != 0u && s.vtx_info[id[l]].z == 0u) {
-          uint32_t n = s.vtx_info[id[l]].y > 1u ? s.vtx_info[id[l]].y : 1u;
-          sg = sg + objective_gradient(s, p, id[l], load_pos(pos, id[l])) / float(n);
-        }
-        pg = pg + tmul(B[l], sg);
-      }
-      // Reduced Hessian (S:2339-2366): entry (r, c) = vol <D_r, DP(F)[D_c]>
-      // with direction matrices D_c = sum_l (B_l e_c) grad_l^T.
-      m3 D[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        m3 acc = mzero();
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-          f3 col = c == 0 ? B[l].c0 : (c == 1 ? B[l].c1 : B[l].c2);
-          acc.c0 = acc.c0 + gr[l].x * col;
-          acc.c1 = acc.c1 + gr[l].y * col;
-          acc.c2 = acc.c2 + gr[l].z * col;
-        }
-        D[c] = acc;
-      }
-      float h00 = e.vol * snh_second(sp, D[0], D[0], e.lam, e.mu);
-      float h11 = e.vol * snh_second(sp, D[1], D[1], e.lam, e.mu);
-      float h22 = e.vol * snh_second(sp, D[2], D[2], e.lam, e.mu);
-      float h01 = e.vol * snh_second(sp, D[0], D[1], e.lam, e.mu);
-      float h02 = e.vol * snh_second(sp, D[0], D[2], e.lam, e.mu);
-      float h12 = e.vol * snh_second(sp, D[1], D[2], e.lam, e.mu);
-      ph = {mk(h00, h01, h02), mk(h01, h11, h12), mk(h02, h12, h22)};
-      if (ss < 4) {
-        // Remove the exact source block and source inertia share (S:3237-3259).
-        const f3 gs = gr[ss];
-        const f3 a = mul(sp.F, gs), b = mul(sp.cof, gs);
-        const float cs = e.vol * sp.shear * dot(gs, gs);
-        const float ca = e.vol * 2.0f * e.mu / (sp.den * sp.den);
-        const float cb = e.vol * e.lam;
-        pg = pg - (tg[ss] + w_src * (x - xyz(s.predicted[v])));
-        ph.c0 = ph.c0 - (mk(cs, 0, 0) + (ca * a.x) * a + (cb * b.x) * b);
-        ph.c1 = ph.c1 - (mk(0, cs, 0) + (ca * a.y) * a + (cb * b.y) * b);
-        ph.c2 = ph.c2 - (mk(0, 0, cs) + (ca * a.z) * a + (cb * b.z) * b);
-        ph = msub(ph, mscale(w_src, mid()));
-      }
-    } else {
-      f3 tg[4];
-      corot_gradients(s, p, t, e.ix, tg);
-#pragma unroll
-      for (int l = 0; l < 4; ++l) pg = pg + tmul(B[l], tg[l] + wl[l] * (load_pos(pos, id[l]) - xyz(s.predicted[id[l]])));
-      const m3 R = load_rot(s.trot, t);
-      f3 cols[3];
-      for (int c = 0; c < 3; ++c) {  // S:2368-2398
-        f3 ld[4], wp[4];
-        for (int l = 0; l < 4; ++l) ld[l] = tmul(R, c == 0 ? B[l].c0 : (c == 1 ? B[l].c1 : B[l].c2));
-        for (int rv = 0; rv < 4; ++rv) {
-          f3 lp = mk(0, 0, 0);
-          for (int cv = 0; cv < 4; ++cv) lp = lp + mul(stiffness_block(s, t, rv, cv), ld[cv]);
-          wp[rv] = mul(R, lp);
-        }
-        f3 rc;
-        rc.x = dot(B[0].c0, wp[0]) + dot(B[1].c0, wp[1]) + dot(B[2].c0, wp[2]) + dot(B[3].c0, wp[3]);
-        rc.y = dot(B[0].c1, wp[0]) + dot(B[1].c1, wp[1]) + dot(B[2].c1, wp[2]) + dot(B[3].c1, wp[3]);
-        rc.z = dot(B[0].c2, wp[0]) + dot(B[1].c2, wp[1]) + dot(B[2].c2, wp[2]) + dot(B[3].c2, wp[3]);
-        cols[c] = rc;
-      }
-      ph = {cols[0], cols[1], cols[2]};
-      if (ss < 4) {
-        pg = pg - (tg[ss] + w_src * (x - xyz(s.predicted[v])));
-        ph = msub(ph, madd(mul(mul(R, stiffness_block(s, t, ss, ss)), tr(R)), mscale(w_src, mid())));
-      }
-    }
-    // Distributed inertia and target Hessians sum_l w_l B_l^T B_l (S:2453-2477).
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      float wt = wl[l];
-      if (p.objective_flags & kTarget) {
-        uint4 il = s.vtx_info[id[l]];
-        if (il.z == 0u) wt += s.objectives[2 * size_t(id[l]) + 1].w / float(il.y > 1u ? il.y : 1u);
-      }
-      const m3& Bl = B[l];
-      f3 t0 = tmul(Bl, Bl.c0), t1 = tmul(Bl, Bl.c1), t2 = tmul(Bl, Bl.c2);
-      ph.c0 = ph.c0 + wt * t0;
-      ph.c1 = ph.c1 + wt * t1;
-      ph.c2 = ph.c2 + wt * t2;
-    }
-    if (ss < 4 && p.objective_flags != 0u) {
-      uint32_t n = inf.y > 1u ? inf.y : 1u;
-      pg = pg - objective_gradient(s, p, v, x) / float(n);
-      if (p.objective_flags & kTarget) ph = msub(ph, mscale(target_stiffness(s, p, v) / float(n), mid()));
-    }
-    L.g = L.g + w * pg;
-    L.H = madd(L.H, mscale(w, ph));
+    if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
+    const float* ub = reinterpret_cast<const float*>(s.cub + 10 * rec + 1);
+    if constexpr (!kCorot)
+      cubature_record_snh(s, p, v, x, head.x, ub, Rs_t, w, L);
+    else
+      cubature_record_corot(s, p, v, x, head.x, ub, Rs_t, w, L);
   }
   return L;
 }
@@ -631,23 +658,20 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
   const float4* pos = s.pos[p.src];
   const uint4 inf = s.vtx_info[v];
   Trial tr{initial_min, true, 0.0f};
-  const float sin = s.vtx_rest[v].w * p.inv_dt2;
   const f3 pred = xyz(s.predicted[v]);
-  float energy = quadratic_delta(x, pred, d, sin) + floor_delta(p, x.y, d.y);
+  float energy = quadratic_delta(x, pred, d, s.vtx_rest[v].w * p.inv_dt2) + floor_delta(p, x.y, d.y);
   if (p.objective_flags != 0u) energy += objective_delta(s, p, v, x, d);
-  // Incident tetrahedra: F' = F + d grad^T. In closed form det(F') = J + d.b
-  // and <D, F' + F> = 2 d.a + |d|^2 |grad|^2 with a = F grad, b = cof grad;
-  // the pair and triple terms of the determinant delta vanish for rank one.
+  // Incident tetrahedra: F' = F + d g^T. In closed form det(F') = J + d.b
+  // and <D, F' + F> = 2 d.a + |d|^2 |g|^2 with a = F g, b = cof g; the pair
+  // and triple terms of the determinant delta (GL:510-524) vanish for rank one.
   for (uint32_t k = 0; k < inf.y; ++k) {
     const uint32_t t = s.adjacency[inf.x + k];
     const Tet e = load_tet(s, t);
-    const uint32_t slot = local_slot(e.ix, v);
-    const f3 gr = shape_gradient(e, slot);
+    const f3 gr = shape_gradient(e, local_slot(e.ix, v));
     const m3 F = deformation(e, pos);
     const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
     const float J = dot(F.c0, c12);
     const f3 b = gr.x * c12 + gr.y * c20 + gr.z * c01;
-    const f3 a = mul(F, gr);
     const float jd = dot(d, b);
     const float det_trial = J + jd;
     if (!finite_s(det_trial)) {
@@ -655,76 +679,60 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
       return tr;
     }
     tr.min_det = fminf(tr.min_det, det_trial);
-    const float i1 = frob(F, F);
-    const float id = 2.0f * dot(d, a) + dot(d, d) * dot(gr, gr);
-    const float lr = log_one_plus(id / (i1 + 1.0f));
+    const float id = 2.0f * dot(d, mul(F, gr)) + dot(d, d) * dot(gr, gr);
+    const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
     energy += e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
   }
   if (p.cubature_k > 0) {
-    const f3 ds = tmul(load_rot(s.vrot, v), d);  // R_s^T d
-    const float w_src = dist_inertia_weight(s, p, v);
+    const f3 ds = tmul(load_rot(rot_source(s, p), v), d);  // R_s^T d
+    const float n_src = float(inf.y > 1u ? inf.y : 1u);
+    const float w_src = s.vtx_rest[v].w * p.inv_dt2 / n_src;
     for (uint32_t k = 0; k < p.cubature_k; ++k) {
       const size_t rec = size_t(v) * p.cubature_k + k;
       const uint4 head = s.cub[10 * rec];
-      const uint32_t t = head.x;
       const float w = __uint_as_float(head.y);
-      if (t == kEmptyTet || t >= p.tet_count || !(w > 0.0f)) continue;
-      const Tet e = load_tet(s, t);
-      const uint32_t id[4] = {e.ix.x, e.ix.y, e.ix.z, e.ix.w};
-      const float4* ub = reinterpret_cast<const float4*>(s.cub + 10 * rec + 1);
-      float u[36];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) {
-        float4 c = ub[q];
-        u[4 * q] = c.x;
-        u[4 * q + 1] = c.y;
-        u[4 * q + 2] = c.z;
-        u[4 * q + 3] = c.w;
-      }
-      m3 Dd = mzero();
-      f3 bd[4];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        const float* bb = u + 9 * l;
-        f3 ud = mk(bb[0] * ds.x + bb[1] * ds.y + bb[2] * ds.z, bb[3] * ds.x + bb[4] * ds.y + bb[5] * ds.z,
-                   bb[6] * ds.x + bb[7] * ds.y + bb[8] * ds.z);
-        bd[l] = mul(load_rot(s.vrot, id[l]), ud);  // B_l d
-        const f3 gl = shape_gradient(e, l);
-        Dd.c0 = Dd.c0 + gl.x * bd[l];
-        Dd.c1 = Dd.c1 + gl.y * bd[l];
-        Dd.c2 = Dd.c2 + gl.z * bd[l];
-      }
+      if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
+      const Tet e = load_tet(s, head.x);
+      const float* ub = reinterpret_cast<const float*>(s.cub + 10 * rec + 1);
       const m3 F = deformation(e, pos);
+      m3 Dd = mzero();
+      float compl_e = 0.0f;
+      int ss = 4;
+#pragma unroll 1
+      for (int l = 0; l < 4; ++l) {
+        const uint32_t vl = tet_vertex(e.ix, l);
+        if (vl == v) ss = l;
+        const f3 bd = mul(load_rot(rot_source(s, p), vl), ubar_times(ub, l, ds));  // B_l d
+        const f3 gl = shape_gradient(e, l);
+        Dd.c0 = Dd.c0 + gl.x * bd;
+        Dd.c1 = Dd.c1 + gl.y * bd;
+        Dd.c2 = Dd.c2 + gl.z * bd;
+        const uint4 il = s.vtx_info[vl];
+        if (il.z == 0u) {
+          const float n = float(il.y > 1u ? il.y : 1u);
+          const f3 xl = load_pos(pos, vl);
+          compl_e += quadratic_delta(xl, xyz(s.predicted[vl]), bd, s.vtx_rest[vl].w * p.inv_dt2 / n);
+          if (p.objective_flags != 0u) compl_e += objective_delta(s, p, vl, xl, bd) / n;
+        }
+      }
       const float det_trial = mdet(madd(F, Dd));
       if (!finite_s(det_trial)) {
         tr.geometry_valid = false;
         return tr;
       }
       tr.min_det = fminf(tr.min_det, det_trial);
-      float compl_e = snh_delta_general(F, Dd, e.lam, e.mu) * e.vol;
-#pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        const uint32_t vl = id[l];
-        const f3 xl = load_pos(pos, vl);
-        compl_e += quadratic_delta(xl, xyz(s.predicted[vl]), bd[l], dist_inertia_weight(s, p, vl));
-        if (p.objective_flags != 0u) {
-          uint4 il = s.vtx_info[vl];
-          if (il.z == 0u) compl_e += objective_delta(s, p, vl, xl, bd[l]) / float(il.y > 1u ? il.y : 1u);
-        }
-      }
-      const uint32_t ss = local_slot(e.ix, v);
+      compl_e += snh_delta_general(F, Dd, e.lam, e.mu) * e.vol;
       if (ss < 4) {
         const f3 gs = shape_gradient(e, ss);
         const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
         const float J = dot(F.c0, c12);
         const f3 b = gs.x * c12 + gs.y * c20 + gs.z * c01;
-        const f3 a = mul(F, gs);
         const float jd = dot(d, b);
-        const float id = 2.0f * dot(d, a) + dot(d, d) * dot(gs, gs);
+        const float id = 2.0f * dot(d, mul(F, gs)) + dot(d, d) * dot(gs, gs);
         const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
         compl_e -= e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
         compl_e -= quadratic_delta(x, pred, d, w_src);
-        if (p.objective_flags != 0u) compl_e -= objective_delta(s, p, v, x, d) / float(inf.y > 1u ? inf.y : 1u);
+        if (p.objective_flags != 0u) compl_e -= objective_delta(s, p, v, x, d) / n_src;
       }
       energy += w * compl_e;
     }
@@ -831,6 +839,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassPara
 
 // jgs2Solve (S:2992-3487), one thread per vertex (Jacobi) or per vertex of
 // the active color (graph-colored Gauss-Seidel, in place).
+template <bool kCorot>
 __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
   const uint32_t lane = logical_block() * kBlock + threadIdx.x;
   uint32_t v;
@@ -845,6 +854,13 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
   if (stop_latched(s, p)) {
     tgt[v] = s.pos[p.src][v];
     return;
+  }
+  if (p.fused_rot) {  // this sweep's vertexPolarRotation result (S:2960-2989)
+    const float4* n = s.vrot_next + 3 * size_t(v);
+    float4* o = s.vrot + 3 * size_t(v);
+    o[0] = n[0];
+    o[1] = n[1];
+    o[2] = n[2];
   }
   const uint4 inf = s.vtx_info[v];
   if (inf.z != 0u) {
@@ -873,9 +889,9 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
       return;
     }
   }
-  const Local L = gather_local(s, p, v);
+  const Local L = gather_local<kCorot>(s, p, v);
   const float inertia = s.vtx_rest[v].w * p.inv_dt2;
-  if (!p.globalization) {  // legacy regularized solve (S:2479-2512, S:3475-3486)
+  if constexpr (kCorot) {  // legacy regularized solve (S:2479-2512, S:3475-3486)
     const m3& m = L.H;
     const f3 rhs = -L.g;
     float dsc = fmaxf(1.0f, fmaxf(fabsf(m.c0.x), fmaxf(fabsf(m.c1.y), fabsf(m.c2.z))));
@@ -1226,6 +1242,8 @@ __device__ __forceinline__ float max_abs(f3 v) { return fmaxf(fabsf(v.x), fmaxf(
 
 // convergenceGradient (S:3986-4097) fused with scan one of reduceConvergence
 // (S:4119-4189): component gradients at the candidate and their scales.
+// With `compute_next_rot` the same gather also evaluates the next sweep's
+// vertexPolarRotation (S:2960-2989): its source pose is this candidate.
 __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, PassParams p) {
   __shared__ ConvPartial sh[kBlock];
   ConvPartial r;
@@ -1237,12 +1255,37 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
     f3 comp5[5];
     if (!latched) {
       const uint4 inf = s.vtx_info[v];
-      if (inf.z != 0u) {
+      const bool pinned = inf.z != 0u;
+      const float4* pos = s.pos[p.tgt];
+      const f3 x = load_pos(pos, v);
+      f3 mat = mk(0, 0, 0);
+      m3 avg = mzero();
+      float tv = 0.0f;
+      if (!pinned || p.compute_next_rot) {
+        for (uint32_t k = 0; k < inf.y; ++k) {
+          const uint32_t t = s.adjacency[inf.x + k];
+          const Tet e = load_tet(s, t);
+          const m3 F = deformation(e, pos);
+          if (p.compute_next_rot) {
+            avg = madd(avg, mscale(e.vol, F));
+            tv += e.vol;
+          }
+          const uint32_t slot = local_slot(e.ix, v);
+          if (pinned || slot >= 4) continue;
+          const SnhPoint sp = snh_point(F, e.lam, e.mu);
+          mat = mat + e.vol * piola_times(sp, shape_gradient(e, slot));
+        }
+      }
+      if (p.compute_next_rot) {
+        m3 R = mid();
+        if (inf.y != 0u) R = polar_rotation(tv > 0.0f ? mscale(1.0f / tv, avg) : mid(), p.polar_eps);
+        store_rot(s.vrot_next, v, R);
+      }
+      if (pinned) {
         for (int k = 0; k < 5; ++k) comp5[k] = mk(0, 0, 0);
       } else {
-        const float4* pos = s.pos[p.tgt];
-        const f3 x = load_pos(pos, v);
         comp5[0] = (s.vtx_rest[v].w * p.inv_dt2) * (x - xyz(s.predicted[v]));
+        comp5[1] = mat;
         comp5[2] = mk(0, 0, 0);
         comp5[3] = mk(0, 0, 0);
         if (p.objective_flags != 0u) {
@@ -1252,16 +1295,6 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
             if (t.w > 0.0f) comp5[3] = t.w * (x - xyz(t));
           }
         }
-        f3 mat = mk(0, 0, 0);
-        for (uint32_t k = 0; k < inf.y; ++k) {
-          const uint32_t t = s.adjacency[inf.x + k];
-          const Tet e = load_tet(s, t);
-          const uint32_t slot = local_slot(e.ix, v);
-          if (slot >= 4) continue;
-          const SnhPoint sp = snh_point(deformation(e, pos), e.lam, e.mu);
-          mat = mat + e.vol * piola_times(sp, shape_gradient(e, slot));
-        }
-        comp5[1] = mat;
         comp5[4] = mk(0, 0, 0);
         if (p.floor_k > 0.0f && x.y < p.floor_y) comp5[4].y = p.floor_k * (x.y - p.floor_y);
       }
@@ -1270,7 +1303,7 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
       for (int k = 0; k < 5; ++k) comp5[k] = xyz(c[k]);
     }
     f3 total = mk(0, 0, 0);
-    bool tv = true;
+    bool tv_ok = true;
     for (int k = 0; k < 5; ++k) {
       const bool ok = finite3(comp5[k]);
       const f3 val = ok ? comp5[k] : mk(0, 0, 0);
@@ -1278,10 +1311,10 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
       if (k < 4) r.ch[k] = fmaxf(r.ch[k], vs);
       else r.fin[0] = fmaxf(r.fin[0], vs);
       total = total + val;
-      tv = tv && ok;
+      tv_ok = tv_ok && ok;
     }
-    tv = tv && finite3(total);
-    if (tv) r.fin[1] = fmaxf(r.fin[1], max_abs(total));
+    tv_ok = tv_ok && finite3(total);
+    if (tv_ok) r.fin[1] = fmaxf(r.fin[1], max_abs(total));
     else r.valid = 0u;
     const float up = s.final_update[v].x;
     if (finite_s(up) && up >= 0.0f) r.max_update = fmaxf(r.max_update, up);
@@ -1494,6 +1527,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(DeviceState s, PassParams p
   s.velocity[v] = make_float4(vel.x, vel.y, vel.z, 0.0f);
 }
 
+template <bool kCorot>
 __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, PassParams p) {
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
@@ -1504,7 +1538,7 @@ __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, P
     for (int k = 0; k < 9; ++k) h[k] = 0.0f;
     return;
   }
-  const Local L = gather_local(s, p, v);
+  const Local L = gather_local<kCorot>(s, p, v);
   g[0] = L.g.x;
   g[1] = L.g.y;
   g[2] = L.g.z;
@@ -1547,7 +1581,10 @@ void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t 
 void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t n = p.colored ? p.color_end - p.color_begin : p.vertex_count;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_solve, dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
+  if (p.corotated)
+    hipLaunchKernelGGL(k_solve<true>, dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
+  else
+    hipLaunchKernelGGL(k_solve<false>, dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
 }
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
   hipLaunchKernelGGL(k_vertex_energy, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
@@ -1579,7 +1616,10 @@ void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t st) 
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
 }
 void launch_debug_local_systems(const DeviceState& s, const PassParams& p, hipStream_t st) {
-  hipLaunchKernelGGL(k_debug_local_systems, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+  if (p.corotated)
+    hipLaunchKernelGGL(k_debug_local_systems<true>, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+  else
+    hipLaunchKernelGGL(k_debug_local_systems<false>, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
 }
 
 }  // namespace jgs2
