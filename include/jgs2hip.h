@@ -54,6 +54,26 @@ typedef enum {
 #define JGS2_MAX_GLOBALIZED_ITERATIONS_PER_SUBMISSION 2048
 
 /*
+ * Optional triangle cloth layer sharing the vertex state: `JGS2GpuClothInput`
+ * (src/simulation/gpu/cloth-layout.ts:89-104). Hinges list the oriented
+ * triangles (0,1,2) and (1,0,3) of each interior edge.
+ */
+typedef struct {
+  uint32_t triangle_count;
+  const uint32_t* triangle_indices;          /* 3 per triangle */
+  const float* triangle_inverse_rest_bases;  /* 4 per triangle, row-major */
+  const float* triangle_rest_areas;          /* 1 per triangle */
+  uint32_t hinge_count;
+  const uint32_t* hinge_indices;             /* 4 per hinge */
+  const float* hinge_rest_angles;            /* 1 per hinge */
+  const float* hinge_rest_edge_lengths;      /* 1 per hinge */
+  double young_modulus;
+  double poisson_ratio;
+  double thickness;
+  double bending_stiffness;
+} jgs2_cloth_input;
+
+/*
  * Scene input: the arrays of `JGS2GpuInput` (layout.ts:41-90), same element
  * order and meaning. Optional arrays may be NULL.
  */
@@ -77,6 +97,7 @@ typedef struct {
   const float* external_forces;     /* optional, 3 per vertex */
   const float* target_positions;    /* optional, 3 per vertex (with target_stiffnesses) */
   const float* target_stiffnesses;  /* optional, 1 per vertex */
+  const jgs2_cloth_input* cloth;    /* optional triangle cloth layer */
 } jgs2_scene_input;
 
 /* Schedules (jgs2-solver.ts:62). */

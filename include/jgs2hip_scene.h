@@ -48,6 +48,37 @@ typedef struct {
 
 jgs2_status jgs2_scene_build_cuboid(const jgs2_cuboid_desc* desc, jgs2_scene** out);
 
+/*
+ * A pinned cloth drape in the shape of the reference's cloth scene
+ * (clothDefinition, src/scenes/index.ts:632-705): a fixed one-cell stable
+ * Neo-Hookean collider block (rho 1000, E 1.1e5, nu 0.34) and an n x n quad
+ * sheet split into triangles (i00, i01, i10), (i10, i01, i11), pinned at
+ * grid vertices (0, 0) and (n/2, 0), with StVK membrane and quadratic
+ * dihedral bending. Columns x >= fold_column are mirrored back over column
+ * fold_column - 1 and lifted by fold_lift (fold_column = 0: flat sheet).
+ * Rest data follow precomputeStVKTriangleRest and the signed dihedral
+ * (src/simulation/cpu/cloth.ts:115-140, :258-350), masses
+ * density * thickness * area / 3 per corner (precompute.ts:46-93), hinges in
+ * the half-edge order of buildJGS2GpuClothInput (src/scenes/index.ts:882-973).
+ */
+typedef struct {
+  uint32_t cells;             /* quads per side */
+  double origin[3];           /* sheet corner (x, y, z); the sheet spans x and z */
+  double size[2];             /* sheet extent along x and z */
+  uint32_t fold_column;
+  double fold_lift;
+  double collider_origin[3];
+  double collider_size[3];
+  double density;             /* kg / m^3 */
+  double young_modulus;
+  double poisson_ratio;
+  double thickness;
+  double bending_stiffness;
+  uint32_t cubature_k;        /* records per vertex (collider vertices are pinned) */
+} jgs2_cloth_desc;
+
+jgs2_status jgs2_scene_build_cloth_drape(const jgs2_cloth_desc* desc, jgs2_scene** out);
+
 /* Pointers into the scene's arrays, valid until jgs2_scene_destroy(). */
 jgs2_status jgs2_scene_get_input(const jgs2_scene* scene, jgs2_scene_input* out);
 
@@ -62,6 +93,13 @@ void jgs2_scene_destroy(jgs2_scene* scene);
  * 811-837). `starts`/`counts` receive V entries, `tets` 4T entries. */
 jgs2_status jgs2_build_adjacency(uint32_t vertex_count, uint32_t tet_count, const uint32_t* tet_indices,
                                  uint32_t* starts, uint32_t* counts, uint32_t* tets);
+
+/* Vertex -> element incidence CSR of triangles (width 3) or hinges (width
+ * 4) (buildIncidentElementCsr, src/simulation/gpu/cloth-layout.ts:183-207):
+ * `rows` receives vertex_count + 1 offsets, `elements` width * element_count
+ * element ids, ascending within each row. */
+jgs2_status jgs2_build_element_incidence(uint32_t vertex_count, const uint32_t* element_indices, uint32_t width,
+                                         uint32_t element_count, uint32_t* rows, uint32_t* elements);
 
 /* Deterministic greedy coloring over CSR conflict groups
  * (buildGreedyVertexColoring, src/simulation/cpu/vertex-coloring.ts:20-110).

@@ -368,6 +368,15 @@ typedef struct {
   const float* objectives;           // 8V or NULL
   const uint32_t* colors;            // V or NULL (graph-colored GS)
   uint32_t color_count;
+  // Optional triangle cloth layer (src/simulation/gpu/cloth-layout.ts:89-104).
+  uint32_t cloth_triangle_count;
+  const uint32_t* cloth_triangles;          // 3 per triangle
+  const float* cloth_inverse_rest_bases;    // 4 per triangle, row-major
+  const float* cloth_rest_areas;            // 1 per triangle
+  uint32_t cloth_hinge_count;
+  const uint32_t* cloth_hinges;             // 4 per hinge: triangles (0,1,2), (1,0,3)
+  const float* cloth_hinge_rest;            // 2 per hinge: angle, edge length
+  double cloth_young_modulus, cloth_poisson_ratio, cloth_thickness, cloth_bending_stiffness;
 } jgs2o_input;
 
 typedef struct {
@@ -406,6 +415,12 @@ template <class T> struct Sim {
   bool globalization = true;
   uint32_t objective_flags = 0;
   T scene_scale = T(0);
+  // Cloth layer (cloth-layout.ts:305-452): records, material, incidence CSR.
+  uint32_t NTRI = 0, NHINGE = 0;
+  std::vector<uint32_t> tri_idx, hinge_idx;
+  std::vector<T> tri_inv, tri_area, hinge_rest;
+  T cloth_lam = T(0), cloth_mu = T(0), cloth_thick = T(0), cloth_bend = T(0);
+  std::vector<uint32_t> tri_rows, tri_elems, hinge_rows, hinge_elems;
 
   // ---- dynamic state (layout.ts:418-503)
   std::vector<V3<T>> pos[2];             // posA, posB
@@ -765,7 +780,20 @@ template <class T> struct Sim {
       T pen = std::min(T(0), x.y - p.floor_y);
       e += T(0.5) * p.floor_k * pen * pen;
     }
-    // Cloth-owned and IPC-owned energies are zero for tetrahedral scenes.
+    // Cloth elements owned by their smallest vertex id (S:2585-2643).
+    for (uint32_t k = tri_rows[v]; k < tri_rows[v + 1]; ++k) {
+      uint32_t tri = tri_elems[k];
+      if (!tri_owned(tri, v)) continue;
+      T tra = T(2) * tri_area[tri];
+      T stretch = length(tri_raw_normal(tri, buf, 0xffffffffu, v3<T>(0, 0, 0))) / tra;
+      if (!(tra > T(0)) || !finite_s(stretch) || !(stretch > T(kDeterminantFloor))) return T(kF32Max);
+      e += membrane_at(tri, buf, 0xffffffffu, v3<T>(0, 0, 0)).energy;
+    }
+    for (uint32_t k = hinge_rows[v]; k < hinge_rows[v + 1]; ++k) {
+      uint32_t h = hinge_elems[k];
+      if (!hinge_owned(h, v)) continue;
+      e += bending_energy_at(h, buf, 0xffffffffu, v3<T>(0, 0, 0));
+    }
     return e;
   }
   void store_vertex_energy(uint32_t v) {  // S:2647-2658
@@ -773,6 +801,175 @@ template <class T> struct Sim {
     T ce = vertex_implicit_energy_at(p.tgt, v);
     bool sv = finite_s(se), cv = finite_s(ce);
     vertex_energy[v] = V4<T>{sv ? se : T(0), cv ? ce : T(0), sv ? T(1) : T(0), cv ? T(1) : T(0)};
+  }
+
+
+  // ---------------------------------------------------------------- cloth (S:1225-1845)
+  V3<T> cloth_pos(int buf, uint32_t v, uint32_t ov, V3<T> op) const {  // S:1520-1531
+    return v == ov ? op : load(buf, v);
+  }
+  struct Membrane {
+    T energy;
+    V3<T> f0, f1, stress, grad[3];
+  };
+  Membrane membrane_at(uint32_t tri, int buf, uint32_t ov, V3<T> op) const {  // S:1546-1604
+    const uint32_t* ix = &tri_idx[3 * size_t(tri)];
+    V3<T> x0 = cloth_pos(buf, ix[0], ov, op);
+    V3<T> e01 = cloth_pos(buf, ix[1], ov, op) - x0;
+    V3<T> e02 = cloth_pos(buf, ix[2], ov, op) - x0;
+    const T* inv = &tri_inv[4 * size_t(tri)];
+    Membrane m;
+    m.f0 = e01 * inv[0] + e02 * inv[2];
+    m.f1 = e01 * inv[1] + e02 * inv[3];
+    T e00 = T(0.5) * (dot(m.f0, m.f0) - T(1));
+    T e01s = T(0.5) * dot(m.f0, m.f1);
+    T e11 = T(0.5) * (dot(m.f1, m.f1) - T(1));
+    T trace = e00 + e11;
+    m.stress = v3(T(2) * cloth_mu * e00 + cloth_lam * trace, T(2) * cloth_mu * e01s,
+                  T(2) * cloth_mu * e11 + cloth_lam * trace);
+    V3<T> p0 = m.f0 * m.stress.x + m.f1 * m.stress.y;
+    V3<T> p1 = m.f0 * m.stress.y + m.f1 * m.stress.z;
+    T aw = tri_area[tri] * cloth_thick;
+    for (int l = 0; l < 3; ++l) {
+      T sx, sy;
+      tri_shape_gradient(tri, l, sx, sy);
+      m.grad[l] = aw * (p0 * sx + p1 * sy);
+    }
+    T density = cloth_mu * (e00 * e00 + T(2) * e01s * e01s + e11 * e11) + T(0.5) * cloth_lam * trace * trace;
+    m.energy = aw * density;
+    return m;
+  }
+  void tri_shape_gradient(uint32_t tri, int l, T& x, T& y) const {  // S:1533-1544
+    const T* inv = &tri_inv[4 * size_t(tri)];
+    if (l == 1) {
+      x = inv[0];
+      y = inv[1];
+    } else if (l == 2) {
+      x = inv[2];
+      y = inv[3];
+    } else {
+      x = -inv[0] - inv[2];
+      y = -inv[1] - inv[3];
+    }
+  }
+  M3<T> membrane_local_hessian(uint32_t tri, int slot, int buf) const {  // S:1608-1658
+    Membrane ev = membrane_at(tri, buf, 0xffffffffu, v3<T>(0, 0, 0));
+    T sx, sy;
+    tri_shape_gradient(tri, slot, sx, sy);
+    T aw = tri_area[tri] * cloth_thick;
+    V3<T> col[3];
+    for (int c = 0; c < 3; ++c) {
+      V3<T> axis = v3<T>(0, 0, 0);
+      axis[c] = T(1);
+      V3<T> df0 = axis * sx, df1 = axis * sy;
+      T de00 = sx * ev.f0[c];
+      T de01 = T(0.5) * (sx * ev.f1[c] + sy * ev.f0[c]);
+      T de11 = sy * ev.f1[c];
+      T dtr = de00 + de11;
+      V3<T> ds = v3(T(2) * cloth_mu * de00 + cloth_lam * dtr, T(2) * cloth_mu * de01, T(2) * cloth_mu * de11 + cloth_lam * dtr);
+      V3<T> dp0 = df0 * ev.stress.x + df1 * ev.stress.y + ev.f0 * ds.x + ev.f1 * ds.y;
+      V3<T> dp1 = df0 * ev.stress.y + df1 * ev.stress.z + ev.f0 * ds.y + ev.f1 * ds.z;
+      col[c] = aw * (dp0 * sx + dp1 * sy);
+    }
+    M3<T> r = mcols(col[0], col[1], col[2]);
+    return T(0.5) * (r + transpose(r));
+  }
+  V3<T> tri_raw_normal(uint32_t tri, int buf, uint32_t ov, V3<T> op) const {  // S:1660-1686
+    const uint32_t* ix = &tri_idx[3 * size_t(tri)];
+    V3<T> x0 = cloth_pos(buf, ix[0], ov, op);
+    return cross(cloth_pos(buf, ix[1], ov, op) - x0, cloth_pos(buf, ix[2], ov, op) - x0);
+  }
+  static V3<T> normalized_differential(V3<T> u, T len, V3<T> d) {  // S:1688-1698
+    return (d - u * dot(u, d)) / len;
+  }
+  struct Dihedral {
+    T angle, edge_length;
+    bool valid;
+    V3<T> grad[4];
+  };
+  Dihedral dihedral_at(uint32_t hinge, int buf, uint32_t ov, V3<T> op) const {  // S:1700-1810
+    const uint32_t* ix = &hinge_idx[4 * size_t(hinge)];
+    V3<T> x[4];
+    for (int l = 0; l < 4; ++l) x[l] = cloth_pos(buf, ix[l], ov, op);
+    Dihedral out;
+    out.angle = T(0);
+    out.edge_length = T(0);
+    out.valid = false;
+    for (auto& g : out.grad) g = v3<T>(0, 0, 0);
+    V3<T> edge = x[1] - x[0];
+    T el = length(edge);
+    V3<T> to2 = x[2] - x[0];
+    V3<T> rev = -edge;
+    V3<T> to3 = x[3] - x[1];
+    V3<T> rn0 = cross(edge, to2), rn1 = cross(rev, to3);
+    T nl0 = length(rn0), nl1 = length(rn1);
+    T es0 = std::max(el, std::max(length(to2), length(to2 - edge)));
+    T es1 = std::max(el, std::max(length(to3), length(to3 - rev)));
+    bool ok = finite_s(el) && el > T(0) && finite_s(nl0) && finite_s(nl1) && nl0 > T(1.0e-12) * es0 * es0 &&
+              nl1 > T(1.0e-12) * es1 * es1;
+    if (!ok) return out;
+    V3<T> tangent = edge / el, n0 = rn0 / nl0, n1 = rn1 / nl1;
+    V3<T> nc = cross(n0, n1);
+    T sine = dot(tangent, nc), cosine = dot(n0, n1);
+    T den = sine * sine + cosine * cosine;
+    if (!finite_s(den) || !(den > T(0))) return out;
+    V3<T> grads[4];
+    for (int c = 0; c < 12; ++c) {
+      int dv = c / 3, ax = c % 3;
+      V3<T> dl[4] = {v3<T>(0, 0, 0), v3<T>(0, 0, 0), v3<T>(0, 0, 0), v3<T>(0, 0, 0)};
+      dl[dv][ax] = T(1);
+      V3<T> de = dl[1] - dl[0];
+      V3<T> dt = normalized_differential(tangent, el, de);
+      V3<T> dto2 = dl[2] - dl[0], dto3 = dl[3] - dl[1];
+      V3<T> dn0r = cross(de, to2) + cross(edge, dto2);
+      V3<T> dn1r = cross(-de, to3) + cross(rev, dto3);
+      V3<T> dn0 = normalized_differential(n0, nl0, dn0r);
+      V3<T> dn1 = normalized_differential(n1, nl1, dn1r);
+      T dcos = dot(dn0, n1) + dot(n0, dn1);
+      T dsin = dot(dt, nc) + dot(tangent, cross(dn0, n1) + cross(n0, dn1));
+      grads[dv][ax] = (cosine * dsin - sine * dcos) / den;
+    }
+    T angle = std::atan2(sine, cosine);
+    bool fin = finite_s(angle);
+    for (auto& g : grads) fin = fin && finite_v(g);
+    if (!fin) return out;
+    out.angle = angle;
+    out.edge_length = el;
+    out.valid = true;
+    for (int l = 0; l < 4; ++l) out.grad[l] = grads[l];
+    return out;
+  }
+  static T wrapped_angle_difference(T a, T r) {  // S:1812-1815
+    T d = a - r;
+    return std::atan2(std::sin(d), std::cos(d));
+  }
+  T bending_energy_at(uint32_t hinge, int buf, uint32_t ov, V3<T> op) const {  // S:1817-1836
+    Dihedral ev = dihedral_at(hinge, buf, ov, op);
+    if (!ev.valid) return T(kF32Max);
+    T ad = wrapped_angle_difference(ev.angle, hinge_rest[2 * size_t(hinge)]);
+    T k = cloth_bend * hinge_rest[2 * size_t(hinge) + 1];
+    return T(0.5) * k * ad * ad;
+  }
+  uint32_t tri_slot(uint32_t tri, uint32_t v) const {  // S:1505-1510
+    const uint32_t* ix = &tri_idx[3 * size_t(tri)];
+    if (ix[0] == v) return 0;
+    if (ix[1] == v) return 1;
+    if (ix[2] == v) return 2;
+    return 3;
+  }
+  uint32_t hinge_slot(uint32_t h, uint32_t v) const {  // S:1512-1518
+    const uint32_t* ix = &hinge_idx[4 * size_t(h)];
+    for (uint32_t l = 0; l < 4; ++l)
+      if (ix[l] == v) return l;
+    return 4;
+  }
+  bool tri_owned(uint32_t tri, uint32_t v) const {  // S:1838-1840
+    const uint32_t* ix = &tri_idx[3 * size_t(tri)];
+    return v == std::min(ix[0], std::min(ix[1], ix[2]));
+  }
+  bool hinge_owned(uint32_t h, uint32_t v) const {  // S:1842-1844
+    const uint32_t* ix = &hinge_idx[4 * size_t(h)];
+    return v == std::min(std::min(ix[0], ix[1]), std::min(ix[2], ix[3]));
   }
 
   // --------------------------------------------------------------- restricted model
@@ -802,6 +999,19 @@ template <class T> struct Sim {
       if (!finite_s(dv)) return Geometry{T(0), false};
       minimum = std::min(minimum, dv);
     }
+    // Cloth signed relative area stretch (S:2062-2116).
+    for (uint32_t k = tri_rows[sv]; k < tri_rows[sv + 1]; ++k) {
+      uint32_t tri = tri_elems[k];
+      if (tri_slot(tri, sv) >= 3) continue;
+      V3<T> sn = tri_raw_normal(tri, p.src, 0xffffffffu, v3<T>(0, 0, 0));
+      V3<T> tn = tri_raw_normal(tri, p.src, sv, load(p.src, sv) + d);
+      T tra = T(2) * tri_area[tri];
+      T ss = length(sn) / tra;
+      T tm = length(tn) / tra;
+      T ts = dot(sn, tn) > T(0) ? tm : -tm;
+      if (!(tra > T(0)) || !finite_s(ss) || !finite_s(ts)) return Geometry{T(0), false};
+      minimum = std::min(minimum, std::min(ss, ts));
+    }
     return Geometry{minimum, true};
   }
   T restricted_energy_delta(uint32_t sv, V3<T> d) const {  // S:2120-2337 (tetrahedral part)
@@ -814,6 +1024,21 @@ template <class T> struct Sim {
       M3<T> i = deformation(t);
       M3<T> f = exact_trial_deformation(t, sv, d);
       result += material_delta(t, i, f);
+    }
+    V3<T> tp = sp + d;
+    for (uint32_t k = tri_rows[sv]; k < tri_rows[sv + 1]; ++k) {  // S:2203-2236
+      uint32_t tri = tri_elems[k];
+      if (tri_slot(tri, sv) >= 3) continue;
+      Membrane a = membrane_at(tri, p.src, 0xffffffffu, v3<T>(0, 0, 0));
+      Membrane b = membrane_at(tri, p.src, sv, tp);
+      result += b.energy - a.energy;
+    }
+    for (uint32_t k = hinge_rows[sv]; k < hinge_rows[sv + 1]; ++k) {  // S:2237-2271
+      uint32_t h = hinge_elems[k];
+      if (hinge_slot(h, sv) >= 4) continue;
+      T a = bending_energy_at(h, p.src, 0xffffffffu, v3<T>(0, 0, 0));
+      T b = bending_energy_at(h, p.src, sv, tp);
+      result += b - a;
     }
     for (uint32_t s = 0; s < K; ++s) {
       size_t rec = size_t(sv) * K + s;
@@ -869,6 +1094,31 @@ template <class T> struct Sim {
           H = H + stable_local_hessian(t, slot);
         else
           H = H + R * stiffness_block(t, slot, slot) * transpose(R);
+      }
+    }
+    for (uint32_t k = tri_rows[v]; k < tri_rows[v + 1]; ++k) {  // S:3117-3144
+      uint32_t tri = tri_elems[k];
+      uint32_t slot = tri_slot(tri, v);
+      if (slot < 3) {
+        Membrane m = membrane_at(tri, p.src, 0xffffffffu, v3<T>(0, 0, 0));
+        g = g + m.grad[slot];
+        H = H + membrane_local_hessian(tri, int(slot), p.src);
+      }
+    }
+    for (uint32_t k = hinge_rows[v]; k < hinge_rows[v + 1]; ++k) {  // S:3145-3184
+      uint32_t h = hinge_elems[k];
+      uint32_t slot = hinge_slot(h, v);
+      if (slot < 4) {
+        Dihedral d = dihedral_at(h, p.src, 0xffffffffu, v3<T>(0, 0, 0));
+        if (!d.valid) {
+          g = g + v3<T>(T(kF32Max), T(kF32Max), T(kF32Max));
+        } else {
+          T ad = wrapped_angle_difference(d.angle, hinge_rest[2 * size_t(h)]);
+          T k2 = cloth_bend * hinge_rest[2 * size_t(h) + 1];
+          V3<T> ag = d.grad[slot];
+          g = g + (k2 * ad) * ag;
+          H = H + k2 * outer(ag, ag);
+        }
       }
     }
     if (p.floor_k > T(0) && x.y < p.floor_y) {  // S:3188-3191
@@ -1162,6 +1412,19 @@ template <class T> struct Sim {
       tet_glob[2 * size_t(t) + 1] =
           V4<T>{sev ? se : T(0), cev ? ce : T(0), sev ? T(1) : T(0), cev ? T(1) : T(0)};
     }
+    for (uint32_t tri = 0; tri < NTRI; ++tri) {  // S:3508-3549
+      V3<T> sn = tri_raw_normal(tri, p.src, 0xffffffffu, v3<T>(0, 0, 0));
+      V3<T> cn = tri_raw_normal(tri, p.tgt, 0xffffffffu, v3<T>(0, 0, 0));
+      T tra = T(2) * tri_area[tri];
+      T ss = length(sn) / tra;
+      T cm = length(cn) / tra;
+      T cs = dot(sn, cn) > T(0) ? cm : -cm;
+      bool sgv = tra > T(0) && finite_v(sn) && finite_s(ss);
+      bool cgv = tra > T(0) && finite_v(cn) && finite_s(cs);
+      size_t e = size_t(TT) + tri;
+      tet_glob[2 * e] = V4<T>{sgv ? ss : T(0), cgv ? cs : T(0), sgv ? T(1) : T(0), cgv ? T(1) : T(0)};
+      tet_glob[2 * e + 1] = V4<T>{0, 0, 1, 1};
+    }
   }
   static bool local_status_failure(uint32_t s) {  // S:3603-3609
     return s == kStatusShiftLimit || s == kStatusNonDescent || s == kStatusLineSearchFailed ||
@@ -1191,7 +1454,7 @@ template <class T> struct Sim {
     std::vector<CandLane> lanes(kWorkgroup);
     for (uint32_t lane = 0; lane < kWorkgroup; ++lane) {
       CandLane r{{T(kF32Max), T(kF32Max)}, {T(0), T(0)}, T(0), 0, 0, 0, 0, kReductionMask};
-      for (uint32_t t = lane; t < TT; t += kWorkgroup) {
+      for (uint32_t t = lane; t < TT + NTRI; t += kWorkgroup) {
         V4<T> geo = tet_glob[2 * size_t(t)];
         V4<T> en = tet_glob[2 * size_t(t) + 1];
         bool sgv = geo.z == T(1) && finite_s(geo.x);
@@ -1349,6 +1612,23 @@ template <class T> struct Sim {
           V3<T> sg[4];
           stable_gradient_at(t, p.tgt, sg);
           mat = mat + sg[slot];
+        }
+      }
+      for (uint32_t k = tri_rows[v]; k < tri_rows[v + 1]; ++k) {  // S:4028-4051
+        uint32_t tri = tri_elems[k];
+        uint32_t slot = tri_slot(tri, v);
+        if (slot < 3) mat = mat + membrane_at(tri, p.tgt, 0xffffffffu, v3<T>(0, 0, 0)).grad[slot];
+      }
+      for (uint32_t k = hinge_rows[v]; k < hinge_rows[v + 1]; ++k) {  // S:4052-4085
+        uint32_t h = hinge_elems[k];
+        uint32_t slot = hinge_slot(h, v);
+        if (slot >= 4) continue;
+        Dihedral d = dihedral_at(h, p.tgt, 0xffffffffu, v3<T>(0, 0, 0));
+        if (!d.valid) {
+          mat = mat + v3<T>(T(kF32Max), T(kF32Max), T(kF32Max));
+        } else {
+          T ad = wrapped_angle_difference(d.angle, hinge_rest[2 * size_t(h)]);
+          mat = mat + (cloth_bend * hinge_rest[2 * size_t(h) + 1] * ad) * d.grad[slot];
         }
       }
       V3<T> contact = v3<T>(0, 0, 0);
@@ -1710,6 +1990,36 @@ template <class T> struct Sim {
       colors.assign(V, 0);
       color_count = 1;
     }
+    NTRI = in.cloth_triangle_count;
+    NHINGE = in.cloth_hinge_count;
+    tri_idx.assign(in.cloth_triangles, in.cloth_triangles + 3 * size_t(NTRI));
+    hinge_idx.assign(in.cloth_hinges, in.cloth_hinges + 4 * size_t(NHINGE));
+    tri_inv.resize(4 * size_t(NTRI));
+    tri_area.resize(NTRI);
+    hinge_rest.resize(2 * size_t(NHINGE));
+    for (size_t i = 0; i < tri_inv.size(); ++i) tri_inv[i] = T(in.cloth_inverse_rest_bases[i]);
+    for (size_t i = 0; i < NTRI; ++i) tri_area[i] = T(in.cloth_rest_areas[i]);
+    for (size_t i = 0; i < hinge_rest.size(); ++i) hinge_rest[i] = T(in.cloth_hinge_rest[i]);
+    if (NTRI) {  // cloth-layout.ts:335-343, f64 then rounded like the packed arena
+      double E = in.cloth_young_modulus, nu = in.cloth_poisson_ratio;
+      cloth_lam = T(float(E * nu / (1 - nu * nu)));
+      cloth_mu = T(float(E / (2 * (1 + nu))));
+      cloth_thick = T(float(in.cloth_thickness));
+      cloth_bend = T(float(in.cloth_bending_stiffness));
+    }
+    // buildIncidentElementCsr (cloth-layout.ts:183-207).
+    auto csr = [&](const std::vector<uint32_t>& idx, uint32_t width, std::vector<uint32_t>& rows,
+                   std::vector<uint32_t>& elems) {
+      rows.assign(size_t(V) + 1, 0);
+      for (uint32_t x : idx) rows[x + 1] += 1;
+      for (uint32_t v = 0; v < V; ++v) rows[v + 1] += rows[v];
+      elems.assign(idx.size(), 0);
+      std::vector<uint32_t> cur(rows.begin(), rows.end() - 1);
+      for (size_t e = 0; e < idx.size() / width; ++e)
+        for (uint32_t l = 0; l < width; ++l) elems[cur[idx[width * e + l]]++] = uint32_t(e);
+    };
+    csr(tri_idx, 3, tri_rows, tri_elems);
+    csr(hinge_idx, 4, hinge_rows, hinge_elems);
     pos[0].resize(V);
     pos[1].resize(V);
     predicted.resize(V);
@@ -1726,7 +2036,7 @@ template <class T> struct Sim {
     body_corr.assign(bodies, V4<T>{0, 0, 0, 0});
     final_update.assign(V, V4<T>{0, 0, 0, 0});
     local_diag.assign(4 * size_t(V), V4<T>{0, 0, 0, 0});
-    tet_glob.assign(2 * size_t(TT), V4<T>{0, 0, 0, 0});
+    tet_glob.assign(2 * (size_t(TT) + NTRI), V4<T>{0, 0, 0, 0});
     vertex_energy.assign(V, V4<T>{0, 0, 0, 0});
     conv.assign(5 * size_t(V), v3<T>(0, 0, 0));
     for (auto& c : control) c = V4<T>{0, 0, 0, 0};
@@ -1828,6 +2138,61 @@ template <class T> void restricted_model(Sim<T>& s, const jgs2o_settings& st, ui
   out[0] = double(g.minimum);
   out[1] = g.valid ? 1.0 : 0.0;
   out[2] = double(s.restricted_energy_delta(v, dd));
+}
+
+
+// Rest data of one StVK triangle (src/simulation/cpu/cloth.ts:115-140):
+// row-major inverse of Dm = [[|e01|, e02.t], [0, e02.b]] and the rest area.
+void stvk_triangle_rest(const double* x, double* inv, double* area) {
+  double e01[3], e02[3];
+  for (int a = 0; a < 3; ++a) {
+    e01[a] = x[3 + a] - x[a];
+    e02[a] = x[6 + a] - x[a];
+  }
+  double c[3] = {e01[1] * e02[2] - e01[2] * e02[1], e01[2] * e02[0] - e01[0] * e02[2],
+                 e01[0] * e02[1] - e01[1] * e02[0]};
+  double twice = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  double len = std::sqrt(e01[0] * e01[0] + e01[1] * e01[1] + e01[2] * e01[2]);
+  double t[3], n[3], b[3];
+  for (int a = 0; a < 3; ++a) {
+    t[a] = e01[a] / len;
+    n[a] = c[a] / twice;
+  }
+  b[0] = n[1] * t[2] - n[2] * t[1];
+  b[1] = n[2] * t[0] - n[0] * t[2];
+  b[2] = n[0] * t[1] - n[1] * t[0];
+  double mx = e02[0] * t[0] + e02[1] * t[1] + e02[2] * t[2];
+  double my = e02[0] * b[0] + e02[1] * b[1] + e02[2] * b[2];
+  inv[0] = 1.0 / len;
+  inv[1] = -mx / (len * my);
+  inv[2] = 0.0;
+  inv[3] = 1.0 / my;
+  *area = 0.5 * twice;
+}
+
+// A four-vertex scene holding one triangle (0,1,2) and one hinge (0,1,2,3).
+template <class T> Sim<T> cloth_probe(const double* rest12, const double* pos12, double young, double poisson,
+                                      double thickness, double bending) {
+  Sim<T> s;
+  s.V = 4;
+  s.NTRI = 1;
+  s.NHINGE = 1;
+  s.tri_idx = {0, 1, 2};
+  s.hinge_idx = {0, 1, 2, 3};
+  double inv[4], area;
+  stvk_triangle_rest(rest12, inv, &area);
+  for (int i = 0; i < 4; ++i) s.tri_inv.push_back(T(inv[i]));
+  s.tri_area = {T(area)};
+  s.cloth_lam = T(young * poisson / (1 - poisson * poisson));
+  s.cloth_mu = T(young / (2 * (1 + poisson)));
+  s.cloth_thick = T(thickness);
+  s.cloth_bend = T(bending);
+  s.pos[0].resize(4);
+  for (int v = 0; v < 4; ++v) s.pos[0][v] = v3(T(rest12[3 * v]), T(rest12[3 * v + 1]), T(rest12[3 * v + 2]));
+  typename Sim<T>::Dihedral r = s.dihedral_at(0, 0, 0xffffffffu, v3<T>(0, 0, 0));
+  s.hinge_rest = {r.angle, r.edge_length};
+  for (int v = 0; v < 4; ++v) s.pos[0][v] = v3(T(pos12[3 * v]), T(pos12[3 * v + 1]), T(pos12[3 * v + 2]));
+  return s;
 }
 
 }  // namespace
@@ -2035,6 +2400,41 @@ void jgs2o_set_pose(void* hv, const float* positions, const float* predicted) {
   };
   if (h->d) run(*h->d); else run(*h->f);
 }
+
+
+// StVK membrane of triangle (0,1,2) and quadratic dihedral bending of hinge
+// (0,1,2,3) at `pos12`, rest data from `rest12` (S:1546-1836). out:
+// [0] membrane energy, [1..9] membrane gradient, [10] bending energy,
+// [11..22] bending gradient, [23] angle, [24] rest angle, [25] rest edge,
+// [26..34] membrane local Hessian of vertex 0 (row-major), [35] hinge valid.
+void jgs2o_cloth_probe(int precision, const double* rest12, const double* pos12, double young, double poisson,
+                       double thickness, double bending, double* out) {
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    Sim<T> s = cloth_probe<T>(rest12, pos12, young, poisson, thickness, bending);
+    auto m = s.membrane_at(0, 0, 0xffffffffu, v3<T>(0, 0, 0));
+    out[0] = double(m.energy);
+    for (int l = 0; l < 3; ++l)
+      for (int a = 0; a < 3; ++a) out[1 + 3 * l + a] = double(m.grad[l][a]);
+    auto d = s.dihedral_at(0, 0, 0xffffffffu, v3<T>(0, 0, 0));
+    out[10] = double(s.bending_energy_at(0, 0, 0xffffffffu, v3<T>(0, 0, 0)));
+    T ad = Sim<T>::wrapped_angle_difference(d.angle, s.hinge_rest[0]);
+    T k = s.cloth_bend * s.hinge_rest[1];
+    for (int l = 0; l < 4; ++l)
+      for (int a = 0; a < 3; ++a) out[11 + 3 * l + a] = double(k * ad * d.grad[l][a]);
+    out[23] = double(d.angle);
+    out[24] = double(s.hinge_rest[0]);
+    out[25] = double(s.hinge_rest[1]);
+    M3<T> h = s.membrane_local_hessian(0, 0, 0);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) out[26 + 3 * r + c] = double(h.c[c][r]);
+    out[35] = d.valid ? 1.0 : 0.0;
+  };
+  if (precision == 64) run(0.0); else run(0.0f);
+}
+
+// Rest data of a triangle: inv[4] row-major, area.
+void jgs2o_triangle_rest(const double* x9, double* inv4, double* area) { stvk_triangle_rest(x9, inv4, area); }
 
 // Deterministic greedy coloring (src/simulation/cpu/vertex-coloring.ts:20-110)
 // restated over a CSR list of conflict groups; returns the color count.

@@ -42,6 +42,17 @@ class OracleInput(C.Structure):
         ("objectives", f32p),
         ("colors", u32p),
         ("color_count", C.c_uint32),
+        ("cloth_triangle_count", C.c_uint32),
+        ("cloth_triangles", u32p),
+        ("cloth_inverse_rest_bases", f32p),
+        ("cloth_rest_areas", f32p),
+        ("cloth_hinge_count", C.c_uint32),
+        ("cloth_hinges", u32p),
+        ("cloth_hinge_rest", f32p),
+        ("cloth_young_modulus", C.c_double),
+        ("cloth_poisson_ratio", C.c_double),
+        ("cloth_thickness", C.c_double),
+        ("cloth_bending_stiffness", C.c_double),
     ]
 
 
@@ -97,6 +108,8 @@ def _load() -> C.CDLL:
         "jgs2o_globalization_solve": (C.c_int, [C.c_int, f64p, f64p, C.c_double, f64p]),
         "jgs2o_min_eigenvalue": (C.c_double, [C.c_int, f64p]),
         "jgs2o_greedy_coloring": (C.c_uint32, [C.c_uint32, u32p, C.c_uint32, u32p, u32p]),
+        "jgs2o_cloth_probe": (None, [C.c_int, f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_double, f64p]),
+        "jgs2o_triangle_rest": (None, [f64p, f64p, f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -169,7 +182,8 @@ def greedy_coloring(vertex_count: int, groups) -> np.ndarray:
 
 
 def conflict_groups(scene) -> list:
-    """buildJGS2InputVertexColoring conflict groups (jgs2-solver.ts:308-367)."""
+    """buildJGS2InputVertexColoring conflict groups (jgs2-solver.ts:308-367),
+    cloth triangles and hinges included (jgs2-solver.ts:329-351)."""
     T, K, V = scene.tet_count, scene.cubature_k, scene.vertex_count
     tets = scene.tet_indices.reshape(T, 4)
     groups = [list(t) for t in tets]
@@ -182,6 +196,10 @@ def conflict_groups(scene) -> list:
                 if t == 0xFFFFFFFF or not (w[v, k] > 0):
                     continue
                 groups.append([v, *tets[t]])
+    cloth = getattr(scene, "cloth", None)
+    if cloth is not None:
+        groups.extend(list(t) for t in cloth.triangle_indices.reshape(-1, 3))
+        groups.extend(list(h) for h in cloth.hinge_indices.reshape(-1, 4))
     return groups
 
 
@@ -231,6 +249,20 @@ class Oracle:
         if colors is not None:
             inp.colors = up(colors)
             inp.color_count = int(np.max(colors)) + 1
+        cloth = getattr(scene, "cloth", None)
+        if cloth is not None:
+            inp.cloth_triangle_count = cloth.triangle_count
+            inp.cloth_triangles = up(cloth.triangle_indices)
+            inp.cloth_inverse_rest_bases = fp(cloth.triangle_inverse_rest_bases)
+            inp.cloth_rest_areas = fp(cloth.triangle_rest_areas)
+            inp.cloth_hinge_count = cloth.hinge_count
+            inp.cloth_hinges = up(cloth.hinge_indices)
+            rest = np.stack([cloth.hinge_rest_angles, cloth.hinge_rest_edge_lengths], axis=1).reshape(-1)
+            inp.cloth_hinge_rest = fp(rest)
+            inp.cloth_young_modulus = float(cloth.young_modulus)
+            inp.cloth_poisson_ratio = float(cloth.poisson_ratio)
+            inp.cloth_thickness = float(cloth.thickness)
+            inp.cloth_bending_stiffness = float(cloth.bending_stiffness)
         self._handle = lib.jgs2o_create(C.byref(inp), precision)
 
     def __del__(self):
@@ -340,3 +372,28 @@ def globalization_solve(H, g, inertia_scale, precision=64):
 def min_eigenvalue(H, precision=64) -> float:
     h = np.ascontiguousarray(H, dtype=np.float64).reshape(9)
     return lib.jgs2o_min_eigenvalue(precision, h.ctypes.data_as(f64p))
+
+
+def cloth_probe(rest12, pos12, young, poisson, thickness, bending, precision=64) -> dict:
+    """StVK membrane of triangle (0,1,2) and dihedral bending of hinge
+    (0,1,2,3) of a four-vertex sheet with rest pose `rest12` evaluated at
+    `pos12` (src/simulation/cpu/cloth.ts restated)."""
+    r = np.ascontiguousarray(rest12, dtype=np.float64).reshape(12)
+    x = np.ascontiguousarray(pos12, dtype=np.float64).reshape(12)
+    out = np.empty(36)
+    lib.jgs2o_cloth_probe(precision, r.ctypes.data_as(f64p), x.ctypes.data_as(f64p), float(young), float(poisson),
+                          float(thickness), float(bending), out.ctypes.data_as(f64p))
+    return {
+        "membrane_energy": out[0], "membrane_gradient": out[1:10].reshape(3, 3).copy(),
+        "bending_energy": out[10], "bending_gradient": out[11:23].reshape(4, 3).copy(), "angle": out[23],
+        "rest_angle": out[24], "rest_edge_length": out[25], "membrane_hessian_vertex0": out[26:35].reshape(3, 3).copy(),
+        "hinge_valid": bool(out[35]),
+    }
+
+
+def triangle_rest(x9):
+    x = np.ascontiguousarray(x9, dtype=np.float64).reshape(9)
+    inv = np.empty(4)
+    area = C.c_double()
+    lib.jgs2o_triangle_rest(x.ctypes.data_as(f64p), inv.ctypes.data_as(f64p), C.byref(area))
+    return inv.reshape(2, 2), area.value

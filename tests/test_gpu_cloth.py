@@ -1,0 +1,107 @@
+"""GPU parity of the cloth layer (StVK membrane + dihedral bending) against
+the oracle on the reference's pinned cloth drape (src/scenes/index.ts:
+632-705) and on a larger flat sheet.
+
+Tolerances: local gradients and Hessians within 1e-5 relative, one frame
+within 1e-5 relative L2, 60 frames within 1e-3 of the FP64 oracle
+(relative to the displacement), schedule colors bit-exact.
+"""
+
+import numpy as np
+import pytest
+
+import jgs2
+from jgs2 import scenes
+import jgs2_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GH_TOL = 1e-5
+FRAME_TOL = 1e-5
+LONG_TOL = 1e-3
+
+
+def xyz(p):
+    return np.asarray(p, np.float64).reshape(-1, 4)[:, :3]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.fixture(scope="module")
+def drape():
+    return scenes.cloth_drape()
+
+
+def test_cloth_local_systems_match_oracle(drape):
+    settings = dict(scenes.CLOTH_SETTINGS)
+    print("case: cloth local systems, drape V=%d" % drape.vertex_count, flush=True)
+    solver = jgs2.JGS2GpuSolver.create(drape, settings)
+    solver.step_frames_exact_iterations(12, scenes.CLOTH_ITERATIONS, settings)
+    g_gpu, h_gpu = solver.debug_local_systems(settings)
+    oracle = O.Oracle(drape, 64)
+    oracle.set_pose(solver.read_positions(), solver.read_predicted_positions())
+    g_ref, h_ref = oracle.local_systems(settings)
+    free = drape.vertex_info.reshape(-1, 4)[:, 2] == 0
+    g_err = np.linalg.norm(g_gpu[free] - g_ref[free], axis=1) / np.maximum(
+        np.linalg.norm(g_ref[free], axis=1), 1e-6 * np.abs(h_ref[free]).max(axis=(1, 2)))
+    h_err = np.abs(h_gpu[free] - h_ref[free]).max(axis=(1, 2)) / np.abs(h_ref[free]).max(axis=(1, 2))
+    print("max g rel", g_err.max(), "max H rel", h_err.max(), flush=True)
+    assert h_err.max() < GH_TOL
+    assert g_err.max() < 1e-4
+    solver.destroy()
+
+
+def test_cloth_one_frame_matches_oracle(drape):
+    settings = dict(scenes.CLOTH_SETTINGS)
+    print("case: cloth one frame", flush=True)
+    solver = jgs2.JGS2GpuSolver.create(drape, settings)
+    solver.step_exact_iterations(scenes.CLOTH_ITERATIONS, settings)
+    x = xyz(solver.read_positions())
+    ref = O.Oracle(drape, 64)
+    ref.step_frames(1, scenes.CLOTH_ITERATIONS, settings)
+    x0 = xyz(drape.positions)
+    e, d = rel(x, ref.positions()), rel(x - x0, ref.positions() - x0)
+    print(f"rel L2 {e:.3e}, displacement rel {d:.3e}", flush=True)
+    assert e < FRAME_TOL
+    assert d < 1e-3
+    assert solver.read_globalization_control().reshape(4, 4)[1, 0] == 1.0
+    solver.destroy()
+
+
+@pytest.mark.parametrize("schedule", ["jacobi", "graph-colored-gauss-seidel"])
+def test_cloth_sixty_frames_match_fp64_oracle(drape, schedule):
+    settings = dict(scenes.CLOTH_SETTINGS, schedule=schedule)
+    print(f"case: cloth 60 frames, {schedule}", flush=True)
+    solver = jgs2.JGS2GpuSolver.create(drape, settings)
+    colors = solver.read_schedule_colors() if schedule != "jacobi" else None
+    if colors is not None:
+        assert np.array_equal(colors, O.greedy_coloring(drape.vertex_count, O.conflict_groups(drape)))
+    ref = O.Oracle(drape, 64, colors=colors)
+    for chunk in range(3):
+        solver.step_frames_exact_iterations(20, scenes.CLOTH_ITERATIONS, settings)
+        ref.step_frames(20, scenes.CLOTH_ITERATIONS, settings)
+    x, xr, x0 = xyz(solver.read_positions()), ref.positions(), xyz(drape.positions)
+    e, d = rel(x, xr), rel(x - x0, xr - x0)
+    print(f"60-frame rel L2 {e:.3e}, displacement rel {d:.3e}", flush=True)
+    assert np.isfinite(x).all()
+    assert d < LONG_TOL
+    hist = solver.read_globalization_history().reshape(64, 8, 4)
+    assert np.array_equal(hist[:scenes.CLOTH_ITERATIONS, 1, 0], ref.history()[:scenes.CLOTH_ITERATIONS, 1, 0])
+    solver.destroy()
+
+
+def test_large_flat_cloth_matches_fp64_oracle():
+    scene = scenes.cloth_drape(cells=32, fold_column=0)
+    settings = dict(scenes.CLOTH_SETTINGS)
+    print(f"case: flat 32x32 cloth V={scene.vertex_count}, 20 Jacobi frames", flush=True)
+    solver = jgs2.JGS2GpuSolver.create(scene, settings)
+    solver.step_frames_exact_iterations(20, scenes.CLOTH_ITERATIONS, settings)
+    ref = O.Oracle(scene, 64)
+    ref.step_frames(20, scenes.CLOTH_ITERATIONS, settings)
+    x, xr, x0 = xyz(solver.read_positions()), ref.positions(), xyz(scene.positions)
+    d = rel(x - x0, xr - x0)
+    print(f"displacement rel {d:.3e}", flush=True)
+    assert d < LONG_TOL
+    solver.destroy()

@@ -20,12 +20,13 @@ struct PassParams {
   uint32_t corotated;         // legacy co-rotated material path
   uint32_t fused_rot;         // solve reads frames from vrot_next and adopts them
   uint32_t compute_next_rot;  // convergence pass also computes the next frames
-  uint32_t pad1, pad2;
+  uint32_t tri_count, hinge_count;  // cloth layer sizes (0 = no cloth)
   float dt, inv_dt, inv_dt2, max_step;
   float gx, gy, gz, floor_y;
   float floor_k, regularization, polar_eps, damping;
   float tangential_damping, contact_margin, ipc_dhat, ipc_kappa;
-  float residual_tol, update_tol, scene_scale, pad0;
+  float residual_tol, update_tol, scene_scale;
+  uint32_t energy_after_sweep;  // vertex-energy pass that replaces the in-solve one
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
@@ -37,6 +38,15 @@ struct PassParams {
 //   adjacency uint32[4T]
 //   cub       uint4[10 * V * K] record = {tet, weight bits, 0, 0} + 9 float4 Ubar
 //   objectives float4[2V]
+//   tri_ix    uint4[NT]  cloth triangle {v0, v1, v2, owner}; owner = the
+//                        vertex with the smallest caller id, which sums the
+//                        element into its vertex energy
+//   tri_inv   float4[NT] inverse rest basis (row-major 2x2)
+//   tri_area  float[NT]  rest area
+//   hinge_ix  uint4[NH]  hinge {v0, v1, v2, v3}: triangles (0,1,2), (1,0,3)
+//   hinge_rest float4[NH] {rest angle, rest edge length, owner bits, 0}
+//   tri_rows/hinge_rows uint32[V+1], *_elems: vertex -> element CSR, rows
+//                        ascending in element id
 struct DeviceState {
   uint32_t V = 0, T = 0, K = 0, bodies = 1;
   float4* vtx_rest = nullptr;
@@ -68,6 +78,18 @@ struct DeviceState {
   uint32_t partial_capacity = 0;
   float* debug_g = nullptr;      // debug local systems
   float* debug_h = nullptr;
+  // Cloth layer.
+  uint32_t NT = 0, NH = 0;
+  float cloth_lam = 0, cloth_mu = 0, cloth_thick = 0, cloth_bend = 0;
+  uint4* tri_ix = nullptr;
+  float4* tri_inv = nullptr;
+  float* tri_area = nullptr;
+  uint4* hinge_ix = nullptr;
+  float4* hinge_rest = nullptr;
+  uint32_t* tri_rows = nullptr;
+  uint32_t* tri_elems = nullptr;
+  uint32_t* hinge_rows = nullptr;
+  uint32_t* hinge_elems = nullptr;
 };
 
 constexpr uint32_t kHistoryCapacity = 64;
@@ -86,7 +108,7 @@ void launch_body_correction(const DeviceState& s, const PassParams& p, hipStream
 void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_clear_frame(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_debug_local_systems(const DeviceState& s, const PassParams& p, hipStream_t stream);
-uint32_t partial_blocks_needed(uint32_t V, uint32_t T);
+uint32_t partial_blocks_needed(uint32_t V, uint32_t elements);
 size_t partial_bytes(uint32_t blocks);
 
 }  // namespace jgs2

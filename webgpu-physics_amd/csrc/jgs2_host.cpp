@@ -89,7 +89,8 @@ void free_device(DeviceState& d) {
   void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_idx, d.tet_dm, d.tet_k, d.adjacency, d.cub, d.objectives,
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.conv, d.control,
-                  d.history, d.partials, d.debug_g, d.debug_h};
+                  d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
+                  d.hinge_rest, d.tri_rows, d.tri_elems, d.hinge_rows, d.hinge_elems};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   d = DeviceState{};
@@ -338,6 +339,78 @@ jgs2_status validate_input(const jgs2_scene_input& in) {
   return JGS2_OK;
 }
 
+// validateJGS2GpuClothInput (cloth-layout.ts:209-297).
+jgs2_status validate_cloth(const jgs2_cloth_input& c, uint32_t V) {
+  const uint32_t NT = c.triangle_count, NH = c.hinge_count;
+  if ((NT && (!c.triangle_indices || !c.triangle_inverse_rest_bases || !c.triangle_rest_areas)) ||
+      (NH && (!c.hinge_indices || !c.hinge_rest_angles || !c.hinge_rest_edge_lengths)))
+    return fail(JGS2_ERROR_TYPE, "A required cloth array is NULL.");
+  auto distinct = [&](const uint32_t* ix, uint32_t width, uint32_t element, const char* label) -> jgs2_status {
+    for (uint32_t l = 0; l < width; ++l) {
+      if (ix[l] >= V)
+        return fail(JGS2_ERROR_RANGE, "%s %u index %u is outside vertexCount %u.", label, element, ix[l], V);
+      for (uint32_t q = 0; q < l; ++q)
+        if (ix[q] == ix[l]) return fail(JGS2_ERROR_RANGE, "%s %u must reference %u distinct vertices.", label, element, width);
+    }
+    return JGS2_OK;
+  };
+  for (uint32_t t = 0; t < NT; ++t) {
+    jgs2_status st = distinct(c.triangle_indices + 3 * size_t(t), 3, t, "Triangle");
+    if (st != JGS2_OK) return st;
+  }
+  for (uint32_t h = 0; h < NH; ++h) {
+    jgs2_status st = distinct(c.hinge_indices + 4 * size_t(h), 4, h, "Hinge");
+    if (st != JGS2_OK) return st;
+  }
+  size_t bad = 0;
+  if (NT && !finite_array(c.triangle_inverse_rest_bases, 4 * size_t(NT), &bad))
+    return fail(JGS2_ERROR_RANGE, "triangleInverseRestBases[%zu] must be finite.", bad);
+  if (NT && !finite_array(c.triangle_rest_areas, NT, &bad))
+    return fail(JGS2_ERROR_RANGE, "triangleRestAreas[%zu] must be finite.", bad);
+  if (NH && !finite_array(c.hinge_rest_angles, NH, &bad))
+    return fail(JGS2_ERROR_RANGE, "hingeRestAngles[%zu] must be finite.", bad);
+  if (NH && !finite_array(c.hinge_rest_edge_lengths, NH, &bad))
+    return fail(JGS2_ERROR_RANGE, "hingeRestEdgeLengths[%zu] must be finite.", bad);
+  for (uint32_t t = 0; t < NT; ++t) {
+    if (!(c.triangle_rest_areas[t] > 0)) return fail(JGS2_ERROR_RANGE, "triangleRestAreas[%u] must be positive.", t);
+    const float* b = c.triangle_inverse_rest_bases + 4 * size_t(t);
+    const float det = b[0] * b[3] - b[1] * b[2];
+    if (det == 0 || !std::isfinite(det))
+      return fail(JGS2_ERROR_RANGE, "triangleInverseRestBases for triangle %u must be invertible.", t);
+  }
+  for (uint32_t h = 0; h < NH; ++h)
+    if (!(c.hinge_rest_edge_lengths[h] > 0)) return fail(JGS2_ERROR_RANGE, "hingeRestEdgeLengths[%u] must be positive.", h);
+  if (!(c.young_modulus > 0) || !std::isfinite(c.young_modulus))
+    return fail(JGS2_ERROR_RANGE, "Cloth Young's modulus must be finite and positive.");
+  if (!(c.poisson_ratio >= 0 && c.poisson_ratio < 0.5))
+    return fail(JGS2_ERROR_RANGE, "Cloth Poisson ratio must be finite and in [0, 0.5).");
+  if (!(c.thickness > 0) || !std::isfinite(c.thickness))
+    return fail(JGS2_ERROR_RANGE, "Cloth thickness must be finite and positive.");
+  if (!(c.bending_stiffness >= 0) || !std::isfinite(c.bending_stiffness))
+    return fail(JGS2_ERROR_RANGE, "Cloth bending stiffness must be finite and nonnegative.");
+  return JGS2_OK;
+}
+
+// Vertex -> element CSR in device vertex order: the caller-order CSR
+// (buildIncidentElementCsr, cloth-layout.ts:183-207) with its rows moved to
+// their device vertex; each row stays ascending in element id.
+jgs2_status incident_csr(const uint32_t* idx, uint32_t width, uint32_t count, const std::vector<uint32_t>& vold,
+                         std::vector<uint32_t>& rows, std::vector<uint32_t>& elems) {
+  const uint32_t V = uint32_t(vold.size());
+  std::vector<uint32_t> crow(size_t(V) + 1), celem(size_t(count) * width);
+  jgs2_status st = jgs2_build_element_incidence(V, idx, width, count, crow.data(), celem.data());
+  if (st != JGS2_OK) return st;
+  rows.assign(size_t(V) + 1, 0);
+  elems.clear();
+  elems.reserve(celem.size());
+  for (uint32_t nv = 0; nv < V; ++nv) {
+    const uint32_t ov = vold[nv];
+    elems.insert(elems.end(), celem.begin() + crow[ov], celem.begin() + crow[ov + 1]);
+    rows[nv + 1] = uint32_t(elems.size());
+  }
+  return JGS2_OK;
+}
+
 // minimumJGS2InputDeformationDeterminant (layout.ts:549-588), f64.
 double min_input_determinant(const jgs2_scene_input& in) {
   auto det_cols = [](double c0x, double c0y, double c0z, double c1x, double c1y, double c1z, double c2x, double c2y,
@@ -362,7 +435,8 @@ double min_input_determinant(const jgs2_scene_input& in) {
 }
 
 // buildJGS2InputVertexColoring (jgs2-solver.ts:308-367): conflict groups of
-// every tetrahedron and every positive cubature record's {vertex} + tet.
+// every tetrahedron, every positive cubature record's {vertex} + tet, and
+// every cloth triangle and hinge.
 jgs2_status color_vertices(const jgs2_scene_input& in, std::vector<uint32_t>& colors, uint32_t& count) {
   const uint32_t V = in.vertex_count, T = in.tet_count, K = in.cubature_k;
   std::vector<uint32_t> offsets, members;
@@ -382,6 +456,16 @@ jgs2_status color_vertices(const jgs2_scene_input& in, std::vector<uint32_t>& co
       for (int l = 0; l < 4; ++l) members.push_back(in.tet_indices[4 * size_t(t) + l]);
       offsets.push_back(uint32_t(members.size()));
     }
+  if (in.cloth) {  // every cloth triangle and hinge is a conflict group
+    for (uint32_t t = 0; t < in.cloth->triangle_count; ++t) {
+      for (int l = 0; l < 3; ++l) members.push_back(in.cloth->triangle_indices[3 * size_t(t) + l]);
+      offsets.push_back(uint32_t(members.size()));
+    }
+    for (uint32_t h = 0; h < in.cloth->hinge_count; ++h) {
+      for (int l = 0; l < 4; ++l) members.push_back(in.cloth->hinge_indices[4 * size_t(h) + l]);
+      offsets.push_back(uint32_t(members.size()));
+    }
+  }
   colors.resize(V);
   return jgs2_greedy_vertex_coloring(V, offsets.data(), uint32_t(offsets.size() - 1), members.data(), colors.data(),
                                      &count);
@@ -465,6 +549,8 @@ PassParams make_params(const jgs2_solver* s, const jgs2_settings& st, bool stop)
   p.globalization = s->globalization ? 1u : 0u;
   p.stop_enabled = (s->globalization && stop) ? 1u : 0u;
   p.corotated = s->corotated ? 1u : 0u;
+  p.tri_count = s->dev.NT;
+  p.hinge_count = s->dev.NH;
   // packUniforms (jgs2-solver.ts:604-730): f64 expressions rounded to f32.
   const double inv = 1.0 / st.timestep;
   p.dt = round_f32(st.timestep);
@@ -537,6 +623,11 @@ void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations,
       PassParams ps = p;
       ps.fused_rot = fused ? 1u : 0u;
       record_solve(s, ps);
+      if (s->globalization && (d.NT || d.NH)) {
+        PassParams pe = p;
+        pe.energy_after_sweep = 1;
+        launch_vertex_energy(d, pe, q);
+      }
     }
     if (s->globalization) {
       launch_reduce_candidate(d, p, q);
@@ -735,6 +826,13 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
   const uint32_t V = in.vertex_count, T = in.tet_count, K = in.cubature_k;
   const bool corotated = in.tet_meta[3] == JGS2_MATERIAL_COROTATED_LINEAR;
   const bool globalization = !corotated;
+  const uint32_t NT = in.cloth ? in.cloth->triangle_count : 0u, NH = in.cloth ? in.cloth->hinge_count : 0u;
+  if (in.cloth) {
+    st = validate_cloth(*in.cloth, V);
+    if (st != JGS2_OK) return st;
+    if (NT > 0 && !globalization)  // jgs2-solver.ts:998-1001
+      return fail(JGS2_ERROR_RANGE, "Triangle cloth requires the stable Neo-Hookean production path.");
+  }
   std::vector<uint32_t> colors;
   uint32_t color_count = 0;
   st = color_vertices(in, colors, color_count);
@@ -927,6 +1025,42 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
     rot_t[3 * size_t(t) + 1] = make_float4(0, 1, 0, 0);
     rot_t[3 * size_t(t) + 2] = make_float4(0, 0, 1, 0);
   }
+  // Cloth records in device vertex ids; elements keep caller order.
+  std::vector<uint4> tri_ix(NT), hinge_ix(NH);
+  std::vector<float4> tri_inv(NT), hinge_rest(NH);
+  std::vector<float> tri_area(NT);
+  std::vector<uint32_t> tri_rows, tri_elems, hinge_rows, hinge_elems;
+  if (in.cloth) {
+    const jgs2_cloth_input& c = *in.cloth;
+    for (uint32_t t = 0; t < NT; ++t) {
+      const uint32_t* ix = &c.triangle_indices[3 * size_t(t)];
+      const uint32_t owner = std::min(ix[0], std::min(ix[1], ix[2]));  // S:1838-1840
+      tri_ix[t] = make_uint4(vnew[ix[0]], vnew[ix[1]], vnew[ix[2]], vnew[owner]);
+      const float* b = &c.triangle_inverse_rest_bases[4 * size_t(t)];
+      tri_inv[t] = make_float4(b[0], b[1], b[2], b[3]);
+      tri_area[t] = c.triangle_rest_areas[t];
+    }
+    for (uint32_t h = 0; h < NH; ++h) {
+      const uint32_t* ix = &c.hinge_indices[4 * size_t(h)];
+      const uint32_t owner = std::min(std::min(ix[0], ix[1]), std::min(ix[2], ix[3]));  // S:1842-1844
+      hinge_ix[h] = make_uint4(vnew[ix[0]], vnew[ix[1]], vnew[ix[2]], vnew[ix[3]]);
+      uint32_t ob = vnew[owner];
+      float of;
+      std::memcpy(&of, &ob, 4);
+      hinge_rest[h] = make_float4(c.hinge_rest_angles[h], c.hinge_rest_edge_lengths[h], of, 0.0f);
+    }
+    if (jgs2_status e = incident_csr(c.triangle_indices, 3, NT, vold, tri_rows, tri_elems)) return cleanup(e);
+    if (jgs2_status e = incident_csr(c.hinge_indices, 4, NH, vold, hinge_rows, hinge_elems)) return cleanup(e);
+    // packJGS2GpuClothArena material words (cloth-layout.ts:335-343): f64
+    // expressions rounded to f32.
+    const double E = c.young_modulus, nu = c.poisson_ratio;
+    d.cloth_lam = float(E * nu / (1 - nu * nu));
+    d.cloth_mu = float(E / (2 * (1 + nu)));
+    d.cloth_thick = float(c.thickness);
+    d.cloth_bend = float(c.bending_stiffness);
+    d.NT = NT;
+    d.NH = NH;
+  }
   TRY(dalloc(&d.vtx_rest, V));
   TRY(dalloc(&d.vtx_info, V));
   TRY(dalloc(&d.tet_idx, T));
@@ -948,12 +1082,23 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
   TRY(dalloc(&d.body_corr, s->bodies));
   TRY(dalloc(&d.final_update, V));
   TRY(dalloc(&d.local_diag, 4 * size_t(V)));
-  TRY(dalloc(&d.tet_glob, 2 * size_t(T)));
+  TRY(dalloc(&d.tet_glob, 2 * (size_t(T) + NT)));
+  if (in.cloth) {
+    TRY(dalloc(&d.tri_ix, NT));
+    TRY(dalloc(&d.tri_inv, NT));
+    TRY(dalloc(&d.tri_area, NT));
+    TRY(dalloc(&d.hinge_ix, NH));
+    TRY(dalloc(&d.hinge_rest, NH));
+    TRY(dalloc(&d.tri_rows, size_t(V) + 1));
+    TRY(dalloc(&d.tri_elems, tri_elems.size()));
+    TRY(dalloc(&d.hinge_rows, size_t(V) + 1));
+    TRY(dalloc(&d.hinge_elems, hinge_elems.size()));
+  }
   TRY(dalloc(&d.vertex_energy, V));
   TRY(dalloc(&d.conv, 5 * size_t(V)));
   TRY(dalloc(&d.control, 4));
   TRY(dalloc(&d.history, size_t(kHistoryCapacity) * 8));
-  d.partial_capacity = partial_blocks_needed(V, T);
+  d.partial_capacity = partial_blocks_needed(V, T + NT);
   TRY(dalloc(reinterpret_cast<char**>(&d.partials), partial_bytes(d.partial_capacity)));
   hipStream_t q = s->stream;
   TRY_HIP(hipMemcpyAsync(d.vtx_rest, vtx_rest.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
@@ -968,6 +1113,20 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
   TRY_HIP(hipMemcpyAsync(d.objectives, dev_objectives.data(), 32 * size_t(V), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.color_order, order.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.colors, dev_colors.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
+  if (in.cloth) {
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+      return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, q) : hipSuccess;
+    };
+    TRY_HIP(up(d.tri_ix, tri_ix.data(), NT * sizeof(uint4)));
+    TRY_HIP(up(d.tri_inv, tri_inv.data(), NT * sizeof(float4)));
+    TRY_HIP(up(d.tri_area, tri_area.data(), NT * sizeof(float)));
+    TRY_HIP(up(d.hinge_ix, hinge_ix.data(), NH * sizeof(uint4)));
+    TRY_HIP(up(d.hinge_rest, hinge_rest.data(), NH * sizeof(float4)));
+    TRY_HIP(up(d.tri_rows, tri_rows.data(), tri_rows.size() * 4));
+    TRY_HIP(up(d.tri_elems, tri_elems.data(), tri_elems.size() * 4));
+    TRY_HIP(up(d.hinge_rows, hinge_rows.data(), hinge_rows.size() * 4));
+    TRY_HIP(up(d.hinge_elems, hinge_elems.data(), hinge_elems.size() * 4));
+  }
   for (float4* region : {d.pos[0], d.pos[1], d.predicted, d.old})
     TRY_HIP(hipMemcpyAsync(region, positions.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.velocity, velocities.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
@@ -975,13 +1134,13 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
   TRY_HIP(hipMemcpyAsync(d.vrot_next, rot_v.data(), rot_v.size() * sizeof(float4), hipMemcpyHostToDevice, q));
   TRY_HIP(hipMemcpyAsync(d.trot, rot_t.data(), rot_t.size() * sizeof(float4), hipMemcpyHostToDevice, q));
   {
-    std::vector<float4> zeros(std::max<size_t>({size_t(5) * V, size_t(4) * V, size_t(2) * T, size_t(kHistoryCapacity) * 8,
+    std::vector<float4> zeros(std::max<size_t>({size_t(5) * V, size_t(4) * V, 2 * (size_t(T) + NT), size_t(kHistoryCapacity) * 8,
                                                 size_t(s->bodies)}),
                               make_float4(0, 0, 0, 0));
     TRY_HIP(hipMemcpyAsync(d.body_corr, zeros.data(), s->bodies * sizeof(float4), hipMemcpyHostToDevice, q));
     TRY_HIP(hipMemcpyAsync(d.final_update, zeros.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
     TRY_HIP(hipMemcpyAsync(d.local_diag, zeros.data(), 4 * size_t(V) * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.tet_glob, zeros.data(), 2 * size_t(T) * sizeof(float4), hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.tet_glob, zeros.data(), 2 * (size_t(T) + NT) * sizeof(float4), hipMemcpyHostToDevice, q));
     TRY_HIP(hipMemcpyAsync(d.vertex_energy, zeros.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
     TRY_HIP(hipMemcpyAsync(d.conv, zeros.data(), 5 * size_t(V) * sizeof(float4), hipMemcpyHostToDevice, q));
     TRY_HIP(hipMemcpyAsync(d.control, zeros.data(), 4 * sizeof(float4), hipMemcpyHostToDevice, q));

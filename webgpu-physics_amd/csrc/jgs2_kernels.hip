@@ -367,6 +367,190 @@ __device__ __forceinline__ float floor_delta(const PassParams& p, float by, floa
   float fp = fminf(0.0f, by + dy - p.floor_y);
   return 0.5f * p.floor_k * (fp - ip) * (fp + ip);
 }
+// ------------------------------------------------------------------ cloth (S:1225-1845)
+// StVK membrane triangles and quadratic dihedral hinges on the shared vertex
+// state. `pos` is read with vertex `ov` replaced by `op`, the restricted
+// trial's moved vertex (S:1520-1531); kNoVertex reads `pos` unchanged.
+constexpr uint32_t kNoVertex = 0xffffffffu;
+__device__ __forceinline__ f3 cloth_pos(const float4* pos, uint32_t v, uint32_t ov, f3 op) {
+  return v == ov ? op : load_pos(pos, v);
+}
+__device__ __forceinline__ float len3(f3 v) { return sqrtf(dot(v, v)); }
+__device__ __forceinline__ void tri_shape_gradient(float4 inv, uint32_t l, float& x, float& y) {  // S:1533-1544
+  if (l == 1) {
+    x = inv.x;
+    y = inv.y;
+  } else if (l == 2) {
+    x = inv.z;
+    y = inv.w;
+  } else {
+    x = -inv.x - inv.z;
+    y = -inv.y - inv.w;
+  }
+}
+__device__ __forceinline__ uint32_t tri_slot(uint4 ix, uint32_t v) {  // S:1505-1510
+  return ix.x == v ? 0u : (ix.y == v ? 1u : (ix.z == v ? 2u : 3u));
+}
+__device__ __forceinline__ uint32_t hinge_slot(uint4 ix, uint32_t v) {  // S:1512-1518
+  return ix.x == v ? 0u : (ix.y == v ? 1u : (ix.z == v ? 2u : (ix.w == v ? 3u : 4u)));
+}
+struct Membrane {
+  float energy;
+  f3 f0, f1, stress;  // deformation columns and the Green strain's StVK stress (xx, xy, yy)
+};
+__device__ Membrane membrane_at(const DeviceState& s, uint32_t tri, const float4* pos, uint32_t ov, f3 op) {  // S:1546-1604
+  const uint4 ix = s.tri_ix[tri];
+  const float4 inv = s.tri_inv[tri];
+  const f3 x0 = cloth_pos(pos, ix.x, ov, op);
+  const f3 e01 = cloth_pos(pos, ix.y, ov, op) - x0;
+  const f3 e02 = cloth_pos(pos, ix.z, ov, op) - x0;
+  Membrane m;
+  m.f0 = e01 * inv.x + e02 * inv.z;
+  m.f1 = e01 * inv.y + e02 * inv.w;
+  const float e00 = 0.5f * (dot(m.f0, m.f0) - 1.0f);
+  const float e01s = 0.5f * dot(m.f0, m.f1);
+  const float e11 = 0.5f * (dot(m.f1, m.f1) - 1.0f);
+  const float trace = e00 + e11;
+  const float mu = s.cloth_mu, lam = s.cloth_lam;
+  m.stress = mk(2.0f * mu * e00 + lam * trace, 2.0f * mu * e01s, 2.0f * mu * e11 + lam * trace);
+  const float density = mu * (e00 * e00 + 2.0f * e01s * e01s + e11 * e11) + 0.5f * lam * trace * trace;
+  m.energy = (s.tri_area[tri] * s.cloth_thick) * density;
+  return m;
+}
+__device__ __forceinline__ f3 membrane_gradient(const DeviceState& s, uint32_t tri, const Membrane& m, uint32_t slot) {
+  float sx, sy;
+  tri_shape_gradient(s.tri_inv[tri], slot, sx, sy);
+  const f3 p0 = m.f0 * m.stress.x + m.f1 * m.stress.y;
+  const f3 p1 = m.f0 * m.stress.y + m.f1 * m.stress.z;
+  return (s.tri_area[tri] * s.cloth_thick) * (p0 * sx + p1 * sy);
+}
+// Diagonal 3x3 block of the membrane Hessian for one corner, symmetrized
+// (S:1608-1658).
+__device__ m3 membrane_local_hessian(const DeviceState& s, uint32_t tri, const Membrane& ev, uint32_t slot) {
+  float sx, sy;
+  tri_shape_gradient(s.tri_inv[tri], slot, sx, sy);
+  const float aw = s.tri_area[tri] * s.cloth_thick;
+  const float mu = s.cloth_mu, lam = s.cloth_lam;
+  const float f0c[3] = {ev.f0.x, ev.f0.y, ev.f0.z}, f1c[3] = {ev.f1.x, ev.f1.y, ev.f1.z};
+  f3 col[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const f3 axis = mk(c == 0 ? 1.0f : 0.0f, c == 1 ? 1.0f : 0.0f, c == 2 ? 1.0f : 0.0f);
+    const f3 df0 = axis * sx, df1 = axis * sy;
+    const float de00 = sx * f0c[c];
+    const float de01 = 0.5f * (sx * f1c[c] + sy * f0c[c]);
+    const float de11 = sy * f1c[c];
+    const float dtr = de00 + de11;
+    const f3 ds = mk(2.0f * mu * de00 + lam * dtr, 2.0f * mu * de01, 2.0f * mu * de11 + lam * dtr);
+    const f3 dp0 = df0 * ev.stress.x + df1 * ev.stress.y + ev.f0 * ds.x + ev.f1 * ds.y;
+    const f3 dp1 = df0 * ev.stress.y + df1 * ev.stress.z + ev.f0 * ds.y + ev.f1 * ds.z;
+    col[c] = aw * (dp0 * sx + dp1 * sy);
+  }
+  const m3 r = {col[0], col[1], col[2]};
+  return mscale(0.5f, madd(r, tr(r)));
+}
+__device__ __forceinline__ f3 tri_raw_normal(const DeviceState& s, uint32_t tri, const float4* pos, uint32_t ov, f3 op) {  // S:1660-1686
+  const uint4 ix = s.tri_ix[tri];
+  const f3 x0 = cloth_pos(pos, ix.x, ov, op);
+  return cross(cloth_pos(pos, ix.y, ov, op) - x0, cloth_pos(pos, ix.z, ov, op) - x0);
+}
+// Signed dihedral angle of triangles (0,1,2) and (1,0,3) and the gradient
+// of one corner `slot` (4 = none). Validity covers every corner's gradient,
+// like the reference, which forms all twelve entries (S:1700-1810).
+struct Dihedral {
+  float angle;
+  bool valid;
+  f3 grad;
+};
+__device__ Dihedral dihedral_at(const DeviceState& s, uint32_t h, const float4* pos, uint32_t ov, f3 op, uint32_t slot) {
+  const uint4 ix = s.hinge_ix[h];
+  const f3 x0 = cloth_pos(pos, ix.x, ov, op), x1 = cloth_pos(pos, ix.y, ov, op);
+  const f3 x2 = cloth_pos(pos, ix.z, ov, op), x3 = cloth_pos(pos, ix.w, ov, op);
+  Dihedral out{0.0f, false, mk(0, 0, 0)};
+  const f3 edge = x1 - x0;
+  const float el = len3(edge);
+  const f3 to2 = x2 - x0, rev = -edge, to3 = x3 - x1;
+  const f3 rn0 = cross(edge, to2), rn1 = cross(rev, to3);
+  const float nl0 = len3(rn0), nl1 = len3(rn1);
+  const float es0 = fmaxf(el, fmaxf(len3(to2), len3(to2 - edge)));
+  const float es1 = fmaxf(el, fmaxf(len3(to3), len3(to3 - rev)));
+  const bool ok = finite_s(el) && el > 0.0f && finite_s(nl0) && finite_s(nl1) && nl0 > 1.0e-12f * es0 * es0 &&
+                  nl1 > 1.0e-12f * es1 * es1;
+  if (!ok) return out;
+  const f3 tangent = edge / el, n0 = rn0 / nl0, n1 = rn1 / nl1;
+  const f3 nc = cross(n0, n1);
+  const float sine = dot(tangent, nc), cosine = dot(n0, n1);
+  const float den = sine * sine + cosine * cosine;
+  if (!finite_s(den) || !(den > 0.0f)) return out;
+  bool fin = true;
+  f3 keep = mk(0, 0, 0);
+#pragma unroll 1
+  for (uint32_t dv = 0; dv < 4; ++dv) {
+    float gc[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const f3 u = mk(ax == 0 ? 1.0f : 0.0f, ax == 1 ? 1.0f : 0.0f, ax == 2 ? 1.0f : 0.0f);
+      const f3 z = mk(0, 0, 0);
+      const f3 d0 = dv == 0 ? u : z, d1 = dv == 1 ? u : z, d2 = dv == 2 ? u : z, d3 = dv == 3 ? u : z;
+      const f3 de = d1 - d0;
+      const f3 dt = (de - tangent * dot(tangent, de)) / el;
+      const f3 dto2 = d2 - d0, dto3 = d3 - d1;
+      const f3 dn0r = cross(de, to2) + cross(edge, dto2);
+      const f3 dn1r = cross(-de, to3) + cross(rev, dto3);
+      const f3 dn0 = (dn0r - n0 * dot(n0, dn0r)) / nl0;
+      const f3 dn1 = (dn1r - n1 * dot(n1, dn1r)) / nl1;
+      const float dcos = dot(dn0, n1) + dot(n0, dn1);
+      const float dsin = dot(dt, nc) + dot(tangent, cross(dn0, n1) + cross(n0, dn1));
+      gc[ax] = (cosine * dsin - sine * dcos) / den;
+    }
+    const f3 g = mk(gc[0], gc[1], gc[2]);
+    fin = fin && finite3(g);
+    if (dv == slot) keep = g;
+  }
+  const float angle = atan2f(sine, cosine);
+  if (!finite_s(angle) || !fin) return out;
+  out.angle = angle;
+  out.valid = true;
+  out.grad = keep;
+  return out;
+}
+__device__ __forceinline__ float wrapped_angle_difference(float a, float r) {  // S:1812-1815
+  const float d = a - r;
+  return atan2f(sinf(d), cosf(d));
+}
+__device__ __forceinline__ float bending_energy_at(const DeviceState& s, uint32_t h, const float4* pos, uint32_t ov, f3 op) {  // S:1817-1836
+  const Dihedral ev = dihedral_at(s, h, pos, ov, op, 4u);
+  if (!ev.valid) return kF32Max;
+  const float4 r = s.hinge_rest[h];
+  const float ad = wrapped_angle_difference(ev.angle, r.x);
+  const float k = s.cloth_bend * r.y;
+  return 0.5f * k * ad * ad;
+}
+// Membrane and bending contributions to one vertex's force (S:4028-4085).
+__device__ __forceinline__ f3 cloth_gradient(const DeviceState& s, const float4* pos, uint32_t v) {
+  f3 g = mk(0, 0, 0);
+  for (uint32_t k = s.tri_rows[v]; k < s.tri_rows[v + 1]; ++k) {
+    const uint32_t tri = s.tri_elems[k];
+    const uint32_t slot = tri_slot(s.tri_ix[tri], v);
+    if (slot < 3) g = g + membrane_gradient(s, tri, membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)), slot);
+  }
+  for (uint32_t k = s.hinge_rows[v]; k < s.hinge_rows[v + 1]; ++k) {
+    const uint32_t h = s.hinge_elems[k];
+    const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
+    if (slot >= 4) continue;
+    const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+    if (!d.valid) {
+      g = g + mk(kF32Max, kF32Max, kF32Max);
+    } else {
+      const float4 r = s.hinge_rest[h];
+      const float ad = wrapped_angle_difference(d.angle, r.x);
+      g = g + (s.cloth_bend * r.y * ad) * d.grad;
+    }
+  }
+  return g;
+}
+
+template <bool kCloth>
 __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const PassParams& p, const float4* pos, uint32_t v) {  // S:2557-2645
   bool pin = s.vtx_info[v].z != 0u;
   f3 x = load_pos(pos, v);
@@ -390,11 +574,28 @@ __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const Pa
     float pen = fminf(0.0f, x.y - p.floor_y);
     e += 0.5f * p.floor_k * pen * pen;
   }
+  if constexpr (kCloth) {
+    // Cloth elements are summed once, by their owner vertex (S:2585-2643).
+    for (uint32_t k = s.tri_rows[v]; k < s.tri_rows[v + 1]; ++k) {
+      const uint32_t tri = s.tri_elems[k];
+      if (s.tri_ix[tri].w != v) continue;
+      const float tra = 2.0f * s.tri_area[tri];
+      const float stretch = len3(tri_raw_normal(s, tri, pos, kNoVertex, mk(0, 0, 0))) / tra;
+      if (!(tra > 0.0f) || !finite_s(stretch) || !(stretch > kDeterminantFloor)) return kF32Max;
+      e += membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy;
+    }
+    for (uint32_t k = s.hinge_rows[v]; k < s.hinge_rows[v + 1]; ++k) {
+      const uint32_t h = s.hinge_elems[k];
+      if (__float_as_uint(s.hinge_rest[h].z) != v) continue;
+      e += bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
+    }
+  }
   return e;
 }
+template <bool kCloth>
 __device__ __forceinline__ void store_vertex_energy(const DeviceState& s, const PassParams& p, uint32_t v) {  // S:2647-2658
-  float se = vertex_energy_at(s, p, s.pos[p.src], v);
-  float ce = vertex_energy_at(s, p, s.pos[p.tgt], v);
+  float se = vertex_energy_at<kCloth>(s, p, s.pos[p.src], v);
+  float ce = vertex_energy_at<kCloth>(s, p, s.pos[p.tgt], v);
   bool sv = finite_s(se), cv = finite_s(ce);
   s.vertex_energy[v] = make_float4(sv ? se : 0.0f, cv ? ce : 0.0f, sv ? 1.0f : 0.0f, cv ? 1.0f : 0.0f);
 }
@@ -600,7 +801,7 @@ __device__ void cubature_record_corot(const DeviceState& s, const PassParams& p,
 
 // Gather of jgs2Solve (S:3074-3263): inertia, objectives, the exact
 // incident elements, the floor penalty and the cubature correction.
-template <bool kCorot>
+template <bool kCorot, bool kCloth>
 __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_t v) {
   const float4* pos = s.pos[p.src];
   const f3 x = load_pos(pos, v);
@@ -624,6 +825,33 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
       const m3 R = load_rot(s.trot, t);
       L.g = L.g + tg[slot];
       L.H = madd(L.H, mul(mul(R, stiffness_block(s, t, slot, slot)), tr(R)));
+    }
+  }
+  if constexpr (kCloth) {
+#pragma unroll 1
+    for (uint32_t k = s.tri_rows[v]; k < s.tri_rows[v + 1]; ++k) {  // S:3117-3144
+      const uint32_t tri = s.tri_elems[k];
+      const uint32_t slot = tri_slot(s.tri_ix[tri], v);
+      if (slot >= 3) continue;
+      const Membrane m = membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0));
+      L.g = L.g + membrane_gradient(s, tri, m, slot);
+      L.H = madd(L.H, membrane_local_hessian(s, tri, m, slot));
+    }
+#pragma unroll 1
+    for (uint32_t k = s.hinge_rows[v]; k < s.hinge_rows[v + 1]; ++k) {  // S:3145-3184
+      const uint32_t h = s.hinge_elems[k];
+      const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
+      if (slot >= 4) continue;
+      const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+      if (!d.valid) {
+        L.g = L.g + mk(kF32Max, kF32Max, kF32Max);
+      } else {
+        const float4 r = s.hinge_rest[h];
+        const float ad = wrapped_angle_difference(d.angle, r.x);
+        const float k2 = s.cloth_bend * r.y;
+        L.g = L.g + (k2 * ad) * d.grad;
+        L.H = madd(L.H, m3{(k2 * d.grad.x) * d.grad, (k2 * d.grad.y) * d.grad, (k2 * d.grad.z) * d.grad});
+      }
     }
   }
   if (p.floor_k > 0.0f && x.y < p.floor_y) {  // S:3188-3191
@@ -654,6 +882,7 @@ struct Trial {
   bool geometry_valid;
   float energy;
 };
+template <bool kCloth>
 __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uint32_t v, f3 x, f3 d, float initial_min) {
   const float4* pos = s.pos[p.src];
   const uint4 inf = s.vtx_info[v];
@@ -682,6 +911,34 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
     const float id = 2.0f * dot(d, mul(F, gr)) + dot(d, d) * dot(gr, gr);
     const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
     energy += e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
+  }
+  if constexpr (kCloth) {
+    const f3 tp = x + d;
+#pragma unroll 1
+    for (uint32_t k = s.tri_rows[v]; k < s.tri_rows[v + 1]; ++k) {
+      const uint32_t tri = s.tri_elems[k];
+      if (tri_slot(s.tri_ix[tri], v) >= 3) continue;
+      // Signed relative area stretch (S:2062-2116).
+      const f3 sn = tri_raw_normal(s, tri, pos, kNoVertex, mk(0, 0, 0));
+      const f3 tn = tri_raw_normal(s, tri, pos, v, tp);
+      const float tra = 2.0f * s.tri_area[tri];
+      const float ss = len3(sn) / tra;
+      const float tm = len3(tn) / tra;
+      const float ts = dot(sn, tn) > 0.0f ? tm : -tm;
+      if (!(tra > 0.0f) || !finite_s(ss) || !finite_s(ts)) {
+        tr.geometry_valid = false;
+        return tr;
+      }
+      tr.min_det = fminf(tr.min_det, fminf(ss, ts));
+      // Membrane energy delta (S:2203-2236).
+      energy += membrane_at(s, tri, pos, v, tp).energy - membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy;
+    }
+#pragma unroll 1
+    for (uint32_t k = s.hinge_rows[v]; k < s.hinge_rows[v + 1]; ++k) {  // S:2237-2271
+      const uint32_t h = s.hinge_elems[k];
+      if (hinge_slot(s.hinge_ix[h], v) >= 4) continue;
+      energy += bending_energy_at(s, h, pos, v, tp) - bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
+    }
   }
   if (p.cubature_k > 0) {
     const f3 ds = tmul(load_rot(rot_source(s, p), v), d);  // R_s^T d
@@ -839,8 +1096,11 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassPara
 
 // jgs2Solve (S:2992-3487), one thread per vertex (Jacobi) or per vertex of
 // the active color (graph-colored Gauss-Seidel, in place).
-template <bool kCorot>
+// With cloth the assembled vertex energy reads neighbor positions, so it is
+// evaluated by a separate pass after the sweep (k_vertex_energy).
+template <bool kCorot, bool kCloth>
 __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
+  constexpr bool kEnergyInSolve = !kCloth;
   const uint32_t lane = logical_block() * kBlock + threadIdx.x;
   uint32_t v;
   if (p.colored) {
@@ -869,7 +1129,7 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
     s.final_update[v] = make_float4(0, 0, 0, 1);
     if (p.globalization) {
       store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, kPinned);
-      if (!p.colored) store_vertex_energy(s, p, v);
+      if (kEnergyInSolve && !p.colored) store_vertex_energy<false>(s, p, v);
     }
     return;
   }
@@ -885,11 +1145,11 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
       s.final_update[v] = make_float4(0, 0, 0, 1);
       store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, amv ? pre.z : 0.0f, 0, 0,
                        amv ? kLineSearchFailed : kNonfinite);
-      if (!p.colored) store_vertex_energy(s, p, v);
+      if (kEnergyInSolve && !p.colored) store_vertex_energy<false>(s, p, v);
       return;
     }
   }
-  const Local L = gather_local<kCorot>(s, p, v);
+  const Local L = gather_local<kCorot, kCloth>(s, p, v);
   const float inertia = s.vtx_rest[v].w * p.inv_dt2;
   if constexpr (kCorot) {  // legacy regularized solve (S:2479-2512, S:3475-3486)
     const m3& m = L.H;
@@ -968,7 +1228,7 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
         backtracks = b;
         break;
       }
-      const Trial trial = restricted_trial(s, p, v, x, td, initial_min);
+      const Trial trial = restricted_trial<kCloth>(s, p, v, x, td, initial_min);
       if (!trial.geometry_valid) {
         status = kNonfinite;
         min_trial = 0.0f;
@@ -1003,13 +1263,18 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
   s.final_update[v] = make_float4(scaled_length3(accepted_d), 0, 0, 1);
   store_local_diag(s, v, dr.dir, alpha, dr.min_eig * dr.scale, dr.scale, dr.shift * dr.scale, dr.shift, accepted_de,
                    bound, dr.gdotp, dr.residual, min_trial, backtracks, evaluations, status);
-  if (!p.colored) store_vertex_energy(s, p, v);
+  if (kEnergyInSolve && !p.colored) store_vertex_energy<false>(s, p, v);
 }
 
-__global__ void __launch_bounds__(kBlock) k_vertex_energy(DeviceState s, PassParams p) {  // S:3490-3496
+// assembledVertexEnergy (S:3490-3496). After a Jacobi sweep with cloth it
+// stands in for the in-solve evaluation and, like the solve, leaves the
+// records alone once the stop latch is set (`energy_after_sweep`).
+template <bool kCloth>
+__global__ void __launch_bounds__(kBlock) k_vertex_energy(DeviceState s, PassParams p) {
   const uint32_t v = logical_block() * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
-  store_vertex_energy(s, p, v);
+  if (p.energy_after_sweep && stop_latched(s, p)) return;
+  store_vertex_energy<kCloth>(s, p, v);
 }
 
 // ------------------------------------------------------------------ candidate reduction
@@ -1089,9 +1354,24 @@ __global__ void __launch_bounds__(kBlock) k_candidate_partials(DeviceState s, Pa
   const bool latched = stop_latched(s, p);
   if (b < tet_blocks) {
     const uint32_t t = b * kBlock + threadIdx.x;
-    if (t < p.tet_count) {
+    if (t < p.tet_count + p.tri_count) {
       float4 geo, en;
-      if (!latched) {
+      if (!latched && t >= p.tet_count) {
+        // Cloth triangles: signed relative area stretch (S:3508-3549).
+        const uint32_t tri = t - p.tet_count;
+        const f3 sn = tri_raw_normal(s, tri, s.pos[p.src], kNoVertex, mk(0, 0, 0));
+        const f3 cn = tri_raw_normal(s, tri, s.pos[p.tgt], kNoVertex, mk(0, 0, 0));
+        const float tra = 2.0f * s.tri_area[tri];
+        const float ss = len3(sn) / tra;
+        const float cm = len3(cn) / tra;
+        const float cs = dot(sn, cn) > 0.0f ? cm : -cm;
+        const bool sgv = tra > 0.0f && finite3(sn) && finite_s(ss);
+        const bool cgv = tra > 0.0f && finite3(cn) && finite_s(cs);
+        geo = make_float4(sgv ? ss : 0.0f, cgv ? cs : 0.0f, sgv ? 1.0f : 0.0f, cgv ? 1.0f : 0.0f);
+        en = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
+        s.tet_glob[2 * size_t(t)] = geo;
+        s.tet_glob[2 * size_t(t) + 1] = en;
+      } else if (!latched) {
         const Tet e = load_tet(s, t);
         const m3 sf = deformation(e, s.pos[p.src]);
         const m3 cf = deformation(e, s.pos[p.tgt]);
@@ -1244,6 +1524,7 @@ __device__ __forceinline__ float max_abs(f3 v) { return fmaxf(fabsf(v.x), fmaxf(
 // (S:4119-4189): component gradients at the candidate and their scales.
 // With `compute_next_rot` the same gather also evaluates the next sweep's
 // vertexPolarRotation (S:2960-2989): its source pose is this candidate.
+template <bool kCloth>
 __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, PassParams p) {
   __shared__ ConvPartial sh[kBlock];
   ConvPartial r;
@@ -1275,6 +1556,9 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
           const SnhPoint sp = snh_point(F, e.lam, e.mu);
           mat = mat + e.vol * piola_times(sp, shape_gradient(e, slot));
         }
+      }
+      if constexpr (kCloth) {
+        if (!pinned) mat = mat + cloth_gradient(s, pos, v);
       }
       if (p.compute_next_rot) {
         m3 R = mid();
@@ -1527,7 +1811,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(DeviceState s, PassParams p
   s.velocity[v] = make_float4(vel.x, vel.y, vel.z, 0.0f);
 }
 
-template <bool kCorot>
+template <bool kCorot, bool kCloth>
 __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, PassParams p) {
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
@@ -1538,7 +1822,7 @@ __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, P
     for (int k = 0; k < 9; ++k) h[k] = 0.0f;
     return;
   }
-  const Local L = gather_local<kCorot>(s, p, v);
+  const Local L = gather_local<kCorot, kCloth>(s, p, v);
   g[0] = L.g.x;
   g[1] = L.g.y;
   g[2] = L.g.z;
@@ -1551,13 +1835,14 @@ __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, P
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+inline bool has_cloth(const PassParams& p) { return p.tri_count != 0u || p.hinge_count != 0u; }
 
 }  // namespace
 
-uint32_t partial_blocks_needed(uint32_t V, uint32_t T) {
-  // Candidate partials use blocks(T) + blocks(V); convergence uses
-  // 2 * blocks(V) + 1.
-  uint32_t a = blocks_for(T) + blocks_for(V);
+uint32_t partial_blocks_needed(uint32_t V, uint32_t elements) {
+  // Candidate partials use blocks(T + triangles) + blocks(V); convergence
+  // uses 2 * blocks(V) + 1.
+  uint32_t a = blocks_for(elements) + blocks_for(V);
   uint32_t b = 2 * blocks_for(V) + 1;
   return a > b ? a : b;
 }
@@ -1582,15 +1867,20 @@ void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t n = p.colored ? p.color_end - p.color_begin : p.vertex_count;
   if (n == 0) return;
   if (p.corotated)
-    hipLaunchKernelGGL(k_solve<true>, dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
+    hipLaunchKernelGGL((k_solve<true, false>), dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
+  else if (has_cloth(p))
+    hipLaunchKernelGGL((k_solve<false, true>), dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
   else
-    hipLaunchKernelGGL(k_solve<false>, dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
+    hipLaunchKernelGGL((k_solve<false, false>), dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
 }
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
-  hipLaunchKernelGGL(k_vertex_energy, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+  if (has_cloth(p))
+    hipLaunchKernelGGL(k_vertex_energy<true>, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+  else
+    hipLaunchKernelGGL(k_vertex_energy<false>, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
 }
 void launch_reduce_candidate(const DeviceState& s, const PassParams& p, hipStream_t st) {
-  const uint32_t tb = blocks_for(p.tet_count), vb = blocks_for(p.vertex_count);
+  const uint32_t tb = blocks_for(p.tet_count + p.tri_count), vb = blocks_for(p.vertex_count);
   hipLaunchKernelGGL(k_candidate_partials, dim3(tb + vb), dim3(kBlock), 0, st, s, p, tb);
   hipLaunchKernelGGL(k_candidate_final, dim3(1), dim3(kBlock), 0, st, s, p, tb + vb);
 }
@@ -1599,7 +1889,10 @@ void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream
 }
 void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t vb = blocks_for(p.vertex_count);
-  hipLaunchKernelGGL(k_convergence_gradient, dim3(vb), dim3(kBlock), 0, st, s, p);
+  if (has_cloth(p))
+    hipLaunchKernelGGL(k_convergence_gradient<true>, dim3(vb), dim3(kBlock), 0, st, s, p);
+  else
+    hipLaunchKernelGGL(k_convergence_gradient<false>, dim3(vb), dim3(kBlock), 0, st, s, p);
   hipLaunchKernelGGL(k_convergence_scales, dim3(1), dim3(kBlock), 0, st, s, vb);
   hipLaunchKernelGGL(k_convergence_sums, dim3(vb), dim3(kBlock), 0, st, s, p, vb);
   hipLaunchKernelGGL(k_convergence_final, dim3(1), dim3(kBlock), 0, st, s, p, vb);
@@ -1616,10 +1909,13 @@ void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t st) 
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
 }
 void launch_debug_local_systems(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  const dim3 grid(blocks_for(p.vertex_count));
   if (p.corotated)
-    hipLaunchKernelGGL(k_debug_local_systems<true>, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+    hipLaunchKernelGGL((k_debug_local_systems<true, false>), grid, dim3(kBlock), 0, st, s, p);
+  else if (has_cloth(p))
+    hipLaunchKernelGGL((k_debug_local_systems<false, true>), grid, dim3(kBlock), 0, st, s, p);
   else
-    hipLaunchKernelGGL(k_debug_local_systems<false>, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+    hipLaunchKernelGGL((k_debug_local_systems<false, false>), grid, dim3(kBlock), 0, st, s, p);
 }
 
 }  // namespace jgs2

@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <cstring>
 #include <new>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "jgs2_internal.h"
@@ -21,6 +23,11 @@ struct jgs2_scene {
   std::vector<uint32_t> cubature_tet_ids;
   std::vector<float> cubature_weights, cubature_basis;
   std::vector<float> forces, targets, stiffnesses;
+  // Optional cloth layer.
+  bool has_cloth = false;
+  std::vector<uint32_t> cloth_triangles, cloth_hinges;
+  std::vector<float> cloth_inverse, cloth_areas, hinge_angles, hinge_lengths;
+  jgs2_cloth_input cloth{};
 };
 
 namespace {
@@ -139,6 +146,27 @@ jgs2_status jgs2_build_adjacency(uint32_t vertex_count, uint32_t tet_count, cons
   std::vector<uint32_t> cursor(starts, starts + vertex_count);
   for (uint32_t t = 0; t < tet_count; ++t)
     for (int l = 0; l < 4; ++l) tets[cursor[tet_indices[4 * size_t(t) + l]]++] = t;
+  jgs2::clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_build_element_incidence(uint32_t vertex_count, const uint32_t* element_indices, uint32_t width,
+                                         uint32_t element_count, uint32_t* rows, uint32_t* elements) {
+  if (!rows || (element_count && (!element_indices || !elements)))
+    return jgs2::fail(JGS2_ERROR_TYPE, "NULL incidence argument.");
+  if (width != 3 && width != 4) return jgs2::fail(JGS2_ERROR_RANGE, "Element width must be 3 or 4; got %u.", width);
+  const size_t n = size_t(element_count) * width;
+  std::fill(rows, rows + size_t(vertex_count) + 1, 0u);
+  for (size_t i = 0; i < n; ++i) {
+    if (element_indices[i] >= vertex_count)
+      return jgs2::fail(JGS2_ERROR_RANGE, "Element index %u is outside vertexCount %u.", element_indices[i],
+                        vertex_count);
+    rows[element_indices[i] + 1] += 1;
+  }
+  for (uint32_t v = 0; v < vertex_count; ++v) rows[v + 1] += rows[v];
+  std::vector<uint32_t> cursor(rows, rows + vertex_count);
+  for (uint32_t e = 0; e < element_count; ++e)
+    for (uint32_t l = 0; l < width; ++l) elements[cursor[element_indices[size_t(width) * e + l]]++] = e;
   jgs2::clear_error();
   return JGS2_OK;
 }
@@ -406,6 +434,231 @@ jgs2_status jgs2_scene_build_cuboid(const jgs2_cuboid_desc* d, jgs2_scene** out)
   return JGS2_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// precomputeStVKTriangleRest (src/simulation/cpu/cloth.ts:115-140).
+bool triangle_rest(const double* x0, const double* x1, const double* x2, double inv[4], double* area) {
+  double e01[3], e02[3];
+  for (int a = 0; a < 3; ++a) {
+    e01[a] = x1[a] - x0[a];
+    e02[a] = x2[a] - x0[a];
+  }
+  double c[3] = {e01[1] * e02[2] - e01[2] * e02[1], e01[2] * e02[0] - e01[0] * e02[2], e01[0] * e02[1] - e01[1] * e02[0]};
+  double twice = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  double len = std::sqrt(e01[0] * e01[0] + e01[1] * e01[1] + e01[2] * e01[2]);
+  if (!(twice > 1e-12) || !(len > 0)) return false;
+  double t[3], n[3];
+  for (int a = 0; a < 3; ++a) {
+    t[a] = e01[a] / len;
+    n[a] = c[a] / twice;
+  }
+  double b[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
+  double mx = e02[0] * t[0] + e02[1] * t[1] + e02[2] * t[2];
+  double my = e02[0] * b[0] + e02[1] * b[1] + e02[2] * b[2];
+  inv[0] = 1.0 / len;
+  inv[1] = -mx / (len * my);
+  inv[2] = 0.0;
+  inv[3] = 1.0 / my;
+  *area = 0.5 * twice;
+  return true;
+}
+
+// Signed dihedral angle atan2(t.(n0 x n1), n0.n1) of the oriented triangles
+// (0,1,2) and (1,0,3) (src/simulation/cpu/cloth.ts:258-340), with edge length.
+bool hinge_rest(const double* x0, const double* x1, const double* x2, const double* x3, double* angle, double* edge_len) {
+  double e[3], to2[3], to3[3];
+  for (int a = 0; a < 3; ++a) {
+    e[a] = x1[a] - x0[a];
+    to2[a] = x2[a] - x0[a];
+    to3[a] = x3[a] - x1[a];
+  }
+  auto cross = [](const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  auto norm = [](const double* u) { return std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]); };
+  double rev[3] = {-e[0], -e[1], -e[2]}, n0[3], n1[3], nc[3];
+  cross(e, to2, n0);
+  cross(rev, to3, n1);
+  double l = norm(e), l0 = norm(n0), l1 = norm(n1);
+  if (!(l > 0) || !(l0 > 0) || !(l1 > 0)) return false;
+  for (int a = 0; a < 3; ++a) {
+    n0[a] /= l0;
+    n1[a] /= l1;
+  }
+  cross(n0, n1, nc);
+  double sine = (e[0] * nc[0] + e[1] * nc[1] + e[2] * nc[2]) / l;
+  double cosine = n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2];
+  *angle = std::atan2(sine, cosine);
+  *edge_len = l;
+  return true;
+}
+
+}  // namespace
+
+extern "C" jgs2_status jgs2_scene_build_cloth_drape(const jgs2_cloth_desc* d, jgs2_scene** out) {
+  if (!d || !out) return jgs2::fail(JGS2_ERROR_TYPE, "NULL cloth description.");
+  *out = nullptr;
+  if (d->cells < 1 || d->cells > 4096) return jgs2::fail(JGS2_ERROR_RANGE, "cloth cells must lie in [1, 4096].");
+  if (!(d->size[0] > 0) || !(d->size[1] > 0) || !(d->density > 0) || !(d->young_modulus > 0) || !(d->thickness > 0))
+    return jgs2::fail(JGS2_ERROR_RANGE, "Cloth density, youngModulus and thickness must be finite and positive.");
+  if (!(d->poisson_ratio >= 0 && d->poisson_ratio < 0.5))
+    return jgs2::fail(JGS2_ERROR_RANGE, "Cloth poissonRatio must be in [0, 0.5).");
+  if (!(d->bending_stiffness >= 0)) return jgs2::fail(JGS2_ERROR_RANGE, "Cloth bendingStiffness must be finite and nonnegative.");
+  if (d->fold_column > d->cells || d->fold_column == 1)
+    return jgs2::fail(JGS2_ERROR_RANGE, "Cloth fold_column must be 0 or in [2, cells].");
+  // The collider: a fixed one-cell stable Neo-Hookean block
+  // (CLOTH_COLLIDER_MATERIAL, src/scenes/index.ts:227-234).
+  jgs2_cuboid_desc c{};
+  c.cells[0] = c.cells[1] = c.cells[2] = 1;
+  for (int a = 0; a < 3; ++a) {
+    c.origin[a] = d->collider_origin[a];
+    c.size[a] = d->collider_size[a];
+  }
+  c.pin_axis = 3;
+  c.material_model = JGS2_MATERIAL_STABLE_NEO_HOOKEAN;
+  c.density = 1000;
+  c.poisson_ratio = 0.34;
+  c.young_modulus[0] = c.young_modulus[1] = 110000;
+  c.cubature_k = d->cubature_k;
+  jgs2_scene* s = nullptr;
+  jgs2_status st = jgs2_scene_build_cuboid(&c, &s);
+  if (st != JGS2_OK) return st;
+  const uint32_t n = d->cells, VC = s->vertex_count, side = n + 1;
+  const uint32_t V = VC + side * side;
+  s->vertex_count = V;
+  // The collider is fixed: every block vertex pinned, no cubature records.
+  for (uint32_t v = 0; v < VC; ++v) s->vertex_info[4 * size_t(v) + 2] = 1;
+  std::fill(s->cubature_tet_ids.begin(), s->cubature_tet_ids.end(), 0xffffffffu);
+  std::fill(s->cubature_weights.begin(), s->cubature_weights.end(), 0.0f);
+  std::fill(s->cubature_basis.begin(), s->cubature_basis.end(), 0.0f);
+  const size_t K = s->cubature_k;
+  s->cubature_tet_ids.resize(size_t(V) * K, 0xffffffffu);
+  s->cubature_weights.resize(size_t(V) * K, 0.0f);
+  s->cubature_basis.resize(size_t(V) * K * 36, 0.0f);
+  std::vector<double> x(3 * size_t(side) * side);
+  for (uint32_t z = 0; z <= n; ++z)
+    for (uint32_t xx = 0; xx <= n; ++xx) {
+      const size_t i = size_t(xx) + size_t(z) * side;
+      const bool folded = d->fold_column != 0 && xx >= d->fold_column;
+      const double column = folded ? 2.0 * double(d->fold_column - 1) - double(xx) : double(xx);
+      x[3 * i] = d->origin[0] + d->size[0] * column / double(n);
+      x[3 * i + 1] = d->origin[1] + (folded ? d->fold_lift : 0.0);
+      x[3 * i + 2] = d->origin[2] + d->size[1] * double(z) / double(n);
+    }
+  // Sheet vertices: pinned at (0, 0) and (n/2, 0) of the first row, body 1.
+  const uint32_t adjacency_end = uint32_t(s->adjacency.size());
+  for (uint32_t i = 0; i < side * side; ++i) {
+    const uint32_t xx = i % side, z = i / side;
+    const bool pinned = z == 0 && (xx == 0 || xx == n / 2);
+    for (int a = 0; a < 3; ++a) {
+      s->positions.push_back(float(x[3 * size_t(i) + a]));
+      s->vertex_rest.push_back(float(x[3 * size_t(i) + a]));
+    }
+    s->positions.push_back(1.0f);
+    s->vertex_rest.push_back(0.0f);  // mass added from triangles below
+    s->vertex_info.push_back(adjacency_end);
+    s->vertex_info.push_back(0);
+    s->vertex_info.push_back(pinned ? 1u : 0u);
+    s->vertex_info.push_back(1);
+  }
+  // Triangles and cloth masses (precompute.ts:46-93).
+  std::vector<double> mass(side * side, 0.0);
+  for (uint32_t z = 0; z < n; ++z)
+    for (uint32_t xx = 0; xx < n; ++xx) {
+      const uint32_t i00 = xx + z * side, i10 = xx + 1 + z * side, i01 = xx + (z + 1) * side, i11 = xx + 1 + (z + 1) * side;
+      const uint32_t tris[6] = {i00, i01, i10, i10, i01, i11};
+      for (int k = 0; k < 2; ++k) {
+        const uint32_t a = tris[3 * k], b = tris[3 * k + 1], cc = tris[3 * k + 2];
+        double inv[4], area;
+        if (!triangle_rest(&x[3 * size_t(a)], &x[3 * size_t(b)], &x[3 * size_t(cc)], inv, &area)) {
+          delete s;
+          return jgs2::fail(JGS2_ERROR_RANGE, "Cloth triangle must be nondegenerate.");
+        }
+        s->cloth_triangles.push_back(VC + a);
+        s->cloth_triangles.push_back(VC + b);
+        s->cloth_triangles.push_back(VC + cc);
+        for (int q = 0; q < 4; ++q) s->cloth_inverse.push_back(float(inv[q]));
+        s->cloth_areas.push_back(float(area));
+        const double vm = d->density * d->thickness * area / 3.0;
+        mass[a] += vm;
+        mass[b] += vm;
+        mass[cc] += vm;
+      }
+    }
+  for (uint32_t i = 0; i < side * side; ++i) s->vertex_rest[4 * size_t(VC + i) + 3] = float(mass[i]);
+  // Hinges from oriented half-edges in triangle order (scenes/index.ts:894-946).
+  struct Half {
+    uint32_t first, second, opposite;
+  };
+  std::vector<std::pair<uint64_t, Half>> open;
+  std::vector<uint64_t> closed;
+  const size_t NT = s->cloth_areas.size();
+  std::unordered_map<uint64_t, Half> open_map;
+  std::unordered_set<uint64_t> closed_set;
+  for (size_t t = 0; t < NT; ++t) {
+    const uint32_t* ix = &s->cloth_triangles[3 * t];
+    const uint32_t hs[3][3] = {{ix[0], ix[1], ix[2]}, {ix[1], ix[2], ix[0]}, {ix[2], ix[0], ix[1]}};
+    for (auto& h : hs) {
+      const uint64_t key = (uint64_t(std::min(h[0], h[1])) << 32) | std::max(h[0], h[1]);
+      if (closed_set.count(key)) {
+        delete s;
+        return jgs2::fail(JGS2_ERROR_RANGE, "Cloth edge is non-manifold.");
+      }
+      auto it = open_map.find(key);
+      if (it == open_map.end()) {
+        open_map[key] = Half{h[0], h[1], h[2]};
+        continue;
+      }
+      const Half prev = it->second;
+      if (prev.first != h[1] || prev.second != h[0]) {
+        delete s;
+        return jgs2::fail(JGS2_ERROR_RANGE, "Cloth triangles sharing an edge must have consistent orientation.");
+      }
+      s->cloth_hinges.push_back(prev.first);
+      s->cloth_hinges.push_back(prev.second);
+      s->cloth_hinges.push_back(prev.opposite);
+      s->cloth_hinges.push_back(h[2]);
+      open_map.erase(it);
+      closed_set.insert(key);
+    }
+  }
+  const size_t NH = s->cloth_hinges.size() / 4;
+  for (size_t h = 0; h < NH; ++h) {
+    const uint32_t* ix = &s->cloth_hinges[4 * h];
+    double angle = 0, len = 0;
+    if (!hinge_rest(&x[3 * size_t(ix[0] - VC)], &x[3 * size_t(ix[1] - VC)], &x[3 * size_t(ix[2] - VC)],
+                    &x[3 * size_t(ix[3] - VC)], &angle, &len)) {
+      delete s;
+      return jgs2::fail(JGS2_ERROR_RANGE, "Cloth hinge %zu is degenerate.", h);
+    }
+    s->hinge_angles.push_back(float(angle));
+    s->hinge_lengths.push_back(float(len));
+  }
+  s->has_cloth = true;
+  jgs2_cloth_input& ci = s->cloth;
+  ci.triangle_count = uint32_t(NT);
+  ci.triangle_indices = s->cloth_triangles.data();
+  ci.triangle_inverse_rest_bases = s->cloth_inverse.data();
+  ci.triangle_rest_areas = s->cloth_areas.data();
+  ci.hinge_count = uint32_t(NH);
+  ci.hinge_indices = s->cloth_hinges.data();
+  ci.hinge_rest_angles = s->hinge_angles.data();
+  ci.hinge_rest_edge_lengths = s->hinge_lengths.data();
+  ci.young_modulus = d->young_modulus;
+  ci.poisson_ratio = d->poisson_ratio;
+  ci.thickness = d->thickness;
+  ci.bending_stiffness = d->bending_stiffness;
+  *out = s;
+  jgs2::clear_error();
+  return JGS2_OK;
+}
+
+extern "C" {
+
 jgs2_status jgs2_scene_get_input(const jgs2_scene* s, jgs2_scene_input* out) {
   if (!s || !out) return jgs2::fail(JGS2_ERROR_TYPE, "NULL scene.");
   std::memset(out, 0, sizeof(*out));
@@ -427,6 +680,7 @@ jgs2_status jgs2_scene_get_input(const jgs2_scene* s, jgs2_scene_input* out) {
   out->external_forces = s->forces.empty() ? nullptr : s->forces.data();
   out->target_positions = s->targets.empty() ? nullptr : s->targets.data();
   out->target_stiffnesses = s->stiffnesses.empty() ? nullptr : s->stiffnesses.data();
+  out->cloth = s->has_cloth ? &s->cloth : nullptr;
   jgs2::clear_error();
   return JGS2_OK;
 }

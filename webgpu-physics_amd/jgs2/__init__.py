@@ -14,7 +14,7 @@ from ._native import (  # noqa: F401
     JGS2TypeError,
     LIBRARY_PATH,
 )
-from .solver import DEFAULT_STEP_SETTINGS, GpuInput, JGS2GpuSolver  # noqa: F401
+from .solver import DEFAULT_STEP_SETTINGS, ClothData, GpuInput, JGS2GpuSolver  # noqa: F401
 
 
 def device_count() -> int:

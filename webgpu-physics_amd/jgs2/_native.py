@@ -30,6 +30,23 @@ f32p = C.POINTER(C.c_float)
 u32p = C.POINTER(C.c_uint32)
 
 
+class ClothInput(C.Structure):
+    _fields_ = [
+        ("triangle_count", C.c_uint32),
+        ("triangle_indices", u32p),
+        ("triangle_inverse_rest_bases", f32p),
+        ("triangle_rest_areas", f32p),
+        ("hinge_count", C.c_uint32),
+        ("hinge_indices", u32p),
+        ("hinge_rest_angles", f32p),
+        ("hinge_rest_edge_lengths", f32p),
+        ("young_modulus", C.c_double),
+        ("poisson_ratio", C.c_double),
+        ("thickness", C.c_double),
+        ("bending_stiffness", C.c_double),
+    ]
+
+
 class SceneInput(C.Structure):
     _fields_ = [
         ("vertex_count", C.c_uint32),
@@ -51,6 +68,7 @@ class SceneInput(C.Structure):
         ("external_forces", f32p),
         ("target_positions", f32p),
         ("target_stiffnesses", f32p),
+        ("cloth", C.POINTER(ClothInput)),
     ]
 
 
@@ -120,6 +138,24 @@ class CuboidDesc(C.Structure):
     ]
 
 
+class ClothDesc(C.Structure):
+    _fields_ = [
+        ("cells", C.c_uint32),
+        ("origin", C.c_double * 3),
+        ("size", C.c_double * 2),
+        ("fold_column", C.c_uint32),
+        ("fold_lift", C.c_double),
+        ("collider_origin", C.c_double * 3),
+        ("collider_size", C.c_double * 3),
+        ("density", C.c_double),
+        ("young_modulus", C.c_double),
+        ("poisson_ratio", C.c_double),
+        ("thickness", C.c_double),
+        ("bending_stiffness", C.c_double),
+        ("cubature_k", C.c_uint32),
+    ]
+
+
 # Every exported entry point with its ctypes signature. Tests compare this
 # table with the declarations in include/*.h.
 SIGNATURES = {
@@ -157,11 +193,13 @@ SIGNATURES = {
     "jgs2_profile_end": (
         C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "jgs2_scene_build_cuboid": (C.c_int, [C.POINTER(CuboidDesc), C.POINTER(C.c_void_p)]),
+    "jgs2_scene_build_cloth_drape": (C.c_int, [C.POINTER(ClothDesc), C.POINTER(C.c_void_p)]),
     "jgs2_scene_get_input": (C.c_int, [C.c_void_p, C.POINTER(SceneInput)]),
     "jgs2_scene_set_objectives": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
     "jgs2_scene_destroy": (None, [C.c_void_p]),
     "jgs2_build_adjacency": (C.c_int, [C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p]),
     "jgs2_greedy_vertex_coloring": (C.c_int, [C.c_uint32, u32p, C.c_uint32, u32p, u32p, u32p]),
+    "jgs2_build_element_incidence": (C.c_int, [C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p, u32p]),
 }
 
 

@@ -9,6 +9,8 @@ libjgs2hip (include/jgs2hip_scene.h).
   vertices, 3,145,728 tetrahedra) with seeded U(-0.1h, 0.1h) vertex jitter,
   alternating 8-cell z-slabs of E = 1e4 / 1e6 Pa, nu = 0.45, the z = 0 face
   pinned. `scale` shrinks the grid for tests.
+* `cloth_drape`: the reference's pinned cloth drape, a fixed collider block
+  under a pre-folded StVK triangle sheet with dihedral bending.
 """
 
 from __future__ import annotations
@@ -18,7 +20,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .solver import GpuInput
+from .solver import ClothData, GpuInput
 
 # Minimal-scene scripted target (src/scenes/index.ts:252-262).
 TARGET_PULL_START_FRAME = 48
@@ -52,36 +54,86 @@ def _desc(cells, origin, size, *, jitter=0.0, seed=0, pin_axis=0, model=N.MATERI
     return d
 
 
+def _from_handle(handle) -> GpuInput:
+    s = N.SceneInput()
+    N.check(N.lib.jgs2_scene_get_input(handle, C.byref(s)))
+    V, T, K = s.vertex_count, s.tet_count, s.cubature_k
+
+    def f(ptr, n, optional=False):
+        if n == 0 or not ptr:
+            return None if optional else np.zeros(0, dtype=np.float32)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+    cloth = None
+    if s.cloth:
+        c = s.cloth.contents
+        NT, NH = c.triangle_count, c.hinge_count
+        cloth = ClothData(
+            triangle_indices=f(c.triangle_indices, 3 * NT),
+            triangle_inverse_rest_bases=f(c.triangle_inverse_rest_bases, 4 * NT),
+            triangle_rest_areas=f(c.triangle_rest_areas, NT),
+            hinge_indices=f(c.hinge_indices, 4 * NH),
+            hinge_rest_angles=f(c.hinge_rest_angles, NH),
+            hinge_rest_edge_lengths=f(c.hinge_rest_edge_lengths, NH),
+            young_modulus=c.young_modulus, poisson_ratio=c.poisson_ratio, thickness=c.thickness,
+            bending_stiffness=c.bending_stiffness)
+    return GpuInput(
+        vertex_count=V, tet_count=T, cubature_k=K,
+        positions=f(s.positions, 4 * V),
+        vertex_rest=f(s.vertex_rest, 4 * V),
+        vertex_info=f(s.vertex_info, 4 * V),
+        tet_indices=f(s.tet_indices, 4 * T),
+        tet_inverse_dm=f(s.tet_inverse_dm, 12 * T),
+        tet_meta=f(s.tet_meta, 4 * T),
+        adjacency=f(s.adjacency, int(s.adjacency_length)),
+        cubature_tet_ids=f(s.cubature_tet_ids, V * K),
+        cubature_weights=f(s.cubature_weights, V * K),
+        cubature_basis=f(s.cubature_basis, V * K * 36),
+        tet_rest_stiffness=f(s.tet_rest_stiffness, 144 * T, optional=True),
+        cloth=cloth,
+    )
+
+
 def build_cuboid(desc: N.CuboidDesc) -> GpuInput:
     """Run the native scene builder and copy its arrays into a GpuInput."""
     handle = C.c_void_p()
     N.check(N.lib.jgs2_scene_build_cuboid(C.byref(desc), C.byref(handle)))
     try:
-        s = N.SceneInput()
-        N.check(N.lib.jgs2_scene_get_input(handle, C.byref(s)))
-        V, T, K = s.vertex_count, s.tet_count, s.cubature_k
-
-        def f(ptr, n, optional=False):
-            if n == 0 or not ptr:
-                return None if optional else np.zeros(0, dtype=np.float32)
-            return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
-
-        return GpuInput(
-            vertex_count=V, tet_count=T, cubature_k=K,
-            positions=f(s.positions, 4 * V),
-            vertex_rest=f(s.vertex_rest, 4 * V),
-            vertex_info=f(s.vertex_info, 4 * V),
-            tet_indices=f(s.tet_indices, 4 * T),
-            tet_inverse_dm=f(s.tet_inverse_dm, 12 * T),
-            tet_meta=f(s.tet_meta, 4 * T),
-            adjacency=f(s.adjacency, int(s.adjacency_length)),
-            cubature_tet_ids=f(s.cubature_tet_ids, V * K),
-            cubature_weights=f(s.cubature_weights, V * K),
-            cubature_basis=f(s.cubature_basis, V * K * 36),
-            tet_rest_stiffness=f(s.tet_rest_stiffness, 144 * T, optional=True),
-        )
+        return _from_handle(handle)
     finally:
         N.lib.jgs2_scene_destroy(handle)
+
+
+def cloth_drape(cells=4, origin=(-0.8, 1.15, -0.8), size=(1.6, 1.6), fold_column=3, fold_lift=0.079,
+                collider_origin=(-0.65, 0.0, -0.45), collider_size=(0.9, 0.5, 0.9), density=480.0,
+                young=9000.0, poisson=0.3, thickness=0.012, bending=0.045, cubature_k=4) -> GpuInput:
+    """The pinned cloth drape (clothDefinition, src/scenes/index.ts:632-705);
+    the defaults are the reference scene's 4x4 pre-folded sheet."""
+    d = N.ClothDesc()
+    d.cells = int(cells)
+    for i in range(3):
+        d.origin[i] = float(origin[i])
+        d.collider_origin[i] = float(collider_origin[i])
+        d.collider_size[i] = float(collider_size[i])
+    d.size[0], d.size[1] = float(size[0]), float(size[1])
+    d.fold_column = int(fold_column)
+    d.fold_lift = float(fold_lift)
+    d.density = float(density)
+    d.young_modulus = float(young)
+    d.poisson_ratio = float(poisson)
+    d.thickness = float(thickness)
+    d.bending_stiffness = float(bending)
+    d.cubature_k = int(cubature_k)
+    handle = C.c_void_p()
+    N.check(N.lib.jgs2_scene_build_cloth_drape(C.byref(d), C.byref(handle)))
+    try:
+        scene = _from_handle(handle)
+    finally:
+        N.lib.jgs2_scene_destroy(handle)
+    side = int(cells) + 1
+    scene.extras["cloth_offset"] = scene.vertex_count - side * side
+    scene.extras["landmark_vertex"] = scene.extras["cloth_offset"] + 2 + 2 * side  # cloth centre of the 4x4 sheet
+    return scene
 
 
 def cantilever_beam(cells=(16, 4, 4), size=(4.0, 1.0, 1.0), cubature_k=6, model=N.MATERIAL_STABLE_NEO_HOOKEAN,
@@ -140,6 +192,16 @@ LAYERED_BOX_SETTINGS = {
     "schedule": "jacobi",
 }
 LAYERED_BOX_ITERATIONS = 8
+
+# The reference cloth scene's settings (src/scenes/index.ts:698-704).
+CLOTH_SETTINGS = {
+    "timestep": 1.0 / 120.0,
+    "gravity": (0.0, -9.81, 0.0),
+    "floor_height": 0.0,
+    "parity_mode": True,
+    "schedule": "jacobi",
+}
+CLOTH_ITERATIONS = 5
 
 
 def algorithmic_bytes_per_sweep(V: int, T: int, K: int, objectives: bool = False) -> int:

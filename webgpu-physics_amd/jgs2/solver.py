@@ -72,6 +72,53 @@ def settings_struct(overrides: Optional[Mapping] = None):
 
 
 @dataclass
+class ClothData:
+    """Arrays of `JGS2GpuClothInput` (cloth-layout.ts:89-104)."""
+
+    triangle_indices: np.ndarray
+    triangle_inverse_rest_bases: np.ndarray
+    triangle_rest_areas: np.ndarray
+    hinge_indices: np.ndarray
+    hinge_rest_angles: np.ndarray
+    hinge_rest_edge_lengths: np.ndarray
+    young_modulus: float
+    poisson_ratio: float
+    thickness: float
+    bending_stiffness: float
+
+    def __post_init__(self):
+        for name in ("triangle_indices", "hinge_indices"):
+            setattr(self, name, np.ascontiguousarray(getattr(self, name), dtype=np.uint32).reshape(-1))
+        for name in ("triangle_inverse_rest_bases", "triangle_rest_areas", "hinge_rest_angles",
+                     "hinge_rest_edge_lengths"):
+            setattr(self, name, np.ascontiguousarray(getattr(self, name), dtype=np.float32).reshape(-1))
+
+    @property
+    def triangle_count(self) -> int:
+        return self.triangle_indices.size // 3
+
+    @property
+    def hinge_count(self) -> int:
+        return self.hinge_indices.size // 4
+
+    def struct(self) -> N.ClothInput:
+        c = N.ClothInput()
+        c.triangle_count = self.triangle_count
+        c.triangle_indices = N.uptr(self.triangle_indices)
+        c.triangle_inverse_rest_bases = N.fptr(self.triangle_inverse_rest_bases)
+        c.triangle_rest_areas = N.fptr(self.triangle_rest_areas)
+        c.hinge_count = self.hinge_count
+        c.hinge_indices = N.uptr(self.hinge_indices)
+        c.hinge_rest_angles = N.fptr(self.hinge_rest_angles)
+        c.hinge_rest_edge_lengths = N.fptr(self.hinge_rest_edge_lengths)
+        c.young_modulus = float(self.young_modulus)
+        c.poisson_ratio = float(self.poisson_ratio)
+        c.thickness = float(self.thickness)
+        c.bending_stiffness = float(self.bending_stiffness)
+        return c
+
+
+@dataclass
 class GpuInput:
     """Arrays of `JGS2GpuInput` (layout.ts:41-90) as contiguous numpy arrays."""
 
@@ -93,6 +140,7 @@ class GpuInput:
     external_forces: Optional[np.ndarray] = None
     target_positions: Optional[np.ndarray] = None
     target_stiffnesses: Optional[np.ndarray] = None
+    cloth: Optional[ClothData] = None
     extras: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -127,6 +175,11 @@ class GpuInput:
         s.external_forces = N.fptr(self.external_forces)
         s.target_positions = N.fptr(self.target_positions)
         s.target_stiffnesses = N.fptr(self.target_stiffnesses)
+        if self.cloth is not None:
+            # The structure holds raw pointers; keep the cloth block alive
+            # with the returned SceneInput.
+            self._cloth_struct = self.cloth.struct()
+            s.cloth = C.pointer(self._cloth_struct)
         return s
 
     def packed_objectives(self) -> np.ndarray:
