@@ -74,6 +74,19 @@ typedef struct {
 } jgs2_cloth_input;
 
 /*
+ * Optional static IPC contact candidates: `StaticIpcContactCandidates`
+ * (src/simulation/cpu/ipc-contact.ts:20-38) in the order packed by
+ * packIpcContactBuffer (src/simulation/gpu/ipc-contact-layout.ts:271-342):
+ * vertex-triangle records (point, t0, t1, t2) first, then edge-edge records
+ * (a0, a1, b0, b1), four distinct vertices each.
+ */
+typedef struct {
+  uint32_t vertex_triangle_count;
+  uint32_t edge_edge_count;
+  const uint32_t* packed_indices;  /* 4 per candidate */
+} jgs2_contact_input;
+
+/*
  * Scene input: the arrays of `JGS2GpuInput` (layout.ts:41-90), same element
  * order and meaning. Optional arrays may be NULL.
  */
@@ -98,6 +111,7 @@ typedef struct {
   const float* target_positions;    /* optional, 3 per vertex (with target_stiffnesses) */
   const float* target_stiffnesses;  /* optional, 1 per vertex */
   const jgs2_cloth_input* cloth;    /* optional triangle cloth layer */
+  const jgs2_contact_input* contact; /* optional IPC candidates (contactCandidates) */
 } jgs2_scene_input;
 
 /* Schedules (jgs2-solver.ts:62). */

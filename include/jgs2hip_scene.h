@@ -79,6 +79,43 @@ typedef struct {
 
 jgs2_status jgs2_scene_build_cloth_drape(const jgs2_cloth_desc* desc, jgs2_scene** out);
 
+/*
+ * The reference's IPC trough (troughDefinition, src/scenes/index.ts:551-631):
+ * a pinned 2x1x1 cuboid warped into a V (E 1.2e5, nu 0.34, rho 1000) and
+ * `box_count` free stable Neo-Hookean boxes of 0.46 m (E 7e4, nu 0.35,
+ * rho 850) in layers of four slots. box_cells = 1 and seed = 0 give the
+ * reference scene exactly; box_cells = n splits each box into n^3 hexes and
+ * a nonzero seed draws each box's offset within its slot and its yaw. With
+ * `build_candidates` the static VT/EE superset of the boundary surface is
+ * attached (buildStaticIpcContactCandidates).
+ */
+typedef struct {
+  uint32_t box_count;        /* 1..20 */
+  uint32_t box_cells;        /* hexes per box edge */
+  uint32_t seed;
+  uint32_t cubature_k;
+  int32_t build_candidates;
+} jgs2_trough_desc;
+
+jgs2_status jgs2_scene_build_trough(const jgs2_trough_desc* desc, jgs2_scene** out);
+
+/* The boundary surface of the scene's tetrahedra (extractBoundarySurface,
+ * src/simulation/cpu/mesh.ts:366-440): outward triangles in ascending
+ * sorted-key order and their unique sorted edges. Pointers stay valid until
+ * jgs2_scene_destroy(). */
+jgs2_status jgs2_scene_get_surface(const jgs2_scene* scene, const uint32_t** triangles, uint32_t* triangle_count,
+                                   const uint32_t** edges, uint32_t* edge_count);
+
+/* The static IPC candidate superset (buildStaticIpcContactCandidates,
+ * src/simulation/cpu/ipc-contact.ts:600-665): vertex-triangle pairs over all
+ * vertices and canonical triangles, then edge-edge pairs, omitting pairs in
+ * a closed one-ring. Counts are always returned; `packed` (4 indices per
+ * candidate, `capacity` candidates) may be NULL to query them. */
+jgs2_status jgs2_build_static_contact_candidates(uint32_t vertex_count, const uint32_t* triangles,
+                                                 uint32_t triangle_count, const uint32_t* edges, uint32_t edge_count,
+                                                 uint32_t* packed, uint64_t capacity, uint32_t* vt_count,
+                                                 uint32_t* ee_count);
+
 /* Pointers into the scene's arrays, valid until jgs2_scene_destroy(). */
 jgs2_status jgs2_scene_get_input(const jgs2_scene* scene, jgs2_scene_input* out);
 

@@ -198,6 +198,199 @@ template <class T> M3<T> snh_tangent(const M3<T>& f, const M3<T>& h, T lam, T mu
   return shear * h + dshear * f + dvol * cof + vol * dcof;
 }
 
+// ---------------------------------------------------------------- IPC contact (IC)
+// Contact weights satisfy sum_i w_i x_i = point_a - point_b; VT records are
+// (point, t0, t1, t2), EE records (a0, a1, b0, b1) (IC:1-17). The barrier's
+// log term runs in FP64 for both precisions: the product evaluates it in
+// FP64 near d -> 0, which is the FP64 form of IC:318-351.
+constexpr double kIpcLengthSquaredFloor = 1.0e-20;
+constexpr double kIpcParallelRelativeEpsilon = 1.0e-6;
+template <class T> struct Contact {  // IC:29-37
+  V3<T> point_a;
+  T distance;
+  V3<T> point_b;
+  T squared_distance;
+  V3<T> normal;
+  bool valid;
+  V4<T> weights;
+};
+template <class T> V3<T> ipc_safe_unit(V3<T> v, V3<T> fb) {  // IC:56-73
+  T ls = dot(v, v);
+  if (finite_s(ls) && ls > T(kIpcLengthSquaredFloor)) return v * (T(1) / std::sqrt(ls));
+  T fls = dot(fb, fb);
+  if (finite_s(fls) && fls > T(kIpcLengthSquaredFloor)) return fb * (T(1) / std::sqrt(fls));
+  return v3<T>(1, 0, 0);
+}
+template <class T> V3<T> ipc_orthogonal_unit(V3<T> d) {  // IC:75-86
+  V3<T> a = v3(std::fabs(d.x), std::fabs(d.y), std::fabs(d.z));
+  V3<T> axis = v3<T>(1, 0, 0);
+  if (a.y <= a.x && a.y <= a.z) axis = v3<T>(0, 1, 0);
+  else if (a.z <= a.x && a.z <= a.y) axis = v3<T>(0, 0, 1);
+  return ipc_safe_unit(cross(d, axis), v3<T>(1, 0, 0));
+}
+template <class T> Contact<T> ipc_invalid_contact() {  // IC:88-98
+  return Contact<T>{v3<T>(0, 0, 0), T(0), v3<T>(0, 0, 0), T(0), v3<T>(1, 0, 0), false, V4<T>{0, 0, 0, 0}};
+}
+template <class T> Contact<T> ipc_make_contact(V3<T> a, V3<T> b, V4<T> w, V3<T> fb) {  // IC:100-125
+  if (!finite_v(a) || !finite_v(b) || !finite_v4(w)) return ipc_invalid_contact<T>();
+  V3<T> gap = a - b;
+  T sq = dot(gap, gap);
+  if (!finite_s(sq) || sq < T(0)) return ipc_invalid_contact<T>();
+  return Contact<T>{a, std::sqrt(sq), b, sq, ipc_safe_unit(gap, fb), true, w};
+}
+template <class T> struct SegmentPoint {
+  V3<T> point;
+  T parameter, squared_distance;
+};
+template <class T> SegmentPoint<T> ipc_closest_on_segment(V3<T> q, V3<T> s, V3<T> e) {  // IC:127-141
+  V3<T> edge = e - s;
+  T els = dot(edge, edge);
+  T t = T(0);
+  if (finite_s(els) && els > T(kIpcLengthSquaredFloor)) t = std::clamp(dot(q - s, edge) / els, T(0), T(1));
+  V3<T> pt = s + t * edge;
+  V3<T> d = q - pt;
+  return SegmentPoint<T>{pt, t, dot(d, d)};
+}
+template <class T> Contact<T> ipc_point_triangle(V3<T> p, V3<T> t0, V3<T> t1, V3<T> t2) {  // IC:146-224
+  V3<T> e01 = t1 - t0, e02 = t2 - t0, e12 = t2 - t1;
+  T s01 = dot(e01, e01), s02 = dot(e02, e02), s12 = dot(e12, e12);
+  T largest = std::max(s01, std::max(s02, s12));
+  V3<T> an = cross(e01, e02);
+  T as = dot(an, an);
+  T threshold = std::max(T(kIpcLengthSquaredFloor) * T(kIpcLengthSquaredFloor),
+                         T(kIpcParallelRelativeEpsilon) * largest * largest);
+  V3<T> longest = e01;
+  if (s02 > s01 && s02 >= s12) longest = e02;
+  else if (s12 > s01 && s12 > s02) longest = e12;
+  V3<T> fb = ipc_safe_unit(an, ipc_orthogonal_unit(longest));
+  if (finite_s(as) && as > threshold) {
+    T d00 = s01, d01 = dot(e01, e02), d11 = s02;
+    V3<T> po = p - t0;
+    T d20 = dot(po, e01), d21 = dot(po, e02);
+    T den = d00 * d11 - d01 * d01;
+    T b1 = (d11 * d20 - d01 * d21) / den;
+    T b2 = (d00 * d21 - d01 * d20) / den;
+    T b0 = T(1) - b1 - b2;
+    if (b0 >= T(0) && b1 >= T(0) && b2 >= T(0)) {
+      V3<T> closest = b0 * t0 + b1 * t1 + b2 * t2;
+      return ipc_make_contact(p, closest, V4<T>{T(1), -b0, -b1, -b2}, fb);
+    }
+  }
+  SegmentPoint<T> c01 = ipc_closest_on_segment(p, t0, t1);
+  SegmentPoint<T> c12 = ipc_closest_on_segment(p, t1, t2);
+  SegmentPoint<T> c20 = ipc_closest_on_segment(p, t2, t0);
+  SegmentPoint<T> best = c01;
+  V3<T> bary = v3(T(1) - c01.parameter, c01.parameter, T(0));
+  if (c12.squared_distance < best.squared_distance) {
+    best = c12;
+    bary = v3(T(0), T(1) - c12.parameter, c12.parameter);
+  }
+  if (c20.squared_distance < best.squared_distance) {
+    best = c20;
+    bary = v3(c20.parameter, T(0), T(1) - c20.parameter);
+  }
+  return ipc_make_contact(p, best.point, V4<T>{T(1), -bary.x, -bary.y, -bary.z}, fb);
+}
+template <class T> Contact<T> ipc_edge_edge(V3<T> a0, V3<T> a1, V3<T> b0, V3<T> b1) {  // IC:228-308
+  V3<T> da = a1 - a0, db = b1 - b0, off = a0 - b0;
+  T la = dot(da, da), lb = dot(db, db);
+  T scale = std::max(la, lb);
+  T dthr = std::max(T(kIpcLengthSquaredFloor), T(kIpcParallelRelativeEpsilon) * scale);
+  bool adeg = !(la > dthr), bdeg = !(lb > dthr);
+  T pa = T(0), pb = T(0);
+  if (adeg && !bdeg) {
+    pb = std::clamp(dot(db, off) / lb, T(0), T(1));
+  } else if (!adeg) {
+    T proj_a = dot(da, off);
+    if (bdeg) {
+      pa = std::clamp(-proj_a / la, T(0), T(1));
+    } else {
+      T coupling = dot(da, db);
+      T proj_b = dot(db, off);
+      T den = la * lb - coupling * coupling;
+      T pthr = T(kIpcParallelRelativeEpsilon) * la * lb;
+      if (den > pthr) pa = std::clamp((coupling * proj_b - proj_a * lb) / den, T(0), T(1));
+      T pbu = (coupling * pa + proj_b) / lb;
+      if (pbu < T(0)) {
+        pb = T(0);
+        pa = std::clamp(-proj_a / la, T(0), T(1));
+      } else if (pbu > T(1)) {
+        pb = T(1);
+        pa = std::clamp((coupling - proj_a) / la, T(0), T(1));
+      } else {
+        pb = pbu;
+      }
+    }
+  }
+  V3<T> point_a = a0 + pa * da, point_b = b0 + pb * db;
+  V3<T> longest = lb > la ? db : da;
+  V3<T> fb = ipc_safe_unit(cross(da, db), ipc_orthogonal_unit(longest));
+  return ipc_make_contact(point_a, point_b, V4<T>{T(1) - pa, pa, -(T(1) - pb), -pb}, fb);
+}
+template <class T> bool ipc_barrier_active(T d, T dhat) {  // IC:310-314
+  return finite_s(d) && finite_s(dhat) && d > T(0) && d < dhat;
+}
+// B = -(d - dhat)^2 log(d / dhat) and its first two derivatives (IC:316-351),
+// with log(d / dhat) formed in FP64.
+template <class T> T ipc_log_ratio(T d, T dhat) { return T(std::log(double(d) / double(dhat))); }
+template <class T> T ipc_barrier(T d, T dhat) {
+  if (!ipc_barrier_active(d, dhat)) return T(0);
+  T o = d - dhat;
+  return -(o * o) * ipc_log_ratio(d, dhat);
+}
+template <class T> T ipc_barrier_d1(T d, T dhat) {
+  if (!ipc_barrier_active(d, dhat)) return T(0);
+  T o = d - dhat;
+  return T(-2) * o * ipc_log_ratio(d, dhat) - o * o / d;
+}
+template <class T> T ipc_barrier_d2(T d, T dhat) {
+  if (!ipc_barrier_active(d, dhat)) return T(0);
+  T o = d - dhat;
+  return T(-2) * ipc_log_ratio(d, dhat) - T(4) * o / d + o * o / (d * d);
+}
+template <class T> T ipc_normal_gradient_scalar(T w, T k, T d1) {  // IC:354-365
+  if (!finite_s(w) || !finite_s(k) || !finite_s(d1) || k < T(0)) return T(0);
+  return w * k * d1;
+}
+template <class T> T ipc_psd_normal_hessian_scalar(T w, T k, T d2) {  // IC:369-385
+  if (!finite_s(w) || !finite_s(k) || !finite_s(d2) || k <= T(0) || w == T(0) || d2 <= T(0)) return T(0);
+  T s = k * w * w * d2;
+  if (!finite_s(s)) return T(kF32Max);
+  return s;
+}
+template <class T> M3<T> ipc_psd_normal_hessian(V3<T> n, T s) {  // IC:387-395
+  V3<T> u = ipc_safe_unit(n, v3<T>(1, 0, 0));
+  T a = std::max(s, T(0));
+  return mcols(a * u.x * u, a * u.y * u, a * u.z * u);
+}
+// Conservative Lipschitz step cap d(alpha) >= d(0) - alpha L (IC:398-422),
+// in FP64.
+template <class T> T ipc_safe_step_cap(T ds, T dmin, T lip, T safety) {
+  if (!finite_s(ds) || !finite_s(dmin) || !finite_s(lip) || !finite_s(safety) || dmin < T(0) || ds <= dmin ||
+      lip < T(0))
+    return T(0);
+  if (lip == T(0)) return T(1);
+  double f = std::clamp(double(safety), 0.0, 1.0) * (double(ds) - double(dmin)) / double(lip);
+  return T(std::clamp(f, 0.0, 1.0));
+}
+template <class T> T ipc_friction_f1(T slip, T eps) {  // IC:439-446
+  T s = std::max(slip, T(0));
+  if (!(eps > T(0)) || s >= eps) return s > T(0) ? T(1) : T(0);
+  T n = s / eps;
+  return n * (T(2) - n);
+}
+template <class T> T ipc_friction_dissipation(T slip, T eps, T mu, T force) {  // IC:448-472
+  if (!finite_s(slip) || !finite_s(eps) || !finite_s(mu) || !finite_s(force)) return T(0);
+  T s = std::max(slip, T(0));
+  T scale = std::max(mu, T(0)) * std::max(force, T(0));
+  if (!(eps > T(0))) return scale * s;
+  if (s < eps) {
+    T n = s / eps;
+    return scale * eps * n * n * (T(1) - n / T(3));
+  }
+  return scale * (s - eps / T(3));
+}
+
 // ---------------------------------------------------------------- globalization (GL)
 template <class T> M3<T> symmetrize(const M3<T>& m) {  // GL:101-111
   T a01 = T(0.5) * (el(m, 0, 1) + el(m, 1, 0));
@@ -377,6 +570,10 @@ typedef struct {
   const uint32_t* cloth_hinges;             // 4 per hinge: triangles (0,1,2), (1,0,3)
   const float* cloth_hinge_rest;            // 2 per hinge: angle, edge length
   double cloth_young_modulus, cloth_poisson_ratio, cloth_thickness, cloth_bending_stiffness;
+  // Optional static IPC candidates (ipc-contact-layout.ts:271-342).
+  uint32_t contact_vt_count, contact_ee_count;
+  const uint32_t* contact_indices;  // 4 per candidate
+  double ipc_minimum_distance, ipc_friction_coefficient, ipc_friction_velocity_epsilon, ipc_step_safety;
 } jgs2o_input;
 
 typedef struct {
@@ -421,6 +618,12 @@ template <class T> struct Sim {
   std::vector<T> tri_inv, tri_area, hinge_rest;
   T cloth_lam = T(0), cloth_mu = T(0), cloth_thick = T(0), cloth_bend = T(0);
   std::vector<uint32_t> tri_rows, tri_elems, hinge_rows, hinge_elems;
+  // IPC candidates (ipc-contact-layout.ts:29-64): VT first, then EE; the
+  // contact parameters are fixed at create (SV:1096-1105).
+  uint32_t NC = 0, NVT = 0;
+  std::vector<uint32_t> cand;
+  T ipc_dmin = T(0), ipc_mu = T(0), ipc_eps = T(0), ipc_safety = T(0);
+  std::vector<uint32_t> cand_rows, cand_ids, active_count, active_ids;
 
   // ---- dynamic state (layout.ts:418-503)
   std::vector<V3<T>> pos[2];             // posA, posB
@@ -434,6 +637,8 @@ template <class T> struct Sim {
   std::vector<V3<T>> conv;               // 5 per vertex
   V4<T> control[4];
   std::vector<V4<T>> history;            // 64 * 8
+  std::vector<V4<T>> lag_n, lag_w, cscratch, src_c, src_w, tgt_c, tgt_w;  // per candidate
+  T contact_alpha = T(1);
 
   // ---- per-pass "uniform" (S:141-164)
   struct Params {
@@ -794,6 +999,7 @@ template <class T> struct Sim {
       if (!hinge_owned(h, v)) continue;
       e += bending_energy_at(h, buf, 0xffffffffu, v3<T>(0, 0, 0));
     }
+    e += owned_energy(v, buf);  // S:2643
     return e;
   }
   void store_vertex_energy(uint32_t v) {  // S:2647-2658
@@ -803,6 +1009,277 @@ template <class T> struct Sim {
     vertex_energy[v] = V4<T>{sv ? se : T(0), cv ? ce : T(0), sv ? T(1) : T(0), cv ? T(1) : T(0)};
   }
 
+
+  // --------------------------------------------------------------- IPC contact (S:296-874, S:2695-2929)
+  static constexpr uint32_t kNoVertex = 0xffffffffu;
+  bool ipc_enabled() const {  // S:465-471
+    return globalization && p.ipc_kappa > T(0) && NC > 0 && p.ipc_dhat > ipc_dmin;
+  }
+  const uint32_t* cidx(uint32_t c) const { return &cand[4 * size_t(c)]; }
+  uint32_t ctype(uint32_t c) const { return c < NVT ? 0u : 1u; }
+  T rounding_reach() const {  // S:509-513
+    return std::max(std::max(p.scene_scale, p.ipc_dhat), T(1.0e-12)) * T(kUnitRoundoff) *
+           T(kPositionResolutionMultiplier);
+  }
+  T broad_threshold() const { return p.ipc_dhat + rounding_reach(); }  // S:515-517
+  T active_reach() const {  // S:519-523
+    return p.ipc_dhat + T(2) * (std::max(p.max_step, T(0)) + rounding_reach());
+  }
+  static T aabb_lower_bound(uint32_t type, V3<T> p0, V3<T> p1, V3<T> p2, V3<T> p3) {  // S:525-550
+    V3<T> gap;
+    if (type == 0u) {
+      V3<T> mn = vmin(p1, vmin(p2, p3)), mx = vmax(p1, vmax(p2, p3));
+      gap = vmax(vmax(mn - p0, p0 - mx), v3<T>(0, 0, 0));
+    } else {
+      V3<T> mn0 = vmin(p0, p1), mx0 = vmax(p0, p1), mn1 = vmin(p2, p3), mx1 = vmax(p2, p3);
+      gap = vmax(vmax(mn1 - mx0, mn0 - mx1), v3<T>(0, 0, 0));
+    }
+    return length(gap);
+  }
+  static Contact<T> separated_contact(T d) {  // S:552-562
+    return Contact<T>{v3<T>(0, 0, 0), d, v3<T>(0, 0, 0), d * d, v3<T>(1, 0, 0), true, V4<T>{0, 0, 0, 0}};
+  }
+  V3<T> cpos(int buf, uint32_t v, uint32_t mv, V3<T> mp) const { return v == mv ? mp : load(buf, v); }
+  // ipcContactFromPositions / ipcContactAtThreshold / ipcContactAtOverride
+  // (S:564-650).
+  Contact<T> contact_at(uint32_t c, int buf, T threshold, uint32_t mv = kNoVertex,
+                        V3<T> mp = v3<T>(0, 0, 0)) const {
+    const uint32_t* ix = cidx(c);
+    V3<T> p0 = cpos(buf, ix[0], mv, mp), p1 = cpos(buf, ix[1], mv, mp);
+    V3<T> p2 = cpos(buf, ix[2], mv, mp), p3 = cpos(buf, ix[3], mv, mp);
+    T lb = aabb_lower_bound(ctype(c), p0, p1, p2, p3);
+    if (finite_s(lb) && lb >= threshold) return separated_contact(lb);
+    return ctype(c) == 0u ? ipc_point_triangle(p0, p1, p2, p3) : ipc_edge_edge(p0, p1, p2, p3);
+  }
+  T weight_for_vertex(uint32_t c, V4<T> w, uint32_t v) const {  // S:652-663
+    const uint32_t* ix = cidx(c);
+    T r = T(0);
+    if (ix[0] == v) r += w.x;
+    if (ix[1] == v) r += w.y;
+    if (ix[2] == v) r += w.z;
+    if (ix[3] == v) r += w.w;
+    return r;
+  }
+  bool incident(uint32_t c, uint32_t v) const {
+    const uint32_t* ix = cidx(c);
+    return ix[0] == v || ix[1] == v || ix[2] == v || ix[3] == v;
+  }
+  V3<T> weighted_displacement(uint32_t c, V4<T> w, int buf, uint32_t mv, V3<T> mp) const {  // S:665-705
+    const uint32_t* ix = cidx(c);
+    V3<T> r = v3<T>(0, 0, 0);
+    r = r + w.x * (cpos(buf, ix[0], mv, mp) - old[ix[0]]);
+    r = r + w.y * (cpos(buf, ix[1], mv, mp) - old[ix[1]]);
+    r = r + w.z * (cpos(buf, ix[2], mv, mp) - old[ix[2]]);
+    r = r + w.w * (cpos(buf, ix[3], mv, mp) - old[ix[3]]);
+    return r;
+  }
+  void store_cached(uint32_t c, bool target, const Contact<T>& k) {  // S:386-419
+    bool valid = k.valid && finite_s(k.distance) && k.distance >= T(0) && finite_v(k.normal) && finite_v4(k.weights);
+    (target ? tgt_c : src_c)[c] = valid ? V4<T>{k.normal.x, k.normal.y, k.normal.z, k.distance} : V4<T>{0, 0, 0, -1};
+    (target ? tgt_w : src_w)[c] = valid ? k.weights : V4<T>{0, 0, 0, 0};
+  }
+  Contact<T> cached(uint32_t c, bool target) const {  // S:421-448
+    V4<T> nd = (target ? tgt_c : src_c)[c];
+    V4<T> w = (target ? tgt_w : src_w)[c];
+    bool valid = nd.w >= T(0) && finite_v4(nd) && finite_v4(w);
+    T d = valid ? nd.w : T(0);
+    return Contact<T>{v3<T>(0, 0, 0), d, v3<T>(0, 0, 0), d * d, valid ? v3(nd.x, nd.y, nd.z) : v3<T>(1, 0, 0), valid,
+                      valid ? w : V4<T>{0, 0, 0, 0}};
+  }
+  Contact<T> cached_at(uint32_t c, int buf) const {  // S:450-460
+    bool use_target = p.tgt != p.src && buf == p.tgt;
+    if (use_target && contact_alpha == T(0)) use_target = false;
+    return cached(c, use_target);
+  }
+  bool friction_active(uint32_t c) const { return ipc_mu > T(0) && lag_n[c].w > T(0); }
+  T energy_from_contact(uint32_t c, const Contact<T>& k, int buf, uint32_t mv, V3<T> mp) const {  // S:707-740
+    if (!k.valid || !(k.distance > ipc_dmin)) return T(kF32Max);
+    T e = p.ipc_kappa * ipc_barrier(k.distance, p.ipc_dhat);
+    V4<T> lag = lag_n[c];
+    if (ipc_mu > T(0) && lag.w > T(0)) {
+      V3<T> n = ipc_safe_unit(v3(lag.x, lag.y, lag.z), k.normal);
+      V3<T> rel = weighted_displacement(c, lag_w[c], buf, mv, mp);
+      V3<T> tang = rel - n * dot(n, rel);
+      T slip_velocity = length(tang) * p.inv_dt;
+      e += p.dt * ipc_friction_dissipation(slip_velocity, ipc_eps, ipc_mu, lag.w);
+    }
+    return e;
+  }
+  uint32_t energy_owner(uint32_t c) const {  // S:742-748
+    const uint32_t* ix = cidx(c);
+    for (int l = 0; l < 4; ++l)
+      if (!pinned(ix[l])) return ix[l];
+    return ix[0];
+  }
+  // Active candidate rows built by buildActiveContactRows (S:356-371).
+  uint32_t active_begin(uint32_t v) const { return cand_rows[v]; }
+  uint32_t active_end(uint32_t v) const { return cand_rows[v] + active_count[v]; }
+  T owned_energy(uint32_t v, int buf) const {  // S:750-779
+    if (!ipc_enabled()) return T(0);
+    T r = T(0);
+    for (uint32_t i = active_begin(v); i < active_end(v); ++i) {
+      uint32_t c = active_ids[i];
+      Contact<T> k = cached_at(c, buf);
+      if (!k.valid) return T(kF32Max);
+      bool ba = ipc_barrier_active(k.distance, p.ipc_dhat);
+      if (!ba && !friction_active(c)) continue;
+      if (energy_owner(c) == v) {
+        T e = energy_from_contact(c, k, buf, kNoVertex, v3<T>(0, 0, 0));
+        if (!finite_s(e)) return T(kF32Max);
+        r += e;
+      }
+    }
+    return r;
+  }
+  void local_contribution(uint32_t v, int buf, V3<T>& g, M3<T>& H) const {  // S:786-874
+    g = v3<T>(0, 0, 0);
+    H = mzero<T>();
+    if (!ipc_enabled()) return;
+    for (uint32_t i = active_begin(v); i < active_end(v); ++i) {
+      uint32_t c = active_ids[i];
+      Contact<T> k = p.colored ? contact_at(c, buf, p.ipc_dhat) : cached(c, false);
+      if (!k.valid) continue;
+      bool ba = ipc_barrier_active(k.distance, p.ipc_dhat);
+      V4<T> lag = lag_n[c];
+      bool fa = ipc_mu > T(0) && lag.w > T(0);
+      if (!ba && !fa) continue;
+      if (!incident(c, v)) continue;
+      T w = weight_for_vertex(c, k.weights, v);
+      if (w != T(0) && ba) {
+        T d1 = ipc_barrier_d1(k.distance, p.ipc_dhat);
+        T d2 = ipc_barrier_d2(k.distance, p.ipc_dhat);
+        g = g + ipc_normal_gradient_scalar(w, p.ipc_kappa, d1) * k.normal;
+        H = H + ipc_psd_normal_hessian(k.normal, ipc_psd_normal_hessian_scalar(w, p.ipc_kappa, d2));
+      }
+      T lw = weight_for_vertex(c, lag_w[c], v);
+      if (fa && lw != T(0)) {
+        V3<T> n = ipc_safe_unit(v3(lag.x, lag.y, lag.z), k.normal);
+        V3<T> rel = weighted_displacement(c, lag_w[c], buf, kNoVertex, v3<T>(0, 0, 0));
+        V3<T> tang = rel - n * dot(n, rel);
+        T slip = length(tang);
+        T slip_velocity = slip * p.inv_dt;
+        T fs = ipc_mu * lag.w;
+        V3<T> dir = v3<T>(0, 0, 0);
+        if (slip > T(1.0e-12)) dir = tang / slip;
+        T f1 = ipc_friction_f1(slip_velocity, ipc_eps);
+        g = g + (lw * fs * f1) * dir;
+        M3<T> proj = midentity<T>() - ipc_psd_normal_hessian(n, T(1));
+        T tc = T(2) * fs / std::max(ipc_eps * p.dt, T(1.0e-12));
+        if (slip > T(1.0e-12)) tc = fs * f1 / slip;
+        H = H + (lw * lw * std::max(tc, T(0))) * proj;
+      }
+    }
+  }
+  bool pinned_motion_exceeds(uint32_t c, int buf, T bound) const {  // S:484-507
+    const uint32_t* ix = cidx(c);
+    for (int l = 0; l < 4; ++l) {
+      uint32_t v = ix[l];
+      if (!pinned(v)) continue;
+      V3<T> d = v3(rest[v].x, rest[v].y, rest[v].z) - load(buf, v);
+      T dl = length(d);
+      if (!finite_v(d) || !finite_s(dl) || dl > bound) return true;
+    }
+    return false;
+  }
+  void k_lag_contact() {  // S:2713-2744 (reads posA)
+    for (uint32_t c = 0; c < NC; ++c) {
+      Contact<T> k = contact_at(c, 0, broad_threshold());
+      store_cached(c, false, k);
+      V4<T> nf{0, 0, 0, 0}, w{0, 0, 0, 0};
+      if (k.valid && k.distance > ipc_dmin) {
+        T first = ipc_barrier_d1(k.distance, p.ipc_dhat);
+        T force = std::max(T(0), -p.ipc_kappa * first);
+        if (finite_s(force)) {
+          nf = V4<T>{k.normal.x, k.normal.y, k.normal.z, force};
+          w = k.weights;
+        }
+      }
+      lag_n[c] = nf;
+      lag_w[c] = w;
+      cscratch[c] = V4<T>{1, 0, 0, 0};
+    }
+    contact_alpha = T(1);
+  }
+  void k_build_active_rows() {  // S:2747-2785
+    const bool bounded = p.max_step > T(0);
+    const T reach = active_reach();
+    for (uint32_t v = 0; v < V; ++v) {
+      uint32_t n = 0;
+      for (uint32_t i = cand_rows[v]; i < cand_rows[v + 1]; ++i) {
+        uint32_t c = cand_ids[i];
+        Contact<T> k = cached(c, false);
+        bool include = !k.valid || !bounded || k.distance <= reach || friction_active(c) ||
+                       pinned_motion_exceeds(c, p.src, std::max(p.max_step, T(0)));
+        if (include) active_ids[cand_rows[v] + n++] = c;
+      }
+      active_count[v] = n;
+    }
+  }
+  void k_candidate_contact_step() {  // S:2788-2852
+    for (uint32_t c = 0; c < NC; ++c) {
+      if (stop_latched()) {
+        cscratch[c] = V4<T>{1, 0, 0, 1};
+        continue;
+      }
+      const uint32_t* ix = cidx(c);
+      V3<T> d[4];
+      for (int l = 0; l < 4; ++l) d[l] = load(p.tgt, ix[l]) - load(p.src, ix[l]);
+      T lip = T(0);
+      if (ctype(c) == 0u)
+        lip = length(d[0]) + std::max(length(d[1]), std::max(length(d[2]), length(d[3])));
+      else
+        lip = std::max(length(d[0]), length(d[1])) + std::max(length(d[2]), length(d[3]));
+      Contact<T> src = cached(c, false);
+      bool full = src.valid && finite_s(lip) && src.distance - lip > ipc_dmin;
+      if (!full) {
+        src = contact_at(c, p.src, T(kF32Max));
+        full = src.valid && finite_s(lip) && src.distance - lip > ipc_dmin;
+      }
+      bool trial_valid = finite_v(d[0]) && finite_v(d[1]) && finite_v(d[2]) && finite_v(d[3]);
+      bool valid = src.valid && trial_valid && src.distance > ipc_dmin && finite_s(lip);
+      T alpha = T(0);
+      if (valid) {
+        alpha = ipc_safe_step_cap(src.distance, ipc_dmin, lip, ipc_safety);
+        if (full) alpha = T(1);
+      }
+      cscratch[c] = V4<T>{alpha, src.valid ? src.distance : T(0), trial_valid ? src.distance : T(0),
+                          valid ? T(1) : T(0)};
+    }
+  }
+  void k_validate_contact_step() {  // S:2855-2875
+    for (uint32_t c = 0; c < NC; ++c) {
+      Contact<T> k = contact_at(c, p.tgt, broad_threshold());
+      store_cached(c, true, k);
+      bool valid = k.valid && finite_s(k.distance) && k.distance > ipc_dmin;
+      T d = valid ? k.distance : T(0);
+      cscratch[c] = V4<T>{1, d, d, valid ? T(1) : T(0)};
+    }
+  }
+  void k_reduce_contact_step() {  // S:2878-2918 (min and a 0/1 product: order free)
+    T amin = T(1);
+    bool all = true;
+    for (uint32_t c = 0; c < NC; ++c) {
+      V4<T> s = cscratch[c];
+      bool valid = s.w == T(1) && finite_s(s.x) && finite_s(s.y) && finite_s(s.z);
+      amin = std::min(amin, valid ? s.x : T(0));
+      all = all && valid;
+    }
+    contact_alpha = all ? std::clamp(amin, T(0), T(1)) : T(0);
+  }
+  void k_apply_contact_step() {  // S:2921-2929
+    if (stop_latched()) return;
+    T a = std::clamp(contact_alpha, T(0), T(1));
+    for (uint32_t v = 0; v < V; ++v) {
+      V3<T> s = load(p.src, v), c = load(p.tgt, v);
+      pos[p.tgt][v] = s + a * (c - s);
+    }
+  }
+  void k_promote_contact() {  // S:2695-2710
+    if (control[1].x == T(1) && finite_s(contact_alpha) && contact_alpha > T(0)) {
+      src_c = tgt_c;
+      src_w = tgt_w;
+    }
+  }
 
   // ---------------------------------------------------------------- cloth (S:1225-1845)
   V3<T> cloth_pos(int buf, uint32_t v, uint32_t ov, V3<T> op) const {  // S:1520-1531
@@ -982,6 +1459,20 @@ template <class T> struct Sim {
     V4<T> pre = control[0];
     uint32_t pv = validity_bits(pre.w);
     if ((pv & kAcceptedMinimumValid) != 0u && finite_s(pre.z)) minimum = pre.z;
+    if (ipc_enabled()) {  // S:1997-2037
+      V3<T> tp = load(p.src, sv) + d;
+      T dl = length(d);
+      for (uint32_t i = active_begin(sv); i < active_end(sv); ++i) {
+        uint32_t c = active_ids[i];
+        Contact<T> sc = p.colored ? contact_at(c, p.src, p.ipc_dhat) : cached(c, false);
+        if (!sc.valid) return Geometry{T(0), false};
+        if (sc.distance - dl > ipc_dmin) continue;
+        if (!incident(c, sv)) continue;
+        Contact<T> tc = contact_at(c, p.src, p.ipc_dhat, sv, tp);
+        if (!tc.valid) return Geometry{T(0), false};
+        if (!(tc.distance > ipc_dmin)) return Geometry{T(0), true};
+      }
+    }
     for (uint32_t a = 0; a < adj_count(sv); ++a) {
       uint32_t t = adjacency[adj_start(sv) + a];
       T dv = det(exact_trial_deformation(t, sv, d));
@@ -1019,6 +1510,23 @@ template <class T> struct Sim {
     T sin = rest[sv].w * p.inv_dt2;
     T result = quadratic_delta(sp, predicted[sv], d, sin) + floor_delta(sp.y, d.y);
     if (p.objective_flags != 0u) result += objective_delta(sv, sp, d);
+    if (ipc_enabled()) {  // S:2140-2193
+      V3<T> tp = sp + d;
+      T dl = length(d);
+      for (uint32_t i = active_begin(sv); i < active_end(sv); ++i) {
+        uint32_t c = active_ids[i];
+        Contact<T> sc = p.colored ? contact_at(c, p.src, p.ipc_dhat) : cached(c, false);
+        if (!sc.valid) return T(kF32Max);
+        bool may_change = sc.distance - dl < p.ipc_dhat;
+        if (!may_change && !friction_active(c)) continue;
+        if (!incident(c, sv)) continue;
+        Contact<T> tc = separated_contact(p.ipc_dhat);
+        if (may_change) tc = contact_at(c, p.src, p.ipc_dhat, sv, tp);
+        T se = energy_from_contact(c, sc, p.src, kNoVertex, v3<T>(0, 0, 0));
+        T te = energy_from_contact(c, tc, p.src, sv, tp);
+        result += te - se;
+      }
+    }
     for (uint32_t a = 0; a < adj_count(sv); ++a) {
       uint32_t t = adjacency[adj_start(sv) + a];
       M3<T> i = deformation(t);
@@ -1124,6 +1632,13 @@ template <class T> struct Sim {
     if (p.floor_k > T(0) && x.y < p.floor_y) {  // S:3188-3191
       g.y += p.floor_k * (x.y - p.floor_y);
       H.c[1].y += p.floor_k;
+    }
+    {  // S:3193-3195
+      V3<T> ig;
+      M3<T> ih;
+      local_contribution(v, p.src, ig, ih);
+      g = g + ig;
+      H = H + ih;
     }
     for (uint32_t s = 0; s < K; ++s) {  // S:3198-3263
       size_t rec = size_t(v) * K + s;
@@ -1381,7 +1896,9 @@ template <class T> struct Sim {
     // storeAssembledVertexEnergy inside jgs2Solve (S:3084 family): only the
     // vertex's own position enters for tetrahedral scenes, so evaluating it
     // after the dispatch equals the in-dispatch value.
-    if (globalization && !stop_latched()) {
+    // With contact the energy pair is recorded after the safe-step passes
+    // instead (S:3468-3471).
+    if (globalization && !stop_latched() && !ipc_enabled()) {
       for (uint32_t v = 0; v < V; ++v) {
         if (p.colored && colors[v] != p.active_color) continue;
         store_vertex_energy(v);
@@ -1633,6 +2150,12 @@ template <class T> struct Sim {
       }
       V3<T> contact = v3<T>(0, 0, 0);
       if (p.floor_k > T(0) && x.y < p.floor_y) contact.y = p.floor_k * (x.y - p.floor_y);
+      {  // S:4091
+        V3<T> ig;
+        M3<T> ih;
+        local_contribution(v, p.tgt, ig, ih);
+        contact = contact + ig;
+      }
       c[0] = in;
       c[1] = mat;
       c[2] = ef;
@@ -1904,6 +2427,10 @@ template <class T> struct Sim {
     };
     bind(0, 0, false, 0);
     k_predict();
+    // contactEnabled (SV:2213-2214); the contact terms inside the passes
+    // also need ipcEnabled (S:465-471).
+    const bool contact = NC > 0 && p.ipc_kappa > T(0);
+    if (contact) k_lag_contact();
     if (globalization) {
       bind(0, 0, false, 0);
       k_candidate_tet();
@@ -1913,6 +2440,7 @@ template <class T> struct Sim {
       int src = (it % 2 == 0) ? 1 : 0;
       int tgt = (it % 2 == 0) ? 0 : 1;
       bind(src, tgt, false, 0);
+      if (contact) k_build_active_rows();
       if (gs) {
         k_copy_position();
         for (uint32_t c = 0; c < color_count; ++c) {
@@ -1922,16 +2450,26 @@ template <class T> struct Sim {
           k_solve();
         }
         bind(src, tgt, false, 0);
-        if (globalization) k_assembled_vertex_energy();
+        if (globalization && !contact) k_assembled_vertex_energy();
       } else {
         k_tet_polar();
         k_vertex_polar();
         k_solve();
       }
+      if (contact) {  // SV:2347-2400
+        k_candidate_contact_step();
+        k_reduce_contact_step();
+        k_apply_contact_step();
+        k_validate_contact_step();
+        k_reduce_contact_step();
+        k_apply_contact_step();
+        k_assembled_vertex_energy();
+      }
       if (globalization) {
         k_candidate_tet();
         k_reduce_candidate();
         k_apply_candidate();
+        if (contact) k_promote_contact();
         k_convergence_gradient();
         k_reduce_convergence();
       }
@@ -1941,7 +2479,11 @@ template <class T> struct Sim {
       k_copy_position();
     }
     bind(0, 0, false, 0);
-    if (s.horizontal_body_correction) k_body_correction();
+    // resolveStepSettings (SV:1994-2008): objectives and mesh contact keep
+    // their solved centre-of-mass motion.
+    const bool body_correction =
+        s.horizontal_body_correction && !(globalization && (objective_flags != 0u || (NC > 0 && p.ipc_kappa > T(0))));
+    if (body_correction) k_body_correction();
     k_finalize();
   }
 
@@ -2020,6 +2562,24 @@ template <class T> struct Sim {
     };
     csr(tri_idx, 3, tri_rows, tri_elems);
     csr(hinge_idx, 4, hinge_rows, hinge_elems);
+    NVT = in.contact_vt_count;
+    NC = in.contact_vt_count + in.contact_ee_count;
+    cand.assign(in.contact_indices, in.contact_indices + 4 * size_t(NC));
+    ipc_dmin = T(float(in.ipc_minimum_distance));
+    ipc_mu = T(float(in.ipc_friction_coefficient));
+    ipc_eps = T(float(in.ipc_friction_velocity_epsilon));
+    ipc_safety = T(float(in.ipc_step_safety));
+    csr(cand, 4, cand_rows, cand_ids);  // packIpcIncidentCandidateAdjacency (ipc-contact-layout.ts:186-255)
+    active_count.assign(V, 0);
+    active_ids.assign(cand_ids.size(), 0);
+    lag_n.assign(NC, V4<T>{0, 0, 0, 0});
+    lag_w.assign(NC, V4<T>{0, 0, 0, 0});
+    cscratch.assign(NC, V4<T>{1, 0, 0, 0});
+    src_c.assign(NC, V4<T>{0, 0, 0, -1});
+    tgt_c.assign(NC, V4<T>{0, 0, 0, -1});
+    src_w.assign(NC, V4<T>{0, 0, 0, 0});
+    tgt_w.assign(NC, V4<T>{0, 0, 0, 0});
+    contact_alpha = T(1);
     pos[0].resize(V);
     pos[1].resize(V);
     predicted.resize(V);
@@ -2100,6 +2660,13 @@ template <class T> void local_systems(Sim<T>& s, const jgs2o_settings& st, doubl
   s.p.src = 0;
   s.p.tgt = 0;
   s.p.colored = false;
+  if (s.NC > 0 && s.p.ipc_kappa > T(0)) {
+    // Contact state of a frame start at this pose: old = x, lagged contact
+    // and the active rows (S:2661-2692, S:2713-2785).
+    s.old = s.pos[0];
+    s.k_lag_contact();
+    s.k_build_active_rows();
+  }
   s.k_tet_polar();
   s.k_vertex_polar();
   for (uint32_t v = 0; v < s.V; ++v) {
@@ -2129,6 +2696,13 @@ template <class T> void restricted_model(Sim<T>& s, const jgs2o_settings& st, ui
   s.p.src = 0;
   s.p.tgt = 0;
   s.p.colored = false;
+  if (s.NC > 0 && s.p.ipc_kappa > T(0)) {
+    // Contact state of a frame start at this pose: old = x, lagged contact
+    // and the active rows (S:2661-2692, S:2713-2785).
+    s.old = s.pos[0];
+    s.k_lag_contact();
+    s.k_build_active_rows();
+  }
   s.k_tet_polar();
   s.k_vertex_polar();
   s.k_candidate_tet();
@@ -2210,6 +2784,25 @@ void* jgs2o_create(const jgs2o_input* in, int precision) {
     h->f->load_input(*in);
   }
   return h;
+}
+
+// Contact state: per candidate lagged normal/force (4), lagged weights (4),
+// source contact (4), source weights (4); then the safe alpha.
+void jgs2o_read_contact(void* hv, double* out) {
+  Handle* h = static_cast<Handle*>(hv);
+  auto rd = [&](auto* s) {
+    size_t o = 0;
+    for (uint32_t c = 0; c < s->NC; ++c)
+      for (const auto* a : {&s->lag_n, &s->lag_w, &s->src_c, &s->src_w}) {
+        auto v = (*a)[c];
+        out[o++] = double(v.x);
+        out[o++] = double(v.y);
+        out[o++] = double(v.z);
+        out[o++] = double(v.w);
+      }
+    out[o] = double(s->contact_alpha);
+  };
+  if (h->d) rd(h->d); else rd(h->f);
 }
 
 void jgs2o_destroy(void* hv) {
@@ -2431,6 +3024,52 @@ void jgs2o_cloth_probe(int precision, const double* rest12, const double* pos12,
     out[35] = d.valid ? 1.0 : 0.0;
   };
   if (precision == 64) run(0.0); else run(0.0f);
+}
+
+// One IPC candidate (type 0 = VT, 1 = EE) at positions p12: out[0] distance,
+// [1..3] normal, [4..7] weights, [8] valid, [9] kappa B(d), [10] B'(d),
+// [11] B''(d), [12..23] barrier gradient kappa B'(d) w_l n per vertex
+// (IC:146-365).
+void jgs2o_ipc_probe(int precision, int type, const double* p12, double dhat, double kappa, double* out) {
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    V3<T> x[4];
+    for (int l = 0; l < 4; ++l) x[l] = v3(T(p12[3 * l]), T(p12[3 * l + 1]), T(p12[3 * l + 2]));
+    Contact<T> k = type == 0 ? ipc_point_triangle(x[0], x[1], x[2], x[3]) : ipc_edge_edge(x[0], x[1], x[2], x[3]);
+    out[0] = double(k.distance);
+    out[1] = double(k.normal.x);
+    out[2] = double(k.normal.y);
+    out[3] = double(k.normal.z);
+    out[4] = double(k.weights.x);
+    out[5] = double(k.weights.y);
+    out[6] = double(k.weights.z);
+    out[7] = double(k.weights.w);
+    out[8] = k.valid ? 1.0 : 0.0;
+    T d1 = ipc_barrier_d1(k.distance, T(dhat));
+    out[9] = double(T(kappa) * ipc_barrier(k.distance, T(dhat)));
+    out[10] = double(d1);
+    out[11] = double(ipc_barrier_d2(k.distance, T(dhat)));
+    const T w[4] = {k.weights.x, k.weights.y, k.weights.z, k.weights.w};
+    for (int l = 0; l < 4; ++l) {
+      V3<T> g = ipc_normal_gradient_scalar(w[l], T(kappa), d1) * k.normal;
+      out[12 + 3 * l] = double(g.x);
+      out[13 + 3 * l] = double(g.y);
+      out[14 + 3 * l] = double(g.z);
+    }
+  };
+  if (precision == 64) run(0.0); else run(0.0f);
+}
+
+// Friction primitives at one slip: out[0] f1, out[1] lagged dissipation
+// (IC:425-472).
+void jgs2o_ipc_friction(int precision, double slip, double eps, double mu, double force, double* out) {
+  if (precision == 64) {
+    out[0] = ipc_friction_f1(slip, eps);
+    out[1] = ipc_friction_dissipation(slip, eps, mu, force);
+  } else {
+    out[0] = double(ipc_friction_f1(float(slip), float(eps)));
+    out[1] = double(ipc_friction_dissipation(float(slip), float(eps), float(mu), float(force)));
+  }
 }
 
 // Rest data of a triangle: inv[4] row-major, area.

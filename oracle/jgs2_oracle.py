@@ -53,6 +53,13 @@ class OracleInput(C.Structure):
         ("cloth_poisson_ratio", C.c_double),
         ("cloth_thickness", C.c_double),
         ("cloth_bending_stiffness", C.c_double),
+        ("contact_vt_count", C.c_uint32),
+        ("contact_ee_count", C.c_uint32),
+        ("contact_indices", u32p),
+        ("ipc_minimum_distance", C.c_double),
+        ("ipc_friction_coefficient", C.c_double),
+        ("ipc_friction_velocity_epsilon", C.c_double),
+        ("ipc_step_safety", C.c_double),
     ]
 
 
@@ -110,6 +117,9 @@ def _load() -> C.CDLL:
         "jgs2o_greedy_coloring": (C.c_uint32, [C.c_uint32, u32p, C.c_uint32, u32p, u32p]),
         "jgs2o_cloth_probe": (None, [C.c_int, f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_double, f64p]),
         "jgs2o_triangle_rest": (None, [f64p, f64p, f64p]),
+        "jgs2o_ipc_probe": (None, [C.c_int, C.c_int, f64p, C.c_double, C.c_double, f64p]),
+        "jgs2o_ipc_friction": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, f64p]),
+        "jgs2o_read_contact": (None, [C.c_void_p, f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -137,7 +147,11 @@ DEFAULTS = {
     "contact_tangential_damping": 8.0,
     "contact_margin": 0.01,
     "ipc_activation_distance": 0.05,
+    "ipc_minimum_distance": 0.001,
     "ipc_barrier_stiffness": 0.0,
+    "ipc_friction_coefficient": 0.0,
+    "ipc_friction_velocity_epsilon": 0.01,
+    "ipc_step_safety": 0.9,
     "horizontal_body_correction": True,
     "parity_mode": False,
 }
@@ -206,7 +220,7 @@ def conflict_groups(scene) -> list:
 class Oracle:
     """The reference step restated on the CPU, FP32 (precision=32) or FP64."""
 
-    def __init__(self, scene, precision: int = 32, colors=None, objectives=None):
+    def __init__(self, scene, precision: int = 32, colors=None, objectives=None, settings=None):
         self.scene = scene
         self.V = scene.vertex_count
         self.precision = precision
@@ -263,6 +277,16 @@ class Oracle:
             inp.cloth_poisson_ratio = float(cloth.poisson_ratio)
             inp.cloth_thickness = float(cloth.thickness)
             inp.cloth_bending_stiffness = float(cloth.bending_stiffness)
+        contact = getattr(scene, "contact", None)
+        created = resolve(settings)  # contact parameters are fixed at create (SV:1096-1105)
+        inp.ipc_minimum_distance = float(created["ipc_minimum_distance"])
+        inp.ipc_friction_coefficient = float(created["ipc_friction_coefficient"])
+        inp.ipc_friction_velocity_epsilon = float(created["ipc_friction_velocity_epsilon"])
+        inp.ipc_step_safety = float(created["ipc_step_safety"])
+        if contact is not None:
+            inp.contact_vt_count = contact.vertex_triangle_count
+            inp.contact_ee_count = contact.edge_edge_count
+            inp.contact_indices = up(contact.packed_indices)
         self._handle = lib.jgs2o_create(C.byref(inp), precision)
 
     def __del__(self):
@@ -319,6 +343,14 @@ class Oracle:
 
     def history(self) -> np.ndarray:
         return self._read(lib.jgs2o_read_history, 64 * 32).reshape(64, 8, 4)
+
+    def contact_state(self):
+        """Per candidate: lagged normal/force, lagged weights, cached source
+        contact (normal, distance) and weights; and the last safe alpha."""
+        contact = getattr(self.scene, "contact", None)
+        n = 0 if contact is None else contact.candidate_count
+        out = self._read(lib.jgs2o_read_contact, 16 * n + 1)
+        return out[:-1].reshape(n, 4, 4), out[-1]
 
     def vertex_rotations(self) -> np.ndarray:
         return self._read(lib.jgs2o_read_vertex_rotations, 9 * self.V).reshape(-1, 3, 3)
@@ -397,3 +429,21 @@ def triangle_rest(x9):
     area = C.c_double()
     lib.jgs2o_triangle_rest(x.ctypes.data_as(f64p), inv.ctypes.data_as(f64p), C.byref(area))
     return inv.reshape(2, 2), area.value
+
+
+def ipc_probe(kind, p12, dhat, kappa, precision=64) -> dict:
+    """One IPC candidate (kind "vt" or "ee") at 4 positions: contact distance,
+    normal, weights, validity, barrier value and derivatives and the
+    per-vertex barrier gradient."""
+    x = np.ascontiguousarray(p12, dtype=np.float64).reshape(12)
+    out = np.empty(24)
+    lib.jgs2o_ipc_probe(precision, 0 if kind == "vt" else 1, x.ctypes.data_as(f64p), float(dhat), float(kappa),
+                        out.ctypes.data_as(f64p))
+    return {"distance": out[0], "normal": out[1:4].copy(), "weights": out[4:8].copy(), "valid": bool(out[8]),
+            "energy": out[9], "d1": out[10], "d2": out[11], "gradient": out[12:24].reshape(4, 3).copy()}
+
+
+def ipc_friction(slip, eps, mu, force, precision=64):
+    out = np.empty(2)
+    lib.jgs2o_ipc_friction(precision, float(slip), float(eps), float(mu), float(force), out.ctypes.data_as(f64p))
+    return out[0], out[1]

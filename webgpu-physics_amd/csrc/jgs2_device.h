@@ -27,6 +27,8 @@ struct PassParams {
   float tangential_damping, contact_margin, ipc_dhat, ipc_kappa;
   float residual_tol, update_tol, scene_scale;
   uint32_t energy_after_sweep;  // vertex-energy pass that replaces the in-solve one
+  uint32_t contact;             // IPC enabled for this step (ipcEnabled)
+  uint32_t pad3, pad4, pad5;
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
@@ -47,6 +49,14 @@ struct PassParams {
 //   hinge_rest float4[NH] {rest angle, rest edge length, owner bits, 0}
 //   tri_rows/hinge_rows uint32[V+1], *_elems: vertex -> element CSR, rows
 //                        ascending in element id
+//   IPC candidates (NC of them, the first NVT vertex-triangle):
+//   cand      uint4[NC]  vertex ids (device order)
+//   lag_n, lag_w, cscratch, src_c, src_w, tgt_c, tgt_w  float4[NC]: lagged
+//                        normal + force and weights, step scratch, cached
+//                        source/target contact (normal, distance) + weights
+//   contact_ctl float4[1] w = the step's safe alpha
+//   cand_rows uint32[V+1], cand_ids uint32[4 NC]: vertex -> candidate CSR;
+//   active_count uint32[V], active_ids uint32[4 NC]: the active rows
 struct DeviceState {
   uint32_t V = 0, T = 0, K = 0, bodies = 1;
   float4* vtx_rest = nullptr;
@@ -90,6 +100,14 @@ struct DeviceState {
   uint32_t* tri_elems = nullptr;
   uint32_t* hinge_rows = nullptr;
   uint32_t* hinge_elems = nullptr;
+  // IPC contact candidates.
+  uint32_t NC = 0, NVT = 0;
+  float ipc_dmin = 0, ipc_mu = 0, ipc_eps = 0, ipc_safety = 0;
+  uint4* cand = nullptr;
+  float4 *lag_n = nullptr, *lag_w = nullptr, *cscratch = nullptr;
+  float4 *src_c = nullptr, *src_w = nullptr, *tgt_c = nullptr, *tgt_w = nullptr;
+  float4* contact_ctl = nullptr;
+  uint32_t *cand_rows = nullptr, *cand_ids = nullptr, *active_count = nullptr, *active_ids = nullptr;
 };
 
 constexpr uint32_t kHistoryCapacity = 64;
@@ -108,7 +126,12 @@ void launch_body_correction(const DeviceState& s, const PassParams& p, hipStream
 void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_clear_frame(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_debug_local_systems(const DeviceState& s, const PassParams& p, hipStream_t stream);
-uint32_t partial_blocks_needed(uint32_t V, uint32_t elements);
+// IPC passes (encodeFrame's contact dispatches, jgs2-solver.ts:2231-2400).
+void launch_lag_contact(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_build_active_rows(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_promote_contact(const DeviceState& s, const PassParams& p, hipStream_t stream);
+uint32_t partial_blocks_needed(uint32_t V, uint32_t elements, uint32_t candidates);
 size_t partial_bytes(uint32_t blocks);
 
 }  // namespace jgs2

@@ -90,7 +90,9 @@ void free_device(DeviceState& d) {
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
-                  d.hinge_rest, d.tri_rows, d.tri_elems, d.hinge_rows, d.hinge_elems};
+                  d.hinge_rest, d.tri_rows, d.tri_elems, d.hinge_rows, d.hinge_elems, d.cand, d.lag_n, d.lag_w,
+                  d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_rows, d.cand_ids,
+                  d.active_count, d.active_ids};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   d = DeviceState{};
@@ -391,6 +393,27 @@ jgs2_status validate_cloth(const jgs2_cloth_input& c, uint32_t V) {
   return JGS2_OK;
 }
 
+// validateIpcContactCandidatesForGpu and the incidence packer's range check
+// (ipc-contact-layout.ts:117-178, :209-218).
+jgs2_status validate_contact(const jgs2_contact_input& c, uint32_t V) {
+  const uint64_t n = uint64_t(c.vertex_triangle_count) + c.edge_edge_count;
+  if (n > 0x3fffffffull) return fail(JGS2_ERROR_RANGE, "Total IPC candidate count must be an unsigned 32-bit integer.");
+  if (n && !c.packed_indices) return fail(JGS2_ERROR_TYPE, "Packed IPC indices must not be NULL.");
+  for (uint64_t k = 0; k < n; ++k) {
+    const uint32_t* ix = &c.packed_indices[4 * k];
+    const char* type = k < c.vertex_triangle_count ? "VT" : "EE";
+    for (int l = 0; l < 4; ++l) {
+      if (ix[l] >= V)
+        return fail(JGS2_ERROR_RANGE, "IPC candidate references vertex %u, but vertexCount is %u.", ix[l], V);
+      for (int q = 0; q < l; ++q)
+        if (ix[q] == ix[l])
+          return fail(JGS2_ERROR_RANGE, "%s candidate %llu must reference four distinct vertices.", type,
+                      (unsigned long long)k);
+    }
+  }
+  return JGS2_OK;
+}
+
 // Vertex -> element CSR in device vertex order: the caller-order CSR
 // (buildIncidentElementCsr, cloth-layout.ts:183-207) with its rows moved to
 // their device vertex; each row stays ascending in element id.
@@ -435,8 +458,8 @@ double min_input_determinant(const jgs2_scene_input& in) {
 }
 
 // buildJGS2InputVertexColoring (jgs2-solver.ts:308-367): conflict groups of
-// every tetrahedron, every positive cubature record's {vertex} + tet, and
-// every cloth triangle and hinge.
+// every tetrahedron, every positive cubature record's {vertex} + tet, every
+// cloth triangle and hinge, and every IPC candidate.
 jgs2_status color_vertices(const jgs2_scene_input& in, std::vector<uint32_t>& colors, uint32_t& count) {
   const uint32_t V = in.vertex_count, T = in.tet_count, K = in.cubature_k;
   std::vector<uint32_t> offsets, members;
@@ -463,6 +486,13 @@ jgs2_status color_vertices(const jgs2_scene_input& in, std::vector<uint32_t>& co
     }
     for (uint32_t h = 0; h < in.cloth->hinge_count; ++h) {
       for (int l = 0; l < 4; ++l) members.push_back(in.cloth->hinge_indices[4 * size_t(h) + l]);
+      offsets.push_back(uint32_t(members.size()));
+    }
+  }
+  if (in.contact) {  // every IPC candidate (jgs2-solver.ts:352-364)
+    const uint32_t n = in.contact->vertex_triangle_count + in.contact->edge_edge_count;
+    for (uint32_t c = 0; c < n; ++c) {
+      for (int l = 0; l < 4; ++l) members.push_back(in.contact->packed_indices[4 * size_t(c) + l]);
       offsets.push_back(uint32_t(members.size()));
     }
   }
@@ -551,6 +581,8 @@ PassParams make_params(const jgs2_solver* s, const jgs2_settings& st, bool stop)
   p.corotated = s->corotated ? 1u : 0u;
   p.tri_count = s->dev.NT;
   p.hinge_count = s->dev.NH;
+  // contactEnabled (jgs2-solver.ts:2213-2214).
+  p.contact = (s->dev.NC > 0 && st.ipc_barrier_stiffness > 0) ? 1u : 0u;
   // packUniforms (jgs2-solver.ts:604-730): f64 expressions rounded to f32.
   const double inv = 1.0 / st.timestep;
   p.dt = round_f32(st.timestep);
@@ -594,13 +626,16 @@ void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations,
   const bool gs = st.schedule == JGS2_SCHEDULE_GRAPH_COLORED_GAUSS_SEIDEL;
   base.src = 0;
   base.tgt = 0;
+  const bool contact = base.contact != 0u;
   if (s->globalization) launch_clear_frame(d, base, q);
   launch_predict(d, base, q);
+  if (contact) launch_lag_contact(d, base, q);
   if (s->globalization) launch_reduce_candidate(d, base, q);  // source feasibility preflight
   for (uint32_t it = 0; it < iterations; ++it) {
     PassParams p = base;
     p.src = (it % 2 == 0) ? 1u : 0u;
     p.tgt = (it % 2 == 0) ? 0u : 1u;
+    if (contact) launch_build_active_rows(d, p, q);
     if (gs) {
       launch_copy_position(d, p, q);
       for (uint32_t c = 0; c < s->color_count; ++c) {
@@ -613,7 +648,7 @@ void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations,
         launch_vertex_polar(d, pc, q);
         record_solve(s, pc);
       }
-      if (s->globalization) launch_vertex_energy(d, p, q);
+      if (s->globalization && !contact) launch_vertex_energy(d, p, q);
     } else {
       // Sweeps after the first read the frames the previous convergence pass
       // computed at this sweep's source pose.
@@ -623,15 +658,20 @@ void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations,
       PassParams ps = p;
       ps.fused_rot = fused ? 1u : 0u;
       record_solve(s, ps);
-      if (s->globalization && (d.NT || d.NH)) {
+      if (s->globalization && (d.NT || d.NH) && !contact) {
         PassParams pe = p;
         pe.energy_after_sweep = 1;
         launch_vertex_energy(d, pe, q);
       }
     }
+    if (contact) {  // safe step, validation, then the energy pair (jgs2-solver.ts:2347-2400)
+      launch_contact_step(d, p, q);
+      if (s->globalization) launch_vertex_energy(d, p, q);
+    }
     if (s->globalization) {
       launch_reduce_candidate(d, p, q);
       launch_apply_candidate(d, p, q);
+      if (contact) launch_promote_contact(d, p, q);
       PassParams pc = p;
       pc.compute_next_rot = (!gs && it + 1 < iterations) ? 1u : 0u;
       launch_convergence(d, pc, q);
@@ -663,9 +703,16 @@ jgs2_status resolve_step(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask,
                   "before changing timestep.",
                   pre, api, run);
   }
+  if (s->dev.NC > 0 && ov && (mask & JGS2_SET_IPC_MINIMUM_DISTANCE) &&
+      float(ov->ipc_minimum_distance) != float(s->defaults.ipc_minimum_distance))
+    return fail(JGS2_ERROR_RANGE,
+                "ipcMinimumDistance is fixed when an IPC solver is created; recreate the solver before changing it.");
   jgs2_status st = merge_settings(s->defaults, ov, mask, out);
   if (st != JGS2_OK) return st;
-  if (s->globalization && s->objective_flags != 0 && out.horizontal_body_correction)
+  // Objectives and mesh contact keep their solved centre-of-mass motion
+  // (jgs2-solver.ts:1994-2008).
+  const bool contact = s->dev.NC > 0 && out.ipc_barrier_stiffness > 0;
+  if (s->globalization && (s->objective_flags != 0 || contact) && out.horizontal_body_correction)
     out.horizontal_body_correction = 0;
   return JGS2_OK;
 }
@@ -832,6 +879,21 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
     if (st != JGS2_OK) return st;
     if (NT > 0 && !globalization)  // jgs2-solver.ts:998-1001
       return fail(JGS2_ERROR_RANGE, "Triangle cloth requires the stable Neo-Hookean production path.");
+  }
+  const uint32_t NC = in.contact ? in.contact->vertex_triangle_count + in.contact->edge_edge_count : 0u;
+  if (in.contact) {
+    st = validate_contact(*in.contact, V);
+    if (st != JGS2_OK) return st;
+  }
+  if (NC > 0) {  // initially feasible contact distances (jgs2-solver.ts:1041-1066)
+    const double minimum = min_static_contact_distance(in.positions, *in.contact);
+    const double margin = 8.0 * 0x1p-24 * std::max({1.0, resolved.ipc_activation_distance, std::fabs(minimum)});
+    const double required = resolved.ipc_minimum_distance + margin;
+    if (!std::isfinite(minimum) || !(minimum > required))
+      return fail(JGS2_ERROR_RANGE,
+                  "IPC candidates require an initially feasible source pose with distance greater than %.17g "
+                  "(minimum plus an f32 safety margin); got %.17g.",
+                  required, minimum);
   }
   std::vector<uint32_t> colors;
   uint32_t color_count = 0;
@@ -1061,6 +1123,23 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
     d.NT = NT;
     d.NH = NH;
   }
+  // IPC candidate records in device vertex ids; the contact parameters are
+  // fixed at create (jgs2-solver.ts:1096-1105).
+  std::vector<uint4> cand(NC);
+  std::vector<uint32_t> cand_rows, cand_ids;
+  if (NC > 0) {
+    for (uint32_t c = 0; c < NC; ++c) {
+      const uint32_t* ix = &in.contact->packed_indices[4 * size_t(c)];
+      cand[c] = make_uint4(vnew[ix[0]], vnew[ix[1]], vnew[ix[2]], vnew[ix[3]]);
+    }
+    if (jgs2_status e = incident_csr(in.contact->packed_indices, 4, NC, vold, cand_rows, cand_ids)) return cleanup(e);
+    d.NC = NC;
+    d.NVT = in.contact->vertex_triangle_count;
+  }
+  d.ipc_dmin = float(resolved.ipc_minimum_distance);
+  d.ipc_mu = float(resolved.ipc_friction_coefficient);
+  d.ipc_eps = float(resolved.ipc_friction_velocity_epsilon);
+  d.ipc_safety = float(resolved.ipc_step_safety);
   TRY(dalloc(&d.vtx_rest, V));
   TRY(dalloc(&d.vtx_info, V));
   TRY(dalloc(&d.tet_idx, T));
@@ -1098,7 +1177,16 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
   TRY(dalloc(&d.conv, 5 * size_t(V)));
   TRY(dalloc(&d.control, 4));
   TRY(dalloc(&d.history, size_t(kHistoryCapacity) * 8));
-  d.partial_capacity = partial_blocks_needed(V, T + NT);
+  if (NC > 0) {
+    TRY(dalloc(&d.cand, NC));
+    for (float4** a : {&d.lag_n, &d.lag_w, &d.cscratch, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRY(dalloc(a, NC));
+    TRY(dalloc(&d.contact_ctl, 1));
+    TRY(dalloc(&d.cand_rows, size_t(V) + 1));
+    TRY(dalloc(&d.cand_ids, 4 * size_t(NC)));
+    TRY(dalloc(&d.active_count, V));
+    TRY(dalloc(&d.active_ids, 4 * size_t(NC)));
+  }
+  d.partial_capacity = partial_blocks_needed(V, T + NT, NC);
   TRY(dalloc(reinterpret_cast<char**>(&d.partials), partial_bytes(d.partial_capacity)));
   hipStream_t q = s->stream;
   TRY_HIP(hipMemcpyAsync(d.vtx_rest, vtx_rest.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
@@ -1126,6 +1214,24 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
     TRY_HIP(up(d.tri_elems, tri_elems.data(), tri_elems.size() * 4));
     TRY_HIP(up(d.hinge_rows, hinge_rows.data(), hinge_rows.size() * 4));
     TRY_HIP(up(d.hinge_elems, hinge_elems.data(), hinge_elems.size() * 4));
+  }
+  if (NC > 0) {  // packIpcContactBuffer initial records (ipc-contact-layout.ts:302-341)
+    std::vector<float4> zero4(NC, make_float4(0, 0, 0, 0)), scratch(NC, make_float4(1, 0, 0, 0)),
+        none(NC, make_float4(0, 0, 0, -1));
+    const float4 ctl = make_float4(0, 0, 0, 1);
+    std::vector<uint32_t> zero_counts(V, 0);
+    TRY_HIP(hipMemcpyAsync(d.cand, cand.data(), NC * sizeof(uint4), hipMemcpyHostToDevice, q));
+    for (float4* a : {d.lag_n, d.lag_w, d.src_w, d.tgt_w})
+      TRY_HIP(hipMemcpyAsync(a, zero4.data(), NC * sizeof(float4), hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.cscratch, scratch.data(), NC * sizeof(float4), hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.src_c, none.data(), NC * sizeof(float4), hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.tgt_c, none.data(), NC * sizeof(float4), hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.contact_ctl, &ctl, sizeof(float4), hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.cand_rows, cand_rows.data(), cand_rows.size() * 4, hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.cand_ids, cand_ids.data(), cand_ids.size() * 4, hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.active_count, zero_counts.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
+    TRY_HIP(hipMemcpyAsync(d.active_ids, cand_ids.data(), cand_ids.size() * 4, hipMemcpyHostToDevice, q));
+    TRY_HIP(hipStreamSynchronize(q));
   }
   for (float4* region : {d.pos[0], d.pos[1], d.predicted, d.old})
     TRY_HIP(hipMemcpyAsync(region, positions.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
@@ -1422,6 +1528,14 @@ jgs2_status jgs2_debug_local_systems(jgs2_solver* s, const jgs2_settings* ov, ui
   PassParams p = make_params(s, r, false);
   p.src = 0;
   p.tgt = 0;
+  if (p.contact) {
+    // Contact state of a frame start at the current pose: old = x, lagged
+    // contact and active rows. The next frame's predict and lag pass rewrite
+    // all of it, so stepping continues unaffected.
+    HIP_CHECK(hipMemcpyAsync(d.old, d.pos[0], s->V * sizeof(float4), hipMemcpyDeviceToDevice, s->stream));
+    launch_lag_contact(d, p, s->stream);
+    launch_build_active_rows(d, p, s->stream);
+  }
   if (s->corotated) launch_tet_polar(d, p, s->stream);
   launch_vertex_polar(d, p, s->stream);
   launch_debug_local_systems(d, p, s->stream);

@@ -13,4 +13,8 @@ namespace jgs2 {
 jgs2_status fail(jgs2_status status, const char* format, ...);
 void clear_error();
 
+// minimumStaticIpcContactDistance (src/simulation/cpu/ipc-contact.ts:349-391)
+// in f64 over positions with 4 floats per vertex.
+double min_static_contact_distance(const float* positions4, const jgs2_contact_input& contact);
+
 }  // namespace jgs2

@@ -550,8 +550,329 @@ __device__ __forceinline__ f3 cloth_gradient(const DeviceState& s, const float4*
   return g;
 }
 
-template <bool kCloth>
-__device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const PassParams& p, const float4* pos, uint32_t v) {  // S:2557-2645
+// ------------------------------------------------------------------ IPC contact (IC, S:296-874)
+// Contact weights satisfy sum_i w_i x_i = point_a - point_b over a record's
+// four vertices; VT records are (point, t0, t1, t2), EE (a0, a1, b0, b1)
+// (ipc-contact-wgsl.ts:1-17). Bulk arithmetic is FP32 like the reference;
+// the barrier's log term and the safe-step fraction are formed in FP64.
+constexpr float kIpcLengthSquaredFloor = 1.0e-20f;
+constexpr float kIpcParallelRelativeEpsilon = 1.0e-6f;
+struct Contact {
+  float distance;
+  f3 normal;
+  bool valid;
+  float4 weights;
+};
+__device__ __forceinline__ f3 vmin3(f3 a, f3 b) { return mk(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
+__device__ __forceinline__ f3 vmax3(f3 a, f3 b) { return mk(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
+__device__ __forceinline__ float clamp01f(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+__device__ __forceinline__ f3 ipc_safe_unit(f3 v, f3 fb) {  // IC:56-73
+  const float ls = dot(v, v);
+  if (finite_s(ls) && ls > kIpcLengthSquaredFloor) return v * (1.0f / sqrtf(ls));
+  const float fls = dot(fb, fb);
+  if (finite_s(fls) && fls > kIpcLengthSquaredFloor) return fb * (1.0f / sqrtf(fls));
+  return mk(1, 0, 0);
+}
+__device__ __forceinline__ f3 ipc_orthogonal_unit(f3 d) {  // IC:75-86
+  const f3 a = mk(fabsf(d.x), fabsf(d.y), fabsf(d.z));
+  f3 axis = mk(1, 0, 0);
+  if (a.y <= a.x && a.y <= a.z) axis = mk(0, 1, 0);
+  else if (a.z <= a.x && a.z <= a.y) axis = mk(0, 0, 1);
+  return ipc_safe_unit(cross(d, axis), mk(1, 0, 0));
+}
+__device__ __forceinline__ Contact ipc_invalid() { return Contact{0.0f, mk(1, 0, 0), false, make_float4(0, 0, 0, 0)}; }
+__device__ __forceinline__ Contact ipc_make_contact(f3 a, f3 b, float4 w, f3 fb) {  // IC:100-125
+  if (!finite3(a) || !finite3(b) || !finite4(w)) return ipc_invalid();
+  const f3 gap = a - b;
+  const float sq = dot(gap, gap);
+  if (!finite_s(sq) || sq < 0.0f) return ipc_invalid();
+  return Contact{sqrtf(sq), ipc_safe_unit(gap, fb), true, w};
+}
+__device__ __forceinline__ void ipc_closest_on_segment(f3 q, f3 s0, f3 e, f3& pt, float& t, float& sq) {  // IC:127-141
+  const f3 edge = e - s0;
+  const float els = dot(edge, edge);
+  t = 0.0f;
+  if (finite_s(els) && els > kIpcLengthSquaredFloor) t = clamp01f(dot(q - s0, edge) / els);
+  pt = s0 + t * edge;
+  const f3 d = q - pt;
+  sq = dot(d, d);
+}
+__device__ Contact ipc_point_triangle(f3 p, f3 t0, f3 t1, f3 t2) {  // IC:146-224
+  const f3 e01 = t1 - t0, e02 = t2 - t0, e12 = t2 - t1;
+  const float s01 = dot(e01, e01), s02 = dot(e02, e02), s12 = dot(e12, e12);
+  const float largest = fmaxf(s01, fmaxf(s02, s12));
+  const f3 an = cross(e01, e02);
+  const float as = dot(an, an);
+  const float threshold = fmaxf(kIpcLengthSquaredFloor * kIpcLengthSquaredFloor,
+                                kIpcParallelRelativeEpsilon * largest * largest);
+  f3 longest = e01;
+  if (s02 > s01 && s02 >= s12) longest = e02;
+  else if (s12 > s01 && s12 > s02) longest = e12;
+  const f3 fb = ipc_safe_unit(an, ipc_orthogonal_unit(longest));
+  if (finite_s(as) && as > threshold) {
+    const float d01 = dot(e01, e02);
+    const f3 po = p - t0;
+    const float d20 = dot(po, e01), d21 = dot(po, e02);
+    const float den = s01 * s02 - d01 * d01;
+    const float b1 = (s02 * d20 - d01 * d21) / den;
+    const float b2 = (s01 * d21 - d01 * d20) / den;
+    const float b0 = 1.0f - b1 - b2;
+    if (b0 >= 0.0f && b1 >= 0.0f && b2 >= 0.0f)
+      return ipc_make_contact(p, b0 * t0 + b1 * t1 + b2 * t2, make_float4(1.0f, -b0, -b1, -b2), fb);
+  }
+  f3 p01, p12, p20;
+  float u01, u12, u20, q01, q12, q20;
+  ipc_closest_on_segment(p, t0, t1, p01, u01, q01);
+  ipc_closest_on_segment(p, t1, t2, p12, u12, q12);
+  ipc_closest_on_segment(p, t2, t0, p20, u20, q20);
+  f3 best = p01;
+  float bq = q01;
+  f3 bary = mk(1.0f - u01, u01, 0.0f);
+  if (q12 < bq) {
+    best = p12;
+    bq = q12;
+    bary = mk(0.0f, 1.0f - u12, u12);
+  }
+  if (q20 < bq) {
+    best = p20;
+    bary = mk(u20, 0.0f, 1.0f - u20);
+  }
+  return ipc_make_contact(p, best, make_float4(1.0f, -bary.x, -bary.y, -bary.z), fb);
+}
+__device__ Contact ipc_edge_edge(f3 a0, f3 a1, f3 b0, f3 b1) {  // IC:228-308
+  const f3 da = a1 - a0, db = b1 - b0, off = a0 - b0;
+  const float la = dot(da, da), lb = dot(db, db);
+  const float dthr = fmaxf(kIpcLengthSquaredFloor, kIpcParallelRelativeEpsilon * fmaxf(la, lb));
+  const bool adeg = !(la > dthr), bdeg = !(lb > dthr);
+  float pa = 0.0f, pb = 0.0f;
+  if (adeg && !bdeg) {
+    pb = clamp01f(dot(db, off) / lb);
+  } else if (!adeg) {
+    const float proj_a = dot(da, off);
+    if (bdeg) {
+      pa = clamp01f(-proj_a / la);
+    } else {
+      const float coupling = dot(da, db), proj_b = dot(db, off);
+      const float den = la * lb - coupling * coupling;
+      if (den > kIpcParallelRelativeEpsilon * la * lb) pa = clamp01f((coupling * proj_b - proj_a * lb) / den);
+      const float pbu = (coupling * pa + proj_b) / lb;
+      if (pbu < 0.0f) {
+        pb = 0.0f;
+        pa = clamp01f(-proj_a / la);
+      } else if (pbu > 1.0f) {
+        pb = 1.0f;
+        pa = clamp01f((coupling - proj_a) / la);
+      } else {
+        pb = pbu;
+      }
+    }
+  }
+  const f3 longest = lb > la ? db : da;
+  const f3 fb = ipc_safe_unit(cross(da, db), ipc_orthogonal_unit(longest));
+  return ipc_make_contact(a0 + pa * da, b0 + pb * db, make_float4(1.0f - pa, pa, -(1.0f - pb), -pb), fb);
+}
+__device__ __forceinline__ bool ipc_barrier_active(float d, float dhat) {  // IC:310-314
+  return finite_s(d) && finite_s(dhat) && d > 0.0f && d < dhat;
+}
+__device__ __forceinline__ float ipc_log_ratio(float d, float dhat) { return float(log(double(d) / double(dhat))); }
+__device__ __forceinline__ float ipc_barrier(float d, float dhat) {  // IC:316-325
+  if (!ipc_barrier_active(d, dhat)) return 0.0f;
+  const float o = d - dhat;
+  return -(o * o) * ipc_log_ratio(d, dhat);
+}
+__device__ __forceinline__ float ipc_barrier_d1(float d, float dhat) {  // IC:327-337
+  if (!ipc_barrier_active(d, dhat)) return 0.0f;
+  const float o = d - dhat;
+  return -2.0f * o * ipc_log_ratio(d, dhat) - o * o / d;
+}
+__device__ __forceinline__ float ipc_barrier_d2(float d, float dhat) {  // IC:339-351
+  if (!ipc_barrier_active(d, dhat)) return 0.0f;
+  const float o = d - dhat;
+  return -2.0f * ipc_log_ratio(d, dhat) - 4.0f * o / d + o * o / (d * d);
+}
+__device__ __forceinline__ float ipc_safe_step_cap(float ds, float dmin, float lip, float safety) {  // IC:398-422
+  if (!finite_s(ds) || !finite_s(dmin) || !finite_s(lip) || !finite_s(safety) || dmin < 0.0f || ds <= dmin ||
+      lip < 0.0f)
+    return 0.0f;
+  if (lip == 0.0f) return 1.0f;
+  const double f = fmin(fmax(double(safety), 0.0), 1.0) * (double(ds) - double(dmin)) / double(lip);
+  return float(fmin(fmax(f, 0.0), 1.0));
+}
+__device__ __forceinline__ float ipc_friction_f1(float slip, float eps) {  // IC:439-446
+  const float s = fmaxf(slip, 0.0f);
+  if (!(eps > 0.0f) || s >= eps) return s > 0.0f ? 1.0f : 0.0f;
+  const float n = s / eps;
+  return n * (2.0f - n);
+}
+__device__ __forceinline__ float ipc_friction_dissipation(float slip, float eps, float mu, float force) {  // IC:448-472
+  if (!finite_s(slip) || !finite_s(eps) || !finite_s(mu) || !finite_s(force)) return 0.0f;
+  const float s = fmaxf(slip, 0.0f);
+  const float scale = fmaxf(mu, 0.0f) * fmaxf(force, 0.0f);
+  if (!(eps > 0.0f)) return scale * s;
+  if (s < eps) {
+    const float n = s / eps;
+    return scale * eps * n * n * (1.0f - n / 3.0f);
+  }
+  return scale * (s - eps / 3.0f);
+}
+__device__ __forceinline__ float ipc_rounding_reach(const PassParams& p) {  // S:509-513
+  return fmaxf(fmaxf(p.scene_scale, p.ipc_dhat), 1.0e-12f) * kUnitRoundoff * kPositionResolutionMultiplier;
+}
+__device__ __forceinline__ float ipc_aabb_lower_bound(bool vt, f3 p0, f3 p1, f3 p2, f3 p3) {  // S:525-550
+  f3 gap;
+  if (vt) {
+    const f3 mn = vmin3(p1, vmin3(p2, p3)), mx = vmax3(p1, vmax3(p2, p3));
+    gap = vmax3(vmax3(mn - p0, p0 - mx), mk(0, 0, 0));
+  } else {
+    const f3 mn0 = vmin3(p0, p1), mx0 = vmax3(p0, p1), mn1 = vmin3(p2, p3), mx1 = vmax3(p2, p3);
+    gap = vmax3(vmax3(mn1 - mx0, mn0 - mx1), mk(0, 0, 0));
+  }
+  return sqrtf(dot(gap, gap));
+}
+// ipcContactFromPositions / ipcContactAtThreshold / ipcContactAtOverride
+// (S:564-650): the AABB gap stands in for distances at or beyond `threshold`.
+__device__ Contact ipc_contact_at(const DeviceState& s, uint32_t c, const float4* pos, float threshold,
+                                  uint32_t mv = kNoVertex, f3 mp = f3{0, 0, 0}) {
+  const uint4 ix = s.cand[c];
+  const bool vt = c < s.NVT;
+  const f3 p0 = cloth_pos(pos, ix.x, mv, mp), p1 = cloth_pos(pos, ix.y, mv, mp);
+  const f3 p2 = cloth_pos(pos, ix.z, mv, mp), p3 = cloth_pos(pos, ix.w, mv, mp);
+  const float lb = ipc_aabb_lower_bound(vt, p0, p1, p2, p3);
+  if (finite_s(lb) && lb >= threshold) return Contact{lb, mk(1, 0, 0), true, make_float4(0, 0, 0, 0)};
+  return vt ? ipc_point_triangle(p0, p1, p2, p3) : ipc_edge_edge(p0, p1, p2, p3);
+}
+__device__ __forceinline__ float ipc_weight_for_vertex(uint4 ix, float4 w, uint32_t v) {  // S:652-663
+  float r = 0.0f;
+  if (ix.x == v) r += w.x;
+  if (ix.y == v) r += w.y;
+  if (ix.z == v) r += w.z;
+  if (ix.w == v) r += w.w;
+  return r;
+}
+__device__ __forceinline__ f3 ipc_weighted_displacement(const DeviceState& s, uint4 ix, float4 w, const float4* pos,
+                                                        uint32_t mv, f3 mp) {  // S:665-705
+  f3 r = mk(0, 0, 0);
+  r = r + w.x * (cloth_pos(pos, ix.x, mv, mp) - load_pos(s.old, ix.x));
+  r = r + w.y * (cloth_pos(pos, ix.y, mv, mp) - load_pos(s.old, ix.y));
+  r = r + w.z * (cloth_pos(pos, ix.z, mv, mp) - load_pos(s.old, ix.z));
+  r = r + w.w * (cloth_pos(pos, ix.w, mv, mp) - load_pos(s.old, ix.w));
+  return r;
+}
+__device__ __forceinline__ void ipc_store_cached(const DeviceState& s, uint32_t c, bool target, const Contact& k) {  // S:386-419
+  const bool valid = k.valid && finite_s(k.distance) && k.distance >= 0.0f && finite3(k.normal) && finite4(k.weights);
+  (target ? s.tgt_c : s.src_c)[c] =
+      valid ? make_float4(k.normal.x, k.normal.y, k.normal.z, k.distance) : make_float4(0, 0, 0, -1);
+  (target ? s.tgt_w : s.src_w)[c] = valid ? k.weights : make_float4(0, 0, 0, 0);
+}
+__device__ __forceinline__ Contact ipc_cached(const DeviceState& s, uint32_t c, bool target) {  // S:421-448
+  const float4 nd = (target ? s.tgt_c : s.src_c)[c];
+  const float4 w = (target ? s.tgt_w : s.src_w)[c];
+  const bool valid = nd.w >= 0.0f && finite4(nd) && finite4(w);
+  return Contact{valid ? nd.w : 0.0f, valid ? xyz(nd) : mk(1, 0, 0), valid, valid ? w : make_float4(0, 0, 0, 0)};
+}
+__device__ __forceinline__ bool ipc_friction_active(const DeviceState& s, uint32_t c) {
+  return s.ipc_mu > 0.0f && s.lag_n[c].w > 0.0f;
+}
+__device__ float ipc_energy_from_contact(const DeviceState& s, const PassParams& p, uint32_t c, uint4 ix,
+                                         const Contact& k, const float4* pos, uint32_t mv, f3 mp) {  // S:707-740
+  if (!k.valid || !(k.distance > s.ipc_dmin)) return kF32Max;
+  float e = p.ipc_kappa * ipc_barrier(k.distance, p.ipc_dhat);
+  const float4 lag = s.lag_n[c];
+  if (s.ipc_mu > 0.0f && lag.w > 0.0f) {
+    const f3 n = ipc_safe_unit(xyz(lag), k.normal);
+    const f3 rel = ipc_weighted_displacement(s, ix, s.lag_w[c], pos, mv, mp);
+    const f3 tang = rel - n * dot(n, rel);
+    const float slip_velocity = sqrtf(dot(tang, tang)) * p.inv_dt;
+    e += p.dt * ipc_friction_dissipation(slip_velocity, s.ipc_eps, s.ipc_mu, lag.w);
+  }
+  return e;
+}
+__device__ __forceinline__ uint32_t ipc_energy_owner(const DeviceState& s, uint4 ix) {  // S:742-748
+  if (s.vtx_info[ix.x].z == 0u) return ix.x;
+  if (s.vtx_info[ix.y].z == 0u) return ix.y;
+  if (s.vtx_info[ix.z].z == 0u) return ix.z;
+  if (s.vtx_info[ix.w].z == 0u) return ix.w;
+  return ix.x;
+}
+// Barrier and lagged-friction terms of one vertex (ipcLocalContribution,
+// S:786-874) over its active candidate row.
+__device__ void ipc_local_contribution(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
+                                       f3& g, m3& H) {
+  const uint32_t begin = s.cand_rows[v], end = begin + s.active_count[v];
+#pragma unroll 1
+  for (uint32_t i = begin; i < end; ++i) {
+    const uint32_t c = s.active_ids[i];
+    const Contact k = p.colored ? ipc_contact_at(s, c, pos, p.ipc_dhat) : ipc_cached(s, c, false);
+    if (!k.valid) continue;
+    const bool ba = ipc_barrier_active(k.distance, p.ipc_dhat);
+    const float4 lag = s.lag_n[c];
+    const bool fa = s.ipc_mu > 0.0f && lag.w > 0.0f;
+    if (!ba && !fa) continue;
+    const uint4 ix = s.cand[c];
+    const float w = ipc_weight_for_vertex(ix, k.weights, v);
+    if (w != 0.0f && ba) {
+      const float d1 = ipc_barrier_d1(k.distance, p.ipc_dhat);
+      const float d2 = ipc_barrier_d2(k.distance, p.ipc_dhat);
+      // ipcNormalGradientScalar / ipcPsdNormalHessianScalar (IC:354-385).
+      const float gs = (finite_s(w) && finite_s(p.ipc_kappa) && finite_s(d1) && p.ipc_kappa >= 0.0f) ? w * p.ipc_kappa * d1 : 0.0f;
+      g = g + gs * k.normal;
+      float hs = 0.0f;
+      if (finite_s(w) && finite_s(p.ipc_kappa) && finite_s(d2) && p.ipc_kappa > 0.0f && w != 0.0f && d2 > 0.0f) {
+        hs = p.ipc_kappa * w * w * d2;
+        if (!finite_s(hs)) hs = kF32Max;
+      }
+      const f3 u = ipc_safe_unit(k.normal, mk(1, 0, 0));
+      const float a = fmaxf(hs, 0.0f);
+      H.c0 = H.c0 + (a * u.x) * u;
+      H.c1 = H.c1 + (a * u.y) * u;
+      H.c2 = H.c2 + (a * u.z) * u;
+    }
+    const float lw = ipc_weight_for_vertex(ix, s.lag_w[c], v);
+    if (fa && lw != 0.0f) {
+      const f3 n = ipc_safe_unit(xyz(lag), k.normal);
+      const f3 rel = ipc_weighted_displacement(s, ix, s.lag_w[c], pos, kNoVertex, mk(0, 0, 0));
+      const f3 tang = rel - n * dot(n, rel);
+      const float slip = sqrtf(dot(tang, tang));
+      const float fs = s.ipc_mu * lag.w;
+      f3 dir = mk(0, 0, 0);
+      if (slip > 1.0e-12f) dir = tang / slip;
+      const float f1 = ipc_friction_f1(slip * p.inv_dt, s.ipc_eps);
+      g = g + (lw * fs * f1) * dir;
+      float tc = 2.0f * fs / fmaxf(s.ipc_eps * p.dt, 1.0e-12f);
+      if (slip > 1.0e-12f) tc = fs * f1 / slip;
+      const float cf = lw * lw * fmaxf(tc, 0.0f);
+      // (I - n n^T) scaled; n is the unit lagged normal (IC:387-395).
+      const f3 un = ipc_safe_unit(n, mk(1, 0, 0));
+      H.c0 = H.c0 + cf * (mk(1, 0, 0) - un.x * un);
+      H.c1 = H.c1 + cf * (mk(0, 1, 0) - un.y * un);
+      H.c2 = H.c2 + cf * (mk(0, 0, 1) - un.z * un);
+    }
+  }
+}
+// ipcOwnedEnergy (S:750-779): candidates owned by this vertex, read from the
+// source or (after a nonzero safe step) the target contact cache.
+__device__ float ipc_owned_energy(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
+                                  bool use_target) {
+  float r = 0.0f;
+  const uint32_t begin = s.cand_rows[v], end = begin + s.active_count[v];
+#pragma unroll 1
+  for (uint32_t i = begin; i < end; ++i) {
+    const uint32_t c = s.active_ids[i];
+    const Contact k = ipc_cached(s, c, use_target);
+    if (!k.valid) return kF32Max;
+    if (!ipc_barrier_active(k.distance, p.ipc_dhat) && !ipc_friction_active(s, c)) continue;
+    const uint4 ix = s.cand[c];
+    if (ipc_energy_owner(s, ix) != v) continue;
+    const float e = ipc_energy_from_contact(s, p, c, ix, k, pos, kNoVertex, mk(0, 0, 0));
+    if (!finite_s(e)) return kF32Max;
+    r += e;
+  }
+  return r;
+}
+
+template <bool kCloth, bool kIpc>
+__device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const PassParams& p, const float4* pos, uint32_t v,
+                                                  bool is_target) {  // S:2557-2645
   bool pin = s.vtx_info[v].z != 0u;
   f3 x = load_pos(pos, v);
   float e = 0.0f;
@@ -590,12 +911,16 @@ __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const Pa
       e += bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
     }
   }
+  if constexpr (kIpc) {  // S:2643 with ipcCachedContactAt (S:450-460)
+    const bool use_target = p.tgt != p.src && is_target && s.contact_ctl[0].w != 0.0f;
+    e += ipc_owned_energy(s, p, v, pos, use_target);
+  }
   return e;
 }
-template <bool kCloth>
+template <bool kCloth, bool kIpc>
 __device__ __forceinline__ void store_vertex_energy(const DeviceState& s, const PassParams& p, uint32_t v) {  // S:2647-2658
-  float se = vertex_energy_at<kCloth>(s, p, s.pos[p.src], v);
-  float ce = vertex_energy_at<kCloth>(s, p, s.pos[p.tgt], v);
+  float se = vertex_energy_at<kCloth, kIpc>(s, p, s.pos[p.src], v, false);
+  float ce = vertex_energy_at<kCloth, kIpc>(s, p, s.pos[p.tgt], v, true);
   bool sv = finite_s(se), cv = finite_s(ce);
   s.vertex_energy[v] = make_float4(sv ? se : 0.0f, cv ? ce : 0.0f, sv ? 1.0f : 0.0f, cv ? 1.0f : 0.0f);
 }
@@ -801,7 +1126,7 @@ __device__ void cubature_record_corot(const DeviceState& s, const PassParams& p,
 
 // Gather of jgs2Solve (S:3074-3263): inertia, objectives, the exact
 // incident elements, the floor penalty and the cubature correction.
-template <bool kCorot, bool kCloth>
+template <bool kCorot, bool kCloth, bool kIpc>
 __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_t v) {
   const float4* pos = s.pos[p.src];
   const f3 x = load_pos(pos, v);
@@ -858,6 +1183,13 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
     L.g.y += p.floor_k * (x.y - p.floor_y);
     L.H.c1.y += p.floor_k;
   }
+  if constexpr (kIpc) {  // S:3193-3195
+    f3 ig = mk(0, 0, 0);
+    m3 ih = mzero();
+    ipc_local_contribution(s, p, v, pos, ig, ih);
+    L.g = L.g + ig;
+    L.H = madd(L.H, ih);
+  }
   if (p.cubature_k == 0) return L;
   const m3 Rs_t = tr(load_rot(rot_source(s, p), v));
 #pragma unroll 1
@@ -882,7 +1214,7 @@ struct Trial {
   bool geometry_valid;
   float energy;
 };
-template <bool kCloth>
+template <bool kCloth, bool kIpc>
 __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uint32_t v, f3 x, f3 d, float initial_min) {
   const float4* pos = s.pos[p.src];
   const uint4 inf = s.vtx_info[v];
@@ -890,6 +1222,42 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
   const f3 pred = xyz(s.predicted[v]);
   float energy = quadratic_delta(x, pred, d, s.vtx_rest[v].w * p.inv_dt2) + floor_delta(p, x.y, d.y);
   if (p.objective_flags != 0u) energy += objective_delta(s, p, v, x, d);
+  if constexpr (kIpc) {
+    // Contact feasibility of the moved vertex (S:1997-2037) and the contact
+    // energy delta (S:2140-2193) in one pass over its active row.
+    const f3 tp = x + d;
+    const float dl = sqrtf(dot(d, d));
+    const uint32_t begin = s.cand_rows[v], end = begin + s.active_count[v];
+#pragma unroll 1
+    for (uint32_t i = begin; i < end; ++i) {
+      const uint32_t c = s.active_ids[i];
+      const Contact sc = p.colored ? ipc_contact_at(s, c, pos, p.ipc_dhat) : ipc_cached(s, c, false);
+      if (!sc.valid) {
+        tr.geometry_valid = false;
+        return tr;
+      }
+      const bool near = !(sc.distance - dl > s.ipc_dmin);
+      const bool may_change = sc.distance - dl < p.ipc_dhat;
+      const bool fa = ipc_friction_active(s, c);
+      if (!near && !may_change && !fa) continue;
+      Contact tc = Contact{p.ipc_dhat, mk(1, 0, 0), true, make_float4(0, 0, 0, 0)};
+      if (near || may_change) tc = ipc_contact_at(s, c, pos, p.ipc_dhat, v, tp);
+      if (near) {
+        if (!tc.valid) {
+          tr.geometry_valid = false;
+          return tr;
+        }
+        if (!(tc.distance > s.ipc_dmin)) {
+          tr.min_det = 0.0f;
+          return tr;
+        }
+      }
+      if (!may_change && !fa) continue;
+      const uint4 ix = s.cand[c];
+      energy += ipc_energy_from_contact(s, p, c, ix, tc, pos, v, tp) -
+                ipc_energy_from_contact(s, p, c, ix, sc, pos, kNoVertex, mk(0, 0, 0));
+    }
+  }
   // Incident tetrahedra: F' = F + d g^T. In closed form det(F') = J + d.b
   // and <D, F' + F> = 2 d.a + |d|^2 |g|^2 with a = F g, b = cof g; the pair
   // and triple terms of the determinant delta (GL:510-524) vanish for rank one.
@@ -1098,9 +1466,9 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassPara
 // the active color (graph-colored Gauss-Seidel, in place).
 // With cloth the assembled vertex energy reads neighbor positions, so it is
 // evaluated by a separate pass after the sweep (k_vertex_energy).
-template <bool kCorot, bool kCloth>
+template <bool kCorot, bool kCloth, bool kIpc>
 __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
-  constexpr bool kEnergyInSolve = !kCloth;
+  constexpr bool kEnergyInSolve = !kCloth && !kIpc;
   const uint32_t lane = logical_block() * kBlock + threadIdx.x;
   uint32_t v;
   if (p.colored) {
@@ -1129,7 +1497,7 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
     s.final_update[v] = make_float4(0, 0, 0, 1);
     if (p.globalization) {
       store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, kPinned);
-      if (kEnergyInSolve && !p.colored) store_vertex_energy<false>(s, p, v);
+      if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
     }
     return;
   }
@@ -1145,11 +1513,11 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
       s.final_update[v] = make_float4(0, 0, 0, 1);
       store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, amv ? pre.z : 0.0f, 0, 0,
                        amv ? kLineSearchFailed : kNonfinite);
-      if (kEnergyInSolve && !p.colored) store_vertex_energy<false>(s, p, v);
+      if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
       return;
     }
   }
-  const Local L = gather_local<kCorot, kCloth>(s, p, v);
+  const Local L = gather_local<kCorot, kCloth, kIpc>(s, p, v);
   const float inertia = s.vtx_rest[v].w * p.inv_dt2;
   if constexpr (kCorot) {  // legacy regularized solve (S:2479-2512, S:3475-3486)
     const m3& m = L.H;
@@ -1228,7 +1596,7 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
         backtracks = b;
         break;
       }
-      const Trial trial = restricted_trial<kCloth>(s, p, v, x, td, initial_min);
+      const Trial trial = restricted_trial<kCloth, kIpc>(s, p, v, x, td, initial_min);
       if (!trial.geometry_valid) {
         status = kNonfinite;
         min_trial = 0.0f;
@@ -1263,18 +1631,18 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
   s.final_update[v] = make_float4(scaled_length3(accepted_d), 0, 0, 1);
   store_local_diag(s, v, dr.dir, alpha, dr.min_eig * dr.scale, dr.scale, dr.shift * dr.scale, dr.shift, accepted_de,
                    bound, dr.gdotp, dr.residual, min_trial, backtracks, evaluations, status);
-  if (kEnergyInSolve && !p.colored) store_vertex_energy<false>(s, p, v);
+  if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
 }
 
 // assembledVertexEnergy (S:3490-3496). After a Jacobi sweep with cloth it
 // stands in for the in-solve evaluation and, like the solve, leaves the
 // records alone once the stop latch is set (`energy_after_sweep`).
-template <bool kCloth>
+template <bool kCloth, bool kIpc>
 __global__ void __launch_bounds__(kBlock) k_vertex_energy(DeviceState s, PassParams p) {
   const uint32_t v = logical_block() * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
   if (p.energy_after_sweep && stop_latched(s, p)) return;
-  store_vertex_energy<kCloth>(s, p, v);
+  store_vertex_energy<kCloth, kIpc>(s, p, v);
 }
 
 // ------------------------------------------------------------------ candidate reduction
@@ -1524,7 +1892,7 @@ __device__ __forceinline__ float max_abs(f3 v) { return fmaxf(fabsf(v.x), fmaxf(
 // (S:4119-4189): component gradients at the candidate and their scales.
 // With `compute_next_rot` the same gather also evaluates the next sweep's
 // vertexPolarRotation (S:2960-2989): its source pose is this candidate.
-template <bool kCloth>
+template <bool kCloth, bool kIpc>
 __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, PassParams p) {
   __shared__ ConvPartial sh[kBlock];
   ConvPartial r;
@@ -1581,6 +1949,12 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
         }
         comp5[4] = mk(0, 0, 0);
         if (p.floor_k > 0.0f && x.y < p.floor_y) comp5[4].y = p.floor_k * (x.y - p.floor_y);
+        if constexpr (kIpc) {  // S:4091
+          f3 ig = mk(0, 0, 0);
+          m3 ih = mzero();
+          ipc_local_contribution(s, p, v, pos, ig, ih);
+          comp5[4] = comp5[4] + ig;
+        }
       }
       for (int k = 0; k < 5; ++k) c[k] = make_float4(comp5[k].x, comp5[k].y, comp5[k].z, 0.0f);
     } else {
@@ -1811,7 +2185,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(DeviceState s, PassParams p
   s.velocity[v] = make_float4(vel.x, vel.y, vel.z, 0.0f);
 }
 
-template <bool kCorot, bool kCloth>
+template <bool kCorot, bool kCloth, bool kIpc>
 __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, PassParams p) {
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
@@ -1822,7 +2196,7 @@ __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, P
     for (int k = 0; k < 9; ++k) h[k] = 0.0f;
     return;
   }
-  const Local L = gather_local<kCorot, kCloth>(s, p, v);
+  const Local L = gather_local<kCorot, kCloth, kIpc>(s, p, v);
   g[0] = L.g.x;
   g[1] = L.g.y;
   g[2] = L.g.z;
@@ -1834,17 +2208,156 @@ __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, P
   }
 }
 
+
+// ------------------------------------------------------------------ IPC passes (S:2695-2929)
+__device__ __forceinline__ float ipc_broad_threshold(const PassParams& p) {  // S:515-517
+  return p.ipc_dhat + ipc_rounding_reach(p);
+}
+// lagContact (S:2713-2744): source contact cache of the frame-start pose
+// (posA) and the lagged normal force and weights for friction.
+__global__ void __launch_bounds__(kBlock) k_lag_contact(DeviceState s, PassParams p) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= s.NC) return;
+  const Contact k = ipc_contact_at(s, c, s.pos[0], ipc_broad_threshold(p));
+  ipc_store_cached(s, c, false, k);
+  float4 nf = make_float4(0, 0, 0, 0), w = make_float4(0, 0, 0, 0);
+  if (k.valid && k.distance > s.ipc_dmin) {
+    const float force = fmaxf(0.0f, -p.ipc_kappa * ipc_barrier_d1(k.distance, p.ipc_dhat));
+    if (finite_s(force)) {
+      nf = make_float4(k.normal.x, k.normal.y, k.normal.z, force);
+      w = k.weights;
+    }
+  }
+  s.lag_n[c] = nf;
+  s.lag_w[c] = w;
+  s.cscratch[c] = make_float4(1, 0, 0, 0);
+  if (c == 0) s.contact_ctl[0].w = 1.0f;
+}
+// buildActiveContactRows (S:2747-2785): keep candidates the bounded step can
+// bring into the barrier, with friction, or touched by unbounded pin motion.
+__global__ void __launch_bounds__(kBlock) k_build_active_rows(DeviceState s, PassParams p) {
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  const uint32_t first = s.cand_rows[v], last = s.cand_rows[v + 1];
+  const bool bounded = p.max_step > 0.0f;
+  const float reach = p.ipc_dhat + 2.0f * (fmaxf(p.max_step, 0.0f) + ipc_rounding_reach(p));
+  const float bound = fmaxf(p.max_step, 0.0f);
+  const float4* src = s.pos[p.src];
+  uint32_t n = 0;
+  for (uint32_t i = first; i < last; ++i) {
+    const uint32_t c = s.cand_ids[i];
+    const Contact k = ipc_cached(s, c, false);
+    bool include = !k.valid || !bounded || k.distance <= reach || ipc_friction_active(s, c);
+    if (!include) {  // ipcCandidatePinnedMotionExceedsBound (S:484-507)
+      const uint4 ix = s.cand[c];
+      const uint32_t vs[4] = {ix.x, ix.y, ix.z, ix.w};
+      for (int l = 0; l < 4 && !include; ++l) {
+        if (s.vtx_info[vs[l]].z == 0u) continue;
+        const f3 d = xyz(s.vtx_rest[vs[l]]) - load_pos(src, vs[l]);
+        const float dl = sqrtf(dot(d, d));
+        include = !finite3(d) || !finite_s(dl) || dl > bound;
+      }
+    }
+    if (include) s.active_ids[first + n++] = c;
+  }
+  s.active_count[v] = n;
+}
+// candidateContactStep (S:2788-2852): per-candidate conservative step cap.
+__global__ void __launch_bounds__(kBlock) k_candidate_contact_step(DeviceState s, PassParams p) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= s.NC) return;
+  if (stop_latched(s, p)) {
+    s.cscratch[c] = make_float4(1, 0, 0, 1);
+    return;
+  }
+  const uint4 ix = s.cand[c];
+  const float4* src = s.pos[p.src];
+  const float4* tgt = s.pos[p.tgt];
+  const f3 d0 = load_pos(tgt, ix.x) - load_pos(src, ix.x), d1 = load_pos(tgt, ix.y) - load_pos(src, ix.y);
+  const f3 d2 = load_pos(tgt, ix.z) - load_pos(src, ix.z), d3 = load_pos(tgt, ix.w) - load_pos(src, ix.w);
+  const float l0 = sqrtf(dot(d0, d0)), l1 = sqrtf(dot(d1, d1)), l2 = sqrtf(dot(d2, d2)), l3 = sqrtf(dot(d3, d3));
+  const float lip = c < s.NVT ? l0 + fmaxf(l1, fmaxf(l2, l3)) : fmaxf(l0, l1) + fmaxf(l2, l3);
+  Contact src_k = ipc_cached(s, c, false);
+  bool full = src_k.valid && finite_s(lip) && src_k.distance - lip > s.ipc_dmin;
+  if (!full) {
+    src_k = ipc_contact_at(s, c, src, kF32Max);
+    full = src_k.valid && finite_s(lip) && src_k.distance - lip > s.ipc_dmin;
+  }
+  const bool trial_valid = finite3(d0) && finite3(d1) && finite3(d2) && finite3(d3);
+  const bool valid = src_k.valid && trial_valid && src_k.distance > s.ipc_dmin && finite_s(lip);
+  float alpha = 0.0f;
+  if (valid) alpha = full ? 1.0f : ipc_safe_step_cap(src_k.distance, s.ipc_dmin, lip, s.ipc_safety);
+  s.cscratch[c] = make_float4(alpha, src_k.valid ? src_k.distance : 0.0f, trial_valid ? src_k.distance : 0.0f,
+                              valid ? 1.0f : 0.0f);
+}
+// validateContactStep (S:2855-2875): exact check of the scaled target pose.
+__global__ void __launch_bounds__(kBlock) k_validate_contact_step(DeviceState s, PassParams p) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= s.NC) return;
+  const Contact k = ipc_contact_at(s, c, s.pos[p.tgt], ipc_broad_threshold(p));
+  ipc_store_cached(s, c, true, k);
+  const bool valid = k.valid && finite_s(k.distance) && k.distance > s.ipc_dmin;
+  const float d = valid ? k.distance : 0.0f;
+  s.cscratch[c] = make_float4(1.0f, d, d, valid ? 1.0f : 0.0f);
+}
+// reduceContactStep (S:2878-2918): minimum alpha and an all-valid flag. Min
+// and a 0/1 product are order independent, so block partials are exact.
+__device__ __forceinline__ void contact_min(float4& a, const float4& b) {
+  a = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), fminf(a.w, b.w));
+}
+__global__ void __launch_bounds__(kBlock) k_reduce_contact_partials(DeviceState s) {
+  __shared__ float4 sh[kBlock];
+  float4 r = make_float4(1.0f, kF32Max, kF32Max, 1.0f);
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c < s.NC) {
+    const float4 sc = s.cscratch[c];
+    const bool valid = sc.w == 1.0f && finite_s(sc.x) && finite_s(sc.y) && finite_s(sc.z);
+    r = make_float4(valid ? sc.x : 0.0f, valid ? sc.y : 0.0f, valid ? sc.z : 0.0f, valid ? 1.0f : 0.0f);
+    r.x = fminf(1.0f, r.x);
+  }
+  block_reduce(r, sh, contact_min);
+  if (threadIdx.x == 0) reinterpret_cast<float4*>(s.partials)[blockIdx.x] = r;
+}
+__global__ void __launch_bounds__(kBlock) k_reduce_contact_final(DeviceState s, uint32_t blocks) {
+  __shared__ float4 sh[kBlock];
+  float4 r = make_float4(1.0f, kF32Max, kF32Max, 1.0f);
+  const float4* parts = reinterpret_cast<const float4*>(s.partials);
+  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) contact_min(r, parts[i]);
+  block_reduce(r, sh, contact_min);
+  if (threadIdx.x == 0) s.contact_ctl[0].w = r.w == 1.0f ? fminf(fmaxf(r.x, 0.0f), 1.0f) : 0.0f;
+}
+// applyContactStep (S:2921-2929).
+__global__ void __launch_bounds__(kBlock) k_apply_contact_step(DeviceState s, PassParams p) {
+  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  if (v >= p.vertex_count || stop_latched(s, p)) return;
+  const float a = fminf(fmaxf(s.contact_ctl[0].w, 0.0f), 1.0f);
+  const f3 src = load_pos(s.pos[p.src], v), c = load_pos(s.pos[p.tgt], v);
+  const f3 x = src + a * (c - src);
+  s.pos[p.tgt][v] = make_float4(x.x, x.y, x.z, 1.0f);
+}
+// promoteAcceptedContactCache (S:2695-2710).
+__global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassParams p) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= s.NC) return;
+  const float alpha = s.contact_ctl[0].w;
+  if (s.control[1].x == 1.0f && finite_s(alpha) && alpha > 0.0f) {
+    s.src_c[c] = s.tgt_c[c];
+    s.src_w[c] = s.tgt_w[c];
+  }
+}
+
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 inline bool has_cloth(const PassParams& p) { return p.tri_count != 0u || p.hinge_count != 0u; }
 
 }  // namespace
 
-uint32_t partial_blocks_needed(uint32_t V, uint32_t elements) {
+uint32_t partial_blocks_needed(uint32_t V, uint32_t elements, uint32_t candidates) {
   // Candidate partials use blocks(T + triangles) + blocks(V); convergence
-  // uses 2 * blocks(V) + 1.
+  // uses 2 * blocks(V) + 1; the contact step blocks(candidates).
   uint32_t a = blocks_for(elements) + blocks_for(V);
   uint32_t b = 2 * blocks_for(V) + 1;
-  return a > b ? a : b;
+  uint32_t c = blocks_for(candidates);
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
 }
 size_t partial_bytes(uint32_t blocks) {
   size_t a = sizeof(CandPartial), b = sizeof(ConvPartial);
@@ -1863,21 +2376,30 @@ void launch_tet_polar(const DeviceState& s, const PassParams& p, hipStream_t st)
 void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t st) {
   hipLaunchKernelGGL(k_vertex_polar, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
 }
+// Kernel variant for the step's features: co-rotated (no cloth or contact
+// terms), or stable Neo-Hookean with or without the cloth and IPC terms.
+#define JGS2_FEATURE_DISPATCH(KERNEL, GRID, ...)                                                      \
+  do {                                                                                               \
+    const bool cl = has_cloth(p), ipc = p.contact != 0u;                                            \
+    if (p.corotated) hipLaunchKernelGGL((KERNEL<true, false, false>), GRID, dim3(kBlock), 0, st, __VA_ARGS__); \
+    else if (cl && ipc) hipLaunchKernelGGL((KERNEL<false, true, true>), GRID, dim3(kBlock), 0, st, __VA_ARGS__); \
+    else if (cl) hipLaunchKernelGGL((KERNEL<false, true, false>), GRID, dim3(kBlock), 0, st, __VA_ARGS__); \
+    else if (ipc) hipLaunchKernelGGL((KERNEL<false, false, true>), GRID, dim3(kBlock), 0, st, __VA_ARGS__); \
+    else hipLaunchKernelGGL((KERNEL<false, false, false>), GRID, dim3(kBlock), 0, st, __VA_ARGS__); \
+  } while (0)
+
 void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t n = p.colored ? p.color_end - p.color_begin : p.vertex_count;
   if (n == 0) return;
-  if (p.corotated)
-    hipLaunchKernelGGL((k_solve<true, false>), dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
-  else if (has_cloth(p))
-    hipLaunchKernelGGL((k_solve<false, true>), dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
-  else
-    hipLaunchKernelGGL((k_solve<false, false>), dim3(blocks_for(n)), dim3(kBlock), 0, st, s, p);
+  JGS2_FEATURE_DISPATCH(k_solve, dim3(blocks_for(n)), s, p);
 }
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
-  if (has_cloth(p))
-    hipLaunchKernelGGL(k_vertex_energy<true>, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
-  else
-    hipLaunchKernelGGL(k_vertex_energy<false>, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+  const dim3 grid(blocks_for(p.vertex_count));
+  const bool cl = has_cloth(p), ipc = p.contact != 0u;
+  if (cl && ipc) hipLaunchKernelGGL((k_vertex_energy<true, true>), grid, dim3(kBlock), 0, st, s, p);
+  else if (cl) hipLaunchKernelGGL((k_vertex_energy<true, false>), grid, dim3(kBlock), 0, st, s, p);
+  else if (ipc) hipLaunchKernelGGL((k_vertex_energy<false, true>), grid, dim3(kBlock), 0, st, s, p);
+  else hipLaunchKernelGGL((k_vertex_energy<false, false>), grid, dim3(kBlock), 0, st, s, p);
 }
 void launch_reduce_candidate(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t tb = blocks_for(p.tet_count + p.tri_count), vb = blocks_for(p.vertex_count);
@@ -1889,10 +2411,11 @@ void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream
 }
 void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t vb = blocks_for(p.vertex_count);
-  if (has_cloth(p))
-    hipLaunchKernelGGL(k_convergence_gradient<true>, dim3(vb), dim3(kBlock), 0, st, s, p);
-  else
-    hipLaunchKernelGGL(k_convergence_gradient<false>, dim3(vb), dim3(kBlock), 0, st, s, p);
+  const bool cl = has_cloth(p), ipc = p.contact != 0u;
+  if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient<true, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
+  else if (cl) hipLaunchKernelGGL((k_convergence_gradient<true, false>), dim3(vb), dim3(kBlock), 0, st, s, p);
+  else if (ipc) hipLaunchKernelGGL((k_convergence_gradient<false, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
+  else hipLaunchKernelGGL((k_convergence_gradient<false, false>), dim3(vb), dim3(kBlock), 0, st, s, p);
   hipLaunchKernelGGL(k_convergence_scales, dim3(1), dim3(kBlock), 0, st, s, vb);
   hipLaunchKernelGGL(k_convergence_sums, dim3(vb), dim3(kBlock), 0, st, s, p, vb);
   hipLaunchKernelGGL(k_convergence_final, dim3(1), dim3(kBlock), 0, st, s, p, vb);
@@ -1909,13 +2432,32 @@ void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t st) 
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
 }
 void launch_debug_local_systems(const DeviceState& s, const PassParams& p, hipStream_t st) {
-  const dim3 grid(blocks_for(p.vertex_count));
-  if (p.corotated)
-    hipLaunchKernelGGL((k_debug_local_systems<true, false>), grid, dim3(kBlock), 0, st, s, p);
-  else if (has_cloth(p))
-    hipLaunchKernelGGL((k_debug_local_systems<false, true>), grid, dim3(kBlock), 0, st, s, p);
-  else
-    hipLaunchKernelGGL((k_debug_local_systems<false, false>), grid, dim3(kBlock), 0, st, s, p);
+  JGS2_FEATURE_DISPATCH(k_debug_local_systems, dim3(blocks_for(p.vertex_count)), s, p);
+}
+void launch_lag_contact(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  if (s.NC == 0) return;
+  hipLaunchKernelGGL(k_lag_contact, dim3(blocks_for(s.NC)), dim3(kBlock), 0, st, s, p);
+}
+void launch_build_active_rows(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_build_active_rows, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+}
+// The six contact step dispatches of one iteration (jgs2-solver.ts:2347-2388):
+// candidate caps, reduce, apply, exact validation, reduce, apply.
+void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  const uint32_t cb = blocks_for(s.NC), vb = blocks_for(p.vertex_count);
+  if (s.NC == 0) return;
+  hipLaunchKernelGGL(k_candidate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_reduce_contact_partials, dim3(cb), dim3(kBlock), 0, st, s);
+  hipLaunchKernelGGL(k_reduce_contact_final, dim3(1), dim3(kBlock), 0, st, s, cb);
+  hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_validate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_reduce_contact_partials, dim3(cb), dim3(kBlock), 0, st, s);
+  hipLaunchKernelGGL(k_reduce_contact_final, dim3(1), dim3(kBlock), 0, st, s, cb);
+  hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p);
+}
+void launch_promote_contact(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  if (s.NC == 0) return;
+  hipLaunchKernelGGL(k_promote_contact, dim3(blocks_for(s.NC)), dim3(kBlock), 0, st, s, p);
 }
 
 }  // namespace jgs2

@@ -10,25 +10,7 @@
 #include <unordered_set>
 #include <vector>
 
-#include "jgs2_internal.h"
-#include "jgs2hip_scene.h"
-
-struct jgs2_scene {
-  uint32_t vertex_count = 0;
-  uint32_t tet_count = 0;
-  uint32_t cubature_k = 0;
-  std::vector<float> positions, vertex_rest;
-  std::vector<uint32_t> vertex_info, tet_indices, adjacency;
-  std::vector<float> tet_inverse_dm, tet_meta, tet_rest_stiffness;
-  std::vector<uint32_t> cubature_tet_ids;
-  std::vector<float> cubature_weights, cubature_basis;
-  std::vector<float> forces, targets, stiffnesses;
-  // Optional cloth layer.
-  bool has_cloth = false;
-  std::vector<uint32_t> cloth_triangles, cloth_hinges;
-  std::vector<float> cloth_inverse, cloth_areas, hinge_angles, hinge_lengths;
-  jgs2_cloth_input cloth{};
-};
+#include "jgs2_scene_internal.h"
 
 namespace {
 
@@ -123,6 +105,203 @@ void linear_stiffness(const double* inv, double volume, double young, double poi
 }
 
 }  // namespace
+
+namespace jgs2 {
+
+void cuboid_grid(const uint32_t cells[3], const double origin[3], const double size[3], uint32_t material,
+                 uint32_t body, bool fixed, Mesh& m) {
+  const uint64_t nx = cells[0], ny = cells[1], nz = cells[2];
+  const uint64_t sy = nx + 1, sz = (nx + 1) * (ny + 1);
+  const size_t V = size_t((nx + 1) * (ny + 1) * (nz + 1)), T = size_t(nx * ny * nz * 6);
+  auto vid = [&](uint64_t x, uint64_t y, uint64_t z) { return uint32_t(x + y * sy + z * sz); };
+  m.x.assign(3 * V, 0.0);
+  m.fixed.assign(V, fixed ? 1 : 0);
+  m.body.assign(V, body);
+  m.tets.resize(4 * T);
+  m.material.assign(T, material);
+  for (uint64_t z = 0; z <= nz; ++z)  // mesh.ts:131-146
+    for (uint64_t y = 0; y <= ny; ++y)
+      for (uint64_t xx = 0; xx <= nx; ++xx) {
+        const uint32_t v = vid(xx, y, z);
+        const uint64_t g[3] = {xx, y, z};
+        for (int a = 0; a < 3; ++a) m.x[3 * size_t(v) + a] = origin[a] + (size[a] * double(g[a])) / double(cells[a]);
+      }
+  size_t o = 0;
+  for (uint64_t z = 0; z < nz; ++z)  // mesh.ts:152-191
+    for (uint64_t y = 0; y < ny; ++y)
+      for (uint64_t xx = 0; xx < nx; ++xx) {
+        const uint32_t v000 = vid(xx, y, z), v100 = vid(xx + 1, y, z), v010 = vid(xx, y + 1, z),
+                       v110 = vid(xx + 1, y + 1, z), v001 = vid(xx, y, z + 1), v101 = vid(xx + 1, y, z + 1),
+                       v011 = vid(xx, y + 1, z + 1), v111 = vid(xx + 1, y + 1, z + 1);
+        const uint32_t cell[24] = {v000, v100, v110, v111, v000, v110, v010, v111, v000, v010, v011, v111,
+                                   v000, v011, v001, v111, v000, v001, v101, v111, v000, v101, v100, v111};
+        std::memcpy(&m.tets[o], cell, sizeof(cell));
+        o += 24;
+      }
+}
+
+jgs2_status orient(Mesh& m) {
+  for (uint32_t t = 0; t < m.tet_count(); ++t) {
+    uint32_t* ix = &m.tets[4 * size_t(t)];
+    double r[9];
+    rest_matrix(m.x, ix, r);
+    const double det = determinant3(r);
+    if (std::fabs(det) <= 1e-14) return fail(JGS2_ERROR_RANGE, "Tetrahedron %u is degenerate.", t);
+    if (det < 0) std::swap(ix[1], ix[2]);
+  }
+  return JGS2_OK;
+}
+
+void append(Mesh& into, const Mesh& m) {
+  const uint32_t off = into.vertex_count();
+  into.x.insert(into.x.end(), m.x.begin(), m.x.end());
+  into.fixed.insert(into.fixed.end(), m.fixed.begin(), m.fixed.end());
+  into.body.insert(into.body.end(), m.body.begin(), m.body.end());
+  into.material.insert(into.material.end(), m.material.begin(), m.material.end());
+  for (uint32_t i : m.tets) into.tets.push_back(i + off);
+}
+
+jgs2_status pack_scene(const Mesh& m, const std::vector<Material>& materials, uint32_t cubature_k,
+                       bool rest_stiffness, double cubature_h, jgs2_scene* s) {
+  const uint32_t V = m.vertex_count(), T = m.tet_count();
+  s->vertex_count = V;
+  s->tet_count = T;
+  s->cubature_k = cubature_k;
+  s->tet_indices = m.tets;
+  const std::vector<double>& x = m.x;
+  // Rest data (fem.ts:167-209), lumped masses (fem.ts:211-238) and the
+  // packed tetrahedron records (scenes/index.ts:1019-1050).
+  s->tet_inverse_dm.assign(12 * size_t(T), 0.0f);
+  s->tet_meta.resize(4 * size_t(T));
+  if (rest_stiffness) s->tet_rest_stiffness.resize(144 * size_t(T));
+  std::vector<double> mass(V, 0.0);
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint32_t* ix = &s->tet_indices[4 * size_t(t)];
+    double r[9], inv[9];
+    rest_matrix(x, ix, r);
+    const double volume = std::fabs(determinant3(r)) / 6.0;
+    if (!(volume > 1e-14) || !invert3(r, inv)) return fail(JGS2_ERROR_RANGE, "Cannot assemble a degenerate tetrahedron.");
+    for (int c = 0; c < 3; ++c)
+      for (int q = 0; q < 3; ++q) s->tet_inverse_dm[12 * size_t(t) + 4 * c + q] = float(inv[3 * q + c]);
+    if (m.material[t] >= materials.size()) return fail(JGS2_ERROR_RANGE, "Tetrahedron %u has no material.", t);
+    const Material& mat = materials[m.material[t]];
+    const double E = mat.young, nu = mat.poisson;
+    double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+    double mu = E / (2 * (1 + nu));
+    if (mat.model == JGS2_MATERIAL_STABLE_NEO_HOOKEAN) {
+      // computeStableNeoHookeanParameters (stable-neo-hookean.ts:68-109).
+      const double adj_mu = (4.0 / 3.0) * mu;
+      const double adj_lam = lam + (5.0 / 6.0) * mu;
+      lam = adj_lam;
+      mu = adj_mu;
+    }
+    s->tet_meta[4 * size_t(t)] = float(volume);
+    s->tet_meta[4 * size_t(t) + 1] = float(lam);
+    s->tet_meta[4 * size_t(t) + 2] = float(mu);
+    s->tet_meta[4 * size_t(t) + 3] = float(mat.model);
+    if (rest_stiffness) linear_stiffness(inv, volume, E, nu, &s->tet_rest_stiffness[144 * size_t(t)]);
+    const double vm = mat.density * volume / 4.0;
+    for (int l = 0; l < 4; ++l) mass[ix[l]] += vm;
+  }
+  // Canonical CSR and vertex records (scenes/index.ts:975-1017).
+  std::vector<uint32_t> starts(V), counts(V);
+  s->adjacency.resize(4 * size_t(T));
+  jgs2_build_adjacency(V, T, s->tet_indices.data(), starts.data(), counts.data(), s->adjacency.data());
+  s->positions.resize(4 * size_t(V));
+  s->vertex_rest.resize(4 * size_t(V));
+  s->vertex_info.resize(4 * size_t(V));
+  for (uint32_t v = 0; v < V; ++v) {
+    for (int a = 0; a < 3; ++a) {
+      s->positions[4 * size_t(v) + a] = float(x[3 * size_t(v) + a]);
+      s->vertex_rest[4 * size_t(v) + a] = float(x[3 * size_t(v) + a]);
+    }
+    s->positions[4 * size_t(v) + 3] = 1.0f;
+    s->vertex_rest[4 * size_t(v) + 3] = float(mass[v]);
+    s->vertex_info[4 * size_t(v)] = starts[v];
+    s->vertex_info[4 * size_t(v) + 1] = counts[v];
+    s->vertex_info[4 * size_t(v) + 2] = m.fixed[v];
+    s->vertex_info[4 * size_t(v) + 3] = m.body[v];
+  }
+  // Cubature records. The reference trains them offline per scene with a
+  // dense exact basis and a nonnegative least-squares selection
+  // (src/simulation/cpu/precompute.ts:141-380); the step itself only
+  // consumes fixed-width records (layout.ts:24-27). This generator writes
+  // records of the same form and role, deterministically and in O(V):
+  //   * up to K-2 samples on incident tetrahedra (weight 1), spread evenly
+  //     over the vertex's CSR row;
+  //   * the remaining samples on second-ring tetrahedra that are not
+  //     incident (weight 1/4), which the step treats without the exact-block
+  //     subtraction;
+  //   * each sample's four Ubar blocks are phi(r) (I + a d e^T) with
+  //     r = |x_j - x_i| / h, phi = exp(-r^2), d the unit offset, e a fixed
+  //     axis and a = 0.1, so Ubar is I at the source vertex, decays with
+  //     distance, is zero on pinned vertices, and is not symmetric.
+  const uint32_t K = cubature_k;
+  const size_t R = size_t(V) * K;
+  s->cubature_tet_ids.assign(R, 0xffffffffu);
+  s->cubature_weights.assign(R, 0.0f);
+  s->cubature_basis.assign(R * 36, 0.0f);
+  if (K > 0) {
+    const double hh = cubature_h;
+    const double e[3] = {1.0 / std::sqrt(14.0), 2.0 / std::sqrt(14.0), 3.0 / std::sqrt(14.0)};
+    for (uint32_t v = 0; v < V; ++v) {
+      if (m.fixed[v] || counts[v] == 0) continue;
+      const uint32_t* row = &s->adjacency[starts[v]];
+      const uint32_t n = counts[v];
+      const uint32_t incident_slots = K >= 2 ? std::min<uint32_t>(K - 2, n) : std::min<uint32_t>(K, n);
+      uint32_t written = 0;
+      auto write_record = [&](uint32_t tet, float weight) {
+        size_t rec = size_t(v) * K + written;
+        s->cubature_tet_ids[rec] = tet;
+        s->cubature_weights[rec] = weight;
+        const uint32_t* ix = &s->tet_indices[4 * size_t(tet)];
+        for (int l = 0; l < 4; ++l) {
+          float* b = &s->cubature_basis[rec * 36 + 9 * size_t(l)];
+          uint32_t j = ix[l];
+          if (m.fixed[j]) continue;
+          double off[3], r2 = 0;
+          for (int a = 0; a < 3; ++a) {
+            off[a] = x[3 * size_t(j) + a] - x[3 * size_t(v) + a];
+            r2 += off[a] * off[a];
+          }
+          double r = std::sqrt(r2);
+          double phi = std::exp(-r2 / (hh * hh));
+          double dir[3] = {0, 0, 0};
+          if (r > 0)
+            for (int a = 0; a < 3; ++a) dir[a] = off[a] / r;
+          for (int rr = 0; rr < 3; ++rr)
+            for (int cc = 0; cc < 3; ++cc) b[3 * rr + cc] = float(phi * ((rr == cc ? 1.0 : 0.0) + 0.1 * dir[rr] * e[cc]));
+        }
+        ++written;
+      };
+      for (uint32_t k = 0; k < incident_slots; ++k) write_record(row[(size_t(k) * n) / incident_slots], 1.0f);
+      // Second ring: tetrahedra incident to the vertices of the first
+      // incident tetrahedra, not incident to v, in CSR order.
+      for (uint32_t a = 0; a < n && written < K; ++a) {
+        const uint32_t* ix = &s->tet_indices[4 * size_t(row[a])];
+        for (int l = 0; l < 4 && written < K; ++l) {
+          uint32_t u = ix[l];
+          if (u == v) continue;
+          const uint32_t* urow = &s->adjacency[starts[u]];
+          for (uint32_t b = 0; b < counts[u] && written < K; ++b) {
+            uint32_t cand = urow[(b * 7u + v) % counts[u]];
+            const uint32_t* cx = &s->tet_indices[4 * size_t(cand)];
+            bool incident = cx[0] == v || cx[1] == v || cx[2] == v || cx[3] == v;
+            bool seen = false;
+            for (uint32_t q = 0; q < written; ++q) seen |= s->cubature_tet_ids[size_t(v) * K + q] == cand;
+            if (!incident && !seen) {
+              write_record(cand, 0.25f);
+              break;
+            }
+          }
+        }
+      }
+    }
+  }
+  return JGS2_OK;
+}
+
+}  // namespace jgs2
 
 extern "C" {
 
@@ -238,196 +417,40 @@ jgs2_status jgs2_scene_build_cuboid(const jgs2_cuboid_desc* d, jgs2_scene** out)
   const uint64_t T64 = nx * ny * nz * 6;
   if (V64 >= 0xffffffffull || T64 * 4 >= 0xffffffffull)
     return jgs2::fail(JGS2_ERROR_RANGE, "Cuboid is too large for 32-bit indices.");
+  jgs2::Mesh m;
+  jgs2::cuboid_grid(d->cells, d->origin, d->size, 0, 0, false, m);
+  const uint32_t V = m.vertex_count(), T = m.tet_count();
+  // Pinned face and seeded vertex jitter (before orientation).
+  const uint64_t sy = nx + 1, sz = (nx + 1) * (ny + 1);
+  double h[3] = {d->size[0] / double(nx), d->size[1] / double(ny), d->size[2] / double(nz)};
+  for (uint32_t v = 0; v < V; ++v) {
+    const uint64_t g[3] = {v % sy, (v % sz) / sy, v / sz};
+    if (d->pin_axis < 3 && g[d->pin_axis] == 0) m.fixed[v] = 1;
+  }
+  if (d->jitter_fraction > 0) {
+    CorpusRandom rng(d->seed);
+    for (uint32_t v = 0; v < V; ++v)
+      for (int a = 0; a < 3; ++a) m.x[3 * size_t(v) + a] += (2.0 * rng.next() - 1.0) * d->jitter_fraction * h[a];
+  }
+  // Alternating material layers by cell index along `layer_axis`.
+  if (d->layer_cells) {
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint64_t cell = t / 6;
+      const uint64_t g[3] = {cell % nx, (cell / nx) % ny, cell / (nx * ny)};
+      m.material[t] = uint32_t((g[d->layer_axis % 3] / d->layer_cells) & 1u);
+    }
+  }
+  jgs2_status st = jgs2::orient(m);
+  if (st != JGS2_OK) return st;
+  std::vector<jgs2::Material> mats = {
+      {d->material_model, d->density, d->young_modulus[0], d->poisson_ratio},
+      {d->material_model, d->density, d->layer_cells ? d->young_modulus[1] : d->young_modulus[0], d->poisson_ratio}};
   jgs2_scene* s = new (std::nothrow) jgs2_scene();
   if (!s) return jgs2::fail(JGS2_ERROR_MEMORY, "Out of host memory.");
-  const uint32_t V = uint32_t(V64), T = uint32_t(T64);
-  s->vertex_count = V;
-  s->tet_count = T;
-  s->cubature_k = d->cubature_k;
-  const uint64_t sy = nx + 1, sz = (nx + 1) * (ny + 1);
-  auto vid = [&](uint64_t x, uint64_t y, uint64_t z) { return uint32_t(x + y * sy + z * sz); };
-
-  // Vertices (mesh.ts:131-146) with optional seeded jitter.
-  std::vector<double> x(3 * size_t(V));
-  std::vector<uint8_t> fixed(V, 0);
-  double h[3] = {d->size[0] / double(nx), d->size[1] / double(ny), d->size[2] / double(nz)};
-  CorpusRandom rng(d->seed);
-  for (uint64_t z = 0; z <= nz; ++z)
-    for (uint64_t y = 0; y <= ny; ++y)
-      for (uint64_t xx = 0; xx <= nx; ++xx) {
-        uint32_t v = vid(xx, y, z);
-        uint64_t g[3] = {xx, y, z};
-        for (int a = 0; a < 3; ++a) x[3 * size_t(v) + a] = d->origin[a] + (d->size[a] * double(g[a])) / double(d->cells[a]);
-        if (d->pin_axis < 3 && g[d->pin_axis] == 0) fixed[v] = 1;
-      }
-  if (d->jitter_fraction > 0) {
-    for (uint32_t v = 0; v < V; ++v)
-      for (int a = 0; a < 3; ++a) x[3 * size_t(v) + a] += (2.0 * rng.next() - 1.0) * d->jitter_fraction * h[a];
-  }
-
-  // Six Kuhn tetrahedra per cell (mesh.ts:152-191) and orientation (mesh.ts:80-99).
-  s->tet_indices.resize(4 * size_t(T));
-  std::vector<uint32_t> layer(T, 0);
-  size_t o = 0;
-  for (uint64_t z = 0; z < nz; ++z)
-    for (uint64_t y = 0; y < ny; ++y)
-      for (uint64_t xx = 0; xx < nx; ++xx) {
-        uint32_t v000 = vid(xx, y, z), v100 = vid(xx + 1, y, z), v010 = vid(xx, y + 1, z),
-                 v110 = vid(xx + 1, y + 1, z), v001 = vid(xx, y, z + 1), v101 = vid(xx + 1, y, z + 1),
-                 v011 = vid(xx, y + 1, z + 1), v111 = vid(xx + 1, y + 1, z + 1);
-        const uint32_t cell[24] = {v000, v100, v110, v111, v000, v110, v010, v111, v000, v010, v011, v111,
-                                   v000, v011, v001, v111, v000, v001, v101, v111, v000, v101, v100, v111};
-        uint64_t g[3] = {xx, y, z};
-        uint32_t lay = d->layer_cells ? uint32_t((g[d->layer_axis % 3] / d->layer_cells) & 1u) : 0u;
-        for (int k = 0; k < 6; ++k) layer[o / 4 + k] = lay;
-        std::memcpy(&s->tet_indices[o], cell, sizeof(cell));
-        o += 24;
-      }
-  for (uint32_t t = 0; t < T; ++t) {
-    uint32_t* ix = &s->tet_indices[4 * size_t(t)];
-    double m[9];
-    rest_matrix(x, ix, m);
-    double det = determinant3(m);
-    if (std::fabs(det) <= 1e-14) {
-      delete s;
-      return jgs2::fail(JGS2_ERROR_RANGE, "Tetrahedron %u is degenerate.", t);
-    }
-    if (det < 0) std::swap(ix[1], ix[2]);
-  }
-
-  // Rest data (fem.ts:167-209), lumped masses (fem.ts:211-238) and the
-  // packed tetrahedron records (scenes/index.ts:1019-1050).
-  s->tet_inverse_dm.assign(12 * size_t(T), 0.0f);
-  s->tet_meta.resize(4 * size_t(T));
-  if (d->pack_rest_stiffness) s->tet_rest_stiffness.resize(144 * size_t(T));
-  std::vector<double> mass(V, 0.0);
-  for (uint32_t t = 0; t < T; ++t) {
-    const uint32_t* ix = &s->tet_indices[4 * size_t(t)];
-    double m[9], inv[9];
-    rest_matrix(x, ix, m);
-    double volume = std::fabs(determinant3(m)) / 6.0;
-    if (!(volume > 1e-14) || !invert3(m, inv)) {
-      delete s;
-      return jgs2::fail(JGS2_ERROR_RANGE, "Cannot assemble a degenerate tetrahedron.");
-    }
-    for (int c = 0; c < 3; ++c)
-      for (int r = 0; r < 3; ++r) s->tet_inverse_dm[12 * size_t(t) + 4 * c + r] = float(inv[3 * r + c]);
-    double E = d->young_modulus[layer[t]], nu = d->poisson_ratio;
-    double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
-    double mu = E / (2 * (1 + nu));
-    float tag = float(d->material_model);
-    if (d->material_model == JGS2_MATERIAL_STABLE_NEO_HOOKEAN) {
-      // computeStableNeoHookeanParameters (stable-neo-hookean.ts:68-109).
-      double adj_mu = (4.0 / 3.0) * mu;
-      double adj_lam = lam + (5.0 / 6.0) * mu;
-      lam = adj_lam;
-      mu = adj_mu;
-    }
-    s->tet_meta[4 * size_t(t)] = float(volume);
-    s->tet_meta[4 * size_t(t) + 1] = float(lam);
-    s->tet_meta[4 * size_t(t) + 2] = float(mu);
-    s->tet_meta[4 * size_t(t) + 3] = tag;
-    if (d->pack_rest_stiffness)
-      linear_stiffness(inv, volume, E, nu, &s->tet_rest_stiffness[144 * size_t(t)]);
-    double vm = d->density * volume / 4.0;
-    for (int l = 0; l < 4; ++l) mass[ix[l]] += vm;
-  }
-
-  // Canonical CSR and vertex records (scenes/index.ts:975-1017).
-  std::vector<uint32_t> starts(V), counts(V);
-  s->adjacency.resize(4 * size_t(T));
-  jgs2_build_adjacency(V, T, s->tet_indices.data(), starts.data(), counts.data(), s->adjacency.data());
-  s->positions.resize(4 * size_t(V));
-  s->vertex_rest.resize(4 * size_t(V));
-  s->vertex_info.resize(4 * size_t(V));
-  for (uint32_t v = 0; v < V; ++v) {
-    for (int a = 0; a < 3; ++a) {
-      s->positions[4 * size_t(v) + a] = float(x[3 * size_t(v) + a]);
-      s->vertex_rest[4 * size_t(v) + a] = float(x[3 * size_t(v) + a]);
-    }
-    s->positions[4 * size_t(v) + 3] = 1.0f;
-    s->vertex_rest[4 * size_t(v) + 3] = float(mass[v]);
-    s->vertex_info[4 * size_t(v)] = starts[v];
-    s->vertex_info[4 * size_t(v) + 1] = counts[v];
-    s->vertex_info[4 * size_t(v) + 2] = fixed[v];
-    s->vertex_info[4 * size_t(v) + 3] = 0;
-  }
-
-  // Cubature records. The reference trains them offline per scene with a
-  // dense exact basis and a nonnegative least-squares selection
-  // (src/simulation/cpu/precompute.ts:141-380); the step itself only
-  // consumes fixed-width records (layout.ts:24-27). This generator writes
-  // records of the same form and role, deterministically and in O(V):
-  //   * up to K-2 samples on incident tetrahedra (weight 1), spread evenly
-  //     over the vertex's CSR row;
-  //   * the remaining samples on second-ring tetrahedra that are not
-  //     incident (weight 1/4), which the step treats without the exact-block
-  //     subtraction;
-  //   * each sample's four Ubar blocks are phi(r) (I + a d e^T) with
-  //     r = |x_j - x_i| / h, phi = exp(-r^2), d the unit offset, e a fixed
-  //     axis and a = 0.1, so Ubar is I at the source vertex, decays with
-  //     distance, is zero on pinned vertices, and is not symmetric.
-  const uint32_t K = d->cubature_k;
-  const size_t R = size_t(V) * K;
-  s->cubature_tet_ids.assign(R, 0xffffffffu);
-  s->cubature_weights.assign(R, 0.0f);
-  s->cubature_basis.assign(R * 36, 0.0f);
-  if (K > 0) {
-    const double hh = std::cbrt(h[0] * h[1] * h[2]);
-    const double e[3] = {1.0 / std::sqrt(14.0), 2.0 / std::sqrt(14.0), 3.0 / std::sqrt(14.0)};
-    for (uint32_t v = 0; v < V; ++v) {
-      if (fixed[v] || counts[v] == 0) continue;
-      const uint32_t* row = &s->adjacency[starts[v]];
-      const uint32_t n = counts[v];
-      const uint32_t incident_slots = K >= 2 ? std::min<uint32_t>(K - 2, n) : std::min<uint32_t>(K, n);
-      uint32_t written = 0;
-      auto write_record = [&](uint32_t tet, float weight) {
-        size_t rec = size_t(v) * K + written;
-        s->cubature_tet_ids[rec] = tet;
-        s->cubature_weights[rec] = weight;
-        const uint32_t* ix = &s->tet_indices[4 * size_t(tet)];
-        for (int l = 0; l < 4; ++l) {
-          float* b = &s->cubature_basis[rec * 36 + 9 * size_t(l)];
-          uint32_t j = ix[l];
-          if (fixed[j]) continue;
-          double off[3], r2 = 0;
-          for (int a = 0; a < 3; ++a) {
-            off[a] = x[3 * size_t(j) + a] - x[3 * size_t(v) + a];
-            r2 += off[a] * off[a];
-          }
-          double r = std::sqrt(r2);
-          double phi = std::exp(-r2 / (hh * hh));
-          double dir[3] = {0, 0, 0};
-          if (r > 0)
-            for (int a = 0; a < 3; ++a) dir[a] = off[a] / r;
-          for (int rr = 0; rr < 3; ++rr)
-            for (int cc = 0; cc < 3; ++cc) b[3 * rr + cc] = float(phi * ((rr == cc ? 1.0 : 0.0) + 0.1 * dir[rr] * e[cc]));
-        }
-        ++written;
-      };
-      for (uint32_t k = 0; k < incident_slots; ++k) write_record(row[(size_t(k) * n) / incident_slots], 1.0f);
-      // Second ring: tetrahedra incident to the vertices of the first
-      // incident tetrahedra, not incident to v, in CSR order.
-      for (uint32_t a = 0; a < n && written < K; ++a) {
-        const uint32_t* ix = &s->tet_indices[4 * size_t(row[a])];
-        for (int l = 0; l < 4 && written < K; ++l) {
-          uint32_t u = ix[l];
-          if (u == v) continue;
-          const uint32_t* urow = &s->adjacency[starts[u]];
-          for (uint32_t b = 0; b < counts[u] && written < K; ++b) {
-            uint32_t cand = urow[(b * 7u + v) % counts[u]];
-            const uint32_t* cx = &s->tet_indices[4 * size_t(cand)];
-            bool incident = cx[0] == v || cx[1] == v || cx[2] == v || cx[3] == v;
-            bool seen = false;
-            for (uint32_t q = 0; q < written; ++q) seen |= s->cubature_tet_ids[size_t(v) * K + q] == cand;
-            if (!incident && !seen) {
-              write_record(cand, 0.25f);
-              break;
-            }
-          }
-        }
-      }
-    }
+  st = jgs2::pack_scene(m, mats, d->cubature_k, d->pack_rest_stiffness != 0, std::cbrt(h[0] * h[1] * h[2]), s);
+  if (st != JGS2_OK) {
+    delete s;
+    return st;
   }
   *out = s;
   jgs2::clear_error();
@@ -681,6 +704,7 @@ jgs2_status jgs2_scene_get_input(const jgs2_scene* s, jgs2_scene_input* out) {
   out->target_positions = s->targets.empty() ? nullptr : s->targets.data();
   out->target_stiffnesses = s->stiffnesses.empty() ? nullptr : s->stiffnesses.data();
   out->cloth = s->has_cloth ? &s->cloth : nullptr;
+  out->contact = s->has_contact ? &s->contact : nullptr;
   jgs2::clear_error();
   return JGS2_OK;
 }

@@ -47,6 +47,14 @@ class ClothInput(C.Structure):
     ]
 
 
+class ContactInput(C.Structure):
+    _fields_ = [
+        ("vertex_triangle_count", C.c_uint32),
+        ("edge_edge_count", C.c_uint32),
+        ("packed_indices", u32p),
+    ]
+
+
 class SceneInput(C.Structure):
     _fields_ = [
         ("vertex_count", C.c_uint32),
@@ -69,6 +77,7 @@ class SceneInput(C.Structure):
         ("target_positions", f32p),
         ("target_stiffnesses", f32p),
         ("cloth", C.POINTER(ClothInput)),
+        ("contact", C.POINTER(ContactInput)),
     ]
 
 
@@ -156,6 +165,16 @@ class ClothDesc(C.Structure):
     ]
 
 
+class TroughDesc(C.Structure):
+    _fields_ = [
+        ("box_count", C.c_uint32),
+        ("box_cells", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("cubature_k", C.c_uint32),
+        ("build_candidates", C.c_int32),
+    ]
+
+
 # Every exported entry point with its ctypes signature. Tests compare this
 # table with the declarations in include/*.h.
 SIGNATURES = {
@@ -194,6 +213,12 @@ SIGNATURES = {
         C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "jgs2_scene_build_cuboid": (C.c_int, [C.POINTER(CuboidDesc), C.POINTER(C.c_void_p)]),
     "jgs2_scene_build_cloth_drape": (C.c_int, [C.POINTER(ClothDesc), C.POINTER(C.c_void_p)]),
+    "jgs2_scene_build_trough": (C.c_int, [C.POINTER(TroughDesc), C.POINTER(C.c_void_p)]),
+    "jgs2_scene_get_surface": (
+        C.c_int, [C.c_void_p, C.POINTER(u32p), C.POINTER(C.c_uint32), C.POINTER(u32p), C.POINTER(C.c_uint32)]),
+    "jgs2_build_static_contact_candidates": (
+        C.c_int, [C.c_uint32, u32p, C.c_uint32, u32p, C.c_uint32, u32p, C.c_uint64, C.POINTER(C.c_uint32),
+                  C.POINTER(C.c_uint32)]),
     "jgs2_scene_get_input": (C.c_int, [C.c_void_p, C.POINTER(SceneInput)]),
     "jgs2_scene_set_objectives": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
     "jgs2_scene_destroy": (None, [C.c_void_p]),

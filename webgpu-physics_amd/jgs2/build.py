@@ -1,7 +1,7 @@
 """Builds libjgs2hip.so in-tree with hipcc for gfx950.
 
 The library is the product: HIP kernels (csrc/jgs2_kernels.hip) plus the
-C-ABI host code (csrc/jgs2_host.cpp, csrc/jgs2_scene.cpp). It is written to
+C-ABI host code (csrc/jgs2_host.cpp, csrc/jgs2_scene*.cpp). It is written to
 webgpu-physics_amd/lib/libjgs2hip.so so that it travels with the tree.
 """
 
@@ -36,6 +36,7 @@ SOURCES = [
     ("jgs2_kernels.hip", True),
     ("jgs2_host.cpp", False),
     ("jgs2_scene.cpp", False),
+    ("jgs2_scene_contact.cpp", False),
 ]
 
 

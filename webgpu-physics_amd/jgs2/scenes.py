@@ -20,7 +20,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .solver import ClothData, GpuInput
+from .solver import ClothData, ContactData, GpuInput
 
 # Minimal-scene scripted target (src/scenes/index.ts:252-262).
 TARGET_PULL_START_FRAME = 48
@@ -77,6 +77,14 @@ def _from_handle(handle) -> GpuInput:
             hinge_rest_edge_lengths=f(c.hinge_rest_edge_lengths, NH),
             young_modulus=c.young_modulus, poisson_ratio=c.poisson_ratio, thickness=c.thickness,
             bending_stiffness=c.bending_stiffness)
+    contact = None
+    if s.contact:
+        c = s.contact.contents
+        contact = ContactData(packed_indices=f(c.packed_indices, 4 * (c.vertex_triangle_count + c.edge_edge_count)),
+                              vertex_triangle_count=c.vertex_triangle_count, edge_edge_count=c.edge_edge_count)
+    tri_p, tri_n, edge_p, edge_n = N.u32p(), C.c_uint32(), N.u32p(), C.c_uint32()
+    N.check(N.lib.jgs2_scene_get_surface(handle, C.byref(tri_p), C.byref(tri_n), C.byref(edge_p), C.byref(edge_n)))
+    surface = {"triangles": f(tri_p, 3 * tri_n.value), "edges": f(edge_p, 2 * edge_n.value)}
     return GpuInput(
         vertex_count=V, tet_count=T, cubature_k=K,
         positions=f(s.positions, 4 * V),
@@ -91,6 +99,8 @@ def _from_handle(handle) -> GpuInput:
         cubature_basis=f(s.cubature_basis, V * K * 36),
         tet_rest_stiffness=f(s.tet_rest_stiffness, 144 * T, optional=True),
         cloth=cloth,
+        contact=contact,
+        extras={"surface": surface},
     )
 
 
@@ -102,6 +112,58 @@ def build_cuboid(desc: N.CuboidDesc) -> GpuInput:
         return _from_handle(handle)
     finally:
         N.lib.jgs2_scene_destroy(handle)
+
+
+def trough(box_count=10, box_cells=1, seed=0, cubature_k=6, candidates=True) -> GpuInput:
+    """The reference's IPC trough (troughDefinition, src/scenes/index.ts:
+    551-631) with its static VT/EE candidate superset; box_cells > 1 and a
+    nonzero seed give the finer boxes and seeded placements of the large
+    contact workload."""
+    d = N.TroughDesc()
+    d.box_count, d.box_cells, d.seed, d.cubature_k = int(box_count), int(box_cells), int(seed), int(cubature_k)
+    d.build_candidates = 1 if candidates else 0
+    handle = C.c_void_p()
+    N.check(N.lib.jgs2_scene_build_trough(C.byref(d), C.byref(handle)))
+    try:
+        return _from_handle(handle)
+    finally:
+        N.lib.jgs2_scene_destroy(handle)
+
+
+def static_contact_candidates(vertex_count, triangles, edges) -> ContactData:
+    """buildStaticIpcContactCandidates (src/simulation/cpu/ipc-contact.ts:600-665)."""
+    tri = np.ascontiguousarray(triangles, np.uint32).reshape(-1)
+    edg = np.ascontiguousarray(edges, np.uint32).reshape(-1)
+    vt, ee = C.c_uint32(), C.c_uint32()
+    N.check(N.lib.jgs2_build_static_contact_candidates(vertex_count, N.uptr(tri), tri.size // 3, N.uptr(edg),
+                                                        edg.size // 2, None, 0, C.byref(vt), C.byref(ee)))
+    packed = np.empty(4 * (vt.value + ee.value), np.uint32)
+    N.check(N.lib.jgs2_build_static_contact_candidates(vertex_count, N.uptr(tri), tri.size // 3, N.uptr(edg),
+                                                        edg.size // 2, N.uptr(packed), vt.value + ee.value,
+                                                        C.byref(vt), C.byref(ee)))
+    return ContactData(packed_indices=packed, vertex_triangle_count=vt.value, edge_edge_count=ee.value)
+
+
+# The reference app's IPC settings for its contact scenes (src/App.tsx:
+# 529-574), in exact-iteration form.
+TROUGH_SETTINGS = {
+    "timestep": 1.0 / 60.0,
+    "gravity": (0.0, -9.81, 0.0),
+    "floor_height": -0.18,
+    "floor_stiffness": 0.0,
+    "velocity_damping": 0.997,
+    "contact_tangential_damping": 0.0,
+    "horizontal_body_correction": False,
+    "ipc_activation_distance": 0.08,
+    "ipc_minimum_distance": 0.003,
+    "ipc_barrier_stiffness": 100000.0,
+    "ipc_friction_coefficient": 0.45,
+    "ipc_friction_velocity_epsilon": 0.05,
+    "ipc_step_safety": 0.9,
+    "max_step": 0.075,
+    "schedule": "jacobi",
+}
+TROUGH_ITERATIONS = 3
 
 
 def cloth_drape(cells=4, origin=(-0.8, 1.15, -0.8), size=(1.6, 1.6), fold_column=3, fold_lift=0.079,

@@ -119,6 +119,30 @@ class ClothData:
 
 
 @dataclass
+class ContactData:
+    """`StaticIpcContactCandidates` (src/simulation/cpu/ipc-contact.ts:20-38):
+    VT records first, then EE, four vertex ids each."""
+
+    packed_indices: np.ndarray
+    vertex_triangle_count: int
+    edge_edge_count: int
+
+    def __post_init__(self):
+        self.packed_indices = np.ascontiguousarray(self.packed_indices, dtype=np.uint32).reshape(-1)
+
+    @property
+    def candidate_count(self) -> int:
+        return self.vertex_triangle_count + self.edge_edge_count
+
+    def struct(self) -> N.ContactInput:
+        c = N.ContactInput()
+        c.vertex_triangle_count = int(self.vertex_triangle_count)
+        c.edge_edge_count = int(self.edge_edge_count)
+        c.packed_indices = N.uptr(self.packed_indices)
+        return c
+
+
+@dataclass
 class GpuInput:
     """Arrays of `JGS2GpuInput` (layout.ts:41-90) as contiguous numpy arrays."""
 
@@ -141,6 +165,7 @@ class GpuInput:
     target_positions: Optional[np.ndarray] = None
     target_stiffnesses: Optional[np.ndarray] = None
     cloth: Optional[ClothData] = None
+    contact: Optional[ContactData] = None
     extras: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -180,6 +205,9 @@ class GpuInput:
             # with the returned SceneInput.
             self._cloth_struct = self.cloth.struct()
             s.cloth = C.pointer(self._cloth_struct)
+        if self.contact is not None:
+            self._contact_struct = self.contact.struct()
+            s.contact = C.pointer(self._contact_struct)
         return s
 
     def packed_objectives(self) -> np.ndarray:
