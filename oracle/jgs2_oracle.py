@@ -197,7 +197,8 @@ def greedy_coloring(vertex_count: int, groups) -> np.ndarray:
 
 def conflict_groups(scene) -> list:
     """buildJGS2InputVertexColoring conflict groups (jgs2-solver.ts:308-367),
-    cloth triangles and hinges included (jgs2-solver.ts:329-351)."""
+    cloth triangles, hinges and IPC candidates included (jgs2-solver.ts:
+    329-364)."""
     T, K, V = scene.tet_count, scene.cubature_k, scene.vertex_count
     tets = scene.tet_indices.reshape(T, 4)
     groups = [list(t) for t in tets]
@@ -214,6 +215,9 @@ def conflict_groups(scene) -> list:
     if cloth is not None:
         groups.extend(list(t) for t in cloth.triangle_indices.reshape(-1, 3))
         groups.extend(list(h) for h in cloth.hinge_indices.reshape(-1, 4))
+    contact = getattr(scene, "contact", None)
+    if contact is not None:
+        groups.extend(list(c) for c in contact.packed_indices.reshape(-1, 4))
     return groups
 
 
