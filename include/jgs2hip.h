@@ -270,6 +270,91 @@ jgs2_status jgs2_profile_begin(jgs2_solver* solver);
 jgs2_status jgs2_profile_end(jgs2_solver* solver, double* local_solve_ms_per_launch, uint64_t* local_solve_launches,
                              double* frame_ms_total);
 
+/*
+ * Batches. No counterpart in the reference, whose JGS2GpuSolver steps one
+ * scene per object; these entry points serve callers that step many
+ * independent scenes of uneven size at once (continuous-batching servers).
+ *
+ * A batch is one device arena created at fixed capacities. Scenes are
+ * admitted into free 256-entry blocks of its vertex, tetrahedron, triangle
+ * and candidate ranges and retired at any time between submissions. Each
+ * pass of a frame is one kernel launch over every resident scene;
+ * reductions, the line-search control, history and the convergence latch
+ * stay per scene, so every scene's results are bit-identical to stepping it
+ * alone. Admitted scenes are jgs2_solver handles: every read, objective,
+ * info and debug call above applies to them; jgs2_step* does not (they are
+ * stepped by jgs2_batch_step_frames*), and jgs2_destroy() retires them.
+ * jgs2_batch_destroy() frees the arena and invalidates every handle of it.
+ *
+ * Scenes of one batch share: the material path (stable Neo-Hookean or
+ * co-rotated), the settings given at create (timestep, contact
+ * parameters), per-step overrides, and one cloth material.
+ */
+typedef struct jgs2_batch jgs2_batch;
+
+typedef struct {
+  uint64_t max_vertices;    /* summed over resident scenes, each rounded up to 256 */
+  uint64_t max_tets;        /* likewise */
+  uint64_t max_triangles;   /* likewise */
+  uint64_t max_hinges;
+  uint64_t max_candidates;  /* IPC candidates, each scene rounded up to 256 */
+  uint64_t max_bodies;
+  uint32_t max_scenes;
+  uint32_t cubature_k;      /* records per vertex; scenes may use fewer */
+  uint32_t corotated;       /* 1 = legacy co-rotated material path */
+  uint32_t reserved;
+} jgs2_batch_config;
+
+typedef struct {
+  uint32_t scene_count;
+  uint32_t max_scenes;
+  uint32_t solve_block;  /* threads per block of the local solve */
+  uint32_t xcd_remap;    /* XCD-aware block order of per-vertex kernels */
+  uint32_t use_graphs;
+  uint32_t reserved;
+  uint64_t vertex_capacity;
+  uint64_t vertex_blocks_used;
+  uint64_t vertex_extent;  /* vertices every launch covers (highest used block) */
+  uint64_t tet_extent;
+  uint64_t arena_bytes;    /* device bytes the arena holds */
+} jgs2_batch_info;
+
+/* Device bytes a batch of this configuration allocates (for admission
+ * against an HBM budget before creating it). */
+jgs2_status jgs2_batch_arena_bytes(const jgs2_batch_config* config, uint64_t* bytes);
+jgs2_status jgs2_batch_create(const jgs2_batch_config* config, const jgs2_settings* overrides,
+                              uint64_t override_mask, jgs2_batch** out);
+jgs2_status jgs2_batch_destroy(jgs2_batch* batch);
+/* Validates and uploads a scene (the checks of jgs2_create). Returns
+ * JGS2_ERROR_MEMORY, with the batch unchanged, when the arena has no room
+ * for it now. */
+jgs2_status jgs2_batch_admit(jgs2_batch* batch, const jgs2_scene_input* input, jgs2_solver** scene);
+jgs2_status jgs2_batch_retire(jgs2_batch* batch, jgs2_solver* scene);
+/* stepFrames / stepFramesExactIterations over every resident scene. */
+jgs2_status jgs2_batch_step_frames(jgs2_batch* batch, uint32_t frame_count, const jgs2_settings* overrides,
+                                   uint64_t override_mask);
+jgs2_status jgs2_batch_step_frames_exact_iterations(jgs2_batch* batch, uint32_t frame_count, uint32_t iterations,
+                                                    const jgs2_settings* overrides, uint64_t override_mask);
+jgs2_status jgs2_batch_sync(jgs2_batch* batch);
+jgs2_status jgs2_batch_get_info(jgs2_batch* batch, jgs2_batch_info* out);
+/* Records each distinct frame once as a hipGraph and replays it; the graph
+ * is rebuilt after admission, retirement, objective changes or different
+ * step settings. */
+jgs2_status jgs2_batch_use_graphs(jgs2_batch* batch, int enable);
+/* Launch shape of the local solve: 64, 128 or 256 threads per block, and
+ * the XCD-aware block order on or off. Results do not depend on either. */
+jgs2_status jgs2_batch_set_launch(jgs2_batch* batch, uint32_t solve_block, uint32_t xcd_remap);
+/* Times the local solve over the resident scenes under every launch shape
+ * (`repeats` launches each) and keeps the fastest. */
+jgs2_status jgs2_batch_autotune(jgs2_batch* batch, const jgs2_settings* overrides, uint64_t override_mask,
+                                uint32_t repeats, double* best_ms_per_launch);
+jgs2_status jgs2_batch_profile_begin(jgs2_batch* batch);
+jgs2_status jgs2_batch_profile_end(jgs2_batch* batch, double* local_solve_ms_per_launch,
+                                   uint64_t* local_solve_launches, double* frame_ms_total);
+/* The batch a handle lives in (jgs2_create() makes a private one-scene
+ * batch), for the launch, graph and autotune controls above. */
+jgs2_status jgs2_solver_batch(jgs2_solver* solver, jgs2_batch** out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,8 @@ namespace jgs2 {
 
 // Per-pass uniform block, the role of the reference's SimParams
 // (src/simulation/gpu/jgs2-shader.ts:141-164), passed by value to kernels.
+// Counts are arena extents: kernels run over every block below them and
+// skip blocks that belong to no scene.
 struct PassParams {
   uint32_t vertex_count, tet_count, cubature_k, body_count;
   uint32_t src, tgt;          // 0 = posA, 1 = posB
@@ -25,10 +27,13 @@ struct PassParams {
   float gx, gy, gz, floor_y;
   float floor_k, regularization, polar_eps, damping;
   float tangential_damping, contact_margin, ipc_dhat, ipc_kappa;
-  float residual_tol, update_tol, scene_scale;
+  float residual_tol, update_tol;
+  uint32_t solve_block;         // threads per block of the local solve (autotuned)
   uint32_t energy_after_sweep;  // vertex-energy pass that replaces the in-solve one
   uint32_t contact;             // IPC enabled for this step (ipcEnabled)
-  uint32_t pad3, pad4, pad5;
+  uint32_t scene_count;         // scene slots in use
+  uint32_t xcd_remap;           // XCD-aware block order for vertex kernels (autotuned)
+  uint32_t color_index;         // colored pass: the color being swept
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
@@ -47,16 +52,21 @@ struct PassParams {
 //   tri_area  float[NT]  rest area
 //   hinge_ix  uint4[NH]  hinge {v0, v1, v2, v3}: triangles (0,1,2), (1,0,3)
 //   hinge_rest float4[NH] {rest angle, rest edge length, owner bits, 0}
-//   tri_rows/hinge_rows uint32[V+1], *_elems: vertex -> element CSR, rows
-//                        ascending in element id
+//   tri_row/hinge_row uint2[V] {begin, count}, *_elems: vertex -> element
+//                        CSR, each row ascending in element id
 //   IPC candidates (NC of them, the first NVT vertex-triangle):
 //   cand      uint4[NC]  vertex ids (device order)
 //   lag_n, lag_w, cscratch, src_c, src_w, tgt_c, tgt_w  float4[NC]: lagged
 //                        normal + force and weights, step scratch, cached
 //                        source/target contact (normal, distance) + weights
 //   contact_ctl float4[1] w = the step's safe alpha
-//   cand_rows uint32[V+1], cand_ids uint32[4 NC]: vertex -> candidate CSR;
-//   active_count uint32[V], active_ids uint32[4 NC]: the active rows
+//   cand_row uint2[V] {begin, count}, cand_ids uint32[4 NC]: vertex ->
+//   candidate CSR; active_count uint32[V], active_ids uint32[4 NC]: the
+//   active rows; cand_meta uint32[NC]: 1 for edge-edge records
+// Scenes: every scene owns whole 256-entry blocks of the vertex,
+// tetrahedron, triangle and candidate ranges (block tables *_scene map a
+// block to its scene, kNoScene for free blocks); per-scene control (4),
+// history (64 x 8), contact alpha, scene scale and convergence scales.
 struct DeviceState {
   uint32_t V = 0, T = 0, K = 0, bodies = 1;
   float4* vtx_rest = nullptr;
@@ -96,10 +106,11 @@ struct DeviceState {
   float* tri_area = nullptr;
   uint4* hinge_ix = nullptr;
   float4* hinge_rest = nullptr;
-  uint32_t* tri_rows = nullptr;
+  uint2* tri_row = nullptr;
   uint32_t* tri_elems = nullptr;
-  uint32_t* hinge_rows = nullptr;
+  uint2* hinge_row = nullptr;
   uint32_t* hinge_elems = nullptr;
+  float4* tri_glob = nullptr;   // 2 per triangle: candidate area stretch
   // IPC contact candidates.
   uint32_t NC = 0, NVT = 0;
   float ipc_dmin = 0, ipc_mu = 0, ipc_eps = 0, ipc_safety = 0;
@@ -107,8 +118,22 @@ struct DeviceState {
   float4 *lag_n = nullptr, *lag_w = nullptr, *cscratch = nullptr;
   float4 *src_c = nullptr, *src_w = nullptr, *tgt_c = nullptr, *tgt_w = nullptr;
   float4* contact_ctl = nullptr;
-  uint32_t *cand_rows = nullptr, *cand_ids = nullptr, *active_count = nullptr, *active_ids = nullptr;
+  uint2* cand_row = nullptr;
+  uint32_t *cand_ids = nullptr, *active_count = nullptr, *active_ids = nullptr, *cand_meta = nullptr;
+  // Scene tables.
+  uint32_t S = 0;
+  uint32_t *vblock_scene = nullptr, *tblock_scene = nullptr, *rblock_scene = nullptr, *cblock_scene = nullptr;
+  uint4* scene_blk0 = nullptr;  // {vb0, vb1, tb0, tb1}
+  uint4* scene_blk1 = nullptr;  // {rb0, rb1, cb0, cb1}
+  uint4* scene_cnt = nullptr;   // {V, T, schedule colors (0 = free slot), bodies}
+  uint4* scene_end = nullptr;   // absolute ends {vertex, tet, triangle, candidate}
+  float* scene_scale = nullptr;
+  void* scene_conv = nullptr;   // per-scene convergence scale records
+  uint4* body_rng = nullptr;    // per body {scene vertex begin, end, 0, 0}
 };
+
+constexpr uint32_t kNoScene = 0xffffffffu;
+constexpr uint32_t kSceneBlock = 256;  // scene ranges are multiples of this
 
 constexpr uint32_t kHistoryCapacity = 64;
 
@@ -119,6 +144,7 @@ void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t 
 void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_reduce_candidate(const DeviceState& s, const PassParams& p, hipStream_t stream);
+void launch_reduce_candidate_scene(const DeviceState& s, const PassParams& p, uint32_t scene, hipStream_t stream);
 void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t stream);
@@ -131,7 +157,10 @@ void launch_lag_contact(const DeviceState& s, const PassParams& p, hipStream_t s
 void launch_build_active_rows(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_promote_contact(const DeviceState& s, const PassParams& p, hipStream_t stream);
-uint32_t partial_blocks_needed(uint32_t V, uint32_t elements, uint32_t candidates);
+uint32_t partial_blocks_needed(uint32_t vertices, uint32_t tets, uint32_t triangles, uint32_t candidates);
+size_t scene_conv_bytes(uint32_t scenes);
+// Local-solve timing of one launch configuration (autotuner).
+void launch_solve_config(const DeviceState& s, const PassParams& p, hipStream_t stream);
 size_t partial_bytes(uint32_t blocks);
 
 }  // namespace jgs2

@@ -1,7 +1,13 @@
 // libjgs2hip host side: the device half of JGS2GpuSolver
 // (src/simulation/gpu/jgs2-solver.ts) behind a C ABI. Validation, coloring,
 // packing and frame orchestration run here; every pass of the frame is a
-// HIP kernel in jgs2_kernels.hip enqueued on the solver's stream.
+// HIP kernel in jgs2_kernels.hip enqueued on the batch's stream.
+//
+// Every scene lives in a batch: a device arena whose vertex, tetrahedron,
+// triangle and candidate ranges are handed out in 256-entry blocks. A frame
+// is one launch per pass over all resident scenes; reductions, line-search
+// control, history and the convergence latch are per scene. jgs2_create()
+// builds a private one-scene batch sized exactly for its input.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -10,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -48,6 +55,12 @@ constexpr double kMaxRuntimeTolerance = 1e-3;  // jgs2-globalization.ts:24
     if (e_ != hipSuccess) return fail(JGS2_ERROR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
+#define TRYS(expr)                      \
+  do {                                  \
+    jgs2_status t_ = (expr);            \
+    if (t_ != JGS2_OK) return t_;       \
+  } while (0)
+
 float round_f32(double v) { return float(v); }
 
 }  // namespace
@@ -55,34 +68,134 @@ float round_f32(double v) { return float(v); }
 
 using namespace jgs2;
 
+namespace {
+
+// First-fit allocator over [0, capacity) units with coalescing free list.
+struct RangeAlloc {
+  uint64_t capacity = 0;
+  std::map<uint64_t, uint64_t> free;  // begin -> length
+  void init(uint64_t cap) {
+    capacity = cap;
+    free.clear();
+    if (cap) free[0] = cap;
+  }
+  bool alloc(uint64_t n, uint64_t& at) {
+    if (n == 0) {
+      at = 0;
+      return true;
+    }
+    for (auto it = free.begin(); it != free.end(); ++it) {
+      if (it->second < n) continue;
+      at = it->first;
+      const uint64_t rest = it->second - n;
+      free.erase(it);
+      if (rest) free[at + n] = rest;
+      return true;
+    }
+    return false;
+  }
+  void release(uint64_t at, uint64_t n) {
+    if (n == 0) return;
+    auto it = free.emplace(at, n).first;
+    auto next = std::next(it);
+    if (next != free.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free.erase(next);
+    }
+    if (it != free.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free.erase(it);
+      }
+    }
+  }
+  // One past the highest allocated unit.
+  uint64_t high() const {
+    if (free.empty()) return capacity;
+    auto last = std::prev(free.end());
+    return last->first + last->second == capacity ? last->first : capacity;
+  }
+};
+
+struct Span {
+  uint64_t at = 0, n = 0;
+};
+
+}  // namespace
+
+// One scene resident in a batch arena. Handles returned by jgs2_create
+// own a private one-scene batch; handles returned by jgs2_batch_admit
+// belong to the batch that stepped them.
 struct jgs2_solver {
   bool destroyed = false;
-  DeviceState dev;
-  hipStream_t stream = nullptr;
+  jgs2_batch* batch = nullptr;
+  bool owns_batch = false;
+  uint32_t slot = 0;
   jgs2_settings defaults{};
   double preprocessing_timestep = 0;
   float scene_scale = 0;
   bool globalization = true;
   bool corotated = false;
-  uint32_t V = 0, T = 0, K = 0, bodies = 1, color_count = 1, free_vertices = 0;
+  uint32_t V = 0, T = 0, K = 0, NT = 0, NH = 0, NC = 0, NVT = 0, bodies = 1, color_count = 1, free_vertices = 0;
+  // Arena spans: vertex/tet/triangle/candidate spans in 256-entry blocks;
+  // the others in entries.
+  Span vblk, tblk, rblk, cblk, adj, hin, tel, hel, cel, body;
   std::vector<uint32_t> colors;         // by caller vertex id
   std::vector<uint32_t> color_offsets;  // color_count + 1
-  // Device order: vertices and tetrahedra are stored along a Morton curve of
-  // their rest positions so that neighbors share cache lines and L2; the
-  // boundary maps ids both ways.
+  std::vector<uint32_t> color_order;    // arena vertex ids by color
+  // Device order within the scene: vertices may be stored along a Morton
+  // curve of their rest positions; the boundary maps ids both ways.
   std::vector<uint32_t> vnew, vold, tnew, told;
   std::vector<float> objectives;        // 8 per vertex (host mirror)
   std::vector<uint8_t> pinned;
   uint32_t objective_flags = 0;
   uint32_t active_forces = 0, active_targets = 0;
-  // Profiling of the local-solve kernel.
+  // packJGS2GpuClothArena material words (cloth-layout.ts:335-343): lambda,
+  // mu, thickness, bending stiffness; f64 expressions rounded to f32.
+  bool has_cloth = false;
+  float cloth_mat[4] = {0, 0, 0, 0};
+  uint64_t vbase() const { return vblk.at * kSceneBlock; }
+  uint64_t tbase() const { return tblk.at * kSceneBlock; }
+  uint64_t rbase() const { return rblk.at * kSceneBlock; }
+  uint64_t cbase() const { return cblk.at * kSceneBlock; }
+};
+
+// A device arena holding many independent scenes. Every pass of a frame is
+// one launch over all resident scenes; reductions are per scene.
+struct jgs2_batch {
+  DeviceState dev;
+  hipStream_t stream = nullptr;
+  jgs2_settings defaults{};
+  jgs2_batch_config cfg{};
+  bool corotated = false;
+  RangeAlloc va, ta, ra, ca, adja, hina, tela, hela, cela, bodya, slota;
+  std::vector<jgs2_solver*> slots;
+  uint64_t version = 0;      // bumps on every admission or retirement
+  uint64_t synced_version = ~0ull;
+  uint32_t color_count = 1;
+  std::vector<uint32_t> color_offsets;
+  uint32_t solve_block = 256, xcd_remap = 1;
+  uint64_t arena_bytes = 0;
+  // Cloth material shared by every resident cloth scene (one set of
+  // material words per arena).
+  uint32_t cloth_scenes = 0;
+  // hipGraph frames
+  bool use_graphs = false;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_key = 0;
+  bool graph_valid = false;
+  // Local-solve profiling
   bool profiling = false;
   std::vector<hipEvent_t> events;
   size_t event_used = 0;
   hipEvent_t frame_begin = nullptr, frame_end = nullptr;
-  bool frame_timing = false;
+  uint32_t active_scenes() const {
+    uint32_t n = 0;
+    for (auto* s : slots) n += s ? 1u : 0u;
+    return n;
+  }
 };
-
 namespace {
 
 void free_device(DeviceState& d) {
@@ -90,22 +203,26 @@ void free_device(DeviceState& d) {
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
-                  d.hinge_rest, d.tri_rows, d.tri_elems, d.hinge_rows, d.hinge_elems, d.cand, d.lag_n, d.lag_w,
-                  d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_rows, d.cand_ids,
-                  d.active_count, d.active_ids};
+                  d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.cand, d.lag_n, d.lag_w,
+                  d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
+                  d.active_count, d.active_ids, d.cand_meta, d.vblock_scene, d.tblock_scene, d.rblock_scene,
+                  d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_conv,
+                  d.body_rng};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   d = DeviceState{};
 }
 
-template <class T> jgs2_status dalloc(T** ptr, size_t count) {
+template <class T> jgs2_status dalloc(T** ptr, size_t count, uint64_t* bytes = nullptr) {
   *ptr = nullptr;
   if (count == 0) count = 1;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T));
   if (e != hipSuccess)
     return fail(JGS2_ERROR_MEMORY, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+  if (bytes) *bytes += count * sizeof(T);
   return JGS2_OK;
 }
+
 
 // ---------------------------------------------------------------- settings
 // resolveJGS2StepSettings + validateStepSettings (jgs2-solver.ts:429-550).
@@ -414,26 +531,6 @@ jgs2_status validate_contact(const jgs2_contact_input& c, uint32_t V) {
   return JGS2_OK;
 }
 
-// Vertex -> element CSR in device vertex order: the caller-order CSR
-// (buildIncidentElementCsr, cloth-layout.ts:183-207) with its rows moved to
-// their device vertex; each row stays ascending in element id.
-jgs2_status incident_csr(const uint32_t* idx, uint32_t width, uint32_t count, const std::vector<uint32_t>& vold,
-                         std::vector<uint32_t>& rows, std::vector<uint32_t>& elems) {
-  const uint32_t V = uint32_t(vold.size());
-  std::vector<uint32_t> crow(size_t(V) + 1), celem(size_t(count) * width);
-  jgs2_status st = jgs2_build_element_incidence(V, idx, width, count, crow.data(), celem.data());
-  if (st != JGS2_OK) return st;
-  rows.assign(size_t(V) + 1, 0);
-  elems.clear();
-  elems.reserve(celem.size());
-  for (uint32_t nv = 0; nv < V; ++nv) {
-    const uint32_t ov = vold[nv];
-    elems.insert(elems.end(), celem.begin() + crow[ov], celem.begin() + crow[ov + 1]);
-    rows[nv + 1] = uint32_t(elems.size());
-  }
-  return JGS2_OK;
-}
-
 // minimumJGS2InputDeformationDeterminant (layout.ts:549-588), f64.
 double min_input_determinant(const jgs2_scene_input& in) {
   auto det_cols = [](double c0x, double c0y, double c0z, double c1x, double c1y, double c1z, double c2x, double c2y,
@@ -568,21 +665,370 @@ void morton_orders(const jgs2_scene_input& in, std::vector<uint32_t>& vnew, std:
   }
 }
 
+
+// ---------------------------------------------------------------- batch arena
+uint64_t blocks_of(uint64_t n) { return (n + kSceneBlock - 1) / kSceneBlock; }
+
+
+// Allocates every arena array for the configured capacities.
+jgs2_status batch_allocate(jgs2_batch* b) {
+  const jgs2_batch_config& c = b->cfg;
+  DeviceState& d = b->dev;
+  uint64_t* by = &b->arena_bytes;
+  const uint64_t Vc = std::max<uint64_t>(1, blocks_of(c.max_vertices)) * kSceneBlock;
+  const uint64_t Tc = std::max<uint64_t>(1, blocks_of(c.max_tets)) * kSceneBlock;
+  const uint64_t Rc = blocks_of(c.max_triangles) * kSceneBlock;
+  const uint64_t Cc = blocks_of(c.max_candidates) * kSceneBlock;
+  const uint64_t Hc = c.max_hinges, Bc = std::max<uint64_t>(1, c.max_bodies), S = std::max<uint32_t>(1, c.max_scenes);
+  const uint64_t K = c.cubature_k;
+  if (Vc > 0xffffff00ull || Tc * 4 > 0xffffff00ull || Cc * 4 > 0xffffff00ull || Rc * 3 > 0xffffff00ull ||
+      Hc * 4 > 0xffffff00ull)
+    return fail(JGS2_ERROR_RANGE, "Batch capacities exceed 32-bit indexing.");
+  d.K = uint32_t(K);
+  d.S = uint32_t(S);
+  TRYS(dalloc(&d.vtx_rest, Vc, by));
+  TRYS(dalloc(&d.vtx_info, Vc, by));
+  TRYS(dalloc(&d.objectives, 2 * Vc, by));
+  TRYS(dalloc(&d.color_order, Vc, by));
+  TRYS(dalloc(&d.colors, Vc, by));
+  TRYS(dalloc(&d.pos[0], Vc, by));
+  TRYS(dalloc(&d.pos[1], Vc, by));
+  TRYS(dalloc(&d.predicted, Vc, by));
+  TRYS(dalloc(&d.velocity, Vc, by));
+  TRYS(dalloc(&d.old, Vc, by));
+  TRYS(dalloc(&d.vrot, 3 * Vc, by));
+  TRYS(dalloc(&d.vrot_next, 3 * Vc, by));
+  TRYS(dalloc(&d.final_update, Vc, by));
+  TRYS(dalloc(&d.local_diag, 4 * Vc, by));
+  TRYS(dalloc(&d.vertex_energy, Vc, by));
+  TRYS(dalloc(&d.conv, 5 * Vc, by));
+  TRYS(dalloc(&d.tri_row, Vc, by));
+  TRYS(dalloc(&d.hinge_row, Vc, by));
+  TRYS(dalloc(&d.cand_row, Vc, by));
+  TRYS(dalloc(&d.active_count, Vc, by));
+  TRYS(dalloc(&d.cub, 10 * Vc * std::max<uint64_t>(K, 1), by));
+  TRYS(dalloc(&d.tet_idx, Tc, by));
+  TRYS(dalloc(&d.tet_dm, 3 * Tc, by));
+  if (b->corotated) TRYS(dalloc(&d.tet_k, 144 * Tc, by));
+  TRYS(dalloc(&d.trot, 3 * Tc, by));
+  TRYS(dalloc(&d.tet_glob, 2 * Tc, by));
+  TRYS(dalloc(&d.adjacency, 4 * Tc, by));
+  TRYS(dalloc(&d.tri_ix, Rc, by));
+  TRYS(dalloc(&d.tri_inv, Rc, by));
+  TRYS(dalloc(&d.tri_area, Rc, by));
+  TRYS(dalloc(&d.tri_glob, 2 * Rc, by));
+  TRYS(dalloc(&d.tri_elems, 3 * Rc, by));
+  TRYS(dalloc(&d.hinge_ix, Hc, by));
+  TRYS(dalloc(&d.hinge_rest, Hc, by));
+  TRYS(dalloc(&d.hinge_elems, 4 * Hc, by));
+  TRYS(dalloc(&d.cand, Cc, by));
+  TRYS(dalloc(&d.cand_meta, Cc, by));
+  for (float4** a : {&d.lag_n, &d.lag_w, &d.cscratch, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRYS(dalloc(a, Cc, by));
+  TRYS(dalloc(&d.cand_ids, 4 * Cc, by));
+  TRYS(dalloc(&d.active_ids, 4 * Cc, by));
+  TRYS(dalloc(&d.body_corr, Bc, by));
+  TRYS(dalloc(&d.body_rng, Bc, by));
+  TRYS(dalloc(&d.control, 4 * S, by));
+  TRYS(dalloc(&d.history, size_t(kHistoryCapacity) * 8 * S, by));
+  TRYS(dalloc(&d.contact_ctl, S, by));
+  TRYS(dalloc(&d.scene_scale, S, by));
+  TRYS(dalloc(reinterpret_cast<char**>(&d.scene_conv), scene_conv_bytes(uint32_t(S)), by));
+  TRYS(dalloc(&d.scene_blk0, S, by));
+  TRYS(dalloc(&d.scene_blk1, S, by));
+  TRYS(dalloc(&d.scene_cnt, S, by));
+  TRYS(dalloc(&d.scene_end, S, by));
+  TRYS(dalloc(&d.vblock_scene, Vc / kSceneBlock, by));
+  TRYS(dalloc(&d.tblock_scene, Tc / kSceneBlock, by));
+  TRYS(dalloc(&d.rblock_scene, std::max<uint64_t>(1, Rc / kSceneBlock), by));
+  TRYS(dalloc(&d.cblock_scene, std::max<uint64_t>(1, Cc / kSceneBlock), by));
+  d.partial_capacity = partial_blocks_needed(uint32_t(Vc), uint32_t(Tc), uint32_t(Rc), uint32_t(Cc));
+  TRYS(dalloc(reinterpret_cast<char**>(&d.partials), partial_bytes(d.partial_capacity), by));
+  // Free blocks belong to no scene; scene slots start inactive.
+  hipStream_t q = b->stream;
+  std::vector<uint32_t> none(std::max({Vc, Tc, Rc, Cc}) / kSceneBlock + 1, kNoScene);
+  HIP_CHECK(hipMemcpyAsync(d.vblock_scene, none.data(), 4 * (Vc / kSceneBlock), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipMemcpyAsync(d.tblock_scene, none.data(), 4 * (Tc / kSceneBlock), hipMemcpyHostToDevice, q));
+  if (Rc) HIP_CHECK(hipMemcpyAsync(d.rblock_scene, none.data(), 4 * (Rc / kSceneBlock), hipMemcpyHostToDevice, q));
+  if (Cc) HIP_CHECK(hipMemcpyAsync(d.cblock_scene, none.data(), 4 * (Cc / kSceneBlock), hipMemcpyHostToDevice, q));
+  std::vector<uint4> zero_cnt(S, make_uint4(0, 0, 0, 0));
+  HIP_CHECK(hipMemcpyAsync(d.scene_cnt, zero_cnt.data(), S * sizeof(uint4), hipMemcpyHostToDevice, q));
+  std::vector<uint4> no_body(Bc, make_uint4(0, 0, 0, kNoScene));
+  HIP_CHECK(hipMemcpyAsync(d.body_rng, no_body.data(), Bc * sizeof(uint4), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipStreamSynchronize(q));
+  b->va.init(Vc / kSceneBlock);
+  b->ta.init(Tc / kSceneBlock);
+  b->ra.init(Rc / kSceneBlock);
+  b->ca.init(Cc / kSceneBlock);
+  b->adja.init(4 * Tc);
+  b->hina.init(Hc);
+  b->tela.init(3 * Rc);
+  b->hela.init(4 * Hc);
+  b->cela.init(4 * Cc);
+  b->bodya.init(Bc);
+  b->slota.init(S);
+  b->slots.assign(S, nullptr);
+  return JGS2_OK;
+}
+
+void release_spans(jgs2_batch* b, jgs2_solver* s) {
+  b->va.release(s->vblk.at, s->vblk.n);
+  b->ta.release(s->tblk.at, s->tblk.n);
+  b->ra.release(s->rblk.at, s->rblk.n);
+  b->ca.release(s->cblk.at, s->cblk.n);
+  b->adja.release(s->adj.at, s->adj.n);
+  b->hina.release(s->hin.at, s->hin.n);
+  b->tela.release(s->tel.at, s->tel.n);
+  b->hela.release(s->hel.at, s->hel.n);
+  b->cela.release(s->cel.at, s->cel.n);
+  b->bodya.release(s->body.at, s->body.n);
+  b->slota.release(s->slot, 1);
+}
+
+template <class T> hipError_t upload(T* base, uint64_t at, const std::vector<T>& v, hipStream_t q) {
+  if (v.empty()) return hipSuccess;
+  return hipMemcpyAsync(base + at, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, q);
+}
+
+// Packs one validated scene into freshly allocated spans of the arena
+// (packJGS2GpuInput, src/scenes/index.ts:975-1109, with the cloth arena and
+// IPC buffers of cloth-layout.ts and ipc-contact-layout.ts), in device order
+// and with arena-absolute indices.
+jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_input& in) {
+  DeviceState& d = b->dev;
+  hipStream_t q = b->stream;
+  const uint32_t V = s->V, T = s->T, K = b->cfg.cubature_k, NT = s->NT, NH = s->NH, NC = s->NC;
+  const uint64_t VB = s->vbase(), TB = s->tbase(), RB = s->rbase(), CB = s->cbase();
+  const uint64_t Vp = s->vblk.n * kSceneBlock;  // padded vertex count
+  const std::vector<uint32_t>& vnew = s->vnew;
+  const std::vector<uint32_t>& vold = s->vold;
+  const std::vector<uint32_t>& tnew = s->tnew;
+  const std::vector<uint32_t>& told = s->told;
+  // Vertex records; padding vertices are pinned, massless and unattached.
+  std::vector<float4> vtx_rest(Vp, make_float4(0, 0, 0, 0)), positions(Vp, make_float4(0, 0, 0, 1)),
+      velocities(Vp, make_float4(0, 0, 0, 0)), rot_v(3 * Vp), zeros4(Vp, make_float4(0, 0, 0, 0));
+  std::vector<uint4> vtx_info(Vp, make_uint4(uint32_t(s->adj.at), 0, 1, kNoScene));
+  std::vector<uint32_t> adjacency(s->adj.n);
+  {
+    uint32_t cursor = 0;
+    for (uint32_t nv = 0; nv < V; ++nv) {
+      const uint32_t ov = vold[nv];
+      const float* r = &in.vertex_rest[4 * size_t(ov)];
+      const uint32_t* i = &in.vertex_info[4 * size_t(ov)];
+      vtx_rest[nv] = make_float4(r[0], r[1], r[2], r[3]);
+      vtx_info[nv] = make_uint4(uint32_t(s->adj.at) + cursor, i[1], i[2], uint32_t(s->body.at) + i[3]);
+      // Rows stay ascending in caller tet ids (the reference's summation
+      // order); entries are arena tet ids.
+      for (uint32_t k = 0; k < i[1]; ++k) adjacency[cursor + k] = uint32_t(TB) + tnew[in.adjacency[i[0] + k]];
+      cursor += i[1];
+      const float* x = &in.positions[4 * size_t(ov)];
+      positions[nv] = make_float4(x[0], x[1], x[2], 1.0f);
+      if (in.velocities)
+        velocities[nv] = make_float4(in.velocities[4 * size_t(ov)], in.velocities[4 * size_t(ov) + 1],
+                                     in.velocities[4 * size_t(ov) + 2], 0.0f);
+    }
+  }
+  for (uint64_t v = 0; v < Vp; ++v) {
+    rot_v[3 * v] = make_float4(1, 0, 0, 0);
+    rot_v[3 * v + 1] = make_float4(0, 1, 0, 0);
+    rot_v[3 * v + 2] = make_float4(0, 0, 1, 0);
+  }
+  std::vector<float4> dev_objectives(2 * Vp, make_float4(0, 0, 0, 0));
+  for (uint32_t nv = 0; nv < V; ++nv) {
+    const float* o = &s->objectives[8 * size_t(vold[nv])];
+    dev_objectives[2 * size_t(nv)] = make_float4(o[0], o[1], o[2], o[3]);
+    dev_objectives[2 * size_t(nv) + 1] = make_float4(o[4], o[5], o[6], o[7]);
+  }
+  // Tetrahedra.
+  std::vector<uint4> tet_idx(T);
+  std::vector<float4> tet_dm(3 * size_t(T)), rot_t(3 * size_t(T));
+  std::vector<float> tet_k;
+  if (s->corotated) tet_k.resize(144 * size_t(T));
+  for (uint32_t nt = 0; nt < T; ++nt) {
+    const uint32_t ot = told[nt];
+    const uint32_t* i = &in.tet_indices[4 * size_t(ot)];
+    const float* m = &in.tet_inverse_dm[12 * size_t(ot)];
+    const float* a = &in.tet_meta[4 * size_t(ot)];
+    tet_idx[nt] = make_uint4(uint32_t(VB) + vnew[i[0]], uint32_t(VB) + vnew[i[1]], uint32_t(VB) + vnew[i[2]],
+                             uint32_t(VB) + vnew[i[3]]);
+    tet_dm[3 * size_t(nt)] = make_float4(m[0], m[1], m[2], a[0]);
+    tet_dm[3 * size_t(nt) + 1] = make_float4(m[4], m[5], m[6], a[1]);
+    tet_dm[3 * size_t(nt) + 2] = make_float4(m[8], m[9], m[10], a[2]);
+    if (s->corotated) std::memcpy(&tet_k[144 * size_t(nt)], &in.tet_rest_stiffness[144 * size_t(ot)], 144 * sizeof(float));
+    rot_t[3 * size_t(nt)] = make_float4(1, 0, 0, 0);
+    rot_t[3 * size_t(nt) + 1] = make_float4(0, 1, 0, 0);
+    rot_t[3 * size_t(nt) + 2] = make_float4(0, 0, 1, 0);
+  }
+  // Cubature records, K per vertex; unused records are empty.
+  const uint32_t Kin = in.cubature_k;
+  std::vector<uint4> cub(10 * Vp * std::max<uint32_t>(K, 1), make_uint4(0, 0, 0, 0));
+  for (uint64_t r = 0; r < Vp * K; ++r) cub[10 * r] = make_uint4(0xffffffffu, 0, 0, 0);
+  for (uint32_t nv = 0; nv < V; ++nv)
+    for (uint32_t k = 0; k < Kin; ++k) {
+      const size_t ro = size_t(vold[nv]) * Kin + k, rn = size_t(nv) * K + k;
+      uint32_t wbits;
+      std::memcpy(&wbits, &in.cubature_weights[ro], 4);
+      const uint32_t t = in.cubature_tet_ids[ro];
+      cub[10 * rn] = make_uint4(t < T ? uint32_t(TB) + tnew[t] : 0xffffffffu, wbits, 0, 0);
+      std::memcpy(&cub[10 * rn + 1], &in.cubature_basis[36 * ro], 36 * sizeof(float));
+    }
+  // Cloth records, vertex -> element CSR rows {begin, count}.
+  std::vector<uint2> tri_row(Vp, make_uint2(uint32_t(s->tel.at), 0)), hinge_row(Vp, make_uint2(uint32_t(s->hel.at), 0)),
+      cand_row(Vp, make_uint2(uint32_t(s->cel.at), 0));
+  std::vector<uint4> tri_ix(NT), hinge_ix(NH), cand(NC);
+  std::vector<float4> tri_inv(NT), hinge_rest(NH);
+  std::vector<float> tri_area(NT);
+  std::vector<uint32_t> tri_elems, hinge_elems, cand_ids, cand_meta(NC, 0);
+  auto rows_of = [&](const uint32_t* idx, uint32_t width, uint32_t count, uint64_t el_base, uint64_t id_base,
+                     std::vector<uint2>& rows, std::vector<uint32_t>& elems) -> jgs2_status {
+    std::vector<uint32_t> crow(size_t(V) + 1), celem(size_t(count) * width);
+    TRYS(jgs2_build_element_incidence(V, idx, width, count, crow.data(), celem.data()));
+    elems.clear();
+    for (uint32_t nv = 0; nv < V; ++nv) {
+      const uint32_t ov = vold[nv];
+      rows[nv] = make_uint2(uint32_t(el_base + elems.size()), crow[ov + 1] - crow[ov]);
+      for (uint32_t k = crow[ov]; k < crow[ov + 1]; ++k) elems.push_back(uint32_t(id_base) + celem[k]);
+    }
+    return JGS2_OK;
+  };
+  if (in.cloth) {
+    const jgs2_cloth_input& c = *in.cloth;
+    for (uint32_t t = 0; t < NT; ++t) {
+      const uint32_t* ix = &c.triangle_indices[3 * size_t(t)];
+      const uint32_t owner = std::min(ix[0], std::min(ix[1], ix[2]));  // S:1838-1840
+      tri_ix[t] = make_uint4(uint32_t(VB) + vnew[ix[0]], uint32_t(VB) + vnew[ix[1]], uint32_t(VB) + vnew[ix[2]],
+                             uint32_t(VB) + vnew[owner]);
+      const float* bb = &c.triangle_inverse_rest_bases[4 * size_t(t)];
+      tri_inv[t] = make_float4(bb[0], bb[1], bb[2], bb[3]);
+      tri_area[t] = c.triangle_rest_areas[t];
+    }
+    for (uint32_t h = 0; h < NH; ++h) {
+      const uint32_t* ix = &c.hinge_indices[4 * size_t(h)];
+      const uint32_t owner = std::min(std::min(ix[0], ix[1]), std::min(ix[2], ix[3]));  // S:1842-1844
+      hinge_ix[h] = make_uint4(uint32_t(VB) + vnew[ix[0]], uint32_t(VB) + vnew[ix[1]], uint32_t(VB) + vnew[ix[2]],
+                               uint32_t(VB) + vnew[ix[3]]);
+      const uint32_t ob = uint32_t(VB) + vnew[owner];
+      float of;
+      std::memcpy(&of, &ob, 4);
+      hinge_rest[h] = make_float4(c.hinge_rest_angles[h], c.hinge_rest_edge_lengths[h], of, 0.0f);
+    }
+    TRYS(rows_of(c.triangle_indices, 3, NT, s->tel.at, RB, tri_row, tri_elems));
+    TRYS(rows_of(c.hinge_indices, 4, NH, s->hel.at, s->hin.at, hinge_row, hinge_elems));
+  }
+  if (NC > 0) {
+    for (uint32_t c = 0; c < NC; ++c) {
+      const uint32_t* ix = &in.contact->packed_indices[4 * size_t(c)];
+      cand[c] = make_uint4(uint32_t(VB) + vnew[ix[0]], uint32_t(VB) + vnew[ix[1]], uint32_t(VB) + vnew[ix[2]],
+                           uint32_t(VB) + vnew[ix[3]]);
+      cand_meta[c] = c < s->NVT ? 0u : 1u;
+    }
+    TRYS(rows_of(in.contact->packed_indices, 4, NC, s->cel.at, CB, cand_row, cand_ids));
+  }
+#define UP(base, at, vec) HIP_CHECK(upload((base), (at), (vec), q))
+  UP(d.vtx_rest, VB, vtx_rest);
+  UP(d.vtx_info, VB, vtx_info);
+  UP(d.adjacency, s->adj.at, adjacency);
+  UP(d.objectives, 2 * VB, dev_objectives);
+  for (float4* region : {d.pos[0], d.pos[1], d.predicted, d.old}) UP(region, VB, positions);
+  UP(d.velocity, VB, velocities);
+  UP(d.vrot, 3 * VB, rot_v);
+  UP(d.vrot_next, 3 * VB, rot_v);
+  for (float4* region : {d.final_update, d.vertex_energy}) UP(region, VB, zeros4);
+  {
+    std::vector<float4> z(5 * Vp, make_float4(0, 0, 0, 0));
+    HIP_CHECK(hipMemcpyAsync(d.local_diag + 4 * VB, z.data(), 4 * Vp * sizeof(float4), hipMemcpyHostToDevice, q));
+    HIP_CHECK(hipMemcpyAsync(d.conv + 5 * VB, z.data(), 5 * Vp * sizeof(float4), hipMemcpyHostToDevice, q));
+  }
+  UP(d.cub, 10 * VB * std::max<uint32_t>(K, 1), cub);
+  UP(d.tet_idx, TB, tet_idx);
+  UP(d.tet_dm, 3 * TB, tet_dm);
+  if (s->corotated) UP(d.tet_k, 144 * TB, tet_k);
+  UP(d.trot, 3 * TB, rot_t);
+  UP(d.tri_row, VB, tri_row);
+  UP(d.hinge_row, VB, hinge_row);
+  UP(d.cand_row, VB, cand_row);
+  {
+    std::vector<uint32_t> zc(Vp, 0);
+    UP(d.active_count, VB, zc);
+  }
+  UP(d.tri_ix, RB, tri_ix);
+  UP(d.tri_inv, RB, tri_inv);
+  UP(d.tri_area, RB, tri_area);
+  UP(d.tri_elems, s->tel.at, tri_elems);
+  UP(d.hinge_ix, s->hin.at, hinge_ix);
+  UP(d.hinge_rest, s->hin.at, hinge_rest);
+  UP(d.hinge_elems, s->hel.at, hinge_elems);
+  if (NC > 0) {  // packIpcContactBuffer initial records (ipc-contact-layout.ts:302-341)
+    std::vector<float4> z(NC, make_float4(0, 0, 0, 0)), scratch(NC, make_float4(1, 0, 0, 0)),
+        none(NC, make_float4(0, 0, 0, -1));
+    UP(d.cand, CB, cand);
+    UP(d.cand_meta, CB, cand_meta);
+    for (float4* a : {d.lag_n, d.lag_w, d.src_w, d.tgt_w}) UP(a, CB, z);
+    UP(d.cscratch, CB, scratch);
+    UP(d.src_c, CB, none);
+    UP(d.tgt_c, CB, none);
+    UP(d.cand_ids, s->cel.at, cand_ids);
+    UP(d.active_ids, s->cel.at, cand_ids);
+  }
+  // Bodies: every body of the scene scans the scene's vertex range.
+  {
+    std::vector<uint4> rng(s->body.n, make_uint4(uint32_t(VB), uint32_t(VB + V), s->objective_flags, s->slot));
+    std::vector<float4> z(s->body.n, make_float4(0, 0, 0, 0));
+    UP(d.body_rng, s->body.at, rng);
+    UP(d.body_corr, s->body.at, z);
+  }
+  // Block tables and scene records.
+  const uint32_t sl = s->slot;
+  {
+    std::vector<uint32_t> tab(std::max({s->vblk.n, s->tblk.n, s->rblk.n, s->cblk.n}), sl);
+    HIP_CHECK(hipMemcpyAsync(d.vblock_scene + s->vblk.at, tab.data(), 4 * s->vblk.n, hipMemcpyHostToDevice, q));
+    HIP_CHECK(hipMemcpyAsync(d.tblock_scene + s->tblk.at, tab.data(), 4 * s->tblk.n, hipMemcpyHostToDevice, q));
+    if (s->rblk.n) HIP_CHECK(hipMemcpyAsync(d.rblock_scene + s->rblk.at, tab.data(), 4 * s->rblk.n, hipMemcpyHostToDevice, q));
+    if (s->cblk.n) HIP_CHECK(hipMemcpyAsync(d.cblock_scene + s->cblk.at, tab.data(), 4 * s->cblk.n, hipMemcpyHostToDevice, q));
+  }
+  const uint4 blk0 = make_uint4(uint32_t(s->vblk.at), uint32_t(s->vblk.at + s->vblk.n), uint32_t(s->tblk.at),
+                                uint32_t(s->tblk.at + s->tblk.n));
+  const uint4 blk1 = make_uint4(uint32_t(s->rblk.at), uint32_t(s->rblk.at + s->rblk.n), uint32_t(s->cblk.at),
+                                uint32_t(s->cblk.at + s->cblk.n));
+  const uint4 cnt = make_uint4(V, T, s->color_count, s->bodies);
+  const uint4 end = make_uint4(uint32_t(VB + V), uint32_t(TB + T), uint32_t(RB + NT), uint32_t(CB + NC));
+  const float4 ctl = make_float4(0, 0, 0, 1);
+  std::vector<float4> zero_ctl(4, make_float4(0, 0, 0, 0)), zero_hist(size_t(kHistoryCapacity) * 8, make_float4(0, 0, 0, 0));
+  HIP_CHECK(hipMemcpyAsync(d.scene_blk0 + sl, &blk0, sizeof(uint4), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipMemcpyAsync(d.scene_blk1 + sl, &blk1, sizeof(uint4), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipMemcpyAsync(d.scene_end + sl, &end, sizeof(uint4), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipMemcpyAsync(d.scene_scale + sl, &s->scene_scale, sizeof(float), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipMemcpyAsync(d.contact_ctl + sl, &ctl, sizeof(float4), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipMemcpyAsync(d.control + 4 * size_t(sl), zero_ctl.data(), 4 * sizeof(float4), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipMemcpyAsync(d.history + size_t(sl) * kHistoryCapacity * 8, zero_hist.data(),
+                           zero_hist.size() * sizeof(float4), hipMemcpyHostToDevice, q));
+  HIP_CHECK(hipMemcpyAsync(d.scene_cnt + sl, &cnt, sizeof(uint4), hipMemcpyHostToDevice, q));
+#undef UP
+  HIP_CHECK(hipStreamSynchronize(q));
+  return JGS2_OK;
+}
+
 // ---------------------------------------------------------------- pass parameters
-PassParams make_params(const jgs2_solver* s, const jgs2_settings& st, bool stop) {
+PassParams make_params(jgs2_batch* b, const jgs2_settings& st, bool stop) {
   PassParams p{};
-  p.vertex_count = s->V;
-  p.tet_count = s->T;
-  p.cubature_k = s->K;
-  p.body_count = s->bodies;
-  p.objective_flags = s->objective_flags;
-  p.globalization = s->globalization ? 1u : 0u;
-  p.stop_enabled = (s->globalization && stop) ? 1u : 0u;
-  p.corotated = s->corotated ? 1u : 0u;
-  p.tri_count = s->dev.NT;
-  p.hinge_count = s->dev.NH;
+  b->dev.NC = uint32_t(b->ca.high() * kSceneBlock);
+  uint32_t flags = 0;
+  for (auto* s : b->slots)
+    if (s) flags |= s->objective_flags;
+  p.vertex_count = uint32_t(b->va.high() * kSceneBlock);
+  p.tet_count = uint32_t(b->ta.high() * kSceneBlock);
+  p.tri_count = uint32_t(b->ra.high() * kSceneBlock);
+  p.hinge_count = uint32_t(b->hina.high());
+  p.body_count = uint32_t(b->bodya.high());
+  p.scene_count = uint32_t(b->slota.high());
+  p.cubature_k = b->cfg.cubature_k;
+  p.objective_flags = flags;
+  p.globalization = b->corotated ? 0u : 1u;
+  p.stop_enabled = (!b->corotated && stop) ? 1u : 0u;
+  p.corotated = b->corotated ? 1u : 0u;
   // contactEnabled (jgs2-solver.ts:2213-2214).
-  p.contact = (s->dev.NC > 0 && st.ipc_barrier_stiffness > 0) ? 1u : 0u;
+  p.contact = (b->dev.NC > 0 && st.ipc_barrier_stiffness > 0) ? 1u : 0u;
+  p.solve_block = b->solve_block;
+  p.xcd_remap = b->xcd_remap;
   // packUniforms (jgs2-solver.ts:604-730): f64 expressions rounded to f32.
   const double inv = 1.0 / st.timestep;
   p.dt = round_f32(st.timestep);
@@ -603,34 +1049,67 @@ PassParams make_params(const jgs2_solver* s, const jgs2_settings& st, bool stop)
   p.ipc_kappa = round_f32(st.ipc_barrier_stiffness);
   p.residual_tol = round_f32(st.residual_tolerance);
   p.update_tol = round_f32(st.normalized_update_tolerance);
-  p.scene_scale = s->scene_scale;
   return p;
 }
 
-void record_solve(jgs2_solver* s, const PassParams& p) {
-  if (s->profiling && s->event_used + 2 <= s->events.size()) {
-    (void)hipEventRecord(s->events[s->event_used], s->stream);
-    launch_solve(s->dev, p, s->stream);
-    (void)hipEventRecord(s->events[s->event_used + 1], s->stream);
-    s->event_used += 2;
-    return;
+// The batch's color-sorted vertex list: color c holds every resident
+// scene's color-c vertices, ascending in arena id (vertex-coloring.ts:92-107
+// per scene).
+jgs2_status sync_colors(jgs2_batch* b) {
+  if (b->synced_version == b->version) return JGS2_OK;
+  std::vector<jgs2_solver*> order;
+  uint32_t cc = 1;
+  for (auto* s : b->slots)
+    if (s) {
+      order.push_back(s);
+      cc = std::max(cc, s->color_count);
+    }
+  std::sort(order.begin(), order.end(), [](jgs2_solver* a, jgs2_solver* c) { return a->vblk.at < c->vblk.at; });
+  b->color_count = cc;
+  b->color_offsets.assign(cc + 1, 0);
+  std::vector<uint32_t> list;
+  for (uint32_t c = 0; c < cc; ++c) {
+    for (auto* s : order)
+      if (c < s->color_count)
+        list.insert(list.end(), s->color_order.begin() + s->color_offsets[c], s->color_order.begin() + s->color_offsets[c + 1]);
+    b->color_offsets[c + 1] = uint32_t(list.size());
   }
-  launch_solve(s->dev, p, s->stream);
+  if (!list.empty()) {
+    HIP_CHECK(hipMemcpyAsync(b->dev.color_order, list.data(), 4 * list.size(), hipMemcpyHostToDevice, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+  }
+  b->synced_version = b->version;
+  b->graph_valid = false;
+  return JGS2_OK;
 }
 
-// encodeFrame (jgs2-solver.ts:2207-2494), one HIP launch per pass.
-void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations, bool stop) {
-  const DeviceState& d = s->dev;
-  hipStream_t q = s->stream;
-  PassParams base = make_params(s, st, stop);
+void record_solve(jgs2_batch* b, const PassParams& p) {
+  if (b->profiling && b->event_used + 2 <= b->events.size()) {
+    (void)hipEventRecord(b->events[b->event_used], b->stream);
+    launch_solve(b->dev, p, b->stream);
+    (void)hipEventRecord(b->events[b->event_used + 1], b->stream);
+    b->event_used += 2;
+    return;
+  }
+  launch_solve(b->dev, p, b->stream);
+}
+
+// encodeFrame (jgs2-solver.ts:2207-2494), one launch per pass over every
+// resident scene.
+void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, bool stop) {
+  const DeviceState& d = b->dev;
+  hipStream_t q = b->stream;
+  PassParams base = make_params(b, st, stop);
+  const bool globalization = !b->corotated;
   const bool gs = st.schedule == JGS2_SCHEDULE_GRAPH_COLORED_GAUSS_SEIDEL;
+  const bool cloth = base.tri_count != 0u || base.hinge_count != 0u;
+  const bool contact = base.contact != 0u;
   base.src = 0;
   base.tgt = 0;
-  const bool contact = base.contact != 0u;
-  if (s->globalization) launch_clear_frame(d, base, q);
+  if (globalization) launch_clear_frame(d, base, q);
   launch_predict(d, base, q);
   if (contact) launch_lag_contact(d, base, q);
-  if (s->globalization) launch_reduce_candidate(d, base, q);  // source feasibility preflight
+  if (globalization) launch_reduce_candidate(d, base, q);  // source feasibility preflight
   for (uint32_t it = 0; it < iterations; ++it) {
     PassParams p = base;
     p.src = (it % 2 == 0) ? 1u : 0u;
@@ -638,27 +1117,28 @@ void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations,
     if (contact) launch_build_active_rows(d, p, q);
     if (gs) {
       launch_copy_position(d, p, q);
-      for (uint32_t c = 0; c < s->color_count; ++c) {
+      for (uint32_t c = 0; c < b->color_count; ++c) {
         PassParams pc = p;
         pc.src = p.tgt;
         pc.colored = 1;
-        pc.color_begin = s->color_offsets[c];
-        pc.color_end = s->color_offsets[c + 1];
-        if (s->corotated) launch_tet_polar(d, pc, q);
+        pc.color_index = c;
+        pc.color_begin = b->color_offsets[c];
+        pc.color_end = b->color_offsets[c + 1];
+        if (b->corotated) launch_tet_polar(d, pc, q);
         launch_vertex_polar(d, pc, q);
-        record_solve(s, pc);
+        record_solve(b, pc);
       }
-      if (s->globalization && !contact) launch_vertex_energy(d, p, q);
+      if (globalization && !contact) launch_vertex_energy(d, p, q);
     } else {
       // Sweeps after the first read the frames the previous convergence pass
       // computed at this sweep's source pose.
-      const bool fused = s->globalization && it > 0;
-      if (s->corotated) launch_tet_polar(d, p, q);
+      const bool fused = globalization && it > 0;
+      if (b->corotated) launch_tet_polar(d, p, q);
       if (!fused) launch_vertex_polar(d, p, q);
       PassParams ps = p;
       ps.fused_rot = fused ? 1u : 0u;
-      record_solve(s, ps);
-      if (s->globalization && (d.NT || d.NH) && !contact) {
+      record_solve(b, ps);
+      if (globalization && cloth && !contact) {
         PassParams pe = p;
         pe.energy_after_sweep = 1;
         launch_vertex_energy(d, pe, q);
@@ -666,9 +1146,9 @@ void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations,
     }
     if (contact) {  // safe step, validation, then the energy pair (jgs2-solver.ts:2347-2400)
       launch_contact_step(d, p, q);
-      if (s->globalization) launch_vertex_energy(d, p, q);
+      if (globalization) launch_vertex_energy(d, p, q);
     }
-    if (s->globalization) {
+    if (globalization) {
       launch_reduce_candidate(d, p, q);
       launch_apply_candidate(d, p, q);
       if (contact) launch_promote_contact(d, p, q);
@@ -687,33 +1167,40 @@ void enqueue_frame(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations,
   launch_finalize(d, base, q);
 }
 
-jgs2_status check_usable(jgs2_solver* s) {
-  if (!s) return fail(JGS2_ERROR_TYPE, "NULL solver.");
-  if (s->destroyed) return fail(JGS2_ERROR_STATE, "JGS2GpuSolver has been destroyed.");
+uint64_t fnv(uint64_t h, const void* data, size_t n) {
+  const unsigned char* c = static_cast<const unsigned char*>(data);
+  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  return h;
+}
+
+jgs2_status check_batch(jgs2_batch* b) {
+  if (!b) return fail(JGS2_ERROR_TYPE, "NULL batch.");
   return JGS2_OK;
 }
 
-// resolveStepSettings (jgs2-solver.ts:1982-2009).
-jgs2_status resolve_step(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask, const char* api, jgs2_settings& out) {
+// resolveStepSettings (jgs2-solver.ts:1982-2009) for the batch.
+jgs2_status resolve_batch_step(jgs2_batch* b, const jgs2_settings* ov, uint64_t mask, const char* api,
+                               jgs2_settings& out) {
   if (ov && (mask & JGS2_SET_TIMESTEP)) {
-    const double pre = s->preprocessing_timestep, run = ov->timestep;
+    const double pre = b->defaults.timestep, run = ov->timestep;
     if (!(std::isfinite(run) && run > 0 && float(pre) == float(run)))
       return fail(JGS2_ERROR_RANGE,
                   "JGS2 was precomputed for timestep %.17g, but %s() requested %.17g. Regenerate the precomputation "
                   "before changing timestep.",
                   pre, api, run);
   }
-  if (s->dev.NC > 0 && ov && (mask & JGS2_SET_IPC_MINIMUM_DISTANCE) &&
-      float(ov->ipc_minimum_distance) != float(s->defaults.ipc_minimum_distance))
+  bool any_contact = false;
+  for (auto* s : b->slots)
+    if (s) any_contact |= s->NC > 0;
+  if (any_contact && ov && (mask & JGS2_SET_IPC_MINIMUM_DISTANCE) &&
+      float(ov->ipc_minimum_distance) != float(b->defaults.ipc_minimum_distance))
     return fail(JGS2_ERROR_RANGE,
                 "ipcMinimumDistance is fixed when an IPC solver is created; recreate the solver before changing it.");
-  jgs2_status st = merge_settings(s->defaults, ov, mask, out);
+  jgs2_status st = merge_settings(b->defaults, ov, mask, out);
   if (st != JGS2_OK) return st;
   // Objectives and mesh contact keep their solved centre-of-mass motion
-  // (jgs2-solver.ts:1994-2008).
-  const bool contact = s->dev.NC > 0 && out.ipc_barrier_stiffness > 0;
-  if (s->globalization && (s->objective_flags != 0 || contact) && out.horizontal_body_correction)
-    out.horizontal_body_correction = 0;
+  // (jgs2-solver.ts:1994-2008); the body-correction kernels apply that rule
+  // per scene.
   return JGS2_OK;
 }
 
@@ -722,19 +1209,51 @@ uint32_t normalize_odd(int32_t iterations) {  // normalizeOddIterationCount (lay
   return uint32_t(i % 2 == 0 ? i + 1 : i);
 }
 
-jgs2_status submit(jgs2_solver* s, const jgs2_settings& st, uint32_t iterations, uint32_t frames, bool stop) {
-  if (s->globalization) {
+// submitFrames (jgs2-solver.ts:2026-2072): limits, then the frames, replayed
+// from a captured hipGraph when graphs are enabled.
+jgs2_status submit(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, uint32_t frames, bool stop) {
+  TRYS(sync_colors(b));
+  if (!b->corotated) {
     if (iterations > kHistoryCapacity)
       return fail(JGS2_ERROR_RANGE, "Stable JGS2 records at most %u nonlinear iterations per frame; got %u.",
                   kHistoryCapacity, iterations);
-    const uint32_t sub = st.schedule == JGS2_SCHEDULE_GRAPH_COLORED_GAUSS_SEIDEL ? s->color_count : 1u;
+    const uint32_t sub = st.schedule == JGS2_SCHEDULE_GRAPH_COLORED_GAUSS_SEIDEL ? b->color_count : 1u;
     if (uint64_t(iterations) * sub > uint64_t(JGS2_MAX_GLOBALIZED_ITERATIONS_PER_SUBMISSION / frames))
       return fail(JGS2_ERROR_RANGE,
                   "Stable JGS2 may encode at most %u total nonlinear iterations per submission; got %u frames * %u "
                   "iterations.",
                   JGS2_MAX_GLOBALIZED_ITERATIONS_PER_SUBMISSION, frames, iterations * sub);
   }
-  for (uint32_t f = 0; f < frames; ++f) enqueue_frame(s, st, iterations, stop);
+  if (b->active_scenes() == 0) return fail(JGS2_ERROR_STATE, "The batch holds no scene.");
+  if (b->use_graphs && !b->profiling) {
+    uint64_t key = 1469598103934665603ull;
+    key = fnv(key, &b->version, sizeof(b->version));
+    key = fnv(key, &st, sizeof(st));
+    key = fnv(key, &iterations, sizeof(iterations));
+    const uint32_t flags[3] = {stop ? 1u : 0u, b->solve_block, b->xcd_remap};
+    key = fnv(key, flags, sizeof(flags));
+    for (auto* s : b->slots) {
+      const uint32_t f = s ? s->objective_flags : 0u;
+      key = fnv(key, &f, sizeof(f));
+    }
+    if (!b->graph_valid || key != b->graph_key) {
+      if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+      b->graph_exec = nullptr;
+      b->graph_valid = false;
+      hipGraph_t graph = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_frame(b, st, iterations, stop);
+      HIP_CHECK(hipStreamEndCapture(b->stream, &graph));
+      hipError_t e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(JGS2_ERROR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+      b->graph_key = key;
+      b->graph_valid = true;
+    }
+    for (uint32_t f = 0; f < frames; ++f) HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
+  } else {
+    for (uint32_t f = 0; f < frames; ++f) enqueue_frame(b, st, iterations, stop);
+  }
   HIP_CHECK(hipGetLastError());
   clear_error();
   return JGS2_OK;
@@ -759,22 +1278,364 @@ void recount_objectives(jgs2_solver* s) {  // computePackedObjectiveActivity (jg
   s->objective_flags = (forces ? 1u : 0u) | (targets ? 2u : 0u);
 }
 
+// Validates a scene input and fills the host-side scene record (everything
+// but the arena spans) under the batch's resolved settings. Runs entirely
+// on the host, before any device is touched (layout.ts:223-416,
+// jgs2-solver.ts:940-1066).
+jgs2_status prepare_scene(const jgs2_scene_input& in, const jgs2_settings& resolved, jgs2_solver* s) {
+  TRYS(validate_input(in));
+  const uint32_t V = in.vertex_count, T = in.tet_count, K = in.cubature_k;
+  const bool corotated = in.tet_meta[3] == JGS2_MATERIAL_COROTATED_LINEAR;
+  const bool globalization = !corotated;
+  const uint32_t NT = in.cloth ? in.cloth->triangle_count : 0u, NH = in.cloth ? in.cloth->hinge_count : 0u;
+  if (in.cloth) {
+    TRYS(validate_cloth(*in.cloth, V));
+    if (NT > 0 && !globalization)  // jgs2-solver.ts:998-1001
+      return fail(JGS2_ERROR_RANGE, "Triangle cloth requires the stable Neo-Hookean production path.");
+  }
+  const uint32_t NC = in.contact ? in.contact->vertex_triangle_count + in.contact->edge_edge_count : 0u;
+  if (in.contact) TRYS(validate_contact(*in.contact, V));
+  if (NC > 0) {  // initially feasible contact distances (jgs2-solver.ts:1041-1066)
+    const double minimum = min_static_contact_distance(in.positions, *in.contact);
+    const double margin = 8.0 * 0x1p-24 * std::max({1.0, resolved.ipc_activation_distance, std::fabs(minimum)});
+    const double required = resolved.ipc_minimum_distance + margin;
+    if (!std::isfinite(minimum) || !(minimum > required))
+      return fail(JGS2_ERROR_RANGE,
+                  "IPC candidates require an initially feasible source pose with distance greater than %.17g "
+                  "(minimum plus an f32 safety margin); got %.17g.",
+                  required, minimum);
+  }
+  uint32_t color_count = 0;
+  TRYS(color_vertices(in, s->colors, color_count));
+  if (color_count > kMaxColor + 1)
+    return fail(JGS2_ERROR_RANGE,
+                "Colored Gauss-Seidel supports at most %u colors; the small-scene conflict graph needs %u.",
+                kMaxColor + 1, color_count);
+  s->V = V;
+  s->objectives.assign(8 * size_t(V), 0.0f);
+  for (uint32_t v = 0; v < V; ++v) {  // packJGS2VertexObjectives (layout.ts:680-716)
+    float* o = &s->objectives[8 * size_t(v)];
+    if (in.external_forces) {
+      o[0] = in.external_forces[3 * size_t(v)];
+      o[1] = in.external_forces[3 * size_t(v) + 1];
+      o[2] = in.external_forces[3 * size_t(v) + 2];
+    }
+    o[4] = in.target_positions ? in.target_positions[3 * size_t(v)] : in.positions[4 * size_t(v)];
+    o[5] = in.target_positions ? in.target_positions[3 * size_t(v) + 1] : in.positions[4 * size_t(v) + 1];
+    o[6] = in.target_positions ? in.target_positions[3 * size_t(v) + 2] : in.positions[4 * size_t(v) + 2];
+    o[7] = in.target_stiffnesses ? in.target_stiffnesses[v] : 0.0f;
+  }
+  recount_objectives(s);
+  if (!globalization && s->objective_flags != 0)
+    return fail(JGS2_ERROR_RANGE,
+                "Active external forces and quadratic targets require the stable Neo-Hookean production path.");
+  if (globalization) {
+    const double minimum = min_input_determinant(in);
+    if (!std::isfinite(minimum) || !(minimum > kDeterminantFloor))
+      return fail(JGS2_ERROR_RANGE,
+                  "A stable Neo-Hookean production solve requires an initially feasible pose with minimum J greater "
+                  "than 0.0001; got %.17g.",
+                  minimum);
+  }
+  s->defaults = resolved;
+  s->preprocessing_timestep = resolved.timestep;
+  s->globalization = globalization;
+  s->corotated = corotated;
+  s->T = T;
+  s->K = K;
+  s->NT = NT;
+  s->NH = NH;
+  s->NC = NC;
+  s->NVT = in.contact ? in.contact->vertex_triangle_count : 0u;
+  s->has_cloth = NT + NH > 0;
+  if (s->has_cloth) {
+    const double E = in.cloth->young_modulus, nu = in.cloth->poisson_ratio;
+    s->cloth_mat[0] = float(E * nu / (1 - nu * nu));
+    s->cloth_mat[1] = float(E / (2 * (1 + nu)));
+    s->cloth_mat[2] = float(in.cloth->thickness);
+    s->cloth_mat[3] = float(in.cloth->bending_stiffness);
+  }
+  uint32_t max_body = 0, free_count = 0;
+  s->pinned.resize(V);
+  for (uint32_t v = 0; v < V; ++v) {
+    max_body = std::max(max_body, in.vertex_info[4 * size_t(v) + 3]);
+    s->pinned[v] = in.vertex_info[4 * size_t(v) + 2] != 0;
+    free_count += s->pinned[v] ? 0u : 1u;
+  }
+  s->bodies = max_body + 1;
+  s->free_vertices = free_count;
+  s->color_count = color_count;
+  {  // Initial dynamic scene scale (jgs2-solver.ts:732-766).
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t dyn = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+      if (s->pinned[v]) continue;
+      ++dyn;
+      for (int a = 0; a < 3; ++a) {
+        double c = in.positions[4 * size_t(v) + a];
+        mn[a] = std::min(mn[a], c);
+        mx[a] = std::max(mx[a], c);
+      }
+    }
+    double diag = dyn ? std::hypot(std::hypot(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]) : 0.0;
+    s->scene_scale = float((dyn > 0 && std::isfinite(diag) && diag > 0) ? diag : 1e-12);
+  }
+  morton_orders(in, s->vnew, s->vold, s->tnew, s->told);
+  return JGS2_OK;
+}
+
+// The arena configuration that holds exactly one prepared scene.
+jgs2_batch_config exact_config(const jgs2_solver* s) {
+  jgs2_batch_config c{};
+  c.max_vertices = s->V;
+  c.max_tets = s->T;
+  c.max_triangles = s->NT;
+  c.max_hinges = s->NH;
+  c.max_candidates = s->NC;
+  c.max_bodies = s->bodies;
+  c.max_scenes = 1;
+  c.cubature_k = s->K;
+  c.corotated = s->corotated ? 1u : 0u;
+  return c;
+}
+
+// Facts of a prepared scene the batch must share.
+jgs2_status check_fits(const jgs2_batch* b, const jgs2_solver* s) {
+  if (s->corotated != b->corotated)
+    return fail(JGS2_ERROR_RANGE, "A batch holds either stable Neo-Hookean or co-rotated scenes, not both.");
+  if (s->K > b->cfg.cubature_k)
+    return fail(JGS2_ERROR_RANGE, "The scene has %u cubature records per vertex; the batch holds at most %u.", s->K,
+                b->cfg.cubature_k);
+  if (s->has_cloth && b->cloth_scenes > 0) {
+    const float* m = s->cloth_mat;
+    const DeviceState& d = b->dev;
+    if (m[0] != d.cloth_lam || m[1] != d.cloth_mu || m[2] != d.cloth_thick || m[3] != d.cloth_bend)
+      return fail(JGS2_ERROR_RANGE, "Every resident cloth scene of a batch shares one cloth material.");
+  }
+  if (float(s->defaults.ipc_minimum_distance) != b->dev.ipc_dmin && s->NC > 0)
+    return fail(JGS2_ERROR_RANGE, "The scene's ipcMinimumDistance differs from the batch's.");
+  return JGS2_OK;
+}
+
+// Marks a scene's blocks and slot free on the device.
+jgs2_status clear_scene_tables(jgs2_batch* b, const jgs2_solver* s) {
+  const uint4 zero = make_uint4(0, 0, 0, 0);
+  HIP_CHECK(hipMemcpy(b->dev.scene_cnt + s->slot, &zero, sizeof(uint4), hipMemcpyHostToDevice));
+  std::vector<uint32_t> none(std::max({s->vblk.n, s->tblk.n, s->rblk.n, s->cblk.n, uint64_t(1)}), kNoScene);
+  HIP_CHECK(hipMemcpy(b->dev.vblock_scene + s->vblk.at, none.data(), 4 * s->vblk.n, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(b->dev.tblock_scene + s->tblk.at, none.data(), 4 * s->tblk.n, hipMemcpyHostToDevice));
+  if (s->rblk.n) HIP_CHECK(hipMemcpy(b->dev.rblock_scene + s->rblk.at, none.data(), 4 * s->rblk.n, hipMemcpyHostToDevice));
+  if (s->cblk.n) HIP_CHECK(hipMemcpy(b->dev.cblock_scene + s->cblk.at, none.data(), 4 * s->cblk.n, hipMemcpyHostToDevice));
+  if (s->body.n) {
+    std::vector<uint4> rng(s->body.n, make_uint4(0, 0, 0, kNoScene));
+    HIP_CHECK(hipMemcpy(b->dev.body_rng + s->body.at, rng.data(), rng.size() * sizeof(uint4), hipMemcpyHostToDevice));
+  }
+  return JGS2_OK;
+}
+
+void forget_scene(jgs2_batch* b, jgs2_solver* s) {
+  b->slots[s->slot] = nullptr;
+  if (s->has_cloth) b->cloth_scenes -= 1;
+  release_spans(b, s);
+  b->version++;
+  b->graph_valid = false;
+}
+
+// Places a prepared scene: spans for every arena range, packing, and the
+// GPU preflight of the uploaded f32 source (validateGpuInitialStableSource,
+// jgs2-solver.ts:820-893). Takes ownership of `s` (deleted on failure).
+// JGS2_ERROR_MEMORY means the arena has no room for the scene now.
+jgs2_status place_scene(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_input& in, jgs2_solver** out) {
+  jgs2_status st = check_fits(b, s);
+  if (st != JGS2_OK) {
+    delete s;
+    return st;
+  }
+  s->batch = b;
+  uint64_t slot = 0;
+  if (!b->slota.alloc(1, slot)) {
+    delete s;
+    return fail(JGS2_ERROR_MEMORY, "The batch holds its maximum of %u scenes.", b->cfg.max_scenes);
+  }
+  s->slot = uint32_t(slot);
+  bool ok = true;
+  auto take = [&](RangeAlloc& a, uint64_t n, Span& sp) {
+    if (!ok) return;
+    ok = a.alloc(n, sp.at);
+    sp.n = ok ? n : 0;
+  };
+  take(b->va, blocks_of(s->V), s->vblk);
+  take(b->ta, blocks_of(s->T), s->tblk);
+  take(b->ra, blocks_of(s->NT), s->rblk);
+  take(b->ca, blocks_of(s->NC), s->cblk);
+  take(b->adja, in.adjacency_length, s->adj);
+  take(b->hina, s->NH, s->hin);
+  take(b->tela, 3 * uint64_t(s->NT), s->tel);
+  take(b->hela, 4 * uint64_t(s->NH), s->hel);
+  take(b->cela, 4 * uint64_t(s->NC), s->cel);
+  take(b->bodya, s->bodies, s->body);
+  if (!ok) {
+    release_spans(b, s);
+    delete s;
+    return fail(JGS2_ERROR_MEMORY, "The batch arena has no room for this scene now.");
+  }
+  // Color-sorted vertex list of the scene in arena ids.
+  s->color_offsets.assign(s->color_count + 1, 0);
+  for (uint32_t v = 0; v < s->V; ++v) s->color_offsets[s->colors[v] + 1] += 1;
+  for (uint32_t c = 0; c < s->color_count; ++c) s->color_offsets[c + 1] += s->color_offsets[c];
+  s->color_order.assign(s->V, 0);
+  std::vector<uint32_t> dev_colors(size_t(s->vblk.n) * kSceneBlock, 0);
+  {
+    std::vector<uint32_t> cursor(s->color_offsets.begin(), s->color_offsets.end() - 1);
+    for (uint32_t nv = 0; nv < s->V; ++nv) {
+      const uint32_t c = s->colors[s->vold[nv]];
+      s->color_order[cursor[c]++] = uint32_t(s->vbase()) + nv;
+      dev_colors[nv] = c;
+    }
+  }
+  if (s->has_cloth && b->cloth_scenes == 0) {
+    b->dev.cloth_lam = s->cloth_mat[0];
+    b->dev.cloth_mu = s->cloth_mat[1];
+    b->dev.cloth_thick = s->cloth_mat[2];
+    b->dev.cloth_bend = s->cloth_mat[3];
+  }
+  st = pack_into_arena(b, s, in);
+  if (st == JGS2_OK && hipMemcpy(b->dev.colors + s->vbase(), dev_colors.data(), 4 * dev_colors.size(),
+                                 hipMemcpyHostToDevice) != hipSuccess)
+    st = fail(JGS2_ERROR_DEVICE, "Uploading schedule colors failed.");
+  b->slots[s->slot] = s;
+  if (s->has_cloth) b->cloth_scenes += 1;
+  b->version++;
+  b->graph_valid = false;
+  if (st == JGS2_OK && !b->corotated) {
+    PassParams p = make_params(b, b->defaults, false);
+    p.src = 0;
+    p.tgt = 0;
+    launch_reduce_candidate_scene(b->dev, p, s->slot, b->stream);
+    float control[4] = {0, 0, 0, 0};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(control, b->dev.control + 4 * size_t(s->slot), sizeof(control), hipMemcpyDeviceToHost,
+                         b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    const uint32_t required = 1u | 2u | 4u;
+    const float bits = control[3];
+    const bool feasible = std::isfinite(control[0]) && std::isfinite(control[1]) && std::isfinite(control[2]) &&
+                          control[0] > kDeterminantFloor && control[1] > kDeterminantFloor &&
+                          control[2] > kDeterminantFloor && bits == std::floor(bits) &&
+                          (uint32_t(bits) & required) == required;
+    if (e != hipSuccess)
+      st = fail(JGS2_ERROR_DEVICE, "The source-feasibility preflight failed: %s", hipGetErrorString(e));
+    else if (!feasible)
+      st = fail(JGS2_ERROR_INFEASIBLE,
+                "The uploaded stable Neo-Hookean source pose is not feasible under the production GPU f32 "
+                "determinant calculation; source J=%g, candidate J=%g, accepted J=%g.",
+                double(control[0]), double(control[1]), double(control[2]));
+  }
+  if (st != JGS2_OK) {
+    const std::string message = g_last_error;
+    (void)clear_scene_tables(b, s);
+    forget_scene(b, s);
+    delete s;
+    g_last_error = message;
+    return st;
+  }
+  *out = s;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status retire(jgs2_batch* b, jgs2_solver* s) {
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  TRYS(clear_scene_tables(b, s));
+  forget_scene(b, s);
+  s->destroyed = true;
+  return JGS2_OK;
+}
+
+jgs2_status batch_create(const jgs2_batch_config& cfg, const jgs2_settings& resolved, jgs2_batch** out) {
+  int device_count = 0;
+  if (hipGetDeviceCount(&device_count) != hipSuccess || device_count < 1)
+    return fail(JGS2_ERROR_DEVICE, "No HIP device is available; libjgs2hip has no CPU fallback.");
+  jgs2_batch* b = new (std::nothrow) jgs2_batch();
+  if (!b) return fail(JGS2_ERROR_MEMORY, "Out of host memory.");
+  b->cfg = cfg;
+  b->defaults = resolved;
+  b->corotated = cfg.corotated != 0;
+  jgs2_status st = JGS2_OK;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    st = fail(JGS2_ERROR_DEVICE, "hipStreamCreateWithFlags failed.");
+  if (st == JGS2_OK) st = batch_allocate(b);
+  // Contact parameters are fixed when the arena is created
+  // (jgs2-solver.ts:1096-1105).
+  b->dev.ipc_dmin = float(resolved.ipc_minimum_distance);
+  b->dev.ipc_mu = float(resolved.ipc_friction_coefficient);
+  b->dev.ipc_eps = float(resolved.ipc_friction_velocity_epsilon);
+  b->dev.ipc_safety = float(resolved.ipc_step_safety);
+  if (st != JGS2_OK) {
+    free_device(b->dev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return st;
+  }
+  *out = b;
+  return JGS2_OK;
+}
+
+void batch_destroy(jgs2_batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto*& s : b->slots)
+    if (s) {
+      s->destroyed = true;
+      delete s;
+      s = nullptr;
+    }
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  free_device(b->dev);
+  for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+  if (b->frame_begin) (void)hipEventDestroy(b->frame_begin);
+  if (b->frame_end) (void)hipEventDestroy(b->frame_end);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// ---------------------------------------------------------------- per-scene access
+jgs2_status check_usable(jgs2_solver* s) {
+  if (!s) return fail(JGS2_ERROR_TYPE, "NULL solver.");
+  if (s->destroyed) return fail(JGS2_ERROR_STATE, "JGS2GpuSolver has been destroyed.");
+  return JGS2_OK;
+}
+
+jgs2_status check_owner(jgs2_solver* s) {
+  TRYS(check_usable(s));
+  if (!s->owns_batch) return fail(JGS2_ERROR_STATE, "A batch-resident scene is stepped through jgs2_batch_step_frames.");
+  return JGS2_OK;
+}
+
 jgs2_status upload_objectives(jgs2_solver* s, size_t first_vertex, size_t vertex_count) {
-  // The host mirror is in caller vertex ids; device records are in device order.
-  std::vector<float> staged(8 * vertex_count);
+  // The host mirror is in caller vertex ids; device records are in arena order.
+  jgs2_batch* b = s->batch;
+  float4* base = b->dev.objectives + 2 * s->vbase();
   if (vertex_count == s->V) {
+    std::vector<float> staged(8 * vertex_count);
     for (uint32_t nv = 0; nv < s->V; ++nv)
       std::memcpy(&staged[8 * size_t(nv)], &s->objectives[8 * size_t(s->vold[nv])], 32);
-    HIP_CHECK(hipMemcpyAsync(s->dev.objectives, staged.data(), 32 * size_t(s->V), hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(base, staged.data(), 32 * size_t(s->V), hipMemcpyHostToDevice, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
   } else {
     for (size_t i = 0; i < vertex_count; ++i) {
       const size_t ov = first_vertex + i;
-      HIP_CHECK(hipMemcpyAsync(s->dev.objectives + 2 * size_t(s->vnew[ov]), &s->objectives[8 * ov], 32,
-                               hipMemcpyHostToDevice, s->stream));
+      HIP_CHECK(hipMemcpyAsync(base + 2 * size_t(s->vnew[ov]), &s->objectives[8 * ov], 32, hipMemcpyHostToDevice,
+                               b->stream));
     }
+    HIP_CHECK(hipStreamSynchronize(b->stream));
   }
-  HIP_CHECK(hipStreamSynchronize(s->stream));
   recount_objectives(s);
+  // The scene's bodies carry its objective flags for the body correction.
+  std::vector<uint4> rng(s->body.n, make_uint4(uint32_t(s->vbase()), uint32_t(s->vbase() + s->V), s->objective_flags,
+                                               s->slot));
+  HIP_CHECK(hipMemcpy(b->dev.body_rng + s->body.at, rng.data(), rng.size() * sizeof(uint4), hipMemcpyHostToDevice));
+  b->graph_valid = false;
   return JGS2_OK;
 }
 
@@ -786,20 +1647,21 @@ jgs2_status objective_supported(jgs2_solver* s) {
   return JGS2_OK;
 }
 
-// Copies a device array to the caller. Per-vertex arrays (`stride` floats
-// per vertex) are returned in caller vertex ids.
-jgs2_status read_buffer(jgs2_solver* s, const void* src, float* dst, size_t count, size_t expected, const char* what,
+// Copies a per-vertex arena array (`stride` floats per vertex) of one scene
+// to the caller in caller vertex ids; `stride` 0 copies `expected` floats
+// verbatim from `src`.
+jgs2_status read_buffer(jgs2_solver* s, const float* src, float* dst, size_t count, size_t expected, const char* what,
                         size_t stride = 0) {
   jgs2_status st = check_usable(s);
   if (st != JGS2_OK) return st;
   if (!dst) return fail(JGS2_ERROR_TYPE, "NULL destination for %s.", what);
   if (count != expected) return fail(JGS2_ERROR_RANGE, "%s needs exactly %zu floats; got %zu.", what, expected, count);
-  HIP_CHECK(hipStreamSynchronize(s->stream));
+  HIP_CHECK(hipStreamSynchronize(s->batch->stream));
   if (stride == 0) {
     HIP_CHECK(hipMemcpy(dst, src, expected * sizeof(float), hipMemcpyDeviceToHost));
   } else {
     std::vector<float> staged(expected);
-    HIP_CHECK(hipMemcpy(staged.data(), src, expected * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(staged.data(), src + stride * s->vbase(), expected * sizeof(float), hipMemcpyDeviceToHost));
     for (uint32_t ov = 0; ov < s->V; ++ov)
       std::memcpy(dst + stride * ov, &staged[stride * size_t(s->vnew[ov])], stride * sizeof(float));
   }
@@ -861,482 +1723,82 @@ jgs2_status jgs2_create(const jgs2_scene_input* input, const jgs2_settings* over
                         jgs2_solver** out) {
   if (!input || !out) return fail(JGS2_ERROR_TYPE, "jgs2_create requires an input and an output handle.");
   *out = nullptr;
-  const jgs2_scene_input& in = *input;
-  jgs2_status st = validate_input(in);
-  if (st != JGS2_OK) return st;
   jgs2_settings defaults;
   jgs2_default_settings(&defaults);
   jgs2_settings resolved;
-  st = merge_settings(defaults, overrides, mask, resolved);
-  if (st != JGS2_OK) return st;
-
-  const uint32_t V = in.vertex_count, T = in.tet_count, K = in.cubature_k;
-  const bool corotated = in.tet_meta[3] == JGS2_MATERIAL_COROTATED_LINEAR;
-  const bool globalization = !corotated;
-  const uint32_t NT = in.cloth ? in.cloth->triangle_count : 0u, NH = in.cloth ? in.cloth->hinge_count : 0u;
-  if (in.cloth) {
-    st = validate_cloth(*in.cloth, V);
-    if (st != JGS2_OK) return st;
-    if (NT > 0 && !globalization)  // jgs2-solver.ts:998-1001
-      return fail(JGS2_ERROR_RANGE, "Triangle cloth requires the stable Neo-Hookean production path.");
-  }
-  const uint32_t NC = in.contact ? in.contact->vertex_triangle_count + in.contact->edge_edge_count : 0u;
-  if (in.contact) {
-    st = validate_contact(*in.contact, V);
-    if (st != JGS2_OK) return st;
-  }
-  if (NC > 0) {  // initially feasible contact distances (jgs2-solver.ts:1041-1066)
-    const double minimum = min_static_contact_distance(in.positions, *in.contact);
-    const double margin = 8.0 * 0x1p-24 * std::max({1.0, resolved.ipc_activation_distance, std::fabs(minimum)});
-    const double required = resolved.ipc_minimum_distance + margin;
-    if (!std::isfinite(minimum) || !(minimum > required))
-      return fail(JGS2_ERROR_RANGE,
-                  "IPC candidates require an initially feasible source pose with distance greater than %.17g "
-                  "(minimum plus an f32 safety margin); got %.17g.",
-                  required, minimum);
-  }
-  std::vector<uint32_t> colors;
-  uint32_t color_count = 0;
-  st = color_vertices(in, colors, color_count);
-  if (st != JGS2_OK) return st;
-  if (color_count > kMaxColor + 1)
-    return fail(JGS2_ERROR_RANGE,
-                "Colored Gauss-Seidel supports at most %u colors; the small-scene conflict graph needs %u.",
-                kMaxColor + 1, color_count);
-  std::vector<float> objectives(8 * size_t(V), 0.0f);
-  for (uint32_t v = 0; v < V; ++v) {  // packJGS2VertexObjectives (layout.ts:680-716)
-    float* o = &objectives[8 * size_t(v)];
-    if (in.external_forces) {
-      o[0] = in.external_forces[3 * size_t(v)];
-      o[1] = in.external_forces[3 * size_t(v) + 1];
-      o[2] = in.external_forces[3 * size_t(v) + 2];
-    }
-    o[4] = in.target_positions ? in.target_positions[3 * size_t(v)] : in.positions[4 * size_t(v)];
-    o[5] = in.target_positions ? in.target_positions[3 * size_t(v) + 1] : in.positions[4 * size_t(v) + 1];
-    o[6] = in.target_positions ? in.target_positions[3 * size_t(v) + 2] : in.positions[4 * size_t(v) + 2];
-    o[7] = in.target_stiffnesses ? in.target_stiffnesses[v] : 0.0f;
-  }
-  bool any_objective = false;
-  for (uint32_t v = 0; v < V && !any_objective; ++v) {
-    const float* o = &objectives[8 * size_t(v)];
-    any_objective = o[0] != 0 || o[1] != 0 || o[2] != 0 || o[7] > 0;
-  }
-  if (!globalization && any_objective)
-    return fail(JGS2_ERROR_RANGE,
-                "Active external forces and quadratic targets require the stable Neo-Hookean production path.");
-  if (globalization) {
-    const double minimum = min_input_determinant(in);
-    if (!std::isfinite(minimum) || !(minimum > kDeterminantFloor))
-      return fail(JGS2_ERROR_RANGE,
-                  "A stable Neo-Hookean production solve requires an initially feasible pose with minimum J greater "
-                  "than 0.0001; got %.17g.",
-                  minimum);
-  }
-  int device_count = 0;
-  if (hipGetDeviceCount(&device_count) != hipSuccess || device_count < 1)
-    return fail(JGS2_ERROR_DEVICE, "No HIP device is available; libjgs2hip has no CPU fallback.");
-
+  TRYS(merge_settings(defaults, overrides, mask, resolved));
   jgs2_solver* s = new (std::nothrow) jgs2_solver();
   if (!s) return fail(JGS2_ERROR_MEMORY, "Out of host memory.");
-  s->defaults = resolved;
-  s->preprocessing_timestep = resolved.timestep;
-  s->globalization = globalization;
-  s->corotated = corotated;
-  s->V = V;
-  s->T = T;
-  s->K = K;
-  uint32_t max_body = 0, free_count = 0;
-  s->pinned.resize(V);
-  for (uint32_t v = 0; v < V; ++v) {
-    max_body = std::max(max_body, in.vertex_info[4 * size_t(v) + 3]);
-    s->pinned[v] = in.vertex_info[4 * size_t(v) + 2] != 0;
-    free_count += s->pinned[v] ? 0u : 1u;
-  }
-  s->bodies = max_body + 1;
-  s->free_vertices = free_count;
-  s->colors = colors;
-  s->color_count = color_count;
-  s->objectives = objectives;
-  recount_objectives(s);
-  // Initial dynamic scene scale (jgs2-solver.ts:732-766).
-  {
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    uint32_t dyn = 0;
-    for (uint32_t v = 0; v < V; ++v) {
-      if (s->pinned[v]) continue;
-      ++dyn;
-      for (int a = 0; a < 3; ++a) {
-        double c = in.positions[4 * size_t(v) + a];
-        mn[a] = std::min(mn[a], c);
-        mx[a] = std::max(mx[a], c);
-      }
-    }
-    double diag = dyn ? std::hypot(std::hypot(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]) : 0.0;
-    s->scene_scale = float((dyn > 0 && std::isfinite(diag) && diag > 0) ? diag : 1e-12);
-  }
-  // Space-filling device order of vertices and tetrahedra.
-  morton_orders(in, s->vnew, s->vold, s->tnew, s->told);
-  // Color-sorted vertex order (vertex-coloring.ts:92-107), in device ids.
-  s->color_offsets.assign(color_count + 1, 0);
-  for (uint32_t v = 0; v < V; ++v) s->color_offsets[colors[v] + 1] += 1;
-  for (uint32_t c = 0; c < color_count; ++c) s->color_offsets[c + 1] += s->color_offsets[c];
-  std::vector<uint32_t> order(V);
-  std::vector<uint32_t> dev_colors(V);
-  {
-    std::vector<uint32_t> cursor(s->color_offsets.begin(), s->color_offsets.end() - 1);
-    for (uint32_t nv = 0; nv < V; ++nv) {
-      const uint32_t c = colors[s->vold[nv]];
-      order[cursor[c]++] = nv;
-      dev_colors[nv] = c;
-    }
-  }
-
-  auto cleanup = [&](jgs2_status code) {
-    free_device(s->dev);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+  jgs2_status st = prepare_scene(*input, resolved, s);
+  if (st != JGS2_OK) {
     delete s;
-    return code;
-  };
-#define TRY(expr)                       \
-  do {                                  \
-    jgs2_status t_ = (expr);            \
-    if (t_ != JGS2_OK) return cleanup(t_); \
-  } while (0)
-#define TRY_HIP(expr)                                                                                 \
-  do {                                                                                                \
-    hipError_t e_ = (expr);                                                                           \
-    if (e_ != hipSuccess) return cleanup(fail(JGS2_ERROR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_))); \
-  } while (0)
-
-  TRY_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  DeviceState& d = s->dev;
-  d.V = V;
-  d.T = T;
-  d.K = K;
-  d.bodies = s->bodies;
-  // Static records in device order.
-  const std::vector<uint32_t>& vnew = s->vnew;
-  const std::vector<uint32_t>& vold = s->vold;
-  const std::vector<uint32_t>& tnew = s->tnew;
-  const std::vector<uint32_t>& told = s->told;
-  std::vector<uint32_t> adjacency(in.adjacency_length);
-  std::vector<float4> vtx_rest(V);
-  std::vector<uint4> vtx_info(V);
-  {
-    uint32_t cursor = 0;
-    for (uint32_t nv = 0; nv < V; ++nv) {
-      const uint32_t ov = vold[nv];
-      const float* r = &in.vertex_rest[4 * size_t(ov)];
-      const uint32_t* i = &in.vertex_info[4 * size_t(ov)];
-      vtx_rest[nv] = make_float4(r[0], r[1], r[2], r[3]);
-      vtx_info[nv] = make_uint4(cursor, i[1], i[2], i[3]);
-      // Row order stays ascending in caller tet ids (the reference's
-      // summation order); entries are device tet ids.
-      for (uint32_t k = 0; k < i[1]; ++k) adjacency[cursor + k] = tnew[in.adjacency[i[0] + k]];
-      cursor += i[1];
-    }
+    return st;
   }
-  std::vector<uint4> tet_idx(T);
-  std::vector<float4> tet_dm(3 * size_t(T));
-  std::vector<float> tet_k;
-  if (corotated) tet_k.resize(144 * size_t(T));
-  for (uint32_t nt = 0; nt < T; ++nt) {
-    const uint32_t ot = told[nt];
-    const uint32_t* i = &in.tet_indices[4 * size_t(ot)];
-    const float* m = &in.tet_inverse_dm[12 * size_t(ot)];
-    const float* a = &in.tet_meta[4 * size_t(ot)];
-    tet_idx[nt] = make_uint4(vnew[i[0]], vnew[i[1]], vnew[i[2]], vnew[i[3]]);
-    tet_dm[3 * size_t(nt)] = make_float4(m[0], m[1], m[2], a[0]);
-    tet_dm[3 * size_t(nt) + 1] = make_float4(m[4], m[5], m[6], a[1]);
-    tet_dm[3 * size_t(nt) + 2] = make_float4(m[8], m[9], m[10], a[2]);
-    if (corotated) std::memcpy(&tet_k[144 * size_t(nt)], &in.tet_rest_stiffness[144 * size_t(ot)], 144 * sizeof(float));
+  jgs2_batch* b = nullptr;
+  st = batch_create(exact_config(s), resolved, &b);
+  if (st != JGS2_OK) {
+    delete s;
+    return st;
   }
-  const size_t R = size_t(V) * K;
-  std::vector<uint4> cub(10 * std::max<size_t>(R, 1));
-  for (uint32_t nv = 0; nv < V; ++nv) {
-    for (uint32_t k = 0; k < K; ++k) {
-      const size_t ro = size_t(vold[nv]) * K + k, rn = size_t(nv) * K + k;
-      uint32_t wbits;
-      std::memcpy(&wbits, &in.cubature_weights[ro], 4);
-      const uint32_t t = in.cubature_tet_ids[ro];
-      cub[10 * rn] = make_uint4(t < T ? tnew[t] : 0xffffffffu, wbits, 0, 0);
-      std::memcpy(&cub[10 * rn + 1], &in.cubature_basis[36 * ro], 36 * sizeof(float));
-    }
+  st = place_scene(b, s, *input, out);
+  if (st != JGS2_OK) {
+    const std::string message = g_last_error;
+    batch_destroy(b);
+    g_last_error = message;
+    return st;
   }
-  std::vector<float4> dev_objectives(2 * size_t(V));
-  for (uint32_t nv = 0; nv < V; ++nv) {
-    const float* o = &s->objectives[8 * size_t(vold[nv])];
-    dev_objectives[2 * size_t(nv)] = make_float4(o[0], o[1], o[2], o[3]);
-    dev_objectives[2 * size_t(nv) + 1] = make_float4(o[4], o[5], o[6], o[7]);
-  }
-  std::vector<float4> positions(V), velocities(V, make_float4(0, 0, 0, 0)), rot_v(3 * size_t(V)),
-      rot_t(3 * size_t(T));
-  for (uint32_t nv = 0; nv < V; ++nv) {
-    const uint32_t ov = vold[nv];
-    const float* x = &in.positions[4 * size_t(ov)];
-    positions[nv] = make_float4(x[0], x[1], x[2], 1.0f);
-    if (in.velocities)
-      velocities[nv] = make_float4(in.velocities[4 * size_t(ov)], in.velocities[4 * size_t(ov) + 1],
-                                   in.velocities[4 * size_t(ov) + 2], 0.0f);
-    rot_v[3 * size_t(nv)] = make_float4(1, 0, 0, 0);
-    rot_v[3 * size_t(nv) + 1] = make_float4(0, 1, 0, 0);
-    rot_v[3 * size_t(nv) + 2] = make_float4(0, 0, 1, 0);
-  }
-  for (uint32_t t = 0; t < T; ++t) {
-    rot_t[3 * size_t(t)] = make_float4(1, 0, 0, 0);
-    rot_t[3 * size_t(t) + 1] = make_float4(0, 1, 0, 0);
-    rot_t[3 * size_t(t) + 2] = make_float4(0, 0, 1, 0);
-  }
-  // Cloth records in device vertex ids; elements keep caller order.
-  std::vector<uint4> tri_ix(NT), hinge_ix(NH);
-  std::vector<float4> tri_inv(NT), hinge_rest(NH);
-  std::vector<float> tri_area(NT);
-  std::vector<uint32_t> tri_rows, tri_elems, hinge_rows, hinge_elems;
-  if (in.cloth) {
-    const jgs2_cloth_input& c = *in.cloth;
-    for (uint32_t t = 0; t < NT; ++t) {
-      const uint32_t* ix = &c.triangle_indices[3 * size_t(t)];
-      const uint32_t owner = std::min(ix[0], std::min(ix[1], ix[2]));  // S:1838-1840
-      tri_ix[t] = make_uint4(vnew[ix[0]], vnew[ix[1]], vnew[ix[2]], vnew[owner]);
-      const float* b = &c.triangle_inverse_rest_bases[4 * size_t(t)];
-      tri_inv[t] = make_float4(b[0], b[1], b[2], b[3]);
-      tri_area[t] = c.triangle_rest_areas[t];
-    }
-    for (uint32_t h = 0; h < NH; ++h) {
-      const uint32_t* ix = &c.hinge_indices[4 * size_t(h)];
-      const uint32_t owner = std::min(std::min(ix[0], ix[1]), std::min(ix[2], ix[3]));  // S:1842-1844
-      hinge_ix[h] = make_uint4(vnew[ix[0]], vnew[ix[1]], vnew[ix[2]], vnew[ix[3]]);
-      uint32_t ob = vnew[owner];
-      float of;
-      std::memcpy(&of, &ob, 4);
-      hinge_rest[h] = make_float4(c.hinge_rest_angles[h], c.hinge_rest_edge_lengths[h], of, 0.0f);
-    }
-    if (jgs2_status e = incident_csr(c.triangle_indices, 3, NT, vold, tri_rows, tri_elems)) return cleanup(e);
-    if (jgs2_status e = incident_csr(c.hinge_indices, 4, NH, vold, hinge_rows, hinge_elems)) return cleanup(e);
-    // packJGS2GpuClothArena material words (cloth-layout.ts:335-343): f64
-    // expressions rounded to f32.
-    const double E = c.young_modulus, nu = c.poisson_ratio;
-    d.cloth_lam = float(E * nu / (1 - nu * nu));
-    d.cloth_mu = float(E / (2 * (1 + nu)));
-    d.cloth_thick = float(c.thickness);
-    d.cloth_bend = float(c.bending_stiffness);
-    d.NT = NT;
-    d.NH = NH;
-  }
-  // IPC candidate records in device vertex ids; the contact parameters are
-  // fixed at create (jgs2-solver.ts:1096-1105).
-  std::vector<uint4> cand(NC);
-  std::vector<uint32_t> cand_rows, cand_ids;
-  if (NC > 0) {
-    for (uint32_t c = 0; c < NC; ++c) {
-      const uint32_t* ix = &in.contact->packed_indices[4 * size_t(c)];
-      cand[c] = make_uint4(vnew[ix[0]], vnew[ix[1]], vnew[ix[2]], vnew[ix[3]]);
-    }
-    if (jgs2_status e = incident_csr(in.contact->packed_indices, 4, NC, vold, cand_rows, cand_ids)) return cleanup(e);
-    d.NC = NC;
-    d.NVT = in.contact->vertex_triangle_count;
-  }
-  d.ipc_dmin = float(resolved.ipc_minimum_distance);
-  d.ipc_mu = float(resolved.ipc_friction_coefficient);
-  d.ipc_eps = float(resolved.ipc_friction_velocity_epsilon);
-  d.ipc_safety = float(resolved.ipc_step_safety);
-  TRY(dalloc(&d.vtx_rest, V));
-  TRY(dalloc(&d.vtx_info, V));
-  TRY(dalloc(&d.tet_idx, T));
-  TRY(dalloc(&d.tet_dm, 3 * size_t(T)));
-  if (corotated) TRY(dalloc(&d.tet_k, 144 * size_t(T)));
-  TRY(dalloc(&d.adjacency, std::max<uint64_t>(in.adjacency_length, 1)));
-  TRY(dalloc(&d.cub, cub.size()));
-  TRY(dalloc(&d.objectives, 2 * size_t(V)));
-  TRY(dalloc(&d.color_order, V));
-  TRY(dalloc(&d.colors, V));
-  TRY(dalloc(&d.pos[0], V));
-  TRY(dalloc(&d.pos[1], V));
-  TRY(dalloc(&d.predicted, V));
-  TRY(dalloc(&d.velocity, V));
-  TRY(dalloc(&d.old, V));
-  TRY(dalloc(&d.vrot, 3 * size_t(V)));
-  TRY(dalloc(&d.vrot_next, 3 * size_t(V)));
-  TRY(dalloc(&d.trot, 3 * size_t(T)));
-  TRY(dalloc(&d.body_corr, s->bodies));
-  TRY(dalloc(&d.final_update, V));
-  TRY(dalloc(&d.local_diag, 4 * size_t(V)));
-  TRY(dalloc(&d.tet_glob, 2 * (size_t(T) + NT)));
-  if (in.cloth) {
-    TRY(dalloc(&d.tri_ix, NT));
-    TRY(dalloc(&d.tri_inv, NT));
-    TRY(dalloc(&d.tri_area, NT));
-    TRY(dalloc(&d.hinge_ix, NH));
-    TRY(dalloc(&d.hinge_rest, NH));
-    TRY(dalloc(&d.tri_rows, size_t(V) + 1));
-    TRY(dalloc(&d.tri_elems, tri_elems.size()));
-    TRY(dalloc(&d.hinge_rows, size_t(V) + 1));
-    TRY(dalloc(&d.hinge_elems, hinge_elems.size()));
-  }
-  TRY(dalloc(&d.vertex_energy, V));
-  TRY(dalloc(&d.conv, 5 * size_t(V)));
-  TRY(dalloc(&d.control, 4));
-  TRY(dalloc(&d.history, size_t(kHistoryCapacity) * 8));
-  if (NC > 0) {
-    TRY(dalloc(&d.cand, NC));
-    for (float4** a : {&d.lag_n, &d.lag_w, &d.cscratch, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRY(dalloc(a, NC));
-    TRY(dalloc(&d.contact_ctl, 1));
-    TRY(dalloc(&d.cand_rows, size_t(V) + 1));
-    TRY(dalloc(&d.cand_ids, 4 * size_t(NC)));
-    TRY(dalloc(&d.active_count, V));
-    TRY(dalloc(&d.active_ids, 4 * size_t(NC)));
-  }
-  d.partial_capacity = partial_blocks_needed(V, T + NT, NC);
-  TRY(dalloc(reinterpret_cast<char**>(&d.partials), partial_bytes(d.partial_capacity)));
-  hipStream_t q = s->stream;
-  TRY_HIP(hipMemcpyAsync(d.vtx_rest, vtx_rest.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.vtx_info, vtx_info.data(), V * sizeof(uint4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.tet_idx, tet_idx.data(), T * sizeof(uint4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.tet_dm, tet_dm.data(), tet_dm.size() * sizeof(float4), hipMemcpyHostToDevice, q));
-  if (corotated)
-    TRY_HIP(hipMemcpyAsync(d.tet_k, tet_k.data(), 144 * size_t(T) * sizeof(float), hipMemcpyHostToDevice, q));
-  if (in.adjacency_length)
-    TRY_HIP(hipMemcpyAsync(d.adjacency, adjacency.data(), in.adjacency_length * 4, hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.cub, cub.data(), cub.size() * sizeof(uint4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.objectives, dev_objectives.data(), 32 * size_t(V), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.color_order, order.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.colors, dev_colors.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
-  if (in.cloth) {
-    auto up = [&](void* dst, const void* src, size_t bytes) {
-      return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, q) : hipSuccess;
-    };
-    TRY_HIP(up(d.tri_ix, tri_ix.data(), NT * sizeof(uint4)));
-    TRY_HIP(up(d.tri_inv, tri_inv.data(), NT * sizeof(float4)));
-    TRY_HIP(up(d.tri_area, tri_area.data(), NT * sizeof(float)));
-    TRY_HIP(up(d.hinge_ix, hinge_ix.data(), NH * sizeof(uint4)));
-    TRY_HIP(up(d.hinge_rest, hinge_rest.data(), NH * sizeof(float4)));
-    TRY_HIP(up(d.tri_rows, tri_rows.data(), tri_rows.size() * 4));
-    TRY_HIP(up(d.tri_elems, tri_elems.data(), tri_elems.size() * 4));
-    TRY_HIP(up(d.hinge_rows, hinge_rows.data(), hinge_rows.size() * 4));
-    TRY_HIP(up(d.hinge_elems, hinge_elems.data(), hinge_elems.size() * 4));
-  }
-  if (NC > 0) {  // packIpcContactBuffer initial records (ipc-contact-layout.ts:302-341)
-    std::vector<float4> zero4(NC, make_float4(0, 0, 0, 0)), scratch(NC, make_float4(1, 0, 0, 0)),
-        none(NC, make_float4(0, 0, 0, -1));
-    const float4 ctl = make_float4(0, 0, 0, 1);
-    std::vector<uint32_t> zero_counts(V, 0);
-    TRY_HIP(hipMemcpyAsync(d.cand, cand.data(), NC * sizeof(uint4), hipMemcpyHostToDevice, q));
-    for (float4* a : {d.lag_n, d.lag_w, d.src_w, d.tgt_w})
-      TRY_HIP(hipMemcpyAsync(a, zero4.data(), NC * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.cscratch, scratch.data(), NC * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.src_c, none.data(), NC * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.tgt_c, none.data(), NC * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.contact_ctl, &ctl, sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.cand_rows, cand_rows.data(), cand_rows.size() * 4, hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.cand_ids, cand_ids.data(), cand_ids.size() * 4, hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.active_count, zero_counts.data(), 4 * size_t(V), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.active_ids, cand_ids.data(), cand_ids.size() * 4, hipMemcpyHostToDevice, q));
-    TRY_HIP(hipStreamSynchronize(q));
-  }
-  for (float4* region : {d.pos[0], d.pos[1], d.predicted, d.old})
-    TRY_HIP(hipMemcpyAsync(region, positions.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.velocity, velocities.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.vrot, rot_v.data(), rot_v.size() * sizeof(float4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.vrot_next, rot_v.data(), rot_v.size() * sizeof(float4), hipMemcpyHostToDevice, q));
-  TRY_HIP(hipMemcpyAsync(d.trot, rot_t.data(), rot_t.size() * sizeof(float4), hipMemcpyHostToDevice, q));
-  {
-    std::vector<float4> zeros(std::max<size_t>({size_t(5) * V, size_t(4) * V, 2 * (size_t(T) + NT), size_t(kHistoryCapacity) * 8,
-                                                size_t(s->bodies)}),
-                              make_float4(0, 0, 0, 0));
-    TRY_HIP(hipMemcpyAsync(d.body_corr, zeros.data(), s->bodies * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.final_update, zeros.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.local_diag, zeros.data(), 4 * size_t(V) * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.tet_glob, zeros.data(), 2 * (size_t(T) + NT) * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.vertex_energy, zeros.data(), V * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.conv, zeros.data(), 5 * size_t(V) * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.control, zeros.data(), 4 * sizeof(float4), hipMemcpyHostToDevice, q));
-    TRY_HIP(hipMemcpyAsync(d.history, zeros.data(), size_t(kHistoryCapacity) * 8 * sizeof(float4),
-                           hipMemcpyHostToDevice, q));
-    TRY_HIP(hipStreamSynchronize(q));
-  }
-  // GPU preflight of the uploaded f32 source (validateGpuInitialStableSource,
-  // jgs2-solver.ts:820-893).
-  if (globalization) {
-    PassParams p = make_params(s, resolved, false);
-    p.src = 0;
-    p.tgt = 0;
-    launch_reduce_candidate(d, p, q);
-    TRY_HIP(hipGetLastError());
-    float control[4];
-    TRY_HIP(hipMemcpyAsync(control, d.control, sizeof(control), hipMemcpyDeviceToHost, q));
-    TRY_HIP(hipStreamSynchronize(q));
-    const uint32_t required = 1u | 2u | 4u;
-    const float bits = control[3];
-    const bool ok = std::isfinite(control[0]) && std::isfinite(control[1]) && std::isfinite(control[2]) &&
-                    control[0] > kDeterminantFloor && control[1] > kDeterminantFloor && control[2] > kDeterminantFloor &&
-                    bits == std::floor(bits) && (uint32_t(bits) & required) == required;
-    if (!ok)
-      return cleanup(fail(JGS2_ERROR_INFEASIBLE,
-                          "The uploaded stable Neo-Hookean source pose is not feasible under the production GPU f32 "
-                          "determinant calculation; source J=%g, candidate J=%g, accepted J=%g.",
-                          double(control[0]), double(control[1]), double(control[2])));
-  }
-#undef TRY
-#undef TRY_HIP
-  *out = s;
-  clear_error();
+  (*out)->owns_batch = true;
   return JGS2_OK;
 }
 
 jgs2_status jgs2_destroy(jgs2_solver* s) {
   if (!s) return JGS2_OK;
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
-  free_device(s->dev);
-  for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
-  if (s->frame_begin) (void)hipEventDestroy(s->frame_begin);
-  if (s->frame_end) (void)hipEventDestroy(s->frame_end);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
-  s->destroyed = true;
-  delete s;
+  if (s->destroyed) return fail(JGS2_ERROR_STATE, "JGS2GpuSolver has been destroyed.");
+  if (s->owns_batch) {
+    batch_destroy(s->batch);  // deletes s
+  } else {
+    TRYS(retire(s->batch, s));
+    delete s;
+  }
   clear_error();
   return JGS2_OK;
 }
 
 jgs2_status jgs2_step(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask) {
-  jgs2_status st = check_usable(s);
-  if (st != JGS2_OK) return st;
+  TRYS(check_owner(s));
   jgs2_settings r;
-  if ((st = resolve_step(s, ov, mask, "step", r)) != JGS2_OK) return st;
-  return submit(s, r, normalize_odd(r.iterations), 1, true);
+  TRYS(resolve_batch_step(s->batch, ov, mask, "step", r));
+  return submit(s->batch, r, normalize_odd(r.iterations), 1, true);
 }
 
 jgs2_status jgs2_step_frames(jgs2_solver* s, uint32_t frames, const jgs2_settings* ov, uint64_t mask) {
-  jgs2_status st = check_usable(s);
-  if (st != JGS2_OK) return st;
-  if ((st = validate_frames(frames)) != JGS2_OK) return st;
+  TRYS(check_owner(s));
+  TRYS(validate_frames(frames));
   jgs2_settings r;
-  if ((st = resolve_step(s, ov, mask, "stepFrames", r)) != JGS2_OK) return st;
-  return submit(s, r, normalize_odd(r.iterations), frames, true);
+  TRYS(resolve_batch_step(s->batch, ov, mask, "stepFrames", r));
+  return submit(s->batch, r, normalize_odd(r.iterations), frames, true);
 }
 
 jgs2_status jgs2_step_exact_iterations(jgs2_solver* s, uint32_t iterations, const jgs2_settings* ov, uint64_t mask) {
-  jgs2_status st = check_usable(s);
-  if (st != JGS2_OK) return st;
+  TRYS(check_owner(s));
   if (iterations < 1) return fail(JGS2_ERROR_RANGE, "iterations must be a positive safe integer; got %u.", iterations);
   jgs2_settings r;
-  if ((st = resolve_step(s, ov, mask, "stepExactIterations", r)) != JGS2_OK) return st;
-  return submit(s, r, iterations, 1, false);
+  TRYS(resolve_batch_step(s->batch, ov, mask, "stepExactIterations", r));
+  return submit(s->batch, r, iterations, 1, false);
 }
 
 jgs2_status jgs2_step_frames_exact_iterations(jgs2_solver* s, uint32_t frames, uint32_t iterations,
                                               const jgs2_settings* ov, uint64_t mask) {
-  jgs2_status st = check_usable(s);
-  if (st != JGS2_OK) return st;
-  if ((st = validate_frames(frames)) != JGS2_OK) return st;
+  TRYS(check_owner(s));
+  TRYS(validate_frames(frames));
   if (iterations < 1) return fail(JGS2_ERROR_RANGE, "iterations must be a positive safe integer; got %u.", iterations);
   jgs2_settings r;
-  if ((st = resolve_step(s, ov, mask, "stepFramesExactIterations", r)) != JGS2_OK) return st;
-  return submit(s, r, iterations, frames, false);
+  TRYS(resolve_batch_step(s->batch, ov, mask, "stepFramesExactIterations", r));
+  return submit(s->batch, r, iterations, frames, false);
 }
 
 jgs2_status jgs2_set_external_force(jgs2_solver* s, uint32_t vertex, const float force[3]) {
-  jgs2_status st = objective_supported(s);
-  if (st != JGS2_OK) return st;
+  TRYS(objective_supported(s));
   if (vertex >= s->V) return fail(JGS2_ERROR_RANGE, "Objective vertex %u is outside vertexCount %u.", vertex, s->V);
   if (!force) return fail(JGS2_ERROR_TYPE, "NULL force.");
   for (int c = 0; c < 3; ++c)
@@ -1353,8 +1815,7 @@ jgs2_status jgs2_set_external_force(jgs2_solver* s, uint32_t vertex, const float
 }
 
 jgs2_status jgs2_replace_external_forces(jgs2_solver* s, const float* forces, size_t count) {
-  jgs2_status st = objective_supported(s);
-  if (st != JGS2_OK) return st;
+  TRYS(objective_supported(s));
   if (!forces) return fail(JGS2_ERROR_TYPE, "NULL forces.");
   if (count != 3 * size_t(s->V))
     return fail(JGS2_ERROR_RANGE, "External forces must contain %zu values; got %zu.", 3 * size_t(s->V), count);
@@ -1377,15 +1838,13 @@ jgs2_status jgs2_replace_external_forces(jgs2_solver* s, const float* forces, si
 }
 
 jgs2_status jgs2_clear_external_forces(jgs2_solver* s) {
-  jgs2_status st = objective_supported(s);
-  if (st != JGS2_OK) return st;
+  TRYS(objective_supported(s));
   std::vector<float> zeros(3 * size_t(s->V), 0.0f);
   return jgs2_replace_external_forces(s, zeros.data(), zeros.size());
 }
 
 jgs2_status jgs2_set_quadratic_target(jgs2_solver* s, uint32_t vertex, const float target[3], float stiffness) {
-  jgs2_status st = objective_supported(s);
-  if (st != JGS2_OK) return st;
+  TRYS(objective_supported(s));
   if (vertex >= s->V) return fail(JGS2_ERROR_RANGE, "Objective vertex %u is outside vertexCount %u.", vertex, s->V);
   if (!target) return fail(JGS2_ERROR_TYPE, "NULL target.");
   for (int c = 0; c < 3; ++c)
@@ -1406,8 +1865,7 @@ jgs2_status jgs2_set_quadratic_target(jgs2_solver* s, uint32_t vertex, const flo
 
 jgs2_status jgs2_replace_quadratic_targets(jgs2_solver* s, const float* positions, size_t position_count,
                                            const float* stiffnesses, size_t stiffness_count) {
-  jgs2_status st = objective_supported(s);
-  if (st != JGS2_OK) return st;
+  TRYS(objective_supported(s));
   if (!positions || !stiffnesses) return fail(JGS2_ERROR_TYPE, "NULL targets.");
   if (position_count != 3 * size_t(s->V))
     return fail(JGS2_ERROR_RANGE, "Quadratic target positions must contain %zu values; got %zu.", 3 * size_t(s->V),
@@ -1435,63 +1893,62 @@ jgs2_status jgs2_replace_quadratic_targets(jgs2_solver* s, const float* position
 }
 
 jgs2_status jgs2_release_quadratic_target(jgs2_solver* s, uint32_t vertex) {
-  jgs2_status st = objective_supported(s);
-  if (st != JGS2_OK) return st;
+  TRYS(objective_supported(s));
   if (vertex >= s->V) return fail(JGS2_ERROR_RANGE, "Objective vertex %u is outside vertexCount %u.", vertex, s->V);
   s->objectives[8 * size_t(vertex) + 7] = 0;
   return upload_objectives(s, vertex, 1);
 }
 
 jgs2_status jgs2_release_all_quadratic_targets(jgs2_solver* s) {
-  jgs2_status st = objective_supported(s);
-  if (st != JGS2_OK) return st;
+  TRYS(objective_supported(s));
   for (uint32_t v = 0; v < s->V; ++v) s->objectives[8 * size_t(v) + 7] = 0;
   return upload_objectives(s, 0, s->V);
 }
 
 jgs2_status jgs2_read_positions(jgs2_solver* s, float* dst, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.pos[0], dst, count, 4 * size_t(s->V), "readPositions", 4);
+  TRYS(check_usable(s));
+  return read_buffer(s, &s->batch->dev.pos[0]->x, dst, count, 4 * size_t(s->V), "readPositions", 4);
 }
 jgs2_status jgs2_read_velocities(jgs2_solver* s, float* dst, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.velocity, dst, count, 4 * size_t(s->V), "readVelocities", 4);
+  TRYS(check_usable(s));
+  return read_buffer(s, &s->batch->dev.velocity->x, dst, count, 4 * size_t(s->V), "readVelocities", 4);
 }
 jgs2_status jgs2_read_predicted_positions(jgs2_solver* s, float* dst, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.predicted, dst, count, 4 * size_t(s->V), "readPredictedPositions", 4);
+  TRYS(check_usable(s));
+  return read_buffer(s, &s->batch->dev.predicted->x, dst, count, 4 * size_t(s->V), "readPredictedPositions", 4);
 }
 jgs2_status jgs2_read_final_iteration_updates(jgs2_solver* s, float* dst, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.final_update, dst, count, 4 * size_t(s->V), "readFinalIterationUpdates", 4);
+  TRYS(check_usable(s));
+  return read_buffer(s, &s->batch->dev.final_update->x, dst, count, 4 * size_t(s->V), "readFinalIterationUpdates", 4);
 }
 jgs2_status jgs2_read_vertex_rotations(jgs2_solver* s, float* dst, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.vrot, dst, count, 12 * size_t(s->V), "readVertexRotations", 12);
+  TRYS(check_usable(s));
+  return read_buffer(s, &s->batch->dev.vrot->x, dst, count, 12 * size_t(s->V), "readVertexRotations", 12);
 }
 jgs2_status jgs2_read_local_diagnostics(jgs2_solver* s, float* dst, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
+  TRYS(check_usable(s));
   if (!s->globalization) return fail(JGS2_ERROR_STATE, "Globalization diagnostics require the stable material path.");
-  return read_buffer(s, s->dev.local_diag, dst, count, 16 * size_t(s->V), "readLocalDiagnostics", 16);
+  return read_buffer(s, &s->batch->dev.local_diag->x, dst, count, 16 * size_t(s->V), "readLocalDiagnostics", 16);
 }
 jgs2_status jgs2_read_globalization_control(jgs2_solver* s, float* dst, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.control, dst, count, 16, "readGlobalizationControl");
+  TRYS(check_usable(s));
+  return read_buffer(s, &s->batch->dev.control[4 * size_t(s->slot)].x, dst, count, 16, "readGlobalizationControl");
 }
 jgs2_status jgs2_read_globalization_history(jgs2_solver* s, float* dst, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
-  return read_buffer(s, s->dev.history, dst, count, size_t(kHistoryCapacity) * 32, "readGlobalizationHistory");
+  TRYS(check_usable(s));
+  return read_buffer(s, &s->batch->dev.history[size_t(s->slot) * kHistoryCapacity * 8].x, dst, count,
+                     size_t(kHistoryCapacity) * 32, "readGlobalizationHistory");
 }
 
 jgs2_status jgs2_sync(jgs2_solver* s) {
-  if (jgs2_status st = check_usable(s)) return st;
-  HIP_CHECK(hipStreamSynchronize(s->stream));
+  TRYS(check_usable(s));
+  HIP_CHECK(hipStreamSynchronize(s->batch->stream));
   clear_error();
   return JGS2_OK;
 }
 
 jgs2_status jgs2_get_info(jgs2_solver* s, jgs2_solver_info* out) {
-  if (jgs2_status st = check_usable(s)) return st;
+  TRYS(check_usable(s));
   if (!out) return fail(JGS2_ERROR_TYPE, "NULL info.");
   out->vertex_count = s->V;
   out->tet_count = s->T;
@@ -1506,7 +1963,7 @@ jgs2_status jgs2_get_info(jgs2_solver* s, jgs2_solver_info* out) {
 }
 
 jgs2_status jgs2_read_schedule_colors(jgs2_solver* s, uint32_t* colors, size_t count) {
-  if (jgs2_status st = check_usable(s)) return st;
+  TRYS(check_usable(s));
   if (!colors || count != s->V) return fail(JGS2_ERROR_RANGE, "Schedule colors need exactly %u entries.", s->V);
   std::memcpy(colors, s->colors.data(), 4 * size_t(s->V));
   clear_error();
@@ -1515,77 +1972,292 @@ jgs2_status jgs2_read_schedule_colors(jgs2_solver* s, uint32_t* colors, size_t c
 
 jgs2_status jgs2_debug_local_systems(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask, float* g, size_t gc,
                                      float* h, size_t hc) {
-  if (jgs2_status st = check_usable(s)) return st;
+  TRYS(check_usable(s));
   if (!g || !h || gc != 3 * size_t(s->V) || hc != 9 * size_t(s->V))
     return fail(JGS2_ERROR_RANGE, "Local systems need 3V gradient and 9V Hessian floats.");
+  jgs2_batch* b = s->batch;
   jgs2_settings r;
-  if (jgs2_status st = resolve_step(s, ov, mask, "debugLocalSystems", r)) return st;
-  DeviceState& d = s->dev;
+  TRYS(resolve_batch_step(b, ov, mask, "debugLocalSystems", r));
+  DeviceState& d = b->dev;
+  const size_t Vc = size_t(b->va.capacity) * kSceneBlock;
   if (!d.debug_g) {
-    if (jgs2_status st = dalloc(&d.debug_g, 3 * size_t(s->V))) return st;
-    if (jgs2_status st = dalloc(&d.debug_h, 9 * size_t(s->V))) return st;
+    TRYS(dalloc(&d.debug_g, 3 * Vc));
+    TRYS(dalloc(&d.debug_h, 9 * Vc));
   }
-  PassParams p = make_params(s, r, false);
+  PassParams p = make_params(b, r, false);
   p.src = 0;
   p.tgt = 0;
+  hipStream_t q = b->stream;
   if (p.contact) {
     // Contact state of a frame start at the current pose: old = x, lagged
     // contact and active rows. The next frame's predict and lag pass rewrite
     // all of it, so stepping continues unaffected.
-    HIP_CHECK(hipMemcpyAsync(d.old, d.pos[0], s->V * sizeof(float4), hipMemcpyDeviceToDevice, s->stream));
-    launch_lag_contact(d, p, s->stream);
-    launch_build_active_rows(d, p, s->stream);
+    HIP_CHECK(hipMemcpyAsync(d.old, d.pos[0], size_t(p.vertex_count) * sizeof(float4), hipMemcpyDeviceToDevice, q));
+    launch_lag_contact(d, p, q);
+    launch_build_active_rows(d, p, q);
   }
-  if (s->corotated) launch_tet_polar(d, p, s->stream);
-  launch_vertex_polar(d, p, s->stream);
-  launch_debug_local_systems(d, p, s->stream);
+  if (b->corotated) launch_tet_polar(d, p, q);
+  launch_vertex_polar(d, p, q);
+  launch_debug_local_systems(d, p, q);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s->stream));
-  std::vector<float> gs(gc), hs(hc);
-  HIP_CHECK(hipMemcpy(gs.data(), d.debug_g, gc * sizeof(float), hipMemcpyDeviceToHost));
-  HIP_CHECK(hipMemcpy(hs.data(), d.debug_h, hc * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipStreamSynchronize(q));
+  const size_t Vp = size_t(s->vblk.n) * kSceneBlock;
+  std::vector<float> gs(3 * Vp), hs(9 * Vp);
+  HIP_CHECK(hipMemcpy(gs.data(), d.debug_g + 3 * s->vbase(), gs.size() * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(hs.data(), d.debug_h + 9 * s->vbase(), hs.size() * sizeof(float), hipMemcpyDeviceToHost));
   for (uint32_t ov = 0; ov < s->V; ++ov) {
     std::memcpy(g + 3 * size_t(ov), &gs[3 * size_t(s->vnew[ov])], 3 * sizeof(float));
     std::memcpy(h + 9 * size_t(ov), &hs[9 * size_t(s->vnew[ov])], 9 * sizeof(float));
   }
+  b->graph_valid = false;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_profile_begin(jgs2_batch* b) {
+  TRYS(check_batch(b));
+  if (b->events.empty()) {
+    b->events.resize(8192);
+    for (auto& e : b->events) HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventCreate(&b->frame_begin));
+    HIP_CHECK(hipEventCreate(&b->frame_end));
+  }
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->profiling = true;
+  b->event_used = 0;
+  HIP_CHECK(hipEventRecord(b->frame_begin, b->stream));
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_profile_end(jgs2_batch* b, double* ms_per_launch, uint64_t* launches, double* total_ms) {
+  TRYS(check_batch(b));
+  if (!b->profiling) return fail(JGS2_ERROR_STATE, "Profiling was not started.");
+  HIP_CHECK(hipEventRecord(b->frame_end, b->stream));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  double sum = 0;
+  for (size_t i = 0; i + 1 < b->event_used; i += 2) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, b->events[i], b->events[i + 1]));
+    sum += ms;
+  }
+  const uint64_t n = b->event_used / 2;
+  float total = 0;
+  HIP_CHECK(hipEventElapsedTime(&total, b->frame_begin, b->frame_end));
+  if (ms_per_launch) *ms_per_launch = n ? sum / double(n) : 0.0;
+  if (launches) *launches = n;
+  if (total_ms) *total_ms = total;
+  b->profiling = false;
   clear_error();
   return JGS2_OK;
 }
 
 jgs2_status jgs2_profile_begin(jgs2_solver* s) {
-  if (jgs2_status st = check_usable(s)) return st;
-  if (s->events.empty()) {
-    s->events.resize(8192);
-    for (auto& e : s->events) HIP_CHECK(hipEventCreate(&e));
-    HIP_CHECK(hipEventCreate(&s->frame_begin));
-    HIP_CHECK(hipEventCreate(&s->frame_end));
-  }
-  HIP_CHECK(hipStreamSynchronize(s->stream));
-  s->profiling = true;
-  s->event_used = 0;
-  HIP_CHECK(hipEventRecord(s->frame_begin, s->stream));
+  TRYS(check_usable(s));
+  return jgs2_batch_profile_begin(s->batch);
+}
+
+jgs2_status jgs2_profile_end(jgs2_solver* s, double* ms_per_launch, uint64_t* launches, double* total_ms) {
+  TRYS(check_usable(s));
+  return jgs2_batch_profile_end(s->batch, ms_per_launch, launches, total_ms);
+}
+
+// ---------------------------------------------------------------- batches
+jgs2_status jgs2_batch_create(const jgs2_batch_config* config, const jgs2_settings* overrides, uint64_t mask,
+                              jgs2_batch** out) {
+  if (!config || !out) return fail(JGS2_ERROR_TYPE, "jgs2_batch_create requires a config and an output handle.");
+  *out = nullptr;
+  if (config->max_scenes < 1 || config->max_vertices < 1 || config->max_tets < 1)
+    return fail(JGS2_ERROR_RANGE, "A batch needs room for at least one scene, vertex and tetrahedron.");
+  if (config->cubature_k > 64) return fail(JGS2_ERROR_RANGE, "cubature_k must be at most 64; got %u.", config->cubature_k);
+  jgs2_settings defaults;
+  jgs2_default_settings(&defaults);
+  jgs2_settings resolved;
+  TRYS(merge_settings(defaults, overrides, mask, resolved));
+  TRYS(batch_create(*config, resolved, out));
   clear_error();
   return JGS2_OK;
 }
 
-jgs2_status jgs2_profile_end(jgs2_solver* s, double* ms_per_launch, uint64_t* launches, double* total_ms) {
-  if (jgs2_status st = check_usable(s)) return st;
-  if (!s->profiling) return fail(JGS2_ERROR_STATE, "jgs2_profile_begin was not called.");
-  HIP_CHECK(hipEventRecord(s->frame_end, s->stream));
-  HIP_CHECK(hipStreamSynchronize(s->stream));
-  double sum = 0;
-  for (size_t i = 0; i + 1 < s->event_used; i += 2) {
-    float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, s->events[i], s->events[i + 1]));
-    sum += ms;
+jgs2_status jgs2_batch_destroy(jgs2_batch* b) {
+  if (!b) return JGS2_OK;
+  batch_destroy(b);
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_admit(jgs2_batch* b, const jgs2_scene_input* input, jgs2_solver** scene) {
+  TRYS(check_batch(b));
+  if (!input || !scene) return fail(JGS2_ERROR_TYPE, "jgs2_batch_admit requires an input and an output handle.");
+  *scene = nullptr;
+  jgs2_solver* s = new (std::nothrow) jgs2_solver();
+  if (!s) return fail(JGS2_ERROR_MEMORY, "Out of host memory.");
+  jgs2_status st = prepare_scene(*input, b->defaults, s);
+  if (st != JGS2_OK) {
+    delete s;
+    return st;
   }
-  const uint64_t n = s->event_used / 2;
-  float total = 0;
-  HIP_CHECK(hipEventElapsedTime(&total, s->frame_begin, s->frame_end));
-  if (ms_per_launch) *ms_per_launch = n ? sum / double(n) : 0.0;
-  if (launches) *launches = n;
-  if (total_ms) *total_ms = total;
-  s->profiling = false;
+  return place_scene(b, s, *input, scene);
+}
+
+jgs2_status jgs2_batch_retire(jgs2_batch* b, jgs2_solver* scene) {
+  TRYS(check_batch(b));
+  TRYS(check_usable(scene));
+  if (scene->batch != b || scene->owns_batch) return fail(JGS2_ERROR_STATE, "The scene is not resident in this batch.");
+  TRYS(retire(b, scene));
+  delete scene;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_step_frames(jgs2_batch* b, uint32_t frames, const jgs2_settings* ov, uint64_t mask) {
+  TRYS(check_batch(b));
+  TRYS(validate_frames(frames));
+  jgs2_settings r;
+  TRYS(resolve_batch_step(b, ov, mask, "stepFrames", r));
+  return submit(b, r, normalize_odd(r.iterations), frames, true);
+}
+
+jgs2_status jgs2_batch_step_frames_exact_iterations(jgs2_batch* b, uint32_t frames, uint32_t iterations,
+                                                    const jgs2_settings* ov, uint64_t mask) {
+  TRYS(check_batch(b));
+  TRYS(validate_frames(frames));
+  if (iterations < 1) return fail(JGS2_ERROR_RANGE, "iterations must be a positive safe integer; got %u.", iterations);
+  jgs2_settings r;
+  TRYS(resolve_batch_step(b, ov, mask, "stepFramesExactIterations", r));
+  return submit(b, r, iterations, frames, false);
+}
+
+jgs2_status jgs2_batch_sync(jgs2_batch* b) {
+  TRYS(check_batch(b));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_get_info(jgs2_batch* b, jgs2_batch_info* out) {
+  TRYS(check_batch(b));
+  if (!out) return fail(JGS2_ERROR_TYPE, "NULL info.");
+  std::memset(out, 0, sizeof(*out));
+  out->scene_count = b->active_scenes();
+  out->max_scenes = uint32_t(b->slots.size());
+  out->solve_block = b->solve_block;
+  out->xcd_remap = b->xcd_remap;
+  out->use_graphs = b->use_graphs ? 1u : 0u;
+  uint64_t used = 0;
+  for (auto* s : b->slots)
+    if (s) used += s->vblk.n;
+  out->vertex_capacity = b->va.capacity * kSceneBlock;
+  out->vertex_blocks_used = used;
+  out->vertex_extent = b->va.high() * kSceneBlock;
+  out->tet_extent = b->ta.high() * kSceneBlock;
+  out->arena_bytes = b->arena_bytes;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_arena_bytes(const jgs2_batch_config* c, uint64_t* bytes) {
+  if (!c || !bytes) return fail(JGS2_ERROR_TYPE, "NULL argument.");
+  // The allocation list of batch_allocate, in bytes.
+  const uint64_t Vc = std::max<uint64_t>(1, blocks_of(c->max_vertices)) * kSceneBlock;
+  const uint64_t Tc = std::max<uint64_t>(1, blocks_of(c->max_tets)) * kSceneBlock;
+  const uint64_t Rc = blocks_of(c->max_triangles) * kSceneBlock;
+  const uint64_t Cc = blocks_of(c->max_candidates) * kSceneBlock;
+  const uint64_t Hc = c->max_hinges, Bc = std::max<uint64_t>(1, c->max_bodies), S = std::max<uint32_t>(1, c->max_scenes);
+  const uint64_t K = std::max<uint64_t>(1, c->cubature_k), f4 = 16, u = 4;
+  auto nz = [](uint64_t n) { return std::max<uint64_t>(n, 1); };
+  uint64_t total = 0;
+  total += Vc * (f4 * (1 + 2 + 5 + 3 + 3 + 1 + 4 + 1 + 5) + 16 + u * 3 + 8 * 3 + u);  // vertex records
+  total += Vc * 10 * K * 16;                                                          // cubature
+  total += Tc * (16 + 3 * f4 + 3 * f4 + 2 * f4 + 4 * u) + (c->corotated ? Tc * 144 * 4 : 0);
+  total += nz(Rc) * (16 + f4 + 4 + 2 * f4) + nz(3 * Rc) * u;
+  total += nz(Hc) * (16 + f4) + nz(4 * Hc) * u;
+  total += nz(Cc) * (16 + u + 7 * f4) + 2 * nz(4 * Cc) * u;
+  total += Bc * (f4 + 16);
+  total += S * (4 * f4 + 64 * 8 * f4 + f4 + 4 + 4 * 16) + scene_conv_bytes(uint32_t(S));
+  total += (Vc + Tc + nz(Rc) + nz(Cc)) / kSceneBlock * u;
+  total += partial_bytes(partial_blocks_needed(uint32_t(Vc), uint32_t(Tc), uint32_t(Rc), uint32_t(Cc)));
+  *bytes = total;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_use_graphs(jgs2_batch* b, int enable) {
+  TRYS(check_batch(b));
+  b->use_graphs = enable != 0;
+  b->graph_valid = false;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_set_launch(jgs2_batch* b, uint32_t solve_block, uint32_t xcd_remap) {
+  TRYS(check_batch(b));
+  if (solve_block != 64 && solve_block != 128 && solve_block != 256)
+    return fail(JGS2_ERROR_RANGE, "solve_block must be 64, 128 or 256; got %u.", solve_block);
+  b->solve_block = solve_block;
+  b->xcd_remap = xcd_remap ? 1u : 0u;
+  b->graph_valid = false;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_autotune(jgs2_batch* b, const jgs2_settings* ov, uint64_t mask, uint32_t repeats,
+                                double* best_ms) {
+  TRYS(check_batch(b));
+  if (b->active_scenes() == 0) return fail(JGS2_ERROR_STATE, "The batch holds no scene.");
+  if (repeats < 1 || repeats > 1000) return fail(JGS2_ERROR_RANGE, "repeats must be in [1, 1000]; got %u.", repeats);
+  jgs2_settings r;
+  TRYS(resolve_batch_step(b, ov, mask, "autotune", r));
+  // Times the Jacobi local solve alone from the accepted pose into the
+  // second position buffer, with the vertex frames the last frame left.
+  // The buffers it writes (trial positions, vertex energies, local
+  // diagnostics) are rewritten by the next frame before they are read, so
+  // stepping is unaffected; only jgs2_read_local_diagnostics() shows the
+  // timing passes until the next frame.
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  const uint32_t blocks[3] = {64, 128, 256};
+  double best = INFINITY;
+  uint32_t best_block = b->solve_block, best_remap = b->xcd_remap;
+  hipError_t err = hipSuccess;
+  for (uint32_t remap = 0; remap < 2 && err == hipSuccess; ++remap)
+    for (uint32_t bi = 0; bi < 3 && err == hipSuccess; ++bi) {
+      b->solve_block = blocks[bi];
+      b->xcd_remap = remap;
+      PassParams p = make_params(b, r, false);
+      p.src = 0;
+      p.tgt = 1;
+      launch_solve_config(b->dev, p, b->stream);  // warm
+      (void)hipEventRecord(e0, b->stream);
+      for (uint32_t k = 0; k < repeats; ++k) launch_solve_config(b->dev, p, b->stream);
+      (void)hipEventRecord(e1, b->stream);
+      err = hipEventSynchronize(e1);
+      float ms = 0;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      const double per = double(ms) / repeats;
+      if (err == hipSuccess && per < best) {
+        best = per;
+        best_block = blocks[bi];
+        best_remap = remap;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err == hipSuccess) err = hipGetLastError();
+  b->solve_block = best_block;
+  b->xcd_remap = best_remap;
+  b->graph_valid = false;
+  if (err != hipSuccess) return fail(JGS2_ERROR_DEVICE, "Autotuning failed: %s", hipGetErrorString(err));
+  if (best_ms) *best_ms = best;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_solver_batch(jgs2_solver* s, jgs2_batch** out) {
+  TRYS(check_usable(s));
+  if (!out) return fail(JGS2_ERROR_TYPE, "NULL output.");
+  *out = s->batch;
   clear_error();
   return JGS2_OK;
 }

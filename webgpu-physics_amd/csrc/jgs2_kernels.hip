@@ -106,8 +106,9 @@ __device__ __forceinline__ float scaled_length3(f3 v) {  // jgs2-globalization-w
 
 // XCD-aware block mapping: consecutive logical blocks land on one XCD so
 // that vertices sharing tetrahedra share an L2. Bijective for any grid.
-__device__ __forceinline__ uint32_t logical_block() {
+__device__ __forceinline__ uint32_t logical_block(uint32_t remap) {
   const uint32_t grid = gridDim.x, bid = blockIdx.x;
+  if (!remap) return bid;
   const uint32_t q = grid / 8u, r = grid % 8u;
   const uint32_t xcd = bid % 8u, j = bid / 8u;
   const uint32_t base = xcd < r ? xcd * (q + 1u) : r * (q + 1u) + (xcd - r) * q;
@@ -121,10 +122,12 @@ __device__ __forceinline__ const float4* rot_source(const DeviceState& s, const 
   return p.fused_rot ? s.vrot_next : s.vrot;
 }
 
-__device__ __forceinline__ bool stop_latched(const DeviceState& s, const PassParams& p) {  // S:909-917
+__device__ __forceinline__ bool stop_latched(const DeviceState& s, const PassParams& p, uint32_t sc) {  // S:909-917
   if (!p.stop_enabled) return false;
-  return (control_bits(s.control[0].w) & kStopLatch) != 0u;
+  return (control_bits(s.control[4 * size_t(sc)].w) & kStopLatch) != 0u;
 }
+// Scene of a vertex (kNoScene in free arena blocks).
+__device__ __forceinline__ uint32_t vscene(const DeviceState& s, uint32_t v) { return s.vblock_scene[v / kSceneBlock]; }
 
 // ------------------------------------------------------------------ element access
 struct Tet {
@@ -529,12 +532,12 @@ __device__ __forceinline__ float bending_energy_at(const DeviceState& s, uint32_
 // Membrane and bending contributions to one vertex's force (S:4028-4085).
 __device__ __forceinline__ f3 cloth_gradient(const DeviceState& s, const float4* pos, uint32_t v) {
   f3 g = mk(0, 0, 0);
-  for (uint32_t k = s.tri_rows[v]; k < s.tri_rows[v + 1]; ++k) {
+  for (uint32_t k = s.tri_row[v].x, ke = k + s.tri_row[v].y; k < ke; ++k) {
     const uint32_t tri = s.tri_elems[k];
     const uint32_t slot = tri_slot(s.tri_ix[tri], v);
     if (slot < 3) g = g + membrane_gradient(s, tri, membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)), slot);
   }
-  for (uint32_t k = s.hinge_rows[v]; k < s.hinge_rows[v + 1]; ++k) {
+  for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {
     const uint32_t h = s.hinge_elems[k];
     const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
     if (slot >= 4) continue;
@@ -715,8 +718,8 @@ __device__ __forceinline__ float ipc_friction_dissipation(float slip, float eps,
   }
   return scale * (s - eps / 3.0f);
 }
-__device__ __forceinline__ float ipc_rounding_reach(const PassParams& p) {  // S:509-513
-  return fmaxf(fmaxf(p.scene_scale, p.ipc_dhat), 1.0e-12f) * kUnitRoundoff * kPositionResolutionMultiplier;
+__device__ __forceinline__ float ipc_rounding_reach(const PassParams& p, float scene_scale) {  // S:509-513
+  return fmaxf(fmaxf(scene_scale, p.ipc_dhat), 1.0e-12f) * kUnitRoundoff * kPositionResolutionMultiplier;
 }
 __device__ __forceinline__ float ipc_aabb_lower_bound(bool vt, f3 p0, f3 p1, f3 p2, f3 p3) {  // S:525-550
   f3 gap;
@@ -734,7 +737,7 @@ __device__ __forceinline__ float ipc_aabb_lower_bound(bool vt, f3 p0, f3 p1, f3 
 __device__ Contact ipc_contact_at(const DeviceState& s, uint32_t c, const float4* pos, float threshold,
                                   uint32_t mv = kNoVertex, f3 mp = f3{0, 0, 0}) {
   const uint4 ix = s.cand[c];
-  const bool vt = c < s.NVT;
+  const bool vt = s.cand_meta[c] == 0u;
   const f3 p0 = cloth_pos(pos, ix.x, mv, mp), p1 = cloth_pos(pos, ix.y, mv, mp);
   const f3 p2 = cloth_pos(pos, ix.z, mv, mp), p3 = cloth_pos(pos, ix.w, mv, mp);
   const float lb = ipc_aabb_lower_bound(vt, p0, p1, p2, p3);
@@ -798,7 +801,7 @@ __device__ __forceinline__ uint32_t ipc_energy_owner(const DeviceState& s, uint4
 // S:786-874) over its active candidate row.
 __device__ void ipc_local_contribution(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
                                        f3& g, m3& H) {
-  const uint32_t begin = s.cand_rows[v], end = begin + s.active_count[v];
+  const uint32_t begin = s.cand_row[v].x, end = begin + s.active_count[v];
 #pragma unroll 1
   for (uint32_t i = begin; i < end; ++i) {
     const uint32_t c = s.active_ids[i];
@@ -854,7 +857,7 @@ __device__ void ipc_local_contribution(const DeviceState& s, const PassParams& p
 __device__ float ipc_owned_energy(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
                                   bool use_target) {
   float r = 0.0f;
-  const uint32_t begin = s.cand_rows[v], end = begin + s.active_count[v];
+  const uint32_t begin = s.cand_row[v].x, end = begin + s.active_count[v];
 #pragma unroll 1
   for (uint32_t i = begin; i < end; ++i) {
     const uint32_t c = s.active_ids[i];
@@ -897,7 +900,7 @@ __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const Pa
   }
   if constexpr (kCloth) {
     // Cloth elements are summed once, by their owner vertex (S:2585-2643).
-    for (uint32_t k = s.tri_rows[v]; k < s.tri_rows[v + 1]; ++k) {
+    for (uint32_t k = s.tri_row[v].x, ke = k + s.tri_row[v].y; k < ke; ++k) {
       const uint32_t tri = s.tri_elems[k];
       if (s.tri_ix[tri].w != v) continue;
       const float tra = 2.0f * s.tri_area[tri];
@@ -905,14 +908,14 @@ __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const Pa
       if (!(tra > 0.0f) || !finite_s(stretch) || !(stretch > kDeterminantFloor)) return kF32Max;
       e += membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy;
     }
-    for (uint32_t k = s.hinge_rows[v]; k < s.hinge_rows[v + 1]; ++k) {
+    for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {
       const uint32_t h = s.hinge_elems[k];
       if (__float_as_uint(s.hinge_rest[h].z) != v) continue;
       e += bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
     }
   }
   if constexpr (kIpc) {  // S:2643 with ipcCachedContactAt (S:450-460)
-    const bool use_target = p.tgt != p.src && is_target && s.contact_ctl[0].w != 0.0f;
+    const bool use_target = p.tgt != p.src && is_target && s.contact_ctl[vscene(s, v)].w != 0.0f;
     e += ipc_owned_energy(s, p, v, pos, use_target);
   }
   return e;
@@ -1154,7 +1157,7 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
   }
   if constexpr (kCloth) {
 #pragma unroll 1
-    for (uint32_t k = s.tri_rows[v]; k < s.tri_rows[v + 1]; ++k) {  // S:3117-3144
+    for (uint32_t k = s.tri_row[v].x, ke = k + s.tri_row[v].y; k < ke; ++k) {  // S:3117-3144
       const uint32_t tri = s.tri_elems[k];
       const uint32_t slot = tri_slot(s.tri_ix[tri], v);
       if (slot >= 3) continue;
@@ -1163,7 +1166,7 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
       L.H = madd(L.H, membrane_local_hessian(s, tri, m, slot));
     }
 #pragma unroll 1
-    for (uint32_t k = s.hinge_rows[v]; k < s.hinge_rows[v + 1]; ++k) {  // S:3145-3184
+    for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {  // S:3145-3184
       const uint32_t h = s.hinge_elems[k];
       const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
       if (slot >= 4) continue;
@@ -1227,7 +1230,7 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
     // energy delta (S:2140-2193) in one pass over its active row.
     const f3 tp = x + d;
     const float dl = sqrtf(dot(d, d));
-    const uint32_t begin = s.cand_rows[v], end = begin + s.active_count[v];
+    const uint32_t begin = s.cand_row[v].x, end = begin + s.active_count[v];
 #pragma unroll 1
     for (uint32_t i = begin; i < end; ++i) {
       const uint32_t c = s.active_ids[i];
@@ -1283,7 +1286,7 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
   if constexpr (kCloth) {
     const f3 tp = x + d;
 #pragma unroll 1
-    for (uint32_t k = s.tri_rows[v]; k < s.tri_rows[v + 1]; ++k) {
+    for (uint32_t k = s.tri_row[v].x, ke = k + s.tri_row[v].y; k < ke; ++k) {
       const uint32_t tri = s.tri_elems[k];
       if (tri_slot(s.tri_ix[tri], v) >= 3) continue;
       // Signed relative area stretch (S:2062-2116).
@@ -1302,7 +1305,7 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
       energy += membrane_at(s, tri, pos, v, tp).energy - membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy;
     }
 #pragma unroll 1
-    for (uint32_t k = s.hinge_rows[v]; k < s.hinge_rows[v + 1]; ++k) {  // S:2237-2271
+    for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {  // S:2237-2271
       const uint32_t h = s.hinge_elems[k];
       if (hinge_slot(s.hinge_ix[h], v) >= 4) continue;
       energy += bending_energy_at(s, h, pos, v, tp) - bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
@@ -1377,14 +1380,15 @@ __device__ __forceinline__ void store_local_diag(const DeviceState& s, uint32_t 
 }
 
 // ------------------------------------------------------------------ kernels
-__global__ void __launch_bounds__(kBlock) k_clear_frame(DeviceState s) {
+__global__ void __launch_bounds__(kBlock) k_clear_frame(DeviceState s, PassParams p) {
   // Control records are cleared by predict's first invocation (S:2687-2691).
-  if (threadIdx.x < 4) s.control[threadIdx.x] = make_float4(0, 0, 0, 0);
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < 4 * p.scene_count) s.control[i] = make_float4(0, 0, 0, 0);
 }
 
 __global__ void __launch_bounds__(kBlock) k_predict(DeviceState s, PassParams p) {  // S:2661-2692
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
-  if (v >= p.vertex_count) return;
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
+  if (v >= p.vertex_count || vscene(s, v) == kNoScene) return;
   const f3 o = load_pos(s.pos[0], v);
   s.old[v] = make_float4(o.x, o.y, o.z, 1.0f);
   const bool pinned = s.vtx_info[v].z != 0u;
@@ -1404,9 +1408,11 @@ __global__ void __launch_bounds__(kBlock) k_predict(DeviceState s, PassParams p)
 __device__ m3 polar_rotation(m3 r, float eps);
 
 __global__ void __launch_bounds__(kBlock) k_tet_polar(DeviceState s, PassParams p) {  // S:2932-2957
-  const uint32_t t = logical_block() * kBlock + threadIdx.x;
+  const uint32_t t = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
   if (t >= p.tet_count) return;
-  if (stop_latched(s, p)) return;
+  const uint32_t sc = s.tblock_scene[t / kSceneBlock];
+  if (sc == kNoScene || t >= s.scene_end[sc].y || stop_latched(s, p, sc)) return;
+  if (p.colored && p.color_index >= s.scene_cnt[sc].z) return;  // the scene's sweep has ended
   const Tet e = load_tet(s, t);
   store_rot(s.trot, t, polar_rotation(deformation(e, s.pos[p.src]), p.polar_eps));
 }
@@ -1442,9 +1448,13 @@ __device__ m3 polar_rotation(m3 r, float eps) {  // S:940-978
 }
 
 __global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassParams p) {  // S:2960-2989
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
-  if (stop_latched(s, p)) return;
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || stop_latched(s, p, sc)) return;
+  // A scene with fewer colors than the batch has finished its sweep: its
+  // frames stay those of its own last color pass.
+  if (p.colored && p.color_index >= s.scene_cnt[sc].z) return;
   const uint4 inf = s.vtx_info[v];
   if (inf.y == 0u) {
     store_rot(s.vrot, v, mid());
@@ -1469,7 +1479,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassPara
 template <bool kCorot, bool kCloth, bool kIpc>
 __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
   constexpr bool kEnergyInSolve = !kCloth && !kIpc;
-  const uint32_t lane = logical_block() * kBlock + threadIdx.x;
+  const uint32_t lane = logical_block(p.xcd_remap) * blockDim.x + threadIdx.x;
   uint32_t v;
   if (p.colored) {
     if (p.color_begin + lane >= p.color_end) return;
@@ -1478,8 +1488,10 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
     if (lane >= p.vertex_count) return;
     v = lane;
   }
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene) return;
   float4* tgt = s.pos[p.tgt];
-  if (stop_latched(s, p)) {
+  if (stop_latched(s, p, sc)) {
     tgt[v] = s.pos[p.src][v];
     return;
   }
@@ -1504,7 +1516,7 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
   const f3 x = load_pos(s.pos[p.src], v);
   float initial_min = kF32Max;
   if (p.globalization) {
-    const float4 pre = s.control[0];
+    const float4 pre = s.control[4 * size_t(sc)];
     const uint32_t pv = control_bits(pre.w) & kValidityMask;
     const bool amv = (pv & kAcceptedMinimumValid) != 0u && finite_s(pre.z);
     if (amv) initial_min = pre.z;
@@ -1548,8 +1560,8 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
   uint32_t status = kNonfinite;
   const f3 full = x + dr.dir;
   const f3 fd = full - x;
-  const float sc = fmaxf(fabsf(x.x), fmaxf(fabsf(x.y), fabsf(x.z)));
-  const float resolution = fmaxf(p.scene_scale, fmaxf(sc, 1.0e-12f)) * kUnitRoundoff * kPositionResolutionMultiplier;
+  const float xs = fmaxf(fabsf(x.x), fmaxf(fabsf(x.y), fabsf(x.z)));
+  const float resolution = fmaxf(s.scene_scale[sc], fmaxf(xs, 1.0e-12f)) * kUnitRoundoff * kPositionResolutionMultiplier;
   const bool below = dr.status == kAccepted && finite3(full) && finite3(fd) && scaled_length3(fd) <= resolution;
   if (dr.status == kAccepted) status = below ? kZeroGradient : kLineSearchFailed;
   else if (dr.status == kZeroGradient) status = kZeroGradient;
@@ -1639,9 +1651,10 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
 // records alone once the stop latch is set (`energy_after_sweep`).
 template <bool kCloth, bool kIpc>
 __global__ void __launch_bounds__(kBlock) k_vertex_energy(DeviceState s, PassParams p) {
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
-  if (p.energy_after_sweep && stop_latched(s, p)) return;
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || (p.energy_after_sweep && stop_latched(s, p, sc))) return;
   store_vertex_energy<kCloth, kIpc>(s, p, v);
 }
 
@@ -1714,32 +1727,34 @@ __device__ __forceinline__ bool local_status_failure(uint32_t st) {  // S:3603-3
 
 // candidateTetrahedron (S:3499-3601) fused with the tetrahedral half of the
 // candidate reduction; blocks past the tetrahedra reduce the vertex records.
-__global__ void __launch_bounds__(kBlock) k_candidate_partials(DeviceState s, PassParams p, uint32_t tet_blocks) {
+// Blocks [0, tb) cover tetrahedra, [tb, tb + rb) cloth triangles and the
+// rest vertices; each block lies inside one scene's range.
+__global__ void __launch_bounds__(kBlock) k_candidate_partials(DeviceState s, PassParams p, uint32_t tb, uint32_t rb) {
   __shared__ CandPartial sh[kBlock];
   CandPartial r;
   cand_identity(r);
   const uint32_t b = blockIdx.x;
-  const bool latched = stop_latched(s, p);
-  if (b < tet_blocks) {
-    const uint32_t t = b * kBlock + threadIdx.x;
-    if (t < p.tet_count + p.tri_count) {
+  uint32_t sc, base, kind;
+  if (b < tb) {
+    sc = s.tblock_scene[b];
+    base = b * kBlock;
+    kind = 0;
+  } else if (b < tb + rb) {
+    sc = s.rblock_scene[b - tb];
+    base = (b - tb) * kBlock;
+    kind = 1;
+  } else {
+    sc = s.vblock_scene[b - tb - rb];
+    base = (b - tb - rb) * kBlock;
+    kind = 2;
+  }
+  if (sc == kNoScene) return;  // uniform per block
+  const bool latched = stop_latched(s, p, sc);
+  if (kind == 0) {
+    const uint32_t t = base + threadIdx.x;
+    if (t < s.scene_end[sc].y) {
       float4 geo, en;
-      if (!latched && t >= p.tet_count) {
-        // Cloth triangles: signed relative area stretch (S:3508-3549).
-        const uint32_t tri = t - p.tet_count;
-        const f3 sn = tri_raw_normal(s, tri, s.pos[p.src], kNoVertex, mk(0, 0, 0));
-        const f3 cn = tri_raw_normal(s, tri, s.pos[p.tgt], kNoVertex, mk(0, 0, 0));
-        const float tra = 2.0f * s.tri_area[tri];
-        const float ss = len3(sn) / tra;
-        const float cm = len3(cn) / tra;
-        const float cs = dot(sn, cn) > 0.0f ? cm : -cm;
-        const bool sgv = tra > 0.0f && finite3(sn) && finite_s(ss);
-        const bool cgv = tra > 0.0f && finite3(cn) && finite_s(cs);
-        geo = make_float4(sgv ? ss : 0.0f, cgv ? cs : 0.0f, sgv ? 1.0f : 0.0f, cgv ? 1.0f : 0.0f);
-        en = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
-        s.tet_glob[2 * size_t(t)] = geo;
-        s.tet_glob[2 * size_t(t) + 1] = en;
-      } else if (!latched) {
+      if (!latched) {
         const Tet e = load_tet(s, t);
         const m3 sf = deformation(e, s.pos[p.src]);
         const m3 cf = deformation(e, s.pos[p.tgt]);
@@ -1772,8 +1787,37 @@ __global__ void __launch_bounds__(kBlock) k_candidate_partials(DeviceState s, Pa
       cand_add_energy(r, 0, en.x, en.z == 1.0f && finite_s(en.x));
       cand_add_energy(r, 1, en.y, en.w == 1.0f && finite_s(en.y));
     }
+  } else if (kind == 1) {
+    const uint32_t tri = base + threadIdx.x;
+    if (tri < s.scene_end[sc].z) {
+      float4 geo;
+      if (!latched) {
+        // Cloth triangles: signed relative area stretch (S:3508-3549).
+        const f3 sn = tri_raw_normal(s, tri, s.pos[p.src], kNoVertex, mk(0, 0, 0));
+        const f3 cn = tri_raw_normal(s, tri, s.pos[p.tgt], kNoVertex, mk(0, 0, 0));
+        const float tra = 2.0f * s.tri_area[tri];
+        const float ss = len3(sn) / tra;
+        const float cm = len3(cn) / tra;
+        const float cs = dot(sn, cn) > 0.0f ? cm : -cm;
+        const bool sgv = tra > 0.0f && finite3(sn) && finite_s(ss);
+        const bool cgv = tra > 0.0f && finite3(cn) && finite_s(cs);
+        geo = make_float4(sgv ? ss : 0.0f, cgv ? cs : 0.0f, sgv ? 1.0f : 0.0f, cgv ? 1.0f : 0.0f);
+        s.tri_glob[2 * size_t(tri)] = geo;
+        s.tri_glob[2 * size_t(tri) + 1] = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
+      } else {
+        geo = s.tri_glob[2 * size_t(tri)];
+      }
+      const bool sgv = geo.z == 1.0f && finite_s(geo.x);
+      const bool cgv = geo.w == 1.0f && finite_s(geo.y);
+      if (sgv) r.minj0 = fminf(r.minj0, geo.x);
+      else r.bits &= ~kSourceGeometryValid;
+      if (cgv) r.minj1 = fminf(r.minj1, geo.y);
+      else r.bits &= ~kCandidateGeometryValid;
+      cand_add_energy(r, 0, 0.0f, true);
+      cand_add_energy(r, 1, 0.0f, true);
+    }
   } else {
-    const uint32_t v = (b - tet_blocks) * kBlock + threadIdx.x;
+    const uint32_t v = base + threadIdx.x;
     if (v < p.vertex_count) {
       const float4* l = s.local_diag + 4 * size_t(v);
       const float4 l0 = l[0], l1 = l[1], l2 = l[2], l3 = l[3];
@@ -1807,16 +1851,27 @@ __global__ void __launch_bounds__(kBlock) k_candidate_partials(DeviceState s, Pa
   if (threadIdx.x == 0) reinterpret_cast<CandPartial*>(s.partials)[b] = r;
 }
 
-__global__ void __launch_bounds__(kBlock) k_candidate_final(DeviceState s, PassParams p, uint32_t blocks) {
+// One block per scene: its tetrahedron, triangle and vertex partials in
+// that order, then the control records (S:3612-3955 final lane).
+__global__ void __launch_bounds__(kBlock) k_candidate_final(DeviceState s, PassParams p, uint32_t tb, uint32_t rb,
+                                                             uint32_t first_scene) {
   __shared__ CandPartial sh[kBlock];
+  const uint32_t sc = first_scene + blockIdx.x;
+  if (s.scene_cnt[sc].z == 0u) return;
+  const uint4 b0 = s.scene_blk0[sc], b1 = s.scene_blk1[sc];
+  const uint32_t nt = b0.w - b0.z, nr = b1.y - b1.x, nv = b0.y - b0.x;
   CandPartial r;
   cand_identity(r);
   const CandPartial* parts = reinterpret_cast<const CandPartial*>(s.partials);
-  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) cand_combine(r, parts[i]);
+  for (uint32_t i = threadIdx.x; i < nt + nr + nv; i += kBlock) {
+    const uint32_t at = i < nt ? b0.z + i : (i < nt + nr ? tb + b1.x + (i - nt) : tb + rb + b0.x + (i - nt - nr));
+    cand_combine(r, parts[at]);
+  }
   block_reduce(r, sh, cand_combine);
-  if (threadIdx.x != 0 || stop_latched(s, p)) return;
+  if (threadIdx.x != 0 || stop_latched(s, p, sc)) return;
+  float4* control = s.control + 4 * size_t(sc);
   uint32_t bits = r.bits;
-  if (p.tet_count == 0) bits &= ~(kSourceGeometryValid | kCandidateGeometryValid);
+  if (s.scene_cnt[sc].y == 0u) bits &= ~(kSourceGeometryValid | kCandidateGeometryValid);
   const bool sgv = (bits & kSourceGeometryValid) != 0u, cgv = (bits & kCandidateGeometryValid) != 0u;
   const bool sev = (bits & kSourceEnergyValid) != 0u, cev = (bits & kCandidateEnergyValid) != 0u;
   const bool accepted = sgv && r.minj0 > kDeterminantFloor && cgv && r.minj1 > kDeterminantFloor;
@@ -1826,21 +1881,22 @@ __global__ void __launch_bounds__(kBlock) k_candidate_final(DeviceState s, PassP
   const bool aev = accepted ? cev : sev;
   if (amv) bits |= kAcceptedMinimumValid;
   uint32_t hist = 0;
-  const float packed = s.control[1].w;
+  const float packed = control[1].w;
   if (finite_nonneg_int(packed, float(kHistoryCapacity))) hist = uint32_t(packed);
   else bits &= ~kLocalNumericsValid;
   const uint32_t cb = bits & kValidityMask;
-  s.control[0] = make_float4(sgv ? r.minj0 : 0.0f, cgv ? r.minj1 : 0.0f, amv ? am : 0.0f, float(cb));
-  s.control[1] = make_float4(accepted ? 1.0f : 0.0f, accepted ? 0.0f : 1.0f, float(r.failures), float(hist));
-  s.control[2] = make_float4(sev ? r.e0 : 0.0f, cev ? r.e1 : 0.0f, aev ? ae : 0.0f, (sev && cev) ? 1.0f : 0.0f);
-  s.control[3] = make_float4(r.max_shift, float(r.max_bt), float(r.skipped), float(r.evals));
+  control[0] = make_float4(sgv ? r.minj0 : 0.0f, cgv ? r.minj1 : 0.0f, amv ? am : 0.0f, float(cb));
+  control[1] = make_float4(accepted ? 1.0f : 0.0f, accepted ? 0.0f : 1.0f, float(r.failures), float(hist));
+  control[2] = make_float4(sev ? r.e0 : 0.0f, cev ? r.e1 : 0.0f, aev ? ae : 0.0f, (sev && cev) ? 1.0f : 0.0f);
+  control[3] = make_float4(r.max_shift, float(r.max_bt), float(r.skipped), float(r.evals));
 }
 
 __global__ void __launch_bounds__(kBlock) k_apply_candidate(DeviceState s, PassParams p) {  // S:3958-3983
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
-  if (stop_latched(s, p)) return;
-  const bool accepted = s.control[1].x == 1.0f;
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || stop_latched(s, p, sc)) return;
+  const bool accepted = s.control[4 * size_t(sc) + 1].x == 1.0f;
   if (!accepted) {
     s.pos[p.tgt][v] = s.pos[p.src][v];
     s.final_update[v] = make_float4(0, 0, 0, 1);
@@ -1898,7 +1954,9 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
   ConvPartial r;
   conv_identity(r);
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  const bool latched = stop_latched(s, p);
+  const uint32_t sc = s.vblock_scene[blockIdx.x];
+  if (sc == kNoScene) return;  // uniform per block
+  const bool latched = stop_latched(s, p, sc);
   if (v < p.vertex_count) {
     float4* c = s.conv + 5 * size_t(v);
     f3 comp5[5];
@@ -1982,22 +2040,28 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
   if (threadIdx.x == 0) reinterpret_cast<ConvPartial*>(s.partials)[blockIdx.x] = r;
 }
 
-// Scale broadcast: combine scan-one partials into slot [blocks].
-__global__ void __launch_bounds__(kBlock) k_convergence_scales(DeviceState s, uint32_t blocks) {
+// Per-scene scales: combine the scene's scan-one partials (one block per
+// scene) into its scale record.
+__global__ void __launch_bounds__(kBlock) k_convergence_scales(DeviceState s) {
   __shared__ ConvPartial sh[kBlock];
+  const uint32_t sc = blockIdx.x;
+  if (s.scene_cnt[sc].z == 0u) return;
+  const uint4 b0 = s.scene_blk0[sc];
   ConvPartial r;
   conv_identity(r);
-  ConvPartial* parts = reinterpret_cast<ConvPartial*>(s.partials);
-  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) conv_max(r, parts[i]);
+  const ConvPartial* parts = reinterpret_cast<const ConvPartial*>(s.partials);
+  for (uint32_t i = b0.x + threadIdx.x; i < b0.y; i += kBlock) conv_max(r, parts[i]);
   block_reduce(r, sh, conv_max);
-  if (threadIdx.x == 0) parts[blocks] = r;
+  if (threadIdx.x == 0) reinterpret_cast<ConvPartial*>(s.scene_conv)[sc] = r;
 }
 
 // Scan two (S:4211-4283): normalized squared sums with the global scales.
 __global__ void __launch_bounds__(kBlock) k_convergence_sums(DeviceState s, PassParams p, uint32_t blocks) {
   __shared__ ConvPartial sh[kBlock];
   ConvPartial* parts = reinterpret_cast<ConvPartial*>(s.partials);
-  const ConvPartial sc = parts[blocks];
+  const uint32_t scene = s.vblock_scene[blockIdx.x];
+  if (scene == kNoScene) return;  // uniform per block
+  const ConvPartial sc = reinterpret_cast<const ConvPartial*>(s.scene_conv)[scene];
   ConvPartial r;
   conv_identity(r);
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
@@ -2040,19 +2104,26 @@ __global__ void __launch_bounds__(kBlock) k_convergence_sums(DeviceState s, Pass
     }
   }
   block_reduce(r, sh, conv_sum);
-  if (threadIdx.x == 0) parts[blocks + 1 + blockIdx.x] = r;
+  if (threadIdx.x == 0) parts[blocks + blockIdx.x] = r;
 }
 
 // Final pass of reduceConvergence (S:4320-4485): norms, history, stop latch.
+// One block per scene: its scan-two partials, norms, history and latch.
 __global__ void __launch_bounds__(kBlock) k_convergence_final(DeviceState s, PassParams p, uint32_t blocks) {
   __shared__ ConvPartial sh[kBlock];
+  const uint32_t scene = blockIdx.x;
+  if (s.scene_cnt[scene].z == 0u) return;
+  const uint4 b0 = s.scene_blk0[scene];
   ConvPartial* parts = reinterpret_cast<ConvPartial*>(s.partials);
-  const ConvPartial sc = parts[blocks];
+  const ConvPartial sc = reinterpret_cast<const ConvPartial*>(s.scene_conv)[scene];
   ConvPartial r;
   conv_identity(r);
-  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) conv_sum(r, parts[blocks + 1 + i]);
+  for (uint32_t i = b0.x + threadIdx.x; i < b0.y; i += kBlock) conv_sum(r, parts[blocks + i]);
   block_reduce(r, sh, conv_sum);
-  if (threadIdx.x != 0 || stop_latched(s, p)) return;
+  if (threadIdx.x != 0 || stop_latched(s, p, scene)) return;
+  float4* control = s.control + 4 * size_t(scene);
+  const float scene_scale = s.scene_scale[scene];
+  const float scene_vertices = float(s.scene_cnt[scene].x);
   float norms[5];
   for (int k = 0; k < 4; ++k) norms[k] = sc.ch[k] * sqrtf(r.ch[k]);
   norms[4] = sc.fin[0] * sqrtf(r.fin[0]);
@@ -2063,23 +2134,23 @@ __global__ void __launch_bounds__(kBlock) k_convergence_final(DeviceState s, Pas
   const float den = fmaxf(1.0f, norms[0] + norms[1] + norms[2] + norms[3] + norms[4]);
   const float rel = gnorm / den;
   const bool rnv = cnv && finite_s(gnorm) && gnorm >= 0.0f && finite_s(den) && den >= 1.0f && finite_s(rel) && rel >= 0.0f;
-  const bool ssv = finite_s(p.scene_scale) && p.scene_scale >= 0.0f;
-  const float nmu = sc.max_update / fmaxf(ssv ? p.scene_scale : 1.0e-12f, 1.0e-12f);
+  const bool ssv = finite_s(scene_scale) && scene_scale >= 0.0f;
+  const float nmu = sc.max_update / fmaxf(ssv ? scene_scale : 1.0e-12f, 1.0e-12f);
   const bool unv = raw_valid && ssv && finite_s(sc.max_update) && sc.max_update >= 0.0f && finite_s(nmu) && nmu >= 0.0f;
-  const float4 geo = s.control[0];
+  const float4 geo = control[0];
   const uint32_t cb = control_bits(geo.w) & kValidityMask;
   const bool sgv = (cb & kSourceGeometryValid) && finite_s(geo.x);
   const bool cgv = (cb & kCandidateGeometryValid) && finite_s(geo.y);
   const bool amv = (cb & kAcceptedMinimumValid) && finite_s(geo.z);
   const bool lnv = (cb & kLocalNumericsValid) != 0u;
-  const float4 acc = s.control[1];
+  const float4 acc = control[1];
   const bool aav = finite_nonneg_int(acc.x, 1.0f), arv = finite_nonneg_int(acc.y, 1.0f);
-  const bool lfv = finite_nonneg_int(acc.z, float(p.vertex_count));
+  const bool lfv = finite_nonneg_int(acc.z, scene_vertices);
   const bool hiv = finite_nonneg_int(acc.w, float(kHistoryCapacity));
   const bool aacc = aav && acc.x == 1.0f, arev = arv && acc.y == 1.0f;
   const uint32_t lf = lfv ? uint32_t(acc.z) : 0u, hi = hiv ? uint32_t(acc.w) : 0u;
-  const float4 cm = s.control[3];
-  const float trials_max = float(p.vertex_count) * float(kTrialCount);
+  const float4 cm = control[3];
+  const float trials_max = scene_vertices * float(kTrialCount);
   const bool cmv = finite_s(cm.x) && cm.x >= 0.0f && finite_nonneg_int(cm.y, float(kMaxBacktracks)) &&
                    finite_nonneg_int(cm.z, trials_max) && finite_nonneg_int(cm.w, trials_max);
   const bool fin = rnv && unv && sgv && cgv && amv && lnv && aav && arv && lfv && hiv && cmv;
@@ -2087,36 +2158,47 @@ __global__ void __launch_bounds__(kBlock) k_convergence_final(DeviceState s, Pas
   const bool us = fin && nmu <= p.update_tol;
   const bool converged = rs && us && aacc && !arev && lf == 0u;
   if (hiv && hi < kHistoryCapacity) {
-    float4* h = s.history + size_t(hi) * 8;
-    h[0] = s.control[0];
+    float4* h = s.history + (size_t(scene) * kHistoryCapacity + hi) * 8;
+    h[0] = control[0];
     h[1] = make_float4(acc.x, acc.y, acc.z, arev ? 1.0f : 0.0f);
-    h[2] = s.control[2];
+    h[2] = control[2];
     h[3] = make_float4(cnv ? norms[0] : 0.0f, cnv ? norms[1] : 0.0f, cnv ? norms[2] : 0.0f, cnv ? norms[3] : 0.0f);
     h[4] = make_float4(cnv ? norms[4] : 0.0f, rnv ? gnorm : 0.0f, rnv ? den : 0.0f, rnv ? rel : 0.0f);
     h[5] = make_float4(unv ? sc.max_update : 0.0f, unv ? nmu : 0.0f, rs ? 1.0f : 0.0f, us ? 1.0f : 0.0f);
     h[6] = make_float4(converged ? 1.0f : 0.0f, fin ? 1.0f : 0.0f, float(hi), cmv ? cm.x : 0.0f);
     h[7] = cmv ? make_float4(cm.y, cm.z, cm.w, 1.0f) : make_float4(0, 0, 0, 1);
   }
-  s.control[1].w = float(hi + 1u);
-  if (converged && p.stop_enabled) s.control[0].w = float(cb | kStopLatch);
+  control[1].w = float(hi + 1u);
+  if (converged && p.stop_enabled) control[0].w = float(cb | kStopLatch);
 }
 
 __global__ void __launch_bounds__(kBlock) k_copy_position(DeviceState s, PassParams p) {  // S:4489-4496
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
-  if (v >= p.vertex_count) return;
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
+  if (v >= p.vertex_count || vscene(s, v) == kNoScene) return;
   const float4 x = s.pos[p.src][v];
   s.pos[p.tgt][v] = make_float4(x.x, x.y, x.z, 1.0f);
 }
 
 // bodyHorizontalCorrection (S:4499-4532): one block per body reduces its
 // vertices in a fixed order.
+// Objectives and mesh contact keep their solved centre-of-mass motion
+// (jgs2-solver.ts:1994-2008): the correction is off for such a scene.
+// body_rng = {scene vertex begin, end, objective flags, scene}.
+__device__ __forceinline__ bool body_correction_off(const DeviceState& s, const PassParams& p, uint4 range) {
+  if (!p.globalization) return false;
+  if (range.z != 0u) return true;
+  const uint4 b1 = s.scene_blk1[range.w];
+  return p.contact != 0u && b1.z != b1.w;
+}
 __global__ void __launch_bounds__(kBlock) k_body_correction(DeviceState s, PassParams p) {
   __shared__ float sm[5][kBlock];
   __shared__ uint32_t sp[kBlock];
   const uint32_t body = blockIdx.x;
+  const uint4 range = s.body_rng[body];  // the owning scene's vertex range
+  if (range.w == kNoScene || body_correction_off(s, p, range)) return;  // uniform per block
   float tm = 0, sx = 0, sz = 0, px = 0, pz = 0;
   uint32_t pinned = 0;
-  for (uint32_t v = threadIdx.x; v < p.vertex_count; v += kBlock) {
+  for (uint32_t v = range.x + threadIdx.x; v < range.y; v += kBlock) {
     const uint4 inf = s.vtx_info[v];
     if (inf.w != body) continue;
     pinned |= inf.z != 0u ? 1u : 0u;
@@ -2152,10 +2234,11 @@ __global__ void __launch_bounds__(kBlock) k_body_correction(DeviceState s, PassP
   }
 }
 __global__ void __launch_bounds__(kBlock) k_apply_body_correction(DeviceState s, PassParams p) {  // S:4535-4551
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
-  if (v >= p.vertex_count) return;
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
+  if (v >= p.vertex_count || vscene(s, v) == kNoScene) return;
   const uint4 inf = s.vtx_info[v];
   if (inf.z != 0u || inf.w >= p.body_count) return;
+  if (body_correction_off(s, p, s.body_rng[inf.w])) return;
   const float4 c = s.body_corr[inf.w];
   float4 x = s.pos[0][v];
   x.x += c.x;
@@ -2164,8 +2247,8 @@ __global__ void __launch_bounds__(kBlock) k_apply_body_correction(DeviceState s,
 }
 
 __global__ void __launch_bounds__(kBlock) k_finalize(DeviceState s, PassParams p) {  // S:4554-4589
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
-  if (v >= p.vertex_count) return;
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
+  if (v >= p.vertex_count || vscene(s, v) == kNoScene) return;
   if (s.vtx_info[v].z != 0u) {
     if (!p.globalization) {
       const float4 r = s.vtx_rest[v];
@@ -2188,7 +2271,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(DeviceState s, PassParams p
 template <bool kCorot, bool kCloth, bool kIpc>
 __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, PassParams p) {
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= p.vertex_count) return;
+  if (v >= p.vertex_count || vscene(s, v) == kNoScene) return;
   float* g = s.debug_g + 3 * size_t(v);
   float* h = s.debug_h + 9 * size_t(v);
   if (s.vtx_info[v].z != 0u) {
@@ -2210,15 +2293,20 @@ __global__ void __launch_bounds__(kBlock) k_debug_local_systems(DeviceState s, P
 
 
 // ------------------------------------------------------------------ IPC passes (S:2695-2929)
-__device__ __forceinline__ float ipc_broad_threshold(const PassParams& p) {  // S:515-517
-  return p.ipc_dhat + ipc_rounding_reach(p);
+__device__ __forceinline__ float ipc_broad_threshold(const PassParams& p, float scene_scale) {  // S:515-517
+  return p.ipc_dhat + ipc_rounding_reach(p, scene_scale);
 }
 // lagContact (S:2713-2744): source contact cache of the frame-start pose
 // (posA) and the lagged normal force and weights for friction.
+__device__ __forceinline__ uint32_t cscene(const DeviceState& s, uint32_t c) { return s.cblock_scene[c / kSceneBlock]; }
 __global__ void __launch_bounds__(kBlock) k_lag_contact(DeviceState s, PassParams p) {
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= s.NC) return;
-  const Contact k = ipc_contact_at(s, c, s.pos[0], ipc_broad_threshold(p));
+  const uint32_t sc = cscene(s, c);
+  if (sc == kNoScene) return;
+  if (c == s.scene_blk1[sc].z * kSceneBlock) s.contact_ctl[sc].w = 1.0f;
+  if (c >= s.scene_end[sc].w) return;
+  const Contact k = ipc_contact_at(s, c, s.pos[0], ipc_broad_threshold(p, s.scene_scale[sc]));
   ipc_store_cached(s, c, false, k);
   float4 nf = make_float4(0, 0, 0, 0), w = make_float4(0, 0, 0, 0);
   if (k.valid && k.distance > s.ipc_dmin) {
@@ -2231,16 +2319,18 @@ __global__ void __launch_bounds__(kBlock) k_lag_contact(DeviceState s, PassParam
   s.lag_n[c] = nf;
   s.lag_w[c] = w;
   s.cscratch[c] = make_float4(1, 0, 0, 0);
-  if (c == 0) s.contact_ctl[0].w = 1.0f;
 }
 // buildActiveContactRows (S:2747-2785): keep candidates the bounded step can
 // bring into the barrier, with friction, or touched by unbounded pin motion.
 __global__ void __launch_bounds__(kBlock) k_build_active_rows(DeviceState s, PassParams p) {
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
-  const uint32_t first = s.cand_rows[v], last = s.cand_rows[v + 1];
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene) return;
+  const uint2 row = s.cand_row[v];
+  const uint32_t first = row.x, last = row.x + row.y;
   const bool bounded = p.max_step > 0.0f;
-  const float reach = p.ipc_dhat + 2.0f * (fmaxf(p.max_step, 0.0f) + ipc_rounding_reach(p));
+  const float reach = p.ipc_dhat + 2.0f * (fmaxf(p.max_step, 0.0f) + ipc_rounding_reach(p, s.scene_scale[sc]));
   const float bound = fmaxf(p.max_step, 0.0f);
   const float4* src = s.pos[p.src];
   uint32_t n = 0;
@@ -2266,7 +2356,9 @@ __global__ void __launch_bounds__(kBlock) k_build_active_rows(DeviceState s, Pas
 __global__ void __launch_bounds__(kBlock) k_candidate_contact_step(DeviceState s, PassParams p) {
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= s.NC) return;
-  if (stop_latched(s, p)) {
+  const uint32_t sc = cscene(s, c);
+  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
+  if (stop_latched(s, p, sc)) {
     s.cscratch[c] = make_float4(1, 0, 0, 1);
     return;
   }
@@ -2276,7 +2368,7 @@ __global__ void __launch_bounds__(kBlock) k_candidate_contact_step(DeviceState s
   const f3 d0 = load_pos(tgt, ix.x) - load_pos(src, ix.x), d1 = load_pos(tgt, ix.y) - load_pos(src, ix.y);
   const f3 d2 = load_pos(tgt, ix.z) - load_pos(src, ix.z), d3 = load_pos(tgt, ix.w) - load_pos(src, ix.w);
   const float l0 = sqrtf(dot(d0, d0)), l1 = sqrtf(dot(d1, d1)), l2 = sqrtf(dot(d2, d2)), l3 = sqrtf(dot(d3, d3));
-  const float lip = c < s.NVT ? l0 + fmaxf(l1, fmaxf(l2, l3)) : fmaxf(l0, l1) + fmaxf(l2, l3);
+  const float lip = s.cand_meta[c] == 0u ? l0 + fmaxf(l1, fmaxf(l2, l3)) : fmaxf(l0, l1) + fmaxf(l2, l3);
   Contact src_k = ipc_cached(s, c, false);
   bool full = src_k.valid && finite_s(lip) && src_k.distance - lip > s.ipc_dmin;
   if (!full) {
@@ -2294,7 +2386,9 @@ __global__ void __launch_bounds__(kBlock) k_candidate_contact_step(DeviceState s
 __global__ void __launch_bounds__(kBlock) k_validate_contact_step(DeviceState s, PassParams p) {
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= s.NC) return;
-  const Contact k = ipc_contact_at(s, c, s.pos[p.tgt], ipc_broad_threshold(p));
+  const uint32_t sc = cscene(s, c);
+  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
+  const Contact k = ipc_contact_at(s, c, s.pos[p.tgt], ipc_broad_threshold(p, s.scene_scale[sc]));
   ipc_store_cached(s, c, true, k);
   const bool valid = k.valid && finite_s(k.distance) && k.distance > s.ipc_dmin;
   const float d = valid ? k.distance : 0.0f;
@@ -2307,30 +2401,39 @@ __device__ __forceinline__ void contact_min(float4& a, const float4& b) {
 }
 __global__ void __launch_bounds__(kBlock) k_reduce_contact_partials(DeviceState s) {
   __shared__ float4 sh[kBlock];
+  const uint32_t sc = s.cblock_scene[blockIdx.x];
+  if (sc == kNoScene) return;  // uniform per block
   float4 r = make_float4(1.0f, kF32Max, kF32Max, 1.0f);
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-  if (c < s.NC) {
-    const float4 sc = s.cscratch[c];
-    const bool valid = sc.w == 1.0f && finite_s(sc.x) && finite_s(sc.y) && finite_s(sc.z);
-    r = make_float4(valid ? sc.x : 0.0f, valid ? sc.y : 0.0f, valid ? sc.z : 0.0f, valid ? 1.0f : 0.0f);
-    r.x = fminf(1.0f, r.x);
+  if (c < s.scene_end[sc].w) {
+    const float4 q = s.cscratch[c];
+    const bool valid = q.w == 1.0f && finite_s(q.x) && finite_s(q.y) && finite_s(q.z);
+    r = make_float4(fminf(1.0f, valid ? q.x : 0.0f), valid ? q.y : 0.0f, valid ? q.z : 0.0f, valid ? 1.0f : 0.0f);
   }
   block_reduce(r, sh, contact_min);
   if (threadIdx.x == 0) reinterpret_cast<float4*>(s.partials)[blockIdx.x] = r;
 }
-__global__ void __launch_bounds__(kBlock) k_reduce_contact_final(DeviceState s, uint32_t blocks) {
+// One block per scene: the scene's safe alpha (zero if any record failed).
+__global__ void __launch_bounds__(kBlock) k_reduce_contact_final(DeviceState s) {
   __shared__ float4 sh[kBlock];
+  const uint32_t sc = blockIdx.x;
+  const uint4 b1 = s.scene_blk1[sc];
+  if (s.scene_cnt[sc].z == 0u || b1.z == b1.w) return;
   float4 r = make_float4(1.0f, kF32Max, kF32Max, 1.0f);
   const float4* parts = reinterpret_cast<const float4*>(s.partials);
-  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) contact_min(r, parts[i]);
+  for (uint32_t i = b1.z + threadIdx.x; i < b1.w; i += kBlock) contact_min(r, parts[i]);
   block_reduce(r, sh, contact_min);
-  if (threadIdx.x == 0) s.contact_ctl[0].w = r.w == 1.0f ? fminf(fmaxf(r.x, 0.0f), 1.0f) : 0.0f;
+  if (threadIdx.x == 0) s.contact_ctl[sc].w = r.w == 1.0f ? fminf(fmaxf(r.x, 0.0f), 1.0f) : 0.0f;
 }
 // applyContactStep (S:2921-2929).
 __global__ void __launch_bounds__(kBlock) k_apply_contact_step(DeviceState s, PassParams p) {
-  const uint32_t v = logical_block() * kBlock + threadIdx.x;
-  if (v >= p.vertex_count || stop_latched(s, p)) return;
-  const float a = fminf(fmaxf(s.contact_ctl[0].w, 0.0f), 1.0f);
+  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
+  if (v >= p.vertex_count) return;
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || stop_latched(s, p, sc)) return;
+  const uint4 b1 = s.scene_blk1[sc];
+  if (b1.z == b1.w) return;  // a scene without candidates takes no contact step
+  const float a = fminf(fmaxf(s.contact_ctl[sc].w, 0.0f), 1.0f);
   const f3 src = load_pos(s.pos[p.src], v), c = load_pos(s.pos[p.tgt], v);
   const f3 x = src + a * (c - src);
   s.pos[p.tgt][v] = make_float4(x.x, x.y, x.z, 1.0f);
@@ -2339,8 +2442,10 @@ __global__ void __launch_bounds__(kBlock) k_apply_contact_step(DeviceState s, Pa
 __global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassParams p) {
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= s.NC) return;
-  const float alpha = s.contact_ctl[0].w;
-  if (s.control[1].x == 1.0f && finite_s(alpha) && alpha > 0.0f) {
+  const uint32_t sc = cscene(s, c);
+  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
+  const float alpha = s.contact_ctl[sc].w;
+  if (s.control[4 * size_t(sc) + 1].x == 1.0f && finite_s(alpha) && alpha > 0.0f) {
     s.src_c[c] = s.tgt_c[c];
     s.src_w[c] = s.tgt_w[c];
   }
@@ -2351,21 +2456,22 @@ inline bool has_cloth(const PassParams& p) { return p.tri_count != 0u || p.hinge
 
 }  // namespace
 
-uint32_t partial_blocks_needed(uint32_t V, uint32_t elements, uint32_t candidates) {
-  // Candidate partials use blocks(T + triangles) + blocks(V); convergence
-  // uses 2 * blocks(V) + 1; the contact step blocks(candidates).
-  uint32_t a = blocks_for(elements) + blocks_for(V);
-  uint32_t b = 2 * blocks_for(V) + 1;
-  uint32_t c = blocks_for(candidates);
+uint32_t partial_blocks_needed(uint32_t vertices, uint32_t tets, uint32_t triangles, uint32_t candidates) {
+  // Candidate partials use blocks(T) + blocks(triangles) + blocks(V);
+  // convergence 2 blocks(V); the contact step blocks(candidates).
+  const uint32_t a = blocks_for(tets) + blocks_for(triangles) + blocks_for(vertices);
+  const uint32_t b = 2 * blocks_for(vertices);
+  const uint32_t c = blocks_for(candidates);
   return a > b ? (a > c ? a : c) : (b > c ? b : c);
 }
 size_t partial_bytes(uint32_t blocks) {
   size_t a = sizeof(CandPartial), b = sizeof(ConvPartial);
   return size_t(blocks) * (a > b ? a : b);
 }
+size_t scene_conv_bytes(uint32_t scenes) { return size_t(scenes) * sizeof(ConvPartial); }
 
-void launch_clear_frame(const DeviceState& s, const PassParams&, hipStream_t st) {
-  hipLaunchKernelGGL(k_clear_frame, dim3(1), dim3(kBlock), 0, st, s);
+void launch_clear_frame(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_frame, dim3(blocks_for(4 * p.scene_count)), dim3(kBlock), 0, st, s, p);
 }
 void launch_predict(const DeviceState& s, const PassParams& p, hipStream_t st) {
   hipLaunchKernelGGL(k_predict, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
@@ -2388,11 +2494,24 @@ void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t 
     else hipLaunchKernelGGL((KERNEL<false, false, false>), GRID, dim3(kBlock), 0, st, __VA_ARGS__); \
   } while (0)
 
+#define JGS2_FEATURE_DISPATCH_BLOCK(KERNEL, GRID, BLOCK, ...)                                        \
+  do {                                                                                               \
+    const bool cl = has_cloth(p), ipc = p.contact != 0u;                                            \
+    if (p.corotated) hipLaunchKernelGGL((KERNEL<true, false, false>), GRID, BLOCK, 0, st, __VA_ARGS__); \
+    else if (cl && ipc) hipLaunchKernelGGL((KERNEL<false, true, true>), GRID, BLOCK, 0, st, __VA_ARGS__); \
+    else if (cl) hipLaunchKernelGGL((KERNEL<false, true, false>), GRID, BLOCK, 0, st, __VA_ARGS__); \
+    else if (ipc) hipLaunchKernelGGL((KERNEL<false, false, true>), GRID, BLOCK, 0, st, __VA_ARGS__); \
+    else hipLaunchKernelGGL((KERNEL<false, false, false>), GRID, BLOCK, 0, st, __VA_ARGS__); \
+  } while (0)
+
+// The local solve runs with the autotuned block size (64, 128 or 256).
 void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t n = p.colored ? p.color_end - p.color_begin : p.vertex_count;
   if (n == 0) return;
-  JGS2_FEATURE_DISPATCH(k_solve, dim3(blocks_for(n)), s, p);
+  const uint32_t block = (p.solve_block == 64u || p.solve_block == 128u) ? p.solve_block : kBlock;
+  JGS2_FEATURE_DISPATCH_BLOCK(k_solve, dim3((n + block - 1) / block), dim3(block), s, p);
 }
+void launch_solve_config(const DeviceState& s, const PassParams& p, hipStream_t st) { launch_solve(s, p, st); }
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const dim3 grid(blocks_for(p.vertex_count));
   const bool cl = has_cloth(p), ipc = p.contact != 0u;
@@ -2402,9 +2521,14 @@ void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t
   else hipLaunchKernelGGL((k_vertex_energy<false, false>), grid, dim3(kBlock), 0, st, s, p);
 }
 void launch_reduce_candidate(const DeviceState& s, const PassParams& p, hipStream_t st) {
-  const uint32_t tb = blocks_for(p.tet_count + p.tri_count), vb = blocks_for(p.vertex_count);
-  hipLaunchKernelGGL(k_candidate_partials, dim3(tb + vb), dim3(kBlock), 0, st, s, p, tb);
-  hipLaunchKernelGGL(k_candidate_final, dim3(1), dim3(kBlock), 0, st, s, p, tb + vb);
+  const uint32_t tb = blocks_for(p.tet_count), rb = blocks_for(p.tri_count), vb = blocks_for(p.vertex_count);
+  hipLaunchKernelGGL(k_candidate_partials, dim3(tb + rb + vb), dim3(kBlock), 0, st, s, p, tb, rb);
+  hipLaunchKernelGGL(k_candidate_final, dim3(p.scene_count), dim3(kBlock), 0, st, s, p, tb, rb, 0u);
+}
+void launch_reduce_candidate_scene(const DeviceState& s, const PassParams& p, uint32_t scene, hipStream_t st) {
+  const uint32_t tb = blocks_for(p.tet_count), rb = blocks_for(p.tri_count), vb = blocks_for(p.vertex_count);
+  hipLaunchKernelGGL(k_candidate_partials, dim3(tb + rb + vb), dim3(kBlock), 0, st, s, p, tb, rb);
+  hipLaunchKernelGGL(k_candidate_final, dim3(1), dim3(kBlock), 0, st, s, p, tb, rb, scene);
 }
 void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream_t st) {
   hipLaunchKernelGGL(k_apply_candidate, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
@@ -2416,9 +2540,9 @@ void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t s
   else if (cl) hipLaunchKernelGGL((k_convergence_gradient<true, false>), dim3(vb), dim3(kBlock), 0, st, s, p);
   else if (ipc) hipLaunchKernelGGL((k_convergence_gradient<false, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
   else hipLaunchKernelGGL((k_convergence_gradient<false, false>), dim3(vb), dim3(kBlock), 0, st, s, p);
-  hipLaunchKernelGGL(k_convergence_scales, dim3(1), dim3(kBlock), 0, st, s, vb);
+  hipLaunchKernelGGL(k_convergence_scales, dim3(p.scene_count), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_convergence_sums, dim3(vb), dim3(kBlock), 0, st, s, p, vb);
-  hipLaunchKernelGGL(k_convergence_final, dim3(1), dim3(kBlock), 0, st, s, p, vb);
+  hipLaunchKernelGGL(k_convergence_final, dim3(p.scene_count), dim3(kBlock), 0, st, s, p, vb);
 }
 void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_position, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
@@ -2448,11 +2572,11 @@ void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t 
   if (s.NC == 0) return;
   hipLaunchKernelGGL(k_candidate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
   hipLaunchKernelGGL(k_reduce_contact_partials, dim3(cb), dim3(kBlock), 0, st, s);
-  hipLaunchKernelGGL(k_reduce_contact_final, dim3(1), dim3(kBlock), 0, st, s, cb);
+  hipLaunchKernelGGL(k_reduce_contact_final, dim3(p.scene_count), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p);
   hipLaunchKernelGGL(k_validate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
   hipLaunchKernelGGL(k_reduce_contact_partials, dim3(cb), dim3(kBlock), 0, st, s);
-  hipLaunchKernelGGL(k_reduce_contact_final, dim3(1), dim3(kBlock), 0, st, s, cb);
+  hipLaunchKernelGGL(k_reduce_contact_final, dim3(p.scene_count), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p);
 }
 void launch_promote_contact(const DeviceState& s, const PassParams& p, hipStream_t st) {

@@ -15,6 +15,7 @@ from ._native import (  # noqa: F401
     LIBRARY_PATH,
 )
 from .solver import DEFAULT_STEP_SETTINGS, ClothData, GpuInput, JGS2GpuSolver  # noqa: F401
+from .batch import BatchCapacity, JGS2Batch  # noqa: F401
 
 
 def device_count() -> int:

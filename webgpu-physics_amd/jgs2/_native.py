@@ -177,6 +177,37 @@ class TroughDesc(C.Structure):
 
 # Every exported entry point with its ctypes signature. Tests compare this
 # table with the declarations in include/*.h.
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("max_vertices", C.c_uint64),
+        ("max_tets", C.c_uint64),
+        ("max_triangles", C.c_uint64),
+        ("max_hinges", C.c_uint64),
+        ("max_candidates", C.c_uint64),
+        ("max_bodies", C.c_uint64),
+        ("max_scenes", C.c_uint32),
+        ("cubature_k", C.c_uint32),
+        ("corotated", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("scene_count", C.c_uint32),
+        ("max_scenes", C.c_uint32),
+        ("solve_block", C.c_uint32),
+        ("xcd_remap", C.c_uint32),
+        ("use_graphs", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("vertex_capacity", C.c_uint64),
+        ("vertex_blocks_used", C.c_uint64),
+        ("vertex_extent", C.c_uint64),
+        ("tet_extent", C.c_uint64),
+        ("arena_bytes", C.c_uint64),
+    ]
+
+
 SIGNATURES = {
     "jgs2_default_settings": (None, [C.POINTER(Settings)]),
     "jgs2_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
@@ -211,6 +242,24 @@ SIGNATURES = {
     "jgs2_profile_begin": (C.c_int, [C.c_void_p]),
     "jgs2_profile_end": (
         C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "jgs2_batch_arena_bytes": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(C.c_uint64)]),
+    "jgs2_batch_create": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(Settings), C.c_uint64, C.POINTER(C.c_void_p)]),
+    "jgs2_batch_destroy": (C.c_int, [C.c_void_p]),
+    "jgs2_batch_admit": (C.c_int, [C.c_void_p, C.POINTER(SceneInput), C.POINTER(C.c_void_p)]),
+    "jgs2_batch_retire": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "jgs2_batch_step_frames": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Settings), C.c_uint64]),
+    "jgs2_batch_step_frames_exact_iterations": (
+        C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(Settings), C.c_uint64]),
+    "jgs2_batch_sync": (C.c_int, [C.c_void_p]),
+    "jgs2_batch_get_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
+    "jgs2_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "jgs2_batch_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "jgs2_batch_autotune": (
+        C.c_int, [C.c_void_p, C.POINTER(Settings), C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+    "jgs2_batch_profile_begin": (C.c_int, [C.c_void_p]),
+    "jgs2_batch_profile_end": (
+        C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "jgs2_solver_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "jgs2_scene_build_cuboid": (C.c_int, [C.POINTER(CuboidDesc), C.POINTER(C.c_void_p)]),
     "jgs2_scene_build_cloth_drape": (C.c_int, [C.POINTER(ClothDesc), C.POINTER(C.c_void_p)]),
     "jgs2_scene_build_trough": (C.c_int, [C.POINTER(TroughDesc), C.POINTER(C.c_void_p)]),

@@ -25,10 +25,15 @@ CXX = os.environ.get("CXX", "g++")
 OFFLOAD_ARCH = "gfx950"
 
 # Packed FP32 instructions are disabled for all device code; the compiler
-# otherwise forms v_pk_*_f32 from float2-shaped code.
+# otherwise forms v_pk_*_f32 from float2-shaped code. Floating-point
+# contraction is limited to single source expressions (-ffp-contract=on):
+# every kernel variant (plain, cloth, contact) then rounds a shared
+# expression the same way, so a scene's results do not depend on which
+# other scenes share its batch.
 DEVICE_FLAGS = [
     f"--offload-arch={OFFLOAD_ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=on",
 ]
 COMMON_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", f"-I{REPO_ROOT / 'include'}", f"-I{CSRC}"]
 
