@@ -1,25 +1,35 @@
 """Benchmark of the JGS2 step on one MI355X.
 
-Workload (default): the single large mesh -- a 1 m x 1 m x 2 m box on a
-64x64x128 hex grid (545,025 vertices, 3,145,728 tetrahedra), seeded vertex
-jitter, alternating soft/stiff layers, stable Neo-Hookean, K = 6 cubature
-records per vertex, 8 Jacobi sweeps per frame. One "step" is one frame.
+Headline workload (`value`): the batched uneven-scene stream -- 512 seeded
+independent scenes (70 % stable Neo-Hookean beams and boxes of 200 to
+200,000 tetrahedra, log-uniform, E log-uniform in [1e4, 1e7] Pa; 30 % StVK
+cloth sheets of 32x32 to 256x256 cells with dihedral bending), K = 6
+cubature records, 8 exact Jacobi sweeps per frame, all resident in one
+batch arena and stepped one launch per pass, frames replayed from a
+hipGraph. One "step" is one frame of every scene. `value` is free-vertex
+updates per second (one local solve of one free vertex in one sweep).
+The same scenes stepped one solver per scene (the plain path) give
+`plain`; `batched_over_plain` is the ratio.
 
-Prints ONE JSON line. `value` is free-vertex updates per second (one local
-solve of one free vertex in one sweep), all inputs resident in HBM.
-`roofline` reports the local-solve kernel: reference-ABI algorithmic bytes
-per launch over its mean launch time measured with HIP events on the
-solver's stream. `cpu_baseline` times the oracle (a single-threaded CPU
-restatement of the same step) on a bounded sample. `accuracy` is the
-trajectory error of the GPU against the FP64 oracle on a reduced grid of the
-same generator.
+Further legs, each in its own object:
+* `serving`: Poisson arrivals of scene-step requests (contact troughs, 1 to
+  120 frames each) served by continuous batching under an HBM budget, and
+  by one solver per request; throughput, p50/p99 frame latency, request
+  latency and queueing.
+* `single_large_mesh`: the 64x64x128 layered box (545,025 vertices,
+  3,145,728 tetrahedra) under Jacobi and graph-colored Gauss-Seidel.
+
+`roofline` reports the local-solve kernel of the headline batch:
+reference-ABI algorithmic bytes per launch over its mean launch time,
+measured with HIP events on the batch's stream. `accuracy` compares GPU
+trajectories of a reduced mix with the FP64 oracle. `cpu_baseline` times
+the oracle's FP32 step on a bounded sample of the same generator.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -33,17 +43,20 @@ METRIC = "JGS2 vertex-updates/s batched vs plain on 1 MI355X at tol; p50/p99 fra
 HBM_PEAK_GBS = 8000.0
 
 
-def nearest_rank(values, q):
-    """Nearest-rank percentile (src/performance/metrics.ts:42-46)."""
-    s = sorted(values)
-    if not s:
-        return 0.0
-    rank = max(1, int(np.ceil(q / 100.0 * len(s))))
-    return float(s[min(rank, len(s)) - 1])
-
-
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def scene_bytes_per_sweep(scene) -> int:
+    """Reference-ABI compulsory bytes of one sweep over one scene: the
+    tetrahedral terms plus cloth records (triangle 4 vec4 + 3 incidence ids,
+    hinge 2 vec4 + 4 incidence ids)."""
+    from jgs2 import scenes
+
+    b = scenes.algorithmic_bytes_per_sweep(scene.vertex_count, scene.tet_count, scene.cubature_k)
+    if scene.cloth is not None:
+        b += scene.cloth.triangle_count * (64 + 12) + scene.cloth.hinge_count * (32 + 16)
+    return b
 
 
 def main(argv=None):
@@ -51,80 +64,84 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--schedule", default="jacobi", choices=["jacobi", "graph-colored-gauss-seidel"])
-    ap.add_argument("--cells", default="64,64,128")
+    ap.add_argument("--scenes", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=20251203)
     ap.add_argument("--iterations", type=int, default=8)
     ap.add_argument("--latency-frames", type=int, default=10)
-    ap.add_argument("--other-schedule-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--large-steps", type=int, default=5)
+    ap.add_argument("--large-cells", default="64,64,128")
+    ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the step is single-GPU here).",
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every leg runs on one GPU).",
               file=sys.stderr)
         return 2
 
     import jgs2
     from jgs2 import scenes
+    from jgs2.batch import JGS2Batch
+    from jgs2.serving import nearest_rank
 
     if jgs2.device_count() < 1:
         print("No HIP device visible; bench.py needs an MI355X.", file=sys.stderr)
         return 1
 
-    cells = tuple(int(c) for c in args.cells.split(","))
-    t0 = time.time()
-    scene = scenes.layered_box(cells=cells)
-    settings = dict(scenes.LAYERED_BOX_SETTINGS, schedule=args.schedule)
-    log(f"scene {cells}: V={scene.vertex_count} T={scene.tet_count} built in {time.time() - t0:.1f}s")
-    t0 = time.time()
-    solver = jgs2.JGS2GpuSolver.create(scene, settings)
-    log(f"solver created in {time.time() - t0:.1f}s; colors={solver.schedule_color_count}")
     it = args.iterations
-    free = solver.free_vertex_count
+    settings = dict(scenes.MIXED_SETTINGS)
+    t0 = time.time()
+    specs = scenes.mixed_scene_specs(count=args.scenes, seed=args.seed)
+    inputs = [scenes.build_mixed_scene(s) for s in specs]
+    V = sum(x.vertex_count for x in inputs)
+    T = sum(x.tet_count for x in inputs)
+    NT = sum(x.cloth.triangle_count for x in inputs if x.cloth is not None)
+    log(f"{len(inputs)} scenes: V={V} T={T} triangles={NT} built in {time.time() - t0:.1f}s")
 
+    # ---------------------------------------------------------- batched
+    t0 = time.time()
+    batch = JGS2Batch.create(_capacity(inputs), settings)
+    members = [batch.admit(x) for x in inputs]
+    free = sum(m.free_vertex_count for m in members)
+    batch.use_graphs(not args.no_graphs)
+    info = batch.info()
+    log(f"batch: {info['scene_count']} scenes admitted in {time.time() - t0:.1f}s, arena {info['arena_bytes'] / 1e9:.2f} GB")
+    autotune_ms = batch.autotune(dict(settings), repeats=5)
+    info = batch.info()
+    log(f"autotune: block {info['solve_block']} xcd_remap {info['xcd_remap']} ({autotune_ms:.3f} ms/launch)")
     for w in range(args.warmup):
-        solver.step_exact_iterations(it, settings)
-        solver.await_idle()
+        batch.step_frames_exact_iterations(1, it, settings)
+        batch.sync()
         log(f"warmup frame {w + 1}/{args.warmup} done")
-
-    solver.await_idle()
+    batch.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        solver.step_exact_iterations(it, settings)
-    solver.await_idle()
+        batch.step_frames_exact_iterations(1, it, settings)
+    batch.sync()
     elapsed = time.perf_counter() - t0
-    log(f"timed {args.steps} frames in {elapsed:.3f}s")
-    updates = args.steps * it * free
-    value = updates / elapsed
-
-    # Per-frame latency as a caller sees it (enqueue + completion).
+    value = args.steps * it * free / elapsed
+    log(f"batched: {args.steps} frames in {elapsed:.3f}s, {value:.4g} vertex-updates/s")
     lat = []
     for _ in range(args.latency_frames):
         a = time.perf_counter()
-        solver.step_exact_iterations(it, settings)
-        solver.await_idle()
+        batch.step_frames_exact_iterations(1, it, settings)
+        batch.sync()
         lat.append((time.perf_counter() - a) * 1e3)
-    log(f"latency frames done: p50 {nearest_rank(lat, 50):.2f} ms")
-
-    # Local-solve kernel time per launch, HIP events on the solver's stream.
-    solver.profile_begin()
-    solver.step_exact_iterations(it, settings)
-    solve_ms, launches, window_ms = solver.profile_end()
-    bytes_per_sweep = scenes.algorithmic_bytes_per_sweep(scene.vertex_count, scene.tet_count, scene.cubature_k)
+    log(f"batched frame latency p50 {nearest_rank(lat, 50):.2f} ms")
+    batch.profile_begin()
+    batch.step_frames_exact_iterations(1, it, settings)
+    solve_ms, launches, _ = batch.profile_end()
+    bytes_per_sweep = sum(scene_bytes_per_sweep(x) for x in inputs)
     achieved = bytes_per_sweep / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
-    x = solver.read_positions().reshape(-1, 4)[:, :3]
-    finite = bool(np.isfinite(x).all())
-    hist = solver.read_globalization_history().reshape(64, 8, 4)
-    dynamics = {"reverted_iterations_last_frame": int(hist[:it, 1, 3].sum()),
-                "max_backtracks_last_frame": int(hist[:it, 7, 0].max()),
-                "relative_residual_last_iteration": float(hist[it - 1, 4, 3])}
-    solver.destroy()
+    finite = all(bool(np.isfinite(m.read_positions()).all()) for m in members)
+    reverted = sum(int(m.read_globalization_history().reshape(64, 8, 4)[:it, 1, 3].sum()) for m in members)
+    batch.destroy()
 
-    other = None
-    if args.other_schedule_steps > 0:
-        other_name = "graph-colored-gauss-seidel" if args.schedule == "jacobi" else "jacobi"
-        other = measure_schedule(scene, dict(settings, schedule=other_name), it, args.other_schedule_steps)
+    # ---------------------------------------------------------- plain
+    plain = measure_plain(inputs, settings, it, args.plain_steps)
 
     result = {
         "metric": METRIC,
@@ -140,16 +157,25 @@ def main(argv=None):
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"layered-box-{cells[0]}x{cells[1]}x{cells[2]}",
-            "vertices": scene.vertex_count,
+            "workload": f"batched-uneven-scenes-{len(inputs)}",
+            "scenes": len(inputs),
+            "tet_scenes": sum(1 for s in specs if s["kind"] == "tet"),
+            "cloth_scenes": sum(1 for s in specs if s["kind"] == "cloth"),
+            "vertices": V,
             "free_vertices": free,
-            "tetrahedra": scene.tet_count,
-            "cubature_k": scene.cubature_k,
-            "schedule": args.schedule,
+            "tetrahedra": T,
+            "cloth_triangles": NT,
+            "cubature_k": max(x.cubature_k for x in inputs),
+            "schedule": "jacobi",
             "iterations_per_frame": it,
-            "mode": "plain (one scene per launch)",
+            "seed": args.seed,
+            "mode": "batched (one launch per pass over all scenes)" + ("" if args.no_graphs else ", hipGraph frames"),
+            "solve_block": info["solve_block"],
+            "xcd_remap": bool(info["xcd_remap"]),
         },
         "frame_ms": {"p50": nearest_rank(lat, 50), "p99": nearest_rank(lat, 99), "frames": len(lat)},
+        "plain": plain,
+        "batched_over_plain": value / plain["vertex_updates_per_s"] if plain["vertex_updates_per_s"] > 0 else None,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_solve (local solve)",
@@ -163,85 +189,209 @@ def main(argv=None):
             "launches_profiled": launches,
         },
         "positions_finite": finite,
-        "dynamics": dynamics,
+        "reverted_iterations_last_frame_all_scenes": reverted,
     }
-    if other is not None:
-        result["other_schedule"] = other
+    del inputs, members
 
+    if args.serve_seconds > 0:
+        result["serving"] = measure_serving(args.serve_seconds, args.seed)
+    if args.large_steps > 0:
+        cells = tuple(int(c) for c in args.large_cells.split(","))
+        result["single_large_mesh"] = measure_large(cells, it, args.large_steps)
     if not args.no_accuracy:
-        result["accuracy"] = accuracy_check(settings, it)
+        result["accuracy"] = accuracy_check(it, args.seed)
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(settings, it)
+        result["cpu_baseline"] = cpu_baseline(it, args.seed)
     print(json.dumps(result), flush=True)
     return 0
 
 
-def measure_schedule(scene, settings, iterations, steps):
-    """Throughput of the second schedule on the same scene (fresh solver)."""
+def _capacity(inputs, slack_scenes=0):
+    from jgs2.batch import BatchCapacity, footprint
+
+    fp = [footprint(x) for x in inputs]
+    total = {k: sum(f[k] for f in fp) for k in fp[0]}
+    return BatchCapacity(max_vertices=total["vertices"], max_tets=total["tets"], max_scenes=len(inputs) + slack_scenes,
+                         cubature_k=max(x.cubature_k for x in inputs), max_triangles=total["triangles"],
+                         max_hinges=total["hinges"], max_candidates=total["candidates"], max_bodies=total["bodies"])
+
+
+def measure_plain(inputs, settings, iterations, steps):
+    """The same scenes, one solver each, every solver stepped by its own
+    submission per frame."""
     import jgs2
 
-    solver = jgs2.JGS2GpuSolver.create(scene, settings)
-    solver.step_exact_iterations(iterations, settings)
-    solver.await_idle()
+    t0 = time.time()
+    solvers = [jgs2.JGS2GpuSolver.create(x, settings) for x in inputs]
+    free = sum(s.free_vertex_count for s in solvers)
+    log(f"plain: {len(solvers)} solvers created in {time.time() - t0:.1f}s")
+    for s in solvers:
+        s.step_exact_iterations(iterations, settings)
+    for s in solvers:
+        s.await_idle()
     t0 = time.perf_counter()
     for _ in range(steps):
-        solver.step_exact_iterations(iterations, settings)
-    solver.await_idle()
+        for s in solvers:
+            s.step_exact_iterations(iterations, settings)
+    for s in solvers:
+        s.await_idle()
     dt = time.perf_counter() - t0
-    hist = solver.read_globalization_history().reshape(64, 8, 4)
-    out = {"schedule": settings["schedule"], "colors": solver.schedule_color_count, "steps": steps,
-           "ms_per_step": dt / steps * 1e3,
-           "vertex_updates_per_s": steps * iterations * solver.free_vertex_count / dt,
-           "reverted_iterations_last_frame": int(hist[:iterations, 1, 3].sum())}
-    solver.destroy()
-    log(f"{settings['schedule']}: {out['ms_per_step']:.2f} ms/frame")
+    for s in solvers:
+        s.destroy()
+    out = {"mode": "plain (one solver and one submission per scene)", "steps": steps, "ms_per_step": dt / steps * 1e3,
+           "vertex_updates_per_s": steps * iterations * free / dt}
+    log(f"plain: {out['ms_per_step']:.1f} ms/frame, {out['vertex_updates_per_s']:.4g} vertex-updates/s")
     return out
 
 
-def _small_scene():
+def measure_serving(seconds, seed):
+    """Poisson arrivals (2000 requests/s) of contact-trough requests asking
+    for 1 to 120 frames, served for `seconds` by continuous batching under an
+    HBM budget and by one solver per request under the same budget."""
+    from jgs2 import scenes
+    from jgs2.serving import BatchServer, PlainServer, Request, capacity_for_budget, poisson_arrivals
+
+    rate, budget, max_scenes = 2000.0, 8 << 30, 512
+    pool = [scenes.trough(box_count=10, seed=s) for s in range(1, 17)]
+    settings = dict(scenes.TROUGH_SETTINGS)
+    it = scenes.TROUGH_ITERATIONS
+    rng = np.random.default_rng(seed)
+    arrivals = poisson_arrivals(rate, seconds, seed)
+    frames = rng.integers(1, 121, size=arrivals.size)
+    picks = rng.integers(0, len(pool), size=arrivals.size)
+
+    def stream():
+        return [Request(id=i, scene=pool[picks[i]], frames=int(frames[i]), arrival=float(arrivals[i]))
+                for i in range(arrivals.size)]
+
+    cap = capacity_for_budget(pool, budget, max_scenes=max_scenes)
+    out = {"workload": f"Poisson {rate:g} requests/s of 10-box contact troughs, 1-120 frames each, "
+                       f"{it} exact Jacobi sweeps per frame, {seconds:g} s",
+           "offered_requests": int(arrivals.size), "hbm_budget_bytes": budget, "max_resident_scenes": cap.max_scenes}
+    server = BatchServer(cap, settings, it, hbm_budget_bytes=budget, use_graphs=True)
+    out["arena_bytes"] = server.arena_bytes
+    rep = server.run(stream(), duration=seconds)
+    server.close()
+    out["batched"] = rep.summary()
+    log(f"serving batched: {out['batched']['vertex_updates_per_s']:.4g} vertex-updates/s, "
+        f"frame p50 {out['batched']['frame_ms']['p50']:.2f} ms")
+    plain = PlainServer(settings, it, hbm_budget_bytes=budget, max_scenes=cap.max_scenes)
+    rep = plain.run(stream(), duration=seconds)
+    plain.close()
+    out["plain"] = rep.summary()
+    log(f"serving plain: {out['plain']['vertex_updates_per_s']:.4g} vertex-updates/s, "
+        f"frame p50 {out['plain']['frame_ms']['p50']:.2f} ms")
+    p = out["plain"]["vertex_updates_per_s"]
+    out["batched_over_plain"] = out["batched"]["vertex_updates_per_s"] / p if p > 0 else None
+    return out
+
+
+def measure_large(cells, iterations, steps):
+    """The layered box as one plain scene: throughput, frame latency and the
+    local-solve roofline under Jacobi, and Gauss-Seidel throughput."""
+    import jgs2
+    from jgs2 import scenes
+    from jgs2.serving import nearest_rank
+
+    scene = scenes.layered_box(cells=cells)
+    out = {"workload": f"layered-box-{cells[0]}x{cells[1]}x{cells[2]}", "vertices": scene.vertex_count,
+           "tetrahedra": scene.tet_count}
+    for schedule in ("jacobi", "graph-colored-gauss-seidel"):
+        settings = dict(scenes.LAYERED_BOX_SETTINGS, schedule=schedule)
+        solver = jgs2.JGS2GpuSolver.create(scene, settings)
+        free = solver.free_vertex_count
+        solver.step_exact_iterations(iterations, settings)
+        solver.await_idle()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            solver.step_exact_iterations(iterations, settings)
+        solver.await_idle()
+        dt = time.perf_counter() - t0
+        leg = {"ms_per_step": dt / steps * 1e3, "vertex_updates_per_s": steps * iterations * free / dt,
+               "colors": solver.schedule_color_count}
+        if schedule == "jacobi":
+            lat = []
+            for _ in range(5):
+                a = time.perf_counter()
+                solver.step_exact_iterations(iterations, settings)
+                solver.await_idle()
+                lat.append((time.perf_counter() - a) * 1e3)
+            leg["frame_ms"] = {"p50": nearest_rank(lat, 50), "p99": nearest_rank(lat, 99)}
+            solver.profile_begin()
+            solver.step_exact_iterations(iterations, settings)
+            solve_ms, launches, _ = solver.profile_end()
+            b = scenes.algorithmic_bytes_per_sweep(scene.vertex_count, scene.tet_count, scene.cubature_k)
+            achieved = b / (solve_ms * 1e-3) / 1e9
+            leg["roofline"] = {"bound": "hbm", "kernel": "k_solve (local solve)", "achieved": achieved,
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                               "algorithmic_bytes_per_launch": b, "ms_per_launch": solve_ms,
+                               "launches_profiled": launches}
+            hist = solver.read_globalization_history().reshape(64, 8, 4)
+            leg["reverted_iterations_last_frame"] = int(hist[:iterations, 1, 3].sum())
+        solver.destroy()
+        out[schedule] = leg
+        log(f"large mesh {schedule}: {leg['ms_per_step']:.2f} ms/frame")
+    return out
+
+
+def _sample_specs(seed):
     from jgs2 import scenes
 
-    return scenes.layered_box(cells=(12, 12, 24))
+    return scenes.mixed_scene_specs(count=8, seed=seed, tet_range=(200, 3000), cloth_cells=(6, 16))
 
 
-def accuracy_check(settings, iterations, frames=2):
-    """GPU vs FP64 oracle on a 12x12x24-cell grid of the same generator."""
-    import jgs2
+def accuracy_check(iterations, seed, frames=2):
+    """A reduced mix of the same generator stepped as one batch on the GPU
+    and scene by scene in the FP64 oracle; the largest relative L2 errors."""
+    from jgs2 import scenes
+    from jgs2.batch import JGS2Batch
 
     sys.path.insert(0, str(REPO / "oracle"))
     import jgs2_oracle as O
 
-    scene = _small_scene()
-    solver = jgs2.JGS2GpuSolver.create(scene, settings)
-    solver.step_frames_exact_iterations(frames, iterations, settings)
-    x_gpu = solver.read_positions().reshape(-1, 4)[:, :3].astype(np.float64)
-    solver.destroy()
-    ref = O.Oracle(scene, 64)
-    ref.step_frames(frames, iterations, settings)
-    x_ref = ref.positions()
-    x0 = scene.positions.reshape(-1, 4)[:, :3].astype(np.float64)
-    err = float(np.linalg.norm(x_gpu - x_ref) / np.linalg.norm(x_ref))
-    derr = float(np.linalg.norm(x_gpu - x_ref) / max(np.linalg.norm(x_ref - x0), 1e-30))
-    log(f"accuracy: rel L2 {err:.3e}, displacement-relative {derr:.3e}")
-    return {"reference": "fp64 oracle", "grid": "12x12x24", "frames": frames, "rel_l2": err,
-            "rel_l2_of_displacement": derr, "tolerance": 1e-3, "within_tol": err <= 1e-3}
+    settings = dict(scenes.MIXED_SETTINGS)
+    inputs = [scenes.build_mixed_scene(s) for s in _sample_specs(seed)]
+    batch = JGS2Batch.create(_capacity(inputs), settings)
+    members = [batch.admit(x) for x in inputs]
+    batch.step_frames_exact_iterations(frames, iterations, settings)
+    worst, worst_disp = 0.0, 0.0
+    for x, m in zip(inputs, members):
+        x_gpu = m.read_positions().reshape(-1, 4)[:, :3].astype(np.float64)
+        ref = O.Oracle(x, 64)
+        ref.step_frames(frames, iterations, settings)
+        x_ref = ref.positions()
+        x0 = x.positions.reshape(-1, 4)[:, :3].astype(np.float64)
+        worst = max(worst, float(np.linalg.norm(x_gpu - x_ref) / np.linalg.norm(x_ref)))
+        worst_disp = max(worst_disp, float(np.linalg.norm(x_gpu - x_ref) / max(np.linalg.norm(x_ref - x0), 1e-30)))
+    batch.destroy()
+    log(f"accuracy: max rel L2 {worst:.3e}, displacement-relative {worst_disp:.3e}")
+    return {"reference": "fp64 oracle", "sample": f"{len(inputs)} scenes of the reduced mixed generator",
+            "frames": frames, "max_rel_l2": worst, "max_rel_l2_of_displacement": worst_disp, "tolerance": 1e-3,
+            "within_tol": worst <= 1e-3}
 
 
-def cpu_baseline(settings, iterations, frames=2):
-    """The oracle's FP32 step (single thread) on the 12x12x24-cell sample."""
+def cpu_baseline(iterations, seed, frames=1):
+    """The oracle's FP32 step (single thread) on the reduced mix."""
+    from jgs2 import scenes
+
     sys.path.insert(0, str(REPO / "oracle"))
     import jgs2_oracle as O
 
-    scene = _small_scene()
-    ref = O.Oracle(scene, 32)
-    free = int((scene.vertex_info.reshape(-1, 4)[:, 2] == 0).sum())
-    t0 = time.perf_counter()
-    ref.step_frames(frames, iterations, settings)
-    dt = time.perf_counter() - t0
-    log(f"cpu baseline: {frames} frames in {dt:.2f}s")
-    return {"value": frames * iterations * free / dt, "unit": "vertex-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP32 step, layered box 12x12x24 cells ({scene.vertex_count} vertices), "
-                      f"{frames} frames x {iterations} Jacobi sweeps, {dt:.2f}s"}
+    settings = dict(scenes.MIXED_SETTINGS)
+    inputs = [scenes.build_mixed_scene(s) for s in _sample_specs(seed)]
+    updates, dt = 0, 0.0
+    for x in inputs:
+        ref = O.Oracle(x, 32)
+        free = int((x.vertex_info.reshape(-1, 4)[:, 2] == 0).sum())
+        t0 = time.perf_counter()
+        ref.step_frames(frames, iterations, settings)
+        dt += time.perf_counter() - t0
+        updates += frames * iterations * free
+    log(f"cpu baseline: {updates} vertex-updates in {dt:.2f}s")
+    return {"value": updates / dt, "unit": "vertex-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP32 step, {len(inputs)} scenes of the reduced mixed generator "
+                      f"({sum(x.vertex_count for x in inputs)} vertices), {frames} frame x {iterations} Jacobi "
+                      f"sweeps, {dt:.2f}s"}
 
 
 if __name__ == "__main__":

@@ -199,3 +199,26 @@ def test_batch_rejects_mismatched_scenes():
     with pytest.raises(jgs2.JGS2StateError):
         member.step()
     batch.destroy()
+
+
+def test_served_requests_match_each_scene_alone():
+    from jgs2.serving import BatchServer, Request, TickClock
+
+    pool = mix()
+    rng = np.random.default_rng(5)
+    reqs = [Request(id=i, scene=pool[int(rng.integers(0, len(pool)))], frames=int(rng.integers(1, 5)),
+                    arrival=float(rng.integers(0, 6))) for i in range(12)]
+    # Room for three of the largest scenes at once: later arrivals wait for
+    # retirements and reuse their blocks.
+    big = max(pool, key=lambda x: x.tet_count)
+    cap = capacity_for([big, big, big] + [p for p in pool if p.contact is not None or p.cloth is not None])
+    print(f"case: served stream of {len(reqs)} requests through a {cap.max_scenes}-slot arena", flush=True)
+    server = BatchServer(cap, SETTINGS, IT, use_graphs=True, keep_results=True, clock=TickClock())
+    rep = server.run(reqs)
+    server.close()
+    assert len(rep.completed) == len(reqs)
+    assert rep.rejected_admissions > 0  # some requests had to wait for room
+    settings = dict(SETTINGS, iterations=IT)
+    for r in rep.completed:
+        alone = run_alone(r.scene, r.frames, settings, True)["positions"]
+        assert np.array_equal(r.result, alone), r.id

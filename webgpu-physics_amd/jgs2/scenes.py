@@ -11,6 +11,9 @@ libjgs2hip (include/jgs2hip_scene.h).
   pinned. `scale` shrinks the grid for tests.
 * `cloth_drape`: the reference's pinned cloth drape, a fixed collider block
   under a pre-folded StVK triangle sheet with dihedral bending.
+* `mixed_scene_specs` / `build_mixed_scene`: the batched uneven-scene
+  workload, seeded: 70 % tetrahedral beams and boxes with log-uniform
+  tetrahedron counts and Young's moduli, 30 % cloth sheets of n x n cells.
 """
 
 from __future__ import annotations
@@ -275,3 +278,64 @@ def algorithmic_bytes_per_sweep(V: int, T: int, K: int, objectives: bool = False
     if objectives:
         b += 32 * V
     return b
+
+
+# ---------------------------------------------------------------- mixed scenes
+MIXED_SETTINGS = {
+    "timestep": 1.0 / 60.0,
+    "gravity": (0.0, -9.81, 0.0),
+    "floor_stiffness": 0.0,
+    "horizontal_body_correction": False,
+    "parity_mode": True,
+    "schedule": "jacobi",
+}
+MIXED_ITERATIONS = 8
+# Metres per hex (the cantilever's 0.25 m cell: at dt = 1/60 finer stiff
+# cells make parallel Jacobi sweeps overshoot and revert) and per cloth cell.
+MIXED_HEX = 0.25
+MIXED_CLOTH_CELL = 0.05
+
+
+def mixed_scene_specs(count=512, seed=20251203, cloth_fraction=0.3, tet_range=(200, 200_000),
+                      cloth_cells=(32, 256), young_range=(1e4, 1e7)):
+    """Seeded descriptors of the batched uneven-scene stream: tetrahedral
+    scenes with log-uniform tetrahedron counts in `tet_range` (beams 4:1:1
+    or cubes, six tetrahedra per hex, E log-uniform in `young_range`,
+    nu uniform in [0.3, 0.45], the x = 0 face pinned) and cloth sheets of
+    n x n cells with n log-uniform in `cloth_cells` (pinned at two corners
+    over a one-cell fixed block far below). Self-contact of the sheets is
+    not generated: it needs a broad phase rather than a static superset."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        if rng.random() < cloth_fraction:
+            lo, hi = np.log(cloth_cells[0]), np.log(cloth_cells[1])
+            specs.append({"kind": "cloth", "cells": int(round(np.exp(rng.uniform(lo, hi))))})
+            continue
+        tets = float(np.exp(rng.uniform(np.log(tet_range[0]), np.log(tet_range[1]))))
+        hexes = max(1.0, tets / 6.0)
+        if rng.random() < 0.5:
+            a = max(1, int(round((hexes / 4.0) ** (1.0 / 3.0))))
+            cells = (4 * a, a, a)
+        else:
+            a = max(1, int(round(hexes ** (1.0 / 3.0))))
+            cells = (a, a, a)
+        specs.append({
+            "kind": "tet", "cells": cells,
+            "young": float(np.exp(rng.uniform(np.log(young_range[0]), np.log(young_range[1])))),
+            "poisson": float(rng.uniform(0.3, 0.45)), "seed": int(rng.integers(1, 2**31)),
+        })
+    return specs
+
+
+def build_mixed_scene(spec) -> GpuInput:
+    if spec["kind"] == "cloth":
+        n = int(spec["cells"])
+        side = n * MIXED_CLOTH_CELL
+        return cloth_drape(cells=n, origin=(-0.5 * side, 2.0, -0.5 * side), size=(side, side), fold_column=0,
+                           fold_lift=0.0, collider_origin=(-0.05, -50.0, -0.05), collider_size=(0.1, 0.1, 0.1))
+    cells = spec["cells"]
+    size = tuple(c * MIXED_HEX for c in cells)
+    return build_cuboid(_desc(cells, (0.0, 0.0, 0.0), size, jitter=0.1, seed=spec["seed"], pin_axis=0,
+                              density=1000.0, poisson=spec["poisson"], young=(spec["young"], spec["young"]),
+                              cubature_k=6))
