@@ -34,13 +34,16 @@ struct PassParams {
   uint32_t scene_count;         // scene slots in use
   uint32_t xcd_remap;           // XCD-aware block order for vertex kernels (autotuned)
   uint32_t color_index;         // colored pass: the color being swept
+  uint32_t solve_lanes;         // lanes per vertex of the local solve (autotuned: 1, 4, 8, 16)
+  uint32_t pad6, pad7, pad8;
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
 //   vtx_rest  float4[V]  rest xyz, lumped mass
 //   vtx_info  uint4[V]   adjacency start/count, pinned, body
-//   tet_idx   uint4[T]
-//   tet_dm    float4[3T] columns of Dm^-1; w = volume, lambda, mu
+//   tet_rec   uint4[4T]  one 64-byte record per tetrahedron: vertex ids,
+//                        then the columns of Dm^-1 as float bits with
+//                        w = volume, lambda, mu
 //   tet_k     float[144T] co-rotated rest stiffness (legacy path only)
 //   adjacency uint32[4T]
 //   cub       uint4[10 * V * K] record = {tet, weight bits, 0, 0} + 9 float4 Ubar
@@ -71,8 +74,7 @@ struct DeviceState {
   uint32_t V = 0, T = 0, K = 0, bodies = 1;
   float4* vtx_rest = nullptr;
   uint4* vtx_info = nullptr;
-  uint4* tet_idx = nullptr;
-  float4* tet_dm = nullptr;
+  uint4* tet_rec = nullptr;
   float* tet_k = nullptr;
   uint32_t* adjacency = nullptr;
   uint4* cub = nullptr;
@@ -85,6 +87,11 @@ struct DeviceState {
   float4* old = nullptr;
   float4* vrot = nullptr;       // 3 per vertex (columns)
   float4* vrot_next = nullptr;  // frames for the next sweep, 3 per vertex
+  // Per-vertex record the cubature terms gather, one 64-byte line per
+  // vertex, paired with vrot ([0]) and vrot_next ([1]): {R.c0, x.x},
+  // {R.c1, x.y}, {R.c2, x.z}, {x - x_pred, inertia weight m/(h^2 n), or -1
+  // for a pinned vertex}, x being the pose the frame was computed at.
+  float4* vcub[2] = {nullptr, nullptr};
   float4* trot = nullptr;       // 3 per tet (legacy path)
   float4* body_corr = nullptr;
   float4* final_update = nullptr;

@@ -175,7 +175,10 @@ struct jgs2_batch {
   uint64_t synced_version = ~0ull;
   uint32_t color_count = 1;
   std::vector<uint32_t> color_offsets;
-  uint32_t solve_block = 256, xcd_remap = 1;
+  // Launch shape of the local solve. Block size and XCD remap leave every
+  // result unchanged (autotuned); lanes per vertex set the summation order
+  // of the gathered terms and stay fixed unless a caller sets them.
+  uint32_t solve_block = 256, xcd_remap = 1, solve_lanes = 8;
   uint64_t arena_bytes = 0;
   // Cloth material shared by every resident cloth scene (one set of
   // material words per arena).
@@ -199,7 +202,7 @@ struct jgs2_batch {
 namespace {
 
 void free_device(DeviceState& d) {
-  void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_idx, d.tet_dm, d.tet_k, d.adjacency, d.cub, d.objectives,
+  void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_rec, d.vcub[0], d.vcub[1], d.tet_k, d.adjacency, d.cub, d.objectives,
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
@@ -698,6 +701,8 @@ jgs2_status batch_allocate(jgs2_batch* b) {
   TRYS(dalloc(&d.old, Vc, by));
   TRYS(dalloc(&d.vrot, 3 * Vc, by));
   TRYS(dalloc(&d.vrot_next, 3 * Vc, by));
+  TRYS(dalloc(&d.vcub[0], 4 * Vc, by));
+  TRYS(dalloc(&d.vcub[1], 4 * Vc, by));
   TRYS(dalloc(&d.final_update, Vc, by));
   TRYS(dalloc(&d.local_diag, 4 * Vc, by));
   TRYS(dalloc(&d.vertex_energy, Vc, by));
@@ -707,8 +712,7 @@ jgs2_status batch_allocate(jgs2_batch* b) {
   TRYS(dalloc(&d.cand_row, Vc, by));
   TRYS(dalloc(&d.active_count, Vc, by));
   TRYS(dalloc(&d.cub, 10 * Vc * std::max<uint64_t>(K, 1), by));
-  TRYS(dalloc(&d.tet_idx, Tc, by));
-  TRYS(dalloc(&d.tet_dm, 3 * Tc, by));
+  TRYS(dalloc(&d.tet_rec, 4 * Tc, by));
   if (b->corotated) TRYS(dalloc(&d.tet_k, 144 * Tc, by));
   TRYS(dalloc(&d.trot, 3 * Tc, by));
   TRYS(dalloc(&d.tet_glob, 2 * Tc, by));
@@ -839,8 +843,16 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
     dev_objectives[2 * size_t(nv) + 1] = make_float4(o[4], o[5], o[6], o[7]);
   }
   // Tetrahedra.
-  std::vector<uint4> tet_idx(T);
-  std::vector<float4> tet_dm(3 * size_t(T)), rot_t(3 * size_t(T));
+  std::vector<uint4> tet_rec(4 * size_t(T));
+  std::vector<float4> rot_t(3 * size_t(T));
+  auto bits4 = [](float a, float b, float c, float d) {
+    uint4 u;
+    std::memcpy(&u.x, &a, 4);
+    std::memcpy(&u.y, &b, 4);
+    std::memcpy(&u.z, &c, 4);
+    std::memcpy(&u.w, &d, 4);
+    return u;
+  };
   std::vector<float> tet_k;
   if (s->corotated) tet_k.resize(144 * size_t(T));
   for (uint32_t nt = 0; nt < T; ++nt) {
@@ -848,11 +860,11 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
     const uint32_t* i = &in.tet_indices[4 * size_t(ot)];
     const float* m = &in.tet_inverse_dm[12 * size_t(ot)];
     const float* a = &in.tet_meta[4 * size_t(ot)];
-    tet_idx[nt] = make_uint4(uint32_t(VB) + vnew[i[0]], uint32_t(VB) + vnew[i[1]], uint32_t(VB) + vnew[i[2]],
+    tet_rec[4 * size_t(nt)] = make_uint4(uint32_t(VB) + vnew[i[0]], uint32_t(VB) + vnew[i[1]], uint32_t(VB) + vnew[i[2]],
                              uint32_t(VB) + vnew[i[3]]);
-    tet_dm[3 * size_t(nt)] = make_float4(m[0], m[1], m[2], a[0]);
-    tet_dm[3 * size_t(nt) + 1] = make_float4(m[4], m[5], m[6], a[1]);
-    tet_dm[3 * size_t(nt) + 2] = make_float4(m[8], m[9], m[10], a[2]);
+    tet_rec[4 * size_t(nt) + 1] = bits4(m[0], m[1], m[2], a[0]);
+    tet_rec[4 * size_t(nt) + 2] = bits4(m[4], m[5], m[6], a[1]);
+    tet_rec[4 * size_t(nt) + 3] = bits4(m[8], m[9], m[10], a[2]);
     if (s->corotated) std::memcpy(&tet_k[144 * size_t(nt)], &in.tet_rest_stiffness[144 * size_t(ot)], 144 * sizeof(float));
     rot_t[3 * size_t(nt)] = make_float4(1, 0, 0, 0);
     rot_t[3 * size_t(nt) + 1] = make_float4(0, 1, 0, 0);
@@ -932,6 +944,21 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
   UP(d.velocity, VB, velocities);
   UP(d.vrot, 3 * VB, rot_v);
   UP(d.vrot_next, 3 * VB, rot_v);
+  {  // Cubature vertex records at the initial pose (identity frames).
+    std::vector<float4> vc(4 * Vp);
+    for (uint64_t v = 0; v < Vp; ++v) {
+      const float4 x = positions[v];
+      const uint4 inf = vtx_info[v];
+      const float n = float(inf.y > 1u ? inf.y : 1u);
+      const float w = inf.z != 0u ? -1.0f : vtx_rest[v].w * float(1.0 / (s->defaults.timestep * s->defaults.timestep)) / n;
+      vc[4 * v] = make_float4(1, 0, 0, x.x);
+      vc[4 * v + 1] = make_float4(0, 1, 0, x.y);
+      vc[4 * v + 2] = make_float4(0, 0, 1, x.z);
+      vc[4 * v + 3] = make_float4(0, 0, 0, w);
+    }
+    UP(d.vcub[0], 4 * VB, vc);
+    UP(d.vcub[1], 4 * VB, vc);
+  }
   for (float4* region : {d.final_update, d.vertex_energy}) UP(region, VB, zeros4);
   {
     std::vector<float4> z(5 * Vp, make_float4(0, 0, 0, 0));
@@ -939,8 +966,7 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
     HIP_CHECK(hipMemcpyAsync(d.conv + 5 * VB, z.data(), 5 * Vp * sizeof(float4), hipMemcpyHostToDevice, q));
   }
   UP(d.cub, 10 * VB * std::max<uint32_t>(K, 1), cub);
-  UP(d.tet_idx, TB, tet_idx);
-  UP(d.tet_dm, 3 * TB, tet_dm);
+  UP(d.tet_rec, 4 * TB, tet_rec);
   if (s->corotated) UP(d.tet_k, 144 * TB, tet_k);
   UP(d.trot, 3 * TB, rot_t);
   UP(d.tri_row, VB, tri_row);
@@ -1028,6 +1054,7 @@ PassParams make_params(jgs2_batch* b, const jgs2_settings& st, bool stop) {
   // contactEnabled (jgs2-solver.ts:2213-2214).
   p.contact = (b->dev.NC > 0 && st.ipc_barrier_stiffness > 0) ? 1u : 0u;
   p.solve_block = b->solve_block;
+  p.solve_lanes = b->solve_lanes;
   p.xcd_remap = b->xcd_remap;
   // packUniforms (jgs2-solver.ts:604-730): f64 expressions rounded to f32.
   const double inv = 1.0 / st.timestep;
@@ -1230,7 +1257,7 @@ jgs2_status submit(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
     key = fnv(key, &b->version, sizeof(b->version));
     key = fnv(key, &st, sizeof(st));
     key = fnv(key, &iterations, sizeof(iterations));
-    const uint32_t flags[3] = {stop ? 1u : 0u, b->solve_block, b->xcd_remap};
+    const uint32_t flags[4] = {stop ? 1u : 0u, b->solve_block, b->xcd_remap, b->solve_lanes};
     key = fnv(key, flags, sizeof(flags));
     for (auto* s : b->slots) {
       const uint32_t f = s ? s->objective_flags : 0u;
@@ -2142,6 +2169,7 @@ jgs2_status jgs2_batch_get_info(jgs2_batch* b, jgs2_batch_info* out) {
   out->scene_count = b->active_scenes();
   out->max_scenes = uint32_t(b->slots.size());
   out->solve_block = b->solve_block;
+  out->solve_lanes = b->solve_lanes;
   out->xcd_remap = b->xcd_remap;
   out->use_graphs = b->use_graphs ? 1u : 0u;
   uint64_t used = 0;
@@ -2190,11 +2218,14 @@ jgs2_status jgs2_batch_use_graphs(jgs2_batch* b, int enable) {
   return JGS2_OK;
 }
 
-jgs2_status jgs2_batch_set_launch(jgs2_batch* b, uint32_t solve_block, uint32_t xcd_remap) {
+jgs2_status jgs2_batch_set_launch(jgs2_batch* b, uint32_t solve_block, uint32_t xcd_remap, uint32_t solve_lanes) {
   TRYS(check_batch(b));
   if (solve_block != 64 && solve_block != 128 && solve_block != 256)
     return fail(JGS2_ERROR_RANGE, "solve_block must be 64, 128 or 256; got %u.", solve_block);
+  if (solve_lanes != 1 && solve_lanes != 4 && solve_lanes != 8 && solve_lanes != 16)
+    return fail(JGS2_ERROR_RANGE, "solve_lanes must be 1, 4, 8 or 16; got %u.", solve_lanes);
   b->solve_block = solve_block;
+  b->solve_lanes = solve_lanes;
   b->xcd_remap = xcd_remap ? 1u : 0u;
   b->graph_valid = false;
   clear_error();

@@ -136,13 +136,17 @@ struct Tet {
   float vol, lam, mu;
 };
 __device__ __forceinline__ Tet load_tet(const DeviceState& s, uint32_t t) {
+  // One 64-byte record: a single cache-line gather per tetrahedron.
+  const uint4* r = s.tet_rec + 4 * size_t(t);
   Tet e;
-  e.ix = s.tet_idx[t];
-  float4 a = s.tet_dm[3 * size_t(t)], b = s.tet_dm[3 * size_t(t) + 1], c = s.tet_dm[3 * size_t(t) + 2];
-  e.inv = {xyz(a), xyz(b), xyz(c)};
-  e.vol = a.w;
-  e.lam = b.w;
-  e.mu = c.w;
+  e.ix = r[0];
+  const uint4 a = r[1], b = r[2], c = r[3];
+  e.inv = {mk(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
+           mk(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)),
+           mk(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z))};
+  e.vol = __uint_as_float(a.w);
+  e.lam = __uint_as_float(b.w);
+  e.mu = __uint_as_float(c.w);
   return e;
 }
 // Shape gradients: rows of Dm^-1 and their negative sum (S:1080-1088).
@@ -170,6 +174,17 @@ __device__ __forceinline__ m3 deformation(const Tet& e, const float4* pos) {
   m3 ds = {e1, e2, e3};
   return mul(ds, e.inv);
 }
+// F with the pose of vertex v taken from a register instead of memory
+// (the same value: an incident tetrahedron's gather skips one load).
+__device__ __forceinline__ f3 pos_or(const float4* pos, uint32_t i, uint32_t v, f3 xv) {
+  return i == v ? xv : load_pos(pos, i);
+}
+__device__ __forceinline__ m3 deformation_at(const Tet& e, const float4* pos, uint32_t v, f3 xv) {
+  f3 p0 = pos_or(pos, e.ix.x, v, xv);
+  f3 e1 = pos_or(pos, e.ix.y, v, xv) - p0, e2 = pos_or(pos, e.ix.z, v, xv) - p0, e3 = pos_or(pos, e.ix.w, v, xv) - p0;
+  m3 ds = {e1, e2, e3};
+  return mul(ds, e.inv);
+}
 __device__ __forceinline__ m3 load_rot(const float4* r, uint32_t i) {
   return {xyz(r[3 * size_t(i)]), xyz(r[3 * size_t(i) + 1]), xyz(r[3 * size_t(i) + 2])};
 }
@@ -177,6 +192,44 @@ __device__ __forceinline__ void store_rot(float4* r, uint32_t i, const m3& m) {
   r[3 * size_t(i)] = make_float4(m.c0.x, m.c0.y, m.c0.z, 0.0f);
   r[3 * size_t(i) + 1] = make_float4(m.c1.x, m.c1.y, m.c1.z, 0.0f);
   r[3 * size_t(i) + 2] = make_float4(m.c2.x, m.c2.y, m.c2.z, 0.0f);
+}
+// The per-vertex cubature record (frame, pose, inertia residual and
+// weight) in one 64-byte line; see DeviceState::vcub.
+struct VCub {
+  m3 R;
+  f3 x, diff;  // diff = x - x_pred
+  float w;     // m / (h^2 n); 0 for a pinned vertex
+  bool pinned;
+};
+__device__ __forceinline__ VCub load_vcub(const float4* vc, uint32_t v) {
+  const float4* r = vc + 4 * size_t(v);
+  const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+  VCub o;
+  o.R = {xyz(a), xyz(b), xyz(c)};
+  o.x = mk(a.w, b.w, c.w);
+  o.diff = xyz(d);
+  o.pinned = d.w < 0.0f;
+  o.w = o.pinned ? 0.0f : d.w;
+  return o;
+}
+// Inertia weight of a vertex's distributed share (S:2401-2408).
+__device__ __forceinline__ float share_weight(const DeviceState& s, const PassParams& p, uint32_t v, uint4 inf) {
+  const float n = float(inf.y > 1u ? inf.y : 1u);
+  return s.vtx_rest[v].w * p.inv_dt2 / n;
+}
+__device__ __forceinline__ void store_vcub(const DeviceState& s, const PassParams& p, float4* vc, uint32_t v, uint4 inf,
+                                           const m3& R, f3 x) {
+  const f3 diff = x - xyz(s.predicted[v]);
+  const float w = inf.z != 0u ? -1.0f : share_weight(s, p, v, inf);
+  float4* r = vc + 4 * size_t(v);
+  r[0] = make_float4(R.c0.x, R.c0.y, R.c0.z, x.x);
+  r[1] = make_float4(R.c1.x, R.c1.y, R.c1.z, x.y);
+  r[2] = make_float4(R.c2.x, R.c2.y, R.c2.z, x.z);
+  r[3] = make_float4(diff.x, diff.y, diff.z, w);
+}
+// The record paired with this sweep's frames (rot_source).
+__device__ __forceinline__ const float4* vcub_source(const DeviceState& s, const PassParams& p) {
+  return p.fused_rot ? s.vcub[1] : s.vcub[0];
 }
 
 // ------------------------------------------------------------------ stable Neo-Hookean
@@ -799,18 +852,16 @@ __device__ __forceinline__ uint32_t ipc_energy_owner(const DeviceState& s, uint4
 }
 // Barrier and lagged-friction terms of one vertex (ipcLocalContribution,
 // S:786-874) over its active candidate row.
-__device__ void ipc_local_contribution(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
-                                       f3& g, m3& H) {
-  const uint32_t begin = s.cand_row[v].x, end = begin + s.active_count[v];
-#pragma unroll 1
-  for (uint32_t i = begin; i < end; ++i) {
-    const uint32_t c = s.active_ids[i];
+// One active candidate's barrier and friction terms for vertex v.
+__device__ void ipc_row_contribution(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
+                                     uint32_t c, f3& g, m3& H) {
+  {
     const Contact k = p.colored ? ipc_contact_at(s, c, pos, p.ipc_dhat) : ipc_cached(s, c, false);
-    if (!k.valid) continue;
+    if (!k.valid) return;
     const bool ba = ipc_barrier_active(k.distance, p.ipc_dhat);
     const float4 lag = s.lag_n[c];
     const bool fa = s.ipc_mu > 0.0f && lag.w > 0.0f;
-    if (!ba && !fa) continue;
+    if (!ba && !fa) return;
     const uint4 ix = s.cand[c];
     const float w = ipc_weight_for_vertex(ix, k.weights, v);
     if (w != 0.0f && ba) {
@@ -851,6 +902,12 @@ __device__ void ipc_local_contribution(const DeviceState& s, const PassParams& p
       H.c2 = H.c2 + cf * (mk(0, 0, 1) - un.z * un);
     }
   }
+}
+__device__ void ipc_local_contribution(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
+                                       f3& g, m3& H) {
+  const uint32_t begin = s.cand_row[v].x, end = begin + s.active_count[v];
+#pragma unroll 1
+  for (uint32_t i = begin; i < end; ++i) ipc_row_contribution(s, p, v, pos, s.active_ids[i], g, H);
 }
 // ipcOwnedEnergy (S:750-779): candidates owned by this vertex, read from the
 // source or (after a nonzero safe step) the target contact cache.
@@ -1003,9 +1060,14 @@ __device__ __forceinline__ float det_second(const m3& F, const m3& A, const m3& 
 // Ubar_l R_s^T; blocks are streamed so only the three D_c stay live.
 __device__ __forceinline__ void cubature_record_snh(const DeviceState& s, const PassParams& p, uint32_t v, f3 x,
                                                     uint32_t t, const float* ub, const m3& Rs_t, float w, Local& L) {
-  const float4* pos = s.pos[p.src];
   const Tet e = load_tet(s, t);
-  const SnhPoint sp = snh_point(deformation(e, pos), e.lam, e.mu);
+  const float4* vc = vcub_source(s, p);
+  // The sample tetrahedron's four vertex records: frame, pose, inertia.
+  VCub q[4];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) q[l] = load_vcub(vc, tet_vertex(e.ix, l));
+  const m3 ds = {q[1].x - q[0].x, q[2].x - q[0].x, q[3].x - q[0].x};
+  const SnhPoint sp = snh_point(mul(ds, e.inv), e.lam, e.mu);
   f3 pg = mk(0, 0, 0);
   float i00 = 0, i11 = 0, i22 = 0, i01 = 0, i02 = 0, i12 = 0;
   m3 D0 = mzero(), D1 = mzero(), D2 = mzero();
@@ -1014,16 +1076,15 @@ __device__ __forceinline__ void cubature_record_snh(const DeviceState& s, const 
   for (int l = 0; l < 4; ++l) {
     const uint32_t vl = tet_vertex(e.ix, l);
     if (vl == v) ss = l;
-    const m3 B = mul(mul(load_rot(rot_source(s, p), vl), ubar_block(ub, l)), Rs_t);
+    const m3 B = mul(mul(q[l].R, ubar_block(ub, l)), Rs_t);
     const f3 gr = shape_gradient(e, l);
-    const uint4 il = s.vtx_info[vl];
-    const f3 xl = load_pos(pos, vl);
-    const float n = float(il.y > 1u ? il.y : 1u);
-    const float wl = il.z == 0u ? s.vtx_rest[vl].w * p.inv_dt2 / n : 0.0f;
-    f3 sg = e.vol * piola_times(sp, gr) + wl * (xl - xyz(s.predicted[vl]));
+    const float wl = q[l].w;
+    f3 sg = e.vol * piola_times(sp, gr) + wl * q[l].diff;
     float wt = wl;
-    if (p.objective_flags != 0u && il.z == 0u) {
-      sg = sg + objective_gradient(s, p, vl, xl) / n;
+    if (p.objective_flags != 0u && !q[l].pinned) {
+      const uint4 il = s.vtx_info[vl];
+      const float n = float(il.y > 1u ? il.y : 1u);
+      sg = sg + objective_gradient(s, p, vl, q[l].x) / n;
       if (p.objective_flags & kTarget) wt += s.objectives[2 * size_t(vl) + 1].w / n;
     }
     pg = pg + tmul(B, sg);
@@ -1194,7 +1255,7 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
     L.H = madd(L.H, ih);
   }
   if (p.cubature_k == 0) return L;
-  const m3 Rs_t = tr(load_rot(rot_source(s, p), v));
+  const m3 Rs_t = tr(load_vcub(vcub_source(s, p), v).R);
 #pragma unroll 1
   for (uint32_t k = 0; k < p.cubature_k; ++k) {  // S:3198-3263
     const size_t rec = size_t(v) * p.cubature_k + k;
@@ -1312,7 +1373,8 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
     }
   }
   if (p.cubature_k > 0) {
-    const f3 ds = tmul(load_rot(rot_source(s, p), v), d);  // R_s^T d
+    const float4* vc = vcub_source(s, p);
+    const f3 ds = tmul(load_vcub(vc, v).R, d);  // R_s^T d
     const float n_src = float(inf.y > 1u ? inf.y : 1u);
     const float w_src = s.vtx_rest[v].w * p.inv_dt2 / n_src;
     for (uint32_t k = 0; k < p.cubature_k; ++k) {
@@ -1322,7 +1384,10 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
       if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
       const Tet e = load_tet(s, head.x);
       const float* ub = reinterpret_cast<const float*>(s.cub + 10 * rec + 1);
-      const m3 F = deformation(e, pos);
+      VCub q[4];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) q[l] = load_vcub(vc, tet_vertex(e.ix, l));
+      const m3 F = mul(m3{q[1].x - q[0].x, q[2].x - q[0].x, q[3].x - q[0].x}, e.inv);
       m3 Dd = mzero();
       float compl_e = 0.0f;
       int ss = 4;
@@ -1330,17 +1395,19 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
       for (int l = 0; l < 4; ++l) {
         const uint32_t vl = tet_vertex(e.ix, l);
         if (vl == v) ss = l;
-        const f3 bd = mul(load_rot(rot_source(s, p), vl), ubar_times(ub, l, ds));  // B_l d
+        const f3 bd = mul(q[l].R, ubar_times(ub, l, ds));  // B_l d
         const f3 gl = shape_gradient(e, l);
         Dd.c0 = Dd.c0 + gl.x * bd;
         Dd.c1 = Dd.c1 + gl.y * bd;
         Dd.c2 = Dd.c2 + gl.z * bd;
-        const uint4 il = s.vtx_info[vl];
-        if (il.z == 0u) {
-          const float n = float(il.y > 1u ? il.y : 1u);
-          const f3 xl = load_pos(pos, vl);
-          compl_e += quadratic_delta(xl, xyz(s.predicted[vl]), bd, s.vtx_rest[vl].w * p.inv_dt2 / n);
-          if (p.objective_flags != 0u) compl_e += objective_delta(s, p, vl, xl, bd) / n;
+        if (!q[l].pinned) {
+          // quadratic_delta(x_l, x_pred_l, bd, w_l) from the stored x - x_pred.
+          compl_e += q[l].w * (dot(q[l].diff, bd) + 0.5f * dot(bd, bd));
+          if (p.objective_flags != 0u) {
+            const uint4 il = s.vtx_info[vl];
+            const float n = float(il.y > 1u ? il.y : 1u);
+            compl_e += objective_delta(s, p, vl, q[l].x, bd) / n;
+          }
         }
       }
       const float det_trial = mdet(madd(F, Dd));
@@ -1456,8 +1523,10 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassPara
   // frames stay those of its own last color pass.
   if (p.colored && p.color_index >= s.scene_cnt[sc].z) return;
   const uint4 inf = s.vtx_info[v];
+  const f3 x = load_pos(s.pos[p.src], v);
   if (inf.y == 0u) {
     store_rot(s.vrot, v, mid());
+    store_vcub(s, p, s.vcub[0], v, inf, mid(), x);
     return;
   }
   m3 avg = mzero();
@@ -1469,7 +1538,459 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassPara
     tv += e.vol;
   }
   avg = tv > 0.0f ? mscale(1.0f / tv, avg) : mid();
-  store_rot(s.vrot, v, polar_rotation(avg, p.polar_eps));
+  const m3 R = polar_rotation(avg, p.polar_eps);
+  store_rot(s.vrot, v, R);
+  store_vcub(s, p, s.vcub[0], v, inf, R, x);
+}
+
+// ------------------------------------------------------------------ cooperative local solve
+// The local solve of one vertex split over LPV lanes of a wavefront. A
+// vertex's items -- active contact rows, incident tetrahedra, cloth
+// triangles, hinges and cubature records, in that canonical order -- are
+// dealt round robin (item j to lane j mod LPV); each lane sums its items'
+// gradient and Hessian terms, the lanes combine them with xor butterflies
+// (commutative pairwise adds, so every lane ends with the same bits), and
+// every lane then runs the same 3x3 Newton step and line-search decisions.
+// The line-search trial splits the same way; the earliest failing item in
+// canonical order decides an infeasible trial exactly as the sequential
+// early returns do (S:1980-2337).
+template <int LPV> __device__ __forceinline__ float lanes_sum(float x) {
+#pragma unroll
+  for (int m = LPV / 2; m >= 1; m >>= 1) x += __shfl_xor(x, m, LPV);
+  return x;
+}
+template <int LPV> __device__ __forceinline__ float lanes_min(float x) {
+#pragma unroll
+  for (int m = LPV / 2; m >= 1; m >>= 1) x = fminf(x, __shfl_xor(x, m, LPV));
+  return x;
+}
+template <int LPV> __device__ __forceinline__ uint32_t lanes_min_u(uint32_t x) {
+#pragma unroll
+  for (int m = LPV / 2; m >= 1; m >>= 1) {
+    const uint32_t o = __shfl_xor(x, m, LPV);
+    x = o < x ? o : x;
+  }
+  return x;
+}
+template <int LPV> __device__ __forceinline__ f3 lanes_sum3(f3 a) {
+  return mk(lanes_sum<LPV>(a.x), lanes_sum<LPV>(a.y), lanes_sum<LPV>(a.z));
+}
+// First item of a type (prefix = items before it in canonical order) that
+// lane `sub` handles.
+template <int LPV> __device__ __forceinline__ uint32_t lane_first(uint32_t sub, uint32_t prefix) {
+  return (sub + LPV - prefix % LPV) % LPV;
+}
+
+template <bool kCloth, bool kIpc, int LPV>
+__device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, uint32_t v, uint32_t sub) {
+  const float4* pos = s.pos[p.src];
+  const f3 x = load_pos(pos, v);
+  const uint4 inf = s.vtx_info[v];
+  Local L;
+  L.g = mk(0, 0, 0);
+  L.H = mzero();
+  if (sub == 0) {  // the vertex's own terms (S:3083-3096, S:3188-3191)
+    const float inertia = s.vtx_rest[v].w * p.inv_dt2;
+    L.g = inertia * (x - xyz(s.predicted[v]));
+    L.H = mscale(inertia + target_stiffness(s, p, v), mid());
+    if (p.objective_flags != 0u) L.g = L.g + objective_gradient(s, p, v, x);
+    if (p.floor_k > 0.0f && x.y < p.floor_y) {
+      L.g.y += p.floor_k * (x.y - p.floor_y);
+      L.H.c1.y += p.floor_k;
+    }
+  }
+  uint32_t prefix = 0;
+  if constexpr (kIpc) {  // S:3193-3195
+    const uint32_t begin = s.cand_row[v].x, n = s.active_count[v];
+#pragma unroll 1
+    for (uint32_t k = lane_first<LPV>(sub, prefix); k < n; k += LPV)
+      ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], L.g, L.H);
+    prefix += n;
+  }
+#pragma unroll 1
+  for (uint32_t k = lane_first<LPV>(sub, prefix); k < inf.y; k += LPV) {  // S:3097-3111
+    const Tet e = load_tet(s, s.adjacency[inf.x + k]);
+    const uint32_t slot = local_slot(e.ix, v);
+    if (slot >= 4) continue;
+    add_incident_block(snh_point(deformation_at(e, pos, v, x), e.lam, e.mu), e, shape_gradient(e, slot), 1.0f, L.g, L.H);
+  }
+  prefix += inf.y;
+  if constexpr (kCloth) {
+    const uint2 tr = s.tri_row[v], hr = s.hinge_row[v];
+#pragma unroll 1
+    for (uint32_t k = lane_first<LPV>(sub, prefix); k < tr.y; k += LPV) {  // S:3117-3144
+      const uint32_t tri = s.tri_elems[tr.x + k];
+      const uint32_t slot = tri_slot(s.tri_ix[tri], v);
+      if (slot >= 3) continue;
+      const Membrane m = membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0));
+      L.g = L.g + membrane_gradient(s, tri, m, slot);
+      L.H = madd(L.H, membrane_local_hessian(s, tri, m, slot));
+    }
+    prefix += tr.y;
+#pragma unroll 1
+    for (uint32_t k = lane_first<LPV>(sub, prefix); k < hr.y; k += LPV) {  // S:3145-3184
+      const uint32_t h = s.hinge_elems[hr.x + k];
+      const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
+      if (slot >= 4) continue;
+      const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+      if (!d.valid) {
+        L.g = L.g + mk(kF32Max, kF32Max, kF32Max);
+      } else {
+        const float4 r = s.hinge_rest[h];
+        const float ad = wrapped_angle_difference(d.angle, r.x);
+        const float k2 = s.cloth_bend * r.y;
+        L.g = L.g + (k2 * ad) * d.grad;
+        L.H = madd(L.H, m3{(k2 * d.grad.x) * d.grad, (k2 * d.grad.y) * d.grad, (k2 * d.grad.z) * d.grad});
+      }
+    }
+    prefix += hr.y;
+  }
+  const uint32_t k0 = lane_first<LPV>(sub, prefix);
+  if (k0 < p.cubature_k) {  // S:3198-3263
+    const m3 Rs_t = tr(load_vcub(vcub_source(s, p), v).R);
+#pragma unroll 1
+    for (uint32_t k = k0; k < p.cubature_k; k += LPV) {
+      const size_t rec = size_t(v) * p.cubature_k + k;
+      const uint4 head = s.cub[10 * rec];
+      const float w = __uint_as_float(head.y);
+      if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
+      cubature_record_snh(s, p, v, x, head.x, reinterpret_cast<const float*>(s.cub + 10 * rec + 1), Rs_t, w, L);
+    }
+  }
+  L.g = lanes_sum3<LPV>(L.g);
+  L.H = {lanes_sum3<LPV>(L.H.c0), lanes_sum3<LPV>(L.H.c1), lanes_sum3<LPV>(L.H.c2)};
+  return L;
+}
+
+// One lane's share of the restricted trial. `fail` is the canonical index of
+// the lane's first failing item times two, plus one for invalid geometry
+// (zero for an infeasible contact distance); ~0 when none failed.
+struct TrialLanes {
+  float energy, min_det;
+  uint32_t fail;
+};
+template <bool kCloth, bool kIpc, int LPV>
+__device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p, uint32_t v, uint32_t sub, f3 x, f3 d,
+                                       float initial_min) {
+  const float4* pos = s.pos[p.src];
+  const uint4 inf = s.vtx_info[v];
+  TrialLanes tr{0.0f, initial_min, 0xffffffffu};
+  const f3 pred = xyz(s.predicted[v]);
+  if (sub == 0) {
+    tr.energy = quadratic_delta(x, pred, d, s.vtx_rest[v].w * p.inv_dt2) + floor_delta(p, x.y, d.y);
+    if (p.objective_flags != 0u) tr.energy += objective_delta(s, p, v, x, d);
+  }
+  uint32_t prefix = 0;
+  if constexpr (kIpc) {  // S:1997-2037, S:2140-2193
+    const f3 tp = x + d;
+    const float dl = sqrtf(dot(d, d));
+    const uint32_t begin = s.cand_row[v].x, n = s.active_count[v];
+#pragma unroll 1
+    for (uint32_t k = lane_first<LPV>(sub, prefix); k < n; k += LPV) {
+      const uint32_t c = s.active_ids[begin + k];
+      const Contact sc = p.colored ? ipc_contact_at(s, c, pos, p.ipc_dhat) : ipc_cached(s, c, false);
+      if (!sc.valid) {
+        tr.fail = 2u * (prefix + k) + 1u;
+        return tr;
+      }
+      const bool near = !(sc.distance - dl > s.ipc_dmin);
+      const bool may_change = sc.distance - dl < p.ipc_dhat;
+      const bool fa = ipc_friction_active(s, c);
+      if (!near && !may_change && !fa) continue;
+      Contact tc = Contact{p.ipc_dhat, mk(1, 0, 0), true, make_float4(0, 0, 0, 0)};
+      if (near || may_change) tc = ipc_contact_at(s, c, pos, p.ipc_dhat, v, tp);
+      if (near) {
+        if (!tc.valid) {
+          tr.fail = 2u * (prefix + k) + 1u;
+          return tr;
+        }
+        if (!(tc.distance > s.ipc_dmin)) {
+          tr.fail = 2u * (prefix + k);
+          return tr;
+        }
+      }
+      if (!may_change && !fa) continue;
+      const uint4 ix = s.cand[c];
+      tr.energy += ipc_energy_from_contact(s, p, c, ix, tc, pos, v, tp) -
+                   ipc_energy_from_contact(s, p, c, ix, sc, pos, kNoVertex, mk(0, 0, 0));
+    }
+    prefix += n;
+  }
+#pragma unroll 1
+  for (uint32_t k = lane_first<LPV>(sub, prefix); k < inf.y; k += LPV) {  // S:2038-2061, S:2120-2139
+    const Tet e = load_tet(s, s.adjacency[inf.x + k]);
+    const f3 gr = shape_gradient(e, local_slot(e.ix, v));
+    const m3 F = deformation_at(e, pos, v, x);
+    const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
+    const float J = dot(F.c0, c12);
+    const f3 b = gr.x * c12 + gr.y * c20 + gr.z * c01;
+    const float jd = dot(d, b);
+    const float det_trial = J + jd;
+    if (!finite_s(det_trial)) {
+      tr.fail = 2u * (prefix + k) + 1u;
+      return tr;
+    }
+    tr.min_det = fminf(tr.min_det, det_trial);
+    const float id = 2.0f * dot(d, mul(F, gr)) + dot(d, d) * dot(gr, gr);
+    const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
+    tr.energy += e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
+  }
+  prefix += inf.y;
+  if constexpr (kCloth) {
+    const f3 tp = x + d;
+    const uint2 trr = s.tri_row[v], hr = s.hinge_row[v];
+#pragma unroll 1
+    for (uint32_t k = lane_first<LPV>(sub, prefix); k < trr.y; k += LPV) {
+      const uint32_t tri = s.tri_elems[trr.x + k];
+      if (tri_slot(s.tri_ix[tri], v) >= 3) continue;
+      // Signed relative area stretch (S:2062-2116).
+      const f3 sn = tri_raw_normal(s, tri, pos, kNoVertex, mk(0, 0, 0));
+      const f3 tn = tri_raw_normal(s, tri, pos, v, tp);
+      const float tra = 2.0f * s.tri_area[tri];
+      const float ss = len3(sn) / tra;
+      const float tm = len3(tn) / tra;
+      const float ts = dot(sn, tn) > 0.0f ? tm : -tm;
+      if (!(tra > 0.0f) || !finite_s(ss) || !finite_s(ts)) {
+        tr.fail = 2u * (prefix + k) + 1u;
+        return tr;
+      }
+      tr.min_det = fminf(tr.min_det, fminf(ss, ts));
+      // Membrane energy delta (S:2203-2236).
+      tr.energy += membrane_at(s, tri, pos, v, tp).energy - membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy;
+    }
+    prefix += trr.y;
+#pragma unroll 1
+    for (uint32_t k = lane_first<LPV>(sub, prefix); k < hr.y; k += LPV) {  // S:2237-2271
+      const uint32_t h = s.hinge_elems[hr.x + k];
+      if (hinge_slot(s.hinge_ix[h], v) >= 4) continue;
+      tr.energy += bending_energy_at(s, h, pos, v, tp) - bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
+    }
+    prefix += hr.y;
+  }
+  const uint32_t k0 = lane_first<LPV>(sub, prefix);
+  if (k0 < p.cubature_k) {  // S:2272-2337
+    const float4* vc = vcub_source(s, p);
+    const f3 ds = tmul(load_vcub(vc, v).R, d);  // R_s^T d
+    const float n_src = float(inf.y > 1u ? inf.y : 1u);
+    const float w_src = s.vtx_rest[v].w * p.inv_dt2 / n_src;
+#pragma unroll 1
+    for (uint32_t k = k0; k < p.cubature_k; k += LPV) {
+      const size_t rec = size_t(v) * p.cubature_k + k;
+      const uint4 head = s.cub[10 * rec];
+      const float w = __uint_as_float(head.y);
+      if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
+      const Tet e = load_tet(s, head.x);
+      const float* ub = reinterpret_cast<const float*>(s.cub + 10 * rec + 1);
+      VCub q[4];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) q[l] = load_vcub(vc, tet_vertex(e.ix, l));
+      const m3 F = mul(m3{q[1].x - q[0].x, q[2].x - q[0].x, q[3].x - q[0].x}, e.inv);
+      m3 Dd = mzero();
+      float compl_e = 0.0f;
+      int ss = 4;
+#pragma unroll 1
+      for (int l = 0; l < 4; ++l) {
+        const uint32_t vl = tet_vertex(e.ix, l);
+        if (vl == v) ss = l;
+        const f3 bd = mul(q[l].R, ubar_times(ub, l, ds));  // B_l d
+        const f3 gl = shape_gradient(e, l);
+        Dd.c0 = Dd.c0 + gl.x * bd;
+        Dd.c1 = Dd.c1 + gl.y * bd;
+        Dd.c2 = Dd.c2 + gl.z * bd;
+        if (!q[l].pinned) {
+          compl_e += q[l].w * (dot(q[l].diff, bd) + 0.5f * dot(bd, bd));
+          if (p.objective_flags != 0u) {
+            const uint4 il = s.vtx_info[vl];
+            const float n = float(il.y > 1u ? il.y : 1u);
+            compl_e += objective_delta(s, p, vl, q[l].x, bd) / n;
+          }
+        }
+      }
+      const float det_trial = mdet(madd(F, Dd));
+      if (!finite_s(det_trial)) {
+        tr.fail = 2u * (prefix + k) + 1u;
+        return tr;
+      }
+      tr.min_det = fminf(tr.min_det, det_trial);
+      compl_e += snh_delta_general(F, Dd, e.lam, e.mu) * e.vol;
+      if (ss < 4) {
+        const f3 gs = shape_gradient(e, ss);
+        const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
+        const float J = dot(F.c0, c12);
+        const f3 b = gs.x * c12 + gs.y * c20 + gs.z * c01;
+        const float jd = dot(d, b);
+        const float id = 2.0f * dot(d, mul(F, gs)) + dot(d, d) * dot(gs, gs);
+        const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
+        compl_e -= e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
+        compl_e -= quadratic_delta(x, pred, d, w_src);
+        if (p.objective_flags != 0u) compl_e -= objective_delta(s, p, v, x, d) / n_src;
+      }
+      tr.energy += w * compl_e;
+    }
+  }
+  return tr;
+}
+template <bool kCloth, bool kIpc, int LPV>
+__device__ Trial trial_lanes(const DeviceState& s, const PassParams& p, uint32_t v, uint32_t sub, f3 x, f3 d,
+                             float initial_min) {
+  const TrialLanes part = trial_lanes_part<kCloth, kIpc, LPV>(s, p, v, sub, x, d, initial_min);
+  const uint32_t fail = lanes_min_u<LPV>(part.fail);
+  const float min_det = lanes_min<LPV>(part.min_det);
+  const float energy = lanes_sum<LPV>(part.energy);
+  if (fail != 0xffffffffu) {
+    if (fail & 1u) return Trial{0.0f, false, 0.0f};
+    return Trial{0.0f, true, 0.0f};  // an infeasible contact distance
+  }
+  return Trial{min_det, true, energy};
+}
+
+// jgs2Solve (S:2992-3487) with LPV lanes per vertex (Jacobi: every vertex;
+// graph-colored Gauss-Seidel: the vertices of the active color, in place).
+template <bool kCloth, bool kIpc, int LPV>
+__global__ void __launch_bounds__(kBlock) k_solve_lanes(DeviceState s, PassParams p) {
+  constexpr bool kEnergyInSolve = !kCloth && !kIpc;
+  const uint32_t gid = logical_block(p.xcd_remap) * blockDim.x + threadIdx.x;
+  const uint32_t lane = gid / LPV, sub = gid % LPV;
+  uint32_t v;
+  if (p.colored) {
+    if (p.color_begin + lane >= p.color_end) return;
+    v = s.color_order[p.color_begin + lane];
+  } else {
+    if (lane >= p.vertex_count) return;
+    v = lane;
+  }
+  // Every branch below is uniform over a vertex's LPV lanes.
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene) return;
+  float4* tgt = s.pos[p.tgt];
+  if (stop_latched(s, p, sc)) {
+    if (sub == 0) tgt[v] = s.pos[p.src][v];
+    return;
+  }
+  if (p.fused_rot && sub < 3) {  // this sweep's vertexPolarRotation result (S:2960-2989)
+    s.vrot[3 * size_t(v) + sub] = s.vrot_next[3 * size_t(v) + sub];
+  }
+  if (s.vtx_info[v].z != 0u) {  // S:2997-3003
+    if (sub == 0) {
+      const float4 r = s.vtx_rest[v];
+      tgt[v] = make_float4(r.x, r.y, r.z, 1.0f);
+      s.final_update[v] = make_float4(0, 0, 0, 1);
+      store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, kPinned);
+      if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
+    }
+    return;
+  }
+  const f3 x = load_pos(s.pos[p.src], v);
+  float initial_min = kF32Max;
+  {
+    const float4 pre = s.control[4 * size_t(sc)];
+    const uint32_t pv = control_bits(pre.w) & kValidityMask;
+    const bool amv = (pv & kAcceptedMinimumValid) != 0u && finite_s(pre.z);
+    if (amv) initial_min = pre.z;
+    if (!(amv && pre.z > kDeterminantFloor)) {
+      if (sub == 0) {
+        tgt[v] = s.pos[p.src][v];
+        s.final_update[v] = make_float4(0, 0, 0, 1);
+        store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, amv ? pre.z : 0.0f, 0, 0,
+                         amv ? kLineSearchFailed : kNonfinite);
+        if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
+      }
+      return;
+    }
+  }
+  const Local L = gather_local_lanes<kCloth, kIpc, LPV>(s, p, v, sub);
+  const float inertia = s.vtx_rest[v].w * p.inv_dt2;
+  // Globalized Newton direction and line search (S:3265-3471).
+  const SolveOut dr = globalization_solve(L.H, L.g, inertia);
+  uint32_t status = kNonfinite;
+  const f3 full = x + dr.dir;
+  const f3 fd = full - x;
+  const float xs = fmaxf(fabsf(x.x), fmaxf(fabsf(x.y), fabsf(x.z)));
+  const float resolution = fmaxf(s.scene_scale[sc], fmaxf(xs, 1.0e-12f)) * kUnitRoundoff * kPositionResolutionMultiplier;
+  const bool below = dr.status == kAccepted && finite3(full) && finite3(fd) && scaled_length3(fd) <= resolution;
+  if (dr.status == kAccepted) status = below ? kZeroGradient : kLineSearchFailed;
+  else if (dr.status == kZeroGradient) status = kZeroGradient;
+  else if (dr.status == kShiftLimit) status = kShiftLimit;
+  else if (dr.status == kCholeskyFailed) status = kCholeskyFailed;
+  else if (dr.status == kNonDescent) status = kNonDescent;
+  float alpha = 0.0f, accepted_de = 0.0f, bound = 0.0f, min_trial = 0.0f;
+  uint32_t backtracks = 0, evaluations = 0;
+  f3 accepted_d = mk(0, 0, 0), accepted_x = x;
+  if (dr.status == kAccepted && !below) {
+    float ta = 1.0f;
+    const float dl = scaled_length3(dr.dir);
+    if (p.max_step > 0.0f && dl > p.max_step) ta = p.max_step / dl;
+    for (uint32_t b = 0; b <= kMaxBacktracks; ++b) {
+      const f3 tp = x + ta * dr.dir;
+      const f3 td = tp - x;
+      if (!finite3(tp) || !finite3(td)) {
+        status = kNonfinite;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      const float tl = scaled_length3(td);
+      if (p.max_step > 0.0f && tl > p.max_step) {
+        ta *= 0.5f;
+        continue;
+      }
+      if (tl <= resolution) {
+        status = kZeroGradient;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      const float tgd = dot(L.g, td);
+      if (!finite_s(tgd)) {
+        status = kNonfinite;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      if (!(tgd < 0.0f)) {
+        status = kZeroGradient;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      const Trial trial = trial_lanes<kCloth, kIpc, LPV>(s, p, v, sub, x, td, initial_min);
+      if (!trial.geometry_valid) {
+        status = kNonfinite;
+        min_trial = 0.0f;
+        backtracks = b;
+        break;
+      }
+      min_trial = trial.min_det;
+      if (trial.min_det > kDeterminantFloor) {
+        evaluations += 1;
+        if (!finite_s(trial.energy)) {
+          status = kNonfinite;
+          backtracks = b;
+          break;
+        }
+        const float tb = kArmijoC1 * tgd;
+        if (trial.energy <= tb) {
+          alpha = ta;
+          accepted_de = trial.energy;
+          bound = tb;
+          backtracks = b;
+          status = kAccepted;
+          accepted_d = td;
+          accepted_x = tp;
+          break;
+        }
+      }
+      ta *= 0.5f;
+    }
+    if (status == kLineSearchFailed) backtracks = kMaxBacktracks;
+  }
+  if (sub == 0) {
+    tgt[v] = make_float4(accepted_x.x, accepted_x.y, accepted_x.z, 1.0f);
+    s.final_update[v] = make_float4(scaled_length3(accepted_d), 0, 0, 1);
+    store_local_diag(s, v, dr.dir, alpha, dr.min_eig * dr.scale, dr.scale, dr.shift * dr.scale, dr.shift, accepted_de,
+                     bound, dr.gdotp, dr.residual, min_trial, backtracks, evaluations, status);
+    if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
+  }
 }
 
 // jgs2Solve (S:2992-3487), one thread per vertex (Jacobi) or per vertex of
@@ -1990,6 +2511,7 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
         m3 R = mid();
         if (inf.y != 0u) R = polar_rotation(tv > 0.0f ? mscale(1.0f / tv, avg) : mid(), p.polar_eps);
         store_rot(s.vrot_next, v, R);
+        store_vcub(s, p, s.vcub[1], v, inf, R, x);
       }
       if (pinned) {
         for (int k = 0; k < 5; ++k) comp5[k] = mk(0, 0, 0);
@@ -2018,6 +2540,114 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
     } else {
       for (int k = 0; k < 5; ++k) comp5[k] = xyz(c[k]);
     }
+    f3 total = mk(0, 0, 0);
+    bool tv_ok = true;
+    for (int k = 0; k < 5; ++k) {
+      const bool ok = finite3(comp5[k]);
+      const f3 val = ok ? comp5[k] : mk(0, 0, 0);
+      const float vs = max_abs(val);
+      if (k < 4) r.ch[k] = fmaxf(r.ch[k], vs);
+      else r.fin[0] = fmaxf(r.fin[0], vs);
+      total = total + val;
+      tv_ok = tv_ok && ok;
+    }
+    tv_ok = tv_ok && finite3(total);
+    if (tv_ok) r.fin[1] = fmaxf(r.fin[1], max_abs(total));
+    else r.valid = 0u;
+    const float up = s.final_update[v].x;
+    if (finite_s(up) && up >= 0.0f) r.max_update = fmaxf(r.max_update, up);
+    else r.valid = 0u;
+  }
+  block_reduce(r, sh, conv_max);
+  if (threadIdx.x == 0) reinterpret_cast<ConvPartial*>(s.partials)[blockIdx.x] = r;
+}
+
+// convergenceGradient with LPV lanes per vertex. A block still covers one
+// 256-vertex arena block (its partial record), in LPV rounds of 256 / LPV
+// vertices; the lanes of a vertex split its incident tetrahedra and combine
+// the material gradient and the rotation average with xor butterflies.
+template <bool kCloth, bool kIpc, int LPV>
+__global__ void __launch_bounds__(kBlock) k_convergence_gradient_lanes(DeviceState s, PassParams p) {
+  __shared__ ConvPartial sh[kBlock];
+  ConvPartial r;
+  conv_identity(r);
+  const uint32_t sc = s.vblock_scene[blockIdx.x];
+  if (sc == kNoScene) return;  // uniform per block
+  const bool latched = stop_latched(s, p, sc);
+  const uint32_t sub = threadIdx.x % LPV;
+#pragma unroll 1
+  for (uint32_t round = 0; round < LPV; ++round) {
+    const uint32_t v = blockIdx.x * kBlock + round * (kBlock / LPV) + threadIdx.x / LPV;
+    if (v >= p.vertex_count) continue;  // uniform per vertex
+    float4* c = s.conv + 5 * size_t(v);
+    f3 comp5[5];
+    if (!latched) {
+      const uint4 inf = s.vtx_info[v];
+      const bool pinned = inf.z != 0u;
+      const float4* pos = s.pos[p.tgt];
+      const f3 x = load_pos(pos, v);
+      f3 mat = mk(0, 0, 0);
+      m3 avg = mzero();
+      float tv = 0.0f;
+      if (!pinned || p.compute_next_rot) {
+#pragma unroll 1
+        for (uint32_t k = sub; k < inf.y; k += LPV) {
+          const Tet e = load_tet(s, s.adjacency[inf.x + k]);
+          const m3 F = deformation_at(e, pos, v, x);
+          if (p.compute_next_rot) {
+            avg = madd(avg, mscale(e.vol, F));
+            tv += e.vol;
+          }
+          const uint32_t slot = local_slot(e.ix, v);
+          if (pinned || slot >= 4) continue;
+          const SnhPoint sp = snh_point(F, e.lam, e.mu);
+          mat = mat + e.vol * piola_times(sp, shape_gradient(e, slot));
+        }
+        mat = lanes_sum3<LPV>(mat);
+        if (p.compute_next_rot) {
+          avg = {lanes_sum3<LPV>(avg.c0), lanes_sum3<LPV>(avg.c1), lanes_sum3<LPV>(avg.c2)};
+          tv = lanes_sum<LPV>(tv);
+        }
+      }
+      if (sub == 0) {
+        if constexpr (kCloth) {
+          if (!pinned) mat = mat + cloth_gradient(s, pos, v);
+        }
+        if (p.compute_next_rot) {
+          m3 R = mid();
+          if (inf.y != 0u) R = polar_rotation(tv > 0.0f ? mscale(1.0f / tv, avg) : mid(), p.polar_eps);
+          store_rot(s.vrot_next, v, R);
+          store_vcub(s, p, s.vcub[1], v, inf, R, x);
+        }
+        if (pinned) {
+          for (int k = 0; k < 5; ++k) comp5[k] = mk(0, 0, 0);
+        } else {
+          comp5[0] = (s.vtx_rest[v].w * p.inv_dt2) * (x - xyz(s.predicted[v]));
+          comp5[1] = mat;
+          comp5[2] = mk(0, 0, 0);
+          comp5[3] = mk(0, 0, 0);
+          if (p.objective_flags != 0u) {
+            if (p.objective_flags & kForce) comp5[2] = -xyz(s.objectives[2 * size_t(v)]);
+            if (p.objective_flags & kTarget) {
+              const float4 t = s.objectives[2 * size_t(v) + 1];
+              if (t.w > 0.0f) comp5[3] = t.w * (x - xyz(t));
+            }
+          }
+          comp5[4] = mk(0, 0, 0);
+          if (p.floor_k > 0.0f && x.y < p.floor_y) comp5[4].y = p.floor_k * (x.y - p.floor_y);
+          if constexpr (kIpc) {  // S:4091
+            f3 ig = mk(0, 0, 0);
+            m3 ih = mzero();
+            ipc_local_contribution(s, p, v, pos, ig, ih);
+            comp5[4] = comp5[4] + ig;
+          }
+        }
+        for (int k = 0; k < 5; ++k) c[k] = make_float4(comp5[k].x, comp5[k].y, comp5[k].z, 0.0f);
+      }
+    } else if (sub == 0) {
+      for (int k = 0; k < 5; ++k) comp5[k] = xyz(c[k]);
+    }
+    if (sub != 0) continue;
     f3 total = mk(0, 0, 0);
     bool tv_ok = true;
     for (int k = 0; k < 5; ++k) {
@@ -2504,9 +3134,29 @@ void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t 
     else hipLaunchKernelGGL((KERNEL<false, false, false>), GRID, BLOCK, 0, st, __VA_ARGS__); \
   } while (0)
 
-// The local solve runs with the autotuned block size (64, 128 or 256).
+#define JGS2_LANES_DISPATCH(LPV, GRID, BLOCK)                                                        \
+  do {                                                                                               \
+    const bool cl = has_cloth(p), ipc = p.contact != 0u;                                            \
+    if (cl && ipc) hipLaunchKernelGGL((k_solve_lanes<true, true, LPV>), GRID, BLOCK, 0, st, s, p);  \
+    else if (cl) hipLaunchKernelGGL((k_solve_lanes<true, false, LPV>), GRID, BLOCK, 0, st, s, p);   \
+    else if (ipc) hipLaunchKernelGGL((k_solve_lanes<false, true, LPV>), GRID, BLOCK, 0, st, s, p);  \
+    else hipLaunchKernelGGL((k_solve_lanes<false, false, LPV>), GRID, BLOCK, 0, st, s, p);          \
+  } while (0)
+
+// The local solve runs with the autotuned block size (64, 128 or 256) and,
+// on the stable Neo-Hookean path, the autotuned lanes per vertex (1, 4, 8
+// or 16; one lane runs the sequential kernel).
 void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t n = p.colored ? p.color_end - p.color_begin : p.vertex_count;
+  if (!p.corotated && p.solve_lanes > 1u) {
+    const uint32_t block = (p.solve_block == 64u || p.solve_block == 128u) ? p.solve_block : kBlock;
+    const uint32_t lanes = p.solve_lanes;
+    const dim3 grid((n * lanes + block - 1) / block);
+    if (lanes == 4u) JGS2_LANES_DISPATCH(4, grid, dim3(block));
+    else if (lanes == 8u) JGS2_LANES_DISPATCH(8, grid, dim3(block));
+    else JGS2_LANES_DISPATCH(16, grid, dim3(block));
+    return;
+  }
   if (n == 0) return;
   const uint32_t block = (p.solve_block == 64u || p.solve_block == 128u) ? p.solve_block : kBlock;
   JGS2_FEATURE_DISPATCH_BLOCK(k_solve, dim3((n + block - 1) / block), dim3(block), s, p);
@@ -2536,7 +3186,21 @@ void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream
 void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t vb = blocks_for(p.vertex_count);
   const bool cl = has_cloth(p), ipc = p.contact != 0u;
-  if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient<true, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
+  if (p.solve_lanes == 4u || p.solve_lanes == 8u || p.solve_lanes == 16u) {
+    // The convergence gather splits a vertex over at most 8 lanes.
+    const uint32_t lanes = p.solve_lanes >= 8u ? 8u : 4u;
+    if (lanes == 8u) {
+      if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, true, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else if (cl) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, false, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else if (ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<false, true, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else hipLaunchKernelGGL((k_convergence_gradient_lanes<false, false, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
+    } else {
+      if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, true, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else if (cl) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, false, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else if (ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<false, true, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else hipLaunchKernelGGL((k_convergence_gradient_lanes<false, false, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
+    }
+  } else if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient<true, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
   else if (cl) hipLaunchKernelGGL((k_convergence_gradient<true, false>), dim3(vb), dim3(kBlock), 0, st, s, p);
   else if (ipc) hipLaunchKernelGGL((k_convergence_gradient<false, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
   else hipLaunchKernelGGL((k_convergence_gradient<false, false>), dim3(vb), dim3(kBlock), 0, st, s, p);

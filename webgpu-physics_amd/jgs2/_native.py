@@ -199,7 +199,7 @@ class BatchInfo(C.Structure):
         ("solve_block", C.c_uint32),
         ("xcd_remap", C.c_uint32),
         ("use_graphs", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("solve_lanes", C.c_uint32),
         ("vertex_capacity", C.c_uint64),
         ("vertex_blocks_used", C.c_uint64),
         ("vertex_extent", C.c_uint64),
@@ -253,7 +253,7 @@ SIGNATURES = {
     "jgs2_batch_sync": (C.c_int, [C.c_void_p]),
     "jgs2_batch_get_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
     "jgs2_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
-    "jgs2_batch_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "jgs2_batch_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     "jgs2_batch_autotune": (
         C.c_int, [C.c_void_p, C.POINTER(Settings), C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "jgs2_batch_profile_begin": (C.c_int, [C.c_void_p]),

@@ -169,13 +169,14 @@ class JGS2Batch:
     def info(self) -> dict:
         i = N.BatchInfo()
         N.check(N.lib.jgs2_batch_get_info(self._h(), C.byref(i)))
-        return {name: getattr(i, name) for name, _ in N.BatchInfo._fields_ if name != "reserved"}
+        return {name: getattr(i, name) for name, _ in N.BatchInfo._fields_}
 
     def use_graphs(self, enable: bool = True) -> None:
         N.check(N.lib.jgs2_batch_use_graphs(self._h(), 1 if enable else 0))
 
-    def set_launch(self, solve_block: int, xcd_remap: bool) -> None:
-        N.check(N.lib.jgs2_batch_set_launch(self._h(), int(solve_block), 1 if xcd_remap else 0))
+    def set_launch(self, solve_block: int, xcd_remap: bool, solve_lanes: Optional[int] = None) -> None:
+        lanes = self.info()["solve_lanes"] if solve_lanes is None else int(solve_lanes)
+        N.check(N.lib.jgs2_batch_set_launch(self._h(), int(solve_block), 1 if xcd_remap else 0, lanes))
 
     def autotune(self, overrides: Optional[Mapping] = None, repeats: int = 20) -> float:
         """Keeps the fastest local-solve launch shape; returns its ms."""
