@@ -151,7 +151,10 @@ void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t 
 void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_reduce_candidate(const DeviceState& s, const PassParams& p, hipStream_t stream);
-void launch_reduce_candidate_scene(const DeviceState& s, const PassParams& p, uint32_t scene, hipStream_t stream);
+// The same reduction over one scene's blocks only ({vb0, vb1, tb0, tb1},
+// {rb0, rb1, cb0, cb1}), for its admission preflight.
+void launch_reduce_candidate_scene(const DeviceState& s, const PassParams& p, uint32_t scene, uint4 blk0, uint4 blk1,
+                                   hipStream_t stream);
 void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t stream);

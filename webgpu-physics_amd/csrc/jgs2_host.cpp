@@ -1538,7 +1538,11 @@ jgs2_status place_scene(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_input& i
     PassParams p = make_params(b, b->defaults, false);
     p.src = 0;
     p.tgt = 0;
-    launch_reduce_candidate_scene(b->dev, p, s->slot, b->stream);
+    const uint4 blk0 = make_uint4(uint32_t(s->vblk.at), uint32_t(s->vblk.at + s->vblk.n), uint32_t(s->tblk.at),
+                                  uint32_t(s->tblk.at + s->tblk.n));
+    const uint4 blk1 = make_uint4(uint32_t(s->rblk.at), uint32_t(s->rblk.at + s->rblk.n), uint32_t(s->cblk.at),
+                                  uint32_t(s->cblk.at + s->cblk.n));
+    launch_reduce_candidate_scene(b->dev, p, s->slot, blk0, blk1, b->stream);
     float control[4] = {0, 0, 0, 0};
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)

@@ -1844,6 +1844,35 @@ __device__ Trial trial_lanes(const DeviceState& s, const PassParams& p, uint32_t
   return Trial{min_det, true, energy};
 }
 
+// vertexPolarRotation with LPV lanes per vertex splitting the incident
+// tetrahedra of the volume-weighted average.
+template <int LPV>
+__global__ void __launch_bounds__(kBlock) k_vertex_polar_lanes(DeviceState s, PassParams p) {
+  const uint32_t gid = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
+  const uint32_t v = gid / LPV, sub = gid % LPV;
+  if (v >= p.vertex_count) return;
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || stop_latched(s, p, sc)) return;
+  if (p.colored && p.color_index >= s.scene_cnt[sc].z) return;
+  const uint4 inf = s.vtx_info[v];
+  const f3 x = load_pos(s.pos[p.src], v);
+  m3 avg = mzero();
+  float tv = 0.0f;
+#pragma unroll 1
+  for (uint32_t k = sub; k < inf.y; k += LPV) {
+    const Tet e = load_tet(s, s.adjacency[inf.x + k]);
+    avg = madd(avg, mscale(e.vol, deformation_at(e, s.pos[p.src], v, x)));
+    tv += e.vol;
+  }
+  avg = {lanes_sum3<LPV>(avg.c0), lanes_sum3<LPV>(avg.c1), lanes_sum3<LPV>(avg.c2)};
+  tv = lanes_sum<LPV>(tv);
+  if (sub != 0) return;
+  m3 R = mid();
+  if (inf.y != 0u) R = polar_rotation(tv > 0.0f ? mscale(1.0f / tv, avg) : mid(), p.polar_eps);
+  store_rot(s.vrot, v, R);
+  store_vcub(s, p, s.vcub[0], v, inf, R, x);
+}
+
 // jgs2Solve (S:2992-3487) with LPV lanes per vertex (Jacobi: every vertex;
 // graph-colored Gauss-Seidel: the vertices of the active color, in place).
 template <bool kCloth, bool kIpc, int LPV>
@@ -2250,11 +2279,19 @@ __device__ __forceinline__ bool local_status_failure(uint32_t st) {  // S:3603-3
 // candidate reduction; blocks past the tetrahedra reduce the vertex records.
 // Blocks [0, tb) cover tetrahedra, [tb, tb + rb) cloth triangles and the
 // rest vertices; each block lies inside one scene's range.
-__global__ void __launch_bounds__(kBlock) k_candidate_partials(DeviceState s, PassParams p, uint32_t tb, uint32_t rb) {
+// `win` = {first tetrahedron block, count, first triangle block, count} and
+// `v0` the first vertex block of the blocks this launch covers (the whole
+// arena, or one scene's ranges for its admission preflight); partials land
+// at their arena-wide positions either way.
+__global__ void __launch_bounds__(kBlock) k_candidate_partials(DeviceState s, PassParams p, uint32_t tb, uint32_t rb,
+                                                               uint4 win, uint32_t v0) {
   __shared__ CandPartial sh[kBlock];
   CandPartial r;
   cand_identity(r);
-  const uint32_t b = blockIdx.x;
+  uint32_t b = blockIdx.x;
+  if (b < win.y) b = win.x + b;
+  else if (b < win.y + win.w) b = tb + win.z + (b - win.y);
+  else b = tb + rb + v0 + (b - win.y - win.w);
   uint32_t sc, base, kind;
   if (b < tb) {
     sc = s.tblock_scene[b];
@@ -2609,10 +2646,44 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient_lanes(DeviceSta
           tv = lanes_sum<LPV>(tv);
         }
       }
-      if (sub == 0) {
-        if constexpr (kCloth) {
-          if (!pinned) mat = mat + cloth_gradient(s, pos, v);
+      if constexpr (kCloth) {  // membrane and bending terms, split over the lanes too
+        f3 cg = mk(0, 0, 0);
+        if (!pinned) {
+          const uint2 trr = s.tri_row[v], hr = s.hinge_row[v];
+#pragma unroll 1
+          for (uint32_t k = lane_first<LPV>(sub, 0); k < trr.y; k += LPV) {
+            const uint32_t tri = s.tri_elems[trr.x + k];
+            const uint32_t slot = tri_slot(s.tri_ix[tri], v);
+            if (slot < 3) cg = cg + membrane_gradient(s, tri, membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)), slot);
+          }
+#pragma unroll 1
+          for (uint32_t k = lane_first<LPV>(sub, trr.y); k < hr.y; k += LPV) {
+            const uint32_t h = s.hinge_elems[hr.x + k];
+            const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
+            if (slot >= 4) continue;
+            const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+            if (!d.valid) {
+              cg = cg + mk(kF32Max, kF32Max, kF32Max);
+            } else {
+              const float4 r = s.hinge_rest[h];
+              cg = cg + (s.cloth_bend * r.y * wrapped_angle_difference(d.angle, r.x)) * d.grad;
+            }
+          }
+          cg = lanes_sum3<LPV>(cg);
+          mat = mat + cg;
         }
+      }
+      f3 ig = mk(0, 0, 0);
+      if constexpr (kIpc) {  // S:4091, the active rows split over the lanes
+        if (!pinned) {
+          m3 ih = mzero();
+          const uint32_t begin = s.cand_row[v].x, n = s.active_count[v];
+#pragma unroll 1
+          for (uint32_t k = sub; k < n; k += LPV) ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], ig, ih);
+          ig = lanes_sum3<LPV>(ig);
+        }
+      }
+      if (sub == 0) {
         if (p.compute_next_rot) {
           m3 R = mid();
           if (inf.y != 0u) R = polar_rotation(tv > 0.0f ? mscale(1.0f / tv, avg) : mid(), p.polar_eps);
@@ -2635,12 +2706,7 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient_lanes(DeviceSta
           }
           comp5[4] = mk(0, 0, 0);
           if (p.floor_k > 0.0f && x.y < p.floor_y) comp5[4].y = p.floor_k * (x.y - p.floor_y);
-          if constexpr (kIpc) {  // S:4091
-            f3 ig = mk(0, 0, 0);
-            m3 ih = mzero();
-            ipc_local_contribution(s, p, v, pos, ig, ih);
-            comp5[4] = comp5[4] + ig;
-          }
+          if constexpr (kIpc) comp5[4] = comp5[4] + ig;
         }
         for (int k = 0; k < 5; ++k) c[k] = make_float4(comp5[k].x, comp5[k].y, comp5[k].z, 0.0f);
       }
@@ -3110,6 +3176,10 @@ void launch_tet_polar(const DeviceState& s, const PassParams& p, hipStream_t st)
   hipLaunchKernelGGL(k_tet_polar, dim3(blocks_for(p.tet_count)), dim3(kBlock), 0, st, s, p);
 }
 void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t st) {
+  if (!p.corotated && p.solve_lanes >= 4u) {
+    hipLaunchKernelGGL(k_vertex_polar_lanes<4>, dim3(blocks_for(4u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
+    return;
+  }
   hipLaunchKernelGGL(k_vertex_polar, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
 }
 // Kernel variant for the step's features: co-rotated (no cloth or contact
@@ -3172,12 +3242,16 @@ void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t
 }
 void launch_reduce_candidate(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t tb = blocks_for(p.tet_count), rb = blocks_for(p.tri_count), vb = blocks_for(p.vertex_count);
-  hipLaunchKernelGGL(k_candidate_partials, dim3(tb + rb + vb), dim3(kBlock), 0, st, s, p, tb, rb);
+  hipLaunchKernelGGL(k_candidate_partials, dim3(tb + rb + vb), dim3(kBlock), 0, st, s, p, tb, rb,
+                     make_uint4(0, tb, 0, rb), 0u);
   hipLaunchKernelGGL(k_candidate_final, dim3(p.scene_count), dim3(kBlock), 0, st, s, p, tb, rb, 0u);
 }
-void launch_reduce_candidate_scene(const DeviceState& s, const PassParams& p, uint32_t scene, hipStream_t st) {
-  const uint32_t tb = blocks_for(p.tet_count), rb = blocks_for(p.tri_count), vb = blocks_for(p.vertex_count);
-  hipLaunchKernelGGL(k_candidate_partials, dim3(tb + rb + vb), dim3(kBlock), 0, st, s, p, tb, rb);
+void launch_reduce_candidate_scene(const DeviceState& s, const PassParams& p, uint32_t scene, uint4 blk0, uint4 blk1,
+                                   hipStream_t st) {
+  const uint32_t tb = blocks_for(p.tet_count), rb = blocks_for(p.tri_count);
+  const uint32_t nt = blk0.w - blk0.z, nr = blk1.y - blk1.x, nv = blk0.y - blk0.x;
+  hipLaunchKernelGGL(k_candidate_partials, dim3(nt + nr + nv), dim3(kBlock), 0, st, s, p, tb, rb,
+                     make_uint4(blk0.z, nt, blk1.x, nr), blk0.x);
   hipLaunchKernelGGL(k_candidate_final, dim3(1), dim3(kBlock), 0, st, s, p, tb, rb, scene);
 }
 void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream_t st) {
