@@ -63,7 +63,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=2)  # at least 1: the autotuner runs after the first
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--seed", type=int, default=20251203)
     ap.add_argument("--iterations", type=int, default=8)
@@ -73,10 +73,16 @@ def main(argv=None):
     ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--large-cells", default="64,64,128")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a cold 1 GiB device buffer (the counter calibration kernel) and exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args(argv)
 
+    args.warmup = max(1, args.warmup)
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every leg runs on one GPU).",
               file=sys.stderr)
@@ -109,13 +115,15 @@ def main(argv=None):
     batch.use_graphs(not args.no_graphs)
     info = batch.info()
     log(f"batch: {info['scene_count']} scenes admitted in {time.time() - t0:.1f}s, arena {info['arena_bytes'] / 1e9:.2f} GB")
-    autotune_ms = batch.autotune(dict(settings), repeats=5)
-    info = batch.info()
-    log(f"autotune: block {info['solve_block']} xcd_remap {info['xcd_remap']} ({autotune_ms:.3f} ms/launch)")
     for w in range(args.warmup):
         batch.step_frames_exact_iterations(1, it, settings)
         batch.sync()
         log(f"warmup frame {w + 1}/{args.warmup} done")
+        if w == 0:
+            autotune_ms = batch.autotune(dict(settings), repeats=5)
+            info = batch.info()
+            log(f"autotune: block {info['solve_block']} xcd_remap {info['xcd_remap']} ({autotune_ms:.3f} ms/launch)")
+    info = batch.info()
     batch.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -141,7 +149,7 @@ def main(argv=None):
     batch.destroy()
 
     # ---------------------------------------------------------- plain
-    plain = measure_plain(inputs, settings, it, args.plain_steps)
+    plain = None if args.no_plain else measure_plain(inputs, settings, it, args.plain_steps)
 
     result = {
         "metric": METRIC,
@@ -175,7 +183,7 @@ def main(argv=None):
         },
         "frame_ms": {"p50": nearest_rank(lat, 50), "p99": nearest_rank(lat, 99), "frames": len(lat)},
         "plain": plain,
-        "batched_over_plain": value / plain["vertex_updates_per_s"] if plain["vertex_updates_per_s"] > 0 else None,
+        "batched_over_plain": value / plain["vertex_updates_per_s"] if plain and plain["vertex_updates_per_s"] > 0 else None,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_solve (local solve)",
@@ -203,6 +211,24 @@ def main(argv=None):
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(it, args.seed)
     print(json.dumps(result), flush=True)
+    return 0
+
+
+def copy_calibration():
+    """A plain copy of a cold 1 GiB device buffer into another (twice, so
+    the buffers exceed every cache): the known-size kernel that scales the
+    FETCH_SIZE and WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = 1 << 28  # 2^28 floats = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    c.copy_(b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 2}), flush=True)
     return 0
 
 

@@ -343,14 +343,15 @@ jgs2_status jgs2_batch_get_info(jgs2_batch* batch, jgs2_batch_info* out);
 jgs2_status jgs2_batch_use_graphs(jgs2_batch* batch, int enable);
 /* Launch shape of the local solve: 64, 128 or 256 threads per block, the
  * XCD-aware block order on or off, and 1, 4, 8 or 16 lanes per vertex
- * (default 8). Results do not depend on the block size or block order;
- * the lane count sets the summation order of each vertex's gathered terms,
- * so results agree across lane counts to FP32 rounding, not bit for bit. */
+ * (default 4). Results do not depend on the block size or block order; the
+ * lane count sets the summation order of each vertex's gathered terms, so
+ * results agree across lane counts to FP32 rounding, not bit for bit. */
 jgs2_status jgs2_batch_set_launch(jgs2_batch* batch, uint32_t solve_block, uint32_t xcd_remap,
                                   uint32_t solve_lanes);
 /* Times the local solve over the resident scenes under every block size and
  * block order (`repeats` launches each, lanes per vertex unchanged) and
- * keeps the fastest; results are unchanged. */
+ * keeps the fastest; results are unchanged. Needs a frame stepped since the
+ * last admission or retirement (JGS2_ERROR_STATE otherwise). */
 jgs2_status jgs2_batch_autotune(jgs2_batch* batch, const jgs2_settings* overrides, uint64_t override_mask,
                                 uint32_t repeats, double* best_ms_per_launch);
 jgs2_status jgs2_batch_profile_begin(jgs2_batch* batch);

@@ -171,13 +171,17 @@ def test_launch_shapes_and_autotune_do_not_change_results():
         batch = JGS2Batch.create(capacity_for(inputs), SETTINGS)
         members = [batch.admit(x) for x in inputs]
         if shape is None:
+            with pytest.raises(jgs2.JGS2StateError, match="at least one frame"):
+                batch.autotune(settings, repeats=5)
+            step_batch(batch, 1, settings, True)
             ms = batch.autotune(settings, repeats=5)
             info = batch.info()
             print(f"autotune: block {info['solve_block']} remap {info['xcd_remap']} {ms:.4f} ms", flush=True)
             assert info["solve_block"] in (64, 128, 256) and ms > 0
         else:
             batch.set_launch(*shape)
-        step_batch(batch, 3, settings, True)
+            step_batch(batch, 1, settings, True)
+        step_batch(batch, 2, settings, True)
         results.append([m.read_positions() for m in members])
         batch.destroy()
     for r in results[1:]:

@@ -34,8 +34,9 @@ struct PassParams {
   uint32_t scene_count;         // scene slots in use
   uint32_t xcd_remap;           // XCD-aware block order for vertex kernels (autotuned)
   uint32_t color_index;         // colored pass: the color being swept
-  uint32_t solve_lanes;         // lanes per vertex of the local solve (autotuned: 1, 4, 8, 16)
-  uint32_t pad6, pad7, pad8;
+  uint32_t solve_lanes;         // lanes per vertex of the local solve for scenes that use lanes (4, 8, 16)
+  uint32_t solve_classes;       // bit 0: scenes solved with lanes resident; bit 1: scenes solved one lane per vertex
+  uint32_t pad7, pad8;
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
@@ -135,6 +136,7 @@ struct DeviceState {
   uint4* scene_cnt = nullptr;   // {V, T, schedule colors (0 = free slot), bodies}
   uint4* scene_end = nullptr;   // absolute ends {vertex, tet, triangle, candidate}
   float* scene_scale = nullptr;
+  uint32_t* scene_lanes = nullptr;  // lanes per vertex of the scene's local solve (1 = one thread per vertex)
   void* scene_conv = nullptr;   // per-scene convergence scale records
   uint4* body_rng = nullptr;    // per body {scene vertex begin, end, 0, 0}
 };

@@ -172,13 +172,15 @@ struct jgs2_batch {
   RangeAlloc va, ta, ra, ca, adja, hina, tela, hela, cela, bodya, slota;
   std::vector<jgs2_solver*> slots;
   uint64_t version = 0;      // bumps on every admission or retirement
+  uint64_t stepped_version = ~0ull;  // the version the last submitted frame ran at
   uint64_t synced_version = ~0ull;
   uint32_t color_count = 1;
   std::vector<uint32_t> color_offsets;
   // Launch shape of the local solve. Block size and XCD remap leave every
   // result unchanged (autotuned); lanes per vertex set the summation order
   // of the gathered terms and stay fixed unless a caller sets them.
-  uint32_t solve_block = 256, xcd_remap = 1, solve_lanes = 8;
+  uint32_t solve_block = 256, xcd_remap = 1, solve_lanes = 4;
+  bool solve_serial_only = false;  // every scene one thread per vertex
   uint64_t arena_bytes = 0;
   // Cloth material shared by every resident cloth scene (one set of
   // material words per arena).
@@ -209,7 +211,7 @@ void free_device(DeviceState& d) {
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.cand, d.lag_n, d.lag_w,
                   d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.vblock_scene, d.tblock_scene, d.rblock_scene,
-                  d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_conv,
+                  d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
                   d.body_rng};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -672,6 +674,16 @@ void morton_orders(const jgs2_scene_input& in, std::vector<uint32_t>& vnew, std:
 // ---------------------------------------------------------------- batch arena
 uint64_t blocks_of(uint64_t n) { return (n + kSceneBlock - 1) / kSceneBlock; }
 
+// Lanes per vertex of a scene's local solve: the batch's lane count, or one
+// thread per vertex on the co-rotated path or when the caller asks for it.
+// Kept per scene so a rule that depends on the scene alone (never on the
+// rest of the batch) can choose it; its results then do not depend on the
+// batch it is in.
+uint32_t scene_lanes(const jgs2_batch* b, const jgs2_solver* s) {
+  (void)s;
+  return (b->corotated || b->solve_serial_only) ? 1u : b->solve_lanes;
+}
+
 
 // Allocates every arena array for the configured capacities.
 jgs2_status batch_allocate(jgs2_batch* b) {
@@ -736,6 +748,7 @@ jgs2_status batch_allocate(jgs2_batch* b) {
   TRYS(dalloc(&d.history, size_t(kHistoryCapacity) * 8 * S, by));
   TRYS(dalloc(&d.contact_ctl, S, by));
   TRYS(dalloc(&d.scene_scale, S, by));
+  TRYS(dalloc(&d.scene_lanes, S, by));
   TRYS(dalloc(reinterpret_cast<char**>(&d.scene_conv), scene_conv_bytes(uint32_t(S)), by));
   TRYS(dalloc(&d.scene_blk0, S, by));
   TRYS(dalloc(&d.scene_blk1, S, by));
@@ -1023,6 +1036,8 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
   HIP_CHECK(hipMemcpyAsync(d.scene_blk1 + sl, &blk1, sizeof(uint4), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.scene_end + sl, &end, sizeof(uint4), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.scene_scale + sl, &s->scene_scale, sizeof(float), hipMemcpyHostToDevice, q));
+  const uint32_t lanes = scene_lanes(b, s);
+  HIP_CHECK(hipMemcpyAsync(d.scene_lanes + sl, &lanes, sizeof(uint32_t), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.contact_ctl + sl, &ctl, sizeof(float4), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.control + 4 * size_t(sl), zero_ctl.data(), 4 * sizeof(float4), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.history + size_t(sl) * kHistoryCapacity * 8, zero_hist.data(),
@@ -1055,6 +1070,8 @@ PassParams make_params(jgs2_batch* b, const jgs2_settings& st, bool stop) {
   p.contact = (b->dev.NC > 0 && st.ipc_barrier_stiffness > 0) ? 1u : 0u;
   p.solve_block = b->solve_block;
   p.solve_lanes = b->solve_lanes;
+  for (auto* s : b->slots)
+    if (s) p.solve_classes |= scene_lanes(b, s) > 1u ? 1u : 2u;
   p.xcd_remap = b->xcd_remap;
   // packUniforms (jgs2-solver.ts:604-730): f64 expressions rounded to f32.
   const double inv = 1.0 / st.timestep;
@@ -1282,6 +1299,7 @@ jgs2_status submit(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
     for (uint32_t f = 0; f < frames; ++f) enqueue_frame(b, st, iterations, stop);
   }
   HIP_CHECK(hipGetLastError());
+  b->stepped_version = b->version;
   clear_error();
   return JGS2_OK;
 }
@@ -2173,7 +2191,7 @@ jgs2_status jgs2_batch_get_info(jgs2_batch* b, jgs2_batch_info* out) {
   out->scene_count = b->active_scenes();
   out->max_scenes = uint32_t(b->slots.size());
   out->solve_block = b->solve_block;
-  out->solve_lanes = b->solve_lanes;
+  out->solve_lanes = b->solve_serial_only ? 1u : b->solve_lanes;
   out->xcd_remap = b->xcd_remap;
   out->use_graphs = b->use_graphs ? 1u : 0u;
   uint64_t used = 0;
@@ -2229,7 +2247,14 @@ jgs2_status jgs2_batch_set_launch(jgs2_batch* b, uint32_t solve_block, uint32_t 
   if (solve_lanes != 1 && solve_lanes != 4 && solve_lanes != 8 && solve_lanes != 16)
     return fail(JGS2_ERROR_RANGE, "solve_lanes must be 1, 4, 8 or 16; got %u.", solve_lanes);
   b->solve_block = solve_block;
-  b->solve_lanes = solve_lanes;
+  b->solve_lanes = solve_lanes == 1 ? b->solve_lanes : solve_lanes;
+  b->solve_serial_only = solve_lanes == 1;
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  for (auto* s : b->slots)
+    if (s) {
+      const uint32_t lanes = scene_lanes(b, s);
+      HIP_CHECK(hipMemcpy(b->dev.scene_lanes + s->slot, &lanes, sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
   b->xcd_remap = xcd_remap ? 1u : 0u;
   b->graph_valid = false;
   clear_error();
@@ -2240,7 +2265,11 @@ jgs2_status jgs2_batch_autotune(jgs2_batch* b, const jgs2_settings* ov, uint64_t
                                 double* best_ms) {
   TRYS(check_batch(b));
   if (b->active_scenes() == 0) return fail(JGS2_ERROR_STATE, "The batch holds no scene.");
-  if (repeats < 1 || repeats > 1000) return fail(JGS2_ERROR_RANGE, "repeats must be in [1, 1000]; got %u.", repeats);
+  if (repeats < 1 || repeats > 64) return fail(JGS2_ERROR_RANGE, "repeats must be in [1, 64]; got %u.", repeats);
+  // The timed solve starts from the last frame's pose and line-search
+  // control; before any frame it would time the early-exit path only.
+  if (b->stepped_version != b->version)
+    return fail(JGS2_ERROR_STATE, "Step at least one frame after the last admission or retirement before autotuning.");
   jgs2_settings r;
   TRYS(resolve_batch_step(b, ov, mask, "autotune", r));
   // Times the Jacobi local solve alone from the accepted pose into the
@@ -2249,39 +2278,53 @@ jgs2_status jgs2_batch_autotune(jgs2_batch* b, const jgs2_settings* ov, uint64_t
   // diagnostics) are rewritten by the next frame before they are read, so
   // stepping is unaffected; only jgs2_read_local_diagnostics() shows the
   // timing passes until the next frame.
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIP_CHECK(hipEventCreate(&e0));
-  HIP_CHECK(hipEventCreate(&e1));
+  // The six shapes are timed round robin, one launch each per round, so
+  // every shape sees the same state and clock; each keeps its median.
+  constexpr uint32_t kShapes = 6;
   const uint32_t blocks[3] = {64, 128, 256};
-  double best = INFINITY;
-  uint32_t best_block = b->solve_block, best_remap = b->xcd_remap;
+  std::vector<hipEvent_t> ev(2 * size_t(kShapes) * repeats, nullptr);
   hipError_t err = hipSuccess;
-  for (uint32_t remap = 0; remap < 2 && err == hipSuccess; ++remap)
-    for (uint32_t bi = 0; bi < 3 && err == hipSuccess; ++bi) {
-      b->solve_block = blocks[bi];
-      b->xcd_remap = remap;
+  for (auto& e : ev)
+    if (err == hipSuccess) err = hipEventCreate(&e);
+  const uint32_t keep_block = b->solve_block, keep_remap = b->xcd_remap;
+  for (uint32_t k = 0; k <= repeats && err == hipSuccess; ++k)  // round 0 warms every shape
+    for (uint32_t c = 0; c < kShapes; ++c) {
+      b->solve_block = blocks[c % 3];
+      b->xcd_remap = c / 3;
       PassParams p = make_params(b, r, false);
       p.src = 0;
       p.tgt = 1;
-      launch_solve_config(b->dev, p, b->stream);  // warm
-      (void)hipEventRecord(e0, b->stream);
-      for (uint32_t k = 0; k < repeats; ++k) launch_solve_config(b->dev, p, b->stream);
-      (void)hipEventRecord(e1, b->stream);
-      err = hipEventSynchronize(e1);
-      float ms = 0;
-      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-      const double per = double(ms) / repeats;
-      if (err == hipSuccess && per < best) {
-        best = per;
-        best_block = blocks[bi];
-        best_remap = remap;
+      if (k == 0) {
+        launch_solve_config(b->dev, p, b->stream);
+        continue;
       }
+      hipEvent_t* e = &ev[2 * (size_t(k - 1) * kShapes + c)];
+      (void)hipEventRecord(e[0], b->stream);
+      launch_solve_config(b->dev, p, b->stream);
+      (void)hipEventRecord(e[1], b->stream);
     }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (err == hipSuccess) err = hipStreamSynchronize(b->stream);
+  double best = INFINITY;
+  uint32_t best_block = keep_block, best_remap = keep_remap;
+  for (uint32_t c = 0; c < kShapes && err == hipSuccess; ++c) {
+    std::vector<float> t(repeats);
+    for (uint32_t k = 0; k < repeats && err == hipSuccess; ++k) {
+      const hipEvent_t* e = &ev[2 * (size_t(k) * kShapes + c)];
+      err = hipEventElapsedTime(&t[k], e[0], e[1]);
+    }
+    std::nth_element(t.begin(), t.begin() + repeats / 2, t.end());
+    const double med = t[repeats / 2];
+    if (err == hipSuccess && med < best) {
+      best = med;
+      best_block = blocks[c % 3];
+      best_remap = c / 3;
+    }
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (err == hipSuccess) err = hipGetLastError();
-  b->solve_block = best_block;
-  b->xcd_remap = best_remap;
+  b->solve_block = err == hipSuccess ? best_block : keep_block;
+  b->xcd_remap = err == hipSuccess ? best_remap : keep_remap;
   b->graph_valid = false;
   if (err != hipSuccess) return fail(JGS2_ERROR_DEVICE, "Autotuning failed: %s", hipGetErrorString(err));
   if (best_ms) *best_ms = best;

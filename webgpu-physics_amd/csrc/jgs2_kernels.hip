@@ -212,6 +212,10 @@ __device__ __forceinline__ VCub load_vcub(const float4* vc, uint32_t v) {
   o.w = o.pinned ? 0.0f : d.w;
   return o;
 }
+__device__ __forceinline__ f3 vcub_pose(const float4* vc, uint32_t v) {
+  const float4* r = vc + 4 * size_t(v);
+  return mk(r[0].w, r[1].w, r[2].w);
+}
 // Inertia weight of a vertex's distributed share (S:2401-2408).
 __device__ __forceinline__ float share_weight(const DeviceState& s, const PassParams& p, uint32_t v, uint4 inf) {
   const float n = float(inf.y > 1u ? inf.y : 1u);
@@ -1062,11 +1066,10 @@ __device__ __forceinline__ void cubature_record_snh(const DeviceState& s, const 
                                                     uint32_t t, const float* ub, const m3& Rs_t, float w, Local& L) {
   const Tet e = load_tet(s, t);
   const float4* vc = vcub_source(s, p);
-  // The sample tetrahedron's four vertex records: frame, pose, inertia.
-  VCub q[4];
-#pragma unroll
-  for (int l = 0; l < 4; ++l) q[l] = load_vcub(vc, tet_vertex(e.ix, l));
-  const m3 ds = {q[1].x - q[0].x, q[2].x - q[0].x, q[3].x - q[0].x};
+  // The sample tetrahedron's pose from its four vertex records (no local
+  // arrays: a runtime-indexed array would live in scratch memory).
+  const f3 x0 = vcub_pose(vc, e.ix.x), x1 = vcub_pose(vc, e.ix.y), x2 = vcub_pose(vc, e.ix.z), x3 = vcub_pose(vc, e.ix.w);
+  const m3 ds = {x1 - x0, x2 - x0, x3 - x0};
   const SnhPoint sp = snh_point(mul(ds, e.inv), e.lam, e.mu);
   f3 pg = mk(0, 0, 0);
   float i00 = 0, i11 = 0, i22 = 0, i01 = 0, i02 = 0, i12 = 0;
@@ -1076,15 +1079,16 @@ __device__ __forceinline__ void cubature_record_snh(const DeviceState& s, const 
   for (int l = 0; l < 4; ++l) {
     const uint32_t vl = tet_vertex(e.ix, l);
     if (vl == v) ss = l;
-    const m3 B = mul(mul(q[l].R, ubar_block(ub, l)), Rs_t);
+    const VCub ql = load_vcub(vc, vl);
+    const m3 B = mul(mul(ql.R, ubar_block(ub, l)), Rs_t);
     const f3 gr = shape_gradient(e, l);
-    const float wl = q[l].w;
-    f3 sg = e.vol * piola_times(sp, gr) + wl * q[l].diff;
+    const float wl = ql.w;
+    f3 sg = e.vol * piola_times(sp, gr) + wl * ql.diff;
     float wt = wl;
-    if (p.objective_flags != 0u && !q[l].pinned) {
+    if (p.objective_flags != 0u && !ql.pinned) {
       const uint4 il = s.vtx_info[vl];
       const float n = float(il.y > 1u ? il.y : 1u);
-      sg = sg + objective_gradient(s, p, vl, q[l].x) / n;
+      sg = sg + objective_gradient(s, p, vl, ql.x) / n;
       if (p.objective_flags & kTarget) wt += s.objectives[2 * size_t(vl) + 1].w / n;
     }
     pg = pg + tmul(B, sg);
@@ -1384,10 +1388,9 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
       if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
       const Tet e = load_tet(s, head.x);
       const float* ub = reinterpret_cast<const float*>(s.cub + 10 * rec + 1);
-      VCub q[4];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) q[l] = load_vcub(vc, tet_vertex(e.ix, l));
-      const m3 F = mul(m3{q[1].x - q[0].x, q[2].x - q[0].x, q[3].x - q[0].x}, e.inv);
+      const f3 x0 = vcub_pose(vc, e.ix.x), x1 = vcub_pose(vc, e.ix.y), x2 = vcub_pose(vc, e.ix.z),
+               x3 = vcub_pose(vc, e.ix.w);
+      const m3 F = mul(m3{x1 - x0, x2 - x0, x3 - x0}, e.inv);
       m3 Dd = mzero();
       float compl_e = 0.0f;
       int ss = 4;
@@ -1395,18 +1398,19 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
       for (int l = 0; l < 4; ++l) {
         const uint32_t vl = tet_vertex(e.ix, l);
         if (vl == v) ss = l;
-        const f3 bd = mul(q[l].R, ubar_times(ub, l, ds));  // B_l d
+        const VCub ql = load_vcub(vc, vl);
+        const f3 bd = mul(ql.R, ubar_times(ub, l, ds));  // B_l d
         const f3 gl = shape_gradient(e, l);
         Dd.c0 = Dd.c0 + gl.x * bd;
         Dd.c1 = Dd.c1 + gl.y * bd;
         Dd.c2 = Dd.c2 + gl.z * bd;
-        if (!q[l].pinned) {
+        if (!ql.pinned) {
           // quadratic_delta(x_l, x_pred_l, bd, w_l) from the stored x - x_pred.
-          compl_e += q[l].w * (dot(q[l].diff, bd) + 0.5f * dot(bd, bd));
+          compl_e += ql.w * (dot(ql.diff, bd) + 0.5f * dot(bd, bd));
           if (p.objective_flags != 0u) {
             const uint4 il = s.vtx_info[vl];
             const float n = float(il.y > 1u ? il.y : 1u);
-            compl_e += objective_delta(s, p, vl, q[l].x, bd) / n;
+            compl_e += objective_delta(s, p, vl, ql.x, bd) / n;
           }
         }
       }
@@ -1781,10 +1785,9 @@ __device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p
       if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
       const Tet e = load_tet(s, head.x);
       const float* ub = reinterpret_cast<const float*>(s.cub + 10 * rec + 1);
-      VCub q[4];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) q[l] = load_vcub(vc, tet_vertex(e.ix, l));
-      const m3 F = mul(m3{q[1].x - q[0].x, q[2].x - q[0].x, q[3].x - q[0].x}, e.inv);
+      const f3 x0 = vcub_pose(vc, e.ix.x), x1 = vcub_pose(vc, e.ix.y), x2 = vcub_pose(vc, e.ix.z),
+               x3 = vcub_pose(vc, e.ix.w);
+      const m3 F = mul(m3{x1 - x0, x2 - x0, x3 - x0}, e.inv);
       m3 Dd = mzero();
       float compl_e = 0.0f;
       int ss = 4;
@@ -1792,17 +1795,18 @@ __device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p
       for (int l = 0; l < 4; ++l) {
         const uint32_t vl = tet_vertex(e.ix, l);
         if (vl == v) ss = l;
-        const f3 bd = mul(q[l].R, ubar_times(ub, l, ds));  // B_l d
+        const VCub ql = load_vcub(vc, vl);
+        const f3 bd = mul(ql.R, ubar_times(ub, l, ds));  // B_l d
         const f3 gl = shape_gradient(e, l);
         Dd.c0 = Dd.c0 + gl.x * bd;
         Dd.c1 = Dd.c1 + gl.y * bd;
         Dd.c2 = Dd.c2 + gl.z * bd;
-        if (!q[l].pinned) {
-          compl_e += q[l].w * (dot(q[l].diff, bd) + 0.5f * dot(bd, bd));
+        if (!ql.pinned) {
+          compl_e += ql.w * (dot(ql.diff, bd) + 0.5f * dot(bd, bd));
           if (p.objective_flags != 0u) {
             const uint4 il = s.vtx_info[vl];
             const float n = float(il.y > 1u ? il.y : 1u);
-            compl_e += objective_delta(s, p, vl, q[l].x, bd) / n;
+            compl_e += objective_delta(s, p, vl, ql.x, bd) / n;
           }
         }
       }
@@ -1890,7 +1894,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_lanes(DeviceState s, PassParam
   }
   // Every branch below is uniform over a vertex's LPV lanes.
   const uint32_t sc = vscene(s, v);
-  if (sc == kNoScene) return;
+  if (sc == kNoScene || s.scene_lanes[sc] != uint32_t(LPV)) return;
   float4* tgt = s.pos[p.tgt];
   if (stop_latched(s, p, sc)) {
     if (sub == 0) tgt[v] = s.pos[p.src][v];
@@ -2039,7 +2043,7 @@ __global__ void __launch_bounds__(kBlock) k_solve(DeviceState s, PassParams p) {
     v = lane;
   }
   const uint32_t sc = vscene(s, v);
-  if (sc == kNoScene) return;
+  if (sc == kNoScene || (!kCorot && s.scene_lanes[sc] != 1u)) return;
   float4* tgt = s.pos[p.tgt];
   if (stop_latched(s, p, sc)) {
     tgt[v] = s.pos[p.src][v];
@@ -3216,20 +3220,22 @@ void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t 
 // The local solve runs with the autotuned block size (64, 128 or 256) and,
 // on the stable Neo-Hookean path, the autotuned lanes per vertex (1, 4, 8
 // or 16; one lane runs the sequential kernel).
+// Each scene is solved either with lanes (its scene_lanes entry equals
+// p.solve_lanes) or one thread per vertex; a launch per class present skips
+// the other class's scenes.
 void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t n = p.colored ? p.color_end - p.color_begin : p.vertex_count;
-  if (!p.corotated && p.solve_lanes > 1u) {
-    const uint32_t block = (p.solve_block == 64u || p.solve_block == 128u) ? p.solve_block : kBlock;
+  if (n == 0) return;
+  const uint32_t block = (p.solve_block == 64u || p.solve_block == 128u) ? p.solve_block : kBlock;
+  if (!p.corotated && (p.solve_classes & 1u) && p.solve_lanes > 1u) {
     const uint32_t lanes = p.solve_lanes;
     const dim3 grid((n * lanes + block - 1) / block);
     if (lanes == 4u) JGS2_LANES_DISPATCH(4, grid, dim3(block));
     else if (lanes == 8u) JGS2_LANES_DISPATCH(8, grid, dim3(block));
     else JGS2_LANES_DISPATCH(16, grid, dim3(block));
-    return;
   }
-  if (n == 0) return;
-  const uint32_t block = (p.solve_block == 64u || p.solve_block == 128u) ? p.solve_block : kBlock;
-  JGS2_FEATURE_DISPATCH_BLOCK(k_solve, dim3((n + block - 1) / block), dim3(block), s, p);
+  if (p.corotated || (p.solve_classes & 2u))
+    JGS2_FEATURE_DISPATCH_BLOCK(k_solve, dim3((n + block - 1) / block), dim3(block), s, p);
 }
 void launch_solve_config(const DeviceState& s, const PassParams& p, hipStream_t st) { launch_solve(s, p, st); }
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
