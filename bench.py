@@ -76,6 +76,9 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a cold 1 GiB device buffer (the counter calibration kernel) and exit")
+    ap.add_argument("--profile-large", default=None, metavar="LANES",
+                    help="only step the layered box two frames per listed lane count (e.g. 1,4,8,16) for a profiler, "
+                         "and exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args(argv)
@@ -83,6 +86,8 @@ def main(argv=None):
     args.warmup = max(1, args.warmup)
     if args.copy_calibration:
         return copy_calibration()
+    if args.profile_large:
+        return profile_large([int(x) for x in args.profile_large.split(",")], args.large_cells, args.iterations)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every leg runs on one GPU).",
               file=sys.stderr)
@@ -144,7 +149,13 @@ def main(argv=None):
     solve_ms, launches, _ = batch.profile_end()
     bytes_per_sweep = sum(scene_bytes_per_sweep(x) for x in inputs)
     achieved = bytes_per_sweep / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
+    traffic = measured_traffic(f"batched-uneven-scenes-{len(inputs)}", info["solve_block"])
+    # Host readback of every scene's positions (PCIe), once per frame if a
+    # caller wanted them all: the PCIe-inclusive rate next to `value`.
+    batch.sync()
+    t0 = time.perf_counter()
     finite = all(bool(np.isfinite(m.read_positions()).all()) for m in members)
+    readback_s = time.perf_counter() - t0
     reverted = sum(int(m.read_globalization_history().reshape(64, 8, 4)[:it, 1, 3].sum()) for m in members)
     batch.destroy()
 
@@ -191,12 +202,14 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": bytes_per_sweep,
             "ms_per_launch": solve_ms,
             "launches_profiled": launches,
         },
         "positions_finite": finite,
+        "readback_ms_per_frame": readback_s * 1e3,
+        "pcie_inclusive_vertex_updates_per_s": it * free / (elapsed / args.steps + readback_s),
         "reverted_iterations_last_frame_all_scenes": reverted,
     }
     del inputs, members
@@ -230,6 +243,38 @@ def copy_calibration():
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 2}), flush=True)
     return 0
+
+
+def profile_large(lanes_list, cells_arg, iterations):
+    """Two Jacobi frames of the layered box per lane count (profiler driver)."""
+    import jgs2
+    from jgs2 import scenes
+    from jgs2.batch import JGS2Batch
+
+    cells = tuple(int(c) for c in cells_arg.split(","))
+    scene = scenes.layered_box(cells=cells)
+    settings = dict(scenes.LAYERED_BOX_SETTINGS)
+    solver = jgs2.JGS2GpuSolver.create(scene, settings)
+    batch = JGS2Batch.of(solver)
+    for lanes in lanes_list:
+        batch.set_launch(256, True, lanes)
+        for f in range(2):
+            solver.step_exact_iterations(iterations, settings)
+            solver.await_idle()
+        log(f"lanes {lanes}: 2 frames done")
+    solver.destroy()
+    return 0
+
+
+def measured_traffic(workload, block):
+    """HBM bytes per local-solve launch of this workload and block size from
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarized in
+    profiles/traffic.json (null when not profiled)."""
+    path = REPO / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    entry = json.loads(path.read_text()).get(workload, {}).get(str(block))
+    return None if entry is None else entry["hbm_bytes_per_launch"]
 
 
 def _capacity(inputs, slack_scenes=0):
@@ -396,28 +441,35 @@ def accuracy_check(iterations, seed, frames=2):
             "within_tol": worst <= 1e-3}
 
 
-def cpu_baseline(iterations, seed, frames=1):
-    """The oracle's FP32 step (single thread) on the reduced mix."""
+def cpu_baseline(iterations, seed, frames=1, budget_s=10.0):
+    """The oracle's FP32 step (single thread) on scenes of the same
+    generator (up to 20,000 tetrahedra or 64x64 cloth cells each), one frame
+    per scene, until about `budget_s` seconds of CPU time."""
     from jgs2 import scenes
 
     sys.path.insert(0, str(REPO / "oracle"))
     import jgs2_oracle as O
 
     settings = dict(scenes.MIXED_SETTINGS)
-    inputs = [scenes.build_mixed_scene(s) for s in _sample_specs(seed)]
-    updates, dt = 0, 0.0
-    for x in inputs:
+    specs = scenes.mixed_scene_specs(count=64, seed=seed, tet_range=(200, 20000), cloth_cells=(8, 64))
+    updates, dt, used, verts = 0, 0.0, 0, 0
+    for spec in specs:
+        if dt >= budget_s:
+            break
+        x = scenes.build_mixed_scene(spec)
         ref = O.Oracle(x, 32)
         free = int((x.vertex_info.reshape(-1, 4)[:, 2] == 0).sum())
         t0 = time.perf_counter()
         ref.step_frames(frames, iterations, settings)
         dt += time.perf_counter() - t0
         updates += frames * iterations * free
+        used += 1
+        verts += x.vertex_count
     log(f"cpu baseline: {updates} vertex-updates in {dt:.2f}s")
     return {"value": updates / dt, "unit": "vertex-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP32 step, {len(inputs)} scenes of the reduced mixed generator "
-                      f"({sum(x.vertex_count for x in inputs)} vertices), {frames} frame x {iterations} Jacobi "
-                      f"sweeps, {dt:.2f}s"}
+            "sample": f"oracle FP32 step (one thread), the first {used} scenes of the mixed generator capped at "
+                      f"20,000 tetrahedra / 64x64 cloth cells ({verts} vertices), {frames} frame x {iterations} "
+                      f"Jacobi sweeps each, {dt:.1f}s"}
 
 
 if __name__ == "__main__":

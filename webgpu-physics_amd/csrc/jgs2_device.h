@@ -128,6 +128,7 @@ struct DeviceState {
   float4* contact_ctl = nullptr;
   uint2* cand_row = nullptr;
   uint32_t *cand_ids = nullptr, *active_count = nullptr, *active_ids = nullptr, *cand_meta = nullptr;
+  uint32_t* cand_keep = nullptr;  // per candidate: kept in its vertices' active rows this iteration
   // Scene tables.
   uint32_t S = 0;
   uint32_t *vblock_scene = nullptr, *tblock_scene = nullptr, *rblock_scene = nullptr, *cblock_scene = nullptr;
@@ -173,6 +174,8 @@ uint32_t partial_blocks_needed(uint32_t vertices, uint32_t tets, uint32_t triang
 size_t scene_conv_bytes(uint32_t scenes);
 // Local-solve timing of one launch configuration (autotuner).
 void launch_solve_config(const DeviceState& s, const PassParams& p, hipStream_t stream);
+// Reads `records` uint4 of the cubature array to evict the caches.
+void launch_cache_flush(const DeviceState& s, uint32_t records, hipStream_t stream);
 size_t partial_bytes(uint32_t blocks);
 
 }  // namespace jgs2

@@ -210,7 +210,7 @@ void free_device(DeviceState& d) {
                   d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.cand, d.lag_n, d.lag_w,
                   d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
-                  d.active_count, d.active_ids, d.cand_meta, d.vblock_scene, d.tblock_scene, d.rblock_scene,
+                  d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
                   d.body_rng};
   for (void* p : ptrs)
@@ -218,9 +218,14 @@ void free_device(DeviceState& d) {
   d = DeviceState{};
 }
 
-template <class T> jgs2_status dalloc(T** ptr, size_t count, uint64_t* bytes = nullptr) {
+// Allocates `count` elements (at least one); with `dry` only counts bytes.
+template <class T> jgs2_status dalloc(T** ptr, size_t count, uint64_t* bytes = nullptr, bool dry = false) {
   *ptr = nullptr;
   if (count == 0) count = 1;
+  if (dry) {
+    if (bytes) *bytes += count * sizeof(T);
+    return JGS2_OK;
+  }
   hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T));
   if (e != hipSuccess)
     return fail(JGS2_ERROR_MEMORY, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
@@ -686,7 +691,7 @@ uint32_t scene_lanes(const jgs2_batch* b, const jgs2_solver* s) {
 
 
 // Allocates every arena array for the configured capacities.
-jgs2_status batch_allocate(jgs2_batch* b) {
+jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   const jgs2_batch_config& c = b->cfg;
   DeviceState& d = b->dev;
   uint64_t* by = &b->arena_bytes;
@@ -701,65 +706,67 @@ jgs2_status batch_allocate(jgs2_batch* b) {
     return fail(JGS2_ERROR_RANGE, "Batch capacities exceed 32-bit indexing.");
   d.K = uint32_t(K);
   d.S = uint32_t(S);
-  TRYS(dalloc(&d.vtx_rest, Vc, by));
-  TRYS(dalloc(&d.vtx_info, Vc, by));
-  TRYS(dalloc(&d.objectives, 2 * Vc, by));
-  TRYS(dalloc(&d.color_order, Vc, by));
-  TRYS(dalloc(&d.colors, Vc, by));
-  TRYS(dalloc(&d.pos[0], Vc, by));
-  TRYS(dalloc(&d.pos[1], Vc, by));
-  TRYS(dalloc(&d.predicted, Vc, by));
-  TRYS(dalloc(&d.velocity, Vc, by));
-  TRYS(dalloc(&d.old, Vc, by));
-  TRYS(dalloc(&d.vrot, 3 * Vc, by));
-  TRYS(dalloc(&d.vrot_next, 3 * Vc, by));
-  TRYS(dalloc(&d.vcub[0], 4 * Vc, by));
-  TRYS(dalloc(&d.vcub[1], 4 * Vc, by));
-  TRYS(dalloc(&d.final_update, Vc, by));
-  TRYS(dalloc(&d.local_diag, 4 * Vc, by));
-  TRYS(dalloc(&d.vertex_energy, Vc, by));
-  TRYS(dalloc(&d.conv, 5 * Vc, by));
-  TRYS(dalloc(&d.tri_row, Vc, by));
-  TRYS(dalloc(&d.hinge_row, Vc, by));
-  TRYS(dalloc(&d.cand_row, Vc, by));
-  TRYS(dalloc(&d.active_count, Vc, by));
-  TRYS(dalloc(&d.cub, 10 * Vc * std::max<uint64_t>(K, 1), by));
-  TRYS(dalloc(&d.tet_rec, 4 * Tc, by));
-  if (b->corotated) TRYS(dalloc(&d.tet_k, 144 * Tc, by));
-  TRYS(dalloc(&d.trot, 3 * Tc, by));
-  TRYS(dalloc(&d.tet_glob, 2 * Tc, by));
-  TRYS(dalloc(&d.adjacency, 4 * Tc, by));
-  TRYS(dalloc(&d.tri_ix, Rc, by));
-  TRYS(dalloc(&d.tri_inv, Rc, by));
-  TRYS(dalloc(&d.tri_area, Rc, by));
-  TRYS(dalloc(&d.tri_glob, 2 * Rc, by));
-  TRYS(dalloc(&d.tri_elems, 3 * Rc, by));
-  TRYS(dalloc(&d.hinge_ix, Hc, by));
-  TRYS(dalloc(&d.hinge_rest, Hc, by));
-  TRYS(dalloc(&d.hinge_elems, 4 * Hc, by));
-  TRYS(dalloc(&d.cand, Cc, by));
-  TRYS(dalloc(&d.cand_meta, Cc, by));
-  for (float4** a : {&d.lag_n, &d.lag_w, &d.cscratch, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRYS(dalloc(a, Cc, by));
-  TRYS(dalloc(&d.cand_ids, 4 * Cc, by));
-  TRYS(dalloc(&d.active_ids, 4 * Cc, by));
-  TRYS(dalloc(&d.body_corr, Bc, by));
-  TRYS(dalloc(&d.body_rng, Bc, by));
-  TRYS(dalloc(&d.control, 4 * S, by));
-  TRYS(dalloc(&d.history, size_t(kHistoryCapacity) * 8 * S, by));
-  TRYS(dalloc(&d.contact_ctl, S, by));
-  TRYS(dalloc(&d.scene_scale, S, by));
-  TRYS(dalloc(&d.scene_lanes, S, by));
-  TRYS(dalloc(reinterpret_cast<char**>(&d.scene_conv), scene_conv_bytes(uint32_t(S)), by));
-  TRYS(dalloc(&d.scene_blk0, S, by));
-  TRYS(dalloc(&d.scene_blk1, S, by));
-  TRYS(dalloc(&d.scene_cnt, S, by));
-  TRYS(dalloc(&d.scene_end, S, by));
-  TRYS(dalloc(&d.vblock_scene, Vc / kSceneBlock, by));
-  TRYS(dalloc(&d.tblock_scene, Tc / kSceneBlock, by));
-  TRYS(dalloc(&d.rblock_scene, std::max<uint64_t>(1, Rc / kSceneBlock), by));
-  TRYS(dalloc(&d.cblock_scene, std::max<uint64_t>(1, Cc / kSceneBlock), by));
+  TRYS(dalloc(&d.vtx_rest, Vc, by, dry));
+  TRYS(dalloc(&d.vtx_info, Vc, by, dry));
+  TRYS(dalloc(&d.objectives, 2 * Vc, by, dry));
+  TRYS(dalloc(&d.color_order, Vc, by, dry));
+  TRYS(dalloc(&d.colors, Vc, by, dry));
+  TRYS(dalloc(&d.pos[0], Vc, by, dry));
+  TRYS(dalloc(&d.pos[1], Vc, by, dry));
+  TRYS(dalloc(&d.predicted, Vc, by, dry));
+  TRYS(dalloc(&d.velocity, Vc, by, dry));
+  TRYS(dalloc(&d.old, Vc, by, dry));
+  TRYS(dalloc(&d.vrot, 3 * Vc, by, dry));
+  TRYS(dalloc(&d.vrot_next, 3 * Vc, by, dry));
+  TRYS(dalloc(&d.vcub[0], 4 * Vc, by, dry));
+  TRYS(dalloc(&d.vcub[1], 4 * Vc, by, dry));
+  TRYS(dalloc(&d.final_update, Vc, by, dry));
+  TRYS(dalloc(&d.local_diag, 4 * Vc, by, dry));
+  TRYS(dalloc(&d.vertex_energy, Vc, by, dry));
+  TRYS(dalloc(&d.conv, 5 * Vc, by, dry));
+  TRYS(dalloc(&d.tri_row, Vc, by, dry));
+  TRYS(dalloc(&d.hinge_row, Vc, by, dry));
+  TRYS(dalloc(&d.cand_row, Vc, by, dry));
+  TRYS(dalloc(&d.active_count, Vc, by, dry));
+  TRYS(dalloc(&d.cub, 10 * Vc * std::max<uint64_t>(K, 1), by, dry));
+  TRYS(dalloc(&d.tet_rec, 4 * Tc, by, dry));
+  if (b->corotated) TRYS(dalloc(&d.tet_k, 144 * Tc, by, dry));
+  TRYS(dalloc(&d.trot, 3 * Tc, by, dry));
+  TRYS(dalloc(&d.tet_glob, 2 * Tc, by, dry));
+  TRYS(dalloc(&d.adjacency, 4 * Tc, by, dry));
+  TRYS(dalloc(&d.tri_ix, Rc, by, dry));
+  TRYS(dalloc(&d.tri_inv, Rc, by, dry));
+  TRYS(dalloc(&d.tri_area, Rc, by, dry));
+  TRYS(dalloc(&d.tri_glob, 2 * Rc, by, dry));
+  TRYS(dalloc(&d.tri_elems, 3 * Rc, by, dry));
+  TRYS(dalloc(&d.hinge_ix, Hc, by, dry));
+  TRYS(dalloc(&d.hinge_rest, Hc, by, dry));
+  TRYS(dalloc(&d.hinge_elems, 4 * Hc, by, dry));
+  TRYS(dalloc(&d.cand, Cc, by, dry));
+  TRYS(dalloc(&d.cand_meta, Cc, by, dry));
+  TRYS(dalloc(&d.cand_keep, Cc, by, dry));
+  for (float4** a : {&d.lag_n, &d.lag_w, &d.cscratch, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRYS(dalloc(a, Cc, by, dry));
+  TRYS(dalloc(&d.cand_ids, 4 * Cc, by, dry));
+  TRYS(dalloc(&d.active_ids, 4 * Cc, by, dry));
+  TRYS(dalloc(&d.body_corr, Bc, by, dry));
+  TRYS(dalloc(&d.body_rng, Bc, by, dry));
+  TRYS(dalloc(&d.control, 4 * S, by, dry));
+  TRYS(dalloc(&d.history, size_t(kHistoryCapacity) * 8 * S, by, dry));
+  TRYS(dalloc(&d.contact_ctl, S, by, dry));
+  TRYS(dalloc(&d.scene_scale, S, by, dry));
+  TRYS(dalloc(&d.scene_lanes, S, by, dry));
+  TRYS(dalloc(reinterpret_cast<char**>(&d.scene_conv), scene_conv_bytes(uint32_t(S)), by, dry));
+  TRYS(dalloc(&d.scene_blk0, S, by, dry));
+  TRYS(dalloc(&d.scene_blk1, S, by, dry));
+  TRYS(dalloc(&d.scene_cnt, S, by, dry));
+  TRYS(dalloc(&d.scene_end, S, by, dry));
+  TRYS(dalloc(&d.vblock_scene, Vc / kSceneBlock, by, dry));
+  TRYS(dalloc(&d.tblock_scene, Tc / kSceneBlock, by, dry));
+  TRYS(dalloc(&d.rblock_scene, std::max<uint64_t>(1, Rc / kSceneBlock), by, dry));
+  TRYS(dalloc(&d.cblock_scene, std::max<uint64_t>(1, Cc / kSceneBlock), by, dry));
   d.partial_capacity = partial_blocks_needed(uint32_t(Vc), uint32_t(Tc), uint32_t(Rc), uint32_t(Cc));
-  TRYS(dalloc(reinterpret_cast<char**>(&d.partials), partial_bytes(d.partial_capacity), by));
+  TRYS(dalloc(reinterpret_cast<char**>(&d.partials), partial_bytes(d.partial_capacity), by, dry));
+  if (dry) return JGS2_OK;
   // Free blocks belong to no scene; scene slots start inactive.
   hipStream_t q = b->stream;
   std::vector<uint32_t> none(std::max({Vc, Tc, Rc, Cc}) / kSceneBlock + 1, kNoScene);
@@ -2208,26 +2215,12 @@ jgs2_status jgs2_batch_get_info(jgs2_batch* b, jgs2_batch_info* out) {
 
 jgs2_status jgs2_batch_arena_bytes(const jgs2_batch_config* c, uint64_t* bytes) {
   if (!c || !bytes) return fail(JGS2_ERROR_TYPE, "NULL argument.");
-  // The allocation list of batch_allocate, in bytes.
-  const uint64_t Vc = std::max<uint64_t>(1, blocks_of(c->max_vertices)) * kSceneBlock;
-  const uint64_t Tc = std::max<uint64_t>(1, blocks_of(c->max_tets)) * kSceneBlock;
-  const uint64_t Rc = blocks_of(c->max_triangles) * kSceneBlock;
-  const uint64_t Cc = blocks_of(c->max_candidates) * kSceneBlock;
-  const uint64_t Hc = c->max_hinges, Bc = std::max<uint64_t>(1, c->max_bodies), S = std::max<uint32_t>(1, c->max_scenes);
-  const uint64_t K = std::max<uint64_t>(1, c->cubature_k), f4 = 16, u = 4;
-  auto nz = [](uint64_t n) { return std::max<uint64_t>(n, 1); };
-  uint64_t total = 0;
-  total += Vc * (f4 * (1 + 2 + 5 + 3 + 3 + 1 + 4 + 1 + 5) + 16 + u * 3 + 8 * 3 + u);  // vertex records
-  total += Vc * 10 * K * 16;                                                          // cubature
-  total += Tc * (16 + 3 * f4 + 3 * f4 + 2 * f4 + 4 * u) + (c->corotated ? Tc * 144 * 4 : 0);
-  total += nz(Rc) * (16 + f4 + 4 + 2 * f4) + nz(3 * Rc) * u;
-  total += nz(Hc) * (16 + f4) + nz(4 * Hc) * u;
-  total += nz(Cc) * (16 + u + 7 * f4) + 2 * nz(4 * Cc) * u;
-  total += Bc * (f4 + 16);
-  total += S * (4 * f4 + 64 * 8 * f4 + f4 + 4 + 4 * 16) + scene_conv_bytes(uint32_t(S));
-  total += (Vc + Tc + nz(Rc) + nz(Cc)) / kSceneBlock * u;
-  total += partial_bytes(partial_blocks_needed(uint32_t(Vc), uint32_t(Tc), uint32_t(Rc), uint32_t(Cc)));
-  *bytes = total;
+  // The allocation list of a batch of this configuration, counted only.
+  jgs2_batch probe;
+  probe.cfg = *c;
+  probe.corotated = c->corotated != 0;
+  TRYS(batch_allocate(&probe, true));
+  *bytes = probe.arena_bytes;
   clear_error();
   return JGS2_OK;
 }
@@ -2287,6 +2280,10 @@ jgs2_status jgs2_batch_autotune(jgs2_batch* b, const jgs2_settings* ov, uint64_t
   for (auto& e : ev)
     if (err == hipSuccess) err = hipEventCreate(&e);
   const uint32_t keep_block = b->solve_block, keep_remap = b->xcd_remap;
+  // Each timed launch follows a 512 MiB stream (past the 256 MB
+  // last-level cache) when the arena's cubature records are that large.
+  const uint64_t cub_records = uint64_t(b->va.capacity) * kSceneBlock * 10 * std::max<uint32_t>(b->cfg.cubature_k, 1);
+  const uint32_t flush_records = uint32_t(std::min<uint64_t>(cub_records, uint64_t(32) << 20));
   for (uint32_t k = 0; k <= repeats && err == hipSuccess; ++k)  // round 0 warms every shape
     for (uint32_t c = 0; c < kShapes; ++c) {
       b->solve_block = blocks[c % 3];
@@ -2299,6 +2296,7 @@ jgs2_status jgs2_batch_autotune(jgs2_batch* b, const jgs2_settings* ov, uint64_t
         continue;
       }
       hipEvent_t* e = &ev[2 * (size_t(k - 1) * kShapes + c)];
+      launch_cache_flush(b->dev, flush_records, b->stream);
       (void)hipEventRecord(e[0], b->stream);
       launch_solve_config(b->dev, p, b->stream);
       (void)hipEventRecord(e[1], b->stream);

@@ -3022,35 +3022,57 @@ __global__ void __launch_bounds__(kBlock) k_lag_contact(DeviceState s, PassParam
 }
 // buildActiveContactRows (S:2747-2785): keep candidates the bounded step can
 // bring into the barrier, with friction, or touched by unbounded pin motion.
-__global__ void __launch_bounds__(kBlock) k_build_active_rows(DeviceState s, PassParams p) {
-  const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
-  if (v >= p.vertex_count) return;
-  const uint32_t sc = vscene(s, v);
-  if (sc == kNoScene) return;
-  const uint2 row = s.cand_row[v];
-  const uint32_t first = row.x, last = row.x + row.y;
+// The predicate is evaluated once per candidate (not once per each of its
+// four vertices), then each vertex's row is compacted by 16 lanes with a
+// wave ballot; the active ids keep the row's ascending order.
+__global__ void __launch_bounds__(kBlock) k_candidate_keep(DeviceState s, PassParams p) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= s.NC) return;
+  const uint32_t sc = cscene(s, c);
+  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
   const bool bounded = p.max_step > 0.0f;
   const float reach = p.ipc_dhat + 2.0f * (fmaxf(p.max_step, 0.0f) + ipc_rounding_reach(p, s.scene_scale[sc]));
   const float bound = fmaxf(p.max_step, 0.0f);
-  const float4* src = s.pos[p.src];
-  uint32_t n = 0;
-  for (uint32_t i = first; i < last; ++i) {
-    const uint32_t c = s.cand_ids[i];
-    const Contact k = ipc_cached(s, c, false);
-    bool include = !k.valid || !bounded || k.distance <= reach || ipc_friction_active(s, c);
-    if (!include) {  // ipcCandidatePinnedMotionExceedsBound (S:484-507)
-      const uint4 ix = s.cand[c];
-      const uint32_t vs[4] = {ix.x, ix.y, ix.z, ix.w};
-      for (int l = 0; l < 4 && !include; ++l) {
-        if (s.vtx_info[vs[l]].z == 0u) continue;
-        const f3 d = xyz(s.vtx_rest[vs[l]]) - load_pos(src, vs[l]);
-        const float dl = sqrtf(dot(d, d));
-        include = !finite3(d) || !finite_s(dl) || dl > bound;
-      }
+  const Contact k = ipc_cached(s, c, false);
+  bool include = !k.valid || !bounded || k.distance <= reach || ipc_friction_active(s, c);
+  if (!include) {  // ipcCandidatePinnedMotionExceedsBound (S:484-507)
+    const float4* src = s.pos[p.src];
+    const uint4 ix = s.cand[c];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const uint32_t vl = tet_vertex(ix, l);
+      if (include || s.vtx_info[vl].z == 0u) continue;
+      const f3 d = xyz(s.vtx_rest[vl]) - load_pos(src, vl);
+      const float dl = sqrtf(dot(d, d));
+      include = !finite3(d) || !finite_s(dl) || dl > bound;
     }
-    if (include) s.active_ids[first + n++] = c;
   }
-  s.active_count[v] = n;
+  s.cand_keep[c] = include ? 1u : 0u;
+}
+__global__ void __launch_bounds__(kBlock) k_compact_active_rows(DeviceState s, PassParams p) {
+  constexpr uint32_t kLanes = 16;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t v = gid / kLanes, sub = gid % kLanes;
+  if (v >= p.vertex_count) return;  // uniform per 16-lane group
+  if (vscene(s, v) == kNoScene) return;
+  const uint2 row = s.cand_row[v];
+  const uint32_t first = row.x, last = row.x + row.y;
+  const uint32_t group = (threadIdx.x % 64u) & ~(kLanes - 1u);
+  uint32_t n = 0;
+  for (uint32_t base = first; base < last; base += kLanes) {  // same trip count on the group's lanes
+    const uint32_t i = base + sub;
+    uint32_t c = 0;
+    bool keep = false;
+    if (i < last) {
+      c = s.cand_ids[i];
+      keep = s.cand_keep[c] != 0u;
+    }
+    const uint64_t wave = __ballot(keep);
+    const uint32_t mine = uint32_t(wave >> group) & 0xffffu;
+    if (keep) s.active_ids[first + n + __popc(mine & ((1u << sub) - 1u))] = c;
+    n += __popc(mine);
+  }
+  if (sub == 0) s.active_count[v] = n;
 }
 // candidateContactStep (S:2788-2852): per-candidate conservative step cap.
 __global__ void __launch_bounds__(kBlock) k_candidate_contact_step(DeviceState s, PassParams p) {
@@ -3238,6 +3260,27 @@ void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
     JGS2_FEATURE_DISPATCH_BLOCK(k_solve, dim3((n + block - 1) / block), dim3(block), s, p);
 }
 void launch_solve_config(const DeviceState& s, const PassParams& p, hipStream_t st) { launch_solve(s, p, st); }
+// Streams `n` records of the cubature array through the caches (the timed
+// local solve of the autotuner then starts as cold as in a frame, where the
+// other passes have replaced the cache contents). The sum lands in the
+// scratch partials, which every reduction rewrites before reading.
+__global__ void __launch_bounds__(kBlock) k_cache_flush(DeviceState s, uint32_t n) {
+  __shared__ uint32_t sh[kBlock];
+  uint32_t acc = 0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) acc ^= s.cub[i].y;
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t r = 0;
+    for (uint32_t k = 0; k < kBlock; ++k) r ^= sh[k];
+    reinterpret_cast<uint32_t*>(s.partials)[blockIdx.x] = r;
+  }
+}
+void launch_cache_flush(const DeviceState& s, uint32_t records, hipStream_t st) {
+  if (records == 0) return;
+  const uint32_t blocks = std::min<uint32_t>(2048u, s.partial_capacity);
+  hipLaunchKernelGGL(k_cache_flush, dim3(blocks), dim3(kBlock), 0, st, s, records);
+}
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const dim3 grid(blocks_for(p.vertex_count));
   const bool cl = has_cloth(p), ipc = p.contact != 0u;
@@ -3307,7 +3350,9 @@ void launch_lag_contact(const DeviceState& s, const PassParams& p, hipStream_t s
   hipLaunchKernelGGL(k_lag_contact, dim3(blocks_for(s.NC)), dim3(kBlock), 0, st, s, p);
 }
 void launch_build_active_rows(const DeviceState& s, const PassParams& p, hipStream_t st) {
-  hipLaunchKernelGGL(k_build_active_rows, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
+  if (s.NC == 0) return;
+  hipLaunchKernelGGL(k_candidate_keep, dim3(blocks_for(s.NC)), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_compact_active_rows, dim3(blocks_for(16u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
 }
 // The six contact step dispatches of one iteration (jgs2-solver.ts:2347-2388):
 // candidate caps, reduce, apply, exact validation, reduce, apply.
