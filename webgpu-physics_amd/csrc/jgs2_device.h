@@ -36,7 +36,8 @@ struct PassParams {
   uint32_t color_index;         // colored pass: the color being swept
   uint32_t solve_lanes;         // lanes per vertex of the local solve for scenes that use lanes (4, 8, 16)
   uint32_t solve_classes;       // bit 0: scenes solved with lanes resident; bit 1: scenes solved one lane per vertex
-  uint32_t pad7, pad8;
+  uint32_t cloth_base;          // cloth element records hold this pass's source pose (k_cloth_base ran)
+  uint32_t pad8;
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
@@ -119,6 +120,8 @@ struct DeviceState {
   uint2* hinge_row = nullptr;
   uint32_t* hinge_elems = nullptr;
   float4* tri_glob = nullptr;   // 2 per triangle: candidate area stretch
+  float4* tri_base = nullptr;   // 3 per triangle: membrane state at a sweep's source pose
+  float4* hinge_base = nullptr; // 4 per hinge: corner gradients, angle, validity, energy at that pose
   // IPC contact candidates.
   uint32_t NC = 0, NVT = 0;
   float ipc_dmin = 0, ipc_mu = 0, ipc_eps = 0, ipc_safety = 0;
@@ -161,6 +164,8 @@ void launch_reduce_candidate_scene(const DeviceState& s, const PassParams& p, ui
 void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t stream);
+// Cloth element records at positions `which` (0 = posA, 1 = posB).
+void launch_cloth_base(const DeviceState& s, const PassParams& p, uint32_t which, hipStream_t stream);
 void launch_body_correction(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_clear_frame(const DeviceState& s, const PassParams& p, hipStream_t stream);

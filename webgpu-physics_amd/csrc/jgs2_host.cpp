@@ -208,7 +208,7 @@ void free_device(DeviceState& d) {
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
-                  d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.cand, d.lag_n, d.lag_w,
+                  d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
                   d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
@@ -738,6 +738,8 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.tri_inv, Rc, by, dry));
   TRYS(dalloc(&d.tri_area, Rc, by, dry));
   TRYS(dalloc(&d.tri_glob, 2 * Rc, by, dry));
+  TRYS(dalloc(&d.tri_base, 3 * Rc, by, dry));
+  TRYS(dalloc(&d.hinge_base, 4 * Hc, by, dry));
   TRYS(dalloc(&d.tri_elems, 3 * Rc, by, dry));
   TRYS(dalloc(&d.hinge_ix, Hc, by, dry));
   TRYS(dalloc(&d.hinge_rest, Hc, by, dry));
@@ -776,6 +778,8 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   if (Cc) HIP_CHECK(hipMemcpyAsync(d.cblock_scene, none.data(), 4 * (Cc / kSceneBlock), hipMemcpyHostToDevice, q));
   std::vector<uint4> zero_cnt(S, make_uint4(0, 0, 0, 0));
   HIP_CHECK(hipMemcpyAsync(d.scene_cnt, zero_cnt.data(), S * sizeof(uint4), hipMemcpyHostToDevice, q));
+  // Hinge entries no scene owns reference no vertex (the element pass skips them).
+  HIP_CHECK(hipMemsetAsync(d.hinge_ix, 0xff, std::max<uint64_t>(Hc, 1) * sizeof(uint4), q));
   std::vector<uint4> no_body(Bc, make_uint4(0, 0, 0, kNoScene));
   HIP_CHECK(hipMemcpyAsync(d.body_rng, no_body.data(), Bc * sizeof(uint4), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipStreamSynchronize(q));
@@ -1181,23 +1185,30 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
       }
       if (globalization && !contact) launch_vertex_energy(d, p, q);
     } else {
-      // Sweeps after the first read the frames the previous convergence pass
-      // computed at this sweep's source pose.
+      // Sweeps after the first read the frames (and cloth element records)
+      // the previous convergence stage computed at this sweep's source pose.
       const bool fused = globalization && it > 0;
       if (b->corotated) launch_tet_polar(d, p, q);
       if (!fused) launch_vertex_polar(d, p, q);
+      if (!fused && globalization && cloth) launch_cloth_base(d, p, p.src, q);
       PassParams ps = p;
       ps.fused_rot = fused ? 1u : 0u;
+      ps.cloth_base = (globalization && cloth) ? 1u : 0u;
       record_solve(b, ps);
       if (globalization && cloth && !contact) {
         PassParams pe = p;
         pe.energy_after_sweep = 1;
+        pe.cloth_base = 1;
         launch_vertex_energy(d, pe, q);
       }
     }
     if (contact) {  // safe step, validation, then the energy pair (jgs2-solver.ts:2347-2400)
       launch_contact_step(d, p, q);
-      if (globalization) launch_vertex_energy(d, p, q);
+      if (globalization) {
+        PassParams pe = p;
+        pe.cloth_base = (!gs && cloth) ? 1u : 0u;
+        launch_vertex_energy(d, pe, q);
+      }
     }
     if (globalization) {
       launch_reduce_candidate(d, p, q);
@@ -1205,6 +1216,10 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
       if (contact) launch_promote_contact(d, p, q);
       PassParams pc = p;
       pc.compute_next_rot = (!gs && it + 1 < iterations) ? 1u : 0u;
+      if (!gs && cloth) {  // element records at the candidate: this pass and the next sweep read them
+        launch_cloth_base(d, p, p.tgt, q);
+        pc.cloth_base = 1;
+      }
       launch_convergence(d, pc, q);
     }
   }

@@ -522,7 +522,8 @@ struct Dihedral {
   bool valid;
   f3 grad;
 };
-__device__ Dihedral dihedral_at(const DeviceState& s, uint32_t h, const float4* pos, uint32_t ov, f3 op, uint32_t slot) {
+__device__ Dihedral dihedral_at(const DeviceState& s, uint32_t h, const float4* pos, uint32_t ov, f3 op, uint32_t slot,
+                                float4* store = nullptr) {
   const uint4 ix = s.hinge_ix[h];
   const f3 x0 = cloth_pos(pos, ix.x, ov, op), x1 = cloth_pos(pos, ix.y, ov, op);
   const f3 x2 = cloth_pos(pos, ix.z, ov, op), x3 = cloth_pos(pos, ix.w, ov, op);
@@ -566,6 +567,7 @@ __device__ Dihedral dihedral_at(const DeviceState& s, uint32_t h, const float4* 
     const f3 g = mk(gc[0], gc[1], gc[2]);
     fin = fin && finite3(g);
     if (dv == slot) keep = g;
+    if (store) store[dv] = make_float4(g.x, g.y, g.z, 0.0f);
   }
   const float angle = atan2f(sine, cosine);
   if (!finite_s(angle) || !fin) return out;
@@ -578,27 +580,60 @@ __device__ __forceinline__ float wrapped_angle_difference(float a, float r) {  /
   const float d = a - r;
   return atan2f(sinf(d), cosf(d));
 }
-__device__ __forceinline__ float bending_energy_at(const DeviceState& s, uint32_t h, const float4* pos, uint32_t ov, f3 op) {  // S:1817-1836
-  const Dihedral ev = dihedral_at(s, h, pos, ov, op, 4u);
+__device__ __forceinline__ float bending_energy_of(const DeviceState& s, uint32_t h, const Dihedral& ev) {  // S:1817-1836
   if (!ev.valid) return kF32Max;
   const float4 r = s.hinge_rest[h];
   const float ad = wrapped_angle_difference(ev.angle, r.x);
   const float k = s.cloth_bend * r.y;
   return 0.5f * k * ad * ad;
 }
+__device__ __forceinline__ float bending_energy_at(const DeviceState& s, uint32_t h, const float4* pos, uint32_t ov, f3 op) {
+  return bending_energy_of(s, h, dihedral_at(s, h, pos, ov, op, 4u));
+}
+// Element records at a sweep's source pose, computed once per element by
+// k_cloth_base (p.cloth_base set) instead of once per incident vertex; the
+// same functions produce them, so they equal the inline evaluations.
+//   tri_base[3t]  {f0, energy}, {f1, 0}, {stress, 0}
+//   hinge_base[4h] {grad0, angle}, {grad1, valid}, {grad2, bending energy}, {grad3, 0}
+__device__ __forceinline__ Membrane membrane_base(const DeviceState& s, const PassParams& p, uint32_t tri,
+                                                  const float4* pos) {
+  if (!p.cloth_base) return membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0));
+  const float4* r = s.tri_base + 3 * size_t(tri);
+  const float4 a = r[0], b = r[1], c = r[2];
+  Membrane m;
+  m.energy = a.w;
+  m.f0 = xyz(a);
+  m.f1 = xyz(b);
+  m.stress = xyz(c);
+  return m;
+}
+__device__ __forceinline__ Dihedral dihedral_base(const DeviceState& s, const PassParams& p, uint32_t h,
+                                                  const float4* pos, uint32_t slot) {
+  if (!p.cloth_base) return dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+  const float4* r = s.hinge_base + 4 * size_t(h);
+  Dihedral d;
+  d.angle = r[0].w;
+  d.valid = r[1].w != 0.0f;
+  d.grad = d.valid && slot < 4u ? xyz(r[slot]) : mk(0, 0, 0);
+  return d;
+}
+__device__ __forceinline__ float bending_base(const DeviceState& s, const PassParams& p, uint32_t h, const float4* pos) {
+  if (!p.cloth_base) return bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
+  return s.hinge_base[4 * size_t(h) + 2].w;
+}
 // Membrane and bending contributions to one vertex's force (S:4028-4085).
-__device__ __forceinline__ f3 cloth_gradient(const DeviceState& s, const float4* pos, uint32_t v) {
+__device__ __forceinline__ f3 cloth_gradient(const DeviceState& s, const PassParams& p, const float4* pos, uint32_t v) {
   f3 g = mk(0, 0, 0);
   for (uint32_t k = s.tri_row[v].x, ke = k + s.tri_row[v].y; k < ke; ++k) {
     const uint32_t tri = s.tri_elems[k];
     const uint32_t slot = tri_slot(s.tri_ix[tri], v);
-    if (slot < 3) g = g + membrane_gradient(s, tri, membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)), slot);
+    if (slot < 3) g = g + membrane_gradient(s, tri, membrane_base(s, p, tri, pos), slot);
   }
   for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {
     const uint32_t h = s.hinge_elems[k];
     const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
     if (slot >= 4) continue;
-    const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+    const Dihedral d = dihedral_base(s, p, h, pos, slot);
     if (!d.valid) {
       g = g + mk(kF32Max, kF32Max, kF32Max);
     } else {
@@ -967,12 +1002,13 @@ __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const Pa
       const float tra = 2.0f * s.tri_area[tri];
       const float stretch = len3(tri_raw_normal(s, tri, pos, kNoVertex, mk(0, 0, 0))) / tra;
       if (!(tra > 0.0f) || !finite_s(stretch) || !(stretch > kDeterminantFloor)) return kF32Max;
-      e += membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy;
+      // At the source pose the element records hold these energies.
+      e += is_target ? membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy : membrane_base(s, p, tri, pos).energy;
     }
     for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {
       const uint32_t h = s.hinge_elems[k];
       if (__float_as_uint(s.hinge_rest[h].z) != v) continue;
-      e += bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
+      e += is_target ? bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0)) : bending_base(s, p, h, pos);
     }
   }
   if constexpr (kIpc) {  // S:2643 with ipcCachedContactAt (S:450-460)
@@ -1226,7 +1262,7 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
       const uint32_t tri = s.tri_elems[k];
       const uint32_t slot = tri_slot(s.tri_ix[tri], v);
       if (slot >= 3) continue;
-      const Membrane m = membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0));
+      const Membrane m = membrane_base(s, p, tri, pos);
       L.g = L.g + membrane_gradient(s, tri, m, slot);
       L.H = madd(L.H, membrane_local_hessian(s, tri, m, slot));
     }
@@ -1235,7 +1271,7 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
       const uint32_t h = s.hinge_elems[k];
       const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
       if (slot >= 4) continue;
-      const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+      const Dihedral d = dihedral_base(s, p, h, pos, slot);
       if (!d.valid) {
         L.g = L.g + mk(kF32Max, kF32Max, kF32Max);
       } else {
@@ -1367,13 +1403,13 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
       }
       tr.min_det = fminf(tr.min_det, fminf(ss, ts));
       // Membrane energy delta (S:2203-2236).
-      energy += membrane_at(s, tri, pos, v, tp).energy - membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy;
+      energy += membrane_at(s, tri, pos, v, tp).energy - membrane_base(s, p, tri, pos).energy;
     }
 #pragma unroll 1
     for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {  // S:2237-2271
       const uint32_t h = s.hinge_elems[k];
       if (hinge_slot(s.hinge_ix[h], v) >= 4) continue;
-      energy += bending_energy_at(s, h, pos, v, tp) - bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
+      energy += bending_energy_at(s, h, pos, v, tp) - bending_base(s, p, h, pos);
     }
   }
   if (p.cubature_k > 0) {
@@ -1626,7 +1662,7 @@ __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, u
       const uint32_t tri = s.tri_elems[tr.x + k];
       const uint32_t slot = tri_slot(s.tri_ix[tri], v);
       if (slot >= 3) continue;
-      const Membrane m = membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0));
+      const Membrane m = membrane_base(s, p, tri, pos);
       L.g = L.g + membrane_gradient(s, tri, m, slot);
       L.H = madd(L.H, membrane_local_hessian(s, tri, m, slot));
     }
@@ -1636,7 +1672,7 @@ __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, u
       const uint32_t h = s.hinge_elems[hr.x + k];
       const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
       if (slot >= 4) continue;
-      const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+      const Dihedral d = dihedral_base(s, p, h, pos, slot);
       if (!d.valid) {
         L.g = L.g + mk(kF32Max, kF32Max, kF32Max);
       } else {
@@ -1760,14 +1796,14 @@ __device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p
       }
       tr.min_det = fminf(tr.min_det, fminf(ss, ts));
       // Membrane energy delta (S:2203-2236).
-      tr.energy += membrane_at(s, tri, pos, v, tp).energy - membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy;
+      tr.energy += membrane_at(s, tri, pos, v, tp).energy - membrane_base(s, p, tri, pos).energy;
     }
     prefix += trr.y;
 #pragma unroll 1
     for (uint32_t k = lane_first<LPV>(sub, prefix); k < hr.y; k += LPV) {  // S:2237-2271
       const uint32_t h = s.hinge_elems[hr.x + k];
       if (hinge_slot(s.hinge_ix[h], v) >= 4) continue;
-      tr.energy += bending_energy_at(s, h, pos, v, tp) - bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0));
+      tr.energy += bending_energy_at(s, h, pos, v, tp) - bending_base(s, p, h, pos);
     }
     prefix += hr.y;
   }
@@ -2024,6 +2060,36 @@ __global__ void __launch_bounds__(kBlock) k_solve_lanes(DeviceState s, PassParam
                      bound, dr.gdotp, dr.residual, min_trial, backtracks, evaluations, status);
     if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
   }
+}
+
+// Cloth element records (membrane state; dihedral angle, validity, four
+// corner gradients and bending energy) at positions `which`, once per
+// element, for the gathers and trials of the vertices that share it.
+__global__ void __launch_bounds__(kBlock) k_cloth_base(DeviceState s, PassParams p, uint32_t which, uint32_t tri_blocks) {
+  const float4* pos = s.pos[which];
+  if (blockIdx.x < tri_blocks) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= p.tri_count) return;
+    const uint32_t sc = s.rblock_scene[t / kSceneBlock];
+    if (sc == kNoScene || t >= s.scene_end[sc].z || stop_latched(s, p, sc)) return;
+    const Membrane m = membrane_at(s, t, pos, kNoVertex, mk(0, 0, 0));
+    float4* r = s.tri_base + 3 * size_t(t);
+    r[0] = make_float4(m.f0.x, m.f0.y, m.f0.z, m.energy);
+    r[1] = make_float4(m.f1.x, m.f1.y, m.f1.z, 0.0f);
+    r[2] = make_float4(m.stress.x, m.stress.y, m.stress.z, 0.0f);
+    return;
+  }
+  const uint32_t h = (blockIdx.x - tri_blocks) * kBlock + threadIdx.x;
+  if (h >= p.hinge_count) return;
+  const uint4 ix = s.hinge_ix[h];
+  if (ix.x >= p.vertex_count) return;  // an entry no resident scene owns
+  const uint32_t sc = vscene(s, ix.x);
+  if (sc == kNoScene || stop_latched(s, p, sc)) return;
+  float4* r = s.hinge_base + 4 * size_t(h);
+  const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), 4u, r);
+  r[0].w = d.angle;
+  r[1].w = d.valid ? 1.0f : 0.0f;
+  r[2].w = bending_energy_of(s, h, d);
 }
 
 // jgs2Solve (S:2992-3487), one thread per vertex (Jacobi) or per vertex of
@@ -2546,7 +2612,7 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
         }
       }
       if constexpr (kCloth) {
-        if (!pinned) mat = mat + cloth_gradient(s, pos, v);
+        if (!pinned) mat = mat + cloth_gradient(s, p, pos, v);
       }
       if (p.compute_next_rot) {
         m3 R = mid();
@@ -2658,14 +2724,14 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient_lanes(DeviceSta
           for (uint32_t k = lane_first<LPV>(sub, 0); k < trr.y; k += LPV) {
             const uint32_t tri = s.tri_elems[trr.x + k];
             const uint32_t slot = tri_slot(s.tri_ix[tri], v);
-            if (slot < 3) cg = cg + membrane_gradient(s, tri, membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)), slot);
+            if (slot < 3) cg = cg + membrane_gradient(s, tri, membrane_base(s, p, tri, pos), slot);
           }
 #pragma unroll 1
           for (uint32_t k = lane_first<LPV>(sub, trr.y); k < hr.y; k += LPV) {
             const uint32_t h = s.hinge_elems[hr.x + k];
             const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
             if (slot >= 4) continue;
-            const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+            const Dihedral d = dihedral_base(s, p, h, pos, slot);
             if (!d.valid) {
               cg = cg + mk(kF32Max, kF32Max, kF32Max);
             } else {
@@ -3330,6 +3396,11 @@ void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t s
   hipLaunchKernelGGL(k_convergence_scales, dim3(p.scene_count), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_convergence_sums, dim3(vb), dim3(kBlock), 0, st, s, p, vb);
   hipLaunchKernelGGL(k_convergence_final, dim3(p.scene_count), dim3(kBlock), 0, st, s, p, vb);
+}
+void launch_cloth_base(const DeviceState& s, const PassParams& p, uint32_t which, hipStream_t st) {
+  const uint32_t tb = blocks_for(p.tri_count), hb = blocks_for(p.hinge_count);
+  if (tb + hb == 0) return;
+  hipLaunchKernelGGL(k_cloth_base, dim3(tb + hb), dim3(kBlock), 0, st, s, p, which, tb);
 }
 void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_position, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
