@@ -284,7 +284,8 @@ def _capacity(inputs, slack_scenes=0):
     total = {k: sum(f[k] for f in fp) for k in fp[0]}
     return BatchCapacity(max_vertices=total["vertices"], max_tets=total["tets"], max_scenes=len(inputs) + slack_scenes,
                          cubature_k=max(x.cubature_k for x in inputs), max_triangles=total["triangles"],
-                         max_hinges=total["hinges"], max_candidates=total["candidates"], max_bodies=total["bodies"])
+                         max_hinges=total["hinges"], max_candidates=total["candidates"], max_bodies=total["bodies"],
+                         max_surface_triangles=total["surface_triangles"], max_surface_edges=total["surface_edges"])
 
 
 def measure_plain(inputs, settings, iterations, steps):

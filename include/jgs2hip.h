@@ -84,6 +84,22 @@ typedef struct {
   uint32_t vertex_triangle_count;
   uint32_t edge_edge_count;
   const uint32_t* packed_indices;  /* 4 per candidate */
+  /*
+   * Broad-phase contact (no counterpart in the reference, which enumerates
+   * the whole superset on the host, ipc-contact.ts:600-665): with no packed
+   * candidates (counts 0, packed_indices NULL) and a surface given, the
+   * candidates are the superset's pairs (same canonical order and one-ring
+   * exclusions) within AABB reach, built on the device at admission and
+   * rebuilt at the frame start or before an iteration when motion since the
+   * last build could bring an unlisted pair within the activation reach.
+   * Stepping gives the superset's results bit for bit. Jacobi schedule only.
+   */
+  uint32_t surface_triangle_count;
+  uint32_t surface_edge_count;
+  const uint32_t* surface_triangles;  /* 3 per triangle (canonicalized like canonicalTriangles, :500-527) */
+  const uint32_t* surface_edges;      /* 2 per edge; triangle edges are added (canonicalEdges, :546-569) */
+  uint64_t candidate_capacity;        /* resident candidates at most; 0 = twice the count at the initial pose + 1024 */
+  double reach_margin;                /* reach past the activation reach in m; 0 = a quarter of it */
 } jgs2_contact_input;
 
 /*
@@ -230,8 +246,39 @@ jgs2_status jgs2_read_local_diagnostics(jgs2_solver* solver, float* dst, size_t 
 jgs2_status jgs2_read_globalization_control(jgs2_solver* solver, float* dst, size_t count); /* 16 */
 jgs2_status jgs2_read_globalization_history(jgs2_solver* solver, float* dst, size_t count); /* 64 * 32 */
 
-/* awaitIdle (jgs2-solver.ts:2496-2499). */
+/* awaitIdle (jgs2-solver.ts:2496-2499). Also reports JGS2_ERROR_MEMORY
+ * when a broad-phase build needed more candidates (or vertex row entries)
+ * than the scene's capacity; results from that build on are invalid. */
 jgs2_status jgs2_sync(jgs2_solver* solver);
+
+/*
+ * The current IPC candidate list (no counterpart in the reference): the
+ * packed records in caller vertex ids and canonical order, VT first. For a
+ * broad-phase scene it is the list of the last build, whose reach is
+ * returned; for a static superset the reach is +inf. `capacity` is in
+ * candidates; `total` receives the count even when it exceeds capacity.
+ */
+jgs2_status jgs2_read_contact_candidates(jgs2_solver* solver, uint32_t* packed, uint64_t capacity,
+                                         uint32_t* vertex_triangle_count, uint64_t* total, float* reach);
+/* Contact list facts of a scene (no counterpart in the reference). */
+typedef struct {
+  uint64_t candidates;          /* current list length (superset: its size) */
+  uint64_t vertex_triangle;     /* of which vertex-triangle */
+  uint64_t capacity;            /* candidate slots the scene holds */
+  uint64_t builds;              /* broad-phase builds so far, admission included */
+  float reach;                  /* reach of the last build (+inf for a static superset) */
+  uint32_t broad_phase;         /* 1 for a broad-phase scene */
+} jgs2_contact_info;
+jgs2_status jgs2_get_contact_info(jgs2_solver* solver, jgs2_contact_info* out);
+/* Rebuilds a broad-phase scene's list at its current positions now (no
+ * counterpart in the reference; stepping results do not change). */
+jgs2_status jgs2_rebuild_contact_candidates(jgs2_solver* solver);
+/* The candidate capacity jgs2_create would choose for a broad-phase scene
+ * whose candidate_capacity is 0 (no counterpart in the reference; for
+ * sizing batch arenas). Validates the input like jgs2_create; 0 when the
+ * scene has no contact. Host only. */
+jgs2_status jgs2_estimate_contact_capacity(const jgs2_scene_input* input, const jgs2_settings* overrides,
+                                           uint64_t override_mask, uint64_t* capacity);
 
 /* Scene facts the reference exposes as readonly members
  * (jgs2-solver.ts:906-916). */
@@ -303,6 +350,8 @@ typedef struct {
   uint32_t cubature_k;      /* records per vertex; scenes may use fewer */
   uint32_t corotated;       /* 1 = legacy co-rotated material path */
   uint32_t reserved;
+  uint64_t max_surface_triangles;  /* broad-phase surfaces, each scene rounded up to 256 */
+  uint64_t max_surface_edges;      /* likewise */
 } jgs2_batch_config;
 
 typedef struct {

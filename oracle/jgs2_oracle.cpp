@@ -30,6 +30,7 @@
 // closed form, momentum of force-free frames) and FP32/FP64 agreement.
 // Parts without such a pin are listed as "parity unpinned" in DESIGN.md.
 
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -3106,6 +3107,115 @@ uint32_t jgs2o_greedy_coloring(uint32_t vertex_count, const uint32_t* group_offs
     count = std::max(count, c + 1);
   }
   return count;
+}
+
+// The reference's static IPC superset (buildStaticIpcContactCandidates,
+// src/simulation/cpu/ipc-contact.ts:600-665, with canonicalTriangles,
+// canonicalEdges, buildOneRing and isInClosedOneRing, :500-592) restricted to
+// AABB reach: a pair is kept when the reference's AABB distance lower bound
+// (ipcAabbDistanceLowerBound, S:525-550) at `positions4` is not above
+// `reach`, or when any of its four points has a non-finite coordinate. The
+// bound is evaluated in f32 with one rounding per operation:
+// gap_a = max(max(lo_a - p_a, p_a - hi_a), 0), lb = sqrt((gx*gx + gy*gy) + gz*gz).
+// reach = +inf gives the whole superset. Returns the candidate count (VT
+// first, point-major over sorted triangles, then EE pairs i < j over sorted
+// edges) and writes up to `capacity` of them; -1 on invalid topology.
+static float bp_lower_bound(const float* lo, const float* hi, const float* lo2, const float* hi2) {
+  float g[3];
+  for (int a = 0; a < 3; ++a) {
+    const float u = lo2[a] - hi[a], w = lo[a] - hi2[a];
+    const float m = std::fmax(u, w);
+    g[a] = std::fmax(m, 0.0f);
+  }
+  const float xx = g[0] * g[0], yy = g[1] * g[1], zz = g[2] * g[2];
+  const float s = (xx + yy) + zz;
+  return std::sqrt(s);
+}
+
+int64_t jgs2o_contact_candidates_within(uint32_t vertex_count, const float* positions4, const uint32_t* triangles,
+                                        uint32_t triangle_count, const uint32_t* edges, uint32_t edge_count,
+                                        float reach, uint32_t* packed, uint64_t capacity, uint32_t* vt_count) {
+  const uint32_t V = vertex_count;
+  std::vector<std::array<uint32_t, 3>> tris;
+  for (uint32_t t = 0; t < triangle_count; ++t) {
+    std::array<uint32_t, 3> f{triangles[3 * size_t(t)], triangles[3 * size_t(t) + 1], triangles[3 * size_t(t) + 2]};
+    std::sort(f.begin(), f.end());
+    if (f[2] >= V || f[0] == f[1] || f[1] == f[2]) return -1;
+    tris.push_back(f);
+  }
+  std::sort(tris.begin(), tris.end());
+  tris.erase(std::unique(tris.begin(), tris.end()), tris.end());
+  std::vector<std::array<uint32_t, 2>> es;
+  for (uint32_t e = 0; e < edge_count; ++e) {
+    const uint32_t a = edges[2 * size_t(e)], b = edges[2 * size_t(e) + 1];
+    if (a >= V || b >= V || a == b) return -1;
+    es.push_back({std::min(a, b), std::max(a, b)});
+  }
+  for (const auto& f : tris) {
+    es.push_back({f[0], f[1]});
+    es.push_back({f[1], f[2]});
+    es.push_back({f[0], f[2]});
+  }
+  std::sort(es.begin(), es.end());
+  es.erase(std::unique(es.begin(), es.end()), es.end());
+  std::vector<std::vector<uint32_t>> ring(V);
+  for (const auto& e : es) {
+    ring[e[0]].push_back(e[1]);
+    ring[e[1]].push_back(e[0]);
+  }
+  auto closed = [&](uint32_t a, uint32_t b) {
+    return a == b || std::find(ring[a].begin(), ring[a].end(), b) != ring[a].end();
+  };
+  auto finite_point = [&](uint32_t v) {
+    const float* q = positions4 + 4 * size_t(v);
+    return std::isfinite(q[0]) && std::isfinite(q[1]) && std::isfinite(q[2]);
+  };
+  auto box = [&](std::initializer_list<uint32_t> ids, float* lo, float* hi) {
+    bool first = true;
+    for (uint32_t v : ids) {
+      const float* q = positions4 + 4 * size_t(v);
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = first ? q[a] : std::fmin(lo[a], q[a]);
+        hi[a] = first ? q[a] : std::fmax(hi[a], q[a]);
+      }
+      first = false;
+    }
+  };
+  uint64_t written = 0;
+  auto emit = [&](uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    if (packed && written < capacity) {
+      uint32_t* o = packed + 4 * written;
+      o[0] = a;
+      o[1] = b;
+      o[2] = c;
+      o[3] = d;
+    }
+    ++written;
+  };
+  for (uint32_t v = 0; v < V; ++v)
+    for (const auto& f : tris) {
+      if (closed(v, f[0]) || closed(v, f[1]) || closed(v, f[2])) continue;
+      float lo[3], hi[3], tlo[3], thi[3];
+      box({v}, lo, hi);
+      box({f[0], f[1], f[2]}, tlo, thi);
+      const bool finite = finite_point(v) && finite_point(f[0]) && finite_point(f[1]) && finite_point(f[2]);
+      if (finite && bp_lower_bound(lo, hi, tlo, thi) > reach) continue;
+      emit(v, f[0], f[1], f[2]);
+    }
+  if (vt_count) *vt_count = uint32_t(written);
+  for (size_t i = 0; i < es.size(); ++i)
+    for (size_t j = i + 1; j < es.size(); ++j) {
+      const auto& a = es[i];
+      const auto& b = es[j];
+      if (closed(a[0], b[0]) || closed(a[0], b[1]) || closed(a[1], b[0]) || closed(a[1], b[1])) continue;
+      float lo[3], hi[3], lo2[3], hi2[3];
+      box({a[0], a[1]}, lo, hi);
+      box({b[0], b[1]}, lo2, hi2);
+      const bool finite = finite_point(a[0]) && finite_point(a[1]) && finite_point(b[0]) && finite_point(b[1]);
+      if (finite && bp_lower_bound(lo, hi, lo2, hi2) > reach) continue;
+      emit(a[0], a[1], b[0], b[1]);
+    }
+  return int64_t(written);
 }
 
 }  // extern "C"

@@ -120,6 +120,8 @@ def _load() -> C.CDLL:
         "jgs2o_ipc_probe": (None, [C.c_int, C.c_int, f64p, C.c_double, C.c_double, f64p]),
         "jgs2o_ipc_friction": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, f64p]),
         "jgs2o_read_contact": (None, [C.c_void_p, f64p]),
+        "jgs2o_contact_candidates_within": (C.c_int64, [C.c_uint32, f32p, u32p, C.c_uint32, u32p, C.c_uint32,
+                                                        C.c_float, u32p, C.c_uint64, u32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -451,3 +453,22 @@ def ipc_friction(slip, eps, mu, force, precision=64):
     out = np.empty(2)
     lib.jgs2o_ipc_friction(precision, float(slip), float(eps), float(mu), float(force), out.ctypes.data_as(f64p))
     return out[0], out[1]
+
+
+def contact_candidates_within(vertex_count: int, positions, triangles, edges, reach: float):
+    """The reference's static VT/EE superset restricted to pairs whose f32
+    AABB distance lower bound at `positions` (4 floats per vertex) is not
+    above `reach` (float('inf') = the whole superset). Returns
+    (packed uint32 [n, 4], vertex-triangle count)."""
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    tri = np.ascontiguousarray(triangles, np.uint32).reshape(-1)
+    edg = np.ascontiguousarray(edges, np.uint32).reshape(-1)
+    vt = C.c_uint32()
+    args = (int(vertex_count), pos.ctypes.data_as(f32p), tri.ctypes.data_as(u32p), tri.size // 3,
+            edg.ctypes.data_as(u32p), edg.size // 2, float(reach))
+    n = lib.jgs2o_contact_candidates_within(*args, None, 0, C.byref(vt))
+    if n < 0:
+        raise ValueError("invalid surface topology")
+    out = np.zeros((max(n, 1), 4), np.uint32)
+    lib.jgs2o_contact_candidates_within(*args, out.ctypes.data_as(u32p), n, C.byref(vt))
+    return out[:n], vt.value

@@ -132,6 +132,61 @@ struct DeviceState {
   uint2* cand_row = nullptr;
   uint32_t *cand_ids = nullptr, *active_count = nullptr, *active_ids = nullptr, *cand_meta = nullptr;
   uint32_t* cand_keep = nullptr;  // per candidate: kept in its vertices' active rows this iteration
+  // Broad-phase contact: scenes whose candidates are the static superset's
+  // pairs within AABB reach, rebuilt on the device when motion since the
+  // last build could bring an unlisted pair into reach.
+  //   bp_tri    uint4[NS]  canonical surface triangles (ascending caller ids)
+  //                        as device vertex ids; w = 0
+  //   bp_edge   uint2[NE]  canonical surface edges, device ids
+  //   bp_tbox, bp_ebox float4[2 NS], [2 NE]: AABB lo (w = 1 if a point is
+  //                        non-finite or an unbounded pin) and hi at the build pose
+  //   bp_tchunk, bp_echunk float4[2 NS/64], [2 NE/64]: AABBs of 64 consecutive
+  //                        primitives (w as above, or-ed)
+  //   ring_row  uint2[V], ring_ids uint32[]: surface one-ring (device ids)
+  //   bp_qv     uint32[V]  canonical (caller-order) vertex q of a scene at
+  //                        vertex-range index VB + q -> its device vertex
+  //   bp_vcnt   uint32[V]  VT matches of query VB + q, then their offsets
+  //   bp_ecnt   uint32[NE] EE matches of edge query, then offsets
+  //   bp_x      float4[V]  positions at the last build
+  //   bp_cpose  float4[V]  the pose the source contact cache was last computed
+  //                        at (the frame start, or the validated target of the
+  //                        last promoted iteration); bp_tval the validated
+  //                        target of the current iteration
+  //   cand_key  uint2[NC]  {query (| 0x80000000 for EE), other} canonical key
+  //   prev_key, prev_lag_n, prev_lag_w, prev_src_c, prev_src_w: the list
+  //                        before a rebuild inside a frame
+  //   bp_rowcnt, bp_rowfill uint32[V]: candidate CSR construction
+  //   bp_ctl    uint4[S]  {flags, VT count, total, needed total}
+  //   bp_reach  float4[S] {reach of the last build, margin, builds, 0}
+  //   bp_motion uint4[2S] per-axis max |x - x_build| and |x - x_frame| (float bits)
+  //   bp_size   uint4[S]  {surface triangles, surface edges, candidate capacity,
+  //                        candidates before the current build}
+  //   bp_blk_v, bp_blk_e, bp_blk_r uint32[blocks]: block sums, then block
+  //                        offsets, of the VT, EE and row-count scans
+  //   bp_list   uint32[]   arena blocks of broad-phase scenes: bp_nv vertex
+  //                        blocks, bp_ns surface-triangle blocks, bp_ne edge
+  //                        blocks, bp_nc candidate blocks
+  uint32_t NS = 0, NE = 0;
+  uint32_t bp_nv = 0, bp_ns = 0, bp_ne = 0, bp_nc = 0;
+  uint32_t* bp_list = nullptr;
+  uint4* bp_size = nullptr;
+  uint32_t* bp_rows_at = nullptr;  // per scene: first entry of its cand_ids / active_ids rows
+  uint32_t *bp_blk_v = nullptr, *bp_blk_e = nullptr, *bp_blk_r = nullptr;
+  uint4* bp_tri = nullptr;
+  uint2* bp_edge = nullptr;
+  float4 *bp_tbox = nullptr, *bp_ebox = nullptr, *bp_tchunk = nullptr, *bp_echunk = nullptr;
+  uint2* ring_row = nullptr;
+  uint32_t* ring_ids = nullptr;
+  uint32_t *bp_qv = nullptr, *bp_vcnt = nullptr, *bp_ecnt = nullptr;
+  float4 *bp_x = nullptr, *bp_cpose = nullptr, *bp_tval = nullptr;
+  uint2 *cand_key = nullptr, *prev_key = nullptr;
+  float4 *prev_lag_n = nullptr, *prev_lag_w = nullptr, *prev_src_c = nullptr, *prev_src_w = nullptr;
+  uint32_t *bp_rowcnt = nullptr, *bp_rowfill = nullptr;
+  uint4* bp_ctl = nullptr;
+  float4* bp_reach = nullptr;
+  uint4* bp_motion = nullptr;
+  uint32_t *sblock_scene = nullptr, *eblock_scene = nullptr;
+  uint4* scene_blk2 = nullptr;  // {sb0, sb1, eb0, eb1}: surface triangle and edge blocks
   // Scene tables.
   uint32_t S = 0;
   uint32_t *vblock_scene = nullptr, *tblock_scene = nullptr, *rblock_scene = nullptr, *cblock_scene = nullptr;
@@ -177,6 +232,15 @@ void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t 
 void launch_promote_contact(const DeviceState& s, const PassParams& p, hipStream_t stream);
 uint32_t partial_blocks_needed(uint32_t vertices, uint32_t tets, uint32_t triangles, uint32_t candidates);
 size_t scene_conv_bytes(uint32_t scenes);
+// Broad-phase contact (jgs2_broad_phase.hip). `check` decides per scene
+// whether the list must be rebuilt before this point of the frame (always
+// for scenes whose flags ask for it); `carry` keeps lagged friction data
+// and recomputes the source contact cache, for a rebuild inside a frame.
+constexpr uint32_t kBpScene = 1u;      // the scene's candidates come from the broad phase
+constexpr uint32_t kBpNeed = 2u;       // rebuild at the next check point
+constexpr uint32_t kBpOverflow = 4u;   // a build needed more candidates or row entries than fit (sticky)
+constexpr uint32_t kBpForce = 8u;      // rebuild at the next check point regardless of motion
+void launch_broad_phase(const DeviceState& s, const PassParams& p, bool check, bool carry, hipStream_t stream);
 // Local-solve timing of one launch configuration (autotuner).
 void launch_solve_config(const DeviceState& s, const PassParams& p, hipStream_t stream);
 // Reads `records` uint4 of the cubature array to evict the caches.

@@ -141,6 +141,13 @@ struct jgs2_solver {
   // Arena spans: vertex/tet/triangle/candidate spans in 256-entry blocks;
   // the others in entries.
   Span vblk, tblk, rblk, cblk, adj, hin, tel, hel, cel, body;
+  // Broad-phase contact: canonical surface, its spans (triangle and edge
+  // blocks, one-ring entries), the candidate capacity and the reach margin.
+  bool bp = false;
+  uint32_t NS = 0, NE = 0;
+  Surface surface;
+  float bp_margin = 0;
+  Span sblk, eblk, rng;
   std::vector<uint32_t> colors;         // by caller vertex id
   std::vector<uint32_t> color_offsets;  // color_count + 1
   std::vector<uint32_t> color_order;    // arena vertex ids by color
@@ -169,7 +176,7 @@ struct jgs2_batch {
   jgs2_settings defaults{};
   jgs2_batch_config cfg{};
   bool corotated = false;
-  RangeAlloc va, ta, ra, ca, adja, hina, tela, hela, cela, bodya, slota;
+  RangeAlloc va, ta, ra, ca, adja, hina, tela, hela, cela, bodya, slota, sa, ea, rga;
   std::vector<jgs2_solver*> slots;
   uint64_t version = 0;      // bumps on every admission or retirement
   uint64_t stepped_version = ~0ull;  // the version the last submitted frame ran at
@@ -212,7 +219,11 @@ void free_device(DeviceState& d) {
                   d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
-                  d.body_rng};
+                  d.body_rng, d.bp_list, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
+                  d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_x,
+                  d.bp_cpose, d.bp_tval, d.cand_key, d.prev_key, d.prev_lag_n, d.prev_lag_w, d.prev_src_c, d.prev_src_w,
+                  d.bp_rowcnt, d.bp_rowfill, d.bp_ctl, d.bp_reach, d.bp_motion, d.sblock_scene, d.eblock_scene,
+                  d.scene_blk2};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   d = DeviceState{};
@@ -768,6 +779,38 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.cblock_scene, std::max<uint64_t>(1, Cc / kSceneBlock), by, dry));
   d.partial_capacity = partial_blocks_needed(uint32_t(Vc), uint32_t(Tc), uint32_t(Rc), uint32_t(Cc));
   TRYS(dalloc(reinterpret_cast<char**>(&d.partials), partial_bytes(d.partial_capacity), by, dry));
+  // Broad-phase contact arrays, only for batches that may hold such scenes.
+  const uint64_t Sc = blocks_of(c.max_surface_triangles) * kSceneBlock, Ec = blocks_of(c.max_surface_edges) * kSceneBlock;
+  if (Sc * 3 > 0xffffff00ull || Ec * 2 > 0xffffff00ull) return fail(JGS2_ERROR_RANGE, "Batch capacities exceed 32-bit indexing.");
+  const bool bp = Sc + Ec > 0;
+  if (bp) {
+    TRYS(dalloc(&d.bp_tri, Sc, by, dry));
+    TRYS(dalloc(&d.bp_tbox, 2 * Sc, by, dry));
+    TRYS(dalloc(&d.bp_tchunk, 2 * (Sc / 64), by, dry));
+    TRYS(dalloc(&d.bp_edge, Ec, by, dry));
+    TRYS(dalloc(&d.bp_ebox, 2 * Ec, by, dry));
+    TRYS(dalloc(&d.bp_echunk, 2 * (Ec / 64), by, dry));
+    TRYS(dalloc(&d.bp_ecnt, Ec, by, dry));
+    TRYS(dalloc(&d.bp_blk_e, Ec / kSceneBlock, by, dry));
+    TRYS(dalloc(&d.ring_row, Vc, by, dry));
+    TRYS(dalloc(&d.ring_ids, 2 * Ec, by, dry));
+    for (uint32_t** a : {&d.bp_qv, &d.bp_vcnt, &d.bp_rowcnt, &d.bp_rowfill}) TRYS(dalloc(a, Vc, by, dry));
+    for (float4** a : {&d.bp_x, &d.bp_cpose, &d.bp_tval}) TRYS(dalloc(a, Vc, by, dry));
+    TRYS(dalloc(&d.bp_blk_v, Vc / kSceneBlock, by, dry));
+    TRYS(dalloc(&d.bp_blk_r, Vc / kSceneBlock, by, dry));
+    TRYS(dalloc(&d.cand_key, Cc, by, dry));
+    TRYS(dalloc(&d.prev_key, Cc, by, dry));
+    for (float4** a : {&d.prev_lag_n, &d.prev_lag_w, &d.prev_src_c, &d.prev_src_w}) TRYS(dalloc(a, Cc, by, dry));
+    TRYS(dalloc(&d.bp_ctl, S, by, dry));
+    TRYS(dalloc(&d.bp_reach, S, by, dry));
+    TRYS(dalloc(&d.bp_size, S, by, dry));
+    TRYS(dalloc(&d.bp_rows_at, S, by, dry));
+    TRYS(dalloc(&d.bp_motion, 2 * S, by, dry));
+    TRYS(dalloc(&d.scene_blk2, S, by, dry));
+    TRYS(dalloc(&d.sblock_scene, Sc / kSceneBlock, by, dry));
+    TRYS(dalloc(&d.eblock_scene, Ec / kSceneBlock, by, dry));
+    TRYS(dalloc(&d.bp_list, (Vc + Sc + Ec + Cc) / kSceneBlock, by, dry));
+  }
   if (dry) return JGS2_OK;
   // Free blocks belong to no scene; scene slots start inactive.
   hipStream_t q = b->stream;
@@ -782,6 +825,13 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   HIP_CHECK(hipMemsetAsync(d.hinge_ix, 0xff, std::max<uint64_t>(Hc, 1) * sizeof(uint4), q));
   std::vector<uint4> no_body(Bc, make_uint4(0, 0, 0, kNoScene));
   HIP_CHECK(hipMemcpyAsync(d.body_rng, no_body.data(), Bc * sizeof(uint4), hipMemcpyHostToDevice, q));
+  if (bp) {
+    std::vector<uint32_t> no_scene(std::max(Sc, Ec) / kSceneBlock + 1, kNoScene);
+    if (Sc) HIP_CHECK(hipMemcpyAsync(d.sblock_scene, no_scene.data(), 4 * (Sc / kSceneBlock), hipMemcpyHostToDevice, q));
+    if (Ec) HIP_CHECK(hipMemcpyAsync(d.eblock_scene, no_scene.data(), 4 * (Ec / kSceneBlock), hipMemcpyHostToDevice, q));
+    HIP_CHECK(hipMemsetAsync(d.bp_ctl, 0, S * sizeof(uint4), q));
+    HIP_CHECK(hipMemsetAsync(d.bp_motion, 0, 2 * S * sizeof(uint4), q));
+  }
   HIP_CHECK(hipStreamSynchronize(q));
   b->va.init(Vc / kSceneBlock);
   b->ta.init(Tc / kSceneBlock);
@@ -794,6 +844,9 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   b->cela.init(4 * Cc);
   b->bodya.init(Bc);
   b->slota.init(S);
+  b->sa.init(Sc / kSceneBlock);
+  b->ea.init(Ec / kSceneBlock);
+  b->rga.init(2 * Ec);
   b->slots.assign(S, nullptr);
   return JGS2_OK;
 }
@@ -810,6 +863,9 @@ void release_spans(jgs2_batch* b, jgs2_solver* s) {
   b->cela.release(s->cel.at, s->cel.n);
   b->bodya.release(s->body.at, s->body.n);
   b->slota.release(s->slot, 1);
+  b->sa.release(s->sblk.at, s->sblk.n);
+  b->ea.release(s->eblk.at, s->eblk.n);
+  b->rga.release(s->rng.at, s->rng.n);
 }
 
 template <class T> hipError_t upload(T* base, uint64_t at, const std::vector<T>& v, hipStream_t q) {
@@ -950,7 +1006,7 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
     TRYS(rows_of(c.triangle_indices, 3, NT, s->tel.at, RB, tri_row, tri_elems));
     TRYS(rows_of(c.hinge_indices, 4, NH, s->hel.at, s->hin.at, hinge_row, hinge_elems));
   }
-  if (NC > 0) {
+  if (NC > 0 && !s->bp) {
     for (uint32_t c = 0; c < NC; ++c) {
       const uint32_t* ix = &in.contact->packed_indices[4 * size_t(c)];
       cand[c] = make_uint4(uint32_t(VB) + vnew[ix[0]], uint32_t(VB) + vnew[ix[1]], uint32_t(VB) + vnew[ix[2]],
@@ -1007,7 +1063,7 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
   UP(d.hinge_ix, s->hin.at, hinge_ix);
   UP(d.hinge_rest, s->hin.at, hinge_rest);
   UP(d.hinge_elems, s->hel.at, hinge_elems);
-  if (NC > 0) {  // packIpcContactBuffer initial records (ipc-contact-layout.ts:302-341)
+  if (NC > 0 && !s->bp) {  // packIpcContactBuffer initial records (ipc-contact-layout.ts:302-341)
     std::vector<float4> z(NC, make_float4(0, 0, 0, 0)), scratch(NC, make_float4(1, 0, 0, 0)),
         none(NC, make_float4(0, 0, 0, -1));
     UP(d.cand, CB, cand);
@@ -1040,7 +1096,7 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
   const uint4 blk1 = make_uint4(uint32_t(s->rblk.at), uint32_t(s->rblk.at + s->rblk.n), uint32_t(s->cblk.at),
                                 uint32_t(s->cblk.at + s->cblk.n));
   const uint4 cnt = make_uint4(V, T, s->color_count, s->bodies);
-  const uint4 end = make_uint4(uint32_t(VB + V), uint32_t(TB + T), uint32_t(RB + NT), uint32_t(CB + NC));
+  const uint4 end = make_uint4(uint32_t(VB + V), uint32_t(TB + T), uint32_t(RB + NT), uint32_t(CB + (s->bp ? 0u : NC)));
   const float4 ctl = make_float4(0, 0, 0, 1);
   std::vector<float4> zero_ctl(4, make_float4(0, 0, 0, 0)), zero_hist(size_t(kHistoryCapacity) * 8, make_float4(0, 0, 0, 0));
   HIP_CHECK(hipMemcpyAsync(d.scene_blk0 + sl, &blk0, sizeof(uint4), hipMemcpyHostToDevice, q));
@@ -1054,6 +1110,47 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
   HIP_CHECK(hipMemcpyAsync(d.history + size_t(sl) * kHistoryCapacity * 8, zero_hist.data(),
                            zero_hist.size() * sizeof(float4), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.scene_cnt + sl, &cnt, sizeof(uint4), hipMemcpyHostToDevice, q));
+  if (d.bp_ctl) {  // broad-phase records (every slot of a batch that may hold such scenes)
+    const uint4 ctl_bp = make_uint4(s->bp ? (kBpScene | kBpForce) : 0u, 0, 0, 0);
+    const float4 reach = make_float4(0, s->bp_margin, 0, 0);
+    const uint4 size = make_uint4(s->NS, s->NE, s->bp ? NC : 0u, 0);
+    const uint4 blk2 = make_uint4(uint32_t(s->sblk.at), uint32_t(s->sblk.at + s->sblk.n), uint32_t(s->eblk.at),
+                                  uint32_t(s->eblk.at + s->eblk.n));
+    const uint4 zero2[2] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+    HIP_CHECK(hipMemcpyAsync(d.bp_ctl + sl, &ctl_bp, sizeof(uint4), hipMemcpyHostToDevice, q));
+    HIP_CHECK(hipMemcpyAsync(d.bp_reach + sl, &reach, sizeof(float4), hipMemcpyHostToDevice, q));
+    HIP_CHECK(hipMemcpyAsync(d.bp_size + sl, &size, sizeof(uint4), hipMemcpyHostToDevice, q));
+    const uint32_t rows_at = uint32_t(s->cel.at);
+    HIP_CHECK(hipMemcpyAsync(d.bp_rows_at + sl, &rows_at, sizeof(uint32_t), hipMemcpyHostToDevice, q));
+    HIP_CHECK(hipMemcpyAsync(d.bp_motion + 2 * size_t(sl), zero2, sizeof(zero2), hipMemcpyHostToDevice, q));
+    HIP_CHECK(hipMemcpyAsync(d.scene_blk2 + sl, &blk2, sizeof(uint4), hipMemcpyHostToDevice, q));
+  }
+  if (s->bp) {  // canonical surface in device ids, one-ring rows, query order
+    const Surface& sf = s->surface;
+    const uint64_t SB = s->sblk.at * kSceneBlock, EB = s->eblk.at * kSceneBlock;
+    std::vector<uint4> tri(sf.tris.size());
+    for (size_t t = 0; t < sf.tris.size(); ++t)
+      tri[t] = make_uint4(uint32_t(VB) + vnew[sf.tris[t][0]], uint32_t(VB) + vnew[sf.tris[t][1]],
+                          uint32_t(VB) + vnew[sf.tris[t][2]], 0u);
+    std::vector<uint2> edge(sf.edges.size());
+    for (size_t e = 0; e < sf.edges.size(); ++e)
+      edge[e] = make_uint2(uint32_t(VB) + vnew[sf.edges[e][0]], uint32_t(VB) + vnew[sf.edges[e][1]]);
+    std::vector<uint2> ring_row(Vp, make_uint2(uint32_t(s->rng.at), 0));
+    std::vector<uint32_t> ring_ids(sf.ring.size()), qv(Vp, 0);
+    for (uint32_t v = 0; v < V; ++v) {
+      ring_row[vnew[v]] = make_uint2(uint32_t(s->rng.at) + sf.ring_start[v], sf.ring_start[v + 1] - sf.ring_start[v]);
+      for (uint32_t k = sf.ring_start[v]; k < sf.ring_start[v + 1]; ++k) ring_ids[k] = uint32_t(VB) + vnew[sf.ring[k]];
+      qv[v] = uint32_t(VB) + vnew[v];
+    }
+    UP(d.bp_tri, SB, tri);
+    UP(d.bp_edge, EB, edge);
+    UP(d.ring_row, VB, ring_row);
+    UP(d.ring_ids, s->rng.at, ring_ids);
+    UP(d.bp_qv, VB, qv);
+    std::vector<uint32_t> tab(std::max<uint64_t>({s->sblk.n, s->eblk.n, 1}), sl);
+    if (s->sblk.n) HIP_CHECK(hipMemcpyAsync(d.sblock_scene + s->sblk.at, tab.data(), 4 * s->sblk.n, hipMemcpyHostToDevice, q));
+    if (s->eblk.n) HIP_CHECK(hipMemcpyAsync(d.eblock_scene + s->eblk.at, tab.data(), 4 * s->eblk.n, hipMemcpyHostToDevice, q));
+  }
 #undef UP
   HIP_CHECK(hipStreamSynchronize(q));
   return JGS2_OK;
@@ -1163,12 +1260,14 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
   base.tgt = 0;
   if (globalization) launch_clear_frame(d, base, q);
   launch_predict(d, base, q);
+  if (contact) launch_broad_phase(d, base, true, false, q);  // lists complete at the frame start
   if (contact) launch_lag_contact(d, base, q);
   if (globalization) launch_reduce_candidate(d, base, q);  // source feasibility preflight
   for (uint32_t it = 0; it < iterations; ++it) {
     PassParams p = base;
     p.src = (it % 2 == 0) ? 1u : 0u;
     p.tgt = (it % 2 == 0) ? 0u : 1u;
+    if (contact && it > 0) launch_broad_phase(d, p, true, true, q);  // and at each later sweep's source
     if (contact) launch_build_active_rows(d, p, q);
     if (gs) {
       launch_copy_position(d, p, q);
@@ -1291,6 +1390,8 @@ jgs2_status submit(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
                   JGS2_MAX_GLOBALIZED_ITERATIONS_PER_SUBMISSION, frames, iterations * sub);
   }
   if (b->active_scenes() == 0) return fail(JGS2_ERROR_STATE, "The batch holds no scene.");
+  if (st.schedule == JGS2_SCHEDULE_GRAPH_COLORED_GAUSS_SEIDEL && b->dev.bp_nv)
+    return fail(JGS2_ERROR_RANGE, "Broad-phase contact scenes step with the Jacobi schedule only.");
   if (b->use_graphs && !b->profiling) {
     uint64_t key = 1469598103934665603ull;
     key = fnv(key, &b->version, sizeof(b->version));
@@ -1345,6 +1446,51 @@ void recount_objectives(jgs2_solver* s) {  // computePackedObjectiveActivity (jg
   s->objective_flags = (forces ? 1u : 0u) | (targets ? 2u : 0u);
 }
 
+// Broad-phase contact of a scene: canonical surface, reach margin, the
+// initially feasible pose (the check of jgs2-solver.ts:1041-1066 over the
+// pairs that can be that close) and the candidate capacity.
+jgs2_status prepare_broad_phase(const jgs2_scene_input& in, const jgs2_settings& st, jgs2_solver* s) {
+  const jgs2_contact_input& c = *in.contact;
+  const uint32_t V = in.vertex_count;
+  TRYS(canonical_surface(V, c.surface_triangles, c.surface_triangle_count, c.surface_edges, c.surface_edge_count,
+                         s->surface));
+  for (uint32_t v = 0; v < V; ++v) {
+    const uint32_t n = s->surface.ring_start[v + 1] - s->surface.ring_start[v];
+    if (n > 64)
+      return fail(JGS2_ERROR_RANGE,
+                  "Broad-phase contact supports surface vertices with at most 64 neighbors; vertex %u has %u.", v, n);
+  }
+  if (!(std::isfinite(c.reach_margin) && c.reach_margin >= 0))
+    return fail(JGS2_ERROR_RANGE, "reach_margin must be finite and nonnegative.");
+  if (!superset_nonempty(s->surface, V)) return JGS2_OK;  // no pair can ever touch: no contact
+  const double rr = std::max({double(s->scene_scale), st.ipc_activation_distance, 1e-12}) * 0x1p-24 * 8.0;
+  const double act = st.max_step > 0 ? st.ipc_activation_distance + 2.0 * (st.max_step + rr) : INFINITY;
+  const double margin = c.reach_margin > 0 ? c.reach_margin : 0.25 * act;
+  const double near = st.ipc_minimum_distance + 8.0 * 0x1p-24 * std::max(1.0, st.ipc_activation_distance);
+  const SurfaceScan scan =
+      scan_surface_pairs(s->surface, V, in.positions, float(act + margin), float(2.0 * near + 1e-6));
+  if (std::isfinite(scan.min_distance)) {
+    const double m = 8.0 * 0x1p-24 * std::max({1.0, st.ipc_activation_distance, std::fabs(scan.min_distance)});
+    const double required = st.ipc_minimum_distance + m;
+    if (!(scan.min_distance > required))
+      return fail(JGS2_ERROR_RANGE,
+                  "IPC candidates require an initially feasible source pose with distance greater than %.17g "
+                  "(minimum plus an f32 safety margin); got %.17g.",
+                  required, scan.min_distance);
+  }
+  const uint64_t estimate = 2 * (scan.vertex_triangle + scan.edge_edge) + 1024;
+  const uint64_t capacity = c.candidate_capacity ? c.candidate_capacity : estimate;
+  if (capacity > 0x3fffffffull)
+    return fail(JGS2_ERROR_RANGE, "Broad-phase candidate capacity %llu exceeds 32-bit indexing.",
+                (unsigned long long)capacity);
+  s->bp = true;
+  s->NC = uint32_t(capacity);
+  s->NS = uint32_t(s->surface.tris.size());
+  s->NE = uint32_t(s->surface.edges.size());
+  s->bp_margin = float(margin);
+  return JGS2_OK;
+}
+
 // Validates a scene input and fills the host-side scene record (everything
 // but the arena spans) under the batch's resolved settings. Runs entirely
 // on the host, before any device is touched (layout.ts:223-416,
@@ -1361,6 +1507,10 @@ jgs2_status prepare_scene(const jgs2_scene_input& in, const jgs2_settings& resol
       return fail(JGS2_ERROR_RANGE, "Triangle cloth requires the stable Neo-Hookean production path.");
   }
   const uint32_t NC = in.contact ? in.contact->vertex_triangle_count + in.contact->edge_edge_count : 0u;
+  const bool bp_mode = in.contact && NC == 0 &&
+                       in.contact->surface_triangle_count + in.contact->surface_edge_count > 0;
+  if (in.contact && NC > 0 && in.contact->surface_triangle_count + in.contact->surface_edge_count > 0)
+    return fail(JGS2_ERROR_RANGE, "IPC contact takes either packed candidates or a broad-phase surface, not both.");
   if (in.contact) TRYS(validate_contact(*in.contact, V));
   if (NC > 0) {  // initially feasible contact distances (jgs2-solver.ts:1041-1066)
     const double minimum = min_static_contact_distance(in.positions, *in.contact);
@@ -1447,6 +1597,7 @@ jgs2_status prepare_scene(const jgs2_scene_input& in, const jgs2_settings& resol
     double diag = dyn ? std::hypot(std::hypot(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]) : 0.0;
     s->scene_scale = float((dyn > 0 && std::isfinite(diag) && diag > 0) ? diag : 1e-12);
   }
+  if (bp_mode) TRYS(prepare_broad_phase(in, resolved, s));
   morton_orders(in, s->vnew, s->vold, s->tnew, s->told);
   return JGS2_OK;
 }
@@ -1463,6 +1614,8 @@ jgs2_batch_config exact_config(const jgs2_solver* s) {
   c.max_scenes = 1;
   c.cubature_k = s->K;
   c.corotated = s->corotated ? 1u : 0u;
+  c.max_surface_triangles = s->NS;
+  c.max_surface_edges = s->NE;
   return c;
 }
 
@@ -1497,6 +1650,56 @@ jgs2_status clear_scene_tables(jgs2_batch* b, const jgs2_solver* s) {
     std::vector<uint4> rng(s->body.n, make_uint4(0, 0, 0, kNoScene));
     HIP_CHECK(hipMemcpy(b->dev.body_rng + s->body.at, rng.data(), rng.size() * sizeof(uint4), hipMemcpyHostToDevice));
   }
+  if (b->dev.bp_ctl) {
+    HIP_CHECK(hipMemcpy(b->dev.bp_ctl + s->slot, &zero, sizeof(uint4), hipMemcpyHostToDevice));
+    std::vector<uint32_t> no(std::max<uint64_t>({s->sblk.n, s->eblk.n, 1}), kNoScene);
+    if (s->sblk.n) HIP_CHECK(hipMemcpy(b->dev.sblock_scene + s->sblk.at, no.data(), 4 * s->sblk.n, hipMemcpyHostToDevice));
+    if (s->eblk.n) HIP_CHECK(hipMemcpy(b->dev.eblock_scene + s->eblk.at, no.data(), 4 * s->eblk.n, hipMemcpyHostToDevice));
+  }
+  return JGS2_OK;
+}
+
+// The arena blocks of every resident broad-phase scene, the grid of every
+// broad-phase launch: vertex blocks, surface triangle blocks, surface edge
+// blocks, candidate blocks.
+jgs2_status update_bp_list(jgs2_batch* b) {
+  DeviceState& d = b->dev;
+  if (!d.bp_list) return JGS2_OK;
+  std::vector<uint32_t> lists[4];
+  for (auto* s : b->slots) {
+    if (!s || !s->bp) continue;
+    const Span* sp[4] = {&s->vblk, &s->sblk, &s->eblk, &s->cblk};
+    for (int k = 0; k < 4; ++k)
+      for (uint64_t i = 0; i < sp[k]->n; ++i) lists[k].push_back(uint32_t(sp[k]->at + i));
+  }
+  std::vector<uint32_t> all;
+  for (auto& l : lists) all.insert(all.end(), l.begin(), l.end());
+  if (!all.empty()) {
+    HIP_CHECK(hipMemcpyAsync(d.bp_list, all.data(), 4 * all.size(), hipMemcpyHostToDevice, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+  }
+  d.bp_nv = uint32_t(lists[0].size());
+  d.bp_ns = uint32_t(lists[1].size());
+  d.bp_ne = uint32_t(lists[2].size());
+  d.bp_nc = uint32_t(lists[3].size());
+  return JGS2_OK;
+}
+
+// A broad-phase build that needed more room than the scene has makes every
+// later result invalid; reported by the next synchronizing call.
+jgs2_status check_bp_overflow(jgs2_batch* b) {
+  DeviceState& d = b->dev;
+  if (!d.bp_ctl || d.bp_nv == 0) return JGS2_OK;
+  std::vector<uint4> ctl(b->slots.size());
+  HIP_CHECK(hipMemcpy(ctl.data(), d.bp_ctl, ctl.size() * sizeof(uint4), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < ctl.size(); ++i) {
+    const jgs2_solver* s = b->slots[i];
+    if (s && s->bp && (ctl[i].x & kBpOverflow))
+      return fail(JGS2_ERROR_MEMORY,
+                  "A broad-phase contact build needed %u candidates (capacity %u) or a vertex row above 8192 entries; "
+                  "results are invalid. Create the scene with a larger candidate_capacity.",
+                  ctl[i].w, s->NC);
+  }
   return JGS2_OK;
 }
 
@@ -1504,6 +1707,7 @@ void forget_scene(jgs2_batch* b, jgs2_solver* s) {
   b->slots[s->slot] = nullptr;
   if (s->has_cloth) b->cloth_scenes -= 1;
   release_spans(b, s);
+  if (s->bp) (void)update_bp_list(b);
   b->version++;
   b->graph_valid = false;
 }
@@ -1541,6 +1745,9 @@ jgs2_status place_scene(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_input& i
   take(b->hela, 4 * uint64_t(s->NH), s->hel);
   take(b->cela, 4 * uint64_t(s->NC), s->cel);
   take(b->bodya, s->bodies, s->body);
+  take(b->sa, blocks_of(s->NS), s->sblk);
+  take(b->ea, blocks_of(s->NE), s->eblk);
+  take(b->rga, 2 * uint64_t(s->NE), s->rng);
   if (!ok) {
     release_spans(b, s);
     delete s;
@@ -1574,6 +1781,26 @@ jgs2_status place_scene(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_input& i
   if (s->has_cloth) b->cloth_scenes += 1;
   b->version++;
   b->graph_valid = false;
+  if (st == JGS2_OK && s->bp) {  // the candidate list at the uploaded pose
+    st = update_bp_list(b);
+    if (st == JGS2_OK) {
+      PassParams p = make_params(b, b->defaults, false);
+      p.src = 0;
+      p.tgt = 0;
+      launch_broad_phase(b->dev, p, false, false, b->stream);
+      uint4 ctl = make_uint4(0, 0, 0, 0);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(&ctl, b->dev.bp_ctl + s->slot, sizeof(uint4), hipMemcpyDeviceToHost, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      if (e != hipSuccess)
+        st = fail(JGS2_ERROR_DEVICE, "The broad-phase contact build failed: %s", hipGetErrorString(e));
+      else if (ctl.x & kBpOverflow)
+        st = fail(JGS2_ERROR_MEMORY,
+                  "The initial broad-phase contact list needs %u candidates (or a vertex row above 8192 entries); "
+                  "the scene's capacity is %u.",
+                  ctl.w, s->NC);
+    }
+  }
   if (st == JGS2_OK && !b->corotated) {
     PassParams p = make_params(b, b->defaults, false);
     p.src = 0;
@@ -2014,6 +2241,97 @@ jgs2_status jgs2_read_globalization_history(jgs2_solver* s, float* dst, size_t c
 jgs2_status jgs2_sync(jgs2_solver* s) {
   TRYS(check_usable(s));
   HIP_CHECK(hipStreamSynchronize(s->batch->stream));
+  TRYS(check_bp_overflow(s->batch));
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_read_contact_candidates(jgs2_solver* s, uint32_t* packed, uint64_t capacity, uint32_t* vt_count,
+                                         uint64_t* total, float* reach) {
+  TRYS(check_usable(s));
+  if (!vt_count || !total || !reach || (capacity && !packed)) return fail(JGS2_ERROR_TYPE, "NULL argument.");
+  jgs2_batch* b = s->batch;
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  TRYS(check_bp_overflow(b));
+  uint32_t n = s->NC, vt = s->NVT;
+  float r = INFINITY;
+  if (s->bp) {
+    uint4 ctl;
+    float4 rc;
+    HIP_CHECK(hipMemcpy(&ctl, b->dev.bp_ctl + s->slot, sizeof(uint4), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(&rc, b->dev.bp_reach + s->slot, sizeof(float4), hipMemcpyDeviceToHost));
+    n = ctl.z;
+    vt = ctl.y;
+    r = rc.x;
+  }
+  *vt_count = vt;
+  *total = n;
+  *reach = r;
+  if (packed && n) {
+    std::vector<uint4> rec(n);
+    HIP_CHECK(hipMemcpy(rec.data(), b->dev.cand + s->cbase(), n * sizeof(uint4), hipMemcpyDeviceToHost));
+    const uint32_t VB = uint32_t(s->vbase());
+    for (uint64_t c = 0; c < std::min<uint64_t>(n, capacity); ++c) {
+      const uint32_t ids[4] = {rec[c].x, rec[c].y, rec[c].z, rec[c].w};
+      for (int l = 0; l < 4; ++l) packed[4 * c + l] = s->vold[ids[l] - VB];
+    }
+  }
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_estimate_contact_capacity(const jgs2_scene_input* input, const jgs2_settings* overrides,
+                                           uint64_t mask, uint64_t* capacity) {
+  if (!input || !capacity) return fail(JGS2_ERROR_TYPE, "NULL argument.");
+  jgs2_settings defaults, resolved;
+  jgs2_default_settings(&defaults);
+  TRYS(merge_settings(defaults, overrides, mask, resolved));
+  jgs2_solver probe;
+  TRYS(prepare_scene(*input, resolved, &probe));
+  *capacity = probe.NC;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_get_contact_info(jgs2_solver* s, jgs2_contact_info* out) {
+  TRYS(check_usable(s));
+  if (!out) return fail(JGS2_ERROR_TYPE, "NULL info.");
+  jgs2_batch* b = s->batch;
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->candidates = s->NC;
+  out->vertex_triangle = s->NVT;
+  out->capacity = s->NC;
+  out->reach = INFINITY;
+  if (s->bp) {
+    uint4 ctl;
+    float4 rc;
+    HIP_CHECK(hipMemcpy(&ctl, b->dev.bp_ctl + s->slot, sizeof(uint4), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(&rc, b->dev.bp_reach + s->slot, sizeof(float4), hipMemcpyDeviceToHost));
+    out->candidates = ctl.z;
+    out->vertex_triangle = ctl.y;
+    out->builds = uint64_t(rc.z);
+    out->reach = rc.x;
+    out->broad_phase = 1;
+  }
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_rebuild_contact_candidates(jgs2_solver* s) {
+  TRYS(check_usable(s));
+  if (!s->bp) return fail(JGS2_ERROR_STATE, "The scene has no broad-phase contact.");
+  jgs2_batch* b = s->batch;
+  const uint4 ctl = make_uint4(kBpScene | kBpForce, 0, 0, 0);
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  HIP_CHECK(hipMemcpy(&b->dev.bp_ctl[s->slot].x, &ctl.x, sizeof(uint32_t), hipMemcpyHostToDevice));
+  PassParams p = make_params(b, b->defaults, false);
+  p.src = 0;
+  p.tgt = 0;
+  launch_broad_phase(b->dev, p, false, false, b->stream);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  TRYS(check_bp_overflow(b));
   clear_error();
   return JGS2_OK;
 }
@@ -2202,6 +2520,7 @@ jgs2_status jgs2_batch_step_frames_exact_iterations(jgs2_batch* b, uint32_t fram
 jgs2_status jgs2_batch_sync(jgs2_batch* b) {
   TRYS(check_batch(b));
   HIP_CHECK(hipStreamSynchronize(b->stream));
+  TRYS(check_bp_overflow(b));
   clear_error();
   return JGS2_OK;
 }

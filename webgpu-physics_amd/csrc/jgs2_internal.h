@@ -1,9 +1,13 @@
 // Internal helpers shared by the host translation units of libjgs2hip.
 #pragma once
 
+#include <array>
+#include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "jgs2hip.h"
 
@@ -16,5 +20,28 @@ void clear_error();
 // minimumStaticIpcContactDistance (src/simulation/cpu/ipc-contact.ts:349-391)
 // in f64 over positions with 4 floats per vertex.
 double min_static_contact_distance(const float* positions4, const jgs2_contact_input& contact);
+
+// Canonical contact surface of a broad-phase scene: canonicalTriangles,
+// canonicalEdges and buildOneRing (src/simulation/cpu/ipc-contact.ts:
+// 500-584). Triangles and edges ascending; ring rows ascending (CSR).
+struct Surface {
+  std::vector<std::array<uint32_t, 3>> tris;
+  std::vector<std::array<uint32_t, 2>> edges;
+  std::vector<uint32_t> ring_start, ring;
+};
+jgs2_status canonical_surface(uint32_t vertex_count, const uint32_t* triangles, uint32_t triangle_count,
+                              const uint32_t* edges, uint32_t edge_count, Surface& out);
+// Whether the static superset of the surface holds any pair.
+bool superset_nonempty(const Surface& s, uint32_t vertex_count);
+// Host scan of the superset's pairs whose f32 AABB distance lower bound at
+// `positions4` is within `reach` (a point that is not finite counts against
+// everything): the pair count, and the minimum f64 distance over pairs whose
+// bound is within `distance_reach` (+inf when there are none).
+struct SurfaceScan {
+  uint64_t vertex_triangle = 0, edge_edge = 0;
+  double min_distance = INFINITY;
+};
+SurfaceScan scan_surface_pairs(const Surface& s, uint32_t vertex_count, const float* positions4, float reach,
+                               float distance_reach);
 
 }  // namespace jgs2

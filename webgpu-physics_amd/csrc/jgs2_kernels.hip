@@ -1498,6 +1498,7 @@ __global__ void __launch_bounds__(kBlock) k_predict(DeviceState s, PassParams p)
   if (v >= p.vertex_count || vscene(s, v) == kNoScene) return;
   const f3 o = load_pos(s.pos[0], v);
   s.old[v] = make_float4(o.x, o.y, o.z, 1.0f);
+  if (s.bp_nv && (s.bp_ctl[vscene(s, v)].x & kBpScene)) s.bp_cpose[v] = make_float4(o.x, o.y, o.z, 1.0f);
   const bool pinned = s.vtx_info[v].z != 0u;
   f3 pr = xyz(s.vtx_rest[v]);
   if (!pinned) {
@@ -2523,8 +2524,13 @@ __global__ void __launch_bounds__(kBlock) k_apply_candidate(DeviceState s, PassP
   const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
   const uint32_t sc = vscene(s, v);
-  if (sc == kNoScene || stop_latched(s, p, sc)) return;
+  if (sc == kNoScene) return;
   const bool accepted = s.control[4 * size_t(sc) + 1].x == 1.0f;
+  if (p.contact && s.bp_nv && (s.bp_ctl[sc].x & kBpScene)) {  // the pose k_promote_contact caches
+    const float alpha = s.contact_ctl[sc].w;
+    if (accepted && finite_s(alpha) && alpha > 0.0f) s.bp_cpose[v] = s.bp_tval[v];
+  }
+  if (stop_latched(s, p, sc)) return;
   if (!accepted) {
     s.pos[p.tgt][v] = s.pos[p.src][v];
     s.final_update[v] = make_float4(0, 0, 0, 1);
@@ -3214,17 +3220,26 @@ __global__ void __launch_bounds__(kBlock) k_reduce_contact_final(DeviceState s) 
   if (threadIdx.x == 0) s.contact_ctl[sc].w = r.w == 1.0f ? fminf(fmaxf(r.x, 0.0f), 1.0f) : 0.0f;
 }
 // applyContactStep (S:2921-2929).
-__global__ void __launch_bounds__(kBlock) k_apply_contact_step(DeviceState s, PassParams p) {
+// The first application's result is the pose the validation caches
+// (kept for broad-phase scenes, whose cache of a newly listed pair is
+// evaluated there).
+__global__ void __launch_bounds__(kBlock) k_apply_contact_step(DeviceState s, PassParams p, uint32_t first) {
   const uint32_t v = logical_block(p.xcd_remap) * kBlock + threadIdx.x;
   if (v >= p.vertex_count) return;
   const uint32_t sc = vscene(s, v);
-  if (sc == kNoScene || stop_latched(s, p, sc)) return;
+  if (sc == kNoScene) return;
+  const bool keep = first && s.bp_nv && (s.bp_ctl[sc].x & kBpScene);
+  if (stop_latched(s, p, sc)) {
+    if (keep) s.bp_tval[v] = s.pos[p.tgt][v];
+    return;
+  }
   const uint4 b1 = s.scene_blk1[sc];
   if (b1.z == b1.w) return;  // a scene without candidates takes no contact step
   const float a = fminf(fmaxf(s.contact_ctl[sc].w, 0.0f), 1.0f);
   const f3 src = load_pos(s.pos[p.src], v), c = load_pos(s.pos[p.tgt], v);
   const f3 x = src + a * (c - src);
   s.pos[p.tgt][v] = make_float4(x.x, x.y, x.z, 1.0f);
+  if (keep) s.bp_tval[v] = make_float4(x.x, x.y, x.z, 1.0f);
 }
 // promoteAcceptedContactCache (S:2695-2710).
 __global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassParams p) {
@@ -3237,6 +3252,516 @@ __global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassP
     s.src_c[c] = s.tgt_c[c];
     s.src_w[c] = s.tgt_w[c];
   }
+}
+
+// ------------------------------------------------------------------ broad-phase contact
+// A broad-phase scene's candidates are the reference's static superset
+// (buildStaticIpcContactCandidates, src/simulation/cpu/ipc-contact.ts:
+// 600-665) restricted to pairs whose AABB distance lower bound (S:525-550) at
+// the build pose is within the build's reach, in the superset's canonical
+// order (VT point-major over sorted triangles, then EE pairs i < j over
+// sorted edges), plus every pair with a non-finite point or a pin that
+// snaps by more than half the step bound. A pair outside the list stays
+// beyond the activation reach (S:519-523) until the next check point, where
+// the list is rebuilt if motion since the build could have brought one
+// within it; such a pair is never active, never limits the safe step and
+// has no lagged friction, so stepping the list gives the superset's results
+// bit for bit.
+
+// ipcAabbDistanceLowerBound with one rounding per operation (no
+// contraction), the sequence the oracle evaluates.
+__device__ __forceinline__ float bp_lower_bound(f3 lo, f3 hi, f3 lo2, f3 hi2) {
+  const float gx = fmaxf(fmaxf(lo2.x - hi.x, lo.x - hi2.x), 0.0f);
+  const float gy = fmaxf(fmaxf(lo2.y - hi.y, lo.y - hi2.y), 0.0f);
+  const float gz = fmaxf(fmaxf(lo2.z - hi.z, lo.z - hi2.z), 0.0f);
+  return __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(gx, gx), __fmul_rn(gy, gy)), __fmul_rn(gz, gz)));
+}
+__device__ __forceinline__ bool bp_building(const DeviceState& s, uint32_t sc) {
+  return sc != kNoScene && (s.bp_ctl[sc].x & (kBpScene | kBpNeed)) == (kBpScene | kBpNeed);
+}
+// A point the reach argument does not cover: non-finite, or a pin whose
+// snap to rest (S:2997-3003) may exceed half the step bound.
+__device__ __forceinline__ bool bp_wild(const DeviceState& s, const PassParams& p, uint32_t v, f3 x) {
+  if (!finite3(x)) return true;
+  if (s.vtx_info[v].z == 0u) return false;
+  const f3 d = xyz(s.vtx_rest[v]) - x;
+  return !(sqrtf(dot(d, d)) <= 0.5f * fmaxf(p.max_step, 0.0f));
+}
+__device__ __forceinline__ uint32_t bp_bits(float a) { return finite_s(a) ? __float_as_uint(fabsf(a)) : 0x7f800000u; }
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
+#pragma unroll
+  for (uint32_t m = 32; m >= 1; m >>= 1) v = max(v, uint32_t(__shfl_xor(int(v), int(m))));
+  return v;
+}
+
+// Per-axis motion of every broad-phase scene since its last build and since
+// the frame started (maxima as float bits, order independent).
+__global__ void __launch_bounds__(kBlock) k_bp_motion(DeviceState s, PassParams p) {
+  const uint32_t vb = s.bp_list[blockIdx.x];
+  const uint32_t sc = s.vblock_scene[vb];
+  if (sc == kNoScene) return;
+  const uint32_t v = vb * kSceneBlock + threadIdx.x;
+  uint32_t m[6] = {0, 0, 0, 0, 0, 0};
+  if (v - s.scene_blk0[sc].x * kSceneBlock < s.scene_cnt[sc].x) {
+    const f3 x = load_pos(s.pos[p.src], v), b = xyz(s.bp_x[v]), o = load_pos(s.old, v);
+    m[0] = bp_bits(x.x - b.x), m[1] = bp_bits(x.y - b.y), m[2] = bp_bits(x.z - b.z);
+    m[3] = bp_bits(x.x - o.x), m[4] = bp_bits(x.y - o.y), m[5] = bp_bits(x.z - o.z);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) m[k] = wave_max_u(m[k]);
+  if (threadIdx.x % 64u == 0u) {
+    uint32_t* acc = reinterpret_cast<uint32_t*>(s.bp_motion + 2 * size_t(sc));
+    for (int k = 0; k < 3; ++k) atomicMax(acc + k, m[k]);
+    for (int k = 0; k < 3; ++k) atomicMax(acc + 4 + k, m[3 + k]);
+  }
+}
+// Per scene: rebuild now? Motion m since the build moves an AABB lower bound
+// by at most 2|m| (m the per-axis maxima), so the list stays complete while
+// 2|m| < reach - activation reach. A rebuild's reach keeps the margin past
+// the activation reach and covers every pair that was inside the barrier at
+// the frame start (the pairs with lagged friction).
+template <bool kCheck>
+__global__ void __launch_bounds__(kBlock) k_bp_decide(DeviceState s, PassParams p, uint32_t carry) {
+  const uint32_t sc = blockIdx.x * kBlock + threadIdx.x;
+  if (sc >= p.scene_count) return;
+  uint4 ctl = s.bp_ctl[sc];
+  if (!(ctl.x & kBpScene) || s.scene_cnt[sc].z == 0u) return;
+  uint4* mot = s.bp_motion + 2 * size_t(sc);
+  const uint4 mb = mot[0], mf = mot[1];
+  mot[0] = make_uint4(0, 0, 0, 0);
+  mot[1] = make_uint4(0, 0, 0, 0);
+  const bool bounded = p.max_step > 0.0f;
+  const float act = bounded ? p.ipc_dhat + 2.0f * (p.max_step + ipc_rounding_reach(p, s.scene_scale[sc])) : INFINITY;
+  float4 r = s.bp_reach[sc];
+  bool need = (ctl.x & kBpForce) != 0u;
+  if (kCheck && !need && r.x != INFINITY) {
+    const float bx = __uint_as_float(mb.x), by = __uint_as_float(mb.y), bz = __uint_as_float(mb.z);
+    const float grow = 2.0f * sqrtf(bx * bx + by * by + bz * bz);
+    need = !(grow < r.x - act - 1.0e-4f * act);
+  }
+  if (need) {
+    const float fx = __uint_as_float(mf.x), fy = __uint_as_float(mf.y), fz = __uint_as_float(mf.z);
+    const float frame = 2.0f * sqrtf(fx * fx + fy * fy + fz * fz);
+    float reach = bounded ? act + r.y : INFINITY;
+    reach = fmaxf(reach, (p.ipc_dhat + frame) * 1.0001f);
+    if (!finite_s(frame)) reach = INFINITY;
+    r.x = reach;
+    r.z += 1.0f;  // builds
+    s.bp_reach[sc] = r;
+    uint4 sz = s.bp_size[sc];
+    sz.w = carry ? ctl.z : 0u;  // the list a carrying rebuild looks lagged data up in
+    s.bp_size[sc] = sz;
+    ctl.x = (ctl.x | kBpNeed) & ~kBpForce;
+  } else {
+    ctl.x &= ~kBpNeed;
+  }
+  s.bp_ctl[sc] = ctl;
+}
+// Triangle and edge AABBs at the source pose, and the AABB of each run of
+// 64 consecutive primitives (w = 1 when a point is wild).
+__device__ __forceinline__ void bp_chunk_box(f3 lo, f3 hi, float wild, float4* chunk) {
+#pragma unroll
+  for (uint32_t m = 32; m >= 1; m >>= 1) {
+    lo.x = fminf(lo.x, __shfl_xor(lo.x, int(m)));
+    lo.y = fminf(lo.y, __shfl_xor(lo.y, int(m)));
+    lo.z = fminf(lo.z, __shfl_xor(lo.z, int(m)));
+    hi.x = fmaxf(hi.x, __shfl_xor(hi.x, int(m)));
+    hi.y = fmaxf(hi.y, __shfl_xor(hi.y, int(m)));
+    hi.z = fmaxf(hi.z, __shfl_xor(hi.z, int(m)));
+    wild = fmaxf(wild, __shfl_xor(wild, int(m)));
+  }
+  if (threadIdx.x % 64u == 0u) {
+    chunk[0] = make_float4(lo.x, lo.y, lo.z, wild);
+    chunk[1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+  }
+}
+__global__ void __launch_bounds__(kBlock) k_bp_bounds(DeviceState s, PassParams p) {
+  const bool tris = blockIdx.x < s.bp_ns;
+  const uint32_t blk = s.bp_list[s.bp_nv + blockIdx.x];
+  const uint32_t sc = tris ? s.sblock_scene[blk] : s.eblock_scene[blk];
+  if (!bp_building(s, sc)) return;
+  const uint32_t i = blk * kSceneBlock + threadIdx.x;
+  const uint4 b2 = s.scene_blk2[sc];
+  const uint32_t end = (tris ? b2.x : b2.z) * kSceneBlock + (tris ? s.bp_size[sc].x : s.bp_size[sc].y);
+  const float4* pos = s.pos[p.src];
+  f3 lo = mk(INFINITY, INFINITY, INFINITY), hi = mk(-INFINITY, -INFINITY, -INFINITY);
+  float wild = 0.0f;
+  if (i < end) {
+    if (tris) {
+      const uint4 ix = s.bp_tri[i];
+      const f3 a = load_pos(pos, ix.x), b = load_pos(pos, ix.y), c = load_pos(pos, ix.z);
+      lo = vmin3(a, vmin3(b, c));
+      hi = vmax3(a, vmax3(b, c));
+      wild = (bp_wild(s, p, ix.x, a) || bp_wild(s, p, ix.y, b) || bp_wild(s, p, ix.z, c)) ? 1.0f : 0.0f;
+      s.bp_tbox[2 * size_t(i)] = make_float4(lo.x, lo.y, lo.z, wild);
+      s.bp_tbox[2 * size_t(i) + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+    } else {
+      const uint2 ix = s.bp_edge[i];
+      const f3 a = load_pos(pos, ix.x), b = load_pos(pos, ix.y);
+      lo = vmin3(a, b);
+      hi = vmax3(a, b);
+      wild = (bp_wild(s, p, ix.x, a) || bp_wild(s, p, ix.y, b)) ? 1.0f : 0.0f;
+      s.bp_ebox[2 * size_t(i)] = make_float4(lo.x, lo.y, lo.z, wild);
+      s.bp_ebox[2 * size_t(i) + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+    }
+  }
+  bp_chunk_box(lo, hi, wild, (tris ? s.bp_tchunk : s.bp_echunk) + 2 * size_t(i / 64u));
+}
+// One wavefront per query: a vertex (VT, canonical vertex order) or an edge
+// (EE). Lanes test 64 chunk boxes at a time; the surviving chunks are
+// visited in order and each lane tests one primitive, so matches come out
+// in canonical order and a ballot gives each its slot. The count pass
+// stores the number of matches of each query, the write pass the records.
+template <bool kWrite>
+__global__ void __launch_bounds__(kBlock) k_bp_vt(DeviceState s, PassParams p) {
+  const uint32_t vb = s.bp_list[blockIdx.x / 64u];
+  const uint32_t sc = s.vblock_scene[vb];
+  if (!bp_building(s, sc)) return;
+  const uint32_t lane = threadIdx.x % 64u;
+  const uint32_t qa = vb * kSceneBlock + (blockIdx.x % 64u) * 4u + threadIdx.x / 64u;
+  const uint32_t q = qa - s.scene_blk0[sc].x * kSceneBlock;
+  if (q >= s.scene_cnt[sc].x) {  // padding: no matches (uniform per wavefront)
+    if (!kWrite && lane == 0u) s.bp_vcnt[qa] = 0u;
+    return;
+  }
+  const uint32_t dv = s.bp_qv[qa];
+  const f3 x = load_pos(s.pos[p.src], dv);
+  const bool wild = bp_wild(s, p, dv, x);
+  const uint2 ring = s.ring_row[dv];
+  const uint32_t my_ring = lane < ring.y ? s.ring_ids[ring.x + lane] : kNoVertex;
+  const uint4 sz = s.bp_size[sc];
+  const uint32_t t0 = s.scene_blk2[sc].x * kSceneBlock, nt = sz.x, nchunk = (nt + 63u) / 64u;
+  const float reach = s.bp_reach[sc].x;
+  const uint32_t cb = s.scene_blk1[sc].z * kSceneBlock, at = kWrite ? s.bp_vcnt[qa] : 0u;
+  uint32_t n = 0;
+  for (uint32_t c0 = 0; c0 < nchunk; c0 += 64u) {
+    bool pass = false;
+    if (c0 + lane < nchunk) {
+      const size_t ci = t0 / 64u + c0 + lane;
+      const float4 lo = s.bp_tchunk[2 * ci], hi = s.bp_tchunk[2 * ci + 1];
+      pass = wild || lo.w != 0.0f || !(bp_lower_bound(x, x, xyz(lo), xyz(hi)) > reach);
+    }
+    uint64_t chunks = __ballot(pass);
+    while (chunks) {
+      const uint32_t k = uint32_t(__builtin_ctzll(chunks));
+      chunks &= chunks - 1ull;
+      const uint32_t tl = (c0 + k) * 64u + lane;
+      bool in = false;
+      uint4 ix = make_uint4(kNoVertex, kNoVertex, kNoVertex, 0);
+      if (tl < nt) {
+        const size_t t = t0 + tl;
+        const float4 lo = s.bp_tbox[2 * t], hi = s.bp_tbox[2 * t + 1];
+        in = wild || lo.w != 0.0f || !(bp_lower_bound(x, x, xyz(lo), xyz(hi)) > reach);
+        ix = s.bp_tri[t];
+      }
+      // isInClosedOneRing (ipc-contact.ts:586-592) against the query vertex.
+      bool ringed = ix.x == dv || ix.y == dv || ix.z == dv;
+      for (uint32_t j = 0; j < ring.y; ++j) {
+        const uint32_t r = uint32_t(__shfl(int(my_ring), int(j)));
+        ringed = ringed || r == ix.x || r == ix.y || r == ix.z;
+      }
+      in = in && !ringed;
+      const uint64_t hits = __ballot(in);
+      if (kWrite && in) {
+        const uint32_t slot = at + n + uint32_t(__popcll(hits & ((1ull << lane) - 1ull)));
+        if (slot < sz.z) {
+          s.cand[cb + slot] = make_uint4(dv, ix.x, ix.y, ix.z);
+          s.cand_meta[cb + slot] = 0u;
+          s.cand_key[cb + slot] = make_uint2(q, tl);
+        }
+      }
+      n += uint32_t(__popcll(hits));
+    }
+  }
+  if (!kWrite && lane == 0u) s.bp_vcnt[qa] = n;
+}
+template <bool kWrite>
+__global__ void __launch_bounds__(kBlock) k_bp_ee(DeviceState s, PassParams p) {
+  const uint32_t eb = s.bp_list[s.bp_nv + s.bp_ns + blockIdx.x / 64u];
+  const uint32_t sc = s.eblock_scene[eb];
+  if (!bp_building(s, sc)) return;
+  const uint32_t lane = threadIdx.x % 64u;
+  const uint32_t ea = eb * kSceneBlock + (blockIdx.x % 64u) * 4u + threadIdx.x / 64u;
+  const uint32_t e0 = s.scene_blk2[sc].z * kSceneBlock;
+  const uint32_t i = ea - e0;
+  const uint4 sz = s.bp_size[sc];
+  if (i >= sz.y) {  // padding: no matches (uniform per wavefront)
+    if (!kWrite && lane == 0u) s.bp_ecnt[ea] = 0u;
+    return;
+  }
+  const uint2 e = s.bp_edge[ea];
+  const float4 elo = s.bp_ebox[2 * size_t(ea)], ehi = s.bp_ebox[2 * size_t(ea) + 1];
+  const bool wild = elo.w != 0.0f;
+  const uint2 r0 = s.ring_row[e.x], r1 = s.ring_row[e.y];
+  const uint32_t my0 = lane < r0.y ? s.ring_ids[r0.x + lane] : kNoVertex;
+  const uint32_t my1 = lane < r1.y ? s.ring_ids[r1.x + lane] : kNoVertex;
+  const uint32_t ne = sz.y, nchunk = (ne + 63u) / 64u;
+  const float reach = s.bp_reach[sc].x;
+  const uint32_t cb = s.scene_blk1[sc].z * kSceneBlock, at = kWrite ? s.bp_ecnt[ea] : 0u;
+  uint32_t n = 0;
+  for (uint32_t c0 = (i + 1u) / 64u; c0 < nchunk; c0 += 64u) {
+    bool pass = false;
+    if (c0 + lane < nchunk) {
+      const size_t ci = e0 / 64u + c0 + lane;
+      const float4 lo = s.bp_echunk[2 * ci], hi = s.bp_echunk[2 * ci + 1];
+      pass = wild || lo.w != 0.0f || !(bp_lower_bound(xyz(elo), xyz(ehi), xyz(lo), xyz(hi)) > reach);
+    }
+    uint64_t chunks = __ballot(pass);
+    while (chunks) {
+      const uint32_t k = uint32_t(__builtin_ctzll(chunks));
+      chunks &= chunks - 1ull;
+      const uint32_t jl = (c0 + k) * 64u + lane;
+      bool in = false;
+      uint2 o = make_uint2(kNoVertex, kNoVertex);
+      if (jl > i && jl < ne) {
+        const size_t j = e0 + jl;
+        const float4 lo = s.bp_ebox[2 * j], hi = s.bp_ebox[2 * j + 1];
+        in = wild || lo.w != 0.0f || !(bp_lower_bound(xyz(elo), xyz(ehi), xyz(lo), xyz(hi)) > reach);
+        o = s.bp_edge[j];
+      }
+      // Topologically adjacent edges (ipc-contact.ts:647-651): an endpoint of
+      // one in the closed one-ring of an endpoint of the other.
+      bool adjacent = o.x == e.x || o.x == e.y || o.y == e.x || o.y == e.y;
+      for (uint32_t j = 0; j < r0.y; ++j) {
+        const uint32_t r = uint32_t(__shfl(int(my0), int(j)));
+        adjacent = adjacent || r == o.x || r == o.y;
+      }
+      for (uint32_t j = 0; j < r1.y; ++j) {
+        const uint32_t r = uint32_t(__shfl(int(my1), int(j)));
+        adjacent = adjacent || r == o.x || r == o.y;
+      }
+      in = in && !adjacent;
+      const uint64_t hits = __ballot(in);
+      if (kWrite && in) {
+        const uint32_t slot = at + n + uint32_t(__popcll(hits & ((1ull << lane) - 1ull)));
+        if (slot < sz.z) {
+          s.cand[cb + slot] = make_uint4(e.x, e.y, o.x, o.y);
+          s.cand_meta[cb + slot] = 1u;
+          s.cand_key[cb + slot] = make_uint2(i | 0x80000000u, jl);
+        }
+      }
+      n += uint32_t(__popcll(hits));
+    }
+  }
+  if (!kWrite && lane == 0u) s.bp_ecnt[ea] = n;
+}
+// Scans: block sums of a count array over the listed blocks, one serial pass
+// per scene over its blocks in order, then exclusive offsets in place.
+enum : uint32_t { kBpScanVt = 0, kBpScanEe = 1, kBpScanRows = 2 };
+__device__ __forceinline__ uint32_t block_sum_u(uint32_t v, uint32_t* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) sh[threadIdx.x] += sh[threadIdx.x + stride];
+    __syncthreads();
+  }
+  const uint32_t r = sh[0];
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ uint32_t block_exclusive_u(uint32_t v, uint32_t* sh) {  // Hillis-Steele
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t d = 1; d < kBlock; d <<= 1) {
+    const uint32_t add = threadIdx.x >= d ? sh[threadIdx.x - d] : 0u;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  return sh[threadIdx.x] - v;
+}
+struct BpScan {
+  uint32_t* data;
+  uint32_t* blk;
+  const uint32_t* scene_of;
+};
+__device__ __forceinline__ BpScan bp_scan_target(const DeviceState& s, uint32_t which) {
+  if (which == kBpScanVt) return BpScan{s.bp_vcnt, s.bp_blk_v, s.vblock_scene};
+  if (which == kBpScanEe) return BpScan{s.bp_ecnt, s.bp_blk_e, s.eblock_scene};
+  return BpScan{s.bp_rowcnt, s.bp_blk_r, s.vblock_scene};
+}
+__device__ __forceinline__ uint32_t bp_list_block(const DeviceState& s, uint32_t which, uint32_t i) {
+  return s.bp_list[which == kBpScanEe ? s.bp_nv + s.bp_ns + i : i];
+}
+__global__ void __launch_bounds__(kBlock) k_bp_block_sums(DeviceState s, uint32_t which) {
+  __shared__ uint32_t sh[kBlock];
+  const uint32_t blk = bp_list_block(s, which, blockIdx.x);
+  const BpScan t = bp_scan_target(s, which);
+  if (!bp_building(s, t.scene_of[blk])) return;
+  const uint32_t v = t.data[blk * kSceneBlock + threadIdx.x];
+  const uint32_t sum = block_sum_u(v, sh);
+  if (threadIdx.x == 0) t.blk[blk] = sum;
+}
+// One thread per scene. VT offsets run over the vertex blocks, EE offsets
+// continue after the VT total; the row scan runs over the vertex blocks.
+__global__ void __launch_bounds__(kBlock) k_bp_scene_scan(DeviceState s, PassParams p, uint32_t rows) {
+  const uint32_t sc = blockIdx.x * kBlock + threadIdx.x;
+  if (sc >= p.scene_count || !bp_building(s, sc) || s.scene_cnt[sc].z == 0u) return;
+  const uint4 b0 = s.scene_blk0[sc], b2 = s.scene_blk2[sc];
+  uint32_t run = 0;
+  if (rows) {
+    for (uint32_t b = b0.x; b < b0.y; ++b) {
+      const uint32_t c = s.bp_blk_r[b];
+      s.bp_blk_r[b] = run;
+      run += c;
+    }
+    return;
+  }
+  for (uint32_t b = b0.x; b < b0.y; ++b) {
+    const uint32_t c = s.bp_blk_v[b];
+    s.bp_blk_v[b] = run;
+    run += c;
+  }
+  const uint32_t vt = run;
+  for (uint32_t b = b2.z; b < b2.w; ++b) {
+    const uint32_t c = s.bp_blk_e[b];
+    s.bp_blk_e[b] = run;
+    run += c;
+  }
+  uint4 ctl = s.bp_ctl[sc];
+  const uint32_t cap = s.bp_size[sc].z;
+  ctl.y = min(vt, cap);
+  ctl.z = min(run, cap);
+  ctl.w = run;
+  if (run > cap) ctl.x |= kBpOverflow;
+  s.bp_ctl[sc] = ctl;
+  uint4 end = s.scene_end[sc];
+  end.w = s.scene_blk1[sc].z * kSceneBlock + ctl.z;
+  s.scene_end[sc] = end;
+}
+__global__ void __launch_bounds__(kBlock) k_bp_scan_apply(DeviceState s, uint32_t which) {
+  __shared__ uint32_t sh[kBlock];
+  const uint32_t blk = bp_list_block(s, which, blockIdx.x);
+  const BpScan t = bp_scan_target(s, which);
+  if (!bp_building(s, t.scene_of[blk])) return;
+  const size_t i = size_t(blk) * kSceneBlock + threadIdx.x;
+  const uint32_t v = t.data[i];
+  t.data[i] = t.blk[blk] + block_exclusive_u(v, sh);
+}
+// The list before a rebuild, for lagged-friction look-ups.
+__global__ void __launch_bounds__(kBlock) k_bp_save_prev(DeviceState s) {
+  const uint32_t blk = s.bp_list[s.bp_nv + s.bp_ns + s.bp_ne + blockIdx.x];
+  const uint32_t sc = s.cblock_scene[blk];
+  if (!bp_building(s, sc)) return;
+  const uint32_t c = blk * kSceneBlock + threadIdx.x;
+  if (c - s.scene_blk1[sc].z * kSceneBlock >= s.bp_size[sc].w) return;
+  s.prev_key[c] = s.cand_key[c];
+  s.prev_lag_n[c] = s.lag_n[c];
+  s.prev_lag_w[c] = s.lag_w[c];
+  s.prev_src_c[c] = s.src_c[c];
+  s.prev_src_w[c] = s.src_w[c];
+}
+// Vertex -> candidate rows: counts, offsets (scan), fill, sort.
+__global__ void __launch_bounds__(kBlock) k_bp_row_zero(DeviceState s) {
+  const uint32_t vb = s.bp_list[blockIdx.x];
+  if (!bp_building(s, s.vblock_scene[vb])) return;
+  const uint32_t v = vb * kSceneBlock + threadIdx.x;
+  s.bp_rowcnt[v] = 0u;
+  s.bp_rowfill[v] = 0u;
+}
+template <bool kFill>
+__global__ void __launch_bounds__(kBlock) k_bp_rows(DeviceState s, PassParams p, uint32_t carry) {
+  const uint32_t blk = s.bp_list[s.bp_nv + s.bp_ns + s.bp_ne + blockIdx.x];
+  const uint32_t sc = s.cblock_scene[blk];
+  if (!bp_building(s, sc)) return;
+  const uint32_t c = blk * kSceneBlock + threadIdx.x;
+  if (c >= s.scene_end[sc].w) return;
+  const uint4 ix = s.cand[c];
+  if (!kFill) {
+    atomicAdd(s.bp_rowcnt + ix.x, 1u);
+    atomicAdd(s.bp_rowcnt + ix.y, 1u);
+    atomicAdd(s.bp_rowcnt + ix.z, 1u);
+    atomicAdd(s.bp_rowcnt + ix.w, 1u);
+    return;
+  }
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const uint32_t v = tet_vertex(ix, l);
+    s.cand_ids[s.cand_row[v].x + atomicAdd(s.bp_rowfill + v, 1u)] = c;
+  }
+  if (!carry) return;
+  // A rebuild inside a frame. A pair already listed keeps its lagged
+  // friction data and source contact cache. A new pair had no lagged force
+  // at the frame start (every such pair stays listed all frame), and its
+  // cache is what the superset would hold: the contact at the pose of the
+  // frame start's lag pass or of the last promoted validation
+  // (S:2713-2744, S:2695-2710).
+  const uint2 key = s.cand_key[c];
+  const uint32_t first = s.scene_blk1[sc].z * kSceneBlock, last = first + s.bp_size[sc].w;
+  uint32_t lo = first, hi = last;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2u;
+    const uint2 k = s.prev_key[mid];
+    if (k.x < key.x || (k.x == key.x && k.y < key.y)) lo = mid + 1u;
+    else hi = mid;
+  }
+  if (lo < last && s.prev_key[lo].x == key.x && s.prev_key[lo].y == key.y) {
+    s.lag_n[c] = s.prev_lag_n[lo];
+    s.lag_w[c] = s.prev_lag_w[lo];
+    s.src_c[c] = s.prev_src_c[lo];
+    s.src_w[c] = s.prev_src_w[lo];
+  } else {
+    s.lag_n[c] = make_float4(0, 0, 0, 0);
+    s.lag_w[c] = make_float4(0, 0, 0, 0);
+    ipc_store_cached(s, c, false, ipc_contact_at(s, c, s.bp_cpose, ipc_broad_threshold(p, s.scene_scale[sc])));
+  }
+}
+__global__ void __launch_bounds__(kBlock) k_bp_row_offsets(DeviceState s) {
+  const uint32_t vb = s.bp_list[blockIdx.x];
+  const uint32_t sc = s.vblock_scene[vb];
+  if (!bp_building(s, sc)) return;
+  const uint32_t v = vb * kSceneBlock + threadIdx.x;
+  // bp_rowcnt holds exclusive offsets after the scan; the count is the
+  // difference to the next vertex (or to four times the list length).
+  const uint32_t end = 4u * s.bp_ctl[sc].z;
+  const uint32_t last = s.scene_blk0[sc].y * kSceneBlock;
+  const uint32_t at = s.bp_rowcnt[v];
+  const uint32_t next = v + 1u < last ? s.bp_rowcnt[v + 1u] : end;
+  s.cand_row[v] = make_uint2(s.bp_rows_at[sc] + at, next - at);
+}
+constexpr uint32_t kBpMaxRow = 8192;
+// One block per vertex: bitonic sort of its row in LDS (ascending ids, the
+// order the local solve sums contact rows in).
+__global__ void __launch_bounds__(kBlock) k_bp_row_sort(DeviceState s) {
+  __shared__ uint32_t keys[kBpMaxRow];
+  const uint32_t vb = s.bp_list[blockIdx.x / kSceneBlock];
+  const uint32_t sc = s.vblock_scene[vb];
+  if (!bp_building(s, sc)) return;
+  const uint32_t v = vb * kSceneBlock + blockIdx.x % kSceneBlock;
+  const uint2 row = s.cand_row[v];
+  const uint32_t n = row.y;
+  if (n < 2u) return;
+  if (n > kBpMaxRow) {
+    if (threadIdx.x == 0) atomicOr(&s.bp_ctl[sc].x, kBpOverflow);
+    return;
+  }
+  uint32_t m = 2;
+  while (m < n) m <<= 1;
+  for (uint32_t i = threadIdx.x; i < m; i += kBlock) keys[i] = i < n ? s.cand_ids[row.x + i] : 0xffffffffu;
+  __syncthreads();
+  for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < m; i += kBlock) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const uint32_t a = keys[i], b = keys[l];
+          if ((a > b) == ((i & k) == 0u)) {
+            keys[i] = b;
+            keys[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) s.cand_ids[row.x + i] = keys[i];
+}
+// The build pose of every rebuilt scene.
+__global__ void __launch_bounds__(kBlock) k_bp_commit(DeviceState s, PassParams p) {
+  const uint32_t vb = s.bp_list[blockIdx.x];
+  if (!bp_building(s, s.vblock_scene[vb])) return;
+  const uint32_t v = vb * kSceneBlock + threadIdx.x;
+  s.bp_x[v] = s.pos[p.src][v];
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
@@ -3433,15 +3958,51 @@ void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t 
   hipLaunchKernelGGL(k_candidate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
   hipLaunchKernelGGL(k_reduce_contact_partials, dim3(cb), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_reduce_contact_final, dim3(p.scene_count), dim3(kBlock), 0, st, s);
-  hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p, 1u);
   hipLaunchKernelGGL(k_validate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
   hipLaunchKernelGGL(k_reduce_contact_partials, dim3(cb), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_reduce_contact_final, dim3(p.scene_count), dim3(kBlock), 0, st, s);
-  hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p, 0u);
 }
 void launch_promote_contact(const DeviceState& s, const PassParams& p, hipStream_t st) {
   if (s.NC == 0) return;
   hipLaunchKernelGGL(k_promote_contact, dim3(blocks_for(s.NC)), dim3(kBlock), 0, st, s, p);
+}
+// Broad-phase check point: decide per scene, then (for the scenes that
+// rebuild) AABBs, counts, offsets, records and vertex rows. Every kernel
+// runs over the arena blocks of broad-phase scenes and skips scenes that do
+// not rebuild, so a frame's launches are fixed (graph capture).
+void launch_broad_phase(const DeviceState& s, const PassParams& p, bool check, bool carry, hipStream_t st) {
+  if (s.bp_nv == 0) return;
+  const dim3 B(kBlock);
+  const dim3 scenes(blocks_for(p.scene_count));
+  const uint32_t cr = carry ? 1u : 0u;
+  if (check) {
+    hipLaunchKernelGGL(k_bp_motion, dim3(s.bp_nv), B, 0, st, s, p);
+    hipLaunchKernelGGL(k_bp_decide<true>, scenes, B, 0, st, s, p, cr);
+  } else {
+    hipLaunchKernelGGL(k_bp_decide<false>, scenes, B, 0, st, s, p, cr);
+  }
+  if (s.bp_ns + s.bp_ne) hipLaunchKernelGGL(k_bp_bounds, dim3(s.bp_ns + s.bp_ne), B, 0, st, s, p);
+  hipLaunchKernelGGL(k_bp_vt<false>, dim3(64 * s.bp_nv), B, 0, st, s, p);
+  if (s.bp_ne) hipLaunchKernelGGL(k_bp_ee<false>, dim3(64 * s.bp_ne), B, 0, st, s, p);
+  hipLaunchKernelGGL(k_bp_block_sums, dim3(s.bp_nv), B, 0, st, s, uint32_t(kBpScanVt));
+  if (s.bp_ne) hipLaunchKernelGGL(k_bp_block_sums, dim3(s.bp_ne), B, 0, st, s, uint32_t(kBpScanEe));
+  hipLaunchKernelGGL(k_bp_scene_scan, scenes, B, 0, st, s, p, 0u);
+  hipLaunchKernelGGL(k_bp_scan_apply, dim3(s.bp_nv), B, 0, st, s, uint32_t(kBpScanVt));
+  if (s.bp_ne) hipLaunchKernelGGL(k_bp_scan_apply, dim3(s.bp_ne), B, 0, st, s, uint32_t(kBpScanEe));
+  if (carry && s.bp_nc) hipLaunchKernelGGL(k_bp_save_prev, dim3(s.bp_nc), B, 0, st, s);
+  hipLaunchKernelGGL(k_bp_vt<true>, dim3(64 * s.bp_nv), B, 0, st, s, p);
+  if (s.bp_ne) hipLaunchKernelGGL(k_bp_ee<true>, dim3(64 * s.bp_ne), B, 0, st, s, p);
+  hipLaunchKernelGGL(k_bp_row_zero, dim3(s.bp_nv), B, 0, st, s);
+  if (s.bp_nc) hipLaunchKernelGGL(k_bp_rows<false>, dim3(s.bp_nc), B, 0, st, s, p, 0u);
+  hipLaunchKernelGGL(k_bp_block_sums, dim3(s.bp_nv), B, 0, st, s, uint32_t(kBpScanRows));
+  hipLaunchKernelGGL(k_bp_scene_scan, scenes, B, 0, st, s, p, 1u);
+  hipLaunchKernelGGL(k_bp_scan_apply, dim3(s.bp_nv), B, 0, st, s, uint32_t(kBpScanRows));
+  hipLaunchKernelGGL(k_bp_row_offsets, dim3(s.bp_nv), B, 0, st, s);
+  if (s.bp_nc) hipLaunchKernelGGL(k_bp_rows<true>, dim3(s.bp_nc), B, 0, st, s, p, cr);
+  hipLaunchKernelGGL(k_bp_row_sort, dim3(kSceneBlock * s.bp_nv), B, 0, st, s);
+  hipLaunchKernelGGL(k_bp_commit, dim3(s.bp_nv), B, 0, st, s, p);
 }
 
 }  // namespace jgs2

@@ -183,6 +183,219 @@ double min_static_contact_distance(const float* positions4, const jgs2_contact_i
   return minimum;
 }
 
+jgs2_status canonical_surface(uint32_t V, const uint32_t* triangles, uint32_t triangle_count, const uint32_t* edges,
+                              uint32_t edge_count, Surface& out) {
+  if ((triangle_count && !triangles) || (edge_count && !edges))
+    return fail(JGS2_ERROR_TYPE, "NULL contact-surface array.");
+  out = Surface{};
+  for (uint32_t t = 0; t < triangle_count; ++t) {  // canonicalTriangles (ipc-contact.ts:500-527)
+    std::array<uint32_t, 3> f{triangles[3 * size_t(t)], triangles[3 * size_t(t) + 1], triangles[3 * size_t(t) + 2]};
+    std::sort(f.begin(), f.end());
+    for (uint32_t v : f)
+      if (v >= V) return fail(JGS2_ERROR_RANGE, "Surface vertex index %u is outside [0, %u).", v, V);
+    if (f[0] == f[1] || f[1] == f[2])
+      return fail(JGS2_ERROR_RANGE, "Surface triangles must have three distinct vertices.");
+    out.tris.push_back(f);
+  }
+  std::sort(out.tris.begin(), out.tris.end());
+  out.tris.erase(std::unique(out.tris.begin(), out.tris.end()), out.tris.end());
+  for (uint32_t e = 0; e < edge_count; ++e) {  // canonicalEdges (:546-569)
+    const uint32_t a = edges[2 * size_t(e)], b = edges[2 * size_t(e) + 1];
+    if (a >= V || b >= V) return fail(JGS2_ERROR_RANGE, "Surface vertex index %u is outside [0, %u).", std::max(a, b), V);
+    if (a == b) return fail(JGS2_ERROR_RANGE, "Surface edges must have two distinct vertices.");
+    out.edges.push_back({std::min(a, b), std::max(a, b)});
+  }
+  for (const auto& f : out.tris) {
+    out.edges.push_back({f[0], f[1]});
+    out.edges.push_back({f[1], f[2]});
+    out.edges.push_back({f[0], f[2]});
+  }
+  std::sort(out.edges.begin(), out.edges.end());
+  out.edges.erase(std::unique(out.edges.begin(), out.edges.end()), out.edges.end());
+  out.ring_start.assign(size_t(V) + 1, 0);  // buildOneRing (:571-584)
+  for (const auto& e : out.edges) {
+    out.ring_start[e[0] + 1] += 1;
+    out.ring_start[e[1] + 1] += 1;
+  }
+  for (uint32_t v = 0; v < V; ++v) out.ring_start[v + 1] += out.ring_start[v];
+  out.ring.assign(out.ring_start[V], 0);
+  std::vector<uint32_t> cursor(out.ring_start.begin(), out.ring_start.end() - 1);
+  for (const auto& e : out.edges) {
+    out.ring[cursor[e[0]]++] = e[1];
+    out.ring[cursor[e[1]]++] = e[0];
+  }
+  for (uint32_t v = 0; v < V; ++v) std::sort(out.ring.begin() + out.ring_start[v], out.ring.begin() + out.ring_start[v + 1]);
+  return JGS2_OK;
+}
+
+namespace {
+
+bool in_closed_ring(const Surface& s, uint32_t a, uint32_t b) {  // isInClosedOneRing (:586-592)
+  return a == b || std::binary_search(s.ring.begin() + s.ring_start[a], s.ring.begin() + s.ring_start[a + 1], b);
+}
+
+struct Box {
+  float lo[3], hi[3];
+  bool wild;
+};
+
+float box_lower_bound(const Box& a, const Box& b) {  // ipcAabbDistanceLowerBound (S:525-550)
+  float g[3];
+  for (int k = 0; k < 3; ++k) g[k] = std::fmax(std::fmax(b.lo[k] - a.hi[k], a.lo[k] - b.hi[k]), 0.0f);
+  const float xx = g[0] * g[0], yy = g[1] * g[1], zz = g[2] * g[2];
+  return std::sqrt((xx + yy) + zz);
+}
+
+Box box_of(const float* positions4, std::initializer_list<uint32_t> ids) {
+  Box b{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}, false};
+  for (uint32_t v : ids)
+    for (int k = 0; k < 3; ++k) {
+      const float c = positions4[4 * size_t(v) + k];
+      if (!std::isfinite(c)) b.wild = true;
+      b.lo[k] = std::fmin(b.lo[k], c);
+      b.hi[k] = std::fmax(b.hi[k], c);
+    }
+  return b;
+}
+
+// A uniform grid over the finite points' bounds: every item is listed in
+// each cell its box, grown by `grow`, overlaps.
+struct Grid {
+  double origin[3] = {0, 0, 0}, cell = 1;
+  int64_t dims[3] = {1, 1, 1};
+  std::vector<uint64_t> keys;  // (cell << 32) | item, sorted
+  void cells_of(const Box& b, double grow, int64_t lo[3], int64_t hi[3]) const {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::clamp<int64_t>(int64_t(std::floor((double(b.lo[k]) - grow - origin[k]) / cell)), 0, dims[k] - 1);
+      hi[k] = std::clamp<int64_t>(int64_t(std::floor((double(b.hi[k]) + grow - origin[k]) / cell)), 0, dims[k] - 1);
+    }
+  }
+  uint64_t index(int64_t x, int64_t y, int64_t z) const { return uint64_t((z * dims[1] + y) * dims[0] + x); }
+  void insert(const std::vector<Box>& boxes, double grow) {
+    keys.clear();
+    for (size_t i = 0; i < boxes.size(); ++i) {
+      if (boxes[i].wild) continue;
+      int64_t lo[3], hi[3];
+      cells_of(boxes[i], grow, lo, hi);
+      for (int64_t z = lo[2]; z <= hi[2]; ++z)
+        for (int64_t y = lo[1]; y <= hi[1]; ++y)
+          for (int64_t x = lo[0]; x <= hi[0]; ++x) keys.push_back(index(x, y, z) << 32 | uint64_t(i));
+    }
+    std::sort(keys.begin(), keys.end());
+  }
+  template <class F> void visit(uint64_t c, F&& f) const {
+    auto it = std::lower_bound(keys.begin(), keys.end(), c << 32);
+    for (; it != keys.end() && (*it >> 32) == c; ++it) f(uint32_t(*it & 0xffffffffu));
+  }
+};
+
+}  // namespace
+
+bool superset_nonempty(const Surface& s, uint32_t V) {
+  for (const auto& f : s.tris) {  // a vertex outside a triangle's closed rings
+    uint64_t near = 3 + (s.ring_start[f[0] + 1] - s.ring_start[f[0]]) + (s.ring_start[f[1] + 1] - s.ring_start[f[1]]) +
+                    (s.ring_start[f[2] + 1] - s.ring_start[f[2]]);
+    if (near < V) return true;
+    for (uint32_t v = 0; v < V; ++v)
+      if (!in_closed_ring(s, v, f[0]) && !in_closed_ring(s, v, f[1]) && !in_closed_ring(s, v, f[2])) return true;
+  }
+  for (size_t i = 0; i < s.edges.size(); ++i)
+    for (size_t j = i + 1; j < s.edges.size(); ++j) {
+      const auto& a = s.edges[i];
+      const auto& b = s.edges[j];
+      if (!in_closed_ring(s, a[0], b[0]) && !in_closed_ring(s, a[0], b[1]) && !in_closed_ring(s, a[1], b[0]) &&
+          !in_closed_ring(s, a[1], b[1]))
+        return true;
+    }
+  return false;
+}
+
+SurfaceScan scan_surface_pairs(const Surface& s, uint32_t V, const float* positions4, float reach,
+                               float distance_reach) {
+  SurfaceScan out;
+  std::vector<Box> pts(V), tris(s.tris.size()), edges(s.edges.size());
+  for (uint32_t v = 0; v < V; ++v) pts[v] = box_of(positions4, {v});
+  for (size_t t = 0; t < s.tris.size(); ++t) tris[t] = box_of(positions4, {s.tris[t][0], s.tris[t][1], s.tris[t][2]});
+  for (size_t e = 0; e < s.edges.size(); ++e) edges[e] = box_of(positions4, {s.edges[e][0], s.edges[e][1]});
+  Grid grid;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (const Box& b : pts)
+    if (!b.wild)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], double(b.lo[k]));
+        hi[k] = std::max(hi[k], double(b.hi[k]));
+      }
+  const double grow = std::isfinite(reach) ? double(reach) * (1 + 1e-6) + 1e-12 : INFINITY;
+  const bool everything = !std::isfinite(grow) || !(lo[0] <= hi[0]);
+  if (!everything) {
+    const double extent = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-9});
+    grid.cell = std::max(grow, extent / 128.0);
+    for (int k = 0; k < 3; ++k) {
+      grid.origin[k] = lo[k];
+      grid.dims[k] = std::max<int64_t>(1, int64_t(std::ceil((hi[k] - lo[k]) / grid.cell)) + 1);
+    }
+  }
+  auto consider = [&](bool vt, const Box& a, const Box& b, std::initializer_list<uint32_t> ids) {
+    const float lb = box_lower_bound(a, b);
+    if (!(a.wild || b.wild || !(lb > reach))) return;
+    (vt ? out.vertex_triangle : out.edge_edge) += 1;
+    if (a.wild || b.wild || lb > distance_reach) return;
+    double p[4][3];
+    int n = 0;
+    for (uint32_t v : ids) {
+      for (int k = 0; k < 3; ++k) p[n][k] = positions4[4 * size_t(v) + k];
+      ++n;
+    }
+    const double d = vt ? vertex_triangle_distance(p[0], p[1], p[2], p[3]) : edge_edge_distance(p[0], p[1], p[2], p[3]);
+    out.min_distance = std::min(out.min_distance, d);
+  };
+  // VT: each point against the triangles listed in its cell.
+  if (!everything) grid.insert(tris, grow);
+  for (uint32_t v = 0; v < V; ++v) {
+    auto test = [&](uint32_t t) {
+      const auto& f = s.tris[t];
+      if (in_closed_ring(s, v, f[0]) || in_closed_ring(s, v, f[1]) || in_closed_ring(s, v, f[2])) return;
+      consider(true, pts[v], tris[t], {v, f[0], f[1], f[2]});
+    };
+    if (everything || pts[v].wild) {
+      for (uint32_t t = 0; t < s.tris.size(); ++t) test(t);
+      continue;
+    }
+    int64_t c0[3], c1[3];
+    grid.cells_of(pts[v], 0.0, c0, c1);
+    grid.visit(grid.index(c0[0], c0[1], c0[2]), test);
+    for (uint32_t t = 0; t < s.tris.size(); ++t)
+      if (tris[t].wild) test(t);
+  }
+  // EE: each edge against the later edges listed in the cells it overlaps.
+  if (!everything) grid.insert(edges, grow);
+  std::vector<uint32_t> stamp(s.edges.size(), 0xffffffffu);
+  for (uint32_t i = 0; i < s.edges.size(); ++i) {
+    const auto& a = s.edges[i];
+    auto test = [&](uint32_t j) {
+      if (j <= i || stamp[j] == i) return;
+      stamp[j] = i;
+      const auto& b = s.edges[j];
+      if (in_closed_ring(s, a[0], b[0]) || in_closed_ring(s, a[0], b[1]) || in_closed_ring(s, a[1], b[0]) ||
+          in_closed_ring(s, a[1], b[1]))
+        return;
+      consider(false, edges[i], edges[j], {a[0], a[1], b[0], b[1]});
+    };
+    if (everything || edges[i].wild) {
+      for (uint32_t j = i + 1; j < s.edges.size(); ++j) test(j);
+      continue;
+    }
+    int64_t c0[3], c1[3];
+    grid.cells_of(edges[i], 0.0, c0, c1);
+    for (int64_t z = c0[2]; z <= c1[2]; ++z)
+      for (int64_t y = c0[1]; y <= c1[1]; ++y)
+        for (int64_t x = c0[0]; x <= c1[0]; ++x) grid.visit(grid.index(x, y, z), test);
+    for (uint32_t j = i + 1; j < s.edges.size(); ++j)
+      if (edges[j].wild) test(j);
+  }
+  return out;
+}
+
 namespace {
 
 // rotateEulerXyz (mesh.ts:210-235): x, then y, then z.

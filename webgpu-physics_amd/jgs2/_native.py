@@ -52,6 +52,12 @@ class ContactInput(C.Structure):
         ("vertex_triangle_count", C.c_uint32),
         ("edge_edge_count", C.c_uint32),
         ("packed_indices", u32p),
+        ("surface_triangle_count", C.c_uint32),
+        ("surface_edge_count", C.c_uint32),
+        ("surface_triangles", u32p),
+        ("surface_edges", u32p),
+        ("candidate_capacity", C.c_uint64),
+        ("reach_margin", C.c_double),
     ]
 
 
@@ -177,6 +183,17 @@ class TroughDesc(C.Structure):
 
 # Every exported entry point with its ctypes signature. Tests compare this
 # table with the declarations in include/*.h.
+class ContactInfo(C.Structure):
+    _fields_ = [
+        ("candidates", C.c_uint64),
+        ("vertex_triangle", C.c_uint64),
+        ("capacity", C.c_uint64),
+        ("builds", C.c_uint64),
+        ("reach", C.c_float),
+        ("broad_phase", C.c_uint32),
+    ]
+
+
 class BatchConfig(C.Structure):
     _fields_ = [
         ("max_vertices", C.c_uint64),
@@ -189,6 +206,8 @@ class BatchConfig(C.Structure):
         ("cubature_k", C.c_uint32),
         ("corotated", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("max_surface_triangles", C.c_uint64),
+        ("max_surface_edges", C.c_uint64),
     ]
 
 
@@ -235,6 +254,12 @@ SIGNATURES = {
     "jgs2_read_globalization_control": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
     "jgs2_read_globalization_history": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
     "jgs2_sync": (C.c_int, [C.c_void_p]),
+    "jgs2_read_contact_candidates": (C.c_int, [C.c_void_p, u32p, C.c_uint64, C.POINTER(C.c_uint32),
+                                               C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
+    "jgs2_rebuild_contact_candidates": (C.c_int, [C.c_void_p]),
+    "jgs2_get_contact_info": (C.c_int, [C.c_void_p, C.POINTER(ContactInfo)]),
+    "jgs2_estimate_contact_capacity": (C.c_int, [C.POINTER(SceneInput), C.POINTER(Settings), C.c_uint64,
+                                                 C.POINTER(C.c_uint64)]),
     "jgs2_get_info": (C.c_int, [C.c_void_p, C.POINTER(SolverInfo)]),
     "jgs2_read_schedule_colors": (C.c_int, [C.c_void_p, u32p, C.c_size_t]),
     "jgs2_debug_local_systems": (

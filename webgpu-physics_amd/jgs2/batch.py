@@ -34,6 +34,8 @@ class BatchCapacity:
     max_candidates: int = 0
     max_bodies: int = 1
     corotated: bool = False
+    max_surface_triangles: int = 0
+    max_surface_edges: int = 0
 
     def struct(self) -> N.BatchConfig:
         c = N.BatchConfig()
@@ -46,6 +48,8 @@ class BatchCapacity:
         c.max_scenes = int(self.max_scenes)
         c.cubature_k = int(self.cubature_k)
         c.corotated = 1 if self.corotated else 0
+        c.max_surface_triangles = int(self.max_surface_triangles)
+        c.max_surface_edges = int(self.max_surface_edges)
         return c
 
     def arena_bytes(self) -> int:
@@ -67,10 +71,12 @@ def footprint(input_: GpuInput) -> dict:
     NT = input_.cloth.triangle_count if input_.cloth is not None else 0
     NH = input_.cloth.hinge_count if input_.cloth is not None else 0
     NC = input_.contact.candidate_count if input_.contact is not None else 0
+    NS, NE = input_.contact.canonical_counts() if hasattr(input_.contact, "canonical_counts") else (0, 0)
     bodies = int(input_.vertex_info.reshape(-1, 4)[:, 3].max()) + 1 if V else 1
     return {
         "vertices": 256 * blocks(V), "tets": 256 * blocks(T), "triangles": 256 * blocks(NT), "hinges": NH,
-        "candidates": 256 * blocks(NC), "bodies": bodies,
+        "candidates": 256 * blocks(NC), "bodies": bodies, "surface_triangles": 256 * blocks(NS),
+        "surface_edges": 256 * blocks(NE),
     }
 
 

@@ -23,7 +23,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .solver import ClothData, ContactData, GpuInput
+from .solver import BroadPhaseContact, ClothData, ContactData, GpuInput, estimate_contact_capacity
 
 # Minimal-scene scripted target (src/scenes/index.ts:252-262).
 TARGET_PULL_START_FRAME = 48
@@ -117,20 +117,29 @@ def build_cuboid(desc: N.CuboidDesc) -> GpuInput:
         N.lib.jgs2_scene_destroy(handle)
 
 
-def trough(box_count=10, box_cells=1, seed=0, cubature_k=6, candidates=True) -> GpuInput:
+def trough(box_count=10, box_cells=1, seed=0, cubature_k=6, candidates=True, broad_phase=False,
+           settings=None, reach_margin=0.0, candidate_capacity=0) -> GpuInput:
     """The reference's IPC trough (troughDefinition, src/scenes/index.ts:
     551-631) with its static VT/EE candidate superset; box_cells > 1 and a
     nonzero seed give the finer boxes and seeded placements of the large
-    contact workload."""
+    contact workload. `broad_phase` replaces the superset by the device
+    broad phase over the boundary surface, its capacity sized for
+    `settings` (the trough's by default) unless given."""
     d = N.TroughDesc()
     d.box_count, d.box_cells, d.seed, d.cubature_k = int(box_count), int(box_cells), int(seed), int(cubature_k)
-    d.build_candidates = 1 if candidates else 0
+    d.build_candidates = 1 if (candidates and not broad_phase) else 0
     handle = C.c_void_p()
     N.check(N.lib.jgs2_scene_build_trough(C.byref(d), C.byref(handle)))
     try:
-        return _from_handle(handle)
+        x = _from_handle(handle)
     finally:
         N.lib.jgs2_scene_destroy(handle)
+    if broad_phase:
+        surf = x.extras["surface"]
+        x.contact = BroadPhaseContact(surf["triangles"], surf["edges"], int(candidate_capacity), float(reach_margin))
+        if not candidate_capacity:
+            x.contact.candidate_capacity = estimate_contact_capacity(x, settings or TROUGH_SETTINGS)
+    return x
 
 
 def static_contact_candidates(vertex_count, triangles, edges) -> ContactData:

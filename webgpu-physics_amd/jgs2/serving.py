@@ -81,7 +81,9 @@ def capacity_for_budget(template: Sequence[GpuInput], budget_bytes: int, max_sce
         return BatchCapacity(max_vertices=int(mean["vertices"] * n) + 256, max_tets=int(mean["tets"] * n) + 256,
                              max_scenes=n, cubature_k=k_max, max_triangles=int(mean["triangles"] * n),
                              max_hinges=int(mean["hinges"] * n), max_candidates=int(mean["candidates"] * n),
-                             max_bodies=int(mean["bodies"] * n) + 1)
+                             max_bodies=int(mean["bodies"] * n) + 1,
+                             max_surface_triangles=int(mean["surface_triangles"] * n),
+                             max_surface_edges=int(mean["surface_edges"] * n))
 
     lo, hi = 0, max_scenes
     while lo < hi:

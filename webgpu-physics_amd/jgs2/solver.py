@@ -143,6 +143,59 @@ class ContactData:
 
 
 @dataclass
+class BroadPhaseContact:
+    """IPC contact whose candidates are the static superset's pairs within
+    AABB reach, built and rebuilt on the device (no counterpart in the
+    reference, which enumerates the whole superset on the host,
+    src/simulation/cpu/ipc-contact.ts:600-665). Stepping gives the
+    superset's results bit for bit; Jacobi schedule only.
+    `candidate_capacity` 0 lets the library choose (twice the count at the
+    initial pose plus 1024); `reach_margin` 0 is a quarter of the
+    activation reach."""
+
+    surface_triangles: np.ndarray
+    surface_edges: np.ndarray
+    candidate_capacity: int = 0
+    reach_margin: float = 0.0
+    vertex_triangle_count: int = 0
+    edge_edge_count: int = 0
+
+    def __post_init__(self):
+        self.surface_triangles = np.ascontiguousarray(self.surface_triangles, dtype=np.uint32).reshape(-1)
+        self.surface_edges = np.ascontiguousarray(self.surface_edges, dtype=np.uint32).reshape(-1)
+
+    @property
+    def candidate_count(self) -> int:
+        """Candidate slots the scene occupies in an arena."""
+        return int(self.candidate_capacity)
+
+    @property
+    def surface_triangle_count(self) -> int:
+        return self.surface_triangles.size // 3
+
+    @property
+    def surface_edge_count(self) -> int:
+        return self.surface_edges.size // 2
+
+    def canonical_counts(self):
+        """(unique triangles, unique edges including triangle edges): the
+        surface ranges the scene occupies in an arena."""
+        t = np.unique(np.sort(self.surface_triangles.reshape(-1, 3), axis=1), axis=0)
+        e = np.concatenate([np.sort(self.surface_edges.reshape(-1, 2), axis=1), t[:, [0, 1]], t[:, [1, 2]], t[:, [0, 2]]])
+        return len(t), len(np.unique(e, axis=0)) if len(e) else 0
+
+    def struct(self) -> N.ContactInput:
+        c = N.ContactInput()
+        c.surface_triangle_count = self.surface_triangle_count
+        c.surface_edge_count = self.surface_edge_count
+        c.surface_triangles = N.uptr(self.surface_triangles)
+        c.surface_edges = N.uptr(self.surface_edges)
+        c.candidate_capacity = int(self.candidate_capacity)
+        c.reach_margin = float(self.reach_margin)
+        return c
+
+
+@dataclass
 class GpuInput:
     """Arrays of `JGS2GpuInput` (layout.ts:41-90) as contiguous numpy arrays."""
 
@@ -165,7 +218,7 @@ class GpuInput:
     target_positions: Optional[np.ndarray] = None
     target_stiffnesses: Optional[np.ndarray] = None
     cloth: Optional[ClothData] = None
-    contact: Optional[ContactData] = None
+    contact: Optional[object] = None  # ContactData or BroadPhaseContact
     extras: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -289,6 +342,29 @@ class JGS2GpuSolver:
     def await_idle(self) -> None:
         N.check(N.lib.jgs2_sync(self._h()))
 
+    # -------------------------------------------------------------- contact list
+    def read_contact_candidates(self):
+        """The current IPC candidate list: (packed uint32 [n, 4] in caller
+        vertex ids, VT count, reach of the build; +inf for a static
+        superset)."""
+        vt, total, reach = C.c_uint32(), C.c_uint64(), C.c_float()
+        N.check(N.lib.jgs2_read_contact_candidates(self._h(), None, 0, C.byref(vt), C.byref(total), C.byref(reach)))
+        out = np.zeros((max(int(total.value), 1), 4), np.uint32)
+        N.check(N.lib.jgs2_read_contact_candidates(self._h(), N.uptr(out), int(total.value), C.byref(vt),
+                                                   C.byref(total), C.byref(reach)))
+        return out[: int(total.value)], int(vt.value), float(reach.value)
+
+    def contact_info(self) -> dict:
+        """Contact list facts: length, VT part, capacity, broad-phase builds
+        so far and the last build's reach."""
+        i = N.ContactInfo()
+        N.check(N.lib.jgs2_get_contact_info(self._h(), C.byref(i)))
+        return {name: getattr(i, name) for name, _ in N.ContactInfo._fields_}
+
+    def rebuild_contact_candidates(self) -> None:
+        """Rebuilds a broad-phase scene's list at its current positions."""
+        N.check(N.lib.jgs2_rebuild_contact_candidates(self._h()))
+
     # -------------------------------------------------------------- objectives
     def set_external_force(self, vertex: int, force) -> None:
         f = np.ascontiguousarray(force, dtype=np.float32)
@@ -370,3 +446,13 @@ class JGS2GpuSolver:
         total = C.c_double()
         N.check(N.lib.jgs2_profile_end(self._h(), C.byref(ms), C.byref(n), C.byref(total)))
         return ms.value, n.value, total.value
+
+
+def estimate_contact_capacity(input_: GpuInput, settings: Optional[Mapping] = None) -> int:
+    """The candidate capacity the library chooses for a broad-phase scene
+    given no capacity (0 without contact)."""
+    s, mask = settings_struct(settings)
+    st = input_.struct()
+    out = C.c_uint64()
+    N.check(N.lib.jgs2_estimate_contact_capacity(C.byref(st), C.byref(s), mask, C.byref(out)))
+    return int(out.value)
