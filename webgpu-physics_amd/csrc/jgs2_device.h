@@ -100,6 +100,7 @@ struct DeviceState {
   float4* local_diag = nullptr;  // 4 per vertex
   float4* tet_glob = nullptr;    // 2 per tet
   float4* vertex_energy = nullptr;
+  float2* ipc_energy = nullptr;  // per vertex: owned contact energy at the source and the target pose
   float4* conv = nullptr;        // 5 per vertex
   float4* control = nullptr;     // 4
   float4* history = nullptr;     // 64 * 8
@@ -178,6 +179,7 @@ struct DeviceState {
   uint2* ring_row = nullptr;
   uint32_t* ring_ids = nullptr;
   uint32_t *bp_qv = nullptr, *bp_vcnt = nullptr, *bp_ecnt = nullptr;
+  uint32_t *bp_vprev = nullptr, *bp_eprev = nullptr;  // the query offsets of the last build
   float4 *bp_x = nullptr, *bp_cpose = nullptr, *bp_tval = nullptr;
   uint2 *cand_key = nullptr, *prev_key = nullptr;
   float4 *prev_lag_n = nullptr, *prev_lag_w = nullptr, *prev_src_c = nullptr, *prev_src_w = nullptr;

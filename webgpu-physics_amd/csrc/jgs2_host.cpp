@@ -213,14 +213,14 @@ namespace {
 void free_device(DeviceState& d) {
   void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_rec, d.vcub[0], d.vcub[1], d.tet_k, d.adjacency, d.cub, d.objectives,
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
-                  d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.conv, d.control,
+                  d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
                   d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
                   d.body_rng, d.bp_list, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
-                  d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_x,
+                  d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_vprev, d.bp_eprev, d.bp_x,
                   d.bp_cpose, d.bp_tval, d.cand_key, d.prev_key, d.prev_lag_n, d.prev_lag_w, d.prev_src_c, d.prev_src_w,
                   d.bp_rowcnt, d.bp_rowfill, d.bp_ctl, d.bp_reach, d.bp_motion, d.sblock_scene, d.eblock_scene,
                   d.scene_blk2};
@@ -734,6 +734,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.final_update, Vc, by, dry));
   TRYS(dalloc(&d.local_diag, 4 * Vc, by, dry));
   TRYS(dalloc(&d.vertex_energy, Vc, by, dry));
+  TRYS(dalloc(&d.ipc_energy, Vc, by, dry));
   TRYS(dalloc(&d.conv, 5 * Vc, by, dry));
   TRYS(dalloc(&d.tri_row, Vc, by, dry));
   TRYS(dalloc(&d.hinge_row, Vc, by, dry));
@@ -791,10 +792,11 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
     TRYS(dalloc(&d.bp_ebox, 2 * Ec, by, dry));
     TRYS(dalloc(&d.bp_echunk, 2 * (Ec / 64), by, dry));
     TRYS(dalloc(&d.bp_ecnt, Ec, by, dry));
+    TRYS(dalloc(&d.bp_eprev, Ec, by, dry));
     TRYS(dalloc(&d.bp_blk_e, Ec / kSceneBlock, by, dry));
     TRYS(dalloc(&d.ring_row, Vc, by, dry));
     TRYS(dalloc(&d.ring_ids, 2 * Ec, by, dry));
-    for (uint32_t** a : {&d.bp_qv, &d.bp_vcnt, &d.bp_rowcnt, &d.bp_rowfill}) TRYS(dalloc(a, Vc, by, dry));
+    for (uint32_t** a : {&d.bp_qv, &d.bp_vcnt, &d.bp_vprev, &d.bp_rowcnt, &d.bp_rowfill}) TRYS(dalloc(a, Vc, by, dry));
     for (float4** a : {&d.bp_x, &d.bp_cpose, &d.bp_tval}) TRYS(dalloc(a, Vc, by, dry));
     TRYS(dalloc(&d.bp_blk_v, Vc / kSceneBlock, by, dry));
     TRYS(dalloc(&d.bp_blk_r, Vc / kSceneBlock, by, dry));

@@ -948,27 +948,6 @@ __device__ void ipc_local_contribution(const DeviceState& s, const PassParams& p
 #pragma unroll 1
   for (uint32_t i = begin; i < end; ++i) ipc_row_contribution(s, p, v, pos, s.active_ids[i], g, H);
 }
-// ipcOwnedEnergy (S:750-779): candidates owned by this vertex, read from the
-// source or (after a nonzero safe step) the target contact cache.
-__device__ float ipc_owned_energy(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
-                                  bool use_target) {
-  float r = 0.0f;
-  const uint32_t begin = s.cand_row[v].x, end = begin + s.active_count[v];
-#pragma unroll 1
-  for (uint32_t i = begin; i < end; ++i) {
-    const uint32_t c = s.active_ids[i];
-    const Contact k = ipc_cached(s, c, use_target);
-    if (!k.valid) return kF32Max;
-    if (!ipc_barrier_active(k.distance, p.ipc_dhat) && !ipc_friction_active(s, c)) continue;
-    const uint4 ix = s.cand[c];
-    if (ipc_energy_owner(s, ix) != v) continue;
-    const float e = ipc_energy_from_contact(s, p, c, ix, k, pos, kNoVertex, mk(0, 0, 0));
-    if (!finite_s(e)) return kF32Max;
-    r += e;
-  }
-  return r;
-}
-
 template <bool kCloth, bool kIpc>
 __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const PassParams& p, const float4* pos, uint32_t v,
                                                   bool is_target) {  // S:2557-2645
@@ -1011,9 +990,9 @@ __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const Pa
       e += is_target ? bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0)) : bending_base(s, p, h, pos);
     }
   }
-  if constexpr (kIpc) {  // S:2643 with ipcCachedContactAt (S:450-460)
-    const bool use_target = p.tgt != p.src && is_target && s.contact_ctl[vscene(s, v)].w != 0.0f;
-    e += ipc_owned_energy(s, p, v, pos, use_target);
+  if constexpr (kIpc) {  // S:2643 with ipcCachedContactAt (S:450-460); summed by k_ipc_energy_lanes
+    const float2 owned = s.ipc_energy[v];
+    e += is_target ? owned.y : owned.x;
   }
   return e;
 }
@@ -2277,6 +2256,58 @@ __global__ void __launch_bounds__(kBlock) k_vertex_energy(DeviceState s, PassPar
   const uint32_t sc = vscene(s, v);
   if (sc == kNoScene || (p.energy_after_sweep && stop_latched(s, p, sc))) return;
   store_vertex_energy<kCloth, kIpc>(s, p, v);
+}
+// ipcOwnedEnergy (S:750-779) of every vertex at the source and the target
+// pose, for the vertex-energy pass. Sixteen lanes test a vertex's active
+// rows round robin; the owned terms are then added in row order (ballot),
+// so each sum is the sequential one bit for bit; any invalid row or
+// non-finite term gives the f32 sentinel as the sequential loop's early
+// return does.
+__global__ void __launch_bounds__(kBlock) k_ipc_energy_lanes(DeviceState s, PassParams p) {
+  constexpr uint32_t kLanes = 16;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t v = gid / kLanes, sub = gid % kLanes;
+  if (v >= p.vertex_count) return;  // uniform per 16-lane group
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || (p.energy_after_sweep && stop_latched(s, p, sc))) return;
+  const uint32_t group = (threadIdx.x % 64u) & ~(kLanes - 1u);
+  const uint32_t first = s.cand_row[v].x, last = first + s.active_count[v];
+  float out[2];
+#pragma unroll
+  for (int which = 0; which < 2; ++which) {
+    const float4* pos = which ? s.pos[p.tgt] : s.pos[p.src];
+    const bool use_target = which == 1 && p.tgt != p.src && s.contact_ctl[sc].w != 0.0f;
+    float r = 0.0f;
+    bool bad = false;
+    for (uint32_t base = first; base < last; base += kLanes) {  // same trip count on the group's lanes
+      const uint32_t i = base + sub;
+      bool owned = false;
+      float e = 0.0f;
+      if (i < last) {
+        const uint32_t c = s.active_ids[i];
+        const Contact k = ipc_cached(s, c, use_target);
+        if (!k.valid) {
+          bad = true;
+        } else if (ipc_barrier_active(k.distance, p.ipc_dhat) || ipc_friction_active(s, c)) {
+          const uint4 ix = s.cand[c];
+          if (ipc_energy_owner(s, ix) == v) {
+            e = ipc_energy_from_contact(s, p, c, ix, k, pos, kNoVertex, mk(0, 0, 0));
+            owned = true;
+            bad = bad || !finite_s(e);
+          }
+        }
+      }
+      uint32_t mask = uint32_t(__ballot(owned) >> group) & 0xffffu;
+      while (mask) {
+        const uint32_t j = uint32_t(__builtin_ctz(mask));
+        mask &= mask - 1u;
+        r += __shfl(e, int(group + j));
+      }
+    }
+    bad = (uint32_t(__ballot(bad) >> group) & 0xffffu) != 0u;
+    out[which] = bad ? kF32Max : r;
+  }
+  if (sub == 0) s.ipc_energy[v] = make_float2(out[0], out[1]);
 }
 
 // ------------------------------------------------------------------ candidate reduction
@@ -3659,47 +3690,82 @@ __global__ void __launch_bounds__(kBlock) k_bp_row_zero(DeviceState s) {
   s.bp_rowcnt[v] = 0u;
   s.bp_rowfill[v] = 0u;
 }
+// Adds one to counters[key] for every active lane, lanes of the wavefront
+// with equal keys combined into one atomic; returns each lane's value
+// before its own increment (equal keys ranked in lane order).
+__device__ __forceinline__ uint32_t wave_counted_add(uint32_t* counters, uint32_t key, bool active) {
+  const uint32_t lane = threadIdx.x % 64u;
+  uint64_t pending = __ballot(active);
+  uint32_t rank = 0;
+  while (pending) {
+    const uint32_t leader = uint32_t(__builtin_ctzll(pending));
+    const uint32_t lk = uint32_t(__shfl(int(key), int(leader)));
+    const uint64_t same = __ballot(active && key == lk) & pending;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counters + lk, uint32_t(__popcll(same)));
+    base = uint32_t(__shfl(int(base), int(leader)));
+    if ((same >> lane) & 1ull) rank = base + uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
+    pending &= ~same;
+  }
+  return rank;
+}
+// Row counts (kFill false) or row entries (kFill true) of every listed
+// candidate. Consecutive candidates share their first vertex (the VT point,
+// the first EE edge), so those increments are combined per wavefront.
 template <bool kFill>
 __global__ void __launch_bounds__(kBlock) k_bp_rows(DeviceState s, PassParams p, uint32_t carry) {
   const uint32_t blk = s.bp_list[s.bp_nv + s.bp_ns + s.bp_ne + blockIdx.x];
   const uint32_t sc = s.cblock_scene[blk];
-  if (!bp_building(s, sc)) return;
+  if (!bp_building(s, sc)) return;  // uniform per block
   const uint32_t c = blk * kSceneBlock + threadIdx.x;
-  if (c >= s.scene_end[sc].w) return;
-  const uint4 ix = s.cand[c];
-  if (!kFill) {
-    atomicAdd(s.bp_rowcnt + ix.x, 1u);
-    atomicAdd(s.bp_rowcnt + ix.y, 1u);
-    atomicAdd(s.bp_rowcnt + ix.z, 1u);
-    atomicAdd(s.bp_rowcnt + ix.w, 1u);
-    return;
-  }
-#pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    const uint32_t v = tet_vertex(ix, l);
-    s.cand_ids[s.cand_row[v].x + atomicAdd(s.bp_rowfill + v, 1u)] = c;
-  }
+  const bool live = c < s.scene_end[sc].w;
+  const uint4 ix = live ? s.cand[c] : make_uint4(0, 0, 0, 0);
+  const bool ee = live && s.cand_meta[c] == 1u;
+  uint32_t* counter = kFill ? s.bp_rowfill : s.bp_rowcnt;
+  const uint32_t r0 = wave_counted_add(counter, ix.x, live);
+  const uint32_t r1e = wave_counted_add(counter, ix.y, ee);
+  if (!live) return;
+  const uint32_t r1 = ee ? r1e : atomicAdd(counter + ix.y, 1u);
+  const uint32_t r2 = atomicAdd(counter + ix.z, 1u), r3 = atomicAdd(counter + ix.w, 1u);
+  if (!kFill) return;
+  s.cand_ids[s.cand_row[ix.x].x + r0] = c;
+  s.cand_ids[s.cand_row[ix.y].x + r1] = c;
+  s.cand_ids[s.cand_row[ix.z].x + r2] = c;
+  s.cand_ids[s.cand_row[ix.w].x + r3] = c;
   if (!carry) return;
   // A rebuild inside a frame. A pair already listed keeps its lagged
   // friction data and source contact cache. A new pair had no lagged force
   // at the frame start (every such pair stays listed all frame), and its
   // cache is what the superset would hold: the contact at the pose of the
   // frame start's lag pass or of the last promoted validation
-  // (S:2713-2744, S:2695-2710).
+  // (S:2713-2744, S:2695-2710). The previous list is searched within the
+  // query's range of it.
   const uint2 key = s.cand_key[c];
-  const uint32_t first = s.scene_blk1[sc].z * kSceneBlock, last = first + s.bp_size[sc].w;
-  uint32_t lo = first, hi = last;
+  const uint32_t cb = s.scene_blk1[sc].z * kSceneBlock, old_total = s.bp_size[sc].w;
+  const uint32_t e0 = s.scene_blk2[sc].z * kSceneBlock, ne = s.bp_size[sc].y;
+  uint32_t lo, hi;
+  if (!(key.x & 0x80000000u)) {
+    const uint32_t qa = s.scene_blk0[sc].x * kSceneBlock + key.x;
+    lo = s.bp_vprev[qa];
+    hi = key.x + 1u < s.scene_cnt[sc].x ? s.bp_vprev[qa + 1u] : (ne ? s.bp_eprev[e0] : old_total);
+  } else {
+    const uint32_t i = key.x & 0x7fffffffu;
+    lo = s.bp_eprev[e0 + i];
+    hi = i + 1u < ne ? s.bp_eprev[e0 + i + 1u] : old_total;
+  }
+  hi = min(hi, old_total);
+  lo = min(lo, hi);
   while (lo < hi) {
     const uint32_t mid = (lo + hi) / 2u;
-    const uint2 k = s.prev_key[mid];
-    if (k.x < key.x || (k.x == key.x && k.y < key.y)) lo = mid + 1u;
+    if (s.prev_key[cb + mid].y < key.y) lo = mid + 1u;
     else hi = mid;
   }
-  if (lo < last && s.prev_key[lo].x == key.x && s.prev_key[lo].y == key.y) {
-    s.lag_n[c] = s.prev_lag_n[lo];
-    s.lag_w[c] = s.prev_lag_w[lo];
-    s.src_c[c] = s.prev_src_c[lo];
-    s.src_w[c] = s.prev_src_w[lo];
+  const uint32_t at = cb + lo;
+  if (lo < old_total && s.prev_key[at].x == key.x && s.prev_key[at].y == key.y) {
+    s.lag_n[c] = s.prev_lag_n[at];
+    s.lag_w[c] = s.prev_lag_w[at];
+    s.src_c[c] = s.prev_src_c[at];
+    s.src_w[c] = s.prev_src_w[at];
   } else {
     s.lag_n[c] = make_float4(0, 0, 0, 0);
     s.lag_w[c] = make_float4(0, 0, 0, 0);
@@ -3756,12 +3822,20 @@ __global__ void __launch_bounds__(kBlock) k_bp_row_sort(DeviceState s) {
   }
   for (uint32_t i = threadIdx.x; i < n; i += kBlock) s.cand_ids[row.x + i] = keys[i];
 }
-// The build pose of every rebuilt scene.
+// The build pose of every rebuilt scene, and its query offsets (the ranges
+// a later rebuild inside a frame searches this list in).
 __global__ void __launch_bounds__(kBlock) k_bp_commit(DeviceState s, PassParams p) {
-  const uint32_t vb = s.bp_list[blockIdx.x];
-  if (!bp_building(s, s.vblock_scene[vb])) return;
-  const uint32_t v = vb * kSceneBlock + threadIdx.x;
-  s.bp_x[v] = s.pos[p.src][v];
+  const bool vertices = blockIdx.x < s.bp_nv;
+  const uint32_t blk = s.bp_list[vertices ? blockIdx.x : s.bp_ns + blockIdx.x];
+  const uint32_t i = blk * kSceneBlock + threadIdx.x;
+  if (vertices) {
+    if (!bp_building(s, s.vblock_scene[blk])) return;
+    s.bp_x[i] = s.pos[p.src][i];
+    s.bp_vprev[i] = s.bp_vcnt[i];
+  } else {
+    if (!bp_building(s, s.eblock_scene[blk])) return;
+    s.bp_eprev[i] = s.bp_ecnt[i];
+  }
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
@@ -3875,6 +3949,7 @@ void launch_cache_flush(const DeviceState& s, uint32_t records, hipStream_t st) 
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const dim3 grid(blocks_for(p.vertex_count));
   const bool cl = has_cloth(p), ipc = p.contact != 0u;
+  if (ipc) hipLaunchKernelGGL(k_ipc_energy_lanes, dim3(blocks_for(16u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
   if (cl && ipc) hipLaunchKernelGGL((k_vertex_energy<true, true>), grid, dim3(kBlock), 0, st, s, p);
   else if (cl) hipLaunchKernelGGL((k_vertex_energy<true, false>), grid, dim3(kBlock), 0, st, s, p);
   else if (ipc) hipLaunchKernelGGL((k_vertex_energy<false, true>), grid, dim3(kBlock), 0, st, s, p);
@@ -4002,7 +4077,7 @@ void launch_broad_phase(const DeviceState& s, const PassParams& p, bool check, b
   hipLaunchKernelGGL(k_bp_row_offsets, dim3(s.bp_nv), B, 0, st, s);
   if (s.bp_nc) hipLaunchKernelGGL(k_bp_rows<true>, dim3(s.bp_nc), B, 0, st, s, p, cr);
   hipLaunchKernelGGL(k_bp_row_sort, dim3(kSceneBlock * s.bp_nv), B, 0, st, s);
-  hipLaunchKernelGGL(k_bp_commit, dim3(s.bp_nv), B, 0, st, s, p);
+  hipLaunchKernelGGL(k_bp_commit, dim3(s.bp_nv + s.bp_ne), B, 0, st, s, p);
 }
 
 }  // namespace jgs2

@@ -177,6 +177,24 @@ TROUGH_SETTINGS = {
 }
 TROUGH_ITERATIONS = 3
 
+# The large contact workload: twenty 8x8x8-cell boxes (3,072 tetrahedra
+# each) dropped at seeded offsets and yaw angles into the pinned V-trough;
+# barrier activation 1 mm (minimum distance 0.1 mm, below it as the
+# reference requires), stiffness 1e5, lagged friction 0.3, 12 Jacobi
+# iterations per frame, the trough's step bound; candidates from the device
+# broad phase (the static superset of this scene has about 5e8 pairs).
+CONTACT_BOXES = {"box_count": 20, "box_cells": 8, "seed": 20251203}
+CONTACT_BOXES_SETTINGS = dict(TROUGH_SETTINGS, ipc_activation_distance=1e-3, ipc_minimum_distance=1e-4,
+                              ipc_barrier_stiffness=1e5, ipc_friction_coefficient=0.3)
+CONTACT_BOXES_ITERATIONS = 12
+
+
+def contact_boxes(seed=CONTACT_BOXES["seed"], box_count=CONTACT_BOXES["box_count"],
+                  box_cells=CONTACT_BOXES["box_cells"], candidate_capacity=0) -> GpuInput:
+    """The large contact workload (see CONTACT_BOXES), with broad-phase contact."""
+    return trough(box_count=box_count, box_cells=box_cells, seed=seed, broad_phase=True,
+                  settings=CONTACT_BOXES_SETTINGS, candidate_capacity=candidate_capacity)
+
 
 def cloth_drape(cells=4, origin=(-0.8, 1.15, -0.8), size=(1.6, 1.6), fold_column=3, fold_lift=0.079,
                 collider_origin=(-0.65, 0.0, -0.45), collider_size=(0.9, 0.5, 0.9), density=480.0,
