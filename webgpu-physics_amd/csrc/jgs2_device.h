@@ -187,6 +187,17 @@ struct DeviceState {
   uint4* bp_ctl = nullptr;
   float4* bp_reach = nullptr;
   uint4* bp_motion = nullptr;
+  // Body-relative motion: a pair's AABB bound moves by at most its bodies'
+  // deviations from their reference vertices plus the difference of those
+  // references' displacements, and no pair of two bodies is closer than the
+  // bodies' AABB gap at the build.
+  //   bp_bodies uint4[S]  {first body, body count, 1 if the body rule applies, 0}
+  //   bp_bref   uint32[B] reference (device) vertex of each body
+  //   bp_bdev   uint4[B]  per-axis max deviation from the reference's displacement (float bits)
+  //   bp_bbox   uint4[2B] body AABB at the build (order-preserving integer keys)
+  uint4* bp_bodies = nullptr;
+  uint32_t* bp_bref = nullptr;
+  uint4 *bp_bdev = nullptr, *bp_bbox = nullptr;
   uint32_t *sblock_scene = nullptr, *eblock_scene = nullptr;
   uint4* scene_blk2 = nullptr;  // {sb0, sb1, eb0, eb1}: surface triangle and edge blocks
   // Scene tables.

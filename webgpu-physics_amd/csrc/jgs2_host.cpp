@@ -222,7 +222,7 @@ void free_device(DeviceState& d) {
                   d.body_rng, d.bp_list, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
                   d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_vprev, d.bp_eprev, d.bp_x,
                   d.bp_cpose, d.bp_tval, d.cand_key, d.prev_key, d.prev_lag_n, d.prev_lag_w, d.prev_src_c, d.prev_src_w,
-                  d.bp_rowcnt, d.bp_rowfill, d.bp_ctl, d.bp_reach, d.bp_motion, d.sblock_scene, d.eblock_scene,
+                  d.bp_rowcnt, d.bp_rowfill, d.bp_ctl, d.bp_reach, d.bp_motion, d.bp_bodies, d.bp_bref, d.bp_bdev, d.bp_bbox, d.sblock_scene, d.eblock_scene,
                   d.scene_blk2};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -808,6 +808,10 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
     TRYS(dalloc(&d.bp_size, S, by, dry));
     TRYS(dalloc(&d.bp_rows_at, S, by, dry));
     TRYS(dalloc(&d.bp_motion, 2 * S, by, dry));
+    TRYS(dalloc(&d.bp_bodies, S, by, dry));
+    TRYS(dalloc(&d.bp_bref, Bc, by, dry));
+    TRYS(dalloc(&d.bp_bdev, Bc, by, dry));
+    TRYS(dalloc(&d.bp_bbox, 2 * Bc, by, dry));
     TRYS(dalloc(&d.scene_blk2, S, by, dry));
     TRYS(dalloc(&d.sblock_scene, Sc / kSceneBlock, by, dry));
     TRYS(dalloc(&d.eblock_scene, Ec / kSceneBlock, by, dry));
@@ -1126,6 +1130,26 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
     HIP_CHECK(hipMemcpyAsync(d.bp_rows_at + sl, &rows_at, sizeof(uint32_t), hipMemcpyHostToDevice, q));
     HIP_CHECK(hipMemcpyAsync(d.bp_motion + 2 * size_t(sl), zero2, sizeof(zero2), hipMemcpyHostToDevice, q));
     HIP_CHECK(hipMemcpyAsync(d.scene_blk2 + sl, &blk2, sizeof(uint4), hipMemcpyHostToDevice, q));
+    // The body rule: up to 64 bodies, every surface primitive within one body.
+    std::vector<uint32_t> bref(s->body.n, 0xffffffffu);  // no vertex
+    bool bodies_ok = s->bp && s->body.n <= 64;
+    for (uint32_t nv = 0; nv < V && bodies_ok; ++nv) {
+      const uint32_t body = in.vertex_info[4 * size_t(vold[nv]) + 3];
+      if (bref[body] == 0xffffffffu) bref[body] = uint32_t(VB) + nv;
+    }
+    if (bodies_ok) {
+      auto body_of = [&](uint32_t v) { return in.vertex_info[4 * size_t(v) + 3]; };
+      for (const auto& f : s->surface.tris)
+        bodies_ok = bodies_ok && body_of(f[0]) == body_of(f[1]) && body_of(f[1]) == body_of(f[2]);
+      for (const auto& e : s->surface.edges) bodies_ok = bodies_ok && body_of(e[0]) == body_of(e[1]);
+    }
+    const uint4 bodies = make_uint4(uint32_t(s->body.at), uint32_t(s->body.n), bodies_ok ? 1u : 0u, 0u);
+    HIP_CHECK(hipMemcpyAsync(d.bp_bodies + sl, &bodies, sizeof(uint4), hipMemcpyHostToDevice, q));
+    if (bodies_ok) {
+      std::vector<uint4> zb(s->body.n, make_uint4(0, 0, 0, 0));
+      UP(d.bp_bref, s->body.at, bref);
+      UP(d.bp_bdev, s->body.at, zb);
+    }
   }
   if (s->bp) {  // canonical surface in device ids, one-ring rows, query order
     const Surface& sf = s->surface;

@@ -3325,18 +3325,59 @@ __device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
   return v;
 }
 
+// Order-preserving integer key of a float (for atomic min/max) and back.
+__device__ __forceinline__ uint32_t f2key(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+// Atomic max (or min) of `value` into counters[key], lanes of the wavefront
+// with equal keys combined into one atomic.
+template <bool kMax>
+__device__ __forceinline__ void wave_keyed_extreme(uint32_t* counters, uint32_t key, uint32_t value, bool active) {
+  const uint32_t lane = threadIdx.x % 64u;
+  uint64_t pending = __ballot(active);
+  while (pending) {
+    const uint32_t leader = uint32_t(__builtin_ctzll(pending));
+    const uint32_t lk = uint32_t(__shfl(int(key), int(leader)));
+    const bool mine = active && key == lk && ((pending >> lane) & 1ull);
+    uint32_t v = mine ? value : (kMax ? 0u : 0xffffffffu);
+#pragma unroll
+    for (uint32_t m = 32; m >= 1; m >>= 1) {
+      const uint32_t o = uint32_t(__shfl_xor(int(v), int(m)));
+      v = kMax ? max(v, o) : min(v, o);
+    }
+    if (lane == leader) {
+      if (kMax) atomicMax(counters + lk, v);
+      else atomicMin(counters + lk, v);
+    }
+    pending &= ~__ballot(mine);
+  }
+}
 // Per-axis motion of every broad-phase scene since its last build and since
-// the frame started (maxima as float bits, order independent).
+// the frame started, and each body's deviation from its reference vertex's
+// motion since the build (maxima as float bits, order independent).
 __global__ void __launch_bounds__(kBlock) k_bp_motion(DeviceState s, PassParams p) {
   const uint32_t vb = s.bp_list[blockIdx.x];
   const uint32_t sc = s.vblock_scene[vb];
-  if (sc == kNoScene) return;
+  if (sc == kNoScene) return;  // uniform per block
   const uint32_t v = vb * kSceneBlock + threadIdx.x;
-  uint32_t m[6] = {0, 0, 0, 0, 0, 0};
-  if (v - s.scene_blk0[sc].x * kSceneBlock < s.scene_cnt[sc].x) {
+  const bool real = v - s.scene_blk0[sc].x * kSceneBlock < s.scene_cnt[sc].x;
+  const bool bodies = s.bp_bodies[sc].z != 0u;
+  uint32_t m[6] = {0, 0, 0, 0, 0, 0}, dev[3] = {0, 0, 0}, body = 0;
+  if (real) {
     const f3 x = load_pos(s.pos[p.src], v), b = xyz(s.bp_x[v]), o = load_pos(s.old, v);
     m[0] = bp_bits(x.x - b.x), m[1] = bp_bits(x.y - b.y), m[2] = bp_bits(x.z - b.z);
     m[3] = bp_bits(x.x - o.x), m[4] = bp_bits(x.y - o.y), m[5] = bp_bits(x.z - o.z);
+    if (bodies) {
+      body = s.vtx_info[v].w;
+      const uint32_t r = s.bp_bref[body];
+      const f3 d = load_pos(s.pos[p.src], r) - xyz(s.bp_x[r]);
+      const f3 e = (x - b) - d;
+      dev[0] = bp_bits(e.x), dev[1] = bp_bits(e.y), dev[2] = bp_bits(e.z);
+    }
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) m[k] = wave_max_u(m[k]);
@@ -3345,6 +3386,50 @@ __global__ void __launch_bounds__(kBlock) k_bp_motion(DeviceState s, PassParams 
     for (int k = 0; k < 3; ++k) atomicMax(acc + k, m[k]);
     for (int k = 0; k < 3; ++k) atomicMax(acc + 4 + k, m[3 + k]);
   }
+  if (bodies) {  // uniform per block
+    uint32_t* bd = reinterpret_cast<uint32_t*>(s.bp_bdev);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wave_keyed_extreme<true>(bd + k, 4u * body, dev[k], real);
+  }
+}
+// Per scene with the body rule: is every unlisted pair of every two bodies
+// (or of one body) still beyond the activation reach?
+__device__ bool bp_bodies_keep(const DeviceState& s, const PassParams& p, uint32_t sc, float reach, float act) {
+  const uint4 bd = s.bp_bodies[sc];
+  const float4* pos = s.pos[p.src];
+  const float slack = 1.0e-4f * act;
+  for (uint32_t a = bd.x; a < bd.x + bd.y; ++a) {
+    const uint32_t ra = s.bp_bref[a];
+    if (ra == kNoVertex) continue;
+    const f3 da = load_pos(pos, ra) - xyz(s.bp_x[ra]);
+    const uint4 ea4 = s.bp_bdev[a];
+    const f3 ea = mk(__uint_as_float(ea4.x), __uint_as_float(ea4.y), __uint_as_float(ea4.z));
+    const uint4 alo = s.bp_bbox[2 * size_t(a)], ahi = s.bp_bbox[2 * size_t(a) + 1];
+    for (uint32_t b = a; b < bd.x + bd.y; ++b) {
+      const uint32_t rb = s.bp_bref[b];
+      if (rb == kNoVertex) continue;
+      f3 bound;
+      float base = reach;
+      if (b == a) {
+        bound = 2.0f * ea;
+      } else {
+        const f3 db = load_pos(pos, rb) - xyz(s.bp_x[rb]);
+        const uint4 eb4 = s.bp_bdev[b];
+        const f3 eb = mk(__uint_as_float(eb4.x), __uint_as_float(eb4.y), __uint_as_float(eb4.z));
+        const f3 dd = da - db;
+        bound = ea + eb + mk(fabsf(dd.x), fabsf(dd.y), fabsf(dd.z));
+        const uint4 blo = s.bp_bbox[2 * size_t(b)], bhi = s.bp_bbox[2 * size_t(b) + 1];
+        const float gap = bp_lower_bound(mk(key2f(alo.x), key2f(alo.y), key2f(alo.z)),
+                                         mk(key2f(ahi.x), key2f(ahi.y), key2f(ahi.z)),
+                                         mk(key2f(blo.x), key2f(blo.y), key2f(blo.z)),
+                                         mk(key2f(bhi.x), key2f(bhi.y), key2f(bhi.z)));
+        if (finite_s(gap)) base = fmaxf(base, gap);
+      }
+      const float grow = sqrtf(dot(bound, bound));
+      if (!(base - grow > act + slack)) return false;
+    }
+  }
+  return true;
 }
 // Per scene: rebuild now? Motion m since the build moves an AABB lower bound
 // by at most 2|m| (m the per-axis maxima), so the list stays complete while
@@ -3369,6 +3454,17 @@ __global__ void __launch_bounds__(kBlock) k_bp_decide(DeviceState s, PassParams 
     const float bx = __uint_as_float(mb.x), by = __uint_as_float(mb.y), bz = __uint_as_float(mb.z);
     const float grow = 2.0f * sqrtf(bx * bx + by * by + bz * bz);
     need = !(grow < r.x - act - 1.0e-4f * act);
+    if (need && s.bp_bodies[sc].z != 0u) need = !bp_bodies_keep(s, p, sc, r.x, act);
+  }
+  if (s.bp_bodies[sc].z != 0u) {  // deviations restart; a rebuild also restarts the body boxes
+    const uint4 bd = s.bp_bodies[sc];
+    for (uint32_t b = bd.x; b < bd.x + bd.y; ++b) {
+      s.bp_bdev[b] = make_uint4(0, 0, 0, 0);
+      if (need) {
+        s.bp_bbox[2 * size_t(b)] = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0);
+        s.bp_bbox[2 * size_t(b) + 1] = make_uint4(0, 0, 0, 0);
+      }
+    }
   }
   if (need) {
     const float fx = __uint_as_float(mf.x), fy = __uint_as_float(mf.y), fz = __uint_as_float(mf.z);
@@ -3829,9 +3925,22 @@ __global__ void __launch_bounds__(kBlock) k_bp_commit(DeviceState s, PassParams 
   const uint32_t blk = s.bp_list[vertices ? blockIdx.x : s.bp_ns + blockIdx.x];
   const uint32_t i = blk * kSceneBlock + threadIdx.x;
   if (vertices) {
-    if (!bp_building(s, s.vblock_scene[blk])) return;
-    s.bp_x[i] = s.pos[p.src][i];
+    const uint32_t sc = s.vblock_scene[blk];
+    if (!bp_building(s, sc)) return;  // uniform per block
+    const float4 x = s.pos[p.src][i];
+    s.bp_x[i] = x;
     s.bp_vprev[i] = s.bp_vcnt[i];
+    if (s.bp_bodies[sc].z != 0u) {  // body AABBs at the build pose
+      const bool real = i - s.scene_blk0[sc].x * kSceneBlock < s.scene_cnt[sc].x;
+      const uint32_t body = real ? s.vtx_info[i].w : 0u;
+      uint32_t* box = reinterpret_cast<uint32_t*>(s.bp_bbox);
+      wave_keyed_extreme<false>(box, 8u * body + 0u, f2key(x.x), real);
+      wave_keyed_extreme<false>(box, 8u * body + 1u, f2key(x.y), real);
+      wave_keyed_extreme<false>(box, 8u * body + 2u, f2key(x.z), real);
+      wave_keyed_extreme<true>(box, 8u * body + 4u, f2key(x.x), real);
+      wave_keyed_extreme<true>(box, 8u * body + 5u, f2key(x.y), real);
+      wave_keyed_extreme<true>(box, 8u * body + 6u, f2key(x.z), real);
+    }
   } else {
     if (!bp_building(s, s.eblock_scene[blk])) return;
     s.bp_eprev[i] = s.bp_ecnt[i];
