@@ -69,7 +69,8 @@ def main(argv=None):
     ap.add_argument("--iterations", type=int, default=8)
     ap.add_argument("--latency-frames", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-seconds", type=float, default=12.0)
+    ap.add_argument("--contact-frames", type=int, default=10)
     ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--large-cells", default="64,64,128")
     ap.add_argument("--no-graphs", action="store_true")
@@ -214,6 +215,8 @@ def main(argv=None):
     }
     del inputs, members
 
+    if args.contact_frames > 0:
+        result["contact_boxes"] = measure_contact_boxes(args.contact_frames)
     if args.serve_seconds > 0:
         result["serving"] = measure_serving(args.serve_seconds, args.seed)
     if args.large_steps > 0:
@@ -316,17 +319,72 @@ def measure_plain(inputs, settings, iterations, steps):
     return out
 
 
+def measure_contact_boxes(frames):
+    """The large contact workload (twenty 8x8x8-cell boxes dropped into the
+    trough, broad-phase candidates) as one plain scene with graph frames:
+    throughput, frame latency, broad-phase builds and the nearest pair."""
+    import jgs2
+    from jgs2 import scenes
+    from jgs2.batch import JGS2Batch
+    from jgs2.serving import nearest_rank
+    from jgs2.solver import ContactData
+
+    t0 = time.time()
+    x = scenes.contact_boxes()
+    settings = dict(scenes.CONTACT_BOXES_SETTINGS)
+    it = scenes.CONTACT_BOXES_ITERATIONS
+    solver = jgs2.JGS2GpuSolver.create(x, settings)
+    JGS2Batch.of(solver).use_graphs(True)
+    log(f"contact boxes: V={x.vertex_count} T={x.tet_count} capacity {x.contact.candidate_capacity}, "
+        f"ready in {time.time() - t0:.1f}s")
+    solver.step_frames_exact_iterations(1, it, settings)
+    solver.await_idle()
+    b0 = solver.contact_info()["builds"]
+    lat = []
+    for _ in range(frames):
+        a = time.perf_counter()
+        solver.step_frames_exact_iterations(1, it, settings)
+        solver.await_idle()
+        lat.append((time.perf_counter() - a) * 1e3)
+    info = solver.contact_info()
+    packed, vt, _ = solver.read_contact_candidates()
+    p = np.asarray(solver.read_positions(), np.float64).reshape(-1, 4)[:, :3]
+    sys.path.insert(0, str(REPO / "tests"))
+    from ipc_geometry import candidate_distances
+
+    nearest = float(candidate_distances(p, ContactData(packed.reshape(-1), vt, len(packed) - vt)).min())
+    solver.destroy()
+    ms = float(np.mean(lat))
+    out = {
+        "workload": "contact-boxes: 20 boxes of 8x8x8 cells (3,072 tetrahedra each) dropped at seeded offsets and "
+                    "yaws into the pinned V-trough; dhat 1 mm, kappa 1e5, friction 0.3, 12 Jacobi iterations",
+        "vertices": x.vertex_count, "tetrahedra": x.tet_count, "free_vertices": solver.free_vertex_count,
+        "frames": frames, "ms_per_frame": ms, "vertex_updates_per_s": solver.free_vertex_count * it / (ms * 1e-3),
+        "frame_ms": {"p50": nearest_rank(lat, 50), "p99": nearest_rank(lat, 99)},
+        "broad_phase": {"candidates": int(info["candidates"]), "capacity": int(info["capacity"]),
+                        "builds_per_frame": (int(info["builds"]) - b0) / frames, "reach_m": float(info["reach"])},
+        "nearest_pair_m": nearest, "minimum_distance_m": settings["ipc_minimum_distance"],
+        "interpenetration_free": nearest > settings["ipc_minimum_distance"],
+    }
+    log(f"contact boxes: {ms:.1f} ms/frame, {out['vertex_updates_per_s']:.4g} vertex-updates/s, "
+        f"{out['broad_phase']['builds_per_frame']:.1f} builds/frame, nearest pair {nearest:.4g} m")
+    return out
+
+
 def measure_serving(seconds, seed):
-    """Poisson arrivals (2000 requests/s) of contact-trough requests asking
-    for 1 to 120 frames, served for `seconds` by continuous batching under an
-    HBM budget and by one solver per request under the same budget."""
+    """Poisson arrivals (2000 requests/s) of contact-box requests (the large
+    contact workload's generator, seeded placements) asking for 1 to 120
+    frames, served for `seconds` by continuous batching under an HBM budget
+    and by one solver per request under the same budget."""
     from jgs2 import scenes
     from jgs2.serving import BatchServer, PlainServer, Request, capacity_for_budget, poisson_arrivals
 
-    rate, budget, max_scenes = 2000.0, 8 << 30, 512
-    pool = [scenes.trough(box_count=10, seed=s) for s in range(1, 17)]
-    settings = dict(scenes.TROUGH_SETTINGS)
-    it = scenes.TROUGH_ITERATIONS
+    rate, budget, max_scenes = 2000.0, 64 << 30, 512
+    t0 = time.time()
+    pool, _ = scenes.contact_boxes_pool(16)
+    log(f"serving pool: {len(pool)} contact-box scenes built in {time.time() - t0:.1f}s")
+    settings = dict(scenes.CONTACT_BOXES_SETTINGS)
+    it = scenes.CONTACT_BOXES_ITERATIONS
     rng = np.random.default_rng(seed)
     arrivals = poisson_arrivals(rate, seconds, seed)
     frames = rng.integers(1, 121, size=arrivals.size)
@@ -337,8 +395,9 @@ def measure_serving(seconds, seed):
                 for i in range(arrivals.size)]
 
     cap = capacity_for_budget(pool, budget, max_scenes=max_scenes)
-    out = {"workload": f"Poisson {rate:g} requests/s of 10-box contact troughs, 1-120 frames each, "
-                       f"{it} exact Jacobi sweeps per frame, {seconds:g} s",
+    out = {"workload": f"Poisson {rate:g} requests/s of contact-box scenes (20 boxes of 8x8x8 cells in the "
+                       f"trough, 16 seeded placements), 1-120 frames each, {it} exact Jacobi sweeps per frame, "
+                       f"{seconds:g} s",
            "offered_requests": int(arrivals.size), "hbm_budget_bytes": budget, "max_resident_scenes": cap.max_scenes}
     server = BatchServer(cap, settings, it, hbm_budget_bytes=budget, use_graphs=True)
     out["arena_bytes"] = server.arena_bytes

@@ -216,7 +216,7 @@ void free_device(DeviceState& d) {
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
-                  d.cscratch, d.src_c, d.src_w, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
+                  d.cscratch, d.src_c, d.src_w, d.src_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
                   d.body_rng, d.bp_list, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
@@ -760,6 +760,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.cand_meta, Cc, by, dry));
   TRYS(dalloc(&d.cand_keep, Cc, by, dry));
   for (float4** a : {&d.lag_n, &d.lag_w, &d.cscratch, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRYS(dalloc(a, Cc, by, dry));
+  TRYS(dalloc(&d.src_d, Cc, by, dry));
   TRYS(dalloc(&d.cand_ids, 4 * Cc, by, dry));
   TRYS(dalloc(&d.active_ids, 4 * Cc, by, dry));
   TRYS(dalloc(&d.body_corr, Bc, by, dry));
@@ -1077,6 +1078,8 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
     for (float4* a : {d.lag_n, d.lag_w, d.src_w, d.tgt_w}) UP(a, CB, z);
     UP(d.cscratch, CB, scratch);
     UP(d.src_c, CB, none);
+    std::vector<float2> summary(NC, make_float2(-1, 0));
+    UP(d.src_d, CB, summary);
     UP(d.tgt_c, CB, none);
     UP(d.cand_ids, s->cel.at, cand_ids);
     UP(d.active_ids, s->cel.at, cand_ids);
@@ -1493,8 +1496,11 @@ jgs2_status prepare_broad_phase(const jgs2_scene_input& in, const jgs2_settings&
   const double act = st.max_step > 0 ? st.ipc_activation_distance + 2.0 * (st.max_step + rr) : INFINITY;
   const double margin = c.reach_margin > 0 ? c.reach_margin : 0.25 * act;
   const double near = st.ipc_minimum_distance + 8.0 * 0x1p-24 * std::max(1.0, st.ipc_activation_distance);
-  const SurfaceScan scan =
-      scan_surface_pairs(s->surface, V, in.positions, float(act + margin), float(2.0 * near + 1e-6));
+  const float distance_reach = float(2.0 * near + 1e-6);
+  // Count the pairs within the first build's reach only when sizing the
+  // capacity; the feasibility check needs the nearest pairs alone.
+  const SurfaceScan scan = scan_surface_pairs(s->surface, V, in.positions,
+                                              c.candidate_capacity ? distance_reach : float(act + margin), distance_reach);
   if (std::isfinite(scan.min_distance)) {
     const double m = 8.0 * 0x1p-24 * std::max({1.0, st.ipc_activation_distance, std::fabs(scan.min_distance)});
     const double required = st.ipc_minimum_distance + m;

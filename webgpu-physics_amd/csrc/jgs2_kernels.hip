@@ -865,6 +865,15 @@ __device__ __forceinline__ Contact ipc_cached(const DeviceState& s, uint32_t c, 
   const bool valid = nd.w >= 0.0f && finite4(nd) && finite4(w);
   return Contact{valid ? nd.w : 0.0f, valid ? xyz(nd) : mk(1, 0, 0), valid, valid ? w : make_float4(0, 0, 0, 0)};
 }
+// Eight bytes per candidate that decide whether a row needs its full
+// records: the cached source distance (-1 when ipc_cached would report the
+// contact invalid) and the lagged normal force. Rewritten wherever the
+// source cache or the lagged force changes.
+__device__ __forceinline__ void ipc_store_summary(const DeviceState& s, uint32_t c) {
+  const float4 nd = s.src_c[c], w = s.src_w[c];
+  const bool valid = nd.w >= 0.0f && finite4(nd) && finite4(w);
+  s.src_d[c] = make_float2(valid ? nd.w : -1.0f, s.lag_n[c].w);
+}
 __device__ __forceinline__ bool ipc_friction_active(const DeviceState& s, uint32_t c) {
   return s.ipc_mu > 0.0f && s.lag_n[c].w > 0.0f;
 }
@@ -895,6 +904,10 @@ __device__ __forceinline__ uint32_t ipc_energy_owner(const DeviceState& s, uint4
 __device__ void ipc_row_contribution(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
                                      uint32_t c, f3& g, m3& H) {
   {
+    if (!p.colored) {  // the same tests on the eight-byte summary first
+      const float2 q = s.src_d[c];
+      if (q.x < 0.0f || (!ipc_barrier_active(q.x, p.ipc_dhat) && !(s.ipc_mu > 0.0f && q.y > 0.0f))) return;
+    }
     const Contact k = p.colored ? ipc_contact_at(s, c, pos, p.ipc_dhat) : ipc_cached(s, c, false);
     if (!k.valid) return;
     const bool ba = ipc_barrier_active(k.distance, p.ipc_dhat);
@@ -1314,6 +1327,11 @@ __device__ Trial restricted_trial(const DeviceState& s, const PassParams& p, uin
 #pragma unroll 1
     for (uint32_t i = begin; i < end; ++i) {
       const uint32_t c = s.active_ids[i];
+      if (!p.colored) {  // the same tests on the eight-byte summary first
+        const float2 q = s.src_d[c];
+        if (q.x >= 0.0f && q.x - dl > s.ipc_dmin && !(q.x - dl < p.ipc_dhat) && !(s.ipc_mu > 0.0f && q.y > 0.0f))
+          continue;
+      }
       const Contact sc = p.colored ? ipc_contact_at(s, c, pos, p.ipc_dhat) : ipc_cached(s, c, false);
       if (!sc.valid) {
         tr.geometry_valid = false;
@@ -1708,6 +1726,11 @@ __device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p
 #pragma unroll 1
     for (uint32_t k = lane_first<LPV>(sub, prefix); k < n; k += LPV) {
       const uint32_t c = s.active_ids[begin + k];
+      if (!p.colored) {  // the same tests on the eight-byte summary first
+        const float2 q = s.src_d[c];
+        if (q.x >= 0.0f && q.x - dl > s.ipc_dmin && !(q.x - dl < p.ipc_dhat) && !(s.ipc_mu > 0.0f && q.y > 0.0f))
+          continue;
+      }
       const Contact sc = p.colored ? ipc_contact_at(s, c, pos, p.ipc_dhat) : ipc_cached(s, c, false);
       if (!sc.valid) {
         tr.fail = 2u * (prefix + k) + 1u;
@@ -3122,6 +3145,7 @@ __global__ void __launch_bounds__(kBlock) k_lag_contact(DeviceState s, PassParam
   s.lag_n[c] = nf;
   s.lag_w[c] = w;
   s.cscratch[c] = make_float4(1, 0, 0, 0);
+  ipc_store_summary(s, c);
 }
 // buildActiveContactRows (S:2747-2785): keep candidates the bounded step can
 // bring into the barrier, with friction, or touched by unbounded pin motion.
@@ -3136,8 +3160,8 @@ __global__ void __launch_bounds__(kBlock) k_candidate_keep(DeviceState s, PassPa
   const bool bounded = p.max_step > 0.0f;
   const float reach = p.ipc_dhat + 2.0f * (fmaxf(p.max_step, 0.0f) + ipc_rounding_reach(p, s.scene_scale[sc]));
   const float bound = fmaxf(p.max_step, 0.0f);
-  const Contact k = ipc_cached(s, c, false);
-  bool include = !k.valid || !bounded || k.distance <= reach || ipc_friction_active(s, c);
+  const float2 q = s.src_d[c];  // the cached source contact's validity and distance, and the lagged force
+  bool include = q.x < 0.0f || !bounded || q.x <= reach || (s.ipc_mu > 0.0f && q.y > 0.0f);
   if (!include) {  // ipcCandidatePinnedMotionExceedsBound (S:484-507)
     const float4* src = s.pos[p.src];
     const uint4 ix = s.cand[c];
@@ -3282,6 +3306,7 @@ __global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassP
   if (s.control[4 * size_t(sc) + 1].x == 1.0f && finite_s(alpha) && alpha > 0.0f) {
     s.src_c[c] = s.tgt_c[c];
     s.src_w[c] = s.tgt_w[c];
+    ipc_store_summary(s, c);
   }
 }
 
@@ -3867,6 +3892,7 @@ __global__ void __launch_bounds__(kBlock) k_bp_rows(DeviceState s, PassParams p,
     s.lag_w[c] = make_float4(0, 0, 0, 0);
     ipc_store_cached(s, c, false, ipc_contact_at(s, c, s.bp_cpose, ipc_broad_threshold(p, s.scene_scale[sc])));
   }
+  ipc_store_summary(s, c);
 }
 __global__ void __launch_bounds__(kBlock) k_bp_row_offsets(DeviceState s) {
   const uint32_t vb = s.bp_list[blockIdx.x];

@@ -349,6 +349,12 @@ SurfaceScan scan_surface_pairs(const Surface& s, uint32_t V, const float* positi
     const double d = vt ? vertex_triangle_distance(p[0], p[1], p[2], p[3]) : edge_edge_distance(p[0], p[1], p[2], p[3]);
     out.min_distance = std::min(out.min_distance, d);
   };
+  // Primitives with a non-finite point are tested against everything.
+  std::vector<uint32_t> wild_tris, wild_edges;
+  for (uint32_t t = 0; t < tris.size(); ++t)
+    if (tris[t].wild) wild_tris.push_back(t);
+  for (uint32_t e = 0; e < edges.size(); ++e)
+    if (edges[e].wild) wild_edges.push_back(e);
   // VT: each point against the triangles listed in its cell.
   if (!everything) grid.insert(tris, grow);
   for (uint32_t v = 0; v < V; ++v) {
@@ -364,8 +370,7 @@ SurfaceScan scan_surface_pairs(const Surface& s, uint32_t V, const float* positi
     int64_t c0[3], c1[3];
     grid.cells_of(pts[v], 0.0, c0, c1);
     grid.visit(grid.index(c0[0], c0[1], c0[2]), test);
-    for (uint32_t t = 0; t < s.tris.size(); ++t)
-      if (tris[t].wild) test(t);
+    for (uint32_t t : wild_tris) test(t);
   }
   // EE: each edge against the later edges listed in the cells it overlaps.
   if (!everything) grid.insert(edges, grow);
@@ -390,8 +395,7 @@ SurfaceScan scan_surface_pairs(const Surface& s, uint32_t V, const float* positi
     for (int64_t z = c0[2]; z <= c1[2]; ++z)
       for (int64_t y = c0[1]; y <= c1[1]; ++y)
         for (int64_t x = c0[0]; x <= c1[0]; ++x) grid.visit(grid.index(x, y, z), test);
-    for (uint32_t j = i + 1; j < s.edges.size(); ++j)
-      if (edges[j].wild) test(j);
+    for (uint32_t j : wild_edges) test(j);
   }
   return out;
 }

@@ -191,9 +191,25 @@ CONTACT_BOXES_ITERATIONS = 12
 
 def contact_boxes(seed=CONTACT_BOXES["seed"], box_count=CONTACT_BOXES["box_count"],
                   box_cells=CONTACT_BOXES["box_cells"], candidate_capacity=0) -> GpuInput:
-    """The large contact workload (see CONTACT_BOXES), with broad-phase contact."""
+    """The large contact workload (see CONTACT_BOXES), with broad-phase contact.
+    Raises JGS2RangeError for a seed whose placement leaves two boxes closer
+    than the minimum distance (rotated boxes in neighboring slots can)."""
     return trough(box_count=box_count, box_cells=box_cells, seed=seed, broad_phase=True,
                   settings=CONTACT_BOXES_SETTINGS, candidate_capacity=candidate_capacity)
+
+
+def contact_boxes_pool(count, first_seed=1, **kw):
+    """`count` feasible contact-box scenes from consecutive seeds, skipping
+    seeds whose initial pose is infeasible. Returns (scenes, seeds)."""
+    out, seeds, seed = [], [], first_seed
+    while len(out) < count:
+        try:
+            out.append(contact_boxes(seed=seed, **kw))
+            seeds.append(seed)
+        except N.JGS2RangeError:
+            pass
+        seed += 1
+    return out, seeds
 
 
 def cloth_drape(cells=4, origin=(-0.8, 1.15, -0.8), size=(1.6, 1.6), fold_column=3, fold_lift=0.079,

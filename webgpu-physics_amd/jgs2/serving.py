@@ -85,10 +85,16 @@ def capacity_for_budget(template: Sequence[GpuInput], budget_bytes: int, max_sce
                              max_surface_triangles=int(mean["surface_triangles"] * n),
                              max_surface_edges=int(mean["surface_edges"] * n))
 
+    def fits(n: int) -> bool:
+        try:
+            return cap(n).arena_bytes() <= budget_bytes
+        except N.JGS2RangeError:  # beyond the arena's 32-bit indexing
+            return False
+
     lo, hi = 0, max_scenes
     while lo < hi:
         mid = (lo + hi + 1) // 2
-        if cap(mid).arena_bytes() <= budget_bytes:
+        if fits(mid):
             lo = mid
         else:
             hi = mid - 1
@@ -282,7 +288,8 @@ class PlainServer(_Server):
         need = BatchCapacity(max_vertices=f["vertices"], max_tets=f["tets"], max_scenes=1,
                              cubature_k=request.scene.cubature_k, max_triangles=f["triangles"],
                              max_hinges=f["hinges"], max_candidates=f["candidates"],
-                             max_bodies=f["bodies"]).arena_bytes()
+                             max_bodies=f["bodies"], max_surface_triangles=f["surface_triangles"],
+                             max_surface_edges=f["surface_edges"]).arena_bytes()
         if len(self.active) >= self.max_scenes or self.used + need > self.budget:
             return None
         solver = JGS2GpuSolver.create(request.scene, self.settings)
