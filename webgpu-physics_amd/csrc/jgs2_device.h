@@ -130,6 +130,7 @@ struct DeviceState {
   float4 *lag_n = nullptr, *lag_w = nullptr, *cscratch = nullptr;
   float4 *src_c = nullptr, *src_w = nullptr, *tgt_c = nullptr, *tgt_w = nullptr;
   float2* src_d = nullptr;  // per candidate: cached source distance (-1 invalid), lagged force
+  float* tgt_d = nullptr;   // per candidate: cached target distance (-1 invalid)
   float4* contact_ctl = nullptr;
   uint2* cand_row = nullptr;
   uint32_t *cand_ids = nullptr, *active_count = nullptr, *active_ids = nullptr, *cand_meta = nullptr;

@@ -216,7 +216,7 @@ void free_device(DeviceState& d) {
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
-                  d.cscratch, d.src_c, d.src_w, d.src_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
+                  d.cscratch, d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
                   d.body_rng, d.bp_list, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
@@ -761,6 +761,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.cand_keep, Cc, by, dry));
   for (float4** a : {&d.lag_n, &d.lag_w, &d.cscratch, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRYS(dalloc(a, Cc, by, dry));
   TRYS(dalloc(&d.src_d, Cc, by, dry));
+  TRYS(dalloc(&d.tgt_d, Cc, by, dry));
   TRYS(dalloc(&d.cand_ids, 4 * Cc, by, dry));
   TRYS(dalloc(&d.active_ids, 4 * Cc, by, dry));
   TRYS(dalloc(&d.body_corr, Bc, by, dry));

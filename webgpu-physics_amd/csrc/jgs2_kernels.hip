@@ -2308,10 +2308,12 @@ __global__ void __launch_bounds__(kBlock) k_ipc_energy_lanes(DeviceState s, Pass
       float e = 0.0f;
       if (i < last) {
         const uint32_t c = s.active_ids[i];
-        const Contact k = ipc_cached(s, c, use_target);
-        if (!k.valid) {
+        const float2 q = s.src_d[c];  // summaries decide first: validity, barrier, lagged force
+        const float dq = use_target ? s.tgt_d[c] : q.x;
+        if (dq < 0.0f) {
           bad = true;
-        } else if (ipc_barrier_active(k.distance, p.ipc_dhat) || ipc_friction_active(s, c)) {
+        } else if (ipc_barrier_active(dq, p.ipc_dhat) || (s.ipc_mu > 0.0f && q.y > 0.0f)) {
+          const Contact k = ipc_cached(s, c, use_target);
           const uint4 ix = s.cand[c];
           if (ipc_energy_owner(s, ix) == v) {
             e = ipc_energy_from_contact(s, p, c, ix, k, pos, kNoVertex, mk(0, 0, 0));
@@ -3239,6 +3241,10 @@ __global__ void __launch_bounds__(kBlock) k_validate_contact_step(DeviceState s,
   if (sc == kNoScene || c >= s.scene_end[sc].w) return;
   const Contact k = ipc_contact_at(s, c, s.pos[p.tgt], ipc_broad_threshold(p, s.scene_scale[sc]));
   ipc_store_cached(s, c, true, k);
+  {  // the target cache's distance as ipc_cached reads it (-1 when invalid)
+    const float4 nd = s.tgt_c[c], w = s.tgt_w[c];
+    s.tgt_d[c] = (nd.w >= 0.0f && finite4(nd) && finite4(w)) ? nd.w : -1.0f;
+  }
   const bool valid = k.valid && finite_s(k.distance) && k.distance > s.ipc_dmin;
   const float d = valid ? k.distance : 0.0f;
   s.cscratch[c] = make_float4(1.0f, d, d, valid ? 1.0f : 0.0f);
