@@ -99,7 +99,7 @@ typedef struct {
   const uint32_t* surface_triangles;  /* 3 per triangle (canonicalized like canonicalTriangles, :500-527) */
   const uint32_t* surface_edges;      /* 2 per edge; triangle edges are added (canonicalEdges, :546-569) */
   uint64_t candidate_capacity;        /* resident candidates at most; 0 = twice the count at the initial pose + 1024 */
-  double reach_margin;                /* reach past the activation reach in m; 0 = a quarter of it */
+  double reach_margin;                /* reach past the activation reach in m; 0 = half of it */
 } jgs2_contact_input;
 
 /*

@@ -112,7 +112,7 @@ def test_non_finite_points_are_always_candidates():
 @pytest.mark.parametrize("boxes,cells", [(10, 1), (20, 2)])
 def test_library_capacity_estimate_counts_the_oracle_list(boxes, cells):
     # The library sizes a broad-phase scene at twice the number of pairs
-    # within its first build's reach (activation reach plus a quarter) plus
+    # within its first build's reach (activation reach plus half) plus
     # 1024; its host count must be the oracle's count at that reach.
     st = scenes.TROUGH_SETTINGS
     x = scenes.trough(box_count=boxes, box_cells=cells, seed=1, broad_phase=True)
@@ -122,5 +122,5 @@ def test_library_capacity_estimate_counts_the_oracle_list(boxes, cells):
     rr = max(scale, st["ipc_activation_distance"], 1e-12) * 2.0**-24 * 8.0
     act = st["ipc_activation_distance"] + 2.0 * (st["max_step"] + rr)
     tri, edg = _surface(x)
-    packed, _ = O.contact_candidates_within(x.vertex_count, x.positions, tri, edg, float(np.float32(1.25 * act)))
+    packed, _ = O.contact_candidates_within(x.vertex_count, x.positions, tri, edg, float(np.float32(1.5 * act)))
     assert x.contact.candidate_capacity == 2 * len(packed) + 1024

@@ -1495,7 +1495,7 @@ jgs2_status prepare_broad_phase(const jgs2_scene_input& in, const jgs2_settings&
   if (!superset_nonempty(s->surface, V)) return JGS2_OK;  // no pair can ever touch: no contact
   const double rr = std::max({double(s->scene_scale), st.ipc_activation_distance, 1e-12}) * 0x1p-24 * 8.0;
   const double act = st.max_step > 0 ? st.ipc_activation_distance + 2.0 * (st.max_step + rr) : INFINITY;
-  const double margin = c.reach_margin > 0 ? c.reach_margin : 0.25 * act;
+  const double margin = c.reach_margin > 0 ? c.reach_margin : 0.5 * act;
   const double near = st.ipc_minimum_distance + 8.0 * 0x1p-24 * std::max(1.0, st.ipc_activation_distance);
   const float distance_reach = float(2.0 * near + 1e-6);
   // Count the pairs within the first build's reach only when sizing the

@@ -190,12 +190,12 @@ CONTACT_BOXES_ITERATIONS = 12
 
 
 def contact_boxes(seed=CONTACT_BOXES["seed"], box_count=CONTACT_BOXES["box_count"],
-                  box_cells=CONTACT_BOXES["box_cells"], candidate_capacity=0) -> GpuInput:
+                  box_cells=CONTACT_BOXES["box_cells"], candidate_capacity=0, reach_margin=0.0) -> GpuInput:
     """The large contact workload (see CONTACT_BOXES), with broad-phase contact.
     Raises JGS2RangeError for a seed whose placement leaves two boxes closer
     than the minimum distance (rotated boxes in neighboring slots can)."""
     return trough(box_count=box_count, box_cells=box_cells, seed=seed, broad_phase=True,
-                  settings=CONTACT_BOXES_SETTINGS, candidate_capacity=candidate_capacity)
+                  settings=CONTACT_BOXES_SETTINGS, candidate_capacity=candidate_capacity, reach_margin=reach_margin)
 
 
 def contact_boxes_pool(count, first_seed=1, **kw):

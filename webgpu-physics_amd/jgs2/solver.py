@@ -150,7 +150,7 @@ class BroadPhaseContact:
     src/simulation/cpu/ipc-contact.ts:600-665). Stepping gives the
     superset's results bit for bit; Jacobi schedule only.
     `candidate_capacity` 0 lets the library choose (twice the count at the
-    initial pose plus 1024); `reach_margin` 0 is a quarter of the
+    initial pose plus 1024); `reach_margin` 0 is half the
     activation reach."""
 
     surface_triangles: np.ndarray
