@@ -77,6 +77,8 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a cold 1 GiB device buffer (the counter calibration kernel) and exit")
+    ap.add_argument("--profile-contact", type=int, default=0, metavar="FRAMES",
+                    help="profiler driver: only the contact-box workload, FRAMES frames after 3 warm ones")
     ap.add_argument("--profile-large", default=None, metavar="LANES",
                     help="only step the layered box two frames per listed lane count (e.g. 1,4,8,16) for a profiler, "
                          "and exit")
@@ -87,6 +89,8 @@ def main(argv=None):
     args.warmup = max(1, args.warmup)
     if args.copy_calibration:
         return copy_calibration()
+    if args.profile_contact:
+        return profile_contact(args.profile_contact)
     if args.profile_large:
         return profile_large([int(x) for x in args.profile_large.split(",")], args.large_cells, args.iterations)
     if args.gpus != 1:
@@ -245,6 +249,31 @@ def copy_calibration():
     c.copy_(b)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 2}), flush=True)
+    return 0
+
+
+def profile_contact(frames):
+    """The contact-box workload alone, graph frames (profiler driver)."""
+    import jgs2
+    from jgs2 import scenes
+    from jgs2.batch import JGS2Batch
+
+    x = scenes.contact_boxes()
+    settings = dict(scenes.CONTACT_BOXES_SETTINGS)
+    it = scenes.CONTACT_BOXES_ITERATIONS
+    solver = jgs2.JGS2GpuSolver.create(x, settings)
+    JGS2Batch.of(solver).use_graphs(True)
+    solver.step_frames_exact_iterations(3, it, settings)
+    solver.await_idle()
+    b0 = solver.contact_info()["builds"]
+    t0 = time.perf_counter()
+    solver.step_frames_exact_iterations(frames, it, settings)
+    solver.await_idle()
+    dt = time.perf_counter() - t0
+    info = solver.contact_info()
+    log(f"contact boxes: {frames} frames, {dt / frames * 1e3:.2f} ms/frame, "
+        f"{(info['builds'] - b0) / frames:.2f} builds/frame, list {info['candidates']}")
+    solver.destroy()
     return 0
 
 
