@@ -253,16 +253,14 @@ def copy_calibration():
 
 
 def profile_contact(frames):
-    """The contact-box workload alone, graph frames (profiler driver)."""
+    """The contact-box workload alone, eager launches (profiler driver)."""
     import jgs2
     from jgs2 import scenes
-    from jgs2.batch import JGS2Batch
 
     x = scenes.contact_boxes()
     settings = dict(scenes.CONTACT_BOXES_SETTINGS)
     it = scenes.CONTACT_BOXES_ITERATIONS
     solver = jgs2.JGS2GpuSolver.create(x, settings)
-    JGS2Batch.of(solver).use_graphs(True)
     solver.step_frames_exact_iterations(3, it, settings)
     solver.await_idle()
     b0 = solver.contact_info()["builds"]
