@@ -89,6 +89,13 @@ def build() -> Path:
     return LIBRARY_PATH
 
 
+# Mesh arguments of the precompute entry points (oracle/jgs2_oracle_precompute.cpp):
+# V, positions (f64, 3 per vertex), T, tetrahedra, fixed flags, material ids,
+# material count, materials (density, young, poisson), models (1 = stable
+# Neo-Hookean, 0 = co-rotated linear), timestep.
+_MESH = [C.c_uint32, f64p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, f64p, u32p, C.c_double]
+
+
 def _load() -> C.CDLL:
     if not LIBRARY_PATH.exists():
         build()
@@ -122,6 +129,14 @@ def _load() -> C.CDLL:
         "jgs2o_read_contact": (None, [C.c_void_p, f64p]),
         "jgs2o_contact_candidates_within": (C.c_int64, [C.c_uint32, f32p, u32p, C.c_uint32, u32p, C.c_uint32,
                                                         C.c_float, u32p, C.c_uint64, u32p]),
+        "jgs2o_precompute_error": (C.c_char_p, []),
+        "jgs2o_cubature_corpus": (C.c_int, _MESH + [C.c_uint32, f64p, f64p, f64p, u32p]),
+        "jgs2o_low_frequency_modes": (C.c_int, _MESH + [C.c_uint32, C.c_uint32, f64p, f64p]),
+        "jgs2o_exact_basis": (C.c_int, _MESH + [C.c_uint32, f64p]),
+        "jgs2o_precompute_cubature": (C.c_int, _MESH + [C.c_uint32, u32p, f32p, f32p, f64p]),
+        "jgs2o_select_columns": (C.c_int, [C.c_uint32, C.c_uint32, f64p, f64p, C.c_uint32, C.c_double, C.c_uint32,
+                                           u32p, f64p, f64p]),
+        "jgs2o_fit_columns": (C.c_int, [C.c_uint32, C.c_uint32, f64p, f64p, f64p, f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -472,3 +487,96 @@ def contact_candidates_within(vertex_count: int, positions, triangles, edges, re
     out = np.zeros((max(n, 1), 4), np.uint32)
     lib.jgs2o_contact_candidates_within(*args, out.ctypes.data_as(u32p), n, C.byref(vt))
     return out[:n], vt.value
+
+
+class PrecomputeError(RuntimeError):
+    pass
+
+
+def _mesh(positions, tetrahedra, fixed, material_ids, materials, timestep):
+    x = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    t = np.ascontiguousarray(tetrahedra, np.uint32).reshape(-1)
+    f = np.ascontiguousarray(fixed, np.uint32).reshape(-1)
+    m = np.ascontiguousarray(material_ids, np.uint32).reshape(-1)
+    mats = np.ascontiguousarray([[q[0], q[1], q[2]] for q in materials], np.float64).reshape(-1)
+    models = np.ascontiguousarray([1 if (len(q) > 3 and q[3]) else 0 for q in materials], np.uint32)
+    keep = (x, t, f, m, mats, models)
+    args = [x.size // 3, x.ctypes.data_as(f64p), t.size // 4, t.ctypes.data_as(u32p), f.ctypes.data_as(u32p),
+            m.ctypes.data_as(u32p), len(materials), mats.ctypes.data_as(f64p), models.ctypes.data_as(u32p),
+            float(timestep)]
+    return args, keep
+
+
+def _check(rc):
+    if rc < 0:
+        raise PrecomputeError(lib.jgs2o_precompute_error().decode())
+    return rc
+
+
+def cubature_corpus(positions, tetrahedra, fixed, material_ids, materials, timestep):
+    """Training and validation poses of the nonlinear cubature corpus:
+    (training, validation), each a list of (positions, predicted, min J)."""
+    args, keep = _mesh(positions, tetrahedra, fixed, material_ids, materials, timestep)
+    V = args[0]
+    cap = 64
+    pos = np.zeros((cap, V * 3)); pred = np.zeros((cap, V * 3)); mj = np.zeros(cap)
+    nv = C.c_uint32()
+    nt = _check(lib.jgs2o_cubature_corpus(*args, cap, pos.ctypes.data_as(f64p), pred.ctypes.data_as(f64p),
+                                          mj.ctypes.data_as(f64p), C.byref(nv)))
+    poses = [(pos[i], pred[i], float(mj[i])) for i in range(nt + nv.value)]
+    return poses[:nt], poses[nt:]
+
+
+def low_frequency_modes(positions, tetrahedra, fixed, material_ids, materials, timestep, count, iterations):
+    """(modes [count, 3V], dense rest Hessian over free coordinates)."""
+    args, keep = _mesh(positions, tetrahedra, fixed, material_ids, materials, timestep)
+    V = args[0]
+    n = 3 * int((np.asarray(fixed).reshape(-1) == 0).sum())
+    out = np.zeros((count, 3 * V)); hess = np.zeros((n, n))
+    k = _check(lib.jgs2o_low_frequency_modes(*args, count, iterations, out.ctypes.data_as(f64p), hess.ctypes.data_as(f64p)))
+    return out[:k], hess
+
+
+def exact_basis(positions, tetrahedra, fixed, material_ids, materials, timestep, vertex):
+    """Exact rest basis of one free vertex: [V, 3, 3] (row = coordinate)."""
+    args, keep = _mesh(positions, tetrahedra, fixed, material_ids, materials, timestep)
+    out = np.zeros(args[0] * 9)
+    _check(lib.jgs2o_exact_basis(*args, int(vertex), out.ctypes.data_as(f64p)))
+    return out.reshape(-1, 3, 3)
+
+
+def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, timestep, samples):
+    """Packed cubature records (tetrahedra [V, K], f32 weights [V, K], f32
+    basis [V, K, 36]) and per-vertex statistics [V, 8]: packed residual, f64
+    residual, valid and trivial training poses, nonzero candidates, column
+    rank, packed nonzero candidates, packed column rank (NaN when fixed)."""
+    args, keep = _mesh(positions, tetrahedra, fixed, material_ids, materials, timestep)
+    V, K = args[0], int(samples)
+    tets = np.zeros((V, K), np.uint32); w = np.zeros((V, K), np.float32)
+    basis = np.zeros((V, K, 36), np.float32); stats = np.zeros((V, 8))
+    _check(lib.jgs2o_precompute_cubature(*args, K, tets.ctypes.data_as(u32p), w.ctypes.data_as(f32p),
+                                         basis.ctypes.data_as(f32p), stats.ctypes.data_as(f64p)))
+    return tets, w, basis, stats
+
+
+def select_columns(columns, target, maximum_columns, tolerance=0.01, refinement_passes=4):
+    """Greedy nonnegative column selection: (indices, weights, target norm,
+    residual norm, normalized residual)."""
+    t = np.ascontiguousarray(target, np.float64).reshape(-1)
+    cols = np.ascontiguousarray(np.asarray(columns, np.float64).reshape(-1, t.size))
+    sel = np.zeros(max(1, len(cols)), np.uint32); w = np.zeros(max(1, len(cols))); norms = np.zeros(3)
+    k = _check(lib.jgs2o_select_columns(len(cols), t.size, cols.ctypes.data_as(f64p), t.ctypes.data_as(f64p),
+                                        int(maximum_columns), float(tolerance), int(refinement_passes),
+                                        sel.ctypes.data_as(u32p), w.ctypes.data_as(f64p), norms.ctypes.data_as(f64p)))
+    return [int(i) for i in sel[:k]], w[:k].copy(), norms[0], norms[1], norms[2]
+
+
+def fit_columns(columns, target):
+    """Exact nonnegative fit of the given columns: (weights, residual norm,
+    normalized residual)."""
+    t = np.ascontiguousarray(target, np.float64).reshape(-1)
+    cols = np.ascontiguousarray(np.asarray(columns, np.float64).reshape(-1, t.size))
+    w = np.zeros(max(1, len(cols))); norms = np.zeros(2)
+    _check(lib.jgs2o_fit_columns(len(cols), t.size, cols.ctypes.data_as(f64p), t.ctypes.data_as(f64p),
+                                 w.ctypes.data_as(f64p), norms.ctypes.data_as(f64p)))
+    return w[:len(cols)].copy(), norms[0], norms[1]
