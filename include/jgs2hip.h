@@ -410,6 +410,67 @@ jgs2_status jgs2_batch_profile_end(jgs2_batch* batch, double* local_solve_ms_per
  * batch), for the launch, graph and autotune controls above. */
 jgs2_status jgs2_solver_batch(jgs2_solver* solver, jgs2_batch** out);
 
+/* ---------------------------------------------------------------------
+ * Cubature precompute: buildPrecomputedScene (src/simulation/cpu/
+ * precompute.ts:218-380) with its exact vertex basis (:135-200), nonlinear
+ * pose corpus (nonlinear-cubature.ts:462-620) and greedy NNLS training
+ * (nonlinear-cubature-training.ts:731-843; co-rotated linear scenes:
+ * cubature.ts:714-826). Produces the cubature_* arrays of jgs2_scene_input
+ * in the reference's packing (src/scenes/index.ts:1054-1068). Host only.
+ * ------------------------------------------------------------------- */
+typedef struct jgs2_precompute_material {
+  double density;
+  double young_modulus;
+  double poisson_ratio;
+  uint32_t model;                  /* JGS2_MATERIAL_* */
+  uint32_t reserved;
+} jgs2_precompute_material;
+
+typedef struct jgs2_precompute_input {
+  uint32_t vertex_count;
+  uint32_t tetrahedron_count;
+  uint32_t material_count;
+  uint32_t cubature_samples;       /* K records per vertex (settings.cubatureSamples), 1..12 */
+  const double* positions;         /* 3 per vertex, rest pose (SceneDefinition.mesh.positions) */
+  const uint32_t* tetrahedra;      /* 4 per tetrahedron */
+  const uint8_t* fixed;            /* 1 per vertex, nonzero = pinned */
+  const uint32_t* material_ids;    /* 1 per tetrahedron */
+  const jgs2_precompute_material* materials;
+  double timestep;
+} jgs2_precompute_input;
+
+typedef struct jgs2_precompute_options {
+  /* 0: each vertex's basis over the whole mesh (the reference's definition;
+   * one dense factorization, for meshes up to a few thousand vertices).
+   * k > 0: over the free vertices within k edges of the source, zero outside
+   * (a local solve per vertex; the cost no longer grows with the mesh). */
+  uint32_t basis_rings;
+  uint32_t threads;                /* 0: every hardware thread */
+  uint32_t mode_solver;            /* 0: automatic, 1: dense Cholesky, 2: preconditioned CG */
+  uint32_t reserved;
+  double cg_tolerance;             /* relative residual of the CG mode solves; 0: 1e-12 */
+  /* Largest packed training residual accepted per vertex; 0: the
+   * reference's 0.01 (NONLINEAR_CUBATURE_TRAINING_RESIDUAL_TOLERANCE).
+   * Meshes beyond a few cells need more than K = 6 samples to meet 0.01. */
+  double residual_bound;
+} jgs2_precompute_options;
+
+typedef struct jgs2_precompute_output {
+  uint32_t* cubature_tet_ids;      /* V*K; 0xffffffff marks unused records */
+  float* cubature_weights;         /* V*K */
+  float* cubature_basis;           /* V*K*36: four row-major 3x3 rest-basis blocks per record */
+  double* vertex_residuals;        /* optional, V: packed training residual (NaN when pinned) */
+  uint32_t* vertex_ranks;          /* optional, 3 per vertex: nonzero candidates, column rank, packed rank */
+} jgs2_precompute_output;
+
+/* Errors: JGS2_ERROR_TYPE for NULL arrays; JGS2_ERROR_RANGE for the
+ * reference's input RangeErrors (bad materials, degenerate tetrahedra, mixed
+ * stable Neo-Hookean / co-rotated scenes, no free vertex); JGS2_ERROR_INFEASIBLE
+ * when a vertex's packed training residual exceeds the bound (the
+ * reference's Error at precompute.ts:313-323). */
+jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* input, const jgs2_precompute_options* options,
+                                     const jgs2_precompute_output* output);
+
 #ifdef __cplusplus
 }
 #endif

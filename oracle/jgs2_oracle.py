@@ -133,7 +133,7 @@ def _load() -> C.CDLL:
         "jgs2o_cubature_corpus": (C.c_int, _MESH + [C.c_uint32, f64p, f64p, f64p, u32p]),
         "jgs2o_low_frequency_modes": (C.c_int, _MESH + [C.c_uint32, C.c_uint32, f64p, f64p]),
         "jgs2o_exact_basis": (C.c_int, _MESH + [C.c_uint32, f64p]),
-        "jgs2o_precompute_cubature": (C.c_int, _MESH + [C.c_uint32, u32p, f32p, f32p, f64p]),
+        "jgs2o_precompute_cubature": (C.c_int, _MESH + [C.c_uint32, C.c_double, u32p, f32p, f32p, f64p]),
         "jgs2o_select_columns": (C.c_int, [C.c_uint32, C.c_uint32, f64p, f64p, C.c_uint32, C.c_double, C.c_uint32,
                                            u32p, f64p, f64p]),
         "jgs2o_fit_columns": (C.c_int, [C.c_uint32, C.c_uint32, f64p, f64p, f64p, f64p]),
@@ -545,7 +545,7 @@ def exact_basis(positions, tetrahedra, fixed, material_ids, materials, timestep,
     return out.reshape(-1, 3, 3)
 
 
-def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, timestep, samples):
+def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, timestep, samples, bound=0.01):
     """Packed cubature records (tetrahedra [V, K], f32 weights [V, K], f32
     basis [V, K, 36]) and per-vertex statistics [V, 8]: packed residual, f64
     residual, valid and trivial training poses, nonzero candidates, column
@@ -554,7 +554,7 @@ def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, t
     V, K = args[0], int(samples)
     tets = np.zeros((V, K), np.uint32); w = np.zeros((V, K), np.float32)
     basis = np.zeros((V, K, 36), np.float32); stats = np.zeros((V, 8))
-    _check(lib.jgs2o_precompute_cubature(*args, K, tets.ctypes.data_as(u32p), w.ctypes.data_as(f32p),
+    _check(lib.jgs2o_precompute_cubature(*args, K, float(bound), tets.ctypes.data_as(u32p), w.ctypes.data_as(f32p),
                                          basis.ctypes.data_as(f32p), stats.ctypes.data_as(f64p)))
     return tets, w, basis, stats
 

@@ -1384,15 +1384,17 @@ int jgs2o_fit_columns(uint32_t count, uint32_t rows, const double* columns, cons
   return 0;
 }
 
-// Packed cubature records of every free vertex, K per vertex: tetrahedron
+// Packed cubature records of every free vertex, K per vertex (the
+// reference requires packed residuals <= 0.01; `bound` sets that limit):
+// tetrahedron
 // ids (0xffffffff where unused), f32 weights, 36 f32 basis values per record,
 // and 8 statistics per vertex (packed residual, f64 residual, valid and
 // trivial training poses, nonzero candidates, column rank, packed nonzero
 // candidates, packed column rank; NaN for fixed vertices).
 int jgs2o_precompute_cubature(uint32_t V, const double* x, uint32_t T, const uint32_t* tets, const uint32_t* fixed,
                               const uint32_t* mat, uint32_t M, const double* materials, const uint32_t* models,
-                              double h, uint32_t K, uint32_t* out_tets, float* out_weights, float* out_basis,
-                              double* out_stats) {
+                              double h, uint32_t K, double bound, uint32_t* out_tets, float* out_weights,
+                              float* out_basis, double* out_stats) {
   try {
     const Scene s = make_scene(V, x, T, tets, fixed, mat, M, materials, models, h);
     std::fill(out_tets, out_tets + size_t(V) * K, 0xffffffffu);
@@ -1417,9 +1419,9 @@ int jgs2o_precompute_cubature(uint32_t V, const double* x, uint32_t T, const uin
         for (double& q : tr.stats) q = 0.0;
       } else if (s.snh) {
         tr = train_snh(s, training, frames, v, K);
-        if (tr.stats[0] > 0.01 + 1e-12)
+        if (tr.stats[0] > bound + 1e-12)
           throw Failure("Stable Neo-Hookean Cubature source vertex " + std::to_string(v) +
-                        " has packed training residual above 0.01.");
+                        " has packed training residual above the bound.");
       } else {
         tr = train_linear(s, linear, v, K);
       }

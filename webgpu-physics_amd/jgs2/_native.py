@@ -227,6 +227,52 @@ class BatchInfo(C.Structure):
     ]
 
 
+class PrecomputeMaterial(C.Structure):
+    _fields_ = [
+        ("density", C.c_double),
+        ("young_modulus", C.c_double),
+        ("poisson_ratio", C.c_double),
+        ("model", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class PrecomputeInput(C.Structure):
+    _fields_ = [
+        ("vertex_count", C.c_uint32),
+        ("tetrahedron_count", C.c_uint32),
+        ("material_count", C.c_uint32),
+        ("cubature_samples", C.c_uint32),
+        ("positions", C.POINTER(C.c_double)),
+        ("tetrahedra", u32p),
+        ("fixed", C.POINTER(C.c_uint8)),
+        ("material_ids", u32p),
+        ("materials", C.POINTER(PrecomputeMaterial)),
+        ("timestep", C.c_double),
+    ]
+
+
+class PrecomputeOptions(C.Structure):
+    _fields_ = [
+        ("basis_rings", C.c_uint32),
+        ("threads", C.c_uint32),
+        ("mode_solver", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("cg_tolerance", C.c_double),
+        ("residual_bound", C.c_double),
+    ]
+
+
+class PrecomputeOutput(C.Structure):
+    _fields_ = [
+        ("cubature_tet_ids", u32p),
+        ("cubature_weights", f32p),
+        ("cubature_basis", f32p),
+        ("vertex_residuals", C.POINTER(C.c_double)),
+        ("vertex_ranks", u32p),
+    ]
+
+
 SIGNATURES = {
     "jgs2_default_settings": (None, [C.POINTER(Settings)]),
     "jgs2_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
@@ -299,6 +345,8 @@ SIGNATURES = {
     "jgs2_build_adjacency": (C.c_int, [C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p]),
     "jgs2_greedy_vertex_coloring": (C.c_int, [C.c_uint32, u32p, C.c_uint32, u32p, u32p, u32p]),
     "jgs2_build_element_incidence": (C.c_int, [C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p, u32p]),
+    "jgs2_precompute_cubature": (C.c_int, [C.POINTER(PrecomputeInput), C.POINTER(PrecomputeOptions),
+                                           C.POINTER(PrecomputeOutput)]),
 }
 
 

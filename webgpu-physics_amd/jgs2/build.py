@@ -42,6 +42,7 @@ SOURCES = [
     ("jgs2_host.cpp", False),
     ("jgs2_scene.cpp", False),
     ("jgs2_scene_contact.cpp", False),
+    ("jgs2_precompute.cpp", False),
 ]
 
 
@@ -62,12 +63,12 @@ def build(verbose: bool = True) -> Path:
             cmd = [HIPCC, *DEVICE_FLAGS, *COMMON_FLAGS, "-c", "-x", "hip", src, "-o", obj]
         else:
             # Host translation units: plain C++ against the HIP runtime API.
-            cmd = [CXX, *COMMON_FLAGS, "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-c", src, "-o", obj]
+            cmd = [CXX, *COMMON_FLAGS, "-pthread", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-c", src, "-o", obj]
         _run(cmd)
         objects.append(obj)
     tmp = LIBRARY.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={OFFLOAD_ARCH}", "-shared", "-fPIC", *objects, "-o", tmp,
-          f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
+          f"-L{ROCM}/lib", "-lamdhip64", "-pthread", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, LIBRARY)
     return LIBRARY
 
