@@ -30,6 +30,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 from pathlib import Path
@@ -82,6 +83,10 @@ def main(argv=None):
     ap.add_argument("--profile-large", default=None, metavar="LANES",
                     help="only step the layered box two frames per listed lane count (e.g. 1,4,8,16) for a profiler, "
                          "and exit")
+    ap.add_argument("--precompute-cells", default="16,4,4",
+                    help="cantilever cells of the cubature precompute leg ('' skips it)")
+    ap.add_argument("--profile-precompute", default=None, metavar="CELLS:RINGS",
+                    help="only the cubature precompute of the layered box, e.g. 32,32,64:1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args(argv)
@@ -93,6 +98,10 @@ def main(argv=None):
         return profile_contact(args.profile_contact)
     if args.profile_large:
         return profile_large([int(x) for x in args.profile_large.split(",")], args.large_cells, args.iterations)
+    if args.profile_precompute:
+        cells, rings = args.profile_precompute.split(":")
+        print(json.dumps(precompute_layered_box(tuple(int(c) for c in cells.split(",")), int(rings))), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every leg runs on one GPU).",
               file=sys.stderr)
@@ -226,6 +235,8 @@ def main(argv=None):
     if args.large_steps > 0:
         cells = tuple(int(c) for c in args.large_cells.split(","))
         result["single_large_mesh"] = measure_large(cells, it, args.large_steps)
+    if args.precompute_cells:
+        result["precompute"] = measure_precompute(tuple(int(c) for c in args.precompute_cells.split(",")))
     if not args.no_accuracy:
         result["accuracy"] = accuracy_check(it, args.seed)
     if not args.no_cpu_baseline:
@@ -489,6 +500,65 @@ def measure_large(cells, iterations, steps):
         solver.destroy()
         out[schedule] = leg
         log(f"large mesh {schedule}: {leg['ms_per_step']:.2f} ms/frame")
+    return out
+
+
+def _host_threads():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+def _train(scene, materials, material_ids, rings):
+    from jgs2.precompute import precompute_cubature
+
+    rest = scene.vertex_rest.reshape(-1, 4)[:, :3].astype(np.float64)
+    fixed = scene.vertex_info.reshape(-1, 4)[:, 2] != 0
+    t0 = time.perf_counter()
+    r = precompute_cubature(rest, scene.tet_indices.reshape(-1, 4), fixed, material_ids, materials, 1.0 / 60.0,
+                            scene.cubature_k, rings=rings, threads=_host_threads(), residual_bound=1.0)
+    seconds = time.perf_counter() - t0
+    free = ~fixed
+    return {"rings": rings, "seconds": seconds, "vertices_per_s": float(free.sum()) / seconds,
+            "phases_s": {k: round(float(v), 4) for k, v in r.timings.items() if k.endswith("_s")},
+            "modes_on_gpu": bool(r.timings["gpu"]), "gpu_cg_ms": float(r.timings["gpu_cg_ms"]),
+            "gpu_cg_iterations": r.timings["gpu_cg_iterations"],
+            "packed_residual_max": float(np.nanmax(r.residuals)), "packed_residual_mean": float(np.nanmean(r.residuals)),
+            "records": int((r.tet_ids != 0xFFFFFFFF).sum()), "host_threads": _host_threads()}
+
+
+def measure_precompute(cells):
+    """Cubature precompute (jgs2_precompute_cubature) of the stable Neo-Hookean
+    cantilever: the reference's whole-mesh basis (rings 0) and the 2-ring
+    basis, K = 6, dt = 1/60; host training, modes on the dense factor."""
+    from jgs2 import scenes
+
+    scene = scenes.cantilever_beam(cells=cells, size=(cells[0] * 0.25, cells[1] * 0.25, cells[2] * 0.25))
+    out = {"workload": f"cantilever-{cells[0]}x{cells[1]}x{cells[2]}", "vertices": scene.vertex_count,
+           "tetrahedra": scene.tet_count, "samples": scene.cubature_k}
+    mats = [(1000.0, 1e5, 0.4, "stable-neo-hookean")]
+    for rings in (0, 2):
+        out[f"rings_{rings}"] = _train(scene, mats, np.zeros(scene.tet_count), rings)
+        log(f"precompute {out['workload']} rings {rings}: {out[f'rings_{rings}']['seconds']:.2f} s")
+    return out
+
+
+def precompute_layered_box(cells, rings):
+    """Cubature precompute of the layered box (two stable Neo-Hookean layer
+    materials, E = 1e4 / 1e6 Pa): modes by GPU conjugate gradients, training
+    on the host threads."""
+    from jgs2 import scenes
+
+    scene = scenes.layered_box(cells=cells)
+    mu = scene.tet_meta.reshape(-1, 4)[:, 2]
+    mid = (mu > np.sqrt(mu.min() * mu.max())).astype(np.uint32)
+    mats = [(1000.0, 1e4, 0.45, "stable-neo-hookean"), (1000.0, 1e6, 0.45, "stable-neo-hookean")]
+    out = {"workload": f"layered-box-{cells[0]}x{cells[1]}x{cells[2]}", "vertices": scene.vertex_count,
+           "tetrahedra": scene.tet_count, "samples": scene.cubature_k}
+    log(f"precompute {out['workload']} rings {rings}: start")
+    out.update(_train(scene, mats, mid, rings))
+    log(f"precompute {out['workload']} rings {rings}: {out['seconds']:.1f} s")
     return out
 
 

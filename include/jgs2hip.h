@@ -446,7 +446,10 @@ typedef struct jgs2_precompute_options {
    * (a local solve per vertex; the cost no longer grows with the mesh). */
   uint32_t basis_rings;
   uint32_t threads;                /* 0: every hardware thread */
-  uint32_t mode_solver;            /* 0: automatic, 1: dense Cholesky, 2: preconditioned CG */
+  /* 0: automatic (dense up to 6144 free coordinates, else CG on the GPU when
+   * one is present, else on the host), 1: dense Cholesky, 2: host CG,
+   * 3: GPU CG (JGS2_ERROR_DEVICE without a GPU) */
+  uint32_t mode_solver;
   uint32_t reserved;
   double cg_tolerance;             /* relative residual of the CG mode solves; 0: 1e-12 */
   /* Largest packed training residual accepted per vertex; 0: the
@@ -461,6 +464,10 @@ typedef struct jgs2_precompute_output {
   float* cubature_basis;           /* V*K*36: four row-major 3x3 rest-basis blocks per record */
   double* vertex_residuals;        /* optional, V: packed training residual (NaN when pinned) */
   uint32_t* vertex_ranks;          /* optional, 3 per vertex: nonzero candidates, column rank, packed rank */
+  /* optional, 6 values: seconds for the rest system, the modes and poses,
+   * the per-vertex training; GPU CG milliseconds and iterations; 1 when the
+   * GPU solved the modes */
+  double* timings;
 } jgs2_precompute_output;
 
 /* Errors: JGS2_ERROR_TYPE for NULL arrays; JGS2_ERROR_RANGE for the

@@ -72,3 +72,33 @@ def test_trained_cantilever_matches_fp64_oracle(cells, size, rings):
     hist = solver.read_globalization_history().reshape(64, 8, 4)
     assert np.array_equal(hist[:8, 1, 0], frames.history()[:8, 1, 0])  # same accept decisions
     solver.destroy()
+
+
+def test_device_mode_solver_agrees_with_the_dense_factor():
+    from jgs2.precompute import precompute_cubature
+    from test_precompute_oracle import cuboid
+    print("case: precompute modes by device CG vs dense factor, 4x2x2 and 8x4x4", flush=True)
+    pos, tets, fixed = cuboid((4, 2, 2), (0, 0, 0), (2, 1, 1), lambda p, c: c[0] == 0)
+    args = (pos, tets, fixed, np.zeros(len(tets)), [(1000, 1e5, 0.4, 1)], 1 / 60)
+    a = precompute_cubature(*args, 6, rings=2, residual_bound=1.0, mode_solver="dense")
+    b = precompute_cubature(*args, 6, rings=2, residual_bound=1.0, mode_solver="gpu-cg", cg_tolerance=1e-13)
+    assert b.timings["gpu"] and b.timings["gpu_cg_iterations"] > 0
+    same = np.mean([np.array_equal(a.tet_ids[v], b.tet_ids[v]) for v in range(len(pos))])
+    free = fixed == 0
+    print(f"same selections {same:.3f}, GPU CG {b.timings['gpu_cg_iterations']} iterations "
+          f"{b.timings['gpu_cg_ms']:.1f} ms", flush=True)
+    assert same >= 0.9
+    np.testing.assert_allclose(a.residuals[free], b.residuals[free], rtol=0.05, atol=1e-3)
+    # a system past the dense limit: device CG against host CG
+    pos, tets, fixed = cuboid((16, 8, 8), (0, 0, 0), (2, 1, 1), lambda p, c: c[0] == 0)
+    args = (pos, tets, fixed, np.zeros(len(tets)), [(1000, 1e5, 0.4, 1)], 1 / 60)
+    h = precompute_cubature(*args, 6, rings=1, residual_bound=1.0, mode_solver="cg", cg_tolerance=1e-12)
+    g = precompute_cubature(*args, 6, rings=1, residual_bound=1.0, cg_tolerance=1e-12)
+    assert g.timings["gpu"] and not h.timings["gpu"]
+    same = np.mean([np.array_equal(h.tet_ids[v], g.tet_ids[v]) for v in range(len(pos))])
+    free = fixed == 0
+    print(f"V={len(pos)}: same selections {same:.3f}; host modes {h.timings['modes_s']:.2f} s, "
+          f"device modes {g.timings['modes_s']:.2f} s ({g.timings['gpu_cg_iterations']} CG iterations, "
+          f"{g.timings['gpu_cg_ms']:.0f} ms)", flush=True)
+    assert same >= 0.9
+    np.testing.assert_allclose(h.residuals[free], g.residuals[free], rtol=0.05, atol=1e-3)

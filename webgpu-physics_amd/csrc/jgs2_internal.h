@@ -13,6 +13,28 @@
 
 namespace jgs2 {
 
+// FP64 conjugate gradients on the current HIP device for the cubature
+// precompute's global rest-Hessian solves (jgs2_precompute_kernels.hip): the
+// block-CSR Hessian over free vertices is uploaded once, then solve() runs
+// block-Jacobi preconditioned CG on up to 32 right-hand sides together
+// (n x m row-major, overwritten by the solutions).
+class GpuCg {
+ public:
+  GpuCg();
+  ~GpuCg();
+  GpuCg(const GpuCg&) = delete;
+  GpuCg& operator=(const GpuCg&) = delete;
+  static bool available();
+  jgs2_status upload(uint32_t rows, const uint32_t* start, const uint32_t* col, const double* blocks,
+                     const double* block_jacobi);
+  jgs2_status solve(double* b, uint32_t m, double tol, uint32_t max_iterations, uint32_t* iterations,
+                    double* milliseconds);
+
+ private:
+  struct Impl;
+  Impl* impl_;
+};
+
 // Records the calling thread's last error message and returns `status`.
 jgs2_status fail(jgs2_status status, const char* format, ...);
 void clear_error();

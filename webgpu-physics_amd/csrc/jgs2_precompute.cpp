@@ -17,7 +17,9 @@
 //  * the low-frequency modes of the pose corpus (cubature.ts:224-297,
 //    nonlinear-cubature.ts:338-620) come from block inverse iteration whose
 //    solves use the dense factor for small systems and block-Jacobi
-//    preconditioned conjugate gradients otherwise;
+//    preconditioned FP64 conjugate gradients otherwise, on the GPU when one is
+//    present (jgs2_precompute_kernels.hip: every mode advances in one pass
+//    over the resident Hessian);
 //  * greedy nonnegative selection (cubature.ts:386-691) uses a memoized Gram
 //    matrix and evaluates the NNLS masks shared by every candidate of a step
 //    once, replaying them in mask order so every comparison is the
@@ -31,11 +33,13 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <functional>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -255,6 +259,9 @@ struct Precompute {
   size_t n = 0;                            // 3 * free vertices
   Vec dense_l;                             // dense factor when n <= kDenseModeLimit
   bool dense = false;
+  std::shared_ptr<GpuCg> gpu;              // device CG for the mode solves of large systems
+  mutable double gpu_ms = 0.0;
+  mutable uint64_t gpu_iterations = 0;
 };
 
 void validate_and_load(Precompute& p, const jgs2_precompute_input& in) {
@@ -458,6 +465,17 @@ void build_rest(Precompute& p) {
     for (int c = 0; c < 3; ++c) b[c * 4] += inertia;
   }
   p.dense = p.mode_solver == 1 || (p.mode_solver == 0 && p.n <= kDenseModeLimit);
+  if (!p.dense && (p.mode_solver == 3 || (p.mode_solver == 0 && GpuCg::available()))) {
+    Vec pre(9 * A);
+    for (size_t a = 0; a < A; ++a) {
+      const uint32_t* lo = &p.hb_col[p.hb_start[a]];
+      const size_t k = size_t(std::lower_bound(lo, lo + (p.hb_start[a + 1] - p.hb_start[a]), uint32_t(a)) - p.hb_col.data());
+      inv3(&p.hb[9 * k], &pre[9 * a]);
+    }
+    p.gpu = std::make_shared<GpuCg>();
+    const jgs2_status st = p.gpu->upload(uint32_t(A), p.hb_start.data(), p.hb_col.data(), p.hb.data(), pre.data());
+    if (st != JGS2_OK) raise(st, "Cubature precompute: the device mode solver could not be set up.");
+  }
   if (p.dense) {
     p.dense_l.assign(p.n * p.n, 0.0);
     for (size_t a = 0; a < A; ++a)
@@ -632,6 +650,21 @@ std::vector<Vec> low_modes(const Precompute& p, size_t requested, size_t iterati
   for (size_t it = 0; it < iterations; ++it) {
     if (p.dense) {
       parallel_for(count, p.threads, [&](size_t k, unsigned) { global_solve(p, m[k].data()); });
+    } else if (p.gpu) {
+      for (size_t first = 0; first < count; first += 32) {  // up to 32 right-hand sides per device solve
+        const uint32_t cols = uint32_t(std::min<size_t>(32, count - first));
+        Vec block(n * cols);
+        for (size_t r = 0; r < n; ++r)
+          for (uint32_t j = 0; j < cols; ++j) block[r * cols + j] = m[first + j][r];
+        uint32_t iters = 0;
+        double ms = 0.0;
+        const jgs2_status st = p.gpu->solve(block.data(), cols, p.cg_tolerance, 20000, &iters, &ms);
+        if (st != JGS2_OK) raise(st, "Cubature precompute: the device mode solve failed.");
+        p.gpu_ms += ms;
+        p.gpu_iterations += iters;
+        for (size_t r = 0; r < n; ++r)
+          for (uint32_t j = 0; j < cols; ++j) m[first + j][r] = block[r * cols + j];
+      }
     } else {
       for (size_t k = 0; k < count; ++k) global_solve(p, m[k].data());
     }
@@ -1495,7 +1528,9 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
     Precompute p;
     jgs2_precompute_options opt{};
     if (options) opt = *options;
-    if (opt.mode_solver > 2) raise(JGS2_ERROR_RANGE, "mode_solver must be 0 (automatic), 1 (dense) or 2 (conjugate gradients).");
+    if (opt.mode_solver > 3)
+      raise(JGS2_ERROR_RANGE,
+            "mode_solver must be 0 (automatic), 1 (dense), 2 (host conjugate gradients) or 3 (device conjugate gradients).");
     p.rings = opt.basis_rings;
     p.mode_solver = opt.mode_solver;
     p.cg_tolerance = opt.cg_tolerance > 0 ? opt.cg_tolerance : 1e-12;
@@ -1503,8 +1538,12 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
       raise(JGS2_ERROR_RANGE, "residual_bound must be finite and nonnegative.");
     p.residual_bound = opt.residual_bound > 0 ? opt.residual_bound : 0.01;
     p.threads = opt.threads ? opt.threads : std::max(1u, std::thread::hardware_concurrency());
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    auto seconds = [](clock::time_point a, clock::time_point b) { return std::chrono::duration<double>(b - a).count(); };
     validate_and_load(p, *input);
     build_rest(p);
+    const auto t1 = clock::now();
     const size_t V = p.V, K = p.K;
     std::fill(output->cubature_tet_ids, output->cubature_tet_ids + V * K, 0xffffffffu);
     std::fill(output->cubature_weights, output->cubature_weights + V * K, 0.0f);
@@ -1521,6 +1560,7 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
     } else if (!p.snh) {
       modes = low_modes(p, 8, 24);
     }
+    const auto t2 = clock::now();
     std::vector<Support> supports(p.threads);
     parallel_for(p.active.size(), p.threads, [&](size_t a, unsigned worker) {
       const uint32_t v = p.active[a];
@@ -1547,6 +1587,11 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
         output->vertex_ranks[3 * size_t(v) + 2] = r.packed_rank;
       }
     });
+    if (output->timings) {
+      const double t[6] = {seconds(t0, t1), seconds(t1, t2), seconds(t2, clock::now()), p.gpu_ms,
+                           double(p.gpu_iterations), p.gpu ? 1.0 : 0.0};
+      std::memcpy(output->timings, t, sizeof t);
+    }
     return JGS2_OK;
   } catch (const Error& e) {
     return fail(e.status, "%s", e.message.c_str());

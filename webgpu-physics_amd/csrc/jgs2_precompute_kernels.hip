@@ -1,0 +1,308 @@
+// FP64 conjugate gradients on the GPU for the cubature precompute's global
+// solves: the block inverse iteration that yields the low-frequency rest
+// modes (src/simulation/cpu/cubature.ts:224-297) solves H Y = X for every
+// mode at each of its iterations. The rest Hessian H = M/h^2 + K stays
+// resident in HBM as 3x3 blocks in CSR over free vertices; all m right-hand
+// sides advance together so each iteration reads H once.
+//
+// Layout: vectors are n x m row-major (coordinate-major, the m right-hand
+// sides of one coordinate adjacent), so a wave's lanes read consecutive
+// doubles. Reductions are two-stage and fixed-order (per-block partials, then
+// one pass over the blocks), so results do not depend on scheduling. Each
+// right-hand side runs its own CG recurrence; once converged its step sizes
+// are zero and it no longer changes.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "jgs2_internal.h"
+
+namespace jgs2 {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxRhs = 32;
+constexpr int kDotBlocks = 1024;
+
+// y = H x for m right-hand sides; one thread per (free vertex, rhs).
+__global__ void __launch_bounds__(kThreads)
+k_bsr_spmm(const uint32_t* __restrict__ start, const uint32_t* __restrict__ col, const double* __restrict__ blocks,
+           const double* __restrict__ x, double* __restrict__ y, uint32_t rows, uint32_t m) {
+  const uint32_t per = kThreads / m;
+  const uint32_t j = threadIdx.x % m, local = threadIdx.x / m;
+  const uint32_t a = blockIdx.x * per + local;
+  if (local >= per || a >= rows) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (uint32_t k = start[a]; k < start[a + 1]; ++k) {
+    const double* b = blocks + 9 * size_t(k);
+    const double* xv = x + size_t(3) * col[k] * m + j;
+    const double x0 = xv[0], x1 = xv[m], x2 = xv[2 * m];
+    s0 += b[0] * x0 + b[1] * x1 + b[2] * x2;
+    s1 += b[3] * x0 + b[4] * x1 + b[5] * x2;
+    s2 += b[6] * x0 + b[7] * x1 + b[8] * x2;
+  }
+  double* yv = y + size_t(3) * a * m + j;
+  yv[0] = s0;
+  yv[m] = s1;
+  yv[2 * m] = s2;
+}
+
+// Partial sums of <u_j, v_j> (and <w_j, w_j> when w is given) over a
+// grid-strided set of coordinates, in a fixed order per thread and a fixed
+// LDS tree per block.
+__global__ void __launch_bounds__(kThreads)
+k_dot_partial(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w, size_t n,
+              uint32_t m, double* __restrict__ partial) {
+  __shared__ double su[kThreads], sw[kThreads];
+  const uint32_t per = kThreads / m;
+  const uint32_t j = threadIdx.x % m, local = threadIdx.x / m;
+  double acc = 0.0, accw = 0.0;
+  if (local < per)
+    for (size_t i = size_t(blockIdx.x) * per + local; i < n; i += size_t(gridDim.x) * per) {
+      const size_t e = i * m + j;
+      acc += u[e] * v[e];
+      if (w) accw += w[e] * w[e];
+    }
+  su[threadIdx.x] = acc;
+  sw[threadIdx.x] = accw;
+  __syncthreads();
+  if (threadIdx.x < m) {
+    double s = 0.0, t = 0.0;
+    for (uint32_t r = 0; r < per; ++r) {
+      s += su[r * m + threadIdx.x];
+      t += sw[r * m + threadIdx.x];
+    }
+    partial[(size_t(blockIdx.x) * m + threadIdx.x) * 2] = s;
+    partial[(size_t(blockIdx.x) * m + threadIdx.x) * 2 + 1] = t;
+  }
+}
+
+// Sums the partials of each rhs in block order: out[2 j], out[2 j + 1].
+__global__ void k_dot_final(const double* __restrict__ partial, uint32_t blocks, uint32_t m, double* __restrict__ out) {
+  const uint32_t j = threadIdx.x;
+  if (j >= m) return;
+  double s = 0.0, t = 0.0;
+  for (uint32_t b = 0; b < blocks; ++b) {
+    s += partial[(size_t(b) * m + j) * 2];
+    t += partial[(size_t(b) * m + j) * 2 + 1];
+  }
+  out[2 * j] = s;
+  out[2 * j + 1] = t;
+}
+
+// Per-rhs CG scalars. state[j] = {rz, alpha, beta, done}; dq = <d, q>;
+// rz_new/rr = <r, z>, <r, r> after the update; threshold[j] = (tol |b_j|)^2.
+__global__ void k_cg_alpha(double* __restrict__ state, const double* __restrict__ dq, uint32_t m) {
+  const uint32_t j = threadIdx.x;
+  if (j >= m) return;
+  double* s = state + 4 * j;
+  s[1] = (s[3] != 0.0 || !(dq[2 * j] > 0.0)) ? 0.0 : s[0] / dq[2 * j];
+}
+
+__global__ void k_cg_beta(double* __restrict__ state, const double* __restrict__ rz, const double* __restrict__ threshold,
+                          uint32_t m) {
+  const uint32_t j = threadIdx.x;
+  if (j >= m) return;
+  double* s = state + 4 * j;
+  if (s[3] != 0.0) {
+    s[2] = 0.0;
+    return;
+  }
+  const double rz_new = rz[2 * j], rr = rz[2 * j + 1];
+  if (rr <= threshold[j] || !(s[0] > 0.0)) {
+    s[3] = 1.0;
+    s[2] = 0.0;
+    s[1] = 0.0;
+    return;
+  }
+  s[2] = rz_new / s[0];
+  s[0] = rz_new;
+}
+
+// x += alpha d, r -= alpha q, z = P r (block-Jacobi) for one free vertex and rhs.
+__global__ void __launch_bounds__(kThreads)
+k_cg_update(double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ d,
+            const double* __restrict__ q, const double* __restrict__ pre, const double* __restrict__ state,
+            uint32_t rows, uint32_t m) {
+  const uint32_t per = kThreads / m;
+  const uint32_t j = threadIdx.x % m, local = threadIdx.x / m;
+  const uint32_t a = blockIdx.x * per + local;
+  if (local >= per || a >= rows) return;
+  const double alpha = state[4 * j + 1];
+  const size_t e = size_t(3) * a * m + j;
+  double rv[3];
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = e + size_t(c) * m;
+    x[i] += alpha * d[i];
+    rv[c] = r[i] - alpha * q[i];
+    r[i] = rv[c];
+  }
+  const double* p = pre + 9 * size_t(a);
+  for (int c = 0; c < 3; ++c) z[e + size_t(c) * m] = p[c * 3] * rv[0] + p[c * 3 + 1] * rv[1] + p[c * 3 + 2] * rv[2];
+}
+
+// z = P r (the first preconditioned residual) for one free vertex and rhs.
+__global__ void __launch_bounds__(kThreads)
+k_cg_precondition(const double* __restrict__ r, double* __restrict__ z, const double* __restrict__ pre, uint32_t rows,
+                  uint32_t m) {
+  const uint32_t per = kThreads / m;
+  const uint32_t j = threadIdx.x % m, local = threadIdx.x / m;
+  const uint32_t a = blockIdx.x * per + local;
+  if (local >= per || a >= rows) return;
+  const size_t e = size_t(3) * a * m + j;
+  const double r0 = r[e], r1 = r[e + m], r2 = r[e + 2 * m];
+  const double* p = pre + 9 * size_t(a);
+  for (int c = 0; c < 3; ++c) z[e + size_t(c) * m] = p[c * 3] * r0 + p[c * 3 + 1] * r1 + p[c * 3 + 2] * r2;
+}
+
+// d = z + beta d (d = z when beta is 0).
+__global__ void __launch_bounds__(kThreads)
+k_cg_direction(double* __restrict__ d, const double* __restrict__ z, const double* __restrict__ state, size_t n,
+               uint32_t m) {
+  const size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n * m) return;
+  const double beta = state[4 * (i % m) + 2];
+  d[i] = beta == 0.0 ? z[i] : z[i] + beta * d[i];  // the first direction never reads d
+}
+
+bool hip_ok(hipError_t e) { return e == hipSuccess; }
+
+}  // namespace
+
+struct GpuCg::Impl {
+  uint32_t rows = 0;
+  size_t n = 0;
+  uint32_t* start = nullptr;
+  uint32_t* col = nullptr;
+  double* blocks = nullptr;
+  double* pre = nullptr;
+  hipStream_t stream = nullptr;
+  bool ready = false;
+};
+
+GpuCg::GpuCg() : impl_(new Impl) {}
+
+GpuCg::~GpuCg() {
+  if (impl_->stream) (void)hipStreamSynchronize(impl_->stream);
+  (void)hipFree(impl_->start);
+  (void)hipFree(impl_->col);
+  (void)hipFree(impl_->blocks);
+  (void)hipFree(impl_->pre);
+  if (impl_->stream) (void)hipStreamDestroy(impl_->stream);
+  delete impl_;
+}
+
+bool GpuCg::available() {
+  int count = 0;
+  return hipGetDeviceCount(&count) == hipSuccess && count > 0;
+}
+
+jgs2_status GpuCg::upload(uint32_t rows, const uint32_t* start, const uint32_t* col, const double* blocks,
+                          const double* block_jacobi) {
+  Impl& g = *impl_;
+  g.rows = rows;
+  g.n = size_t(3) * rows;
+  const size_t nnzb = start[rows];
+  if (!hip_ok(hipStreamCreate(&g.stream))) return fail(JGS2_ERROR_DEVICE, "hipStreamCreate failed.");
+  if (!hip_ok(hipMalloc(&g.start, (rows + 1) * sizeof(uint32_t))) || !hip_ok(hipMalloc(&g.col, nnzb * sizeof(uint32_t))) ||
+      !hip_ok(hipMalloc(&g.blocks, 9 * nnzb * sizeof(double))) || !hip_ok(hipMalloc(&g.pre, 9 * size_t(rows) * sizeof(double))))
+    return fail(JGS2_ERROR_MEMORY, "Cubature precompute: device allocation for the rest Hessian failed.");
+  if (!hip_ok(hipMemcpy(g.start, start, (rows + 1) * sizeof(uint32_t), hipMemcpyHostToDevice)) ||
+      !hip_ok(hipMemcpy(g.col, col, nnzb * sizeof(uint32_t), hipMemcpyHostToDevice)) ||
+      !hip_ok(hipMemcpy(g.blocks, blocks, 9 * nnzb * sizeof(double), hipMemcpyHostToDevice)) ||
+      !hip_ok(hipMemcpy(g.pre, block_jacobi, 9 * size_t(rows) * sizeof(double), hipMemcpyHostToDevice)))
+    return fail(JGS2_ERROR_DEVICE, "Cubature precompute: upload of the rest Hessian failed.");
+  g.ready = true;
+  return JGS2_OK;
+}
+
+// Solves H Y = B for m <= 32 right-hand sides given n x m row-major; the
+// solutions overwrite `b`. Each rhs stops at |r| <= tol |b| (or after
+// max_iterations); `iterations` reports the iterations run.
+jgs2_status GpuCg::solve(double* b, uint32_t m, double tol, uint32_t max_iterations, uint32_t* iterations,
+                         double* milliseconds) {
+  Impl& g = *impl_;
+  if (!g.ready) return fail(JGS2_ERROR_STATE, "Cubature precompute: the rest Hessian is not on the device.");
+  if (m == 0 || m > kMaxRhs) return fail(JGS2_ERROR_RANGE, "Cubature precompute: 1 to 32 right-hand sides per solve.");
+  const size_t nm = g.n * m;
+  double *x = nullptr, *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr, *partial = nullptr, *dots = nullptr,
+         *state = nullptr, *threshold = nullptr;
+  auto release = [&]() {
+    for (double* p : {x, r, z, d, q, partial, dots, state, threshold}) (void)hipFree(p);
+  };
+  if (!hip_ok(hipMalloc(&x, nm * 8)) || !hip_ok(hipMalloc(&r, nm * 8)) || !hip_ok(hipMalloc(&z, nm * 8)) ||
+      !hip_ok(hipMalloc(&d, nm * 8)) || !hip_ok(hipMalloc(&q, nm * 8)) ||
+      !hip_ok(hipMalloc(&partial, size_t(kDotBlocks) * m * 2 * 8)) || !hip_ok(hipMalloc(&dots, 2 * m * 8)) ||
+      !hip_ok(hipMalloc(&state, 4 * m * 8)) || !hip_ok(hipMalloc(&threshold, m * 8))) {
+    release();
+    return fail(JGS2_ERROR_MEMORY, "Cubature precompute: device allocation for the CG vectors failed.");
+  }
+  // |b_j|^2 on the host (the right-hand sides are there already)
+  std::vector<double> bn(m, 0.0), th(m), st(4 * m, 0.0), dv(2 * m, 0.0);
+  for (size_t i = 0; i < g.n; ++i)
+    for (uint32_t j = 0; j < m; ++j) bn[j] += b[i * m + j] * b[i * m + j];
+  for (uint32_t j = 0; j < m; ++j) {
+    th[j] = tol * tol * bn[j];
+    st[4 * j + 3] = bn[j] == 0.0 ? 1.0 : 0.0;  // a zero rhs is solved by x = 0
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const uint32_t per = kThreads / m;
+  const uint32_t row_blocks = (g.rows + per - 1) / per;
+  const uint32_t dot_blocks = uint32_t(std::min<size_t>(kDotBlocks, (g.n + per - 1) / per));
+  const uint32_t flat_blocks = uint32_t((nm + kThreads - 1) / kThreads);
+  // state = {rz, alpha, beta, done}; beta = 0 makes the first direction d = z
+  bool ok = hip_ok(hipMemcpyAsync(r, b, nm * 8, hipMemcpyHostToDevice, g.stream)) &&
+            hip_ok(hipMemcpyAsync(threshold, th.data(), m * 8, hipMemcpyHostToDevice, g.stream)) &&
+            hip_ok(hipMemcpyAsync(state, st.data(), 4 * m * 8, hipMemcpyHostToDevice, g.stream)) &&
+            hip_ok(hipMemsetAsync(x, 0, nm * 8, g.stream));
+  (void)hipEventRecord(e0, g.stream);
+  k_cg_precondition<<<row_blocks, kThreads, 0, g.stream>>>(r, z, g.pre, g.rows, m);
+  k_cg_direction<<<flat_blocks, kThreads, 0, g.stream>>>(d, z, state, g.n, m);
+  k_dot_partial<<<dot_blocks, kThreads, 0, g.stream>>>(r, z, nullptr, g.n, m, partial);
+  k_dot_final<<<1, kMaxRhs, 0, g.stream>>>(partial, dot_blocks, m, dots);
+  ok = ok && hip_ok(hipMemcpyAsync(dv.data(), dots, 2 * m * 8, hipMemcpyDeviceToHost, g.stream)) &&
+       hip_ok(hipStreamSynchronize(g.stream));
+  for (uint32_t j = 0; j < m; ++j) st[4 * j] = dv[2 * j];
+  ok = ok && hip_ok(hipMemcpyAsync(state, st.data(), 4 * m * 8, hipMemcpyHostToDevice, g.stream));
+  uint32_t it = 0;
+  std::vector<double> host_state(4 * m);
+  while (ok && it < max_iterations) {
+    k_bsr_spmm<<<row_blocks, kThreads, 0, g.stream>>>(g.start, g.col, g.blocks, d, q, g.rows, m);
+    k_dot_partial<<<dot_blocks, kThreads, 0, g.stream>>>(d, q, nullptr, g.n, m, partial);
+    k_dot_final<<<1, kMaxRhs, 0, g.stream>>>(partial, dot_blocks, m, dots);
+    k_cg_alpha<<<1, kMaxRhs, 0, g.stream>>>(state, dots, m);
+    k_cg_update<<<row_blocks, kThreads, 0, g.stream>>>(x, r, z, d, q, g.pre, state, g.rows, m);
+    k_dot_partial<<<dot_blocks, kThreads, 0, g.stream>>>(r, z, r, g.n, m, partial);
+    k_dot_final<<<1, kMaxRhs, 0, g.stream>>>(partial, dot_blocks, m, dots);
+    k_cg_beta<<<1, kMaxRhs, 0, g.stream>>>(state, dots, threshold, m);
+    k_cg_direction<<<flat_blocks, kThreads, 0, g.stream>>>(d, z, state, g.n, m);
+    ++it;
+    if (it % 8 == 0 || it == max_iterations) {
+      ok = hip_ok(hipMemcpyAsync(host_state.data(), state, 4 * m * 8, hipMemcpyDeviceToHost, g.stream)) &&
+           hip_ok(hipStreamSynchronize(g.stream));
+      bool all = true;
+      for (uint32_t j = 0; j < m; ++j) all = all && host_state[4 * j + 3] != 0.0;
+      if (all) break;
+    }
+  }
+  (void)hipEventRecord(e1, g.stream);
+  ok = ok && hip_ok(hipMemcpyAsync(b, x, nm * 8, hipMemcpyDeviceToHost, g.stream)) &&
+       hip_ok(hipStreamSynchronize(g.stream)) && hip_ok(hipGetLastError());
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  release();
+  if (iterations) *iterations = it;
+  if (milliseconds) *milliseconds = ms;
+  if (!ok) return fail(JGS2_ERROR_DEVICE, "Cubature precompute: conjugate gradients on the device failed.");
+  return JGS2_OK;
+}
+
+}  // namespace jgs2

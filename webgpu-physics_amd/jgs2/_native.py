@@ -270,6 +270,7 @@ class PrecomputeOutput(C.Structure):
         ("cubature_basis", f32p),
         ("vertex_residuals", C.POINTER(C.c_double)),
         ("vertex_ranks", u32p),
+        ("timings", C.POINTER(C.c_double)),
     ]
 
 

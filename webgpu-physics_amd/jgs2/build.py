@@ -43,6 +43,7 @@ SOURCES = [
     ("jgs2_scene.cpp", False),
     ("jgs2_scene_contact.cpp", False),
     ("jgs2_precompute.cpp", False),
+    ("jgs2_precompute_kernels.hip", True),
 ]
 
 

@@ -22,7 +22,7 @@ from . import _native as N
 from .solver import GpuInput
 
 MODELS = {"corotated-linear": N.MATERIAL_COROTATED_LINEAR, "stable-neo-hookean": N.MATERIAL_STABLE_NEO_HOOKEAN}
-MODE_SOLVERS = {"auto": 0, "dense": 1, "cg": 2}
+MODE_SOLVERS = {"auto": 0, "dense": 1, "cg": 2, "gpu-cg": 3}
 
 
 @dataclass
@@ -32,6 +32,7 @@ class CubatureRecords:
     basis: np.ndarray       # [V, K, 36] float32: four row-major 3x3 rest-basis blocks
     residuals: np.ndarray   # [V] packed training residual (NaN for pinned vertices)
     ranks: np.ndarray       # [V, 3] nonzero candidates, column rank, packed column rank
+    timings: dict           # seconds per phase; GPU CG milliseconds and iterations
 
 
 def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, timestep, samples=6, *,
@@ -57,11 +58,14 @@ def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, t
     opt = N.PrecomputeOptions(int(rings), int(threads), MODE_SOLVERS[mode_solver], 0, float(cg_tolerance),
                               float(residual_bound))
     out = CubatureRecords(np.zeros((V, K), np.uint32), np.zeros((V, K), np.float32),
-                          np.zeros((V, K, 36), np.float32), np.zeros(V), np.zeros((V, 3), np.uint32))
+                          np.zeros((V, K, 36), np.float32), np.zeros(V), np.zeros((V, 3), np.uint32), {})
+    timings = np.zeros(6)
     o = N.PrecomputeOutput(out.tet_ids.ctypes.data_as(N.u32p), out.weights.ctypes.data_as(N.f32p),
                            out.basis.ctypes.data_as(N.f32p), out.residuals.ctypes.data_as(C.POINTER(C.c_double)),
-                           out.ranks.ctypes.data_as(N.u32p))
+                           out.ranks.ctypes.data_as(N.u32p), timings.ctypes.data_as(C.POINTER(C.c_double)))
     N.check(N.lib.jgs2_precompute_cubature(C.byref(inp), C.byref(opt), C.byref(o)))
+    out.timings = {"rest_s": timings[0], "modes_s": timings[1], "training_s": timings[2],
+                   "gpu_cg_ms": timings[3], "gpu_cg_iterations": int(timings[4]), "gpu": bool(timings[5])}
     return out
 
 
