@@ -89,11 +89,11 @@ def test_device_mode_solver_agrees_with_the_dense_factor():
           f"{b.timings['gpu_cg_ms']:.1f} ms", flush=True)
     assert same >= 0.9
     np.testing.assert_allclose(a.residuals[free], b.residuals[free], rtol=0.05, atol=1e-3)
-    # a system past the dense limit: device CG against host CG
+    # a larger system: device CG against host CG
     pos, tets, fixed = cuboid((16, 8, 8), (0, 0, 0), (2, 1, 1), lambda p, c: c[0] == 0)
     args = (pos, tets, fixed, np.zeros(len(tets)), [(1000, 1e5, 0.4, 1)], 1 / 60)
     h = precompute_cubature(*args, 6, rings=1, residual_bound=1.0, mode_solver="cg", cg_tolerance=1e-12)
-    g = precompute_cubature(*args, 6, rings=1, residual_bound=1.0, cg_tolerance=1e-12)
+    g = precompute_cubature(*args, 6, rings=1, residual_bound=1.0, mode_solver="gpu-cg", cg_tolerance=1e-12)
     assert g.timings["gpu"] and not h.timings["gpu"]
     same = np.mean([np.array_equal(h.tet_ids[v], g.tet_ids[v]) for v in range(len(pos))])
     free = fixed == 0

@@ -81,30 +81,67 @@ k_dot_partial(const double* __restrict__ u, const double* __restrict__ v, const 
   }
 }
 
-// Sums the partials of each rhs in block order: out[2 j], out[2 j + 1].
-__global__ void k_dot_final(const double* __restrict__ partial, uint32_t blocks, uint32_t m, double* __restrict__ out) {
-  const uint32_t j = threadIdx.x;
-  if (j >= m) return;
-  double s = 0.0, t = 0.0;
-  for (uint32_t b = 0; b < blocks; ++b) {
-    s += partial[(size_t(b) * m + j) * 2];
-    t += partial[(size_t(b) * m + j) * 2 + 1];
+// The scalar kernels below run as one block of kLanes x kMaxRhs threads:
+// thread (lane, j) sums partials lane, lane + kLanes, ... of rhs j, then a
+// fixed LDS tree over the lanes finishes the sum, so the order is fixed.
+constexpr int kLanes = 32;
+
+__device__ void sum_partials(const double* __restrict__ partial, uint32_t blocks, uint32_t m, double* s0, double* s1,
+                             double& a, double& b) {
+  const uint32_t j = threadIdx.x % kMaxRhs, lane = threadIdx.x / kMaxRhs;
+  double x = 0.0, y = 0.0;
+  if (j < m)
+    for (uint32_t blk = lane; blk < blocks; blk += kLanes) {
+      x += partial[(size_t(blk) * m + j) * 2];
+      y += partial[(size_t(blk) * m + j) * 2 + 1];
+    }
+  s0[threadIdx.x] = x;
+  s1[threadIdx.x] = y;
+  __syncthreads();
+  for (uint32_t stride = kLanes / 2; stride > 0; stride /= 2) {
+    if (lane < stride) {
+      s0[threadIdx.x] += s0[threadIdx.x + stride * kMaxRhs];
+      s1[threadIdx.x] += s1[threadIdx.x + stride * kMaxRhs];
+    }
+    __syncthreads();
   }
-  out[2 * j] = s;
-  out[2 * j + 1] = t;
+  a = s0[j];
+  b = s1[j];
 }
 
-// Per-rhs CG scalars. state[j] = {rz, alpha, beta, done}; dq = <d, q>;
-// rz_new/rr = <r, z>, <r, r> after the update; threshold[j] = (tol |b_j|)^2.
-__global__ void k_cg_alpha(double* __restrict__ state, const double* __restrict__ dq, uint32_t m) {
+// out[2 j], out[2 j + 1]: the two sums of rhs j.
+__global__ void __launch_bounds__(kLanes* kMaxRhs)
+k_dot_final(const double* __restrict__ partial, uint32_t blocks, uint32_t m, double* __restrict__ out) {
+  __shared__ double s0[kLanes * kMaxRhs], s1[kLanes * kMaxRhs];
+  double a, b;
+  sum_partials(partial, blocks, m, s0, s1, a, b);
+  const uint32_t j = threadIdx.x;
+  if (j >= m) return;
+  out[2 * j] = a;
+  out[2 * j + 1] = b;
+}
+
+// Per-rhs CG scalars, state[j] = {rz, alpha, beta, done}.
+// alpha = rz / <d, q>; zero once converged or when <d, q> is not positive.
+__global__ void __launch_bounds__(kLanes* kMaxRhs)
+k_cg_alpha(const double* __restrict__ partial, uint32_t blocks, double* __restrict__ state, uint32_t m) {
+  __shared__ double s0[kLanes * kMaxRhs], s1[kLanes * kMaxRhs];
+  double dq, unused;
+  sum_partials(partial, blocks, m, s0, s1, dq, unused);
   const uint32_t j = threadIdx.x;
   if (j >= m) return;
   double* s = state + 4 * j;
-  s[1] = (s[3] != 0.0 || !(dq[2 * j] > 0.0)) ? 0.0 : s[0] / dq[2 * j];
+  s[1] = (s[3] != 0.0 || !(dq > 0.0)) ? 0.0 : s[0] / dq;
 }
 
-__global__ void k_cg_beta(double* __restrict__ state, const double* __restrict__ rz, const double* __restrict__ threshold,
-                          uint32_t m) {
+// beta = <r, z>_new / <r, z>; converged (done, alpha = beta = 0) once
+// <r, r> <= threshold[j] = (tol |b_j|)^2.
+__global__ void __launch_bounds__(kLanes* kMaxRhs)
+k_cg_beta(const double* __restrict__ partial, uint32_t blocks, double* __restrict__ state,
+          const double* __restrict__ threshold, uint32_t m) {
+  __shared__ double s0[kLanes * kMaxRhs], s1[kLanes * kMaxRhs];
+  double rz_new, rr;
+  sum_partials(partial, blocks, m, s0, s1, rz_new, rr);
   const uint32_t j = threadIdx.x;
   if (j >= m) return;
   double* s = state + 4 * j;
@@ -112,7 +149,6 @@ __global__ void k_cg_beta(double* __restrict__ state, const double* __restrict__
     s[2] = 0.0;
     return;
   }
-  const double rz_new = rz[2 * j], rr = rz[2 * j + 1];
   if (rr <= threshold[j] || !(s[0] > 0.0)) {
     s[3] = 1.0;
     s[2] = 0.0;
@@ -265,32 +301,51 @@ jgs2_status GpuCg::solve(double* b, uint32_t m, double tol, uint32_t max_iterati
   k_cg_precondition<<<row_blocks, kThreads, 0, g.stream>>>(r, z, g.pre, g.rows, m);
   k_cg_direction<<<flat_blocks, kThreads, 0, g.stream>>>(d, z, state, g.n, m);
   k_dot_partial<<<dot_blocks, kThreads, 0, g.stream>>>(r, z, nullptr, g.n, m, partial);
-  k_dot_final<<<1, kMaxRhs, 0, g.stream>>>(partial, dot_blocks, m, dots);
+  k_dot_final<<<1, kLanes * kMaxRhs, 0, g.stream>>>(partial, dot_blocks, m, dots);
   ok = ok && hip_ok(hipMemcpyAsync(dv.data(), dots, 2 * m * 8, hipMemcpyDeviceToHost, g.stream)) &&
        hip_ok(hipStreamSynchronize(g.stream));
   for (uint32_t j = 0; j < m; ++j) st[4 * j] = dv[2 * j];
   ok = ok && hip_ok(hipMemcpyAsync(state, st.data(), 4 * m * 8, hipMemcpyHostToDevice, g.stream));
+  // One CG iteration is seven launches; kChunk iterations run as one hipGraph
+  // (no memory-set nodes), after which the host reads the convergence flags.
+  // Iterations past a rhs's convergence leave it unchanged (alpha = 0).
+  constexpr uint32_t kChunk = 16;
+  auto iteration = [&]() {
+    k_bsr_spmm<<<row_blocks, kThreads, 0, g.stream>>>(g.start, g.col, g.blocks, d, q, g.rows, m);
+    k_dot_partial<<<dot_blocks, kThreads, 0, g.stream>>>(d, q, nullptr, g.n, m, partial);
+    k_cg_alpha<<<1, kLanes * kMaxRhs, 0, g.stream>>>(partial, dot_blocks, state, m);
+    k_cg_update<<<row_blocks, kThreads, 0, g.stream>>>(x, r, z, d, q, g.pre, state, g.rows, m);
+    k_dot_partial<<<dot_blocks, kThreads, 0, g.stream>>>(r, z, r, g.n, m, partial);
+    k_cg_beta<<<1, kLanes * kMaxRhs, 0, g.stream>>>(partial, dot_blocks, state, threshold, m);
+    k_cg_direction<<<flat_blocks, kThreads, 0, g.stream>>>(d, z, state, g.n, m);
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (ok && hip_ok(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal))) {
+    for (uint32_t k = 0; k < kChunk; ++k) iteration();
+    if (!hip_ok(hipStreamEndCapture(g.stream, &graph)) || !hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)))
+      exec = nullptr;
+  }
+  (void)hipGetLastError();
   uint32_t it = 0;
   std::vector<double> host_state(4 * m);
   while (ok && it < max_iterations) {
-    k_bsr_spmm<<<row_blocks, kThreads, 0, g.stream>>>(g.start, g.col, g.blocks, d, q, g.rows, m);
-    k_dot_partial<<<dot_blocks, kThreads, 0, g.stream>>>(d, q, nullptr, g.n, m, partial);
-    k_dot_final<<<1, kMaxRhs, 0, g.stream>>>(partial, dot_blocks, m, dots);
-    k_cg_alpha<<<1, kMaxRhs, 0, g.stream>>>(state, dots, m);
-    k_cg_update<<<row_blocks, kThreads, 0, g.stream>>>(x, r, z, d, q, g.pre, state, g.rows, m);
-    k_dot_partial<<<dot_blocks, kThreads, 0, g.stream>>>(r, z, r, g.n, m, partial);
-    k_dot_final<<<1, kMaxRhs, 0, g.stream>>>(partial, dot_blocks, m, dots);
-    k_cg_beta<<<1, kMaxRhs, 0, g.stream>>>(state, dots, threshold, m);
-    k_cg_direction<<<flat_blocks, kThreads, 0, g.stream>>>(d, z, state, g.n, m);
-    ++it;
-    if (it % 8 == 0 || it == max_iterations) {
-      ok = hip_ok(hipMemcpyAsync(host_state.data(), state, 4 * m * 8, hipMemcpyDeviceToHost, g.stream)) &&
-           hip_ok(hipStreamSynchronize(g.stream));
-      bool all = true;
-      for (uint32_t j = 0; j < m; ++j) all = all && host_state[4 * j + 3] != 0.0;
-      if (all) break;
+    if (exec && it + kChunk <= max_iterations) {
+      ok = hip_ok(hipGraphLaunch(exec, g.stream));
+      it += kChunk;
+    } else {
+      const uint32_t steps = std::min(kChunk, max_iterations - it);
+      for (uint32_t k = 0; k < steps; ++k) iteration();
+      it += steps;
     }
+    ok = ok && hip_ok(hipMemcpyAsync(host_state.data(), state, 4 * m * 8, hipMemcpyDeviceToHost, g.stream)) &&
+         hip_ok(hipStreamSynchronize(g.stream));
+    bool all = true;
+    for (uint32_t j = 0; j < m; ++j) all = all && host_state[4 * j + 3] != 0.0;
+    if (all) break;
   }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   (void)hipEventRecord(e1, g.stream);
   ok = ok && hip_ok(hipMemcpyAsync(b, x, nm * 8, hipMemcpyDeviceToHost, g.stream)) &&
        hip_ok(hipStreamSynchronize(g.stream)) && hip_ok(hipGetLastError());
