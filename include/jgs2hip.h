@@ -446,7 +446,7 @@ typedef struct jgs2_precompute_options {
    * (a local solve per vertex; the cost no longer grows with the mesh). */
   uint32_t basis_rings;
   uint32_t threads;                /* 0: every hardware thread */
-  /* 0: automatic (dense up to 6144 free coordinates, else CG on the GPU when
+  /* 0: automatic (dense up to 1536 free coordinates, else CG on the GPU when
    * one is present, else on the host), 1: dense Cholesky, 2: host CG,
    * 3: GPU CG (JGS2_ERROR_DEVICE without a GPU) */
   uint32_t mode_solver;

@@ -51,7 +51,7 @@ namespace {
 
 using Vec = std::vector<double>;
 constexpr double kEps = 1e-14;
-constexpr size_t kDenseModeLimit = 6144;  // free coordinates factored densely
+constexpr size_t kDenseModeLimit = 1536;  // free coordinates factored densely (serial O(n^3))
 
 struct Error {
   jgs2_status status;
