@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <vector>
 
 #include "jgs2_internal.h"
@@ -319,9 +320,13 @@ jgs2_status GpuCg::solve(double* b, uint32_t m, double tol, uint32_t max_iterati
     k_cg_beta<<<1, kLanes * kMaxRhs, 0, g.stream>>>(partial, dot_blocks, state, threshold, m);
     k_cg_direction<<<flat_blocks, kThreads, 0, g.stream>>>(d, z, state, g.n, m);
   };
+  // JGS2_CG_GRAPHS=0 launches every iteration eagerly (same kernels, same
+  // order, same results), for tools that trace kernels outside graphs only.
+  const char* env = std::getenv("JGS2_CG_GRAPHS");
+  const bool graphs = !(env && env[0] == '0');
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  if (ok && hip_ok(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal))) {
+  if (ok && graphs && hip_ok(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal))) {
     for (uint32_t k = 0; k < kChunk; ++k) iteration();
     if (!hip_ok(hipStreamEndCapture(g.stream, &graph)) || !hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)))
       exec = nullptr;
