@@ -410,6 +410,43 @@ jgs2_status jgs2_batch_profile_end(jgs2_batch* batch, double* local_solve_ms_per
  * batch), for the launch, graph and autotune controls above. */
 jgs2_status jgs2_solver_batch(jgs2_solver* solver, jgs2_batch** out);
 
+/* Exact oracle diagnostics, readOracleDiagnostics / the opt-in evaluator
+ * (gpu/jgs2-diagnostics.ts:61-111, evaluator :712-1110; the reference reads
+ * them through jgs2-solver.ts:2620-2907), evaluated on the device in FP64
+ * at the current pose against the last predicted positions, or at caller
+ * poses (4 floats per vertex; NULL = the solver's own). Energy components
+ * of the implicit-Euler incremental potential: inertia 1/2 m/h^2 |x - x~|^2,
+ * elasticity (stable Neo-Hookean vol psi(F), or the co-rotated quadratic),
+ * external force -f.x, quadratic target, analytic floor; plus, with contact
+ * candidates, the barrier sum kappa b(d) (no counterpart in the reference's
+ * evaluator; cloth energies are in neither). */
+typedef struct jgs2_diagnostics {
+  double energy;                     /* sum of the six components */
+  double inertia, elasticity, external_force, quadratic_target, floor_contact, contact_barrier;
+  double gradient_norm_inertia, gradient_norm_material, gradient_norm_external_force, gradient_norm_target,
+      gradient_norm_floor;
+  double gradient_norm;              /* |g| over free vertices */
+  double residual_denominator;       /* sum of the five component norms */
+  double relative_residual;          /* |g| / max(1, denominator) */
+  double maximum_update;             /* largest valid final-iteration |delta| */
+  double minimum_deformation_determinant;
+  double minimum_floor_distance;     /* min signed y - floor over free vertices */
+  double minimum_contact_distance;   /* +inf without candidates */
+  uint32_t active_floor_contacts, active_vertices, active_contact_pairs;
+  uint32_t finite, maximum_update_valid, minimum_determinant_valid, floor_metrics_valid, reserved;
+} jgs2_diagnostics;
+/* vertex_records (optional, 36 doubles per vertex, caller order): gradient
+ * xyz, free flag, row-major 3x3 Hessian block, inertia energy, floor
+ * energy, final update, |g|^2, finite, inertia weight, update valid,
+ * |g_inertia|^2, |g_material|^2, |g_floor|^2, signed floor distance, floor
+ * active, external-force gradient xyz and energy, target gradient xyz and
+ * energy, |g_force|^2, |g_target|^2, 0. tet_records (optional, 4 per
+ * tetrahedron): elastic energy, det F, finite, 0. */
+jgs2_status jgs2_read_oracle_diagnostics(jgs2_solver* solver, const jgs2_settings* overrides, uint64_t override_mask,
+                                         const float* positions, const float* predicted, jgs2_diagnostics* out,
+                                         double* vertex_records, size_t vertex_record_count, double* tet_records,
+                                         size_t tet_record_count);
+
 /* ---------------------------------------------------------------------
  * Cubature precompute: buildPrecomputedScene (src/simulation/cpu/
  * precompute.ts:218-380) with its exact vertex basis (:135-200), nonlinear

@@ -108,6 +108,11 @@ struct DeviceState {
   uint32_t partial_capacity = 0;
   float* debug_g = nullptr;      // debug local systems
   float* debug_h = nullptr;
+  // Exact diagnostics scratch: caller poses (arena order) and FP64 records.
+  float4* diag_x = nullptr;
+  float4* diag_xp = nullptr;
+  double* diag_buf = nullptr;
+  size_t diag_doubles = 0;
   // Cloth layer.
   uint32_t NT = 0, NH = 0;
   float cloth_lam = 0, cloth_mu = 0, cloth_thick = 0, cloth_bend = 0;
@@ -240,6 +245,21 @@ void launch_body_correction(const DeviceState& s, const PassParams& p, hipStream
 void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_clear_frame(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_debug_local_systems(const DeviceState& s, const PassParams& p, hipStream_t stream);
+
+// Exact FP64 diagnostics of one scene (arena ranges [vb, vb + V), [tb, tb + T),
+// candidate slots [cb, cb + C)); records and summary in device doubles.
+struct DiagParams {
+  const float4* x;   // evaluated positions
+  const float4* xp;  // predicted positions
+  uint32_t vb, V, tb, T, cb, C, scene, objective_flags, corotated, pad;
+  double ih2, floor_y, floor_k, kappa, dhat, polar_eps;
+  double* vrec;      // 36 per vertex
+  double* trec;      // 4 per tetrahedron
+  double* trot;      // 9 per tetrahedron (co-rotated)
+  double* crec;      // 2 per candidate slot
+  double* summary;   // 32
+};
+void launch_diagnostics(const DeviceState& s, const DiagParams& q, hipStream_t stream);
 // IPC passes (encodeFrame's contact dispatches, jgs2-solver.ts:2231-2400).
 void launch_lag_contact(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_build_active_rows(const DeviceState& s, const PassParams& p, hipStream_t stream);

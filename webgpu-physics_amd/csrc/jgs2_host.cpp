@@ -214,7 +214,7 @@ void free_device(DeviceState& d) {
   void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_rec, d.vcub[0], d.vcub[1], d.tet_k, d.adjacency, d.cub, d.objectives,
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.conv, d.control,
-                  d.history, d.partials, d.debug_g, d.debug_h, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
+                  d.history, d.partials, d.debug_g, d.debug_h, d.diag_x, d.diag_xp, d.diag_buf, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
                   d.cscratch, d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
@@ -2432,6 +2432,118 @@ jgs2_status jgs2_debug_local_systems(jgs2_solver* s, const jgs2_settings* ov, ui
     std::memcpy(h + 9 * size_t(ov), &hs[9 * size_t(s->vnew[ov])], 9 * sizeof(float));
   }
   b->graph_valid = false;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_read_oracle_diagnostics(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask, const float* positions,
+                                         const float* predicted, jgs2_diagnostics* out, double* vertex_records,
+                                         size_t vertex_record_count, double* tet_records, size_t tet_record_count) {
+  TRYS(check_usable(s));
+  if (!out) return fail(JGS2_ERROR_TYPE, "readOracleDiagnostics needs a summary destination.");
+  if (vertex_records && vertex_record_count != 36 * size_t(s->V))
+    return fail(JGS2_ERROR_RANGE, "Oracle diagnostics vertex records need 36V doubles.");
+  if (tet_records && tet_record_count != 4 * size_t(s->T))
+    return fail(JGS2_ERROR_RANGE, "Oracle diagnostics tetrahedron records need 4T doubles.");
+  jgs2_batch* b = s->batch;
+  jgs2_settings r;
+  TRYS(resolve_batch_step(b, ov, mask, "readOracleDiagnostics", r));
+  if (!(r.floor_stiffness >= 0)) return fail(JGS2_ERROR_RANGE, "floorStiffness must be nonnegative.");
+  DeviceState& d = b->dev;
+  hipStream_t q = b->stream;
+  const size_t Vc = size_t(b->va.capacity) * kSceneBlock;
+  const size_t V = s->V, T = s->T, C = size_t(s->cblk.n) * kSceneBlock;
+  const size_t need = 36 * V + 4 * T + (s->corotated ? 9 * T : 0) + 2 * C + 32;
+  if (!d.diag_x) {
+    TRYS(dalloc(&d.diag_x, Vc));
+    TRYS(dalloc(&d.diag_xp, Vc));
+  }
+  if (d.diag_doubles < need) {
+    if (d.diag_buf) HIP_CHECK(hipFree(d.diag_buf));
+    d.diag_buf = nullptr;
+    d.diag_doubles = 0;
+    TRYS(dalloc(&d.diag_buf, need));
+    d.diag_doubles = need;
+  }
+  HIP_CHECK(hipStreamSynchronize(q));
+  auto stage = [&](const float* src, float4* dst) -> jgs2_status {
+    std::vector<float4> staged(size_t(s->vblk.n) * kSceneBlock, make_float4(0, 0, 0, 0));
+    for (uint32_t ov = 0; ov < s->V; ++ov)
+      staged[s->vnew[ov]] = make_float4(src[4 * ov], src[4 * ov + 1], src[4 * ov + 2], src[4 * ov + 3]);
+    HIP_CHECK(hipMemcpy(dst + s->vbase(), staged.data(), staged.size() * sizeof(float4), hipMemcpyHostToDevice));
+    return JGS2_OK;
+  };
+  if (positions) TRYS(stage(positions, d.diag_x));
+  if (predicted) TRYS(stage(predicted, d.diag_xp));
+  DiagParams p{};
+  p.x = positions ? d.diag_x : d.pos[0];
+  p.xp = predicted ? d.diag_xp : d.predicted;
+  p.vb = uint32_t(s->vbase());
+  p.V = uint32_t(V);
+  p.tb = uint32_t(s->tbase());
+  p.T = uint32_t(T);
+  p.cb = uint32_t(s->cbase());
+  p.C = s->NC || s->bp ? uint32_t(C) : 0u;
+  p.scene = s->slot;
+  p.objective_flags = s->objective_flags;
+  p.corotated = s->corotated ? 1u : 0u;
+  p.ih2 = 1.0 / (r.timestep * r.timestep);
+  p.floor_y = r.floor_height;
+  p.floor_k = r.floor_stiffness;
+  p.kappa = r.ipc_barrier_stiffness;
+  p.dhat = r.ipc_activation_distance;
+  p.polar_eps = r.rotation_epsilon;
+  p.vrec = d.diag_buf;
+  p.trec = p.vrec + 36 * V;
+  p.trot = p.trec + 4 * T;
+  p.crec = p.trot + (s->corotated ? 9 * T : 0);
+  p.summary = p.crec + 2 * C;
+  launch_diagnostics(d, p, q);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(q));
+  double m[32];
+  HIP_CHECK(hipMemcpy(m, p.summary, sizeof m, hipMemcpyDeviceToHost));
+  jgs2_diagnostics o{};
+  o.energy = m[0];
+  o.inertia = m[1];
+  o.elasticity = m[2];
+  o.external_force = m[3];
+  o.quadratic_target = m[4];
+  o.floor_contact = m[5];
+  o.contact_barrier = m[6];
+  o.gradient_norm_inertia = std::sqrt(std::max(m[7], 0.0));
+  o.gradient_norm_material = std::sqrt(std::max(m[8], 0.0));
+  o.gradient_norm_external_force = std::sqrt(std::max(m[9], 0.0));
+  o.gradient_norm_target = std::sqrt(std::max(m[10], 0.0));
+  o.gradient_norm_floor = std::sqrt(std::max(m[11], 0.0));
+  o.gradient_norm = std::sqrt(std::max(m[12], 0.0));
+  o.residual_denominator = o.gradient_norm_inertia + o.gradient_norm_material + o.gradient_norm_external_force +
+                           o.gradient_norm_target + o.gradient_norm_floor;
+  o.relative_residual = o.gradient_norm / std::max(1.0, o.residual_denominator);
+  o.maximum_update = m[14] > 0 ? m[13] : 0.0;
+  o.minimum_deformation_determinant = T ? m[15] : 0.0;
+  o.minimum_floor_distance = m[16];
+  o.minimum_contact_distance = m[20];
+  o.active_floor_contacts = uint32_t(m[17]);
+  o.active_vertices = uint32_t(m[18]);
+  o.active_contact_pairs = uint32_t(m[21]);
+  o.finite = m[19] == 0 && m[22] == 0 && std::isfinite(m[0]) ? 1u : 0u;
+  o.maximum_update_valid = m[14] > 0 ? 1u : 0u;
+  o.minimum_determinant_valid = T && std::isfinite(m[15]) ? 1u : 0u;
+  o.floor_metrics_valid = r.floor_stiffness > 0 ? 1u : 0u;
+  *out = o;
+  if (vertex_records) {
+    std::vector<double> all(36 * size_t(s->vblk.n) * kSceneBlock);
+    HIP_CHECK(hipMemcpy(all.data(), p.vrec, 36 * V * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t ovx = 0; ovx < s->V; ++ovx)
+      std::memcpy(vertex_records + 36 * size_t(ovx), &all[36 * size_t(s->vnew[ovx])], 36 * sizeof(double));
+  }
+  if (tet_records) {
+    std::vector<double> all(4 * T);
+    HIP_CHECK(hipMemcpy(all.data(), p.trec, 4 * T * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t ot = 0; ot < s->T; ++ot)
+      std::memcpy(tet_records + 4 * size_t(ot), &all[4 * size_t(s->tnew[ot])], 4 * sizeof(double));
+  }
   clear_error();
   return JGS2_OK;
 }

@@ -4226,4 +4226,381 @@ void launch_broad_phase(const DeviceState& s, const PassParams& p, bool check, b
   hipLaunchKernelGGL(k_bp_commit, dim3(s.bp_nv + s.bp_ne), B, 0, st, s, p);
 }
 
+
+// ------------------------------------------------------------------ exact diagnostics (FP64)
+// The reference's opt-in oracle diagnostics (gpu/jgs2-diagnostics.ts:
+// 712-1110), evaluated for one scene in double precision from the f32
+// state: per tetrahedron the elastic energy and det F; per vertex the
+// implicit-Euler gradient by component, the diagonal Hessian block and the
+// inertia, external-force, target and floor energies; the scene summary
+// (energy components, component gradient norms, force-balance residual,
+// largest final update, minimum det F, floor metrics). Contact scenes add
+// the barrier energy sum_c kappa b(d_c) over the scene's candidates (the
+// reference's evaluator has no contact term); cloth energies are not part
+// of it (nor of the reference's).
+namespace diag {
+
+struct d3 { double x, y, z; };
+struct dm3 { d3 c0, c1, c2; };  // columns
+__device__ __forceinline__ d3 dk(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ d3 dload(const float4* p, uint32_t v) { const float4 q = p[v]; return {q.x, q.y, q.z}; }
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 operator*(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 dcross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ d3 mv(const dm3& m, d3 v) { return v.x * m.c0 + v.y * m.c1 + v.z * m.c2; }
+__device__ __forceinline__ d3 mtv(const dm3& m, d3 v) { return {ddot(m.c0, v), ddot(m.c1, v), ddot(m.c2, v)}; }
+__device__ __forceinline__ dm3 mm(const dm3& a, const dm3& b) { return {mv(a, b.c0), mv(a, b.c1), mv(a, b.c2)}; }
+__device__ __forceinline__ double ddet(const dm3& m) { return ddot(m.c0, dcross(m.c1, m.c2)); }
+__device__ __forceinline__ double frob(const dm3& a, const dm3& b) { return ddot(a.c0, b.c0) + ddot(a.c1, b.c1) + ddot(a.c2, b.c2); }
+// d det / dF, columns f1 x f2, f2 x f0, f0 x f1
+__device__ __forceinline__ dm3 dcof(const dm3& F) { return {dcross(F.c1, F.c2), dcross(F.c2, F.c0), dcross(F.c0, F.c1)}; }
+// its directional derivative along dF
+__device__ __forceinline__ dm3 dcof_dir(const dm3& F, const dm3& H) {
+  return {dcross(H.c1, F.c2) + dcross(F.c1, H.c2), dcross(H.c2, F.c0) + dcross(F.c2, H.c0),
+          dcross(H.c0, F.c1) + dcross(F.c0, H.c1)};
+}
+__device__ __forceinline__ dm3 dscale(double s, const dm3& m) { return {s * m.c0, s * m.c1, s * m.c2}; }
+__device__ __forceinline__ dm3 dadd(const dm3& a, const dm3& b) { return {a.c0 + b.c0, a.c1 + b.c1, a.c2 + b.c2}; }
+__device__ __forceinline__ bool dfinite(double v) { return isfinite(v); }
+
+struct DTet {
+  uint4 ix;
+  dm3 inv;
+  double vol, lam, mu;
+};
+__device__ __forceinline__ DTet dtet(const DeviceState& s, uint32_t t) {
+  const Tet e = load_tet(s, t);
+  return {e.ix, {{e.inv.c0.x, e.inv.c0.y, e.inv.c0.z}, {e.inv.c1.x, e.inv.c1.y, e.inv.c1.z}, {e.inv.c2.x, e.inv.c2.y, e.inv.c2.z}},
+          e.vol, e.lam, e.mu};
+}
+__device__ __forceinline__ d3 dshape(const DTet& e, uint32_t slot) {  // rows of Dm^-1, and minus their sum
+  const d3 g1 = dk(e.inv.c0.x, e.inv.c1.x, e.inv.c2.x), g2 = dk(e.inv.c0.y, e.inv.c1.y, e.inv.c2.y),
+           g3 = dk(e.inv.c0.z, e.inv.c1.z, e.inv.c2.z);
+  if (slot == 1) return g1;
+  if (slot == 2) return g2;
+  if (slot == 3) return g3;
+  return dk(0, 0, 0) - g1 - g2 - g3;
+}
+__device__ __forceinline__ dm3 ddeform(const DTet& e, const float4* x) {
+  const d3 p0 = dload(x, e.ix.x);
+  const dm3 ds = {dload(x, e.ix.y) - p0, dload(x, e.ix.z) - p0, dload(x, e.ix.w) - p0};
+  return mm(ds, e.inv);
+}
+// Stable Neo-Hookean energy density and stress with the record's adjusted
+// lambda, mu (stable-neo-hookean-wgsl.ts:15-94).
+__device__ __forceinline__ double snh_psi(const dm3& F, double lam, double mu) {
+  const double I1 = frob(F, F), J = ddet(F);
+  return 0.5 * mu * (I1 - 3.0) - 0.75 * mu * (J - 1.0) + 0.5 * lam * (J - 1.0) * (J - 1.0) -
+         0.5 * mu * log((I1 + 1.0) / 4.0);
+}
+__device__ __forceinline__ dm3 snh_P(const dm3& F, double lam, double mu) {
+  const double I1 = frob(F, F), J = ddet(F);
+  return dadd(dscale(mu * (1.0 - 1.0 / (I1 + 1.0)), F), dscale(lam * (J - 1.0) - 0.75 * mu, dcof(F)));
+}
+__device__ __forceinline__ dm3 snh_dP(const dm3& F, const dm3& H, double lam, double mu) {
+  const double I1 = frob(F, F), J = ddet(F), q = I1 + 1.0;
+  const dm3 C = dcof(F);
+  dm3 r = dscale(mu * (1.0 - 1.0 / q), H);
+  r = dadd(r, dscale(2.0 * mu * frob(F, H) / (q * q), F));
+  r = dadd(r, dscale(lam * frob(C, H), C));
+  return dadd(r, dscale(lam * (J - 1.0) - 0.75 * mu, dcof_dir(F, H)));
+}
+__device__ __forceinline__ d3 dunit_or(d3 v, d3 fb, double eps) {
+  const double sq = ddot(v, v);
+  if (sq > eps * eps && sq < 1.0e60) return (1.0 / sqrt(sq)) * v;
+  return fb;
+}
+__device__ __forceinline__ d3 dperp(d3 a, double eps) {
+  const d3 c = fabs(a.x) > 0.8 ? dk(0, 1, 0) : dk(1, 0, 0);
+  return dunit_or(dcross(a, c), dk(0, 0, 1), eps);
+}
+// polarRotation (S:940-978) in double: seven averaging steps with the
+// inverse transpose, then the right-handed column orthonormalization.
+__device__ dm3 dpolar(dm3 r, double eps) {
+  for (int it = 0; it < 7; ++it) {
+    const double d = ddet(r);
+    if (!(fabs(d) > eps) || fabs(d) > 1.0e40) break;
+    r = dscale(0.5, dadd(r, dscale(1.0 / d, dcof(r))));
+  }
+  const d3 c0 = dunit_or(r.c0, dk(1, 0, 0), eps);
+  const d3 c1c = r.c1 - ddot(c0, r.c1) * c0;
+  d3 c1 = dunit_or(c1c, dperp(c0, eps), eps);
+  d3 c2 = dunit_or(dcross(c0, c1), dk(0, 0, 1), eps);
+  if (ddot(c2, r.c2) < 0.0) {
+    c1 = -1.0 * c1;
+    c2 = -1.0 * c2;
+  }
+  c1 = dunit_or(dcross(c2, c0), dperp(c0, eps), eps);
+  return {c0, c1, c2};
+}
+__device__ __forceinline__ dm3 dstiff(const DeviceState& s, uint32_t t, uint32_t rv, uint32_t cv) {
+  const float* k = s.tet_k + size_t(t) * 144;
+  const uint32_t r0 = rv * 3, c0 = cv * 3;
+  return {dk(k[r0 * 12 + c0], k[(r0 + 1) * 12 + c0], k[(r0 + 2) * 12 + c0]),
+          dk(k[r0 * 12 + c0 + 1], k[(r0 + 1) * 12 + c0 + 1], k[(r0 + 2) * 12 + c0 + 1]),
+          dk(k[r0 * 12 + c0 + 2], k[(r0 + 1) * 12 + c0 + 2], k[(r0 + 2) * 12 + c0 + 2])};
+}
+// Co-rotated displacements u_l = R^T (x_l - x_mean) - (X_l - X_mean).
+__device__ void corot_disp(const DeviceState& s, const DTet& e, const float4* x, const dm3& R, d3 u[4]) {
+  const uint32_t id[4] = {e.ix.x, e.ix.y, e.ix.z, e.ix.w};
+  d3 cur[4], rst[4], cc = dk(0, 0, 0), rc = dk(0, 0, 0);
+  for (int l = 0; l < 4; ++l) {
+    cur[l] = dload(x, id[l]);
+    rst[l] = dload(s.vtx_rest, id[l]);
+    cc = cc + cur[l];
+    rc = rc + rst[l];
+  }
+  cc = 0.25 * cc;
+  rc = 0.25 * rc;
+  for (int l = 0; l < 4; ++l) u[l] = mtv(R, cur[l] - cc) - (rst[l] - rc);
+}
+
+constexpr int kVertexRecord = 36;  // doubles per vertex record
+constexpr int kTetRecord = 4;      // energy, det F, finite, 0
+
+}  // namespace diag
+
+// One thread per tetrahedron of the scene: energy, det F and (co-rotated)
+// the frame the vertex pass rotates stiffness blocks with.
+__global__ void __launch_bounds__(kBlock) k_diag_tets(DeviceState s, DiagParams q) {
+  using namespace diag;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= q.T) return;
+  const uint32_t t = q.tb + i;
+  const DTet e = dtet(s, t);
+  const dm3 F = ddeform(e, q.x);
+  const double J = ddet(F);
+  double energy;
+  bool ok = dfinite(J);
+  if (q.corotated) {
+    const dm3 R = dpolar(F, q.polar_eps);
+    double* rr = q.trot + 9 * size_t(i);
+    rr[0] = R.c0.x; rr[1] = R.c0.y; rr[2] = R.c0.z; rr[3] = R.c1.x; rr[4] = R.c1.y; rr[5] = R.c1.z;
+    rr[6] = R.c2.x; rr[7] = R.c2.y; rr[8] = R.c2.z;
+    d3 u[4];
+    corot_disp(s, e, q.x, R, u);
+    energy = 0.0;
+    for (uint32_t r = 0; r < 4; ++r) {
+      d3 g = dk(0, 0, 0);
+      for (uint32_t c = 0; c < 4; ++c) g = g + mv(dstiff(s, t, r, c), u[c]);
+      energy += 0.5 * ddot(u[r], g);
+    }
+  } else {
+    energy = e.vol * snh_psi(F, e.lam, e.mu);
+  }
+  ok = ok && dfinite(energy);
+  double* rec = q.trec + kTetRecord * size_t(i);
+  rec[0] = ok ? energy : 0.0;
+  rec[1] = dfinite(J) ? J : 0.0;
+  rec[2] = ok ? 1.0 : 0.0;
+  rec[3] = 0.0;
+}
+
+// One thread per vertex of the scene (evaluateDiagnosticVertices, :791-950).
+__global__ void __launch_bounds__(kBlock) k_diag_vertices(DeviceState s, DiagParams q) {
+  using namespace diag;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= q.V) return;
+  const uint32_t v = q.vb + i;
+  const uint4 inf = s.vtx_info[v];
+  const d3 x = dload(q.x, v), xp = dload(q.xp, v);
+  const bool active = inf.z == 0u;
+  const float4 fu = s.final_update[v];
+  d3 g = dk(0, 0, 0), gi = dk(0, 0, 0), ge = dk(0, 0, 0), gf = dk(0, 0, 0), gt = dk(0, 0, 0), gfl = dk(0, 0, 0);
+  dm3 H = {dk(0, 0, 0), dk(0, 0, 0), dk(0, 0, 0)};
+  double Ei = 0.0, Ef = 0.0, Et = 0.0, Efl = 0.0, w = 0.0, upd = 0.0, upd_ok = 0.0, floor_on = 0.0;
+  const double sfd = x.y - q.floor_y;
+  bool ok = dfinite(x.x) && dfinite(x.y) && dfinite(x.z) && dfinite(xp.x) && dfinite(xp.y) && dfinite(xp.z);
+  if (active) {
+    w = double(s.vtx_rest[v].w) * q.ih2;
+    const d3 dx = x - xp;
+    gi = w * dx;
+    Ei = 0.5 * w * ddot(dx, dx);
+    H = {dk(w, 0, 0), dk(0, w, 0), dk(0, 0, w)};
+    upd = fu.x;
+    upd_ok = (fu.w >= 0.5f && fu.x >= 0.0f) ? 1.0 : 0.0;
+    if (q.objective_flags & kForce) {
+      const d3 f = dload(s.objectives, 2 * v);
+      gf = dk(0, 0, 0) - f;
+      Ef = -ddot(f, x);
+    }
+    if (q.objective_flags & kTarget) {
+      const float4 tr = s.objectives[2 * size_t(v) + 1];
+      if (tr.w > 0.0f) {
+        const d3 r = x - dk(tr.x, tr.y, tr.z);
+        gt = double(tr.w) * r;
+        Et = 0.5 * double(tr.w) * ddot(r, r);
+        H.c0.x += tr.w;
+        H.c1.y += tr.w;
+        H.c2.z += tr.w;
+      }
+    }
+    for (uint32_t k = 0; k < inf.y; ++k) {
+      const uint32_t t = s.adjacency[inf.x + k];
+      const DTet e = dtet(s, t);
+      const uint32_t slot = local_slot(e.ix, v);
+      if (slot >= 4u) {
+        ok = false;
+        continue;
+      }
+      const dm3 F = ddeform(e, q.x);
+      if (q.corotated) {
+        const double* rr = q.trot + 9 * size_t(t - q.tb);
+        const dm3 R = {dk(rr[0], rr[1], rr[2]), dk(rr[3], rr[4], rr[5]), dk(rr[6], rr[7], rr[8])};
+        d3 u[4];
+        corot_disp(s, e, q.x, R, u);
+        d3 lg = dk(0, 0, 0);
+        for (uint32_t c = 0; c < 4; ++c) lg = lg + mv(dstiff(s, t, slot, c), u[c]);
+        ge = ge + mv(R, lg);
+        const dm3 Kss = dstiff(s, t, slot, slot);
+        const dm3 RK = mm(R, Kss);
+        // R K_ss R^T
+        const dm3 Rt = {dk(R.c0.x, R.c1.x, R.c2.x), dk(R.c0.y, R.c1.y, R.c2.y), dk(R.c0.z, R.c1.z, R.c2.z)};
+        H = dadd(H, mm(RK, Rt));
+      } else {
+        const d3 gs = dshape(e, slot);
+        ge = ge + e.vol * mv(snh_P(F, e.lam, e.mu), gs);
+        // column c of the diagonal block: vol dP[e_c gs^T] gs
+        for (int c = 0; c < 3; ++c) {
+          dm3 dF = {dk(0, 0, 0), dk(0, 0, 0), dk(0, 0, 0)};
+          // row c of dF is gs: dF.cj[c] = gs_j
+          (c == 0 ? dF.c0.x : c == 1 ? dF.c0.y : dF.c0.z) = gs.x;
+          (c == 0 ? dF.c1.x : c == 1 ? dF.c1.y : dF.c1.z) = gs.y;
+          (c == 0 ? dF.c2.x : c == 1 ? dF.c2.y : dF.c2.z) = gs.z;
+          const d3 col = e.vol * mv(snh_dP(F, dF, e.lam, e.mu), gs);
+          if (c == 0) H.c0 = H.c0 + col;
+          else if (c == 1) H.c1 = H.c1 + col;
+          else H.c2 = H.c2 + col;
+        }
+      }
+    }
+    if (q.floor_k > 0.0 && sfd < 0.0) {
+      floor_on = 1.0;
+      Efl = 0.5 * q.floor_k * sfd * sfd;
+      gfl = dk(0, q.floor_k * sfd, 0);
+      H.c1.y += q.floor_k;
+    }
+    g = gi + ge + gf + gt + gfl;
+  }
+  double* r = q.vrec + kVertexRecord * size_t(i);
+  const double gg = ddot(g, g);
+  ok = ok && dfinite(gg) && dfinite(Ei) && dfinite(Ef) && dfinite(Et) && dfinite(Efl) && dfinite(H.c0.x + H.c1.y + H.c2.z);
+  r[0] = g.x; r[1] = g.y; r[2] = g.z; r[3] = active ? 1.0 : 0.0;
+  // row-major Hessian block
+  r[4] = H.c0.x; r[5] = H.c1.x; r[6] = H.c2.x;
+  r[7] = H.c0.y; r[8] = H.c1.y; r[9] = H.c2.y;
+  r[10] = H.c0.z; r[11] = H.c1.z; r[12] = H.c2.z;
+  r[13] = Ei; r[14] = Efl; r[15] = upd; r[16] = gg; r[17] = ok ? 1.0 : 0.0; r[18] = w; r[19] = upd_ok;
+  r[20] = ddot(gi, gi); r[21] = ddot(ge, ge); r[22] = ddot(gfl, gfl); r[23] = sfd; r[24] = floor_on;
+  r[25] = gf.x; r[26] = gf.y; r[27] = gf.z; r[28] = Ef;
+  r[29] = gt.x; r[30] = gt.y; r[31] = gt.z; r[32] = Et;
+  r[33] = ddot(gf, gf); r[34] = ddot(gt, gt); r[35] = 0.0;
+}
+
+// One thread per candidate slot of the scene: distance (f32 geometry, as the
+// step evaluates it) and barrier energy kappa b(d) in double.
+__global__ void __launch_bounds__(kBlock) k_diag_contact(DeviceState s, DiagParams q) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= q.C) return;
+  const uint32_t c = q.cb + i;
+  double* r = q.crec + 2 * size_t(i);
+  r[0] = 0.0;
+  r[1] = -1.0;  // not a live candidate
+  if (c >= s.scene_end[q.scene].w) return;
+  const Contact k = ipc_contact_at(s, c, q.x, q.dhat);
+  if (!k.valid || !finite_s(k.distance)) {
+    r[1] = -2.0;
+    return;
+  }
+  const double d = k.distance, dh = q.dhat;
+  r[1] = d;
+  if (d < dh && d > 0.0) {
+    const double o = d - dh;
+    r[0] = q.kappa * (-(o * o) * log(d / dh));
+  }
+}
+
+// Scene summary, one block: fixed-order strided sums and an LDS tree.
+// summary[]: 0 energy, 1 inertia, 2 elasticity, 3 external force, 4 target,
+// 5 floor, 6 contact barrier, 7-11 squared gradient norms (inertia,
+// material, external force, target, floor), 12 |g|^2, 13 max update,
+// 14 update valid count, 15 min det F, 16 min signed floor distance,
+// 17 active floor contacts, 18 active vertices, 19 nonfinite records,
+// 20 min contact distance, 21 active contact pairs, 22 invalid contacts.
+__global__ void __launch_bounds__(kBlock) k_diag_reduce(DiagParams q) {
+  using namespace diag;
+  __shared__ double red[kBlock];
+  double acc[23];
+  for (int k = 0; k < 23; ++k) acc[k] = 0.0;
+  acc[13] = -INFINITY;
+  acc[15] = INFINITY;
+  acc[16] = INFINITY;
+  acc[20] = INFINITY;
+  for (uint32_t i = threadIdx.x; i < q.V; i += kBlock) {
+    const double* r = q.vrec + kVertexRecord * size_t(i);
+    acc[19] += r[17] != 0.0 ? 0.0 : 1.0;
+    if (r[3] == 0.0) continue;
+    acc[1] += r[13];
+    acc[3] += r[28];
+    acc[4] += r[32];
+    acc[5] += r[14];
+    acc[7] += r[20];
+    acc[8] += r[21];
+    acc[9] += r[33];
+    acc[10] += r[34];
+    acc[11] += r[22];
+    acc[12] += r[16];
+    if (r[19] != 0.0) {
+      acc[13] = fmax(acc[13], r[15]);
+      acc[14] += 1.0;
+    }
+    acc[16] = fmin(acc[16], r[23]);
+    acc[17] += r[24];
+    acc[18] += 1.0;
+  }
+  for (uint32_t i = threadIdx.x; i < q.T; i += kBlock) {
+    const double* r = q.trec + kTetRecord * size_t(i);
+    acc[2] += r[0];
+    acc[15] = fmin(acc[15], r[1]);
+    acc[19] += r[2] != 0.0 ? 0.0 : 1.0;
+  }
+  for (uint32_t i = threadIdx.x; i < q.C; i += kBlock) {
+    const double* r = q.crec + 2 * size_t(i);
+    if (r[1] == -1.0) continue;
+    if (r[1] == -2.0) {
+      acc[22] += 1.0;
+      continue;
+    }
+    acc[6] += r[0];
+    acc[20] = fmin(acc[20], r[1]);
+    acc[21] += r[0] > 0.0 ? 1.0 : 0.0;
+  }
+  for (int k = 1; k < 23; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (uint32_t stride = kBlock / 2; stride > 0; stride /= 2) {
+      if (threadIdx.x < stride) {
+        const double a = red[threadIdx.x], b = red[threadIdx.x + stride];
+        red[threadIdx.x] = k == 13 ? fmax(a, b) : (k == 15 || k == 16 || k == 20) ? fmin(a, b) : a + b;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) q.summary[k] = red[0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double* m = q.summary;
+    m[0] = m[1] + m[2] + m[3] + m[4] + m[5] + m[6];
+  }
+}
+
+void launch_diagnostics(const DeviceState& s, const DiagParams& q, hipStream_t st) {
+  if (q.T) hipLaunchKernelGGL(k_diag_tets, dim3((q.T + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, q);
+  hipLaunchKernelGGL(k_diag_vertices, dim3((q.V + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, q);
+  if (q.C) hipLaunchKernelGGL(k_diag_contact, dim3((q.C + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, q);
+  hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(kBlock), 0, st, q);
+}
+
 }  // namespace jgs2

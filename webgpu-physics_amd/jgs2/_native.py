@@ -227,6 +227,17 @@ class BatchInfo(C.Structure):
     ]
 
 
+class Diagnostics(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "energy", "inertia", "elasticity", "external_force", "quadratic_target", "floor_contact", "contact_barrier",
+        "gradient_norm_inertia", "gradient_norm_material", "gradient_norm_external_force", "gradient_norm_target",
+        "gradient_norm_floor", "gradient_norm", "residual_denominator", "relative_residual", "maximum_update",
+        "minimum_deformation_determinant", "minimum_floor_distance", "minimum_contact_distance")] + [
+        (n, C.c_uint32) for n in ("active_floor_contacts", "active_vertices", "active_contact_pairs", "finite",
+                                  "maximum_update_valid", "minimum_determinant_valid", "floor_metrics_valid",
+                                  "reserved")]
+
+
 class PrecomputeMaterial(C.Structure):
     _fields_ = [
         ("density", C.c_double),
@@ -346,6 +357,9 @@ SIGNATURES = {
     "jgs2_build_adjacency": (C.c_int, [C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p]),
     "jgs2_greedy_vertex_coloring": (C.c_int, [C.c_uint32, u32p, C.c_uint32, u32p, u32p, u32p]),
     "jgs2_build_element_incidence": (C.c_int, [C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p, u32p]),
+    "jgs2_read_oracle_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(Settings), C.c_uint64, f32p, f32p,
+                                               C.POINTER(Diagnostics), C.POINTER(C.c_double), C.c_size_t,
+                                               C.POINTER(C.c_double), C.c_size_t]),
     "jgs2_precompute_cubature": (C.c_int, [C.POINTER(PrecomputeInput), C.POINTER(PrecomputeOptions),
                                            C.POINTER(PrecomputeOutput)]),
 }

@@ -435,6 +435,36 @@ class JGS2GpuSolver:
         N.check(N.lib.jgs2_debug_local_systems(self._h(), C.byref(s), mask, N.fptr(g), g.size, N.fptr(h), h.size))
         return g.reshape(-1, 3), h.reshape(-1, 3, 3)
 
+    def read_oracle_diagnostics(self, overrides: Optional[Mapping] = None, positions=None, predicted=None,
+                                records: bool = False):
+        """readOracleDiagnostics (gpu/jgs2-diagnostics.ts:61-111): the exact
+        FP64 implicit-Euler energy components, gradient norms and metrics at
+        the current pose (or `positions`, 4 floats per vertex) against the
+        last predicted positions (or `predicted`). With `records`, also the
+        per-vertex [V, 36] and per-tetrahedron [T, 4] records."""
+        s, mask = settings_struct(overrides)
+        out = N.Diagnostics()
+        keep = []
+
+        def pose(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float32).reshape(-1)
+            if a.size != 4 * self.vertex_count:
+                raise ValueError("poses need 4 floats per vertex")
+            keep.append(a)
+            return N.fptr(a)
+
+        vrec = np.empty((self.vertex_count, 36)) if records else None
+        trec = np.empty((self.tet_count, 4)) if records else None
+        dp = C.POINTER(C.c_double)
+        N.check(N.lib.jgs2_read_oracle_diagnostics(
+            self._h(), C.byref(s), mask, pose(positions), pose(predicted), C.byref(out),
+            vrec.ctypes.data_as(dp) if records else None, vrec.size if records else 0,
+            trec.ctypes.data_as(dp) if records and trec.size else None, trec.size if records else 0))
+        summary = {name: getattr(out, name) for name, _ in N.Diagnostics._fields_ if name != "reserved"}
+        return (summary, vrec, trec) if records else summary
+
     # -------------------------------------------------------------- profiling
     def profile_begin(self) -> None:
         N.check(N.lib.jgs2_profile_begin(self._h()))
