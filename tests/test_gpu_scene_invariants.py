@@ -32,7 +32,7 @@ import numpy as np
 import pytest
 
 import jgs2
-from jgs2 import scenes
+from jgs2 import manifests, scenes
 from ipc_geometry import candidate_distances
 import jgs2_oracle as O
 
@@ -53,14 +53,6 @@ def xyz(p):
     return np.asarray(p, np.float64).reshape(-1, 4)[:, :3]
 
 
-def rotate_xyz(p, r):  # rotateEulerXyz (src/simulation/cpu/mesh.ts:210-235): x, then y, then z
-    cx, sx, cy, sy, cz, sz = np.cos(r[0]), np.sin(r[0]), np.cos(r[1]), np.sin(r[1]), np.cos(r[2]), np.sin(r[2])
-    rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
-    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
-    rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
-    return p @ (rz @ ry @ rx).T
-
-
 def momenta(x, v, m):
     c = (m[:, None] * x).sum(0) / m.sum()
     return (m[:, None] * v).sum(0), (m[:, None] * np.cross(x - c, v)).sum(0)
@@ -78,22 +70,10 @@ def step(solver, frames, iterations, settings):
 
 def test_force_free_cuboid_momentum():
     f = fixture("phase0.force-free-cuboid")
-    g = f["generator"]["parameters"]
-    mat = f["materials"][0]
     print("case: force-free cuboid, 1200 frames", flush=True)
-    x = scenes.build_cuboid(scenes._desc(g["cells"], g["origin"], g["size"], pin_axis=3, density=mat["density"],
-                                         poisson=mat["poissonRatio"], young=(mat["youngModulus"],) * 2))
-    p0 = rotate_xyz(xyz(x.positions), np.array([0.17, -0.23, 0.31]))
-    v_lin, omega = np.array([0.37, -0.11, 0.23]), np.array([0.41, 0.29, -0.19])
-    c0 = p0.mean(0)
-    vel = v_lin + np.cross(omega, p0 - c0)
-    pos = np.concatenate([p0, np.ones((len(p0), 1))], 1).astype(np.float32)
-    x.positions = pos.reshape(-1)
-    x.velocities = np.concatenate([vel, np.zeros((len(vel), 1))], 1).astype(np.float32).reshape(-1)
-    sim, it = f["simulation"], f["solver"]["iterations"]
-    settings = {"timestep": sim["timestep"], "gravity": tuple(sim["gravity"]), "floor_stiffness": 0.0,
-                "parity_mode": sim["parityMode"], "velocity_damping": 1.0, "contact_tangential_damping": 0.0,
-                "horizontal_body_correction": False, "schedule": "jacobi"}
+    x, settings, it, _ = manifests.force_free_cuboid(MANIFEST)
+    p0 = xyz(x.positions)
+    vel = xyz(x.velocities)
     solver = jgs2.JGS2GpuSolver.create(x, settings)
     mass = x.vertex_rest.reshape(-1, 4)[:, 3].astype(np.float64)
     P0, L0 = momenta(p0, vel, mass)
