@@ -523,6 +523,7 @@ def _train(scene, materials, material_ids, rings):
     return {"rings": rings, "seconds": seconds, "vertices_per_s": float(free.sum()) / seconds,
             "phases_s": {k: round(float(v), 4) for k, v in r.timings.items() if k.endswith("_s")},
             "modes_on_gpu": bool(r.timings["gpu"]), "gpu_cg_ms": float(r.timings["gpu_cg_ms"]),
+            "selection_on_gpu": bool(r.timings["gpu_training"]), "gpu_select_ms": float(r.timings["gpu_select_ms"]),
             "gpu_cg_iterations": r.timings["gpu_cg_iterations"],
             "packed_residual_max": float(np.nanmax(r.residuals)), "packed_residual_mean": float(np.nanmean(r.residuals)),
             "records": int((r.tet_ids != 0xFFFFFFFF).sum()), "host_threads": _host_threads()}

@@ -487,7 +487,11 @@ typedef struct jgs2_precompute_options {
    * one is present, else on the host), 1: dense Cholesky, 2: host CG,
    * 3: GPU CG (JGS2_ERROR_DEVICE without a GPU) */
   uint32_t mode_solver;
-  uint32_t reserved;
+  /* Greedy selection of the training: 0 automatic (on the GPU for stable
+   * Neo-Hookean scenes of 256+ free vertices with ring bases, more than 12
+   * tetrahedra and K <= 8; else on the host threads), 1 host, 2 GPU. Same
+   * records either way. */
+  uint32_t training;
   double cg_tolerance;             /* relative residual of the CG mode solves; 0: 1e-12 */
   /* Largest packed training residual accepted per vertex; 0: the
    * reference's 0.01 (NONLINEAR_CUBATURE_TRAINING_RESIDUAL_TOLERANCE).
@@ -501,9 +505,10 @@ typedef struct jgs2_precompute_output {
   float* cubature_basis;           /* V*K*36: four row-major 3x3 rest-basis blocks per record */
   double* vertex_residuals;        /* optional, V: packed training residual (NaN when pinned) */
   uint32_t* vertex_ranks;          /* optional, 3 per vertex: nonzero candidates, column rank, packed rank */
-  /* optional, 6 values: seconds for the rest system, the modes and poses,
+  /* optional, 8 values: seconds for the rest system, the modes and poses,
    * the per-vertex training; GPU CG milliseconds and iterations; 1 when the
-   * GPU solved the modes */
+   * GPU solved the modes; GPU selection milliseconds; 1 when the GPU
+   * selected */
   double* timings;
 } jgs2_precompute_output;
 

@@ -102,3 +102,25 @@ def test_device_mode_solver_agrees_with_the_dense_factor():
           f"{g.timings['gpu_cg_ms']:.0f} ms)", flush=True)
     assert same >= 0.9
     np.testing.assert_allclose(h.residuals[free], g.residuals[free], rtol=0.05, atol=1e-3)
+
+
+@pytest.mark.parametrize("rings,two_materials", [(1, False), (2, True)])
+def test_device_selection_equals_the_host_selection(rings, two_materials):
+    from jgs2.precompute import precompute_cubature
+    from test_precompute_oracle import cuboid
+    print(f"case: device greedy selection vs host, 10x6x6, rings {rings}, two materials {two_materials}", flush=True)
+    pos, tets, fixed = cuboid((10, 6, 6), (0, 0, 0), (2.5, 1.5, 1.5), lambda p, c: c[0] == 0)
+    mid = ((np.arange(len(tets)) // 48) % 2) if two_materials else np.zeros(len(tets))
+    mats = [(1000, 1e5, 0.4, 1), (1000, 1e6, 0.45, 1)]
+    args = (pos, tets, fixed, mid, mats, 1 / 60)
+    host = precompute_cubature(*args, 6, rings=rings, residual_bound=1.0, training="host")
+    dev = precompute_cubature(*args, 6, rings=rings, residual_bound=1.0, training="gpu")
+    assert dev.timings["gpu_training"] and not host.timings["gpu_training"]
+    same = np.mean([np.array_equal(host.tet_ids[v], dev.tet_ids[v]) for v in range(len(pos))])
+    print(f"V={len(pos)}: identical selections {same:.4f}; host training {host.timings['training_s']:.2f} s, "
+          f"device {dev.timings['training_s']:.2f} s (selection kernel {dev.timings['gpu_select_ms']:.0f} ms)",
+          flush=True)
+    assert np.array_equal(host.tet_ids, dev.tet_ids)
+    assert np.array_equal(host.weights, dev.weights)
+    assert np.array_equal(host.basis, dev.basis)
+    assert np.array_equal(host.residuals, dev.residuals, equal_nan=True)

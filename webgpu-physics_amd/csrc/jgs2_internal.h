@@ -35,6 +35,25 @@ class GpuCg {
   Impl* impl_;
 };
 
+// Greedy nonnegative column selection of the cubature training on the GPU
+// (jgs2_precompute_kernels.hip), one job per source vertex: `count` columns
+// of `rows` doubles at columns[col_offset], the target at
+// targets[tgt_offset], its norm, and (1 + K) * count doubles of device
+// scratch at scratch_offset. Outputs per job: K candidate indices (-1
+// unused), K weights, the final squared residual and the number selected.
+struct GpuSelectJob {
+  uint64_t col_offset;
+  uint64_t tgt_offset;
+  uint64_t scratch_offset;
+  uint32_t count;
+  uint32_t pad;
+  double target_norm;
+};
+jgs2_status gpu_greedy_select(const double* columns, size_t column_doubles, const double* targets,
+                              size_t target_doubles, const GpuSelectJob* jobs, uint32_t job_count, uint32_t rows,
+                              uint32_t K, double tol, uint32_t passes, int32_t* out_sel, double* out_w,
+                              double* out_cur, uint32_t* out_n, double* milliseconds);
+
 // Records the calling thread's last error message and returns `status`.
 jgs2_status fail(jgs2_status status, const char* format, ...);
 void clear_error();

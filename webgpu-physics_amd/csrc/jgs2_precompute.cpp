@@ -1356,25 +1356,56 @@ void snh_columns(const Precompute& p, const std::vector<Pose>& poses, const std:
   }
 }
 
-VertexResult train_vertex_snh(const Precompute& p, const std::vector<Pose>& poses, const std::vector<Vec>& frames,
-                              uint32_t src, Support& s) {
-  solve_basis(p, src, s);
+// Everything of one source vertex's stable Neo-Hookean training except the
+// selection: basis, candidate columns and target, the f32-packed columns,
+// and their ranks.
+struct Prepared {
   Columns C;
+  Vec packed;                   // packed columns, C.rows per candidate
+  Vec norms;
+  int valid = 0;
+  double tn = 0.0;
+  uint32_t nonzero = 0, rank = 0, packed_rank = 0;
+  std::vector<uint32_t> verts;  // basis support, ascending
+  Vec U;                        // 9 per support vertex
+};
+
+void prepare_vertex_snh(const Precompute& p, const std::vector<Pose>& poses, const std::vector<Vec>& frames,
+                        uint32_t src, Support& s, Prepared& P) {
+  solve_basis(p, src, s);
+  Columns& C = P.C;
   C.tet = support_tets(p, s);
   if (p.T <= 12 && p.K < p.T) {  // the subset audit needs every column
     C.tet.resize(p.T);
     std::iota(C.tet.begin(), C.tet.end(), 0u);
   }
-  Vec norms;
-  int valid = 0;
-  snh_columns(p, poses, frames, src, s, s.U, nullptr, C, norms, valid);
-  const double tn = std::sqrt(dotn(C.target.data(), C.target.data(), C.rows));
-  if (tn <= 1e-12) raise(JGS2_ERROR_INFEASIBLE, "Nonlinear Cubature source vertex has no valid nontrivial training target.");
+  snh_columns(p, poses, frames, src, s, s.U, nullptr, C, P.norms, P.valid);
+  P.tn = std::sqrt(dotn(C.target.data(), C.target.data(), C.rows));
+  if (P.tn <= 1e-12)
+    raise(JGS2_ERROR_INFEASIBLE, "Nonlinear Cubature source vertex has no valid nontrivial training target.");
+  column_rank(C, P.nonzero, P.rank);
+  Vec Uf(s.U.size());
+  for (size_t i = 0; i < Uf.size(); ++i) Uf[i] = double(float(s.U[i]));
+  Columns Pk;
+  Pk.tet = C.tet;
+  Vec unused;
+  int pvalid = 0;
+  snh_columns(p, poses, frames, src, s, Uf, &P.norms, Pk, unused, pvalid);
+  if (pvalid != P.valid) raise(JGS2_ERROR_INFEASIBLE, "Packing changed nonlinear Cubature pose validity.");
+  uint32_t pnz = 0;
+  column_rank(Pk, pnz, P.packed_rank);
+  P.packed = std::move(Pk.col);
+  P.verts = s.verts;
+  P.U = s.U;
+}
+
+// Greedy selection and, for the reference's tiny fixtures, the subset audit
+// (training.ts:761-808) on the host.
+Chosen select_host(const Precompute& p, Prepared& P) {
+  Columns& C = P.C;
   C.prepare();
   Chosen ch = greedy_select(C, p.K, 0.01, 12);
-  VertexResult out;
-  column_rank(C, out.nonzero, out.rank);
-  if (p.T <= 12 && p.K < p.T) {  // training.ts:775-808 over candidate (= tetrahedron) order
+  if (p.T <= 12 && p.K < p.T) {  // over candidate (= tetrahedron) order
     Vec scratch(C.rows);
     std::vector<size_t> sub;
     std::function<void(size_t)> visit = [&](size_t next) {
@@ -1382,7 +1413,7 @@ VertexResult train_vertex_snh(const Precompute& p, const std::vector<Pose>& pose
         double w[12], e;
         nnls(C, sub.data(), int(sub.size()), -1, nullptr, w, e, scratch.data());
         const double rn = std::sqrt(residual_sq_of(C, sub, w, scratch.data()));
-        const double nr = tn > 1e-12 ? rn / tn : 0.0;
+        const double nr = P.tn > 1e-12 ? rn / P.tn : 0.0;
         if (nr < ch.normalized - 1e-14) {
           ch.normalized = nr;
           ch.sel = sub;
@@ -1398,33 +1429,27 @@ VertexResult train_vertex_snh(const Precompute& p, const std::vector<Pose>& pose
     };
     visit(0);
   }
-  std::vector<size_t> keep;
-  std::vector<double> kw;
-  for (size_t i = 0; i < ch.sel.size(); ++i)
-    if (ch.w[i] > 1e-10) {
-      keep.push_back(ch.sel[i]);
-      kw.push_back(ch.w[i]);
-    }
-  // f32 packing and the packed residual against the f64 target
-  Vec Uf(s.U.size());
-  for (size_t i = 0; i < Uf.size(); ++i) Uf[i] = double(float(s.U[i]));
-  Columns Pk;
-  Pk.tet = C.tet;
-  Vec unused;
-  int pvalid = 0;
-  snh_columns(p, poses, frames, src, s, Uf, &norms, Pk, unused, pvalid);
-  if (pvalid != valid) raise(JGS2_ERROR_INFEASIBLE, "Packing changed nonlinear Cubature pose validity.");
-  uint32_t pnz = 0;
-  column_rank(Pk, pnz, out.packed_rank);
+  return ch;
+}
+
+// Positive weights, f32 packing, the packed residual against the f64 target
+// and the reference's bound (training.ts:810-843, precompute.ts:313-323).
+VertexResult finish_vertex_snh(const Precompute& p, const Prepared& P, const Chosen& ch, uint32_t src) {
+  VertexResult out;
+  out.nonzero = P.nonzero;
+  out.rank = P.rank;
+  out.packed_rank = P.packed_rank;
+  const Columns& C = P.C;
   Vec res = C.target;
-  for (size_t i = 0; i < keep.size(); ++i) {
-    const float wf = float(kw[i]);
-    out.tets.push_back(C.tet[keep[i]]);
+  for (size_t i = 0; i < ch.sel.size(); ++i) {
+    if (!(ch.w[i] > 1e-10)) continue;
+    const float wf = float(ch.w[i]);
+    out.tets.push_back(C.tet[ch.sel[i]]);
     out.w.push_back(wf);
-    const double* col = Pk.c(keep[i]);
+    const double* col = &P.packed[ch.sel[i] * C.rows];
     for (size_t r = 0; r < res.size(); ++r) res[r] += -double(wf) * col[r];
   }
-  out.residual = tn > 1e-12 ? std::sqrt(dotn(res.data(), res.data(), res.size())) / tn : 0.0;
+  out.residual = P.tn > 1e-12 ? std::sqrt(dotn(res.data(), res.data(), res.size())) / P.tn : 0.0;
   if (out.residual > p.residual_bound + 1e-12) {
     char msg[240];
     std::snprintf(msg, sizeof msg,
@@ -1434,10 +1459,21 @@ VertexResult train_vertex_snh(const Precompute& p, const std::vector<Pose>& pose
   }
   for (uint32_t t : out.tets)
     for (int l = 0; l < 4; ++l) {
-      const int32_t li = s.local[p.tets[4 * size_t(t) + l]];
-      for (int i = 0; i < 9; ++i) out.basis.push_back(li < 0 ? 0.0f : float(s.U[9 * size_t(li) + size_t(i)]));
+      const uint32_t v = p.tets[4 * size_t(t) + l];
+      const auto it = std::lower_bound(P.verts.begin(), P.verts.end(), v);
+      const bool in = it != P.verts.end() && *it == v;
+      const size_t li = size_t(it - P.verts.begin());
+      for (int i = 0; i < 9; ++i) out.basis.push_back(in ? float(P.U[9 * li + size_t(i)]) : 0.0f);
     }
   return out;
+}
+
+VertexResult train_vertex_snh(const Precompute& p, const std::vector<Pose>& poses, const std::vector<Vec>& frames,
+                              uint32_t src, Support& s) {
+  Prepared P;
+  prepare_vertex_snh(p, poses, frames, src, s, P);
+  const Chosen ch = select_host(p, P);
+  return finish_vertex_snh(p, P, ch, src);
 }
 
 // Co-rotated linear training over displacement modes (cubature.ts:312-369, 714-826).
@@ -1562,19 +1598,7 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
     }
     const auto t2 = clock::now();
     std::vector<Support> supports(p.threads);
-    parallel_for(p.active.size(), p.threads, [&](size_t a, unsigned worker) {
-      const uint32_t v = p.active[a];
-      Support& s = supports[worker];
-      if (s.local.size() != p.V) s.local.assign(p.V, -1);
-      VertexResult r;
-      if (p.snh && p.incident[v] == 0) {
-        r.residual = 0.0;
-      } else if (p.snh) {
-        r = train_vertex_snh(p, poses, frames, v, s);
-      } else {
-        r = train_vertex_linear(p, modes, v, s);
-      }
-      for (uint32_t u : s.verts) s.local[u] = -1;
+    auto store = [&](uint32_t v, const VertexResult& r) {
       for (size_t k = 0; k < r.tets.size(); ++k) {
         output->cubature_tet_ids[v * K + k] = r.tets[k];
         output->cubature_weights[v * K + k] = r.w[k];
@@ -1586,10 +1610,90 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
         output->vertex_ranks[3 * size_t(v) + 1] = r.rank;
         output->vertex_ranks[3 * size_t(v) + 2] = r.packed_rank;
       }
-    });
+    };
+    // Selection on the GPU: stable Neo-Hookean, past the reference's tiny
+    // fixtures (no subset audit), at most 8 samples.
+    const bool device_ok = p.snh && p.T > 12 && p.K <= 8;
+    if (opt.training > 2) raise(JGS2_ERROR_RANGE, "training must be 0 (automatic), 1 (host) or 2 (GPU).");
+    if (opt.training == 2 && !device_ok)
+      raise(JGS2_ERROR_RANGE, "GPU training needs a stable Neo-Hookean scene of more than 12 tetrahedra and K <= 8.");
+    if (opt.training == 2 && !GpuCg::available()) raise(JGS2_ERROR_DEVICE, "GPU training needs a HIP device.");
+    const bool device = device_ok && (opt.training == 2 || (opt.training == 0 && p.rings > 0 &&
+                                                             p.active.size() >= 256 && GpuCg::available()));
+    double select_ms = 0.0;
+    if (device) {
+      std::vector<uint32_t> list;
+      for (uint32_t v : p.active) {
+        if (p.incident[v] == 0) {
+          VertexResult r;
+          store(v, r);
+        } else {
+          list.push_back(v);
+        }
+      }
+      const size_t chunk = 4096;
+      for (size_t first = 0; first < list.size(); first += chunk) {
+        const size_t n = std::min(chunk, list.size() - first);
+        std::vector<Prepared> prep(n);
+        parallel_for(n, p.threads, [&](size_t i, unsigned worker) {
+          Support& s = supports[worker];
+          if (s.local.size() != p.V) s.local.assign(p.V, -1);
+          prepare_vertex_snh(p, poses, frames, list[first + i], s, prep[i]);
+          for (uint32_t u : s.verts) s.local[u] = -1;
+        });
+        const uint32_t rows = uint32_t(prep[0].C.rows);
+        std::vector<GpuSelectJob> jobs(n);
+        size_t col_total = 0, scratch_total = 0;
+        for (size_t i = 0; i < n; ++i) {
+          const uint32_t cnt = uint32_t(prep[i].C.tet.size());
+          jobs[i] = GpuSelectJob{col_total, uint64_t(i) * rows, scratch_total, cnt, 0u, prep[i].tn};
+          col_total += size_t(cnt) * rows;
+          scratch_total += size_t(cnt) * (1 + K);
+        }
+        Vec cols(col_total), tgts(n * rows);
+        parallel_for(n, p.threads, [&](size_t i, unsigned) {
+          std::memcpy(&cols[jobs[i].col_offset], prep[i].C.col.data(), prep[i].C.col.size() * sizeof(double));
+          std::memcpy(&tgts[i * rows], prep[i].C.target.data(), rows * sizeof(double));
+        });
+        std::vector<int32_t> sel(n * K);
+        Vec w(n * K), cur(n);
+        std::vector<uint32_t> cnt(n);
+        double ms = 0.0;
+        const jgs2_status st = gpu_greedy_select(cols.data(), cols.size(), tgts.data(), tgts.size(), jobs.data(),
+                                                 uint32_t(n), rows, uint32_t(K), 0.01, 12, sel.data(), w.data(),
+                                                 cur.data(), cnt.data(), &ms);
+        if (st != JGS2_OK) raise(st, "Cubature precompute: the device selection failed.");
+        select_ms += ms;
+        parallel_for(n, p.threads, [&](size_t i, unsigned) {
+          Chosen ch;
+          for (uint32_t k = 0; k < cnt[i]; ++k) {
+            ch.sel.push_back(size_t(sel[i * K + k]));
+            ch.w.push_back(w[i * K + k]);
+          }
+          ch.normalized = std::sqrt(cur[i]) / prep[i].tn;
+          store(list[first + i], finish_vertex_snh(p, prep[i], ch, list[first + i]));
+        });
+      }
+    } else {
+      parallel_for(p.active.size(), p.threads, [&](size_t a, unsigned worker) {
+        const uint32_t v = p.active[a];
+        Support& s = supports[worker];
+        if (s.local.size() != p.V) s.local.assign(p.V, -1);
+        VertexResult r;
+        if (p.snh && p.incident[v] == 0) {
+          r.residual = 0.0;
+        } else if (p.snh) {
+          r = train_vertex_snh(p, poses, frames, v, s);
+        } else {
+          r = train_vertex_linear(p, modes, v, s);
+        }
+        for (uint32_t u : s.verts) s.local[u] = -1;
+        store(v, r);
+      });
+    }
     if (output->timings) {
-      const double t[6] = {seconds(t0, t1), seconds(t1, t2), seconds(t2, clock::now()), p.gpu_ms,
-                           double(p.gpu_iterations), p.gpu ? 1.0 : 0.0};
+      const double t[8] = {seconds(t0, t1), seconds(t1, t2), seconds(t2, clock::now()), p.gpu_ms,
+                           double(p.gpu_iterations), p.gpu ? 1.0 : 0.0, select_ms, device ? 1.0 : 0.0};
       std::memcpy(output->timings, t, sizeof t);
     }
     return JGS2_OK;

@@ -365,4 +365,393 @@ jgs2_status GpuCg::solve(double* b, uint32_t m, double tol, uint32_t max_iterati
   return JGS2_OK;
 }
 
+
+// ------------------------------------------------------------ greedy NNLS selection
+// The cubature training's greedy nonnegative column selection
+// (src/simulation/cpu/cubature.ts:386-691) for many source vertices at once:
+// one wave per vertex, lanes over candidate columns. Every comparison is the
+// reference's: the NNLS masks run in order per candidate, and one lane scans
+// the candidates in order with the 1e-14 relative tolerance and the
+// lowest-index tie rule. Floating-point contraction is off and every sum runs
+// in the host's order, so the selections are the host's bit for bit.
+namespace {
+
+constexpr int kSelK = 8;         // samples per vertex on the device
+constexpr int kSelMasks = 1 << kSelK;
+
+struct MaskResult {
+  double err;
+  double w[kSelK];
+  int ok;
+};
+
+__device__ double sel_dot(const double* a, const double* b, uint32_t rows) {
+#pragma clang fp contract(off)
+  double r = 0.0;
+  for (uint32_t i = 0; i < rows; ++i) r += a[i] * b[i];
+  return r;
+}
+
+// Gaussian elimination with partial pivoting (math.ts:134-199), n <= kSelK.
+__device__ bool sel_solve(double* a, double* b, int n) {
+#pragma clang fp contract(off)
+  for (int p = 0; p < n; ++p) {
+    int best = p;
+    double mag = fabs(a[p * n + p]);
+    for (int r = p + 1; r < n; ++r)
+      if (fabs(a[r * n + p]) > mag) {
+        mag = fabs(a[r * n + p]);
+        best = r;
+      }
+    if (mag <= 1e-14 || !isfinite(mag)) return false;
+    if (best != p) {
+      for (int c = p; c < n; ++c) {
+        const double t = a[p * n + c];
+        a[p * n + c] = a[best * n + c];
+        a[best * n + c] = t;
+      }
+      const double t = b[p];
+      b[p] = b[best];
+      b[best] = t;
+    }
+    const double d = a[p * n + p];
+    for (int r = p + 1; r < n; ++r) {
+      const double m = a[r * n + p] / d;
+      a[r * n + p] = 0.0;
+      for (int c = p + 1; c < n; ++c) a[r * n + c] -= m * a[p * n + c];
+      b[r] -= m * b[p];
+    }
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double v = b[r];
+    for (int c = r + 1; c < n; ++c) v -= a[r * n + c] * b[c];
+    b[r] = v / a[r * n + r];
+  }
+  return true;
+}
+
+// One mask of a trial set of n columns `col[i]` with Gram entries g(i, j)
+// and target dots td[i]: least squares over the mask, rejected when singular
+// or a weight is below -1e-10; the error is |t - sum w_i col_i|^2 formed
+// row by row in column order.
+template <class G>
+__device__ MaskResult sel_fit(const double* const* col, const double* td, G g, int n, unsigned mask, const double* t,
+                              uint32_t rows) {
+#pragma clang fp contract(off)
+  MaskResult f;
+  f.ok = 0;
+  f.err = 0.0;
+  for (int i = 0; i < kSelK; ++i) f.w[i] = 0.0;
+  int act[kSelK], d = 0;
+  for (int i = 0; i < n; ++i)
+    if (mask & (1u << i)) act[d++] = i;
+  double A[kSelK * kSelK], b[kSelK];
+  for (int r = 0; r < d; ++r) {
+    b[r] = td[act[r]];
+    for (int c = 0; c < d; ++c) A[r * d + c] = g(act[r], act[c]);
+  }
+  if (!sel_solve(A, b, d)) return f;
+  for (int i = 0; i < d; ++i)
+    if (b[i] < -1e-10) return f;
+  for (int i = 0; i < d; ++i) f.w[act[i]] = fmax(0.0, b[i]);
+  double err = 0.0;
+  for (uint32_t r = 0; r < rows; ++r) {
+    double v = t[r];
+    for (int i = 0; i < n; ++i)
+      if (f.w[i] != 0.0) v -= col[i][r] * f.w[i];
+    err += v * v;
+  }
+  f.err = err;
+  f.ok = 1;
+  return f;
+}
+
+__device__ double sel_residual_sq(const double* const* col, const double* w, int n, const double* t, uint32_t rows) {
+#pragma clang fp contract(off)
+  double err = 0.0;
+  for (uint32_t r = 0; r < rows; ++r) {
+    double v = t[r];
+    for (int i = 0; i < n; ++i) v -= col[i][r] * w[i];
+    err += v * v;
+  }
+  return err;
+}
+
+struct SelState {
+  int sel[kSelK];
+  double w[kSelK];
+  int n;
+  double cur;
+  int done;
+  // scan results
+  int best, best_slot;
+  double be, bw[kSelK];
+};
+
+__global__ void __launch_bounds__(64)
+k_greedy_select(const double* __restrict__ cols, const double* __restrict__ tgts, const GpuSelectJob* __restrict__ jobs,
+                uint32_t rows, uint32_t K, double tol, uint32_t passes, double* __restrict__ scratch,
+                int32_t* __restrict__ out_sel, double* __restrict__ out_w, double* __restrict__ out_cur,
+                uint32_t* __restrict__ out_n) {
+#pragma clang fp contract(off)
+  __shared__ MaskResult tab[kSelMasks];
+  __shared__ double gsel[kSelK * kSelK], tdsel[kSelK];
+  __shared__ SelState st;
+  const GpuSelectJob job = jobs[blockIdx.x];
+  const uint32_t lane = threadIdx.x;
+  const double* C = cols + job.col_offset;
+  const double* t = tgts + job.tgt_offset;
+  const uint32_t count = job.count;
+  double* esc = scratch + job.scratch_offset;  // per candidate: 1 + K doubles
+  // usable candidates: squared norm above 1e-18
+  uint32_t usable = 0;
+  for (uint32_t c = lane; c < count; c += 64) {
+    const bool u = sel_dot(C + size_t(c) * rows, C + size_t(c) * rows, rows) > 1e-18;
+    esc[size_t(c) * (1 + K)] = u ? 0.0 : -1.0;  // -1 marks an unusable column
+    usable += u ? 1u : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) usable += __shfl_down(usable, off, 64);
+  usable = __shfl(usable, 0, 64);
+  const double tt = sel_dot(t, t, rows);
+  if (lane == 0) {
+    st.n = 0;
+    st.cur = tt;
+    st.done = 0;
+  }
+  __syncthreads();
+  const uint32_t limit = usable < K ? usable : K;
+  const double tn = job.target_norm;
+  auto selected = [&](uint32_t c) {
+    for (int i = 0; i < st.n; ++i)
+      if (st.sel[i] == int(c)) return true;
+    return false;
+  };
+  // Gram of the selected columns and their target dots
+  auto refresh_gram = [&]() {
+    for (int e = int(lane); e < st.n * st.n; e += 64) {
+      const int i = e / st.n, j = e % st.n;
+      gsel[i * kSelK + j] = sel_dot(C + size_t(st.sel[i]) * rows, C + size_t(st.sel[j]) * rows, rows);
+    }
+    for (int i = int(lane); i < st.n; i += 64) tdsel[i] = sel_dot(C + size_t(st.sel[i]) * rows, t, rows);
+    __syncthreads();
+  };
+  if (tn > 1e-12 && limit > 0) {
+    for (uint32_t step = 0; step < limit; ++step) {
+      refresh_gram();
+      const int n = st.n + 1;
+      // masks without the new (last) position, shared by every candidate
+      const double* cs[kSelK];
+      for (int i = 0; i < st.n; ++i) cs[i] = C + size_t(st.sel[i]) * rows;
+      for (unsigned mask = lane + 1; mask < (1u << (n - 1)); mask += 64)
+        tab[mask] = sel_fit(cs, tdsel, [&](int i, int j) { return gsel[i * kSelK + j]; }, n, mask, t, rows);
+      __syncthreads();
+      for (uint32_t c = lane; c < count; c += 64) {
+        double* ec = esc + size_t(c) * (1 + K);
+        if (ec[0] < 0.0 || selected(c)) continue;
+        const double* colc = C + size_t(c) * rows;
+        cs[n - 1] = colc;
+        double gc[kSelK];
+        for (int i = 0; i < n - 1; ++i) gc[i] = sel_dot(colc, cs[i], rows);
+        gc[n - 1] = sel_dot(colc, colc, rows);
+        double td[kSelK];
+        for (int i = 0; i < n - 1; ++i) td[i] = tdsel[i];
+        td[n - 1] = sel_dot(colc, t, rows);
+        auto g = [&](int i, int j) {
+          if (i == n - 1) return gc[j];
+          if (j == n - 1) return gc[i];
+          return gsel[i * kSelK + j];
+        };
+        double best = tt, bw[kSelK];
+        for (int i = 0; i < kSelK; ++i) bw[i] = 0.0;
+        for (unsigned mask = 1; mask < (1u << n); ++mask) {
+          const MaskResult f = (mask & (1u << (n - 1))) ? sel_fit(cs, td, g, n, mask, t, rows) : tab[mask];
+          if (!f.ok) continue;
+          if (f.err < best - 1e-14 * fmax(1.0, best)) {
+            best = f.err;
+            for (int i = 0; i < kSelK; ++i) bw[i] = f.w[i];
+          }
+        }
+        ec[0] = sel_residual_sq(cs, bw, n, t, rows);
+        for (int i = 0; i < n; ++i) ec[1 + i] = bw[i];
+      }
+      __syncthreads();
+      if (lane == 0) {
+        int best = -1;
+        double be = st.cur;
+        for (uint32_t c = 0; c < count; ++c) {
+          const double* ec = esc + size_t(c) * (1 + K);
+          if (ec[0] < 0.0 || selected(c)) continue;
+          const double ex = ec[0], tl = 1e-14 * fmax(1.0, be);
+          if (ex < be - tl || (fabs(ex - be) <= tl && best >= 0 && int(c) < best)) {
+            best = int(c);
+            be = ex;
+          }
+        }
+        if (best < 0) {
+          st.done = 1;
+        } else {
+          const double* ec = esc + size_t(best) * (1 + K);
+          st.sel[st.n] = best;
+          for (int i = 0; i < n; ++i) st.w[i] = ec[1 + i];
+          st.n = n;
+          st.cur = be;
+          if (sqrt(be) / tn < tol) st.done = 1;
+        }
+      }
+      __syncthreads();
+      if (st.done) break;
+      // the marks of usable columns were overwritten by errors: restore them
+      for (uint32_t c = lane; c < count; c += 64) {
+        double* ec = esc + size_t(c) * (1 + K);
+        if (ec[0] >= 0.0) ec[0] = 0.0;
+      }
+      __syncthreads();
+    }
+    // one-for-one replacement passes
+    for (uint32_t pass = 0; pass < passes && st.n > 0; ++pass) {
+      for (uint32_t c = lane; c < count; c += 64) {
+        double* ec = esc + size_t(c) * (1 + K);
+        if (ec[0] >= 0.0) ec[0] = 0.0;
+      }
+      refresh_gram();
+      const int n = st.n;
+      if (lane == 0) {
+        st.best = -1;
+        st.best_slot = -1;
+        st.be = st.cur;
+      }
+      __syncthreads();
+      for (int sl = 0; sl < n; ++sl) {
+        const double* cs[kSelK];
+        for (int i = 0; i < n; ++i) cs[i] = C + size_t(st.sel[i]) * rows;
+        for (unsigned mask = lane + 1; mask < (1u << n); mask += 64)
+          if (!(mask & (1u << sl)))
+            tab[mask] = sel_fit(cs, tdsel, [&](int i, int j) { return gsel[i * kSelK + j]; }, n, mask, t, rows);
+        __syncthreads();
+        for (uint32_t c = lane; c < count; c += 64) {
+          double* ec = esc + size_t(c) * (1 + K);
+          if (ec[0] < 0.0 || selected(c)) continue;
+          const double* colc = C + size_t(c) * rows;
+          const double* ct[kSelK];
+          for (int i = 0; i < n; ++i) ct[i] = i == sl ? colc : cs[i];
+          double gc[kSelK];
+          for (int i = 0; i < n; ++i) gc[i] = i == sl ? sel_dot(colc, colc, rows) : sel_dot(colc, cs[i], rows);
+          double td[kSelK];
+          for (int i = 0; i < n; ++i) td[i] = i == sl ? sel_dot(colc, t, rows) : tdsel[i];
+          auto g = [&](int i, int j) {
+            if (i == sl) return gc[j];
+            if (j == sl) return gc[i];
+            return gsel[i * kSelK + j];
+          };
+          double best = tt, bw[kSelK];
+          for (int i = 0; i < kSelK; ++i) bw[i] = 0.0;
+          for (unsigned mask = 1; mask < (1u << n); ++mask) {
+            const MaskResult f = (mask & (1u << sl)) ? sel_fit(ct, td, g, n, mask, t, rows) : tab[mask];
+            if (!f.ok) continue;
+            if (f.err < best - 1e-14 * fmax(1.0, best)) {
+              best = f.err;
+              for (int i = 0; i < kSelK; ++i) bw[i] = f.w[i];
+            }
+          }
+          ec[0] = sel_residual_sq(ct, bw, n, t, rows);
+          for (int i = 0; i < n; ++i) ec[1 + i] = bw[i];
+        }
+        __syncthreads();
+        if (lane == 0) {
+          for (uint32_t c = 0; c < count; ++c) {
+            const double* ec = esc + size_t(c) * (1 + K);
+            if (ec[0] < 0.0 || selected(c)) continue;
+            const double ex = ec[0], tl = 1e-14 * fmax(1.0, st.be);
+            if (ex < st.be - tl || (fabs(ex - st.be) <= tl && st.best >= 0 && int(c) < st.best)) {
+              st.best = int(c);
+              st.best_slot = sl;
+              st.be = ex;
+              for (int i = 0; i < n; ++i) st.bw[i] = ec[1 + i];
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (st.best_slot < 0) break;
+      __syncthreads();
+      if (lane == 0) {
+        st.sel[st.best_slot] = st.best;
+        for (int i = 0; i < n; ++i) st.w[i] = st.bw[i];
+        st.cur = st.be;
+      }
+      __syncthreads();
+    }
+  }
+  if (lane == 0) {
+    out_n[blockIdx.x] = uint32_t(st.n);
+    out_cur[blockIdx.x] = st.cur;
+    for (uint32_t i = 0; i < K; ++i) {
+      out_sel[size_t(blockIdx.x) * K + i] = int(i) < st.n ? st.sel[i] : -1;
+      out_w[size_t(blockIdx.x) * K + i] = int(i) < st.n ? st.w[i] : 0.0;
+    }
+  }
+}
+
+}  // namespace
+
+jgs2_status gpu_greedy_select(const double* columns, size_t column_doubles, const double* targets,
+                              size_t target_doubles, const GpuSelectJob* jobs, uint32_t job_count, uint32_t rows,
+                              uint32_t K, double tol, uint32_t passes, int32_t* out_sel, double* out_w,
+                              double* out_cur, uint32_t* out_n, double* milliseconds) {
+  if (K == 0 || K > uint32_t(kSelK)) return fail(JGS2_ERROR_RANGE, "Device selection supports 1 to 8 samples.");
+  if (job_count == 0) return JGS2_OK;
+  size_t scratch_doubles = 0;
+  for (uint32_t j = 0; j < job_count; ++j)
+    scratch_doubles = std::max(scratch_doubles, size_t(jobs[j].scratch_offset) + size_t(jobs[j].count) * (1 + K));
+  double *dc = nullptr, *dt = nullptr, *ds = nullptr, *dw = nullptr, *dcur = nullptr;
+  int32_t* dsel = nullptr;
+  uint32_t* dn = nullptr;
+  GpuSelectJob* dj = nullptr;
+  hipStream_t q = nullptr;
+  auto release = [&]() {
+    (void)hipFree(dc);
+    (void)hipFree(dt);
+    (void)hipFree(ds);
+    (void)hipFree(dw);
+    (void)hipFree(dcur);
+    (void)hipFree(dsel);
+    (void)hipFree(dn);
+    (void)hipFree(dj);
+    if (q) (void)hipStreamDestroy(q);
+  };
+  if (!hip_ok(hipStreamCreate(&q)) || !hip_ok(hipMalloc(&dc, column_doubles * 8)) ||
+      !hip_ok(hipMalloc(&dt, target_doubles * 8)) || !hip_ok(hipMalloc(&ds, scratch_doubles * 8)) ||
+      !hip_ok(hipMalloc(&dw, size_t(job_count) * K * 8)) || !hip_ok(hipMalloc(&dcur, size_t(job_count) * 8)) ||
+      !hip_ok(hipMalloc(&dsel, size_t(job_count) * K * 4)) || !hip_ok(hipMalloc(&dn, size_t(job_count) * 4)) ||
+      !hip_ok(hipMalloc(&dj, size_t(job_count) * sizeof(GpuSelectJob)))) {
+    release();
+    return fail(JGS2_ERROR_MEMORY, "Cubature precompute: device allocation for the selection failed.");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  bool ok = hip_ok(hipMemcpyAsync(dc, columns, column_doubles * 8, hipMemcpyHostToDevice, q)) &&
+            hip_ok(hipMemcpyAsync(dt, targets, target_doubles * 8, hipMemcpyHostToDevice, q)) &&
+            hip_ok(hipMemcpyAsync(dj, jobs, size_t(job_count) * sizeof(GpuSelectJob), hipMemcpyHostToDevice, q));
+  (void)hipEventRecord(e0, q);
+  if (ok)
+    hipLaunchKernelGGL(k_greedy_select, dim3(job_count), dim3(64), 0, q, dc, dt, dj, rows, K, tol, passes, ds, dsel,
+                       dw, dcur, dn);
+  (void)hipEventRecord(e1, q);
+  ok = ok && hip_ok(hipGetLastError()) &&
+       hip_ok(hipMemcpyAsync(out_sel, dsel, size_t(job_count) * K * 4, hipMemcpyDeviceToHost, q)) &&
+       hip_ok(hipMemcpyAsync(out_w, dw, size_t(job_count) * K * 8, hipMemcpyDeviceToHost, q)) &&
+       hip_ok(hipMemcpyAsync(out_cur, dcur, size_t(job_count) * 8, hipMemcpyDeviceToHost, q)) &&
+       hip_ok(hipMemcpyAsync(out_n, dn, size_t(job_count) * 4, hipMemcpyDeviceToHost, q)) &&
+       hip_ok(hipStreamSynchronize(q));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  release();
+  if (milliseconds) *milliseconds = ms;
+  if (!ok) return fail(JGS2_ERROR_DEVICE, "Cubature precompute: the device selection failed.");
+  return JGS2_OK;
+}
+
 }  // namespace jgs2

@@ -268,7 +268,7 @@ class PrecomputeOptions(C.Structure):
         ("basis_rings", C.c_uint32),
         ("threads", C.c_uint32),
         ("mode_solver", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("training", C.c_uint32),
         ("cg_tolerance", C.c_double),
         ("residual_bound", C.c_double),
     ]

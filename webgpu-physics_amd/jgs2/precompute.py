@@ -23,6 +23,7 @@ from .solver import GpuInput
 
 MODELS = {"corotated-linear": N.MATERIAL_COROTATED_LINEAR, "stable-neo-hookean": N.MATERIAL_STABLE_NEO_HOOKEAN}
 MODE_SOLVERS = {"auto": 0, "dense": 1, "cg": 2, "gpu-cg": 3}
+TRAINING = {"auto": 0, "host": 1, "gpu": 2}
 
 
 @dataclass
@@ -37,7 +38,7 @@ class CubatureRecords:
 
 def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, timestep, samples=6, *,
                         rings=0, threads=0, mode_solver="auto", cg_tolerance=0.0,
-                        residual_bound=0.0) -> CubatureRecords:
+                        residual_bound=0.0, training="auto") -> CubatureRecords:
     """`materials`: sequence of (density, young_modulus, poisson_ratio, model)
     with model "stable-neo-hookean" / "corotated-linear" (or the MATERIAL_*
     integer). Raises the library's error classes (JGS2RangeError for invalid
@@ -55,17 +56,18 @@ def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, t
     inp = N.PrecomputeInput(V, T, len(materials), K, x.ctypes.data_as(C.POINTER(C.c_double)),
                             t.ctypes.data_as(N.u32p), f.ctypes.data_as(C.POINTER(C.c_uint8)),
                             m.ctypes.data_as(N.u32p), mats, float(timestep))
-    opt = N.PrecomputeOptions(int(rings), int(threads), MODE_SOLVERS[mode_solver], 0, float(cg_tolerance),
-                              float(residual_bound))
+    opt = N.PrecomputeOptions(int(rings), int(threads), MODE_SOLVERS[mode_solver], TRAINING[training],
+                              float(cg_tolerance), float(residual_bound))
     out = CubatureRecords(np.zeros((V, K), np.uint32), np.zeros((V, K), np.float32),
                           np.zeros((V, K, 36), np.float32), np.zeros(V), np.zeros((V, 3), np.uint32), {})
-    timings = np.zeros(6)
+    timings = np.zeros(8)
     o = N.PrecomputeOutput(out.tet_ids.ctypes.data_as(N.u32p), out.weights.ctypes.data_as(N.f32p),
                            out.basis.ctypes.data_as(N.f32p), out.residuals.ctypes.data_as(C.POINTER(C.c_double)),
                            out.ranks.ctypes.data_as(N.u32p), timings.ctypes.data_as(C.POINTER(C.c_double)))
     N.check(N.lib.jgs2_precompute_cubature(C.byref(inp), C.byref(opt), C.byref(o)))
     out.timings = {"rest_s": timings[0], "modes_s": timings[1], "training_s": timings[2],
-                   "gpu_cg_ms": timings[3], "gpu_cg_iterations": int(timings[4]), "gpu": bool(timings[5])}
+                   "gpu_cg_ms": timings[3], "gpu_cg_iterations": int(timings[4]), "gpu": bool(timings[5]),
+                   "gpu_select_ms": timings[6], "gpu_training": bool(timings[7])}
     return out
 
 
