@@ -1631,7 +1631,7 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
           list.push_back(v);
         }
       }
-      const size_t chunk = 4096;
+      const size_t chunk = 16384;  // several waves per SIMD for the selection kernel
       for (size_t first = 0; first < list.size(); first += chunk) {
         const size_t n = std::min(chunk, list.size() - first);
         std::vector<Prepared> prep(n);

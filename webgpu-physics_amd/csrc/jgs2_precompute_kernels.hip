@@ -376,9 +376,7 @@ jgs2_status GpuCg::solve(double* b, uint32_t m, double tol, uint32_t max_iterati
 // in the host's order, so the selections are the host's bit for bit.
 namespace {
 
-constexpr int kSelK = 8;         // samples per vertex on the device
-constexpr int kSelMasks = 1 << kSelK;
-
+template <int kSelK>
 struct MaskResult {
   double err;
   double w[kSelK];
@@ -393,7 +391,7 @@ __device__ double sel_dot(const double* a, const double* b, uint32_t rows) {
 }
 
 // Gaussian elimination with partial pivoting (math.ts:134-199), n <= kSelK.
-__device__ bool sel_solve(double* a, double* b, int n) {
+__device__ __forceinline__ bool sel_solve(double* a, double* b, int n) {
 #pragma clang fp contract(off)
   for (int p = 0; p < n; ++p) {
     int best = p;
@@ -434,11 +432,11 @@ __device__ bool sel_solve(double* a, double* b, int n) {
 // and target dots td[i]: least squares over the mask, rejected when singular
 // or a weight is below -1e-10; the error is |t - sum w_i col_i|^2 formed
 // row by row in column order.
-template <class G>
-__device__ MaskResult sel_fit(const double* const* col, const double* td, G g, int n, unsigned mask, const double* t,
-                              uint32_t rows) {
+template <int kSelK, class G>
+__device__ MaskResult<kSelK> sel_fit(const double* const* col, const double* td, G g, int n, unsigned mask,
+                                     const double* t, uint32_t rows) {
 #pragma clang fp contract(off)
-  MaskResult f;
+  MaskResult<kSelK> f;
   f.ok = 0;
   f.err = 0.0;
   for (int i = 0; i < kSelK; ++i) f.w[i] = 0.0;
@@ -477,6 +475,7 @@ __device__ double sel_residual_sq(const double* const* col, const double* w, int
   return err;
 }
 
+template <int kSelK>
 struct SelState {
   int sel[kSelK];
   double w[kSelK];
@@ -488,15 +487,16 @@ struct SelState {
   double be, bw[kSelK];
 };
 
+template <int kSelK>
 __global__ void __launch_bounds__(64)
 k_greedy_select(const double* __restrict__ cols, const double* __restrict__ tgts, const GpuSelectJob* __restrict__ jobs,
                 uint32_t rows, uint32_t K, double tol, uint32_t passes, double* __restrict__ scratch,
                 int32_t* __restrict__ out_sel, double* __restrict__ out_w, double* __restrict__ out_cur,
                 uint32_t* __restrict__ out_n) {
 #pragma clang fp contract(off)
-  __shared__ MaskResult tab[kSelMasks];
+  __shared__ MaskResult<kSelK> tab[1 << kSelK];
   __shared__ double gsel[kSelK * kSelK], tdsel[kSelK];
-  __shared__ SelState st;
+  __shared__ SelState<kSelK> st;
   const GpuSelectJob job = jobs[blockIdx.x];
   const uint32_t lane = threadIdx.x;
   const double* C = cols + job.col_offset;
@@ -543,7 +543,7 @@ k_greedy_select(const double* __restrict__ cols, const double* __restrict__ tgts
       const double* cs[kSelK];
       for (int i = 0; i < st.n; ++i) cs[i] = C + size_t(st.sel[i]) * rows;
       for (unsigned mask = lane + 1; mask < (1u << (n - 1)); mask += 64)
-        tab[mask] = sel_fit(cs, tdsel, [&](int i, int j) { return gsel[i * kSelK + j]; }, n, mask, t, rows);
+        tab[mask] = sel_fit<kSelK>(cs, tdsel, [&](int i, int j) { return gsel[i * kSelK + j]; }, n, mask, t, rows);
       __syncthreads();
       for (uint32_t c = lane; c < count; c += 64) {
         double* ec = esc + size_t(c) * (1 + K);
@@ -564,7 +564,8 @@ k_greedy_select(const double* __restrict__ cols, const double* __restrict__ tgts
         double best = tt, bw[kSelK];
         for (int i = 0; i < kSelK; ++i) bw[i] = 0.0;
         for (unsigned mask = 1; mask < (1u << n); ++mask) {
-          const MaskResult f = (mask & (1u << (n - 1))) ? sel_fit(cs, td, g, n, mask, t, rows) : tab[mask];
+          const MaskResult<kSelK> f =
+              (mask & (1u << (n - 1))) ? sel_fit<kSelK>(cs, td, g, n, mask, t, rows) : tab[mask];
           if (!f.ok) continue;
           if (f.err < best - 1e-14 * fmax(1.0, best)) {
             best = f.err;
@@ -626,7 +627,7 @@ k_greedy_select(const double* __restrict__ cols, const double* __restrict__ tgts
         for (int i = 0; i < n; ++i) cs[i] = C + size_t(st.sel[i]) * rows;
         for (unsigned mask = lane + 1; mask < (1u << n); mask += 64)
           if (!(mask & (1u << sl)))
-            tab[mask] = sel_fit(cs, tdsel, [&](int i, int j) { return gsel[i * kSelK + j]; }, n, mask, t, rows);
+            tab[mask] = sel_fit<kSelK>(cs, tdsel, [&](int i, int j) { return gsel[i * kSelK + j]; }, n, mask, t, rows);
         __syncthreads();
         for (uint32_t c = lane; c < count; c += 64) {
           double* ec = esc + size_t(c) * (1 + K);
@@ -646,7 +647,7 @@ k_greedy_select(const double* __restrict__ cols, const double* __restrict__ tgts
           double best = tt, bw[kSelK];
           for (int i = 0; i < kSelK; ++i) bw[i] = 0.0;
           for (unsigned mask = 1; mask < (1u << n); ++mask) {
-            const MaskResult f = (mask & (1u << sl)) ? sel_fit(ct, td, g, n, mask, t, rows) : tab[mask];
+            const MaskResult<kSelK> f = (mask & (1u << sl)) ? sel_fit<kSelK>(ct, td, g, n, mask, t, rows) : tab[mask];
             if (!f.ok) continue;
             if (f.err < best - 1e-14 * fmax(1.0, best)) {
               best = f.err;
@@ -698,7 +699,7 @@ jgs2_status gpu_greedy_select(const double* columns, size_t column_doubles, cons
                               size_t target_doubles, const GpuSelectJob* jobs, uint32_t job_count, uint32_t rows,
                               uint32_t K, double tol, uint32_t passes, int32_t* out_sel, double* out_w,
                               double* out_cur, uint32_t* out_n, double* milliseconds) {
-  if (K == 0 || K > uint32_t(kSelK)) return fail(JGS2_ERROR_RANGE, "Device selection supports 1 to 8 samples.");
+  if (K == 0 || K > 8u) return fail(JGS2_ERROR_RANGE, "Device selection supports 1 to 8 samples.");
   if (job_count == 0) return JGS2_OK;
   size_t scratch_doubles = 0;
   for (uint32_t j = 0; j < job_count; ++j)
@@ -734,8 +735,11 @@ jgs2_status gpu_greedy_select(const double* columns, size_t column_doubles, cons
             hip_ok(hipMemcpyAsync(dt, targets, target_doubles * 8, hipMemcpyHostToDevice, q)) &&
             hip_ok(hipMemcpyAsync(dj, jobs, size_t(job_count) * sizeof(GpuSelectJob), hipMemcpyHostToDevice, q));
   (void)hipEventRecord(e0, q);
-  if (ok)
-    hipLaunchKernelGGL(k_greedy_select, dim3(job_count), dim3(64), 0, q, dc, dt, dj, rows, K, tol, passes, ds, dsel,
+  if (ok && K <= 6)  // smaller private arrays and mask table for the usual K
+    hipLaunchKernelGGL(k_greedy_select<6>, dim3(job_count), dim3(64), 0, q, dc, dt, dj, rows, K, tol, passes, ds, dsel,
+                       dw, dcur, dn);
+  else if (ok)
+    hipLaunchKernelGGL(k_greedy_select<8>, dim3(job_count), dim3(64), 0, q, dc, dt, dj, rows, K, tol, passes, ds, dsel,
                        dw, dcur, dn);
   (void)hipEventRecord(e1, q);
   ok = ok && hip_ok(hipGetLastError()) &&
