@@ -237,6 +237,8 @@ def main(argv=None):
         result["single_large_mesh"] = measure_large(cells, it, args.large_steps)
     if args.precompute_cells:
         result["precompute"] = measure_precompute(tuple(int(c) for c in args.precompute_cells.split(",")))
+        if not args.no_cpu_baseline:
+            result["precompute"]["cpu_baseline"] = precompute_cpu_baseline()
     if not args.no_accuracy:
         result["accuracy"] = accuracy_check(it, args.seed)
     if not args.no_cpu_baseline:
@@ -497,6 +499,15 @@ def measure_large(cells, iterations, steps):
                                "launches_profiled": launches}
             hist = solver.read_globalization_history().reshape(64, 8, 4)
             leg["reverted_iterations_last_frame"] = int(hist[:iterations, 1, 3].sum())
+            # exact FP64 diagnostics of the frame's result (one evaluation, host sync included)
+            solver.read_oracle_diagnostics(settings)
+            a = time.perf_counter()
+            d = solver.read_oracle_diagnostics(settings)
+            leg["diagnostics"] = {"ms": (time.perf_counter() - a) * 1e3, "energy": d["energy"],
+                                  "elasticity": d["elasticity"], "inertia": d["inertia"],
+                                  "relative_residual": d["relative_residual"],
+                                  "minimum_deformation_determinant": d["minimum_deformation_determinant"],
+                                  "finite": bool(d["finite"])}
         solver.destroy()
         out[schedule] = leg
         log(f"large mesh {schedule}: {leg['ms_per_step']:.2f} ms/frame")
@@ -543,6 +554,36 @@ def measure_precompute(cells):
         out[f"rings_{rings}"] = _train(scene, mats, np.zeros(scene.tet_count), rings)
         log(f"precompute {out['workload']} rings {rings}: {out[f'rings_{rings}']['seconds']:.2f} s")
     return out
+
+
+def precompute_cpu_baseline():
+    """The FP64 oracle's precompute (the reference's dense algorithm, one
+    thread) beside the library's on the same 4x2x2 stable Neo-Hookean
+    cantilever (45 vertices, 96 tetrahedra, whole-mesh basis): the oracle
+    evaluates every tetrahedron as a candidate, as the reference does."""
+    from jgs2 import scenes
+    from jgs2.precompute import precompute_cubature
+
+    sys.path.insert(0, str(REPO / "oracle"))
+    import jgs2_oracle as O
+
+    scene = scenes.cantilever_beam(cells=(4, 2, 2), size=(1.0, 0.5, 0.5))
+    rest = scene.vertex_rest.reshape(-1, 4)[:, :3].astype(np.float64)
+    fixed = scene.vertex_info.reshape(-1, 4)[:, 2] != 0
+    tets = scene.tet_indices.reshape(-1, 4)
+    mats = [(1000.0, 1e5, 0.4, 1)]
+    mid = np.zeros(scene.tet_count)
+    t0 = time.perf_counter()
+    ot, ow, _, _ = O.precompute_cubature(rest, tets, fixed, mid, mats, 1.0 / 60.0, 6, bound=1.0)
+    oracle_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = precompute_cubature(rest, tets, fixed, mid, mats, 1.0 / 60.0, 6, rings=0, threads=_host_threads(),
+                            residual_bound=1.0)
+    lib_s = time.perf_counter() - t0
+    return {"value": scene.vertex_count / oracle_s, "unit": "vertices/s", "cores": 1, "kind": "port",
+            "sample": "FP64 oracle precompute of the 4x2x2 cantilever (45 vertices, 96 tetrahedra, K = 6)",
+            "oracle_s": oracle_s, "library_s": lib_s, "library_threads": _host_threads(),
+            "same_records": bool(np.array_equal(ot, r.tet_ids) and np.array_equal(ow, r.weights))}
 
 
 def precompute_layered_box(cells, rings):
