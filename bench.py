@@ -85,8 +85,9 @@ def main(argv=None):
                          "and exit")
     ap.add_argument("--precompute-cells", default="16,4,4",
                     help="cantilever cells of the cubature precompute leg ('' skips it)")
-    ap.add_argument("--profile-precompute", default=None, metavar="CELLS:RINGS",
-                    help="only the cubature precompute of the layered box, e.g. 32,32,64:1")
+    ap.add_argument("--profile-precompute", default=None, metavar="CELLS:RINGS[:FRAMES]",
+                    help="only the cubature precompute of the layered box, e.g. 32,32,64:1; with FRAMES, "
+                         "then Jacobi frames on the trained records against the stand-in records")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args(argv)
@@ -99,8 +100,10 @@ def main(argv=None):
     if args.profile_large:
         return profile_large([int(x) for x in args.profile_large.split(",")], args.large_cells, args.iterations)
     if args.profile_precompute:
-        cells, rings = args.profile_precompute.split(":")
-        print(json.dumps(precompute_layered_box(tuple(int(c) for c in cells.split(",")), int(rings))), flush=True)
+        parts = args.profile_precompute.split(":")
+        frames = int(parts[2]) if len(parts) > 2 else 0
+        print(json.dumps(precompute_layered_box(tuple(int(c) for c in parts[0].split(",")), int(parts[1]), frames,
+                                                args.iterations)), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every leg runs on one GPU).",
@@ -521,7 +524,7 @@ def _host_threads():
         return max(1, min(16, os.cpu_count() or 1))
 
 
-def _train(scene, materials, material_ids, rings):
+def _train(scene, materials, material_ids, rings, records=None):
     from jgs2.precompute import precompute_cubature
 
     rest = scene.vertex_rest.reshape(-1, 4)[:, :3].astype(np.float64)
@@ -531,6 +534,8 @@ def _train(scene, materials, material_ids, rings):
                             scene.cubature_k, rings=rings, threads=_host_threads(), residual_bound=1.0)
     seconds = time.perf_counter() - t0
     free = ~fixed
+    if records is not None:
+        records.append(r)
     return {"rings": rings, "seconds": seconds, "vertices_per_s": float(free.sum()) / seconds,
             "phases_s": {k: round(float(v), 4) for k, v in r.timings.items() if k.endswith("_s")},
             "modes_on_gpu": bool(r.timings["gpu"]), "gpu_cg_ms": float(r.timings["gpu_cg_ms"]),
@@ -586,10 +591,13 @@ def precompute_cpu_baseline():
             "same_records": bool(np.array_equal(ot, r.tet_ids) and np.array_equal(ow, r.weights))}
 
 
-def precompute_layered_box(cells, rings):
+def precompute_layered_box(cells, rings, frames=0, iterations=8):
     """Cubature precompute of the layered box (two stable Neo-Hookean layer
-    materials, E = 1e4 / 1e6 Pa): modes by GPU conjugate gradients, training
-    on the host threads."""
+    materials, E = 1e4 / 1e6 Pa): modes and selection on the GPU, bases and
+    columns on the host threads. With `frames`, Jacobi frames of the box on
+    the trained records and on the stand-in records: frame time and the
+    iterations the assembled accept/revert rejected."""
+    import jgs2
     from jgs2 import scenes
 
     scene = scenes.layered_box(cells=cells)
@@ -599,8 +607,35 @@ def precompute_layered_box(cells, rings):
     out = {"workload": f"layered-box-{cells[0]}x{cells[1]}x{cells[2]}", "vertices": scene.vertex_count,
            "tetrahedra": scene.tet_count, "samples": scene.cubature_k}
     log(f"precompute {out['workload']} rings {rings}: start")
-    out.update(_train(scene, mats, mid, rings))
+    trained = []
+    out.update(_train(scene, mats, mid, rings, trained))
     log(f"precompute {out['workload']} rings {rings}: {out['seconds']:.1f} s")
+    if frames > 0:
+        rest = scene.vertex_rest.reshape(-1, 4)[:, :3].astype(np.float64)
+        settings = dict(scenes.LAYERED_BOX_SETTINGS)
+        for name in ("stand-in", "trained"):
+            if name == "trained":
+                r = trained[0]
+                scene.cubature_tet_ids = np.ascontiguousarray(r.tet_ids.reshape(-1))
+                scene.cubature_weights = np.ascontiguousarray(r.weights.reshape(-1))
+                scene.cubature_basis = np.ascontiguousarray(r.basis.reshape(-1))
+            solver = jgs2.JGS2GpuSolver.create(scene, settings)
+            solver.step_exact_iterations(iterations, settings)
+            solver.await_idle()
+            reverted, t0 = 0, time.perf_counter()
+            for _ in range(frames):
+                solver.step_exact_iterations(iterations, settings)
+                solver.await_idle()
+                hist = solver.read_globalization_history().reshape(64, 8, 4)
+                reverted += int(hist[:iterations, 1, 3].sum())
+            ms = (time.perf_counter() - t0) / frames * 1e3
+            x = solver.read_positions().reshape(-1, 4)[:, :3]
+            out[f"frames_{name}"] = {"frames": frames, "ms_per_frame": ms, "reverted_iterations": reverted,
+                                     "iterations": frames * iterations, "finite": bool(np.isfinite(x).all()),
+                                     "largest_motion_m": float(np.abs(x - rest).max())}
+            log(f"layered box on {name} records: {ms:.2f} ms/frame, {reverted} of {frames * iterations} "
+                f"iterations reverted")
+            solver.destroy()
     return out
 
 
