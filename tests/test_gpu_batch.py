@@ -163,6 +163,30 @@ def test_graph_replays_match_eager_frames():
     graphed.destroy()
 
 
+def test_recapture_between_queued_replays_matches_eager():
+    # Two settings alternate between submissions with no synchronization in
+    # between, so every submission recaptures the frame graph while the
+    # previous one's replays may still be queued on the stream.
+    inputs = mix()[:3]
+    a = dict(SETTINGS, iterations=IT)
+    b = dict(SETTINGS, iterations=IT, gravity=(0.0, -4.0, 1.0))
+    print("case: alternating overrides, async graph recaptures", flush=True)
+    eager = JGS2Batch.create(capacity_for(inputs), SETTINGS)
+    graphed = JGS2Batch.create(capacity_for(inputs), SETTINGS)
+    em = [eager.admit(x) for x in inputs]
+    gm = [graphed.admit(x) for x in inputs]
+    graphed.use_graphs(True)
+    for k in range(8):
+        st = a if k % 2 == 0 else b
+        frames = 1 + k % 3
+        eager.step_frames_exact_iterations(frames, IT, st)
+        graphed.step_frames_exact_iterations(frames, IT, st)
+    for i in range(len(inputs)):
+        assert_same(state(gm[i]), state(em[i]), f"scene {i}")
+    eager.destroy()
+    graphed.destroy()
+
+
 def test_launch_shapes_and_autotune_do_not_change_results():
     inputs = mix()
     settings = dict(SETTINGS, iterations=IT)

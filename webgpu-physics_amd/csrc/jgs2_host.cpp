@@ -1434,7 +1434,12 @@ jgs2_status submit(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
       key = fnv(key, &f, sizeof(f));
     }
     if (!b->graph_valid || key != b->graph_key) {
-      if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+      if (b->graph_exec) {
+        // Earlier launches of this exec may still be queued on the stream:
+        // it is destroyed only once they have completed.
+        HIP_CHECK(hipStreamSynchronize(b->stream));
+        (void)hipGraphExecDestroy(b->graph_exec);
+      }
       b->graph_exec = nullptr;
       b->graph_valid = false;
       hipGraph_t graph = nullptr;

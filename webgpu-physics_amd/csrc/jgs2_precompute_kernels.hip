@@ -349,6 +349,9 @@ jgs2_status GpuCg::solve(double* b, uint32_t m, double tol, uint32_t max_iterati
     for (uint32_t j = 0; j < m; ++j) all = all && host_state[4 * j + 3] != 0.0;
     if (all) break;
   }
+  // An error can leave the loop with a replay still queued: drain the
+  // stream before the exec goes away.
+  (void)hipStreamSynchronize(g.stream);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   (void)hipEventRecord(e1, g.stream);
