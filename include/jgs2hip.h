@@ -447,6 +447,21 @@ jgs2_status jgs2_read_oracle_diagnostics(jgs2_solver* solver, const jgs2_setting
                                          double* vertex_records, size_t vertex_record_count, double* tet_records,
                                          size_t tet_record_count);
 
+/* No counterpart in the reference. jgs2_read_oracle_diagnostics for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc. Same kernels and summary,
+ * evaluated at the solver's own pose against its own last predicted
+ * positions (no caller poses, no per-vertex or per-tetrahedron records).
+ * The records of one evaluation live in a fixed 64 MiB device array of the
+ * library (8,388,608 doubles: 36 per vertex, 4 per tetrahedron, 9 more per
+ * co-rotated tetrahedron, 2 per candidate slot, 32 of summary); a larger
+ * scene is a JGS2_ERROR_RANGE. Calls no hipMalloc or hipFree. Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (several
+ * threads, or several solvers from several threads) race; calls from one
+ * thread are serialized, since each waits for its result. */
+jgs2_status jgs2_read_oracle_diagnostics_noalloc(jgs2_solver* solver, const jgs2_settings* overrides,
+                                                 uint64_t override_mask, jgs2_diagnostics* out);
+
 /* ---------------------------------------------------------------------
  * Cubature precompute: buildPrecomputedScene (src/simulation/cpu/
  * precompute.ts:218-380) with its exact vertex basis (:135-200), nonlinear

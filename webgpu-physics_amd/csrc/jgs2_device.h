@@ -260,6 +260,10 @@ struct DiagParams {
   double* summary;   // 32
 };
 void launch_diagnostics(const DeviceState& s, const DiagParams& q, hipStream_t stream);
+// Fixed device buffer of the allocation-free diagnostics, in doubles (64 MiB).
+constexpr size_t kDiagScratchDoubles = size_t(8) << 20;
+// Device address of `device`'s copy of that buffer.
+hipError_t diagnostics_scratch(int device, double** ptr);
 // IPC passes (encodeFrame's contact dispatches, jgs2-solver.ts:2231-2400).
 void launch_lag_contact(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_build_active_rows(const DeviceState& s, const PassParams& p, hipStream_t stream);

@@ -173,6 +173,7 @@ struct jgs2_solver {
 struct jgs2_batch {
   DeviceState dev;
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current at creation, which owns the stream
   jgs2_settings defaults{};
   jgs2_batch_config cfg{};
   bool corotated = false;
@@ -1899,7 +1900,8 @@ jgs2_status batch_create(const jgs2_batch_config& cfg, const jgs2_settings& reso
   b->defaults = resolved;
   b->corotated = cfg.corotated != 0;
   jgs2_status st = JGS2_OK;
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipGetDevice(&b->device) != hipSuccess) st = fail(JGS2_ERROR_DEVICE, "hipGetDevice failed.");
+  if (st == JGS2_OK && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     st = fail(JGS2_ERROR_DEVICE, "hipStreamCreateWithFlags failed.");
   if (st == JGS2_OK) st = batch_allocate(b);
   // Contact parameters are fixed when the arena is created
@@ -2441,48 +2443,28 @@ jgs2_status jgs2_debug_local_systems(jgs2_solver* s, const jgs2_settings* ov, ui
   return JGS2_OK;
 }
 
-jgs2_status jgs2_read_oracle_diagnostics(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask, const float* positions,
-                                         const float* predicted, jgs2_diagnostics* out, double* vertex_records,
-                                         size_t vertex_record_count, double* tet_records, size_t tet_record_count) {
-  TRYS(check_usable(s));
-  if (!out) return fail(JGS2_ERROR_TYPE, "readOracleDiagnostics needs a summary destination.");
-  if (vertex_records && vertex_record_count != 36 * size_t(s->V))
-    return fail(JGS2_ERROR_RANGE, "Oracle diagnostics vertex records need 36V doubles.");
-  if (tet_records && tet_record_count != 4 * size_t(s->T))
-    return fail(JGS2_ERROR_RANGE, "Oracle diagnostics tetrahedron records need 4T doubles.");
+namespace {
+
+// Doubles of one evaluation's device records: 36 per vertex, 4 per
+// tetrahedron (+9 per tetrahedron for co-rotated frames), 2 per candidate
+// slot, 32 of summary.
+size_t diagnostics_doubles(const jgs2_solver* s) {
+  const size_t V = s->V, T = s->T, C = size_t(s->cblk.n) * kSceneBlock;
+  return 36 * V + 4 * T + (s->corotated ? 9 * T : 0) + 2 * C + 32;
+}
+
+// The evaluator's four passes over one scene at positions x against the
+// predicted positions xp, with its records in `buf` (diagnostics_doubles),
+// then the summary. Synchronizes the batch's stream.
+jgs2_status evaluate_diagnostics(jgs2_solver* s, const jgs2_settings& r, const float4* x, const float4* xp, double* buf,
+                                 jgs2_diagnostics* out, DiagParams* used) {
   jgs2_batch* b = s->batch;
-  jgs2_settings r;
-  TRYS(resolve_batch_step(b, ov, mask, "readOracleDiagnostics", r));
-  if (!(r.floor_stiffness >= 0)) return fail(JGS2_ERROR_RANGE, "floorStiffness must be nonnegative.");
   DeviceState& d = b->dev;
   hipStream_t q = b->stream;
-  const size_t Vc = size_t(b->va.capacity) * kSceneBlock;
   const size_t V = s->V, T = s->T, C = size_t(s->cblk.n) * kSceneBlock;
-  const size_t need = 36 * V + 4 * T + (s->corotated ? 9 * T : 0) + 2 * C + 32;
-  if (!d.diag_x) {
-    TRYS(dalloc(&d.diag_x, Vc));
-    TRYS(dalloc(&d.diag_xp, Vc));
-  }
-  if (d.diag_doubles < need) {
-    if (d.diag_buf) HIP_CHECK(hipFree(d.diag_buf));
-    d.diag_buf = nullptr;
-    d.diag_doubles = 0;
-    TRYS(dalloc(&d.diag_buf, need));
-    d.diag_doubles = need;
-  }
-  HIP_CHECK(hipStreamSynchronize(q));
-  auto stage = [&](const float* src, float4* dst) -> jgs2_status {
-    std::vector<float4> staged(size_t(s->vblk.n) * kSceneBlock, make_float4(0, 0, 0, 0));
-    for (uint32_t ov = 0; ov < s->V; ++ov)
-      staged[s->vnew[ov]] = make_float4(src[4 * ov], src[4 * ov + 1], src[4 * ov + 2], src[4 * ov + 3]);
-    HIP_CHECK(hipMemcpy(dst + s->vbase(), staged.data(), staged.size() * sizeof(float4), hipMemcpyHostToDevice));
-    return JGS2_OK;
-  };
-  if (positions) TRYS(stage(positions, d.diag_x));
-  if (predicted) TRYS(stage(predicted, d.diag_xp));
   DiagParams p{};
-  p.x = positions ? d.diag_x : d.pos[0];
-  p.xp = predicted ? d.diag_xp : d.predicted;
+  p.x = x;
+  p.xp = xp;
   p.vb = uint32_t(s->vbase());
   p.V = uint32_t(V);
   p.tb = uint32_t(s->tbase());
@@ -2498,7 +2480,7 @@ jgs2_status jgs2_read_oracle_diagnostics(jgs2_solver* s, const jgs2_settings* ov
   p.kappa = r.ipc_barrier_stiffness;
   p.dhat = r.ipc_activation_distance;
   p.polar_eps = r.rotation_epsilon;
-  p.vrec = d.diag_buf;
+  p.vrec = buf;
   p.trec = p.vrec + 36 * V;
   p.trot = p.trec + 4 * T;
   p.crec = p.trot + (s->corotated ? 9 * T : 0);
@@ -2537,6 +2519,60 @@ jgs2_status jgs2_read_oracle_diagnostics(jgs2_solver* s, const jgs2_settings* ov
   o.minimum_determinant_valid = T && std::isfinite(m[15]) ? 1u : 0u;
   o.floor_metrics_valid = r.floor_stiffness > 0 ? 1u : 0u;
   *out = o;
+  if (used) *used = p;
+  return JGS2_OK;
+}
+
+jgs2_status resolve_diagnostics_settings(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask, const char* api,
+                                         jgs2_settings& r) {
+  TRYS(resolve_batch_step(s->batch, ov, mask, api, r));
+  if (!(r.floor_stiffness >= 0)) return fail(JGS2_ERROR_RANGE, "floorStiffness must be nonnegative.");
+  return JGS2_OK;
+}
+
+}  // namespace
+
+jgs2_status jgs2_read_oracle_diagnostics(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask, const float* positions,
+                                         const float* predicted, jgs2_diagnostics* out, double* vertex_records,
+                                         size_t vertex_record_count, double* tet_records, size_t tet_record_count) {
+  TRYS(check_usable(s));
+  if (!out) return fail(JGS2_ERROR_TYPE, "readOracleDiagnostics needs a summary destination.");
+  if (vertex_records && vertex_record_count != 36 * size_t(s->V))
+    return fail(JGS2_ERROR_RANGE, "Oracle diagnostics vertex records need 36V doubles.");
+  if (tet_records && tet_record_count != 4 * size_t(s->T))
+    return fail(JGS2_ERROR_RANGE, "Oracle diagnostics tetrahedron records need 4T doubles.");
+  jgs2_batch* b = s->batch;
+  jgs2_settings r;
+  TRYS(resolve_diagnostics_settings(s, ov, mask, "readOracleDiagnostics", r));
+  DeviceState& d = b->dev;
+  hipStream_t q = b->stream;
+  const size_t Vc = size_t(b->va.capacity) * kSceneBlock;
+  const size_t V = s->V, T = s->T;
+  const size_t need = diagnostics_doubles(s);
+  if (!d.diag_x) {
+    TRYS(dalloc(&d.diag_x, Vc));
+    TRYS(dalloc(&d.diag_xp, Vc));
+  }
+  if (d.diag_doubles < need) {
+    if (d.diag_buf) HIP_CHECK(hipFree(d.diag_buf));
+    d.diag_buf = nullptr;
+    d.diag_doubles = 0;
+    TRYS(dalloc(&d.diag_buf, need));
+    d.diag_doubles = need;
+  }
+  HIP_CHECK(hipStreamSynchronize(q));
+  auto stage = [&](const float* src, float4* dst) -> jgs2_status {
+    std::vector<float4> staged(size_t(s->vblk.n) * kSceneBlock, make_float4(0, 0, 0, 0));
+    for (uint32_t ov = 0; ov < s->V; ++ov)
+      staged[s->vnew[ov]] = make_float4(src[4 * ov], src[4 * ov + 1], src[4 * ov + 2], src[4 * ov + 3]);
+    HIP_CHECK(hipMemcpy(dst + s->vbase(), staged.data(), staged.size() * sizeof(float4), hipMemcpyHostToDevice));
+    return JGS2_OK;
+  };
+  if (positions) TRYS(stage(positions, d.diag_x));
+  if (predicted) TRYS(stage(predicted, d.diag_xp));
+  DiagParams p{};
+  TRYS(evaluate_diagnostics(s, r, positions ? d.diag_x : d.pos[0], predicted ? d.diag_xp : d.predicted, d.diag_buf,
+                            out, &p));
   if (vertex_records) {
     std::vector<double> all(36 * size_t(s->vblk.n) * kSceneBlock);
     HIP_CHECK(hipMemcpy(all.data(), p.vrec, 36 * V * sizeof(double), hipMemcpyDeviceToHost));
@@ -2549,6 +2585,27 @@ jgs2_status jgs2_read_oracle_diagnostics(jgs2_solver* s, const jgs2_settings* ov
     for (uint32_t ot = 0; ot < s->T; ++ot)
       std::memcpy(tet_records + 4 * size_t(ot), &all[4 * size_t(s->tnew[ot])], 4 * sizeof(double));
   }
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_read_oracle_diagnostics_noalloc(jgs2_solver* s, const jgs2_settings* ov, uint64_t mask,
+                                                 jgs2_diagnostics* out) {
+  TRYS(check_usable(s));
+  if (!out) return fail(JGS2_ERROR_TYPE, "readOracleDiagnostics needs a summary destination.");
+  jgs2_settings r;
+  TRYS(resolve_diagnostics_settings(s, ov, mask, "readOracleDiagnostics", r));
+  const size_t need = diagnostics_doubles(s);
+  if (need > kDiagScratchDoubles)
+    return fail(JGS2_ERROR_RANGE,
+                "Allocation-free diagnostics hold at most %zu doubles of records (36 per vertex, 4 per tetrahedron, "
+                "9 more per co-rotated tetrahedron, 2 per candidate slot, 32); this scene needs %zu.",
+                kDiagScratchDoubles, need);
+  double* buf = nullptr;
+  hipError_t e = diagnostics_scratch(s->batch->device, &buf);
+  if (e != hipSuccess)
+    return fail(JGS2_ERROR_DEVICE, "Looking up the diagnostics buffer failed: %s", hipGetErrorString(e));
+  TRYS(evaluate_diagnostics(s, r, s->batch->dev.pos[0], s->batch->dev.predicted, buf, out, nullptr));
   clear_error();
   return JGS2_OK;
 }

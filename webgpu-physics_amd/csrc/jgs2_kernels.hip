@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "jgs2_device.h"
 
@@ -4601,6 +4602,41 @@ void launch_diagnostics(const DeviceState& s, const DiagParams& q, hipStream_t s
   hipLaunchKernelGGL(k_diag_vertices, dim3((q.V + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, q);
   if (q.C) hipLaunchKernelGGL(k_diag_contact, dim3((q.C + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, q);
   hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(kBlock), 0, st, q);
+}
+
+}  // namespace jgs2
+
+// ------------------------------------------------------------------ fixed diagnostics buffer
+// Working buffer of the allocation-free diagnostics entry point: the per-vertex,
+// per-tetrahedron and per-candidate records and the summary of one evaluation
+// (every word is written before it is read, so nothing needs clearing). Each
+// device that loads the library holds its own copy until the process ends.
+static __device__ __align__(256) double scratch[jgs2::kDiagScratchDoubles];
+
+namespace jgs2 {
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_address[kScratchDevices] = {};
+}  // namespace
+
+// The array's name is only a host-side handle: kernels get the device
+// address, looked up once per device with that device current.
+hipError_t diagnostics_scratch(int device, double** ptr) {
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_address[device]) {
+    int current = -1;
+    hipError_t e = hipGetDevice(&current);
+    if (e == hipSuccess && current != device) e = hipSetDevice(device);
+    void* p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+    if (current >= 0 && current != device) (void)hipSetDevice(current);
+    if (e != hipSuccess) return e;
+    scratch_address[device] = static_cast<double*>(p);
+  }
+  *ptr = scratch_address[device];
+  return hipSuccess;
 }
 
 }  // namespace jgs2

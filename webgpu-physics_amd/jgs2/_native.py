@@ -360,6 +360,8 @@ SIGNATURES = {
     "jgs2_read_oracle_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(Settings), C.c_uint64, f32p, f32p,
                                                C.POINTER(Diagnostics), C.POINTER(C.c_double), C.c_size_t,
                                                C.POINTER(C.c_double), C.c_size_t]),
+    "jgs2_read_oracle_diagnostics_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Settings), C.c_uint64,
+                                                       C.POINTER(Diagnostics)]),
     "jgs2_precompute_cubature": (C.c_int, [C.POINTER(PrecomputeInput), C.POINTER(PrecomputeOptions),
                                            C.POINTER(PrecomputeOutput)]),
 }

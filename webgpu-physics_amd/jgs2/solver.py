@@ -465,6 +465,15 @@ class JGS2GpuSolver:
         summary = {name: getattr(out, name) for name, _ in N.Diagnostics._fields_ if name != "reserved"}
         return (summary, vrec, trec) if records else summary
 
+    def read_oracle_diagnostics_noalloc(self, overrides: Optional[Mapping] = None) -> dict:
+        """The summary of read_oracle_diagnostics at the solver's own pose,
+        evaluated in the library's fixed device buffer (no device
+        allocation); scenes too large for that buffer raise JGS2RangeError."""
+        s, mask = settings_struct(overrides)
+        out = N.Diagnostics()
+        N.check(N.lib.jgs2_read_oracle_diagnostics_noalloc(self._h(), C.byref(s), mask, C.byref(out)))
+        return {name: getattr(out, name) for name, _ in N.Diagnostics._fields_ if name != "reserved"}
+
     # -------------------------------------------------------------- profiling
     def profile_begin(self) -> None:
         N.check(N.lib.jgs2_profile_begin(self._h()))
