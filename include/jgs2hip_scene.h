@@ -49,6 +49,34 @@ typedef struct {
 jgs2_status jgs2_scene_build_cuboid(const jgs2_cuboid_desc* desc, jgs2_scene** out);
 
 /*
+ * A caller tetrahedral mesh packed like packJGS2GpuInput
+ * (src/scenes/index.ts:975-1109): rest matrices, volumes and lumped masses
+ * (computeRestTetraData / lumped masses, src/simulation/cpu/fem.ts:167-238),
+ * adjusted stable Neo-Hookean parameters (stable-neo-hookean.ts:68-109), the
+ * canonical CSR and vertex records. Tetrahedra keep the caller's vertex
+ * order. cubature_k > 0 adds the closed-form stand-in records of
+ * jgs2_scene_build_cuboid with decay length `cubature_length` (0: the edge
+ * of a cube of six times the mean tetrahedron volume).
+ */
+typedef struct {
+  uint32_t vertex_count;
+  uint32_t tet_count;
+  const double* positions;          /* 3 per vertex */
+  const uint32_t* tetrahedra;       /* 4 per tetrahedron */
+  const uint32_t* material_ids;     /* 1 per tetrahedron; NULL = all 0 */
+  const uint8_t* fixed;             /* 1 per vertex, nonzero = pinned; NULL = none */
+  const uint32_t* body_ids;         /* 1 per vertex; NULL = all 0 */
+  uint32_t material_count;
+  uint32_t cubature_k;
+  const jgs2_precompute_material* materials;  /* density, E, nu, model */
+  double cubature_length;
+  int32_t pack_rest_stiffness;
+  int32_t reserved;
+} jgs2_mesh_desc;
+
+jgs2_status jgs2_scene_build_mesh(const jgs2_mesh_desc* desc, jgs2_scene** out);
+
+/*
  * A pinned cloth drape in the shape of the reference's cloth scene
  * (clothDefinition, src/scenes/index.ts:632-705): a fixed one-cell stable
  * Neo-Hookean collider block (rho 1000, E 1.1e5, nu 0.34) and an n x n quad

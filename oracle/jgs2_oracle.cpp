@@ -2773,6 +2773,91 @@ template <class T> Sim<T> cloth_probe(const double* rest12, const double* pos12,
   return s;
 }
 
+// Assembled accept/revert of one Jacobi iteration at caller poses
+// (candidateTetrahedron + reduceCandidate + applyCandidate, S:3499-3983), the
+// GPU form of acceptOrRevertFeasibleJacobiIteration (nonlinear-iteration.ts:
+// 34-102). Source and candidate poses are 3 doubles per vertex; vertex
+// energy records (4 per vertex: source, candidate, source valid, candidate
+// valid) default to valid zeros; every local solve reports an accepted
+// status with one trial. Out: the four control vec4s (16), the positions
+// after apply (3 per vertex) and the final update records (4 per vertex).
+template <class T> void assembled_candidate(Sim<T>& s, const double* src, const double* cand, const double* ve,
+                                            double* ctl, double* pos, double* upd) {
+  s.p.src = 1;
+  s.p.tgt = 0;
+  s.p.colored = false;
+  s.p.stop_enabled = false;
+  for (uint32_t v = 0; v < s.V; ++v) {
+    s.pos[1][v] = v3(T(src[3 * v]), T(src[3 * v + 1]), T(src[3 * v + 2]));
+    s.pos[0][v] = v3(T(cand[3 * v]), T(cand[3 * v + 1]), T(cand[3 * v + 2]));
+    s.vertex_energy[v] = ve ? V4<T>{T(ve[4 * v]), T(ve[4 * v + 1]), T(ve[4 * v + 2]), T(ve[4 * v + 3])}
+                            : V4<T>{T(0), T(0), T(1), T(1)};
+    for (int k = 0; k < 4; ++k) s.local_diag[4 * size_t(v) + k] = V4<T>{0, 0, 0, 0};
+    s.local_diag[4 * size_t(v) + 3].z = T(1);
+  }
+  for (auto& c : s.control) c = V4<T>{0, 0, 0, 0};
+  s.k_candidate_tet();
+  s.k_reduce_candidate();
+  s.k_apply_candidate();
+  for (int r = 0; r < 4; ++r) {
+    ctl[4 * r] = double(s.control[r].x);
+    ctl[4 * r + 1] = double(s.control[r].y);
+    ctl[4 * r + 2] = double(s.control[r].z);
+    ctl[4 * r + 3] = double(s.control[r].w);
+  }
+  copy_v3(s.pos[0], pos);
+  copy_v4(s.final_update, upd);
+}
+
+// Convergence reduction (reduceConvergence, S:4106-4486) of caller records,
+// the GPU form of evaluateJGS2Convergence (nonlinear-iteration.ts:221-339).
+// gradients: 15 doubles per vertex (inertia, material, external force,
+// target, contact); updates: accepted update vectors, 3 per vertex, turned
+// into final update records by applyCandidate (S:3958-3983); control_in: the
+// four control vec4s the candidate reduction left (16). Writes history row
+// control_in[7] (32 doubles) and the control after the pass (16).
+template <class T> void convergence(Sim<T>& s, const double* gradients, const double* updates, const double* control_in,
+                                    double scene_scale, double residual_tol, double update_tol, int stop, double* row,
+                                    double* ctl) {
+  s.p.src = 1;
+  s.p.tgt = 0;
+  s.p.colored = false;
+  s.p.stop_enabled = false;
+  for (uint32_t v = 0; v < s.V; ++v) {
+    s.pos[1][v] = v3<T>(0, 0, 0);
+    s.pos[0][v] = v3(T(updates[3 * v]), T(updates[3 * v + 1]), T(updates[3 * v + 2]));
+    for (int k = 0; k < 5; ++k) {
+      const double* g = gradients + 15 * size_t(v) + 3 * k;
+      s.conv[5 * size_t(v) + k] = v3(T(g[0]), T(g[1]), T(g[2]));
+    }
+  }
+  for (auto& c : s.control) c = V4<T>{0, 0, 0, 0};
+  s.control[1].x = T(1);
+  s.k_apply_candidate();
+  for (int r = 0; r < 4; ++r)
+    s.control[r] = V4<T>{T(control_in[4 * r]), T(control_in[4 * r + 1]), T(control_in[4 * r + 2]),
+                         T(control_in[4 * r + 3])};
+  s.p.scene_scale = T(scene_scale);
+  s.p.residual_tol = T(residual_tol);
+  s.p.update_tol = T(update_tol);
+  s.p.stop_enabled = stop != 0;
+  const uint32_t hi = uint32_t(control_in[7]);
+  s.k_reduce_convergence();
+  for (int r = 0; r < 8; ++r) {
+    const V4<T> h = s.history[size_t(hi) * 8 + r];
+    row[4 * r] = double(h.x);
+    row[4 * r + 1] = double(h.y);
+    row[4 * r + 2] = double(h.z);
+    row[4 * r + 3] = double(h.w);
+  }
+  for (int r = 0; r < 4; ++r) {
+    ctl[4 * r] = double(s.control[r].x);
+    ctl[4 * r + 1] = double(s.control[r].y);
+    ctl[4 * r + 2] = double(s.control[r].z);
+    ctl[4 * r + 3] = double(s.control[r].w);
+  }
+}
+
 }  // namespace
 
 // ============================================================== exported API
@@ -2898,6 +2983,26 @@ void jgs2o_read_vertex_rotations(void* hv, double* out) {
 void jgs2o_local_systems(void* hv, const jgs2o_settings* s, double* g, double* hess) {
   Handle* h = static_cast<Handle*>(hv);
   if (h->d) local_systems(*h->d, *s, g, hess); else local_systems(*h->f, *s, g, hess);
+}
+
+void jgs2o_assembled_candidate(void* hv, const double* source, const double* candidate, const double* vertex_energy,
+                               double* control_out, double* positions_out, double* updates_out) {
+  Handle* h = static_cast<Handle*>(hv);
+  if (h->d)
+    assembled_candidate(*h->d, source, candidate, vertex_energy, control_out, positions_out, updates_out);
+  else
+    assembled_candidate(*h->f, source, candidate, vertex_energy, control_out, positions_out, updates_out);
+}
+void jgs2o_convergence(void* hv, const double* gradients, const double* updates, const double* control_in,
+                       double scene_scale, double residual_tol, double update_tol, int stop, double* row_out,
+                       double* control_out) {
+  Handle* h = static_cast<Handle*>(hv);
+  if (h->d)
+    convergence(*h->d, gradients, updates, control_in, scene_scale, residual_tol, update_tol, stop, row_out,
+                control_out);
+  else
+    convergence(*h->f, gradients, updates, control_in, scene_scale, residual_tol, update_tol, stop, row_out,
+                control_out);
 }
 
 // ---- standalone restatements used to pin pieces in isolation

@@ -115,6 +115,9 @@ def _load() -> C.CDLL:
         "jgs2o_read_history": (None, [C.c_void_p, f64p]),
         "jgs2o_read_vertex_rotations": (None, [C.c_void_p, f64p]),
         "jgs2o_local_systems": (None, [C.c_void_p, C.POINTER(OracleSettings), f64p, f64p]),
+        "jgs2o_assembled_candidate": (None, [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p]),
+        "jgs2o_convergence": (None, [C.c_void_p, f64p, f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_int,
+                                     f64p, f64p]),
         "jgs2o_restricted_model": (None, [C.c_void_p, C.POINTER(OracleSettings), C.c_uint32, f64p, f64p]),
         "jgs2o_set_pose": (None, [C.c_void_p, f32p, f32p]),
         "jgs2o_snh": (None, [C.c_int, f64p, f64p, C.c_double, C.c_double, f64p, f64p, f64p]),
@@ -382,6 +385,34 @@ class Oracle:
         h = np.empty(9 * self.V)
         lib.jgs2o_local_systems(self._handle, C.byref(st), g.ctypes.data_as(f64p), h.ctypes.data_as(f64p))
         return g.reshape(-1, 3), h.reshape(-1, 3, 3)
+
+    def assembled_candidate(self, source, candidate, vertex_energy=None):
+        """One iteration's assembled accept/revert at caller poses (3 doubles
+        per vertex): (control [4, 4], positions after apply [V, 3], final
+        update records [V, 4])."""
+        src = np.ascontiguousarray(source, np.float64).reshape(-1)
+        cand = np.ascontiguousarray(candidate, np.float64).reshape(-1)
+        ve = None if vertex_energy is None else np.ascontiguousarray(vertex_energy, np.float64).reshape(-1)
+        ctl, pos, upd = np.empty(16), np.empty(3 * self.V), np.empty(4 * self.V)
+        lib.jgs2o_assembled_candidate(self._handle, src.ctypes.data_as(f64p), cand.ctypes.data_as(f64p),
+                                      None if ve is None else ve.ctypes.data_as(f64p), ctl.ctypes.data_as(f64p),
+                                      pos.ctypes.data_as(f64p), upd.ctypes.data_as(f64p))
+        return ctl.reshape(4, 4), pos.reshape(-1, 3), upd.reshape(-1, 4)
+
+    def convergence(self, gradients, updates, control, scene_scale, residual_tolerance, update_tolerance,
+                    stop=False):
+        """The convergence reduction of caller records: gradients [V, 5, 3]
+        (inertia, material, external force, target, contact), accepted
+        update vectors [V, 3], control [4, 4] (control[1, 3] = history
+        row). Returns (history row [8, 4], control after [4, 4])."""
+        g = np.ascontiguousarray(gradients, np.float64).reshape(-1)
+        u = np.ascontiguousarray(updates, np.float64).reshape(-1)
+        c = np.ascontiguousarray(control, np.float64).reshape(-1)
+        row, ctl = np.empty(32), np.empty(16)
+        lib.jgs2o_convergence(self._handle, g.ctypes.data_as(f64p), u.ctypes.data_as(f64p), c.ctypes.data_as(f64p),
+                              float(scene_scale), float(residual_tolerance), float(update_tolerance),
+                              1 if stop else 0, row.ctypes.data_as(f64p), ctl.ctypes.data_as(f64p))
+        return row.reshape(8, 4), ctl.reshape(4, 4)
 
     def restricted_model(self, vertex: int, d, settings=None):
         st = settings_struct(resolve(settings))

@@ -14,6 +14,7 @@ import pytest
 import jgs2
 from jgs2 import scenes
 import jgs2_oracle as O
+from gpu_compare import check_local_systems
 
 pytestmark = pytest.mark.gpu
 
@@ -65,6 +66,7 @@ def test_cloth_local_systems_match_oracle(drape):
     print("max g rel", g_err.max(), "fp32 oracle", g_err32.max(), "max H rel", h_err.max(), flush=True)
     assert h_err.max() < GH_TOL
     assert g_err.max() <= max(2.0 * g_err32.max(), GH_TOL)
+    check_local_systems(g_gpu, h_gpu, systems, free, GH_TOL, "cloth drape")
     solver.destroy()
 
 

@@ -16,6 +16,7 @@ import pytest
 import jgs2
 from jgs2 import scenes
 import jgs2_oracle as O
+from gpu_compare import check_local_systems
 from ipc_geometry import candidate_distances
 
 pytestmark = pytest.mark.gpu
@@ -122,4 +123,5 @@ def test_trough_local_systems_in_contact_match_oracle(trough):
     print("max g rel", g_err.max(), "fp32 oracle", g_err32.max(), "max H rel", h_err.max(), flush=True)
     assert h_err.max() < 1e-4
     assert g_err.max() <= max(2.0 * g_err32.max(), 1e-5)
+    check_local_systems(g_gpu, h_gpu, systems, free, 1e-5, "trough in contact")
     solver.destroy()

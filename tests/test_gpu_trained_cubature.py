@@ -14,6 +14,7 @@ import jgs2
 from jgs2 import scenes
 from jgs2.precompute import train_scene
 import jgs2_oracle as O
+from gpu_compare import check_local_systems
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +60,7 @@ def test_trained_cantilever_matches_fp64_oracle(cells, size, rings):
     # held to the f32 rounding the FP32 oracle itself shows at the same pose.
     assert h_err.max() < 1e-5
     assert g_err.max() <= 3 * g_err32.max() + 1e-5
+    check_local_systems(g_gpu, h_gpu, systems, free, 1e-5, "trained cubature")
     solver.destroy()
     solver = jgs2.JGS2GpuSolver.create(scene, settings)
     frames = O.Oracle(scene, 64)

@@ -457,6 +457,66 @@ jgs2_status jgs2_scene_build_cuboid(const jgs2_cuboid_desc* d, jgs2_scene** out)
   return JGS2_OK;
 }
 
+jgs2_status jgs2_scene_build_mesh(const jgs2_mesh_desc* d, jgs2_scene** out) {
+  if (!d || !out || !d->positions || !d->tetrahedra || !d->materials)
+    return jgs2::fail(JGS2_ERROR_TYPE, "NULL mesh description or array.");
+  *out = nullptr;
+  const uint32_t V = d->vertex_count, T = d->tet_count;
+  if (V < 4 || T < 1) return jgs2::fail(JGS2_ERROR_RANGE, "A mesh needs at least one tetrahedron.");
+  if (d->material_count < 1) return jgs2::fail(JGS2_ERROR_RANGE, "A mesh needs at least one material.");
+  jgs2::Mesh m;
+  m.x.assign(d->positions, d->positions + 3 * size_t(V));
+  for (double c : m.x)
+    if (!std::isfinite(c)) return jgs2::fail(JGS2_ERROR_RANGE, "Mesh positions must be finite.");
+  m.tets.assign(d->tetrahedra, d->tetrahedra + 4 * size_t(T));
+  for (uint32_t t = 0; t < T; ++t)
+    for (int a = 0; a < 4; ++a) {
+      const uint32_t i = m.tets[4 * size_t(t) + a];
+      if (i >= V) return jgs2::fail(JGS2_ERROR_RANGE, "Tetrahedron %u references vertex %u of %u.", t, i, V);
+      for (int b = 0; b < a; ++b)
+        if (m.tets[4 * size_t(t) + b] == i)
+          return jgs2::fail(JGS2_ERROR_RANGE, "Tetrahedron %u repeats vertex %u.", t, i);
+    }
+  m.material.assign(T, 0u);
+  if (d->material_ids) m.material.assign(d->material_ids, d->material_ids + T);
+  m.fixed.assign(V, 0u);
+  if (d->fixed)
+    for (uint32_t v = 0; v < V; ++v) m.fixed[v] = d->fixed[v] ? 1u : 0u;
+  m.body.assign(V, 0u);
+  if (d->body_ids) m.body.assign(d->body_ids, d->body_ids + V);
+  std::vector<jgs2::Material> mats;
+  for (uint32_t i = 0; i < d->material_count; ++i) {
+    const jgs2_precompute_material& q = d->materials[i];
+    if (!(q.density > 0) || !(q.young_modulus > 0))
+      return jgs2::fail(JGS2_ERROR_RANGE, "Material density and Young's modulus must be positive.");
+    if (!(q.poisson_ratio >= 0 && q.poisson_ratio < 0.5))
+      return jgs2::fail(JGS2_ERROR_RANGE, "Stable Neo-Hookean material requires Poisson ratio in [0, 0.5).");
+    if (q.model != JGS2_MATERIAL_COROTATED_LINEAR && q.model != JGS2_MATERIAL_STABLE_NEO_HOOKEAN)
+      return jgs2::fail(JGS2_ERROR_RANGE, "Unknown material model %u.", q.model);
+    mats.push_back({int32_t(q.model), q.density, q.young_modulus, q.poisson_ratio});
+  }
+  double h = d->cubature_length;
+  if (!(h > 0)) {
+    double volume = 0;
+    for (uint32_t t = 0; t < T; ++t) {
+      double r[9];
+      rest_matrix(m.x, &m.tets[4 * size_t(t)], r);
+      volume += std::fabs(determinant3(r)) / 6.0;
+    }
+    h = std::cbrt(6.0 * volume / double(T));
+  }
+  jgs2_scene* s = new (std::nothrow) jgs2_scene();
+  if (!s) return jgs2::fail(JGS2_ERROR_MEMORY, "Out of host memory.");
+  jgs2_status st = jgs2::pack_scene(m, mats, d->cubature_k, d->pack_rest_stiffness != 0, h, s);
+  if (st != JGS2_OK) {
+    delete s;
+    return st;
+  }
+  *out = s;
+  jgs2::clear_error();
+  return JGS2_OK;
+}
+
 }  // extern "C"
 
 namespace {

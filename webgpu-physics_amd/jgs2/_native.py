@@ -248,6 +248,24 @@ class PrecomputeMaterial(C.Structure):
     ]
 
 
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("vertex_count", C.c_uint32),
+        ("tet_count", C.c_uint32),
+        ("positions", C.POINTER(C.c_double)),
+        ("tetrahedra", C.POINTER(C.c_uint32)),
+        ("material_ids", C.POINTER(C.c_uint32)),
+        ("fixed", C.POINTER(C.c_uint8)),
+        ("body_ids", C.POINTER(C.c_uint32)),
+        ("material_count", C.c_uint32),
+        ("cubature_k", C.c_uint32),
+        ("materials", C.POINTER(PrecomputeMaterial)),
+        ("cubature_length", C.c_double),
+        ("pack_rest_stiffness", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
 class PrecomputeInput(C.Structure):
     _fields_ = [
         ("vertex_count", C.c_uint32),
@@ -344,6 +362,7 @@ SIGNATURES = {
         C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "jgs2_solver_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "jgs2_scene_build_cuboid": (C.c_int, [C.POINTER(CuboidDesc), C.POINTER(C.c_void_p)]),
+    "jgs2_scene_build_mesh": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     "jgs2_scene_build_cloth_drape": (C.c_int, [C.POINTER(ClothDesc), C.POINTER(C.c_void_p)]),
     "jgs2_scene_build_trough": (C.c_int, [C.POINTER(TroughDesc), C.POINTER(C.c_void_p)]),
     "jgs2_scene_get_surface": (

@@ -117,6 +117,48 @@ def build_cuboid(desc: N.CuboidDesc) -> GpuInput:
         N.lib.jgs2_scene_destroy(handle)
 
 
+def from_mesh(positions, tetrahedra, materials, material_ids=None, fixed=None, body_ids=None, cubature_k=0,
+              cubature_length=0.0, rest_stiffness=False) -> GpuInput:
+    """Pack a caller tetrahedral mesh like packJGS2GpuInput
+    (src/scenes/index.ts:975-1109). `materials` is a list of (density,
+    young, poisson, model) with model N.MATERIAL_*."""
+    x = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    t = np.ascontiguousarray(tetrahedra, np.uint32).reshape(-1)
+    V, T = x.size // 3, t.size // 4
+    mats = (N.PrecomputeMaterial * len(materials))()
+    for i, q in enumerate(materials):
+        mats[i].density, mats[i].young_modulus, mats[i].poisson_ratio = float(q[0]), float(q[1]), float(q[2])
+        mats[i].model = int(q[3]) if len(q) > 3 else N.MATERIAL_STABLE_NEO_HOOKEAN
+    keep = [x, t]
+    d = N.MeshDesc()
+    d.vertex_count, d.tet_count = V, T
+    d.positions = x.ctypes.data_as(C.POINTER(C.c_double))
+    d.tetrahedra = t.ctypes.data_as(C.POINTER(C.c_uint32))
+    if material_ids is not None:
+        m = np.ascontiguousarray(material_ids, np.uint32)
+        keep.append(m)
+        d.material_ids = m.ctypes.data_as(C.POINTER(C.c_uint32))
+    if fixed is not None:
+        f = np.ascontiguousarray(fixed, np.uint8)
+        keep.append(f)
+        d.fixed = f.ctypes.data_as(C.POINTER(C.c_uint8))
+    if body_ids is not None:
+        b = np.ascontiguousarray(body_ids, np.uint32)
+        keep.append(b)
+        d.body_ids = b.ctypes.data_as(C.POINTER(C.c_uint32))
+    d.material_count = len(materials)
+    d.materials = mats
+    d.cubature_k = int(cubature_k)
+    d.cubature_length = float(cubature_length)
+    d.pack_rest_stiffness = 1 if rest_stiffness else 0
+    handle = C.c_void_p()
+    N.check(N.lib.jgs2_scene_build_mesh(C.byref(d), C.byref(handle)))
+    try:
+        return _from_handle(handle)
+    finally:
+        N.lib.jgs2_scene_destroy(handle)
+
+
 def trough(box_count=10, box_cells=1, seed=0, cubature_k=6, candidates=True, broad_phase=False,
            settings=None, reach_margin=0.0, candidate_capacity=0) -> GpuInput:
     """The reference's IPC trough (troughDefinition, src/scenes/index.ts:
