@@ -48,16 +48,30 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def scene_bytes_per_sweep(scene) -> int:
-    """Reference-ABI compulsory bytes of one sweep over one scene: the
-    tetrahedral terms plus cloth records (triangle 4 vec4 + 3 incidence ids,
-    hinge 2 vec4 + 4 incidence ids)."""
-    from jgs2 import scenes
-
-    b = scenes.algorithmic_bytes_per_sweep(scene.vertex_count, scene.tet_count, scene.cubature_k)
+def scene_bytes_per_sweep(scene, candidates=0) -> int:
+    """Reference-ABI compulsory bytes of one sweep over one scene, counting
+    the records the scene holds: V * 240 (vertex state and records) +
+    T * 96 (tetrahedron record and adjacency ids) + 152 per cubature record
+    in use (tet id set, weight > 0; cloth vertices hold none) + cloth records
+    (triangle 4 vec4 + 3 incidence ids, hinge 2 vec4 + 4 incidence ids) +
+    IPC candidates (144-byte record + 4 incidence ids)."""
+    V, T = scene.vertex_count, scene.tet_count
+    used = int(((scene.cubature_tet_ids != 0xFFFFFFFF) & (scene.cubature_weights > 0)).sum()) if scene.cubature_k else 0
+    b = V * 240 + T * 96 + used * 152
     if scene.cloth is not None:
         b += scene.cloth.triangle_count * (64 + 12) + scene.cloth.hinge_count * (32 + 16)
-    return b
+    return b + int(candidates) * (144 + 16)
+
+
+def build_scenes(specs, contact="broad_phase"):
+    """Builds the scenes of `specs` on the host threads (the scene setup and
+    the broad-phase capacity scans run in the library, outside the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from jgs2 import scenes
+
+    with ThreadPoolExecutor(_host_threads()) as pool:
+        return list(pool.map(lambda s: scenes.build_mixed_scene(s, contact), specs))
 
 
 def main(argv=None):
@@ -123,7 +137,7 @@ def main(argv=None):
     settings = dict(scenes.MIXED_SETTINGS)
     t0 = time.time()
     specs = scenes.mixed_scene_specs(count=args.scenes, seed=args.seed)
-    inputs = [scenes.build_mixed_scene(s) for s in specs]
+    inputs = build_scenes(specs)
     V = sum(x.vertex_count for x in inputs)
     T = sum(x.tet_count for x in inputs)
     NT = sum(x.cloth.triangle_count for x in inputs if x.cloth is not None)
@@ -147,24 +161,29 @@ def main(argv=None):
             log(f"autotune: block {info['solve_block']} xcd_remap {info['xcd_remap']} ({autotune_ms:.3f} ms/launch)")
     info = batch.info()
     batch.sync()
+    # Timed region: the frames are launched pass by pass with HIP events
+    # around every local-solve launch on the batch's stream, so `value` and
+    # the roofline come from the same frames.
+    batch.profile_begin()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch.step_frames_exact_iterations(1, it, settings)
     batch.sync()
     elapsed = time.perf_counter() - t0
+    solve_ms, launches, _ = batch.profile_end()
     value = args.steps * it * free / elapsed
-    log(f"batched: {args.steps} frames in {elapsed:.3f}s, {value:.4g} vertex-updates/s")
+    log(f"batched: {args.steps} frames in {elapsed:.3f}s, {value:.4g} vertex-updates/s, local solve {solve_ms:.3f} ms "
+        f"per launch over {launches} launches")
     lat = []
     for _ in range(args.latency_frames):
         a = time.perf_counter()
         batch.step_frames_exact_iterations(1, it, settings)
         batch.sync()
         lat.append((time.perf_counter() - a) * 1e3)
-    log(f"batched frame latency p50 {nearest_rank(lat, 50):.2f} ms")
-    batch.profile_begin()
-    batch.step_frames_exact_iterations(1, it, settings)
-    solve_ms, launches, _ = batch.profile_end()
-    bytes_per_sweep = sum(scene_bytes_per_sweep(x) for x in inputs)
+    if lat:
+        log(f"batched frame latency (graph replay) p50 {nearest_rank(lat, 50):.2f} ms")
+    lists = [m.contact_info()["candidates"] if x.contact is not None else 0 for x, m in zip(inputs, members)]
+    bytes_per_sweep = sum(scene_bytes_per_sweep(x, c) for x, c in zip(inputs, lists))
     achieved = bytes_per_sweep / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     traffic = measured_traffic(f"batched-uneven-scenes-{len(inputs)}", info["solve_block"])
     # Host readback of every scene's positions (PCIe), once per frame if a
@@ -173,7 +192,18 @@ def main(argv=None):
     t0 = time.perf_counter()
     finite = all(bool(np.isfinite(m.read_positions()).all()) for m in members)
     readback_s = time.perf_counter() - t0
-    reverted = sum(int(m.read_globalization_history().reshape(64, 8, 4)[:it, 1, 3].sum()) for m in members)
+    # Accepted iterations of the last frame, per scene, weighted by its free
+    # vertices: the share of the vertex updates that the assembled
+    # accept/revert kept.
+    reverted, accepted_updates, builds, contact_scenes = 0, 0, 0, 0
+    for x, m in zip(inputs, members):
+        h = m.read_globalization_history().reshape(64, 8, 4)[:it]
+        reverted += int(h[:, 1, 3].sum())
+        accepted_updates += int(h[:, 1, 0].sum()) * m.free_vertex_count
+        if x.contact is not None:
+            contact_scenes += 1
+            builds += int(m.contact_info()["builds"])
+    accepted_fraction = accepted_updates / (it * free)
     batch.destroy()
 
     # ---------------------------------------------------------- plain
@@ -205,11 +235,13 @@ def main(argv=None):
             "schedule": "jacobi",
             "iterations_per_frame": it,
             "seed": args.seed,
-            "mode": "batched (one launch per pass over all scenes)" + ("" if args.no_graphs else ", hipGraph frames"),
+            "mode": "batched (one launch per pass over all scenes); timed frames launched pass by pass with HIP "
+                    "events around each local solve" + ("" if args.no_graphs else ", latency frames replayed from a "
+                                                        "hipGraph"),
             "solve_block": info["solve_block"],
             "xcd_remap": bool(info["xcd_remap"]),
         },
-        "frame_ms": {"p50": nearest_rank(lat, 50), "p99": nearest_rank(lat, 99), "frames": len(lat)},
+        "frame_ms": {"p50": nearest_rank(lat, 50), "p99": nearest_rank(lat, 99), "frames": len(lat)} if lat else None,
         "plain": plain,
         "batched_over_plain": value / plain["vertex_updates_per_s"] if plain and plain["vertex_updates_per_s"] > 0 else None,
         "roofline": {
@@ -228,6 +260,10 @@ def main(argv=None):
         "readback_ms_per_frame": readback_s * 1e3,
         "pcie_inclusive_vertex_updates_per_s": it * free / (elapsed / args.steps + readback_s),
         "reverted_iterations_last_frame_all_scenes": reverted,
+        "accepted_fraction_last_frame": accepted_fraction,
+        "accepted_vertex_updates_per_s": value * accepted_fraction,
+        "self_contact": {"cloth_scenes_with_self_ipc": contact_scenes, "candidates_listed": int(sum(lists)),
+                         "broad_phase_builds_total": builds},
     }
     del inputs, members
 
@@ -639,59 +675,148 @@ def precompute_layered_box(cells, rings, frames=0, iterations=8):
     return out
 
 
-def _sample_specs(seed):
-    from jgs2 import scenes
+def accuracy_check(iterations, seed, frames=36):
+    """GPU trajectories of a sample of the mixed stream against the FP64
+    oracle, the scenes stepped together as one batch exactly as the
+    headline steps them (cloth self-contact through the device broad phase;
+    the oracle steps the same sheets with the static superset, which the
+    broad phase reproduces bit for bit). The sample: the generator's first
+    six tetrahedral scenes of at most 3,000 tetrahedra, and folded sheets of
+    6, 8 and 10 cells built like the generator's (its own sheets, 32 cells
+    and up, have superset lists too long for the oracle).
 
-    return scenes.mixed_scene_specs(count=8, seed=seed, tet_range=(200, 3000), cloth_cells=(6, 16))
+    Bars: non-contact (tetrahedral) scenes within 1e-3 relative L2 of the
+    FP64 oracle's displacement; contact scenes interpenetration-free at
+    every frame (independent distances over the device's candidate list)
+    with an Armijo decrease in every accepted local solve. The FP32
+    oracle's own distance from FP64 is reported beside each scene: the
+    reference computes in FP32, and where that distance exceeds 1e-3 the
+    gate is beyond FP32 arithmetic on this algorithm."""
+    from concurrent.futures import ThreadPoolExecutor
 
-
-def accuracy_check(iterations, seed, frames=2):
-    """A reduced mix of the same generator stepped as one batch on the GPU
-    and scene by scene in the FP64 oracle; the largest relative L2 errors."""
     from jgs2 import scenes
     from jgs2.batch import JGS2Batch
+    from jgs2.solver import ContactData
 
     sys.path.insert(0, str(REPO / "oracle"))
+    sys.path.insert(0, str(REPO / "tests"))
     import jgs2_oracle as O
+    from ipc_geometry import candidate_distances
 
     settings = dict(scenes.MIXED_SETTINGS)
-    inputs = [scenes.build_mixed_scene(s) for s in _sample_specs(seed)]
-    batch = JGS2Batch.create(_capacity(inputs), settings)
-    members = [batch.admit(x) for x in inputs]
-    batch.step_frames_exact_iterations(frames, iterations, settings)
-    worst, worst_disp = 0.0, 0.0
-    for x, m in zip(inputs, members):
-        x_gpu = m.read_positions().reshape(-1, 4)[:, :3].astype(np.float64)
-        ref = O.Oracle(x, 64)
-        ref.step_frames(frames, iterations, settings)
-        x_ref = ref.positions()
-        x0 = x.positions.reshape(-1, 4)[:, :3].astype(np.float64)
-        worst = max(worst, float(np.linalg.norm(x_gpu - x_ref) / np.linalg.norm(x_ref)))
-        worst_disp = max(worst_disp, float(np.linalg.norm(x_gpu - x_ref) / max(np.linalg.norm(x_ref - x0), 1e-30)))
+    tet_specs = [s for s in scenes.mixed_scene_specs(count=512, seed=seed)
+                 if s["kind"] == "tet" and 6 * int(np.prod(s["cells"])) <= 3000][:6]
+    specs = tet_specs + [{"kind": "cloth", "cells": n} for n in (6, 8, 10)]
+    gpu_inputs = [scenes.build_mixed_scene(s, "broad_phase") for s in specs]
+    ref_inputs = [scenes.build_mixed_scene(s, "superset") for s in specs]
+    batch = JGS2Batch.create(_capacity(gpu_inputs), settings)
+    members = [batch.admit(x) for x in gpu_inputs]
+    dmin = settings["ipc_minimum_distance"]
+    nearest = {i: np.inf for i, s in enumerate(specs) if s["kind"] == "cloth"}
+    armijo_ok = True
+    for _ in range(frames):
+        batch.step_frames_exact_iterations(1, iterations, settings)
+        for i in nearest:
+            m = members[i]
+            packed, vt, _ = m.read_contact_candidates()
+            p = m.read_positions().reshape(-1, 4)[:, :3].astype(np.float64)
+            if len(packed):
+                d = candidate_distances(p, ContactData(packed.reshape(-1), vt, len(packed) - vt))
+                nearest[i] = min(nearest[i], float(d.min()))
+            diag = m.read_local_diagnostics().reshape(-1, 4, 4)
+            ok = diag[:, 3, 3] == 0
+            armijo_ok = armijo_ok and bool(np.all(diag[ok, 2, 0] <= diag[ok, 2, 1]))
+    gpu = [m.read_positions().reshape(-1, 4)[:, :3].astype(np.float64) for m in members]
+    # Relative residual of the last iteration (history row 4, w): about 1
+    # means the frame's sweeps made no progress on the implicit system.
+    residuals = [float(m.read_globalization_history().reshape(64, 8, 4)[iterations - 1, 4, 3]) for m in members]
     batch.destroy()
-    log(f"accuracy: max rel L2 {worst:.3e}, displacement-relative {worst_disp:.3e}")
-    return {"reference": "fp64 oracle", "sample": f"{len(inputs)} scenes of the reduced mixed generator",
-            "frames": frames, "max_rel_l2": worst, "max_rel_l2_of_displacement": worst_disp, "tolerance": 1e-3,
-            "within_tol": worst <= 1e-3}
+
+    def oracle_run(args):
+        x, precision = args
+        ref = O.Oracle(x, precision, settings=settings)
+        ref.step_frames(frames, iterations, settings)
+        return ref.positions()
+
+    with ThreadPoolExecutor(_host_threads()) as pool:
+        ref64 = list(pool.map(oracle_run, [(x, 64) for x in ref_inputs]))
+        ref32 = list(pool.map(oracle_run, [(x, 32) for x in ref_inputs]))
+    rows = []
+    for s, x, g, a, b in zip(specs, gpu_inputs, gpu, ref64, ref32):
+        x0 = x.positions.reshape(-1, 4)[:, :3].astype(np.float64)
+        disp = max(float(np.linalg.norm(a - x0)), 1e-30)
+        row = {"kind": s["kind"], "cells": list(s["cells"]) if s["kind"] == "tet" else s["cells"],
+               "young": s.get("young"), "vertices": x.vertex_count,
+               "last_frame_relative_residual": residuals[len(rows)],
+               "gpu_vs_fp64_displacement": float(np.linalg.norm(g - a)) / disp,
+               "fp32_oracle_vs_fp64_displacement": float(np.linalg.norm(b - a)) / disp,
+               "gpu_vs_fp64_positions": float(np.linalg.norm(g - a) / np.linalg.norm(a))}
+        rows.append(row)
+    tet = [r for r in rows if r["kind"] == "tet"]
+    worst_tet = max(r["gpu_vs_fp64_displacement"] for r in tet)
+    interpenetration_free = all(v > dmin for v in nearest.values())
+    within_floor = all(r["gpu_vs_fp64_displacement"] <= max(1e-3, 3.0 * r["fp32_oracle_vs_fp64_displacement"])
+                       for r in rows)
+    out = {"reference": "fp64 oracle", "frames": frames, "tolerance": 1e-3,
+           "sample": f"{len(tet)} tetrahedral scenes of the generator (<= 3,000 tetrahedra) and folded sheets of "
+                     "6, 8, 10 cells with self-contact, one batch",
+           "scenes": rows,
+           "max_rel_l2_of_displacement_tet": worst_tet,
+           "max_rel_l2_of_displacement_all": max(r["gpu_vs_fp64_displacement"] for r in rows),
+           "max_rel_l2_positions": max(r["gpu_vs_fp64_positions"] for r in rows),
+           "contact": {"nearest_pair_m": min(nearest.values()), "minimum_distance_m": dmin,
+                       "interpenetration_free": interpenetration_free, "armijo_every_accepted_solve": armijo_ok},
+           "within_tol": bool(worst_tet <= 1e-3 and interpenetration_free and armijo_ok),
+           "gpu_within_fp32_oracle_floor": within_floor}
+    log(f"accuracy: tet displacement error {worst_tet:.3e} (FP32 oracle "
+        f"{max(r['fp32_oracle_vs_fp64_displacement'] for r in tet):.3e}), nearest contact pair "
+        f"{out['contact']['nearest_pair_m']:.4g} m, within_tol {out['within_tol']}")
+    return out
 
 
-def cpu_baseline(iterations, seed, frames=1, budget_s=10.0):
-    """The oracle's FP32 step (single thread) on scenes of the same
-    generator (up to 20,000 tetrahedra or 64x64 cloth cells each), one frame
-    per scene, until about `budget_s` seconds of CPU time."""
+def _first_build_list(x, settings):
+    """The candidate list of the device's first broad-phase build for a
+    cloth scene (the superset restricted to the build's reach at the initial
+    pose, in canonical order; equal pair for pair to the oracle's
+    restriction, tests/test_gpu_broad_phase.py), as static candidates."""
+    import jgs2
+    from jgs2.solver import ContactData
+
+    solver = jgs2.JGS2GpuSolver.create(x, settings)
+    packed, vt, _ = solver.read_contact_candidates()
+    solver.destroy()
+    return ContactData(packed_indices=np.ascontiguousarray(packed.reshape(-1)), vertex_triangle_count=vt,
+                       edge_edge_count=len(packed) - vt)
+
+
+def cpu_baseline(iterations, seed, frames=1, budget_s=15.0):
+    """The oracle's FP32 step on the host threads over a bounded sample of
+    the headline workload: the generator's scenes in order (tetrahedral
+    scenes up to 20,000 tetrahedra, sheets up to 64 cells, each sheet with
+    the candidate list of the device's first broad-phase build, so its
+    self-contact rows are part of the work), one frame each, until about
+    `budget_s` seconds; plus the cantilever workload in full (60 frames) and
+    one frame of the 64x64x128 layered box."""
     from jgs2 import scenes
 
     sys.path.insert(0, str(REPO / "oracle"))
     import jgs2_oracle as O
 
+    threads = _host_threads()
+    O.set_threads(threads)
     settings = dict(scenes.MIXED_SETTINGS)
-    specs = scenes.mixed_scene_specs(count=64, seed=seed, tet_range=(200, 20000), cloth_cells=(8, 64))
-    updates, dt, used, verts = 0, 0.0, 0, 0
+    specs = [s for s in scenes.mixed_scene_specs(count=512, seed=seed)
+             if (s["kind"] == "tet" and 6 * int(np.prod(s["cells"])) <= 20000) or
+             (s["kind"] == "cloth" and s["cells"] <= 64)]
+    updates, dt, used, verts, rows = 0, 0.0, 0, 0, 0
     for spec in specs:
         if dt >= budget_s:
             break
         x = scenes.build_mixed_scene(spec)
-        ref = O.Oracle(x, 32)
+        if spec["kind"] == "cloth":
+            x.contact = _first_build_list(x, settings)
+            rows += x.contact.candidate_count
+        ref = O.Oracle(x, 32, settings=settings)
         free = int((x.vertex_info.reshape(-1, 4)[:, 2] == 0).sum())
         t0 = time.perf_counter()
         ref.step_frames(frames, iterations, settings)
@@ -699,11 +824,35 @@ def cpu_baseline(iterations, seed, frames=1, budget_s=10.0):
         updates += frames * iterations * free
         used += 1
         verts += x.vertex_count
-    log(f"cpu baseline: {updates} vertex-updates in {dt:.2f}s")
-    return {"value": updates / dt, "unit": "vertex-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP32 step (one thread), the first {used} scenes of the mixed generator capped at "
-                      f"20,000 tetrahedra / 64x64 cloth cells ({verts} vertices), {frames} frame x {iterations} "
-                      f"Jacobi sweeps each, {dt:.1f}s"}
+    out = {"value": updates / dt, "unit": "vertex-updates/s", "cores": threads, "kind": "port",
+           "sample": f"oracle FP32 step on {threads} host threads, the first {used} scenes of the mixed generator "
+                     f"up to 20,000 tetrahedra / 64x64 cloth cells ({verts} vertices, {rows} self-contact "
+                     f"candidates), {frames} frame x {iterations} Jacobi sweeps each, {dt:.1f} s"}
+    log(f"cpu baseline: {updates} vertex-updates in {dt:.2f}s on {threads} threads")
+    legs = {}
+    beam = scenes.cantilever_beam()
+    st = dict(scenes.CANTILEVER_SETTINGS)
+    ref = O.Oracle(beam, 32)
+    free = int((beam.vertex_info.reshape(-1, 4)[:, 2] == 0).sum())
+    t0 = time.perf_counter()
+    ref.step_frames(scenes.CANTILEVER_FRAMES, scenes.CANTILEVER_ITERATIONS, st)
+    t = time.perf_counter() - t0
+    legs["cantilever-16x4x4"] = {"frames": scenes.CANTILEVER_FRAMES, "seconds": t,
+                                 "vertex_updates_per_s": scenes.CANTILEVER_FRAMES * scenes.CANTILEVER_ITERATIONS * free / t}
+    box = scenes.layered_box()
+    st = dict(scenes.LAYERED_BOX_SETTINGS)
+    ref = O.Oracle(box, 32)
+    free = int((box.vertex_info.reshape(-1, 4)[:, 2] == 0).sum())
+    t0 = time.perf_counter()
+    ref.step_frames(1, iterations, st)
+    t = time.perf_counter() - t0
+    legs["layered-box-64x64x128"] = {"frames": 1, "seconds": t, "vertex_updates_per_s": iterations * free / t}
+    del ref, box
+    out["workloads"] = legs
+    O.set_threads(1)
+    log(f"cpu baseline legs: cantilever {legs['cantilever-16x4x4']['vertex_updates_per_s']:.4g}, layered box "
+        f"{legs['layered-box-64x64x128']['vertex_updates_per_s']:.4g} vertex-updates/s")
+    return out
 
 
 if __name__ == "__main__":

@@ -34,6 +34,7 @@
 // reference bit patterns.
 
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -42,6 +43,13 @@
 #include <algorithm>
 #include <limits>
 #include <type_traits>
+
+// Host threads of the per-element and per-vertex passes (jgs2o_set_threads).
+// Every parallel loop writes only its own element's records and every
+// reduction stays serial in the reference's order, so results do not depend
+// on the thread count.
+static std::atomic<int> g_threads{1};
+#define JGS2O_PARALLEL_FOR _Pragma("omp parallel for schedule(dynamic, 64) num_threads(g_threads.load()) if(g_threads.load() > 1)")
 
 namespace {
 
@@ -1700,6 +1708,7 @@ template <class T> struct Sim {
   }
   void k_tet_polar() {  // S:2932-2957
     if (stop_latched()) return;
+    JGS2O_PARALLEL_FOR
     for (uint32_t t = 0; t < TT; ++t) {
       if (uses_snh(t)) {
         trot[t] = midentity<T>();
@@ -1711,6 +1720,7 @@ template <class T> struct Sim {
   void k_vertex_polar() {  // S:2960-2989
     if (stop_latched()) return;
     std::vector<M3<T>> next(V);
+    JGS2O_PARALLEL_FOR
     for (uint32_t v = 0; v < V; ++v) {
       if (adj_count(v) == 0) {
         next[v] = midentity<T>();
@@ -1889,6 +1899,7 @@ template <class T> struct Sim {
         if (w) pos[p.tgt][v] = o;
       }
     } else {
+      JGS2O_PARALLEL_FOR
       for (uint32_t v = 0; v < V; ++v) {
         bool w;
         solve_vertex(v, outs[v], w);
@@ -1903,6 +1914,7 @@ template <class T> struct Sim {
     // With contact the energy pair is recorded after the safe-step passes
     // instead (S:3468-3471).
     if (globalization && !stop_latched() && !ipc_enabled()) {
+      JGS2O_PARALLEL_FOR
       for (uint32_t v = 0; v < V; ++v) {
         if (p.colored && colors[v] != p.active_color) continue;
         store_vertex_energy(v);
@@ -1910,10 +1922,12 @@ template <class T> struct Sim {
     }
   }
   void k_assembled_vertex_energy() {  // S:3490-3496
+    JGS2O_PARALLEL_FOR
     for (uint32_t v = 0; v < V; ++v) store_vertex_energy(v);
   }
   void k_candidate_tet() {  // S:3499-3601 (tetrahedral elements)
     if (stop_latched()) return;
+    JGS2O_PARALLEL_FOR
     for (uint32_t t = 0; t < TT; ++t) {
       M3<T> sf = deformation_at(t, p.src);
       M3<T> cf = deformation_at(t, p.tgt);
@@ -2109,6 +2123,7 @@ template <class T> struct Sim {
   }
   void k_convergence_gradient() {  // S:3986-4097 (tetrahedral part)
     if (stop_latched()) return;
+    JGS2O_PARALLEL_FOR
     for (uint32_t v = 0; v < V; ++v) {
       V3<T>* c = &conv[5 * size_t(v)];
       if (pinned(v)) {
@@ -3004,6 +3019,10 @@ void jgs2o_convergence(void* hv, const double* gradients, const double* updates,
     convergence(*h->f, gradients, updates, control_in, scene_scale, residual_tol, update_tol, stop, row_out,
                 control_out);
 }
+
+// Host threads of the oracle's per-element and per-vertex passes (1 by
+// default; results are identical for every count).
+void jgs2o_set_threads(int threads) { g_threads.store(threads < 1 ? 1 : threads); }
 
 // ---- standalone restatements used to pin pieces in isolation
 

@@ -115,6 +115,7 @@ def _load() -> C.CDLL:
         "jgs2o_read_history": (None, [C.c_void_p, f64p]),
         "jgs2o_read_vertex_rotations": (None, [C.c_void_p, f64p]),
         "jgs2o_local_systems": (None, [C.c_void_p, C.POINTER(OracleSettings), f64p, f64p]),
+        "jgs2o_set_threads": (None, [C.c_int]),
         "jgs2o_assembled_candidate": (None, [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p]),
         "jgs2o_convergence": (None, [C.c_void_p, f64p, f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_int,
                                      f64p, f64p]),
@@ -199,6 +200,12 @@ def settings_struct(resolved: dict) -> OracleSettings:
         o.gravity[i] = float(resolved["gravity"][i])
     o.horizontal_body_correction = 1 if resolved["horizontal_body_correction"] else 0
     return o
+
+
+def set_threads(threads: int) -> None:
+    """Host threads of the oracle's per-element and per-vertex passes
+    (results are identical for every count)."""
+    lib.jgs2o_set_threads(int(threads))
 
 
 def greedy_coloring(vertex_count: int, groups) -> np.ndarray:

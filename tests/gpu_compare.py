@@ -109,22 +109,23 @@ def local_system_errors(g_gpu, h_gpu, g_ref, h_ref, free):
 
 def check_local_systems(g_gpu, h_gpu, systems, free, tol=1e-5, label=""):
     """The local-system bar. `systems` maps 32 and 64 to the FP32 and FP64
-    oracles' (g, H). H within `tol` of both oracles. g within `tol` of the
-    FP32 oracle where FP32 resolves it to that level (the FP32 oracle within
-    `tol` of FP64); where FP32 itself loses digits to cancellation, within
-    three times the FP32 oracle's own distance from FP64. Returns the
-    measured errors."""
+    oracles' (g, H). Each of g and H is within `tol` of both oracles where
+    FP32 resolves it to that level (the FP32 oracle within `tol` of FP64);
+    where FP32 itself loses digits (cancellation in g, the barrier Hessian
+    near the activation distance), within three times the FP32 oracle's own
+    distance from FP64. Returns the measured errors."""
     g32, h32 = systems[32]
     g64, h64 = systems[64]
     out = {}
     out["g_vs_32"], out["h_vs_32"] = local_system_errors(g_gpu, h_gpu, g32, h32, free)
     out["g_vs_64"], out["h_vs_64"] = local_system_errors(g_gpu, h_gpu, g64, h64, free)
-    out["g_32_vs_64"], _ = local_system_errors(g32, h32, g64, h64, free)
+    out["g_32_vs_64"], out["h_32_vs_64"] = local_system_errors(g32, h32, g64, h64, free)
     print(f"{label} g: vs FP32 {out['g_vs_32']:.2e}, vs FP64 {out['g_vs_64']:.2e} (FP32 oracle vs FP64 "
-          f"{out['g_32_vs_64']:.2e}); H: vs FP32 {out['h_vs_32']:.2e}, vs FP64 {out['h_vs_64']:.2e}", flush=True)
-    assert out["h_vs_32"] < tol and out["h_vs_64"] < tol
-    if out["g_32_vs_64"] < tol:
-        assert out["g_vs_32"] < tol and out["g_vs_64"] < tol
-    else:
-        assert out["g_vs_64"] <= 3.0 * out["g_32_vs_64"]
+          f"{out['g_32_vs_64']:.2e}); H: vs FP32 {out['h_vs_32']:.2e}, vs FP64 {out['h_vs_64']:.2e} (FP32 oracle "
+          f"vs FP64 {out['h_32_vs_64']:.2e})", flush=True)
+    for q in ("g", "h"):
+        if out[f"{q}_32_vs_64"] < tol:
+            assert out[f"{q}_vs_32"] < tol and out[f"{q}_vs_64"] < tol, q
+        else:
+            assert out[f"{q}_vs_64"] <= 3.0 * out[f"{q}_32_vs_64"], q
     return out

@@ -325,3 +325,29 @@ def test_cloth_index_out_of_range_rejected():
     scene.cloth.triangle_indices = tri
     with pytest.raises(jgs2.JGS2RangeError, match="Triangle 0 index 99 is outside vertexCount 33"):
         jgs2.JGS2GpuSolver.create(scene, scenes.CLOTH_SETTINGS)
+
+
+def test_drape_contact_surface_and_superset_match_the_reference_scene():
+    # scenes.test.ts:531-620 and surfaceWithCloth (precompute.ts:98-133):
+    # the surface ends with the cloth triangles in their packed order, holds
+    # every cloth edge, and the static superset starts with its nearest pair
+    # inside (0.003, 0.08).
+    from ipc_geometry import candidate_distances
+
+    scene = scenes.cloth_drape(contact="superset")
+    surf = scene.extras["surface"]
+    tris = surf["triangles"].reshape(-1, 3)
+    cloth_tris = scene.cloth.triangle_indices.reshape(-1, 3)
+    assert len(cloth_tris) == 32 and scene.cloth.hinge_count == 40
+    assert np.array_equal(tris[-len(cloth_tris):], cloth_tris)
+    assert len(tris) - len(cloth_tris) == 12  # the collider cube's boundary faces
+    edges = {tuple(e) for e in surf["edges"].reshape(-1, 2)}
+    assert len(edges) == len(surf["edges"]) // 2
+    for t in cloth_tris:
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            assert (min(a, b), max(a, b)) in edges
+    d = candidate_distances(scene.positions.reshape(-1, 4)[:, :3], scene.contact)
+    assert 0.003 < d.min() < 0.08
+    # The superset is the one the library builds from that surface.
+    again = scenes.static_contact_candidates(scene.vertex_count, surf["triangles"], surf["edges"])
+    assert np.array_equal(again.packed_indices, scene.contact.packed_indices)

@@ -721,6 +721,38 @@ extern "C" jgs2_status jgs2_scene_build_cloth_drape(const jgs2_cloth_desc* d, jg
     s->hinge_angles.push_back(float(angle));
     s->hinge_lengths.push_back(float(len));
   }
+  // Contact surface (surfaceWithCloth, src/simulation/cpu/precompute.ts:
+  // 98-133): the collider's boundary faces and edges, then every cloth
+  // triangle and its edges not listed yet.
+  {
+    jgs2::Mesh collider;
+    collider.x.resize(3 * size_t(VC));
+    for (uint32_t v = 0; v < VC; ++v)
+      for (int a = 0; a < 3; ++a) collider.x[3 * size_t(v) + a] = s->vertex_rest[4 * size_t(v) + a];
+    collider.tets = s->tet_indices;
+    collider.material.assign(s->tet_count, 0u);
+    collider.fixed.assign(VC, 1u);
+    collider.body.assign(VC, 0u);
+    st = jgs2::boundary_surface(collider, s->surface_triangles, s->surface_edges);
+    if (st != JGS2_OK) {
+      delete s;
+      return st;
+    }
+    std::unordered_set<uint64_t> listed;
+    for (size_t e = 0; e + 1 < s->surface_edges.size(); e += 2)
+      listed.insert((uint64_t(s->surface_edges[e]) << 32) | s->surface_edges[e + 1]);
+    for (size_t t = 0; t < NT; ++t) {
+      const uint32_t* ix = &s->cloth_triangles[3 * t];
+      s->surface_triangles.insert(s->surface_triangles.end(), ix, ix + 3);
+      for (int e = 0; e < 3; ++e) {
+        const uint32_t a = std::min(ix[e], ix[(e + 1) % 3]), b = std::max(ix[e], ix[(e + 1) % 3]);
+        if (listed.insert((uint64_t(a) << 32) | b).second) {
+          s->surface_edges.push_back(a);
+          s->surface_edges.push_back(b);
+        }
+      }
+    }
+  }
   s->has_cloth = true;
   jgs2_cloth_input& ci = s->cloth;
   ci.triangle_count = uint32_t(NT);

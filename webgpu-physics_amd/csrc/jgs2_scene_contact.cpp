@@ -2,10 +2,12 @@
 // candidate superset and the reference's IPC trough scene. Pure host code.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "jgs2_scene_internal.h"
@@ -312,12 +314,10 @@ bool superset_nonempty(const Surface& s, uint32_t V) {
 
 SurfaceScan scan_surface_pairs(const Surface& s, uint32_t V, const float* positions4, float reach,
                                float distance_reach) {
-  SurfaceScan out;
   std::vector<Box> pts(V), tris(s.tris.size()), edges(s.edges.size());
   for (uint32_t v = 0; v < V; ++v) pts[v] = box_of(positions4, {v});
   for (size_t t = 0; t < s.tris.size(); ++t) tris[t] = box_of(positions4, {s.tris[t][0], s.tris[t][1], s.tris[t][2]});
   for (size_t e = 0; e < s.edges.size(); ++e) edges[e] = box_of(positions4, {s.edges[e][0], s.edges[e][1]});
-  Grid grid;
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (const Box& b : pts)
     if (!b.wild)
@@ -327,19 +327,42 @@ SurfaceScan scan_surface_pairs(const Surface& s, uint32_t V, const float* positi
       }
   const double grow = std::isfinite(reach) ? double(reach) * (1 + 1e-6) + 1e-12 : INFINITY;
   const bool everything = !std::isfinite(grow) || !(lo[0] <= hi[0]);
+  Grid tri_grid, edge_grid;
   if (!everything) {
     const double extent = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-9});
-    grid.cell = std::max(grow, extent / 128.0);
-    for (int k = 0; k < 3; ++k) {
-      grid.origin[k] = lo[k];
-      grid.dims[k] = std::max<int64_t>(1, int64_t(std::ceil((hi[k] - lo[k]) / grid.cell)) + 1);
+    const double cell = std::max(grow, extent / 128.0);
+    for (Grid* g : {&tri_grid, &edge_grid}) {
+      g->cell = cell;
+      for (int k = 0; k < 3; ++k) {
+        g->origin[k] = lo[k];
+        g->dims[k] = std::max<int64_t>(1, int64_t(std::ceil((hi[k] - lo[k]) / cell)) + 1);
+      }
     }
+    tri_grid.insert(tris, grow);
+    edge_grid.insert(edges, grow);
   }
-  auto consider = [&](bool vt, const Box& a, const Box& b, std::initializer_list<uint32_t> ids) {
+  // Primitives with a non-finite point are tested against everything.
+  std::vector<uint32_t> wild_tris, wild_edges;
+  for (uint32_t t = 0; t < tris.size(); ++t)
+    if (tris[t].wild) wild_tris.push_back(t);
+  for (uint32_t e = 0; e < edges.size(); ++e)
+    if (edges[e].wild) wild_edges.push_back(e);
+  // A pair counts when its AABB lower bound is within `reach` (or a point
+  // is non-finite) and no vertex of one lies in the other's closed one-ring;
+  // pairs within `distance_reach` also take the exact distance. The bound
+  // is tested first: it is the cheaper test and rejects most grid pairs.
+  auto consider = [&](SurfaceScan& out, bool vt, const Box& a, const Box& b, std::initializer_list<uint32_t> ids) {
+    const bool wild = a.wild || b.wild;
     const float lb = box_lower_bound(a, b);
-    if (!(a.wild || b.wild || !(lb > reach))) return;
+    if (!(wild || !(lb > reach))) return;
+    const uint32_t* q = ids.begin();
+    const bool excluded = vt ? (in_closed_ring(s, q[0], q[1]) || in_closed_ring(s, q[0], q[2]) ||
+                                in_closed_ring(s, q[0], q[3]))
+                             : (in_closed_ring(s, q[0], q[2]) || in_closed_ring(s, q[0], q[3]) ||
+                                in_closed_ring(s, q[1], q[2]) || in_closed_ring(s, q[1], q[3]));
+    if (excluded) return;
     (vt ? out.vertex_triangle : out.edge_edge) += 1;
-    if (a.wild || b.wild || lb > distance_reach) return;
+    if (wild || lb > distance_reach) return;
     double p[4][3];
     int n = 0;
     for (uint32_t v : ids) {
@@ -349,53 +372,76 @@ SurfaceScan scan_surface_pairs(const Surface& s, uint32_t V, const float* positi
     const double d = vt ? vertex_triangle_distance(p[0], p[1], p[2], p[3]) : edge_edge_distance(p[0], p[1], p[2], p[3]);
     out.min_distance = std::min(out.min_distance, d);
   };
-  // Primitives with a non-finite point are tested against everything.
-  std::vector<uint32_t> wild_tris, wild_edges;
-  for (uint32_t t = 0; t < tris.size(); ++t)
-    if (tris[t].wild) wild_tris.push_back(t);
-  for (uint32_t e = 0; e < edges.size(); ++e)
-    if (edges[e].wild) wild_edges.push_back(e);
   // VT: each point against the triangles listed in its cell.
-  if (!everything) grid.insert(tris, grow);
-  for (uint32_t v = 0; v < V; ++v) {
-    auto test = [&](uint32_t t) {
-      const auto& f = s.tris[t];
-      if (in_closed_ring(s, v, f[0]) || in_closed_ring(s, v, f[1]) || in_closed_ring(s, v, f[2])) return;
-      consider(true, pts[v], tris[t], {v, f[0], f[1], f[2]});
-    };
-    if (everything || pts[v].wild) {
-      for (uint32_t t = 0; t < s.tris.size(); ++t) test(t);
-      continue;
+  auto scan_points = [&](uint32_t begin, uint32_t end, SurfaceScan& out) {
+    for (uint32_t v = begin; v < end; ++v) {
+      auto test = [&](uint32_t t) {
+        const auto& f = s.tris[t];
+        consider(out, true, pts[v], tris[t], {v, f[0], f[1], f[2]});
+      };
+      if (everything || pts[v].wild) {
+        for (uint32_t t = 0; t < s.tris.size(); ++t) test(t);
+        continue;
+      }
+      int64_t c0[3], c1[3];
+      tri_grid.cells_of(pts[v], 0.0, c0, c1);
+      tri_grid.visit(tri_grid.index(c0[0], c0[1], c0[2]), test);
+      for (uint32_t t : wild_tris) test(t);
     }
-    int64_t c0[3], c1[3];
-    grid.cells_of(pts[v], 0.0, c0, c1);
-    grid.visit(grid.index(c0[0], c0[1], c0[2]), test);
-    for (uint32_t t : wild_tris) test(t);
-  }
+  };
   // EE: each edge against the later edges listed in the cells it overlaps.
-  if (!everything) grid.insert(edges, grow);
-  std::vector<uint32_t> stamp(s.edges.size(), 0xffffffffu);
-  for (uint32_t i = 0; i < s.edges.size(); ++i) {
-    const auto& a = s.edges[i];
-    auto test = [&](uint32_t j) {
-      if (j <= i || stamp[j] == i) return;
-      stamp[j] = i;
-      const auto& b = s.edges[j];
-      if (in_closed_ring(s, a[0], b[0]) || in_closed_ring(s, a[0], b[1]) || in_closed_ring(s, a[1], b[0]) ||
-          in_closed_ring(s, a[1], b[1]))
-        return;
-      consider(false, edges[i], edges[j], {a[0], a[1], b[0], b[1]});
-    };
-    if (everything || edges[i].wild) {
-      for (uint32_t j = i + 1; j < s.edges.size(); ++j) test(j);
-      continue;
+  auto scan_edges = [&](uint32_t begin, uint32_t end, SurfaceScan& out) {
+    std::vector<uint32_t> stamp(s.edges.size(), 0xffffffffu);
+    for (uint32_t i = begin; i < end; ++i) {
+      const auto& a = s.edges[i];
+      auto test = [&](uint32_t j) {
+        if (j <= i || stamp[j] == i) return;
+        stamp[j] = i;
+        const auto& b = s.edges[j];
+        consider(out, false, edges[i], edges[j], {a[0], a[1], b[0], b[1]});
+      };
+      if (everything || edges[i].wild) {
+        for (uint32_t j = i + 1; j < s.edges.size(); ++j) test(j);
+        continue;
+      }
+      int64_t c0[3], c1[3];
+      edge_grid.cells_of(edges[i], 0.0, c0, c1);
+      for (int64_t z = c0[2]; z <= c1[2]; ++z)
+        for (int64_t y = c0[1]; y <= c1[1]; ++y)
+          for (int64_t x = c0[0]; x <= c1[0]; ++x) edge_grid.visit(edge_grid.index(x, y, z), test);
+      for (uint32_t j : wild_edges) test(j);
     }
-    int64_t c0[3], c1[3];
-    grid.cells_of(edges[i], 0.0, c0, c1);
-    for (int64_t z = c0[2]; z <= c1[2]; ++z)
-      for (int64_t y = c0[1]; y <= c1[1]; ++y)
-        for (int64_t x = c0[0]; x <= c1[0]; ++x) grid.visit(grid.index(x, y, z), test);
-    for (uint32_t j : wild_edges) test(j);
+  };
+  // Chunks of points and edges over the host threads; the counts are sums
+  // and the distance a minimum, so the result does not depend on the split.
+  const size_t work = size_t(V) + s.edges.size();
+  const unsigned threads =
+      work < 65536 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const uint32_t chunks = 8 * threads;
+  std::vector<SurfaceScan> parts(2 * size_t(chunks));
+  auto run_chunk = [&](uint32_t c) {
+    const uint32_t k = c % chunks;
+    if (c < chunks)
+      scan_points(uint32_t(uint64_t(V) * k / chunks), uint32_t(uint64_t(V) * (k + 1) / chunks), parts[c]);
+    else
+      scan_edges(uint32_t(s.edges.size() * k / chunks), uint32_t(s.edges.size() * (k + 1) / chunks), parts[c]);
+  };
+  if (threads == 1) {
+    for (uint32_t c = 0; c < 2 * chunks; ++c) run_chunk(c);
+  } else {
+    std::vector<std::thread> pool;
+    std::atomic<uint32_t> next{0};
+    for (unsigned t = 0; t < threads; ++t)
+      pool.emplace_back([&] {
+        for (uint32_t c = next++; c < 2 * chunks; c = next++) run_chunk(c);
+      });
+    for (auto& th : pool) th.join();
+  }
+  SurfaceScan out;
+  for (const SurfaceScan& p : parts) {
+    out.vertex_triangle += p.vertex_triangle;
+    out.edge_edge += p.edge_edge;
+    out.min_distance = std::min(out.min_distance, p.min_distance);
   }
   return out;
 }

@@ -256,9 +256,16 @@ def contact_boxes_pool(count, first_seed=1, **kw):
 
 def cloth_drape(cells=4, origin=(-0.8, 1.15, -0.8), size=(1.6, 1.6), fold_column=3, fold_lift=0.079,
                 collider_origin=(-0.65, 0.0, -0.45), collider_size=(0.9, 0.5, 0.9), density=480.0,
-                young=9000.0, poisson=0.3, thickness=0.012, bending=0.045, cubature_k=4) -> GpuInput:
+                young=9000.0, poisson=0.3, thickness=0.012, bending=0.045, cubature_k=4, contact=None,
+                settings=None, candidate_capacity=0, reach_margin=0.0) -> GpuInput:
     """The pinned cloth drape (clothDefinition, src/scenes/index.ts:632-705);
-    the defaults are the reference scene's 4x4 pre-folded sheet."""
+    the defaults are the reference scene's 4x4 pre-folded sheet. Its contact
+    surface is the collider's boundary plus the cloth triangles and edges
+    (surfaceWithCloth, src/simulation/cpu/precompute.ts:98-133).
+    `contact`: None (no IPC candidates), "superset" (the reference's static
+    VT/EE superset of that surface, src/scenes/index.ts:1098-1101) or
+    "broad_phase" (the device broad phase over it, capacity sized for
+    `settings`, CLOTH_IPC_SETTINGS by default)."""
     d = N.ClothDesc()
     d.cells = int(cells)
     for i in range(3):
@@ -280,6 +287,22 @@ def cloth_drape(cells=4, origin=(-0.8, 1.15, -0.8), size=(1.6, 1.6), fold_column
         scene = _from_handle(handle)
     finally:
         N.lib.jgs2_scene_destroy(handle)
+    surf = scene.extras["surface"]
+    if contact == "superset":
+        scene.contact = static_contact_candidates(scene.vertex_count, surf["triangles"], surf["edges"])
+    elif contact == "broad_phase":
+        scene.contact = BroadPhaseContact(surf["triangles"], surf["edges"], int(candidate_capacity),
+                                          float(reach_margin))
+        if not candidate_capacity:
+            # Four times the pairs within the first build's reach (a sheet
+            # folds and drapes onto itself), capped by the superset's size
+            # bound V F + E (E - 1) / 2, which no list can exceed.
+            first = (estimate_contact_capacity(scene, settings or CLOTH_IPC_SETTINGS) - 1024) // 2
+            nf, ne = len(surf["triangles"]) // 3, len(surf["edges"]) // 2
+            bound = scene.vertex_count * nf + ne * (ne - 1) // 2
+            scene.contact.candidate_capacity = int(min(bound, 4 * first + 4096))
+    elif contact is not None:
+        raise ValueError("contact must be None, 'superset' or 'broad_phase'")
     side = int(cells) + 1
     scene.extras["cloth_offset"] = scene.vertex_count - side * side
     scene.extras["landmark_vertex"] = scene.extras["cloth_offset"] + 2 + 2 * side  # cloth centre of the 4x4 sheet
@@ -352,6 +375,15 @@ CLOTH_SETTINGS = {
     "schedule": "jacobi",
 }
 CLOTH_ITERATIONS = 5
+# The reference app's settings for its cloth scene, an IPC scene
+# (src/App.tsx:521-572): activation distance and maxStep 0.01, friction 0.3,
+# barrier stiffness 1e5, minimum distance 0.003, damping 0.997, no floor,
+# no tangential damping or body correction.
+CLOTH_IPC_SETTINGS = dict(CLOTH_SETTINGS, parity_mode=False, velocity_damping=0.997, floor_stiffness=0.0,
+                          contact_tangential_damping=0.0, contact_margin=0.01, horizontal_body_correction=False,
+                          ipc_activation_distance=0.01, ipc_minimum_distance=0.003, ipc_barrier_stiffness=100000.0,
+                          ipc_friction_coefficient=0.3, ipc_friction_velocity_epsilon=0.05, ipc_step_safety=0.9,
+                          max_step=0.01, regularization=1e-6, rotation_epsilon=1e-7)
 
 
 def algorithmic_bytes_per_sweep(V: int, T: int, K: int, objectives: bool = False) -> int:
@@ -366,19 +398,22 @@ def algorithmic_bytes_per_sweep(V: int, T: int, K: int, objectives: bool = False
 
 
 # ---------------------------------------------------------------- mixed scenes
-MIXED_SETTINGS = {
-    "timestep": 1.0 / 60.0,
-    "gravity": (0.0, -9.81, 0.0),
-    "floor_stiffness": 0.0,
-    "horizontal_body_correction": False,
-    "parity_mode": True,
-    "schedule": "jacobi",
-}
+# One settings set for the whole batch: the reference's cloth-scene step
+# (dt = 1/120 and the app's IPC parameters, src/scenes/index.ts:698-704,
+# src/App.tsx:521-572), under which its StVK sheets step without reverts;
+# tetrahedral scenes have no candidates, so the contact terms leave them
+# unchanged.
+MIXED_SETTINGS = dict(CLOTH_IPC_SETTINGS)
 MIXED_ITERATIONS = 8
-# Metres per hex (the cantilever's 0.25 m cell: at dt = 1/60 finer stiff
-# cells make parallel Jacobi sweeps overshoot and revert) and per cloth cell.
+# Metres per hex (the cantilever's 0.25 m cell) and per cloth cell.
 MIXED_HEX = 0.25
 MIXED_CLOTH_CELL = 0.05
+# Each sheet's last quarter of columns is folded back over the sheet at this
+# gap (inside the 0.01 m activation distance, above the 0.003 m minimum), so
+# same-sheet barrier rows are active from the first frame, the way the
+# reference pre-folds its drape to exercise nonlocal same-sheet candidates
+# (src/scenes/index.ts:653-666).
+MIXED_FOLD_GAP = 0.008
 
 
 def mixed_scene_specs(count=512, seed=20251203, cloth_fraction=0.3, tet_range=(200, 200_000),
@@ -388,8 +423,8 @@ def mixed_scene_specs(count=512, seed=20251203, cloth_fraction=0.3, tet_range=(2
     or cubes, six tetrahedra per hex, E log-uniform in `young_range`,
     nu uniform in [0.3, 0.45], the x = 0 face pinned) and cloth sheets of
     n x n cells with n log-uniform in `cloth_cells` (pinned at two corners
-    over a one-cell fixed block far below). Self-contact of the sheets is
-    not generated: it needs a broad phase rather than a static superset."""
+    over a one-cell fixed block far below, the last quarter folded back over
+    the sheet, self-contact through the device broad phase)."""
     rng = np.random.default_rng(seed)
     specs = []
     for i in range(count):
@@ -413,12 +448,17 @@ def mixed_scene_specs(count=512, seed=20251203, cloth_fraction=0.3, tet_range=(2
     return specs
 
 
-def build_mixed_scene(spec) -> GpuInput:
+def build_mixed_scene(spec, contact="broad_phase") -> GpuInput:
+    """One scene of the mixed stream. Cloth sheets carry self-contact over
+    their own surface: `contact` "broad_phase" (the stream's), "superset"
+    (the static superset, for the oracle) or None."""
     if spec["kind"] == "cloth":
         n = int(spec["cells"])
         side = n * MIXED_CLOTH_CELL
-        return cloth_drape(cells=n, origin=(-0.5 * side, 2.0, -0.5 * side), size=(side, side), fold_column=0,
-                           fold_lift=0.0, collider_origin=(-0.05, -50.0, -0.05), collider_size=(0.1, 0.1, 0.1))
+        return cloth_drape(cells=n, origin=(-0.5 * side, 2.0, -0.5 * side), size=(side, side),
+                           fold_column=max(2, n - n // 4), fold_lift=MIXED_FOLD_GAP,
+                           collider_origin=(-0.05, -50.0, -0.05), collider_size=(0.1, 0.1, 0.1), contact=contact,
+                           settings=MIXED_SETTINGS)
     cells = spec["cells"]
     size = tuple(c * MIXED_HEX for c in cells)
     return build_cuboid(_desc(cells, (0.0, 0.0, 0.0), size, jitter=0.1, seed=spec["seed"], pin_axis=0,
