@@ -414,6 +414,9 @@ MIXED_CLOTH_CELL = 0.05
 # reference pre-folds its drape to exercise nonlocal same-sheet candidates
 # (src/scenes/index.ts:653-666).
 MIXED_FOLD_GAP = 0.008
+# Broad-phase reach past the activation reach for the sheets (0 = the
+# library default, half the activation reach).
+MIXED_REACH_MARGIN = 0.0
 
 
 def mixed_scene_specs(count=512, seed=20251203, cloth_fraction=0.3, tet_range=(200, 200_000),
@@ -458,7 +461,7 @@ def build_mixed_scene(spec, contact="broad_phase") -> GpuInput:
         return cloth_drape(cells=n, origin=(-0.5 * side, 2.0, -0.5 * side), size=(side, side),
                            fold_column=max(2, n - n // 4), fold_lift=MIXED_FOLD_GAP,
                            collider_origin=(-0.05, -50.0, -0.05), collider_size=(0.1, 0.1, 0.1), contact=contact,
-                           settings=MIXED_SETTINGS)
+                           settings=MIXED_SETTINGS, reach_margin=MIXED_REACH_MARGIN)
     cells = spec["cells"]
     size = tuple(c * MIXED_HEX for c in cells)
     return build_cuboid(_desc(cells, (0.0, 0.0, 0.0), size, jitter=0.1, seed=spec["seed"], pin_axis=0,
