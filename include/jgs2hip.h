@@ -512,6 +512,14 @@ typedef struct jgs2_precompute_options {
    * reference's 0.01 (NONLINEAR_CUBATURE_TRAINING_RESIDUAL_TOLERANCE).
    * Meshes beyond a few cells need more than K = 6 samples to meet 0.01. */
   double residual_bound;
+  /* Nonzero: the reference's exact-all-element fixture packing
+   * (buildPhase0ForceFreeScene, src/scenes/canonical-phase0.ts:144-205)
+   * instead of training: each free vertex keeps its exact equilibrium basis
+   * over the whole mesh (basis_rings must be 0) and packs every tetrahedron
+   * once with weight 1, so cubature_samples must equal tetrahedron_count;
+   * residuals are 0. */
+  uint32_t exact_all_elements;
+  uint32_t reserved;
 } jgs2_precompute_options;
 
 typedef struct jgs2_precompute_output {

@@ -9,19 +9,17 @@ the metric definitions follow the reference's e2e spec
   0.31] rad with linear velocity [0.37, -0.11, 0.23] and angular velocity
   [0.41, 0.29, -0.19], no gravity, floor, damping or pins, dt = 1/120, 12
   Jacobi iterations, 1200 frames: |P_end - P_0| / max(|P_0|, M s) and
-  |L_end - L_0| / max(|L_0|, M s^2) <= 5e-3 (s = initial bounds diagonal).
-  The fixture runs the reference's exact-all-elements cubature; with this
-  build's stand-in records twelve Jacobi iterations leave the same linear
-  momentum drift in the FP64 oracle as on the GPU (1.2e-2 at 1200 frames),
-  so the test pins the GPU's drift to the FP64 oracle's at 100 frames and
-  keeps the fixture's angular bound at 1200 frames.
+  |L_end - L_0| / max(|L_0|, M s^2) <= 5e-3 (s = initial bounds diagonal),
+  with the fixture's exact-all-elements cubature (every vertex's exact
+  whole-mesh basis over all 48 tetrahedra, weight 1); the GPU's drift also
+  equals the FP64 oracle's at 100 frames.
 * baseline.trough at frame 90: pinned error <= 2e-5, minimum IPC distance
   > 3e-3 (independent distances over the static superset), every box's
   lowest point above the trough apex.
 * baseline.cloth at frame 48: pinned error <= 2e-5, smallest triangle area
   over its rest area > 0.1 (the drape here has no contact candidates, so
   the fixture's IPC distance metric does not apply).
-The cubature of these scenes is this build's stand-in (K records per
+The trough and the drape use this build's stand-in cubature records (K per
 vertex), not the reference's trained selection.
 """
 
@@ -102,6 +100,7 @@ def test_force_free_cuboid_momentum():
     travel = np.linalg.norm(xe.mean(0) - p0.mean(0))
     print(f"{frames} frames: linear momentum error {lin:.3e}, angular {ang:.3e}, centre travel {travel:.3f} m", flush=True)
     assert travel > 1.0  # the body moved (0.45 m/s for 10 s)
+    assert lin <= metric(f, "linear-momentum-error")
     assert ang <= metric(f, "angular-momentum-error")
     solver.destroy()
 

@@ -142,3 +142,31 @@ def test_errors():
     bad[0] = 99
     with pytest.raises(jgs2.JGS2RangeError, match="out of range"):
         precompute_cubature(args[0], bad, *args[2:], 6)
+
+
+def test_exact_all_element_packing_of_the_force_free_fixture():
+    # scenes.test.ts:127-200: cubatureK = tetCount = 48, every vertex packs
+    # tetrahedra 0..47 in order with weight 1 and its exact basis at each
+    # tetrahedron's four vertices (to 5 decimals); pinned here against the
+    # oracle's exact basis (itself pinned to the reference's beam golden).
+    import json
+    from pathlib import Path
+
+    import jgs2_oracle as O
+    from jgs2 import manifests
+
+    m = json.loads((Path(__file__).parent / "golden" / "canonical-scenes.v1.json").read_text())
+    x, settings, _, _ = manifests.force_free_cuboid(m)
+    V, T = x.vertex_count, x.tet_count
+    assert (V, T, x.cubature_k) == (27, 48, 48)
+    ids = x.cubature_tet_ids.reshape(V, T)
+    assert np.array_equal(ids, np.tile(np.arange(T, dtype=np.uint32), (V, 1)))
+    assert np.array_equal(x.cubature_weights.reshape(V, T), np.ones((V, T), np.float32))
+    rest = x.vertex_rest.reshape(-1, 4)[:, :3].astype(np.float64)
+    tets = x.tet_indices.reshape(-1, 4)
+    basis = x.cubature_basis.reshape(V, T, 4, 9)
+    for v in (0, 5, 13, 26):
+        U = O.exact_basis(rest, tets, np.zeros(V), np.zeros(T), [(1000.0, 8e4, 0.3, 1)], settings["timestep"], v)
+        for t in range(T):
+            for l in range(4):
+                assert np.abs(basis[v, t, l] - U[tets[t, l]].reshape(9)).max() < 5e-6

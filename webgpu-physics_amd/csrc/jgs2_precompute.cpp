@@ -244,6 +244,7 @@ struct Precompute {
   uint32_t rings = 0;
   double cg_tolerance = 1e-12;
   double residual_bound = 0.01;
+  bool exact_all_elements = false;
   uint32_t mode_solver = 0;
 
   Vec vol, dminv, shape;  // per tetrahedron: volume, Dm^-1 (9), shape gradients (12)
@@ -269,7 +270,7 @@ void validate_and_load(Precompute& p, const jgs2_precompute_input& in) {
     raise(JGS2_ERROR_RANGE, "A cubature precompute needs at least one vertex and one tetrahedron.");
   if (!in.positions || !in.tetrahedra || !in.fixed || !in.material_ids || !in.materials || in.material_count == 0)
     raise(JGS2_ERROR_TYPE, "Cubature precompute input arrays must not be NULL.");
-  if (in.cubature_samples == 0 || in.cubature_samples > 12)
+  if (in.cubature_samples == 0 || (in.cubature_samples > 12 && !p.exact_all_elements))
     raise(JGS2_ERROR_RANGE, "Cubature samples must be between 1 and 12.");
   if (!(in.timestep > 0) || !std::isfinite(in.timestep)) raise(JGS2_ERROR_RANGE, "The timestep must be finite and positive.");
   p.V = in.vertex_count;
@@ -1574,6 +1575,9 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
       raise(JGS2_ERROR_RANGE, "residual_bound must be finite and nonnegative.");
     p.residual_bound = opt.residual_bound > 0 ? opt.residual_bound : 0.01;
     p.threads = opt.threads ? opt.threads : std::max(1u, std::thread::hardware_concurrency());
+    p.exact_all_elements = opt.exact_all_elements != 0;
+    if (p.exact_all_elements && p.rings != 0)
+      raise(JGS2_ERROR_RANGE, "Exact all-element cubature uses the whole-mesh basis (basis_rings 0).");
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
     auto seconds = [](clock::time_point a, clock::time_point b) { return std::chrono::duration<double>(b - a).count(); };
@@ -1586,6 +1590,38 @@ extern "C" jgs2_status jgs2_precompute_cubature(const jgs2_precompute_input* inp
     std::fill(output->cubature_basis, output->cubature_basis + V * K * 36, 0.0f);
     if (output->vertex_residuals) std::fill(output->vertex_residuals, output->vertex_residuals + V, NAN);
     if (output->vertex_ranks) std::fill(output->vertex_ranks, output->vertex_ranks + 3 * V, 0u);
+    if (p.exact_all_elements) {
+      // buildPhase0ForceFreeScene (canonical-phase0.ts:144-205): every free
+      // vertex's exact basis at the four vertices of every tetrahedron,
+      // tetrahedra in order, weight 1, no training.
+      if (K != p.T)
+        raise(JGS2_ERROR_RANGE, "Exact all-element cubature packs every tetrahedron: cubature_samples must equal "
+                                "tetrahedron_count.");
+      std::vector<Support> supports(p.threads);
+      parallel_for(p.active.size(), p.threads, [&](size_t a, unsigned worker) {
+        const uint32_t v = p.active[a];
+        Support& s = supports[worker];
+        if (s.local.size() != p.V) s.local.assign(p.V, -1);
+        solve_basis(p, v, s);
+        for (uint32_t t = 0; t < p.T; ++t) {
+          output->cubature_tet_ids[v * K + t] = t;
+          output->cubature_weights[v * K + t] = 1.0f;
+          float* b = &output->cubature_basis[(v * K + t) * 36];
+          for (int l = 0; l < 4; ++l) {
+            const int32_t li = s.local[p.tets[4 * size_t(t) + l]];
+            for (int e = 0; e < 9; ++e) b[9 * l + e] = li < 0 ? 0.0f : float(s.U[9 * size_t(li) + size_t(e)]);
+          }
+        }
+        if (output->vertex_residuals) output->vertex_residuals[v] = 0.0;
+        for (uint32_t u : s.verts) s.local[u] = -1;
+      });
+      if (output->timings) {
+        const double t[8] = {seconds(t0, t1), 0.0, seconds(t1, clock::now()), p.gpu_ms, double(p.gpu_iterations),
+                             p.gpu ? 1.0 : 0.0, 0.0, 0.0};
+        std::memcpy(output->timings, t, sizeof t);
+      }
+      return JGS2_OK;
+    }
     std::vector<Pose> poses;
     std::vector<Vec> frames, modes;
     bool has_tet_vertex = false;

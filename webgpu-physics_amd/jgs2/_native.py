@@ -289,6 +289,8 @@ class PrecomputeOptions(C.Structure):
         ("training", C.c_uint32),
         ("cg_tolerance", C.c_double),
         ("residual_bound", C.c_double),
+        ("exact_all_elements", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

@@ -124,14 +124,28 @@ def force_free_cuboid(manifest: dict):
     """The phase-0 force-free cuboid as (scene, settings, iterations,
     frames): 2x2x2 cells rotated by [0.17, -0.23, 0.31] rad with linear
     velocity [0.37, -0.11, 0.23] and angular velocity [0.41, 0.29, -0.19]
-    about the vertex mean, no gravity, floor, damping or pins."""
+    about the vertex mean, no gravity, floor, damping or pins, and the
+    manifest's exact-all-elements cubature: every vertex keeps its exact
+    whole-mesh basis and packs all 48 tetrahedra with weight 1
+    (buildPhase0ForceFreeScene, src/scenes/canonical-phase0.ts:144-205)."""
     from . import scenes
+    from .precompute import precompute_cubature
 
     f = fixture(manifest, "phase0.force-free-cuboid")
     g = f["generator"]["parameters"]
     mat = f["materials"][0]
+    _require(f["solver"]["cubature"]["mode"] == "exact-all-elements", "solver.cubature.mode")
     x = scenes.build_cuboid(scenes._desc(g["cells"], g["origin"], g["size"], pin_axis=3, density=mat["density"],
                                          poisson=mat["poissonRatio"], young=(mat["youngModulus"],) * 2))
+    rest = x.vertex_rest.reshape(-1, 4)[:, :3].astype(np.float64)
+    rec = precompute_cubature(rest, x.tet_indices.reshape(-1, 4), np.zeros(x.vertex_count), np.zeros(x.tet_count),
+                              [(mat["density"], mat["youngModulus"], mat["poissonRatio"], "stable-neo-hookean")],
+                              f["simulation"]["timestep"], x.tet_count, exact_all_elements=True)
+    x.cubature_k = x.tet_count
+    x.cubature_tet_ids = np.ascontiguousarray(rec.tet_ids.reshape(-1))
+    x.cubature_weights = np.ascontiguousarray(rec.weights.reshape(-1))
+    x.cubature_basis = np.ascontiguousarray(rec.basis.reshape(-1))
+    x.extras["cubature"] = "exact-all-elements"
     p0 = _rotate_xyz(x.positions.reshape(-1, 4)[:, :3].astype(np.float64), np.array([0.17, -0.23, 0.31]))
     v_lin, omega = np.array([0.37, -0.11, 0.23]), np.array([0.41, 0.29, -0.19])
     vel = v_lin + np.cross(omega, p0 - p0.mean(0))

@@ -38,12 +38,14 @@ class CubatureRecords:
 
 def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, timestep, samples=6, *,
                         rings=0, threads=0, mode_solver="auto", cg_tolerance=0.0,
-                        residual_bound=0.0, training="auto") -> CubatureRecords:
+                        residual_bound=0.0, training="auto", exact_all_elements=False) -> CubatureRecords:
     """`materials`: sequence of (density, young_modulus, poisson_ratio, model)
     with model "stable-neo-hookean" / "corotated-linear" (or the MATERIAL_*
     integer). Raises the library's error classes (JGS2RangeError for invalid
     input, JGS2InfeasibleError when a vertex's packed residual exceeds
-    `residual_bound`, 0 meaning the reference's 0.01)."""
+    `residual_bound`, 0 meaning the reference's 0.01). `exact_all_elements`:
+    the reference's exact all-element packing (every tetrahedron with weight
+    1 and the whole-mesh exact basis; `samples` = tetrahedron count)."""
     x = np.ascontiguousarray(positions, np.float64).reshape(-1)
     t = np.ascontiguousarray(tetrahedra, np.uint32).reshape(-1)
     f = np.ascontiguousarray(np.asarray(fixed).reshape(-1) != 0, np.uint8)
@@ -57,7 +59,7 @@ def precompute_cubature(positions, tetrahedra, fixed, material_ids, materials, t
                             t.ctypes.data_as(N.u32p), f.ctypes.data_as(C.POINTER(C.c_uint8)),
                             m.ctypes.data_as(N.u32p), mats, float(timestep))
     opt = N.PrecomputeOptions(int(rings), int(threads), MODE_SOLVERS[mode_solver], TRAINING[training],
-                              float(cg_tolerance), float(residual_bound))
+                              float(cg_tolerance), float(residual_bound), 1 if exact_all_elements else 0, 0)
     out = CubatureRecords(np.zeros((V, K), np.uint32), np.zeros((V, K), np.float32),
                           np.zeros((V, K, 36), np.float32), np.zeros(V), np.zeros((V, 3), np.uint32), {})
     timings = np.zeros(8)
