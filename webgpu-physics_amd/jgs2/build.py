@@ -2,7 +2,8 @@
 
 The library is the product: HIP kernels (csrc/jgs2_kernels.hip) plus the
 C-ABI host code (csrc/jgs2_host.cpp, csrc/jgs2_scene*.cpp). It is written to
-webgpu-physics_amd/lib/libjgs2hip.so so that it travels with the tree.
+webgpu-physics_amd/lib/libjgs2hip.so so that it travels with the tree; the
+Node binding (napi/) is linked against it as lib/jgs2hip.node.
 """
 
 from __future__ import annotations
@@ -18,6 +19,9 @@ CSRC = PACKAGE_ROOT / "csrc"
 LIB_DIR = PACKAGE_ROOT / "lib"
 LIBRARY = LIB_DIR / "libjgs2hip.so"
 BUILD_DIR = PACKAGE_ROOT / "_build"
+NAPI_SOURCE = PACKAGE_ROOT / "napi" / "jgs2_napi.cpp"
+NAPI_ADDON = LIB_DIR / "jgs2hip.node"
+NODE_INCLUDE = Path(os.environ.get("NODE_INCLUDE", "/usr/include/node"))
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.environ.get("HIPCC", f"{ROCM}/bin/hipcc")
@@ -71,7 +75,22 @@ def build(verbose: bool = True) -> Path:
     _run([HIPCC, f"--offload-arch={OFFLOAD_ARCH}", "-shared", "-fPIC", *objects, "-o", tmp,
           f"-L{ROCM}/lib", "-lamdhip64", "-pthread", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, LIBRARY)
+    build_napi()
     return LIBRARY
+
+
+def build_napi() -> Path | None:
+    """The Node binding (napi/jgs2_napi.cpp) as lib/jgs2hip.node, linked to
+    libjgs2hip.so beside it. Skipped where Node's headers are absent."""
+    if not (NODE_INCLUDE / "node_api.h").exists():
+        print(f"N-API headers not found under {NODE_INCLUDE}; Node binding not built", flush=True)
+        return None
+    tmp = NAPI_ADDON.with_suffix(".node.tmp")
+    _run([CXX, "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-Wextra", f"-I{NODE_INCLUDE}",
+          f"-I{REPO_ROOT / 'include'}", NAPI_SOURCE, "-o", tmp, f"-L{LIB_DIR}", "-ljgs2hip",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, NAPI_ADDON)
+    return NAPI_ADDON
 
 
 if __name__ == "__main__":
