@@ -386,10 +386,16 @@ jgs2_status jgs2_batch_step_frames_exact_iterations(jgs2_batch* batch, uint32_t 
                                                     const jgs2_settings* overrides, uint64_t override_mask);
 jgs2_status jgs2_batch_sync(jgs2_batch* batch);
 jgs2_status jgs2_batch_get_info(jgs2_batch* batch, jgs2_batch_info* out);
-/* Records each distinct frame once as a hipGraph and replays it; the graph
- * is rebuilt after admission, retirement, objective changes or different
- * step settings. */
+/* Records each distinct frame once as a hipGraph and replays it. After an
+ * admission, retirement, objective change or different step settings the
+ * frame is captured again; when only extents and arguments changed the
+ * capture updates the existing exec in place (hipGraphExecUpdate), and
+ * otherwise the exec is rebuilt. */
 jgs2_status jgs2_batch_use_graphs(jgs2_batch* batch, int enable);
+/* How many frame graphs were instantiated from scratch and how many
+ * captures updated the existing exec in place (no counterpart in the
+ * reference). */
+jgs2_status jgs2_batch_graph_stats(jgs2_batch* batch, uint64_t* instantiations, uint64_t* updates);
 /* Launch shape of the local solve: 64, 128 or 256 threads per block, the
  * XCD-aware block order on or off, and 1, 4, 8 or 16 lanes per vertex
  * (default 4). Results do not depend on the block size or block order; the

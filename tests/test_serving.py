@@ -115,17 +115,20 @@ def test_continuous_batching_admits_fifo_as_slots_free_up():
     server = CounterServer(slots=2)
     rep = server.run(reqs)
     steps = [e[1] for e in server.log if e[0] == "step"]
-    # Tick 0: requests 0 and 1; request 2 (arrived at 0.5) waits for a slot,
-    # joins when request 1 finishes; request 3 arrives at 4 into a free slot.
-    assert steps == [[0, 1], [0, 2], [0, 2], [3]]
+    # Tick 0: an idle server admits requests 0 and 1, then steps them.
+    # Request 2 (arrived at 0.5) is admitted during tick 1, once request 1
+    # has finished, while request 0's frame is in flight, and joins from the
+    # next frame; request 3 arrives at 4 into an idle server.
+    assert steps == [[0, 1], [0], [0, 2], [2], [3]]
     assert [r.id for r in rep.completed] == [1, 0, 2, 3]
     done = {r.id: r for r in rep.completed}
-    assert done[2].admitted == 1.0 and done[2].finished == 3.0
+    assert done[2].admitted == 1.0 and done[2].finished == 4.0
     assert done[3].admitted == 4.0 and done[3].finished == 5.0
-    assert rep.vertex_updates == 2 * (10 + 20 + 10 + 30 + 10 + 30 + 40)
+    assert rep.vertex_updates == 2 * (10 + 20 + 10 + 10 + 30 + 30 + 40)
     s = rep.summary()
-    assert s["requests_completed"] == 4 and s["ticks"] == 4
+    assert s["requests_completed"] == 4 and s["ticks"] == 5 and s["admissions"] == 4
     assert s["queueing_s"]["p99"] == 0.5
+    assert s["resident_s"]["p50"] == 1.0 and s["resident_s"]["p99"] == 3.0
     assert all(r.result is not None for r in rep.completed)
 
 

@@ -174,9 +174,13 @@ struct DeviceState {
   //   bp_list   uint32[]   arena blocks of broad-phase scenes: bp_nv vertex
   //                        blocks, bp_ns surface-triangle blocks, bp_ne edge
   //                        blocks, bp_nc candidate blocks
+  //   bp_work   uint32[]   per bp_list segment, the positions of the blocks
+  //                        whose scene rebuilds at the current check point
+  //   bp_work_n uint32[4]  their counts
   uint32_t NS = 0, NE = 0;
   uint32_t bp_nv = 0, bp_ns = 0, bp_ne = 0, bp_nc = 0;
   uint32_t* bp_list = nullptr;
+  uint32_t *bp_work = nullptr, *bp_work_n = nullptr;
   uint4* bp_size = nullptr;
   uint32_t* bp_rows_at = nullptr;  // per scene: first entry of its cand_ids / active_ids rows
   uint32_t *bp_blk_v = nullptr, *bp_blk_e = nullptr, *bp_blk_r = nullptr;

@@ -198,6 +198,8 @@ struct jgs2_batch {
   hipGraphExec_t graph_exec = nullptr;
   uint64_t graph_key = 0;
   bool graph_valid = false;
+  uint64_t graph_instantiations = 0;  // execs built from scratch
+  uint64_t graph_updates = 0;         // captures applied to the existing exec in place
   // Local-solve profiling
   bool profiling = false;
   std::vector<hipEvent_t> events;
@@ -220,7 +222,7 @@ void free_device(DeviceState& d) {
                   d.cscratch, d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
-                  d.body_rng, d.bp_list, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
+                  d.body_rng, d.bp_list, d.bp_work, d.bp_work_n, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
                   d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_vprev, d.bp_eprev, d.bp_x,
                   d.bp_cpose, d.bp_tval, d.cand_key, d.prev_key, d.prev_lag_n, d.prev_lag_w, d.prev_src_c, d.prev_src_w,
                   d.bp_rowcnt, d.bp_rowfill, d.bp_ctl, d.bp_reach, d.bp_motion, d.bp_bodies, d.bp_bref, d.bp_bdev, d.bp_bbox, d.sblock_scene, d.eblock_scene,
@@ -819,6 +821,8 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
     TRYS(dalloc(&d.sblock_scene, Sc / kSceneBlock, by, dry));
     TRYS(dalloc(&d.eblock_scene, Ec / kSceneBlock, by, dry));
     TRYS(dalloc(&d.bp_list, (Vc + Sc + Ec + Cc) / kSceneBlock, by, dry));
+    TRYS(dalloc(&d.bp_work, (Vc + Sc + Ec + Cc) / kSceneBlock, by, dry));
+    TRYS(dalloc(&d.bp_work_n, 4, by, dry));
   }
   if (dry) return JGS2_OK;
   // Free blocks belong to no scene; scene slots start inactive.
@@ -1435,21 +1439,38 @@ jgs2_status submit(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
       key = fnv(key, &f, sizeof(f));
     }
     if (!b->graph_valid || key != b->graph_key) {
-      if (b->graph_exec) {
-        // Earlier launches of this exec may still be queued on the stream:
-        // it is destroyed only once they have completed.
-        HIP_CHECK(hipStreamSynchronize(b->stream));
-        (void)hipGraphExecDestroy(b->graph_exec);
-      }
-      b->graph_exec = nullptr;
+      // Earlier launches of the exec may still be queued on the stream: it
+      // is changed or destroyed only once they have completed.
+      if (b->graph_exec) HIP_CHECK(hipStreamSynchronize(b->stream));
       b->graph_valid = false;
       hipGraph_t graph = nullptr;
       HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
       enqueue_frame(b, st, iterations, stop);
       HIP_CHECK(hipStreamEndCapture(b->stream, &graph));
-      hipError_t e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+      // A mix that changed only in extents and arguments (an admission or
+      // retirement of scenes like the resident ones) keeps the pass
+      // sequence: the new capture is applied to the existing exec in place.
+      // Any other change (a kernel variant, an iteration count, a color
+      // count) fails the update, and the exec is rebuilt.
+      bool updated = false;
+      if (b->graph_exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult why = hipGraphExecUpdateSuccess;
+        updated = hipGraphExecUpdate(b->graph_exec, graph, &bad, &why) == hipSuccess && why == hipGraphExecUpdateSuccess;
+        (void)hipGetLastError();
+        if (!updated) {
+          (void)hipGraphExecDestroy(b->graph_exec);
+          b->graph_exec = nullptr;
+        }
+      }
+      hipError_t e = hipSuccess;
+      if (!updated) e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
-      if (e != hipSuccess) return fail(JGS2_ERROR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+      if (e != hipSuccess) {
+        b->graph_exec = nullptr;
+        return fail(JGS2_ERROR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+      }
+      (updated ? b->graph_updates : b->graph_instantiations) += 1;
       b->graph_key = key;
       b->graph_valid = true;
     }
@@ -2770,6 +2791,15 @@ jgs2_status jgs2_batch_use_graphs(jgs2_batch* b, int enable) {
   TRYS(check_batch(b));
   b->use_graphs = enable != 0;
   b->graph_valid = false;
+  clear_error();
+  return JGS2_OK;
+}
+
+jgs2_status jgs2_batch_graph_stats(jgs2_batch* b, uint64_t* instantiations, uint64_t* updates) {
+  TRYS(check_batch(b));
+  if (!instantiations || !updates) return fail(JGS2_ERROR_TYPE, "NULL argument.");
+  *instantiations = b->graph_instantiations;
+  *updates = b->graph_updates;
   clear_error();
   return JGS2_OK;
 }

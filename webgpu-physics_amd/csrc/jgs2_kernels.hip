@@ -13,6 +13,7 @@
 // neighbors a vertex gathers share one XCD's L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -3347,6 +3348,31 @@ __device__ __forceinline__ float bp_lower_bound(f3 lo, f3 hi, f3 lo2, f3 hi2) {
 __device__ __forceinline__ bool bp_building(const DeviceState& s, uint32_t sc) {
   return sc != kNoScene && (s.bp_ctl[sc].x & (kBpScene | kBpNeed)) == (kBpScene | kBpNeed);
 }
+// Work lists of a build. bp_list has four segments (vertex, surface
+// triangle, surface edge and candidate blocks of every broad-phase scene);
+// after the rebuild decision, bp_work holds, per segment, the positions in
+// that segment of the blocks whose scene rebuilds now (any order), and
+// bp_work_n their counts. Every build pass loops over its segment's list
+// with a bounded grid, so a check point at which no scene rebuilds costs one
+// counter read per workgroup instead of a launch over every listed block.
+enum : uint32_t { kBpSegV = 0, kBpSegS = 1, kBpSegE = 2, kBpSegC = 3 };
+__device__ __forceinline__ uint32_t bp_seg_base(const DeviceState& s, uint32_t seg) {
+  return seg == kBpSegV ? 0u : seg == kBpSegS ? s.bp_nv : seg == kBpSegE ? s.bp_nv + s.bp_ns : s.bp_nv + s.bp_ns + s.bp_ne;
+}
+// Calls f(item) for the work of segment `seg`, `per` items per listed
+// block; item = (position in the segment) * per + sub-index, the block
+// index the pass would have had in a launch over the whole segment. All
+// threads of a workgroup visit the same items; LDS is free again between
+// items.
+template <class F>
+__device__ __forceinline__ void bp_for_work(const DeviceState& s, uint32_t seg, uint32_t per, F&& f) {
+  const uint32_t n = s.bp_work_n[seg] * per;
+  const uint32_t* list = s.bp_work + bp_seg_base(s, seg);
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    f(list[w / per] * per + w % per);
+    __syncthreads();
+  }
+}
 // A point the reach argument does not cover: non-finite, or a pin whose
 // snap to rest (S:2997-3003) may exceed half the step bound.
 __device__ __forceinline__ bool bp_wild(const DeviceState& s, const PassParams& p, uint32_t v, f3 x) {
@@ -3475,6 +3501,7 @@ __device__ bool bp_bodies_keep(const DeviceState& s, const PassParams& p, uint32
 // the frame start (the pairs with lagged friction).
 template <bool kCheck>
 __global__ void __launch_bounds__(kBlock) k_bp_decide(DeviceState s, PassParams p, uint32_t carry) {
+  if (blockIdx.x == 0 && threadIdx.x < 4u) s.bp_work_n[threadIdx.x] = 0u;  // k_bp_worklist counts after
   const uint32_t sc = blockIdx.x * kBlock + threadIdx.x;
   if (sc >= p.scene_count) return;
   uint4 ctl = s.bp_ctl[sc];
@@ -3521,6 +3548,33 @@ __global__ void __launch_bounds__(kBlock) k_bp_decide(DeviceState s, PassParams 
   }
   s.bp_ctl[sc] = ctl;
 }
+// The work lists of the build the decision above asked for.
+__global__ void __launch_bounds__(kBlock) k_bp_worklist(DeviceState s) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t total = s.bp_nv + s.bp_ns + s.bp_ne + s.bp_nc;
+  uint32_t seg = kBpSegC;
+  bool take = false;
+  if (i < total) {
+    seg = i < s.bp_nv ? kBpSegV : i < s.bp_nv + s.bp_ns ? kBpSegS : i < s.bp_nv + s.bp_ns + s.bp_ne ? kBpSegE : kBpSegC;
+    const uint32_t* table = seg == kBpSegV ? s.vblock_scene
+                            : seg == kBpSegS ? s.sblock_scene
+                            : seg == kBpSegE ? s.eblock_scene
+                                             : s.cblock_scene;
+    take = bp_building(s, table[s.bp_list[i]]);
+  }
+  const uint32_t lane = threadIdx.x % 64u;
+  for (uint32_t g = 0; g < 4u; ++g) {
+    const bool mine = take && seg == g;
+    const uint64_t m = __ballot(mine);
+    if (!m) continue;
+    const uint32_t leader = uint32_t(__builtin_ctzll(m));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(s.bp_work_n + g, uint32_t(__popcll(m)));
+    base = uint32_t(__shfl(int(base), int(leader)));
+    const uint32_t first = bp_seg_base(s, g);
+    if (mine) s.bp_work[first + base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)))] = i - first;
+  }
+}
 // Triangle and edge AABBs at the source pose, and the AABB of each run of
 // 64 consecutive primitives (w = 1 when a point is wild).
 __device__ __forceinline__ void bp_chunk_box(f3 lo, f3 hi, float wild, float4* chunk) {
@@ -3539,9 +3593,9 @@ __device__ __forceinline__ void bp_chunk_box(f3 lo, f3 hi, float wild, float4* c
     chunk[1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
   }
 }
-__global__ void __launch_bounds__(kBlock) k_bp_bounds(DeviceState s, PassParams p) {
-  const bool tris = blockIdx.x < s.bp_ns;
-  const uint32_t blk = s.bp_list[s.bp_nv + blockIdx.x];
+__device__ void bp_bounds_item(const DeviceState& s, const PassParams& p, uint32_t item) {
+  const bool tris = item < s.bp_ns;
+  const uint32_t blk = s.bp_list[s.bp_nv + item];
   const uint32_t sc = tris ? s.sblock_scene[blk] : s.eblock_scene[blk];
   if (!bp_building(s, sc)) return;
   const uint32_t i = blk * kSceneBlock + threadIdx.x;
@@ -3571,18 +3625,22 @@ __global__ void __launch_bounds__(kBlock) k_bp_bounds(DeviceState s, PassParams 
   }
   bp_chunk_box(lo, hi, wild, (tris ? s.bp_tchunk : s.bp_echunk) + 2 * size_t(i / 64u));
 }
+__global__ void __launch_bounds__(kBlock) k_bp_bounds(DeviceState s, PassParams p) {
+  bp_for_work(s, kBpSegS, 1u, [&](uint32_t item) { bp_bounds_item(s, p, item); });
+  bp_for_work(s, kBpSegE, 1u, [&](uint32_t item) { bp_bounds_item(s, p, s.bp_ns + item); });
+}
 // One wavefront per query: a vertex (VT, canonical vertex order) or an edge
 // (EE). Lanes test 64 chunk boxes at a time; the surviving chunks are
 // visited in order and each lane tests one primitive, so matches come out
 // in canonical order and a ballot gives each its slot. The count pass
 // stores the number of matches of each query, the write pass the records.
 template <bool kWrite>
-__global__ void __launch_bounds__(kBlock) k_bp_vt(DeviceState s, PassParams p) {
-  const uint32_t vb = s.bp_list[blockIdx.x / 64u];
+__device__ void bp_vt_item(const DeviceState& s, const PassParams& p, uint32_t item) {
+  const uint32_t vb = s.bp_list[item / 64u];
   const uint32_t sc = s.vblock_scene[vb];
   if (!bp_building(s, sc)) return;
   const uint32_t lane = threadIdx.x % 64u;
-  const uint32_t qa = vb * kSceneBlock + (blockIdx.x % 64u) * 4u + threadIdx.x / 64u;
+  const uint32_t qa = vb * kSceneBlock + (item % 64u) * 4u + threadIdx.x / 64u;
   const uint32_t q = qa - s.scene_blk0[sc].x * kSceneBlock;
   if (q >= s.scene_cnt[sc].x) {  // padding: no matches (uniform per wavefront)
     if (!kWrite && lane == 0u) s.bp_vcnt[qa] = 0u;
@@ -3640,12 +3698,16 @@ __global__ void __launch_bounds__(kBlock) k_bp_vt(DeviceState s, PassParams p) {
   if (!kWrite && lane == 0u) s.bp_vcnt[qa] = n;
 }
 template <bool kWrite>
-__global__ void __launch_bounds__(kBlock) k_bp_ee(DeviceState s, PassParams p) {
-  const uint32_t eb = s.bp_list[s.bp_nv + s.bp_ns + blockIdx.x / 64u];
+__global__ void __launch_bounds__(kBlock) k_bp_vt(DeviceState s, PassParams p) {
+  bp_for_work(s, kBpSegV, 64u, [&](uint32_t item) { bp_vt_item<kWrite>(s, p, item); });
+}
+template <bool kWrite>
+__device__ void bp_ee_item(const DeviceState& s, const PassParams& p, uint32_t item) {
+  const uint32_t eb = s.bp_list[s.bp_nv + s.bp_ns + item / 64u];
   const uint32_t sc = s.eblock_scene[eb];
   if (!bp_building(s, sc)) return;
   const uint32_t lane = threadIdx.x % 64u;
-  const uint32_t ea = eb * kSceneBlock + (blockIdx.x % 64u) * 4u + threadIdx.x / 64u;
+  const uint32_t ea = eb * kSceneBlock + (item % 64u) * 4u + threadIdx.x / 64u;
   const uint32_t e0 = s.scene_blk2[sc].z * kSceneBlock;
   const uint32_t i = ea - e0;
   const uint4 sz = s.bp_size[sc];
@@ -3709,6 +3771,10 @@ __global__ void __launch_bounds__(kBlock) k_bp_ee(DeviceState s, PassParams p) {
   }
   if (!kWrite && lane == 0u) s.bp_ecnt[ea] = n;
 }
+template <bool kWrite>
+__global__ void __launch_bounds__(kBlock) k_bp_ee(DeviceState s, PassParams p) {
+  bp_for_work(s, kBpSegE, 64u, [&](uint32_t item) { bp_ee_item<kWrite>(s, p, item); });
+}
 // Scans: block sums of a count array over the listed blocks, one serial pass
 // per scene over its blocks in order, then exclusive offsets in place.
 enum : uint32_t { kBpScanVt = 0, kBpScanEe = 1, kBpScanRows = 2 };
@@ -3749,12 +3815,14 @@ __device__ __forceinline__ uint32_t bp_list_block(const DeviceState& s, uint32_t
 }
 __global__ void __launch_bounds__(kBlock) k_bp_block_sums(DeviceState s, uint32_t which) {
   __shared__ uint32_t sh[kBlock];
-  const uint32_t blk = bp_list_block(s, which, blockIdx.x);
   const BpScan t = bp_scan_target(s, which);
-  if (!bp_building(s, t.scene_of[blk])) return;
-  const uint32_t v = t.data[blk * kSceneBlock + threadIdx.x];
-  const uint32_t sum = block_sum_u(v, sh);
-  if (threadIdx.x == 0) t.blk[blk] = sum;
+  bp_for_work(s, which == kBpScanEe ? kBpSegE : kBpSegV, 1u, [&](uint32_t item) {
+    const uint32_t blk = bp_list_block(s, which, item);
+    if (!bp_building(s, t.scene_of[blk])) return;  // uniform per block
+    const uint32_t v = t.data[blk * kSceneBlock + threadIdx.x];
+    const uint32_t sum = block_sum_u(v, sh);
+    if (threadIdx.x == 0) t.blk[blk] = sum;
+  });
 }
 // One thread per scene. VT offsets run over the vertex blocks, EE offsets
 // continue after the VT total; the row scan runs over the vertex blocks.
@@ -3795,33 +3863,39 @@ __global__ void __launch_bounds__(kBlock) k_bp_scene_scan(DeviceState s, PassPar
 }
 __global__ void __launch_bounds__(kBlock) k_bp_scan_apply(DeviceState s, uint32_t which) {
   __shared__ uint32_t sh[kBlock];
-  const uint32_t blk = bp_list_block(s, which, blockIdx.x);
   const BpScan t = bp_scan_target(s, which);
-  if (!bp_building(s, t.scene_of[blk])) return;
-  const size_t i = size_t(blk) * kSceneBlock + threadIdx.x;
-  const uint32_t v = t.data[i];
-  t.data[i] = t.blk[blk] + block_exclusive_u(v, sh);
+  bp_for_work(s, which == kBpScanEe ? kBpSegE : kBpSegV, 1u, [&](uint32_t item) {
+    const uint32_t blk = bp_list_block(s, which, item);
+    if (!bp_building(s, t.scene_of[blk])) return;  // uniform per block
+    const size_t i = size_t(blk) * kSceneBlock + threadIdx.x;
+    const uint32_t v = t.data[i];
+    t.data[i] = t.blk[blk] + block_exclusive_u(v, sh);
+  });
 }
 // The list before a rebuild, for lagged-friction look-ups.
 __global__ void __launch_bounds__(kBlock) k_bp_save_prev(DeviceState s) {
-  const uint32_t blk = s.bp_list[s.bp_nv + s.bp_ns + s.bp_ne + blockIdx.x];
-  const uint32_t sc = s.cblock_scene[blk];
-  if (!bp_building(s, sc)) return;
-  const uint32_t c = blk * kSceneBlock + threadIdx.x;
-  if (c - s.scene_blk1[sc].z * kSceneBlock >= s.bp_size[sc].w) return;
-  s.prev_key[c] = s.cand_key[c];
-  s.prev_lag_n[c] = s.lag_n[c];
-  s.prev_lag_w[c] = s.lag_w[c];
-  s.prev_src_c[c] = s.src_c[c];
-  s.prev_src_w[c] = s.src_w[c];
+  bp_for_work(s, kBpSegC, 1u, [&](uint32_t item) {
+    const uint32_t blk = s.bp_list[s.bp_nv + s.bp_ns + s.bp_ne + item];
+    const uint32_t sc = s.cblock_scene[blk];
+    if (!bp_building(s, sc)) return;
+    const uint32_t c = blk * kSceneBlock + threadIdx.x;
+    if (c - s.scene_blk1[sc].z * kSceneBlock >= s.bp_size[sc].w) return;
+    s.prev_key[c] = s.cand_key[c];
+    s.prev_lag_n[c] = s.lag_n[c];
+    s.prev_lag_w[c] = s.lag_w[c];
+    s.prev_src_c[c] = s.src_c[c];
+    s.prev_src_w[c] = s.src_w[c];
+  });
 }
 // Vertex -> candidate rows: counts, offsets (scan), fill, sort.
 __global__ void __launch_bounds__(kBlock) k_bp_row_zero(DeviceState s) {
-  const uint32_t vb = s.bp_list[blockIdx.x];
-  if (!bp_building(s, s.vblock_scene[vb])) return;
-  const uint32_t v = vb * kSceneBlock + threadIdx.x;
-  s.bp_rowcnt[v] = 0u;
-  s.bp_rowfill[v] = 0u;
+  bp_for_work(s, kBpSegV, 1u, [&](uint32_t item) {
+    const uint32_t vb = s.bp_list[item];
+    if (!bp_building(s, s.vblock_scene[vb])) return;
+    const uint32_t v = vb * kSceneBlock + threadIdx.x;
+    s.bp_rowcnt[v] = 0u;
+    s.bp_rowfill[v] = 0u;
+  });
 }
 // Adds one to counters[key] for every active lane, lanes of the wavefront
 // with equal keys combined into one atomic; returns each lane's value
@@ -3846,8 +3920,8 @@ __device__ __forceinline__ uint32_t wave_counted_add(uint32_t* counters, uint32_
 // candidate. Consecutive candidates share their first vertex (the VT point,
 // the first EE edge), so those increments are combined per wavefront.
 template <bool kFill>
-__global__ void __launch_bounds__(kBlock) k_bp_rows(DeviceState s, PassParams p, uint32_t carry) {
-  const uint32_t blk = s.bp_list[s.bp_nv + s.bp_ns + s.bp_ne + blockIdx.x];
+__device__ void bp_rows_item(const DeviceState& s, const PassParams& p, uint32_t carry, uint32_t item) {
+  const uint32_t blk = s.bp_list[s.bp_nv + s.bp_ns + s.bp_ne + item];
   const uint32_t sc = s.cblock_scene[blk];
   if (!bp_building(s, sc)) return;  // uniform per block
   const uint32_t c = blk * kSceneBlock + threadIdx.x;
@@ -3906,28 +3980,34 @@ __global__ void __launch_bounds__(kBlock) k_bp_rows(DeviceState s, PassParams p,
   }
   ipc_store_summary(s, c);
 }
+template <bool kFill>
+__global__ void __launch_bounds__(kBlock) k_bp_rows(DeviceState s, PassParams p, uint32_t carry) {
+  bp_for_work(s, kBpSegC, 1u, [&](uint32_t item) { bp_rows_item<kFill>(s, p, carry, item); });
+}
 __global__ void __launch_bounds__(kBlock) k_bp_row_offsets(DeviceState s) {
-  const uint32_t vb = s.bp_list[blockIdx.x];
-  const uint32_t sc = s.vblock_scene[vb];
-  if (!bp_building(s, sc)) return;
-  const uint32_t v = vb * kSceneBlock + threadIdx.x;
-  // bp_rowcnt holds exclusive offsets after the scan; the count is the
-  // difference to the next vertex (or to four times the list length).
-  const uint32_t end = 4u * s.bp_ctl[sc].z;
-  const uint32_t last = s.scene_blk0[sc].y * kSceneBlock;
-  const uint32_t at = s.bp_rowcnt[v];
-  const uint32_t next = v + 1u < last ? s.bp_rowcnt[v + 1u] : end;
-  s.cand_row[v] = make_uint2(s.bp_rows_at[sc] + at, next - at);
+  bp_for_work(s, kBpSegV, 1u, [&](uint32_t item) {
+    const uint32_t vb = s.bp_list[item];
+    const uint32_t sc = s.vblock_scene[vb];
+    if (!bp_building(s, sc)) return;
+    const uint32_t v = vb * kSceneBlock + threadIdx.x;
+    // bp_rowcnt holds exclusive offsets after the scan; the count is the
+    // difference to the next vertex (or to four times the list length).
+    const uint32_t end = 4u * s.bp_ctl[sc].z;
+    const uint32_t last = s.scene_blk0[sc].y * kSceneBlock;
+    const uint32_t at = s.bp_rowcnt[v];
+    const uint32_t next = v + 1u < last ? s.bp_rowcnt[v + 1u] : end;
+    s.cand_row[v] = make_uint2(s.bp_rows_at[sc] + at, next - at);
+  });
 }
 constexpr uint32_t kBpMaxRow = 8192;
+constexpr uint64_t kBpWorkGrid = 16384;  // workgroups of a work-list pass
 // One block per vertex: bitonic sort of its row in LDS (ascending ids, the
 // order the local solve sums contact rows in).
-__global__ void __launch_bounds__(kBlock) k_bp_row_sort(DeviceState s) {
-  __shared__ uint32_t keys[kBpMaxRow];
-  const uint32_t vb = s.bp_list[blockIdx.x / kSceneBlock];
+__device__ void bp_row_sort_item(const DeviceState& s, uint32_t* keys, uint32_t item) {
+  const uint32_t vb = s.bp_list[item / kSceneBlock];
   const uint32_t sc = s.vblock_scene[vb];
   if (!bp_building(s, sc)) return;
-  const uint32_t v = vb * kSceneBlock + blockIdx.x % kSceneBlock;
+  const uint32_t v = vb * kSceneBlock + item % kSceneBlock;
   const uint2 row = s.cand_row[v];
   const uint32_t n = row.y;
   if (n < 2u) return;
@@ -3956,11 +4036,15 @@ __global__ void __launch_bounds__(kBlock) k_bp_row_sort(DeviceState s) {
   }
   for (uint32_t i = threadIdx.x; i < n; i += kBlock) s.cand_ids[row.x + i] = keys[i];
 }
+__global__ void __launch_bounds__(kBlock) k_bp_row_sort(DeviceState s) {
+  __shared__ uint32_t keys[kBpMaxRow];
+  bp_for_work(s, kBpSegV, kSceneBlock, [&](uint32_t item) { bp_row_sort_item(s, keys, item); });
+}
 // The build pose of every rebuilt scene, and its query offsets (the ranges
 // a later rebuild inside a frame searches this list in).
-__global__ void __launch_bounds__(kBlock) k_bp_commit(DeviceState s, PassParams p) {
-  const bool vertices = blockIdx.x < s.bp_nv;
-  const uint32_t blk = s.bp_list[vertices ? blockIdx.x : s.bp_ns + blockIdx.x];
+__device__ void bp_commit_item(const DeviceState& s, const PassParams& p, uint32_t item) {
+  const bool vertices = item < s.bp_nv;
+  const uint32_t blk = s.bp_list[vertices ? item : s.bp_ns + item];
   const uint32_t i = blk * kSceneBlock + threadIdx.x;
   if (vertices) {
     const uint32_t sc = s.vblock_scene[blk];
@@ -3983,6 +4067,10 @@ __global__ void __launch_bounds__(kBlock) k_bp_commit(DeviceState s, PassParams 
     if (!bp_building(s, s.eblock_scene[blk])) return;
     s.bp_eprev[i] = s.bp_ecnt[i];
   }
+}
+__global__ void __launch_bounds__(kBlock) k_bp_commit(DeviceState s, PassParams p) {
+  bp_for_work(s, kBpSegV, 1u, [&](uint32_t item) { bp_commit_item(s, p, item); });
+  bp_for_work(s, kBpSegE, 1u, [&](uint32_t item) { bp_commit_item(s, p, s.bp_nv + item); });
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
@@ -4205,26 +4293,32 @@ void launch_broad_phase(const DeviceState& s, const PassParams& p, bool check, b
   } else {
     hipLaunchKernelGGL(k_bp_decide<false>, scenes, B, 0, st, s, p, cr);
   }
-  if (s.bp_ns + s.bp_ne) hipLaunchKernelGGL(k_bp_bounds, dim3(s.bp_ns + s.bp_ne), B, 0, st, s, p);
-  hipLaunchKernelGGL(k_bp_vt<false>, dim3(64 * s.bp_nv), B, 0, st, s, p);
-  if (s.bp_ne) hipLaunchKernelGGL(k_bp_ee<false>, dim3(64 * s.bp_ne), B, 0, st, s, p);
-  hipLaunchKernelGGL(k_bp_block_sums, dim3(s.bp_nv), B, 0, st, s, uint32_t(kBpScanVt));
-  if (s.bp_ne) hipLaunchKernelGGL(k_bp_block_sums, dim3(s.bp_ne), B, 0, st, s, uint32_t(kBpScanEe));
+  hipLaunchKernelGGL(k_bp_worklist, dim3(blocks_for(s.bp_nv + s.bp_ns + s.bp_ne + s.bp_nc)), B, 0, st, s);
+  // The build passes loop over the work lists; a grid of at most
+  // kBpWorkGrid workgroups keeps a check point without rebuilds cheap and
+  // still fills the chip when every scene rebuilds.
+  auto grid = [](uint64_t items) { return dim3(uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(items, kBpWorkGrid)))); };
+  const uint64_t nv = s.bp_nv, ns = s.bp_ns, ne = s.bp_ne, nc = s.bp_nc;
+  if (ns + ne) hipLaunchKernelGGL(k_bp_bounds, grid(ns + ne), B, 0, st, s, p);
+  hipLaunchKernelGGL(k_bp_vt<false>, grid(64 * nv), B, 0, st, s, p);
+  if (ne) hipLaunchKernelGGL(k_bp_ee<false>, grid(64 * ne), B, 0, st, s, p);
+  hipLaunchKernelGGL(k_bp_block_sums, grid(nv), B, 0, st, s, uint32_t(kBpScanVt));
+  if (ne) hipLaunchKernelGGL(k_bp_block_sums, grid(ne), B, 0, st, s, uint32_t(kBpScanEe));
   hipLaunchKernelGGL(k_bp_scene_scan, scenes, B, 0, st, s, p, 0u);
-  hipLaunchKernelGGL(k_bp_scan_apply, dim3(s.bp_nv), B, 0, st, s, uint32_t(kBpScanVt));
-  if (s.bp_ne) hipLaunchKernelGGL(k_bp_scan_apply, dim3(s.bp_ne), B, 0, st, s, uint32_t(kBpScanEe));
-  if (carry && s.bp_nc) hipLaunchKernelGGL(k_bp_save_prev, dim3(s.bp_nc), B, 0, st, s);
-  hipLaunchKernelGGL(k_bp_vt<true>, dim3(64 * s.bp_nv), B, 0, st, s, p);
-  if (s.bp_ne) hipLaunchKernelGGL(k_bp_ee<true>, dim3(64 * s.bp_ne), B, 0, st, s, p);
-  hipLaunchKernelGGL(k_bp_row_zero, dim3(s.bp_nv), B, 0, st, s);
-  if (s.bp_nc) hipLaunchKernelGGL(k_bp_rows<false>, dim3(s.bp_nc), B, 0, st, s, p, 0u);
-  hipLaunchKernelGGL(k_bp_block_sums, dim3(s.bp_nv), B, 0, st, s, uint32_t(kBpScanRows));
+  hipLaunchKernelGGL(k_bp_scan_apply, grid(nv), B, 0, st, s, uint32_t(kBpScanVt));
+  if (ne) hipLaunchKernelGGL(k_bp_scan_apply, grid(ne), B, 0, st, s, uint32_t(kBpScanEe));
+  if (carry && nc) hipLaunchKernelGGL(k_bp_save_prev, grid(nc), B, 0, st, s);
+  hipLaunchKernelGGL(k_bp_vt<true>, grid(64 * nv), B, 0, st, s, p);
+  if (ne) hipLaunchKernelGGL(k_bp_ee<true>, grid(64 * ne), B, 0, st, s, p);
+  hipLaunchKernelGGL(k_bp_row_zero, grid(nv), B, 0, st, s);
+  if (nc) hipLaunchKernelGGL(k_bp_rows<false>, grid(nc), B, 0, st, s, p, 0u);
+  hipLaunchKernelGGL(k_bp_block_sums, grid(nv), B, 0, st, s, uint32_t(kBpScanRows));
   hipLaunchKernelGGL(k_bp_scene_scan, scenes, B, 0, st, s, p, 1u);
-  hipLaunchKernelGGL(k_bp_scan_apply, dim3(s.bp_nv), B, 0, st, s, uint32_t(kBpScanRows));
-  hipLaunchKernelGGL(k_bp_row_offsets, dim3(s.bp_nv), B, 0, st, s);
-  if (s.bp_nc) hipLaunchKernelGGL(k_bp_rows<true>, dim3(s.bp_nc), B, 0, st, s, p, cr);
-  hipLaunchKernelGGL(k_bp_row_sort, dim3(kSceneBlock * s.bp_nv), B, 0, st, s);
-  hipLaunchKernelGGL(k_bp_commit, dim3(s.bp_nv + s.bp_ne), B, 0, st, s, p);
+  hipLaunchKernelGGL(k_bp_scan_apply, grid(nv), B, 0, st, s, uint32_t(kBpScanRows));
+  hipLaunchKernelGGL(k_bp_row_offsets, grid(nv), B, 0, st, s);
+  if (nc) hipLaunchKernelGGL(k_bp_rows<true>, grid(nc), B, 0, st, s, p, cr);
+  hipLaunchKernelGGL(k_bp_row_sort, grid(uint64_t(kSceneBlock) * nv), B, 0, st, s);
+  hipLaunchKernelGGL(k_bp_commit, grid(nv + ne), B, 0, st, s, p);
 }
 
 

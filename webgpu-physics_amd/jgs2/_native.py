@@ -356,6 +356,7 @@ SIGNATURES = {
     "jgs2_batch_sync": (C.c_int, [C.c_void_p]),
     "jgs2_batch_get_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
     "jgs2_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "jgs2_batch_graph_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "jgs2_batch_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     "jgs2_batch_autotune": (
         C.c_int, [C.c_void_p, C.POINTER(Settings), C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),

@@ -180,6 +180,13 @@ class JGS2Batch:
     def use_graphs(self, enable: bool = True) -> None:
         N.check(N.lib.jgs2_batch_use_graphs(self._h(), 1 if enable else 0))
 
+    def graph_stats(self) -> dict:
+        """Frame graphs instantiated from scratch and captures applied to the
+        existing exec in place."""
+        inst, upd = C.c_uint64(), C.c_uint64()
+        N.check(N.lib.jgs2_batch_graph_stats(self._h(), C.byref(inst), C.byref(upd)))
+        return {"instantiations": int(inst.value), "updates": int(upd.value)}
+
     def set_launch(self, solve_block: int, xcd_remap: bool, solve_lanes: Optional[int] = None) -> None:
         lanes = self.info()["solve_lanes"] if solve_lanes is None else int(solve_lanes)
         N.check(N.lib.jgs2_batch_set_launch(self._h(), int(solve_block), 1 if xcd_remap else 0, lanes))

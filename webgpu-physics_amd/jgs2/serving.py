@@ -9,15 +9,22 @@ Two servers share one interface so their throughput and latency compare
 directly on the same request stream:
 
 * `BatchServer` -- one batch arena; each tick steps every resident scene by
-  one frame in a single submission (hipGraph-replayed when the mix has not
-  changed since the last tick).
+  one frame in a single submission (a hipGraph replay; a changed mix updates
+  the captured graph in place).
 * `PlainServer` -- one `JGS2GpuSolver` per request, each stepped by its own
   submission: the one-scene-per-launch path the reference offers.
 
+A tick launches the frame of the resident scenes and then admits the
+requests that have arrived while the device runs it: admission's host work
+(validation, coloring, packing, the contact capacity scan) overlaps the
+frame, and its uploads queue behind the frame on the device. A request
+admitted during a tick joins from the next frame.
+
 Latency is reported per tick (one frame of every resident scene, as a
-client sees it: enqueue to completion) and per request (arrival to last
-frame; queueing delay separately). Percentiles use the nearest-rank rule
-of the reference's metrics (src/performance/metrics.ts:42-46).
+client sees it: launch to completion) and per request: arrival to last
+frame, split into queueing (arrival to admission) and residency (admission
+to last frame). Percentiles use the nearest-rank rule of the reference's
+metrics (src/performance/metrics.ts:42-46).
 """
 
 from __future__ import annotations
@@ -128,10 +135,14 @@ class ServeReport:
     tick_ms: List[float] = field(default_factory=list)
     completed: List[Request] = field(default_factory=list)
     rejected_admissions: int = 0
+    admissions: int = 0
+    admission_s: float = 0.0  # host time spent admitting, overlapped with frames in flight
+    extra: dict = field(default_factory=dict)
 
     def summary(self) -> dict:
         lat = [r.finished - r.arrival for r in self.completed]
         wait = [r.admitted - r.arrival for r in self.completed]
+        resident = [r.finished - r.admitted for r in self.completed]
         return {
             "ticks": self.ticks,
             "wall_s": self.wall_s,
@@ -141,7 +152,11 @@ class ServeReport:
             "requests_completed": len(self.completed),
             "request_latency_s": {"p50": nearest_rank(lat, 50), "p99": nearest_rank(lat, 99)},
             "queueing_s": {"p50": nearest_rank(wait, 50), "p99": nearest_rank(wait, 99)},
+            "resident_s": {"p50": nearest_rank(resident, 50), "p99": nearest_rank(resident, 99)},
+            "admissions": self.admissions,
+            "admission_host_s": self.admission_s,
             "admissions_deferred": self.rejected_admissions,
+            **self.extra,
         }
 
 
@@ -163,18 +178,26 @@ class _Server:
     def now(self) -> float:
         return self.clock() - self._t0
 
-    # Subclasses: admit one request (None if it does not fit now), step all
-    # active handles one frame, release one handle.
+    # Subclasses: admit one request (None if it does not fit now), launch one
+    # frame of every active handle, wait for it, release one handle. A
+    # subclass whose steps complete synchronously implements `_step_all`.
     def _try_admit(self, request: Request):
         raise NotImplementedError
 
     def _step_all(self) -> None:
         raise NotImplementedError
 
+    def _launch_all(self) -> None:
+        self._step_all()
+
+    def _wait_all(self) -> None:
+        pass
+
     def _release(self, handle) -> None:
         raise NotImplementedError
 
     def _admit_arrivals(self) -> None:
+        t = time.perf_counter()
         while self.queue:
             request = self.queue[0]
             handle = self._try_admit(request)
@@ -185,20 +208,27 @@ class _Server:
             request.admitted = self.now()
             request.free_vertices = handle.free_vertex_count
             self.active.append([request, handle, request.frames])
+            self.report.admissions += 1
+        self.report.admission_s += time.perf_counter() - t
 
     def tick(self) -> bool:
-        self._admit_arrivals()
-        if not self.active:
-            return False
+        if not self.active:  # an idle device: nothing to overlap admission with
+            self._admit_arrivals()
+            if not self.active:
+                return False
+        stepped = list(self.active)
         t = self.clock()
-        self._step_all()
+        self._launch_all()
+        self._admit_arrivals()  # overlaps the frame in flight; joins from the next frame
+        self._wait_all()
         if isinstance(self.clock, TickClock):
             self.clock.step()
         self.report.tick_ms.append((self.clock() - t) * 1e3)
         self.report.ticks += 1
-        self.report.vertex_updates += self.iterations * sum(r.free_vertices for r, _, _ in self.active)
+        self.report.vertex_updates += self.iterations * sum(r.free_vertices for r, _, _ in stepped)
+        joined = self.active[len(stepped):]
         still = []
-        for entry in self.active:
+        for entry in stepped:
             entry[2] -= 1
             request, handle, left = entry
             if left == 0:
@@ -209,7 +239,7 @@ class _Server:
                 self.report.completed.append(request)
             else:
                 still.append(entry)
-        self.active = still
+        self.active = still + joined
         return True
 
     def run(self, requests: Sequence[Request], duration: Optional[float] = None) -> ServeReport:
@@ -259,12 +289,19 @@ class BatchServer(_Server):
         except N.JGS2MemoryError:
             return None
 
-    def _step_all(self) -> None:
+    def _launch_all(self) -> None:
         self.batch.step_frames_exact_iterations(1, self.iterations, self.settings)
+
+    def _wait_all(self) -> None:
         self.batch.sync()
 
     def _release(self, handle) -> None:
         self.batch.retire(handle)
+
+    def run(self, requests, duration=None) -> ServeReport:
+        report = super().run(requests, duration)
+        report.extra["graphs"] = self.batch.graph_stats()
+        return report
 
     def close(self) -> None:
         super().close()
@@ -297,10 +334,13 @@ class PlainServer(_Server):
         self._bytes[id(solver)] = need
         return solver
 
-    def _step_all(self) -> None:
-        for _, solver, _ in self.active:
+    def _launch_all(self) -> None:
+        self._stepped = [solver for _, solver, _ in self.active]
+        for solver in self._stepped:
             solver.step_exact_iterations(self.iterations, self.settings)
-        for _, solver, _ in self.active:
+
+    def _wait_all(self) -> None:
+        for solver in self._stepped:
             solver.await_idle()
 
     def _release(self, handle) -> None:
