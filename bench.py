@@ -102,6 +102,9 @@ def main(argv=None):
     ap.add_argument("--profile-precompute", default=None, metavar="CELLS:RINGS[:FRAMES]",
                     help="only the cubature precompute of the layered box, e.g. 32,32,64:1; with FRAMES, "
                          "then Jacobi frames on the trained records against the stand-in records")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving leg (for --serve-seconds long enough to complete many requests), "
+                         "printed as one JSON object, and exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args(argv)
@@ -113,6 +116,9 @@ def main(argv=None):
         return profile_contact(args.profile_contact)
     if args.profile_large:
         return profile_large([int(x) for x in args.profile_large.split(",")], args.large_cells, args.iterations)
+    if args.serve_only:
+        print(json.dumps({"serving": measure_serving(args.serve_seconds, args.seed)}), flush=True)
+        return 0
     if args.profile_precompute:
         parts = args.profile_precompute.split(":")
         frames = int(parts[2]) if len(parts) > 2 else 0
@@ -480,13 +486,16 @@ def measure_serving(seconds, seed):
            "offered_requests": int(arrivals.size), "hbm_budget_bytes": budget, "max_resident_scenes": cap.max_scenes}
     server = BatchServer(cap, settings, it, hbm_budget_bytes=budget, use_graphs=True)
     out["arena_bytes"] = server.arena_bytes
-    rep = server.run(stream(), duration=seconds)
+    def progress(label):
+        return lambda r, now: log(f"serving {label}: {now:.0f} s, {r.ticks} ticks, {len(r.completed)} completed")
+
+    rep = server.run(stream(), duration=seconds, progress=progress("batched"))
     server.close()
     out["batched"] = rep.summary()
     log(f"serving batched: {out['batched']['vertex_updates_per_s']:.4g} vertex-updates/s, "
         f"frame p50 {out['batched']['frame_ms']['p50']:.2f} ms")
     plain = PlainServer(settings, it, hbm_budget_bytes=budget, max_scenes=cap.max_scenes)
-    rep = plain.run(stream(), duration=seconds)
+    rep = plain.run(stream(), duration=seconds, progress=progress("plain"))
     plain.close()
     out["plain"] = rep.summary()
     log(f"serving plain: {out['plain']['vertex_updates_per_s']:.4g} vertex-updates/s, "

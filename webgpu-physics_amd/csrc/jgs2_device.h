@@ -61,13 +61,14 @@ struct PassParams {
 //                        CSR, each row ascending in element id
 //   IPC candidates (NC of them, the first NVT vertex-triangle):
 //   cand      uint4[NC]  vertex ids (device order)
-//   lag_n, lag_w, cscratch, src_c, src_w, tgt_c, tgt_w  float4[NC]: lagged
-//                        normal + force and weights, step scratch, cached
-//                        source/target contact (normal, distance) + weights
+//   lag_n, lag_w, src_c, src_w, tgt_c, tgt_w  float4[NC]: lagged normal +
+//                        force and weights, cached source/target contact
+//                        (normal, distance) + weights
 //   contact_ctl float4[1] w = the step's safe alpha
 //   cand_row uint2[V] {begin, count}, cand_ids uint32[4 NC]: vertex ->
 //   candidate CSR; active_count uint32[V], active_ids uint32[4 NC]: the
-//   active rows; cand_meta uint32[NC]: 1 for edge-edge records
+//   active rows; cand_meta uint32[NC]: bit 0 set for edge-edge records,
+//   bits 1-2 the record slot of the vertex owning the candidate's energy
 // Scenes: every scene owns whole 256-entry blocks of the vertex,
 // tetrahedron, triangle and candidate ranges (block tables *_scene map a
 // block to its scene, kNoScene for free blocks); per-scene control (4),
@@ -132,7 +133,7 @@ struct DeviceState {
   uint32_t NC = 0, NVT = 0;
   float ipc_dmin = 0, ipc_mu = 0, ipc_eps = 0, ipc_safety = 0;
   uint4* cand = nullptr;
-  float4 *lag_n = nullptr, *lag_w = nullptr, *cscratch = nullptr;
+  float4 *lag_n = nullptr, *lag_w = nullptr;
   float4 *src_c = nullptr, *src_w = nullptr, *tgt_c = nullptr, *tgt_w = nullptr;
   float2* src_d = nullptr;  // per candidate: cached source distance (-1 invalid), lagged force
   float* tgt_d = nullptr;   // per candidate: cached target distance (-1 invalid)

@@ -219,7 +219,7 @@ void free_device(DeviceState& d) {
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.diag_x, d.diag_xp, d.diag_buf, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
-                  d.cscratch, d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
+                  d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
                   d.body_rng, d.bp_list, d.bp_work, d.bp_work_n, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
@@ -762,7 +762,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.cand, Cc, by, dry));
   TRYS(dalloc(&d.cand_meta, Cc, by, dry));
   TRYS(dalloc(&d.cand_keep, Cc, by, dry));
-  for (float4** a : {&d.lag_n, &d.lag_w, &d.cscratch, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRYS(dalloc(a, Cc, by, dry));
+  for (float4** a : {&d.lag_n, &d.lag_w, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRYS(dalloc(a, Cc, by, dry));
   TRYS(dalloc(&d.src_d, Cc, by, dry));
   TRYS(dalloc(&d.tgt_d, Cc, by, dry));
   TRYS(dalloc(&d.cand_ids, 4 * Cc, by, dry));
@@ -1024,7 +1024,11 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
       const uint32_t* ix = &in.contact->packed_indices[4 * size_t(c)];
       cand[c] = make_uint4(uint32_t(VB) + vnew[ix[0]], uint32_t(VB) + vnew[ix[1]], uint32_t(VB) + vnew[ix[2]],
                            uint32_t(VB) + vnew[ix[3]]);
-      cand_meta[c] = c < s->NVT ? 0u : 1u;
+      // Bit 0: edge-edge; bits 1-2: the record slot of the energy owner,
+      // the first unpinned vertex, else the first (S:742-748).
+      uint32_t owner = 0;
+      while (owner < 4 && in.vertex_info[4 * size_t(ix[owner]) + 2] != 0u) ++owner;
+      cand_meta[c] = (c < s->NVT ? 0u : 1u) | ((owner < 4 ? owner : 0u) << 1);
     }
     TRYS(rows_of(in.contact->packed_indices, 4, NC, s->cel.at, CB, cand_row, cand_ids));
   }
@@ -1077,12 +1081,11 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
   UP(d.hinge_rest, s->hin.at, hinge_rest);
   UP(d.hinge_elems, s->hel.at, hinge_elems);
   if (NC > 0 && !s->bp) {  // packIpcContactBuffer initial records (ipc-contact-layout.ts:302-341)
-    std::vector<float4> z(NC, make_float4(0, 0, 0, 0)), scratch(NC, make_float4(1, 0, 0, 0)),
+    std::vector<float4> z(NC, make_float4(0, 0, 0, 0)),
         none(NC, make_float4(0, 0, 0, -1));
     UP(d.cand, CB, cand);
     UP(d.cand_meta, CB, cand_meta);
     for (float4* a : {d.lag_n, d.lag_w, d.src_w, d.tgt_w}) UP(a, CB, z);
-    UP(d.cscratch, CB, scratch);
     UP(d.src_c, CB, none);
     std::vector<float2> summary(NC, make_float2(-1, 0));
     UP(d.src_d, CB, summary);

@@ -831,7 +831,7 @@ __device__ __forceinline__ float ipc_aabb_lower_bound(bool vt, f3 p0, f3 p1, f3 
 __device__ Contact ipc_contact_at(const DeviceState& s, uint32_t c, const float4* pos, float threshold,
                                   uint32_t mv = kNoVertex, f3 mp = f3{0, 0, 0}) {
   const uint4 ix = s.cand[c];
-  const bool vt = s.cand_meta[c] == 0u;
+  const bool vt = (s.cand_meta[c] & 1u) == 0u;
   const f3 p0 = cloth_pos(pos, ix.x, mv, mp), p1 = cloth_pos(pos, ix.y, mv, mp);
   const f3 p2 = cloth_pos(pos, ix.z, mv, mp), p3 = cloth_pos(pos, ix.w, mv, mp);
   const float lb = ipc_aabb_lower_bound(vt, p0, p1, p2, p3);
@@ -893,12 +893,20 @@ __device__ float ipc_energy_from_contact(const DeviceState& s, const PassParams&
   }
   return e;
 }
-__device__ __forceinline__ uint32_t ipc_energy_owner(const DeviceState& s, uint4 ix) {  // S:742-748
-  if (s.vtx_info[ix.x].z == 0u) return ix.x;
-  if (s.vtx_info[ix.y].z == 0u) return ix.y;
-  if (s.vtx_info[ix.z].z == 0u) return ix.z;
-  if (s.vtx_info[ix.w].z == 0u) return ix.w;
-  return ix.x;
+// The record slot (0-3) of the vertex that owns a candidate's energy
+// (S:742-748: the first unpinned vertex, else the first). Pins are fixed at
+// admission, so the slot is computed when a record is written and kept in
+// bits 1-2 of cand_meta, beside the kind bit (bit 0: edge-edge).
+__device__ __forceinline__ uint32_t ipc_owner_slot(const DeviceState& s, uint4 ix) {
+  if (s.vtx_info[ix.x].z == 0u) return 0u;
+  if (s.vtx_info[ix.y].z == 0u) return 1u;
+  if (s.vtx_info[ix.z].z == 0u) return 2u;
+  if (s.vtx_info[ix.w].z == 0u) return 3u;
+  return 0u;
+}
+__device__ __forceinline__ uint32_t ipc_energy_owner(uint4 ix, uint32_t meta) {
+  const uint32_t slot = (meta >> 1) & 3u;
+  return slot == 0u ? ix.x : slot == 1u ? ix.y : slot == 2u ? ix.z : ix.w;
 }
 // Barrier and lagged-friction terms of one vertex (ipcLocalContribution,
 // S:786-874) over its active candidate row.
@@ -2320,9 +2328,9 @@ __global__ void __launch_bounds__(kBlock) k_ipc_energy_lanes(DeviceState s, Pass
         if (dq < 0.0f) {
           bad = true;
         } else if (ipc_barrier_active(dq, p.ipc_dhat) || (s.ipc_mu > 0.0f && q.y > 0.0f)) {
-          const Contact k = ipc_cached(s, c, use_target);
           const uint4 ix = s.cand[c];
-          if (ipc_energy_owner(s, ix) == v) {
+          if (ipc_energy_owner(ix, s.cand_meta[c]) == v) {  // the owner alone loads the cached contact
+            const Contact k = ipc_cached(s, c, use_target);
             e = ipc_energy_from_contact(s, p, c, ix, k, pos, kNoVertex, mk(0, 0, 0));
             owned = true;
             bad = bad || !finite_s(e);
@@ -3153,7 +3161,6 @@ __global__ void __launch_bounds__(kBlock) k_lag_contact(DeviceState s, PassParam
   }
   s.lag_n[c] = nf;
   s.lag_w[c] = w;
-  s.cscratch[c] = make_float4(1, 0, 0, 0);
   ipc_store_summary(s, c);
 }
 // buildActiveContactRows (S:2747-2785): keep candidates the bounded step can
@@ -3210,23 +3217,35 @@ __global__ void __launch_bounds__(kBlock) k_compact_active_rows(DeviceState s, P
   }
   if (sub == 0) s.active_count[v] = n;
 }
-// candidateContactStep (S:2788-2852): per-candidate conservative step cap.
-__global__ void __launch_bounds__(kBlock) k_candidate_contact_step(DeviceState s, PassParams p) {
-  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= s.NC) return;
-  const uint32_t sc = cscene(s, c);
-  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
-  if (stop_latched(s, p, sc)) {
-    s.cscratch[c] = make_float4(1, 0, 0, 1);
-    return;
+// reduceContactStep (S:2878-2918), block half, fused into the two passes
+// that produce the per-candidate records {alpha, source distance, trial
+// distance, valid}: the minimum alpha and an all-valid flag of the block's
+// candidates (min and a 0/1 product are order independent, so the partials
+// are exact); one block per scene combines them (k_reduce_contact_final).
+__device__ __forceinline__ void contact_min(float4& a, const float4& b) {
+  a = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), fminf(a.w, b.w));
+}
+__device__ __forceinline__ void contact_block_partial(const DeviceState& s, bool live, float4 q) {
+  __shared__ float4 sh[kBlock];
+  float4 r = make_float4(1.0f, kF32Max, kF32Max, 1.0f);
+  if (live) {
+    const bool valid = q.w == 1.0f && finite_s(q.x) && finite_s(q.y) && finite_s(q.z);
+    r = make_float4(fminf(1.0f, valid ? q.x : 0.0f), valid ? q.y : 0.0f, valid ? q.z : 0.0f, valid ? 1.0f : 0.0f);
   }
+  block_reduce(r, sh, contact_min);
+  if (threadIdx.x == 0) reinterpret_cast<float4*>(s.partials)[blockIdx.x] = r;
+}
+// candidateContactStep (S:2788-2852): per-candidate conservative step cap.
+__device__ __forceinline__ float4 candidate_contact_step(const DeviceState& s, const PassParams& p, uint32_t c,
+                                                         uint32_t sc) {
+  if (stop_latched(s, p, sc)) return make_float4(1, 0, 0, 1);
   const uint4 ix = s.cand[c];
   const float4* src = s.pos[p.src];
   const float4* tgt = s.pos[p.tgt];
   const f3 d0 = load_pos(tgt, ix.x) - load_pos(src, ix.x), d1 = load_pos(tgt, ix.y) - load_pos(src, ix.y);
   const f3 d2 = load_pos(tgt, ix.z) - load_pos(src, ix.z), d3 = load_pos(tgt, ix.w) - load_pos(src, ix.w);
   const float l0 = sqrtf(dot(d0, d0)), l1 = sqrtf(dot(d1, d1)), l2 = sqrtf(dot(d2, d2)), l3 = sqrtf(dot(d3, d3));
-  const float lip = s.cand_meta[c] == 0u ? l0 + fmaxf(l1, fmaxf(l2, l3)) : fmaxf(l0, l1) + fmaxf(l2, l3);
+  const float lip = (s.cand_meta[c] & 1u) == 0u ? l0 + fmaxf(l1, fmaxf(l2, l3)) : fmaxf(l0, l1) + fmaxf(l2, l3);
   Contact src_k = ipc_cached(s, c, false);
   bool full = src_k.valid && finite_s(lip) && src_k.distance - lip > s.ipc_dmin;
   if (!full) {
@@ -3237,43 +3256,37 @@ __global__ void __launch_bounds__(kBlock) k_candidate_contact_step(DeviceState s
   const bool valid = src_k.valid && trial_valid && src_k.distance > s.ipc_dmin && finite_s(lip);
   float alpha = 0.0f;
   if (valid) alpha = full ? 1.0f : ipc_safe_step_cap(src_k.distance, s.ipc_dmin, lip, s.ipc_safety);
-  s.cscratch[c] = make_float4(alpha, src_k.valid ? src_k.distance : 0.0f, trial_valid ? src_k.distance : 0.0f,
-                              valid ? 1.0f : 0.0f);
+  return make_float4(alpha, src_k.valid ? src_k.distance : 0.0f, trial_valid ? src_k.distance : 0.0f,
+                     valid ? 1.0f : 0.0f);
 }
-// validateContactStep (S:2855-2875): exact check of the scaled target pose.
-__global__ void __launch_bounds__(kBlock) k_validate_contact_step(DeviceState s, PassParams p) {
+// Blocks of no scene write no partial (the final pass reads only a scene's
+// own blocks); a block's scene is uniform, so they leave before any barrier.
+__global__ void __launch_bounds__(kBlock) k_candidate_contact_step(DeviceState s, PassParams p) {
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= s.NC) return;
-  const uint32_t sc = cscene(s, c);
-  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
-  const Contact k = ipc_contact_at(s, c, s.pos[p.tgt], ipc_broad_threshold(p, s.scene_scale[sc]));
-  ipc_store_cached(s, c, true, k);
-  {  // the target cache's distance as ipc_cached reads it (-1 when invalid)
-    const float4 nd = s.tgt_c[c], w = s.tgt_w[c];
-    s.tgt_d[c] = (nd.w >= 0.0f && finite4(nd) && finite4(w)) ? nd.w : -1.0f;
-  }
-  const bool valid = k.valid && finite_s(k.distance) && k.distance > s.ipc_dmin;
-  const float d = valid ? k.distance : 0.0f;
-  s.cscratch[c] = make_float4(1.0f, d, d, valid ? 1.0f : 0.0f);
-}
-// reduceContactStep (S:2878-2918): minimum alpha and an all-valid flag. Min
-// and a 0/1 product are order independent, so block partials are exact.
-__device__ __forceinline__ void contact_min(float4& a, const float4& b) {
-  a = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), fminf(a.w, b.w));
-}
-__global__ void __launch_bounds__(kBlock) k_reduce_contact_partials(DeviceState s) {
-  __shared__ float4 sh[kBlock];
   const uint32_t sc = s.cblock_scene[blockIdx.x];
   if (sc == kNoScene) return;  // uniform per block
-  float4 r = make_float4(1.0f, kF32Max, kF32Max, 1.0f);
+  const bool live = c < s.scene_end[sc].w;
+  contact_block_partial(s, live, live ? candidate_contact_step(s, p, c, sc) : make_float4(0, 0, 0, 0));
+}
+// validateContactStep (S:2855-2875): exact check of the scaled target pose.
+__device__ __forceinline__ float4 validate_contact_step(const DeviceState& s, const PassParams& p, uint32_t c,
+                                                        uint32_t sc) {
+  const Contact k = ipc_contact_at(s, c, s.pos[p.tgt], ipc_broad_threshold(p, s.scene_scale[sc]));
+  ipc_store_cached(s, c, true, k);
+  // The target cache's distance as ipc_cached reads it (-1 when invalid):
+  // ipc_store_cached keeps the contact exactly when this holds.
+  const bool cached = k.valid && finite_s(k.distance) && k.distance >= 0.0f && finite3(k.normal) && finite4(k.weights);
+  s.tgt_d[c] = cached ? k.distance : -1.0f;
+  const bool valid = k.valid && finite_s(k.distance) && k.distance > s.ipc_dmin;
+  const float d = valid ? k.distance : 0.0f;
+  return make_float4(1.0f, d, d, valid ? 1.0f : 0.0f);
+}
+__global__ void __launch_bounds__(kBlock) k_validate_contact_step(DeviceState s, PassParams p) {
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-  if (c < s.scene_end[sc].w) {
-    const float4 q = s.cscratch[c];
-    const bool valid = q.w == 1.0f && finite_s(q.x) && finite_s(q.y) && finite_s(q.z);
-    r = make_float4(fminf(1.0f, valid ? q.x : 0.0f), valid ? q.y : 0.0f, valid ? q.z : 0.0f, valid ? 1.0f : 0.0f);
-  }
-  block_reduce(r, sh, contact_min);
-  if (threadIdx.x == 0) reinterpret_cast<float4*>(s.partials)[blockIdx.x] = r;
+  const uint32_t sc = s.cblock_scene[blockIdx.x];
+  if (sc == kNoScene) return;  // uniform per block
+  const bool live = c < s.scene_end[sc].w;
+  contact_block_partial(s, live, live ? validate_contact_step(s, p, c, sc) : make_float4(0, 0, 0, 0));
 }
 // One block per scene: the scene's safe alpha (zero if any record failed).
 __global__ void __launch_bounds__(kBlock) k_reduce_contact_final(DeviceState s) {
@@ -3319,7 +3332,7 @@ __global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassP
   if (s.control[4 * size_t(sc) + 1].x == 1.0f && finite_s(alpha) && alpha > 0.0f) {
     s.src_c[c] = s.tgt_c[c];
     s.src_w[c] = s.tgt_w[c];
-    ipc_store_summary(s, c);
+    s.src_d[c] = make_float2(s.tgt_d[c], s.lag_n[c].w);  // the summary of the promoted cache
   }
 }
 
@@ -3687,8 +3700,9 @@ __device__ void bp_vt_item(const DeviceState& s, const PassParams& p, uint32_t i
       if (kWrite && in) {
         const uint32_t slot = at + n + uint32_t(__popcll(hits & ((1ull << lane) - 1ull)));
         if (slot < sz.z) {
-          s.cand[cb + slot] = make_uint4(dv, ix.x, ix.y, ix.z);
-          s.cand_meta[cb + slot] = 0u;
+          const uint4 rec = make_uint4(dv, ix.x, ix.y, ix.z);
+          s.cand[cb + slot] = rec;
+          s.cand_meta[cb + slot] = ipc_owner_slot(s, rec) << 1;
           s.cand_key[cb + slot] = make_uint2(q, tl);
         }
       }
@@ -3761,8 +3775,9 @@ __device__ void bp_ee_item(const DeviceState& s, const PassParams& p, uint32_t i
       if (kWrite && in) {
         const uint32_t slot = at + n + uint32_t(__popcll(hits & ((1ull << lane) - 1ull)));
         if (slot < sz.z) {
-          s.cand[cb + slot] = make_uint4(e.x, e.y, o.x, o.y);
-          s.cand_meta[cb + slot] = 1u;
+          const uint4 rec = make_uint4(e.x, e.y, o.x, o.y);
+          s.cand[cb + slot] = rec;
+          s.cand_meta[cb + slot] = 1u | (ipc_owner_slot(s, rec) << 1);
           s.cand_key[cb + slot] = make_uint2(i | 0x80000000u, jl);
         }
       }
@@ -3927,7 +3942,7 @@ __device__ void bp_rows_item(const DeviceState& s, const PassParams& p, uint32_t
   const uint32_t c = blk * kSceneBlock + threadIdx.x;
   const bool live = c < s.scene_end[sc].w;
   const uint4 ix = live ? s.cand[c] : make_uint4(0, 0, 0, 0);
-  const bool ee = live && s.cand_meta[c] == 1u;
+  const bool ee = live && (s.cand_meta[c] & 1u) == 1u;
   uint32_t* counter = kFill ? s.bp_rowfill : s.bp_rowcnt;
   const uint32_t r0 = wave_counted_add(counter, ix.x, live);
   const uint32_t r1e = wave_counted_add(counter, ix.y, ee);
@@ -4266,11 +4281,9 @@ void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t 
   const uint32_t cb = blocks_for(s.NC), vb = blocks_for(p.vertex_count);
   if (s.NC == 0) return;
   hipLaunchKernelGGL(k_candidate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
-  hipLaunchKernelGGL(k_reduce_contact_partials, dim3(cb), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_reduce_contact_final, dim3(p.scene_count), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p, 1u);
   hipLaunchKernelGGL(k_validate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
-  hipLaunchKernelGGL(k_reduce_contact_partials, dim3(cb), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_reduce_contact_final, dim3(p.scene_count), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p, 0u);
 }

@@ -242,16 +242,22 @@ class _Server:
         self.active = still + joined
         return True
 
-    def run(self, requests: Sequence[Request], duration: Optional[float] = None) -> ServeReport:
+    def run(self, requests: Sequence[Request], duration: Optional[float] = None,
+            progress: Optional[Callable[[ServeReport, float], None]] = None, every: float = 20.0) -> ServeReport:
         """Serve `requests` (sorted by arrival) until all are done or
         `duration` seconds pass; requests still queued then are dropped
-        from the report."""
+        from the report. `progress(report, now)` is called about every
+        `every` seconds of the run."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         self._t0 = self.clock()
+        next_report = every
         while pending or self.queue or self.active:
             now = self.now()
             if duration is not None and now >= duration:
                 break
+            if progress is not None and now >= next_report:
+                progress(self.report, now)
+                next_report = now + every
             while pending and pending[0].arrival <= now:
                 self.queue.append(pending.popleft())
             if self.tick():
@@ -298,8 +304,8 @@ class BatchServer(_Server):
     def _release(self, handle) -> None:
         self.batch.retire(handle)
 
-    def run(self, requests, duration=None) -> ServeReport:
-        report = super().run(requests, duration)
+    def run(self, requests, duration=None, progress=None, every=20.0) -> ServeReport:
+        report = super().run(requests, duration, progress, every)
         report.extra["graphs"] = self.batch.graph_stats()
         return report
 
