@@ -167,6 +167,11 @@ def main(argv=None):
             log(f"autotune: block {info['solve_block']} xcd_remap {info['xcd_remap']} ({autotune_ms:.3f} ms/launch)")
     info = batch.info()
     batch.sync()
+
+    def listed():
+        return [m.contact_info()["candidates"] if x.contact is not None else 0 for x, m in zip(inputs, members)]
+
+    lists_before = listed()
     # Timed region: the frames are launched pass by pass with HIP events
     # around every local-solve launch on the batch's stream, so `value` and
     # the roofline come from the same frames.
@@ -180,6 +185,7 @@ def main(argv=None):
     value = args.steps * it * free / elapsed
     log(f"batched: {args.steps} frames in {elapsed:.3f}s, {value:.4g} vertex-updates/s, local solve {solve_ms:.3f} ms "
         f"per launch over {launches} launches")
+    lists_after = listed()
     lat = []
     for _ in range(args.latency_frames):
         a = time.perf_counter()
@@ -188,7 +194,10 @@ def main(argv=None):
         lat.append((time.perf_counter() - a) * 1e3)
     if lat:
         log(f"batched frame latency (graph replay) p50 {nearest_rank(lat, 50):.2f} ms")
-    lists = [m.contact_info()["candidates"] if x.contact is not None else 0 for x, m in zip(inputs, members)]
+    # Candidate lists change at rebuilds (folded sheets separate): the
+    # algorithmic bytes of a sweep count the mean of each scene's list at
+    # the start and the end of the timed frames.
+    lists = [(a + b) / 2 for a, b in zip(lists_before, lists_after)]
     bytes_per_sweep = sum(scene_bytes_per_sweep(x, c) for x, c in zip(inputs, lists))
     achieved = bytes_per_sweep / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     traffic = measured_traffic(f"batched-uneven-scenes-{len(inputs)}", info["solve_block"])
@@ -268,7 +277,9 @@ def main(argv=None):
         "reverted_iterations_last_frame_all_scenes": reverted,
         "accepted_fraction_last_frame": accepted_fraction,
         "accepted_vertex_updates_per_s": value * accepted_fraction,
-        "self_contact": {"cloth_scenes_with_self_ipc": contact_scenes, "candidates_listed": int(sum(lists)),
+        "self_contact": {"cloth_scenes_with_self_ipc": contact_scenes,
+                         "candidates_listed_timed_start": int(sum(lists_before)),
+                         "candidates_listed_timed_end": int(sum(lists_after)),
                          "broad_phase_builds_total": builds},
     }
     del inputs, members

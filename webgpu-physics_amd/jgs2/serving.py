@@ -136,7 +136,10 @@ class ServeReport:
     completed: List[Request] = field(default_factory=list)
     rejected_admissions: int = 0
     admissions: int = 0
-    admission_s: float = 0.0  # host time spent admitting, overlapped with frames in flight
+    # Wall time inside admission calls: host work that overlaps the frame in
+    # flight, then the wait for that frame (an admission's uploads and
+    # feasibility preflight queue behind it).
+    admission_s: float = 0.0
     extra: dict = field(default_factory=dict)
 
     def summary(self) -> dict:
@@ -154,7 +157,7 @@ class ServeReport:
             "queueing_s": {"p50": nearest_rank(wait, 50), "p99": nearest_rank(wait, 99)},
             "resident_s": {"p50": nearest_rank(resident, 50), "p99": nearest_rank(resident, 99)},
             "admissions": self.admissions,
-            "admission_host_s": self.admission_s,
+            "admission_wall_s": self.admission_s,
             "admissions_deferred": self.rejected_admissions,
             **self.extra,
         }
