@@ -151,6 +151,8 @@ struct DeviceState {
   //                        non-finite or an unbounded pin) and hi at the build pose
   //   bp_tchunk, bp_echunk float4[2 NS/64], [2 NE/64]: AABBs of 64 consecutive
   //                        primitives (w as above, or-ed)
+  //   bp_tsuper, bp_esuper same shape: AABBs of 64 consecutive chunks of a
+  //                        scene, at the index of the superchunk's first chunk
   //   ring_row  uint2[V], ring_ids uint32[]: surface one-ring (device ids)
   //   bp_qv     uint32[V]  canonical (caller-order) vertex q of a scene at
   //                        vertex-range index VB + q -> its device vertex
@@ -188,6 +190,7 @@ struct DeviceState {
   uint4* bp_tri = nullptr;
   uint2* bp_edge = nullptr;
   float4 *bp_tbox = nullptr, *bp_ebox = nullptr, *bp_tchunk = nullptr, *bp_echunk = nullptr;
+  float4 *bp_tsuper = nullptr, *bp_esuper = nullptr;
   uint2* ring_row = nullptr;
   uint32_t* ring_ids = nullptr;
   uint32_t *bp_qv = nullptr, *bp_vcnt = nullptr, *bp_ecnt = nullptr;

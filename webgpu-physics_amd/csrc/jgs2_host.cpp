@@ -223,7 +223,7 @@ void free_device(DeviceState& d) {
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
                   d.body_rng, d.bp_list, d.bp_work, d.bp_work_n, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
-                  d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_vprev, d.bp_eprev, d.bp_x,
+                  d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.bp_tsuper, d.bp_esuper, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_vprev, d.bp_eprev, d.bp_x,
                   d.bp_cpose, d.bp_tval, d.cand_key, d.prev_key, d.prev_lag_n, d.prev_lag_w, d.prev_src_c, d.prev_src_w,
                   d.bp_rowcnt, d.bp_rowfill, d.bp_ctl, d.bp_reach, d.bp_motion, d.bp_bodies, d.bp_bref, d.bp_bdev, d.bp_bbox, d.sblock_scene, d.eblock_scene,
                   d.scene_blk2};
@@ -793,9 +793,11 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
     TRYS(dalloc(&d.bp_tri, Sc, by, dry));
     TRYS(dalloc(&d.bp_tbox, 2 * Sc, by, dry));
     TRYS(dalloc(&d.bp_tchunk, 2 * (Sc / 64), by, dry));
+    TRYS(dalloc(&d.bp_tsuper, 2 * (Sc / 64), by, dry));
     TRYS(dalloc(&d.bp_edge, Ec, by, dry));
     TRYS(dalloc(&d.bp_ebox, 2 * Ec, by, dry));
     TRYS(dalloc(&d.bp_echunk, 2 * (Ec / 64), by, dry));
+    TRYS(dalloc(&d.bp_esuper, 2 * (Ec / 64), by, dry));
     TRYS(dalloc(&d.bp_ecnt, Ec, by, dry));
     TRYS(dalloc(&d.bp_eprev, Ec, by, dry));
     TRYS(dalloc(&d.bp_blk_e, Ec / kSceneBlock, by, dry));

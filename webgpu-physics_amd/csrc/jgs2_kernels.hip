@@ -3642,8 +3642,40 @@ __global__ void __launch_bounds__(kBlock) k_bp_bounds(DeviceState s, PassParams 
   bp_for_work(s, kBpSegS, 1u, [&](uint32_t item) { bp_bounds_item(s, p, item); });
   bp_for_work(s, kBpSegE, 1u, [&](uint32_t item) { bp_bounds_item(s, p, s.bp_ns + item); });
 }
+// Superchunks: the AABB of 64 consecutive chunks (4096 primitives, 16
+// blocks) of a scene, stored at the index of its first chunk. The block of a
+// listed segment that starts a superchunk (scene-local block index a
+// multiple of 16) reduces it with its first wavefront.
+__device__ void bp_super_item(const DeviceState& s, uint32_t item) {
+  const bool tris = item < s.bp_ns;
+  const uint32_t blk = s.bp_list[s.bp_nv + item];
+  const uint32_t sc = tris ? s.sblock_scene[blk] : s.eblock_scene[blk];
+  if (!bp_building(s, sc)) return;
+  const uint4 b2 = s.scene_blk2[sc];
+  const uint32_t first = tris ? b2.x : b2.z;
+  const uint32_t local = blk - first;
+  if (local % 16u != 0u || threadIdx.x >= 64u) return;
+  const uint32_t n = tris ? s.bp_size[sc].x : s.bp_size[sc].y, nchunk = (n + 63u) / 64u;
+  const uint32_t c = 4u * local + threadIdx.x;  // scene-local chunk of this lane
+  const size_t gc = size_t(blk) * 4u;          // global index of the superchunk's first chunk
+  const float4* chunk = tris ? s.bp_tchunk : s.bp_echunk;
+  f3 lo = mk(INFINITY, INFINITY, INFINITY), hi = mk(-INFINITY, -INFINITY, -INFINITY);
+  float wild = 0.0f;
+  if (c < nchunk) {
+    const float4 a = chunk[2 * (gc + threadIdx.x)], b = chunk[2 * (gc + threadIdx.x) + 1];
+    lo = xyz(a);
+    hi = xyz(b);
+    wild = a.w;
+  }
+  bp_chunk_box(lo, hi, wild, (tris ? s.bp_tsuper : s.bp_esuper) + 2 * gc);
+}
+__global__ void __launch_bounds__(kBlock) k_bp_super(DeviceState s) {
+  bp_for_work(s, kBpSegS, 1u, [&](uint32_t item) { bp_super_item(s, item); });
+  bp_for_work(s, kBpSegE, 1u, [&](uint32_t item) { bp_super_item(s, s.bp_ns + item); });
+}
 // One wavefront per query: a vertex (VT, canonical vertex order) or an edge
-// (EE). Lanes test 64 chunk boxes at a time; the surviving chunks are
+// (EE). Lanes test 64 superchunk boxes at a time, then the 64 chunk boxes
+// of each surviving superchunk; the surviving chunks are
 // visited in order and each lane tests one primitive, so matches come out
 // in canonical order and a ballot gives each its slot. The count pass
 // stores the number of matches of each query, the write pass the records.
@@ -3668,45 +3700,59 @@ __device__ void bp_vt_item(const DeviceState& s, const PassParams& p, uint32_t i
   const uint32_t t0 = s.scene_blk2[sc].x * kSceneBlock, nt = sz.x, nchunk = (nt + 63u) / 64u;
   const float reach = s.bp_reach[sc].x;
   const uint32_t cb = s.scene_blk1[sc].z * kSceneBlock, at = kWrite ? s.bp_vcnt[qa] : 0u;
+  const uint32_t nsuper = (nchunk + 63u) / 64u;
   uint32_t n = 0;
-  for (uint32_t c0 = 0; c0 < nchunk; c0 += 64u) {
-    bool pass = false;
-    if (c0 + lane < nchunk) {
-      const size_t ci = t0 / 64u + c0 + lane;
-      const float4 lo = s.bp_tchunk[2 * ci], hi = s.bp_tchunk[2 * ci + 1];
-      pass = wild || lo.w != 0.0f || !(bp_lower_bound(x, x, xyz(lo), xyz(hi)) > reach);
+  for (uint32_t s0 = 0; s0 < nsuper; s0 += 64u) {
+    // Superchunk boxes contain their chunk boxes, so a chunk that passes
+    // lies in a superchunk that passes; chunks are still visited in order.
+    bool spass = false;
+    if (s0 + lane < nsuper) {
+      const size_t si = t0 / 64u + size_t(s0 + lane) * 64u;
+      const float4 lo = s.bp_tsuper[2 * si], hi = s.bp_tsuper[2 * si + 1];
+      spass = wild || lo.w != 0.0f || !(bp_lower_bound(x, x, xyz(lo), xyz(hi)) > reach);
     }
-    uint64_t chunks = __ballot(pass);
-    while (chunks) {
-      const uint32_t k = uint32_t(__builtin_ctzll(chunks));
-      chunks &= chunks - 1ull;
-      const uint32_t tl = (c0 + k) * 64u + lane;
-      bool in = false;
-      uint4 ix = make_uint4(kNoVertex, kNoVertex, kNoVertex, 0);
-      if (tl < nt) {
-        const size_t t = t0 + tl;
-        const float4 lo = s.bp_tbox[2 * t], hi = s.bp_tbox[2 * t + 1];
-        in = wild || lo.w != 0.0f || !(bp_lower_bound(x, x, xyz(lo), xyz(hi)) > reach);
-        ix = s.bp_tri[t];
+    uint64_t supers = __ballot(spass);
+    while (supers) {
+      const uint32_t c0 = (s0 + uint32_t(__builtin_ctzll(supers))) * 64u;
+      supers &= supers - 1ull;
+      bool pass = false;
+      if (c0 + lane < nchunk) {
+        const size_t ci = t0 / 64u + c0 + lane;
+        const float4 lo = s.bp_tchunk[2 * ci], hi = s.bp_tchunk[2 * ci + 1];
+        pass = wild || lo.w != 0.0f || !(bp_lower_bound(x, x, xyz(lo), xyz(hi)) > reach);
       }
-      // isInClosedOneRing (ipc-contact.ts:586-592) against the query vertex.
-      bool ringed = ix.x == dv || ix.y == dv || ix.z == dv;
-      for (uint32_t j = 0; j < ring.y; ++j) {
-        const uint32_t r = uint32_t(__shfl(int(my_ring), int(j)));
-        ringed = ringed || r == ix.x || r == ix.y || r == ix.z;
-      }
-      in = in && !ringed;
-      const uint64_t hits = __ballot(in);
-      if (kWrite && in) {
-        const uint32_t slot = at + n + uint32_t(__popcll(hits & ((1ull << lane) - 1ull)));
-        if (slot < sz.z) {
-          const uint4 rec = make_uint4(dv, ix.x, ix.y, ix.z);
-          s.cand[cb + slot] = rec;
-          s.cand_meta[cb + slot] = ipc_owner_slot(s, rec) << 1;
-          s.cand_key[cb + slot] = make_uint2(q, tl);
+      uint64_t chunks = __ballot(pass);
+      while (chunks) {
+        const uint32_t k = uint32_t(__builtin_ctzll(chunks));
+        chunks &= chunks - 1ull;
+        const uint32_t tl = (c0 + k) * 64u + lane;
+        bool in = false;
+        uint4 ix = make_uint4(kNoVertex, kNoVertex, kNoVertex, 0);
+        if (tl < nt) {
+          const size_t t = t0 + tl;
+          const float4 lo = s.bp_tbox[2 * t], hi = s.bp_tbox[2 * t + 1];
+          in = wild || lo.w != 0.0f || !(bp_lower_bound(x, x, xyz(lo), xyz(hi)) > reach);
+          ix = s.bp_tri[t];
         }
+        // isInClosedOneRing (ipc-contact.ts:586-592) against the query vertex.
+        bool ringed = ix.x == dv || ix.y == dv || ix.z == dv;
+        for (uint32_t j = 0; j < ring.y; ++j) {
+          const uint32_t r = uint32_t(__shfl(int(my_ring), int(j)));
+          ringed = ringed || r == ix.x || r == ix.y || r == ix.z;
+        }
+        in = in && !ringed;
+        const uint64_t hits = __ballot(in);
+        if (kWrite && in) {
+          const uint32_t slot = at + n + uint32_t(__popcll(hits & ((1ull << lane) - 1ull)));
+          if (slot < sz.z) {
+            const uint4 rec = make_uint4(dv, ix.x, ix.y, ix.z);
+            s.cand[cb + slot] = rec;
+            s.cand_meta[cb + slot] = ipc_owner_slot(s, rec) << 1;
+            s.cand_key[cb + slot] = make_uint2(q, tl);
+          }
+        }
+        n += uint32_t(__popcll(hits));
       }
-      n += uint32_t(__popcll(hits));
     }
   }
   if (!kWrite && lane == 0u) s.bp_vcnt[qa] = n;
@@ -3738,50 +3784,62 @@ __device__ void bp_ee_item(const DeviceState& s, const PassParams& p, uint32_t i
   const uint32_t ne = sz.y, nchunk = (ne + 63u) / 64u;
   const float reach = s.bp_reach[sc].x;
   const uint32_t cb = s.scene_blk1[sc].z * kSceneBlock, at = kWrite ? s.bp_ecnt[ea] : 0u;
+  const uint32_t cfirst = (i + 1u) / 64u, nsuper = (nchunk + 63u) / 64u;  // later edges only
   uint32_t n = 0;
-  for (uint32_t c0 = (i + 1u) / 64u; c0 < nchunk; c0 += 64u) {
-    bool pass = false;
-    if (c0 + lane < nchunk) {
-      const size_t ci = e0 / 64u + c0 + lane;
-      const float4 lo = s.bp_echunk[2 * ci], hi = s.bp_echunk[2 * ci + 1];
-      pass = wild || lo.w != 0.0f || !(bp_lower_bound(xyz(elo), xyz(ehi), xyz(lo), xyz(hi)) > reach);
+  for (uint32_t s0 = cfirst / 64u; s0 < nsuper; s0 += 64u) {
+    bool spass = false;
+    if (s0 + lane < nsuper) {
+      const size_t si = e0 / 64u + size_t(s0 + lane) * 64u;
+      const float4 lo = s.bp_esuper[2 * si], hi = s.bp_esuper[2 * si + 1];
+      spass = wild || lo.w != 0.0f || !(bp_lower_bound(xyz(elo), xyz(ehi), xyz(lo), xyz(hi)) > reach);
     }
-    uint64_t chunks = __ballot(pass);
-    while (chunks) {
-      const uint32_t k = uint32_t(__builtin_ctzll(chunks));
-      chunks &= chunks - 1ull;
-      const uint32_t jl = (c0 + k) * 64u + lane;
-      bool in = false;
-      uint2 o = make_uint2(kNoVertex, kNoVertex);
-      if (jl > i && jl < ne) {
-        const size_t j = e0 + jl;
-        const float4 lo = s.bp_ebox[2 * j], hi = s.bp_ebox[2 * j + 1];
-        in = wild || lo.w != 0.0f || !(bp_lower_bound(xyz(elo), xyz(ehi), xyz(lo), xyz(hi)) > reach);
-        o = s.bp_edge[j];
+    uint64_t supers = __ballot(spass);
+    while (supers) {
+      const uint32_t c0 = (s0 + uint32_t(__builtin_ctzll(supers))) * 64u;
+      supers &= supers - 1ull;
+      bool pass = false;
+      if (c0 + lane < nchunk && c0 + lane >= cfirst) {
+        const size_t ci = e0 / 64u + c0 + lane;
+        const float4 lo = s.bp_echunk[2 * ci], hi = s.bp_echunk[2 * ci + 1];
+        pass = wild || lo.w != 0.0f || !(bp_lower_bound(xyz(elo), xyz(ehi), xyz(lo), xyz(hi)) > reach);
       }
-      // Topologically adjacent edges (ipc-contact.ts:647-651): an endpoint of
-      // one in the closed one-ring of an endpoint of the other.
-      bool adjacent = o.x == e.x || o.x == e.y || o.y == e.x || o.y == e.y;
-      for (uint32_t j = 0; j < r0.y; ++j) {
-        const uint32_t r = uint32_t(__shfl(int(my0), int(j)));
-        adjacent = adjacent || r == o.x || r == o.y;
-      }
-      for (uint32_t j = 0; j < r1.y; ++j) {
-        const uint32_t r = uint32_t(__shfl(int(my1), int(j)));
-        adjacent = adjacent || r == o.x || r == o.y;
-      }
-      in = in && !adjacent;
-      const uint64_t hits = __ballot(in);
-      if (kWrite && in) {
-        const uint32_t slot = at + n + uint32_t(__popcll(hits & ((1ull << lane) - 1ull)));
-        if (slot < sz.z) {
-          const uint4 rec = make_uint4(e.x, e.y, o.x, o.y);
-          s.cand[cb + slot] = rec;
-          s.cand_meta[cb + slot] = 1u | (ipc_owner_slot(s, rec) << 1);
-          s.cand_key[cb + slot] = make_uint2(i | 0x80000000u, jl);
+      uint64_t chunks = __ballot(pass);
+      while (chunks) {
+        const uint32_t k = uint32_t(__builtin_ctzll(chunks));
+        chunks &= chunks - 1ull;
+        const uint32_t jl = (c0 + k) * 64u + lane;
+        bool in = false;
+        uint2 o = make_uint2(kNoVertex, kNoVertex);
+        if (jl > i && jl < ne) {
+          const size_t j = e0 + jl;
+          const float4 lo = s.bp_ebox[2 * j], hi = s.bp_ebox[2 * j + 1];
+          in = wild || lo.w != 0.0f || !(bp_lower_bound(xyz(elo), xyz(ehi), xyz(lo), xyz(hi)) > reach);
+          o = s.bp_edge[j];
         }
+        // Topologically adjacent edges (ipc-contact.ts:647-651): an endpoint of
+        // one in the closed one-ring of an endpoint of the other.
+        bool adjacent = o.x == e.x || o.x == e.y || o.y == e.x || o.y == e.y;
+        for (uint32_t j = 0; j < r0.y; ++j) {
+          const uint32_t r = uint32_t(__shfl(int(my0), int(j)));
+          adjacent = adjacent || r == o.x || r == o.y;
+        }
+        for (uint32_t j = 0; j < r1.y; ++j) {
+          const uint32_t r = uint32_t(__shfl(int(my1), int(j)));
+          adjacent = adjacent || r == o.x || r == o.y;
+        }
+        in = in && !adjacent;
+        const uint64_t hits = __ballot(in);
+        if (kWrite && in) {
+          const uint32_t slot = at + n + uint32_t(__popcll(hits & ((1ull << lane) - 1ull)));
+          if (slot < sz.z) {
+            const uint4 rec = make_uint4(e.x, e.y, o.x, o.y);
+            s.cand[cb + slot] = rec;
+            s.cand_meta[cb + slot] = 1u | (ipc_owner_slot(s, rec) << 1);
+            s.cand_key[cb + slot] = make_uint2(i | 0x80000000u, jl);
+          }
+        }
+        n += uint32_t(__popcll(hits));
       }
-      n += uint32_t(__popcll(hits));
     }
   }
   if (!kWrite && lane == 0u) s.bp_ecnt[ea] = n;
@@ -4313,6 +4371,7 @@ void launch_broad_phase(const DeviceState& s, const PassParams& p, bool check, b
   auto grid = [](uint64_t items) { return dim3(uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(items, kBpWorkGrid)))); };
   const uint64_t nv = s.bp_nv, ns = s.bp_ns, ne = s.bp_ne, nc = s.bp_nc;
   if (ns + ne) hipLaunchKernelGGL(k_bp_bounds, grid(ns + ne), B, 0, st, s, p);
+  if (ns + ne) hipLaunchKernelGGL(k_bp_super, grid(ns + ne), B, 0, st, s);
   hipLaunchKernelGGL(k_bp_vt<false>, grid(64 * nv), B, 0, st, s, p);
   if (ne) hipLaunchKernelGGL(k_bp_ee<false>, grid(64 * ne), B, 0, st, s, p);
   hipLaunchKernelGGL(k_bp_block_sums, grid(nv), B, 0, st, s, uint32_t(kBpScanVt));
