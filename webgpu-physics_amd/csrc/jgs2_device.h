@@ -102,6 +102,7 @@ struct DeviceState {
   float4* tet_glob = nullptr;    // 2 per tet
   float4* vertex_energy = nullptr;
   float2* ipc_energy = nullptr;  // per vertex: owned contact energy at the source and the target pose
+  float4* ipc_grad = nullptr;    // per vertex: contact gradient at the candidate pose (convergence)
   float4* conv = nullptr;        // 5 per vertex
   float4* control = nullptr;     // 4
   float4* history = nullptr;     // 64 * 8
@@ -179,11 +180,13 @@ struct DeviceState {
   //                        blocks, bp_nc candidate blocks
   //   bp_work   uint32[]   per bp_list segment, the positions of the blocks
   //                        whose scene rebuilds at the current check point
-  //   bp_work_n uint32[4]  their counts
+  //   bp_work_n uint32[8]  their counts, then the count of bp_long
+  //   bp_long   uint32[V]  vertices whose candidate row is longer than a
+  //                        wavefront sorts in registers
   uint32_t NS = 0, NE = 0;
   uint32_t bp_nv = 0, bp_ns = 0, bp_ne = 0, bp_nc = 0;
   uint32_t* bp_list = nullptr;
-  uint32_t *bp_work = nullptr, *bp_work_n = nullptr;
+  uint32_t *bp_work = nullptr, *bp_work_n = nullptr, *bp_long = nullptr;
   uint4* bp_size = nullptr;
   uint32_t* bp_rows_at = nullptr;  // per scene: first entry of its cand_ids / active_ids rows
   uint32_t *bp_blk_v = nullptr, *bp_blk_e = nullptr, *bp_blk_r = nullptr;

@@ -216,13 +216,13 @@ namespace {
 void free_device(DeviceState& d) {
   void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_rec, d.vcub[0], d.vcub[1], d.tet_k, d.adjacency, d.cub, d.objectives,
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
-                  d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.conv, d.control,
+                  d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.ipc_grad, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.diag_x, d.diag_xp, d.diag_buf, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
                   d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
-                  d.body_rng, d.bp_list, d.bp_work, d.bp_work_n, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
+                  d.body_rng, d.bp_list, d.bp_work, d.bp_work_n, d.bp_long, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
                   d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.bp_tsuper, d.bp_esuper, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_vprev, d.bp_eprev, d.bp_x,
                   d.bp_cpose, d.bp_tval, d.cand_key, d.prev_key, d.prev_lag_n, d.prev_lag_w, d.prev_src_c, d.prev_src_w,
                   d.bp_rowcnt, d.bp_rowfill, d.bp_ctl, d.bp_reach, d.bp_motion, d.bp_bodies, d.bp_bref, d.bp_bdev, d.bp_bbox, d.sblock_scene, d.eblock_scene,
@@ -738,6 +738,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.local_diag, 4 * Vc, by, dry));
   TRYS(dalloc(&d.vertex_energy, Vc, by, dry));
   TRYS(dalloc(&d.ipc_energy, Vc, by, dry));
+  if (Cc) TRYS(dalloc(&d.ipc_grad, Vc, by, dry));
   TRYS(dalloc(&d.conv, 5 * Vc, by, dry));
   TRYS(dalloc(&d.tri_row, Vc, by, dry));
   TRYS(dalloc(&d.hinge_row, Vc, by, dry));
@@ -824,7 +825,8 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
     TRYS(dalloc(&d.eblock_scene, Ec / kSceneBlock, by, dry));
     TRYS(dalloc(&d.bp_list, (Vc + Sc + Ec + Cc) / kSceneBlock, by, dry));
     TRYS(dalloc(&d.bp_work, (Vc + Sc + Ec + Cc) / kSceneBlock, by, dry));
-    TRYS(dalloc(&d.bp_work_n, 4, by, dry));
+    TRYS(dalloc(&d.bp_work_n, 8, by, dry));
+    TRYS(dalloc(&d.bp_long, Vc, by, dry));
   }
   if (dry) return JGS2_OK;
   // Free blocks belong to no scene; scene slots start inactive.

@@ -2746,6 +2746,30 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
   if (threadIdx.x == 0) reinterpret_cast<ConvPartial*>(s.partials)[blockIdx.x] = r;
 }
 
+// The contact component of convergenceGradient (S:4091) at the candidate
+// pose, 16 lanes per vertex: a vertex's active rows are strided over its 16
+// lanes and the lane sums combined by an xor butterfly (a fixed order). A
+// contact-heavy vertex has hundreds of rows; split over the convergence
+// gather's 4 or 8 lanes they made that pass latency-bound. Pinned
+// vertices and latched scenes write nothing (the gather reads no contact
+// term for them).
+__global__ void __launch_bounds__(kBlock) k_ipc_gradient_lanes(DeviceState s, PassParams p) {
+  constexpr int kLanes = 16;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t v = gid / kLanes, sub = gid % kLanes;
+  if (v >= p.vertex_count) return;  // uniform per 16-lane group
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || stop_latched(s, p, sc) || s.vtx_info[v].z != 0u) return;
+  const float4* pos = s.pos[p.tgt];
+  f3 g = mk(0, 0, 0);
+  m3 h = mzero();  // the rows' Hessian terms are not needed here
+  const uint32_t begin = s.cand_row[v].x, n = s.active_count[v];
+#pragma unroll 1
+  for (uint32_t k = sub; k < n; k += kLanes) ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], g, h);
+  g = lanes_sum3<kLanes>(g);
+  if (sub == 0) s.ipc_grad[v] = make_float4(g.x, g.y, g.z, 0.0f);
+}
+
 // convergenceGradient with LPV lanes per vertex. A block still covers one
 // 256-vertex arena block (its partial record), in LPV rounds of 256 / LPV
 // vertices; the lanes of a vertex split its incident tetrahedra and combine
@@ -2821,14 +2845,8 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient_lanes(DeviceSta
         }
       }
       f3 ig = mk(0, 0, 0);
-      if constexpr (kIpc) {  // S:4091, the active rows split over the lanes
-        if (!pinned) {
-          m3 ih = mzero();
-          const uint32_t begin = s.cand_row[v].x, n = s.active_count[v];
-#pragma unroll 1
-          for (uint32_t k = sub; k < n; k += LPV) ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], ig, ih);
-          ig = lanes_sum3<LPV>(ig);
-        }
+      if constexpr (kIpc) {  // S:4091, summed over the active rows by k_ipc_gradient_lanes
+        if (!pinned && sub == 0) ig = xyz(s.ipc_grad[v]);
       }
       if (sub == 0) {
         if (p.compute_next_rot) {
@@ -3514,7 +3532,8 @@ __device__ bool bp_bodies_keep(const DeviceState& s, const PassParams& p, uint32
 // the frame start (the pairs with lagged friction).
 template <bool kCheck>
 __global__ void __launch_bounds__(kBlock) k_bp_decide(DeviceState s, PassParams p, uint32_t carry) {
-  if (blockIdx.x == 0 && threadIdx.x < 4u) s.bp_work_n[threadIdx.x] = 0u;  // k_bp_worklist counts after
+  // Work-list counts (k_bp_worklist) and the long-row count (k_bp_row_sort).
+  if (blockIdx.x == 0 && threadIdx.x < 5u) s.bp_work_n[threadIdx.x] = 0u;
   const uint32_t sc = blockIdx.x * kBlock + threadIdx.x;
   if (sc >= p.scene_count) return;
   uint4 ctl = s.bp_ctl[sc];
@@ -4074,13 +4093,73 @@ __global__ void __launch_bounds__(kBlock) k_bp_row_offsets(DeviceState s) {
 }
 constexpr uint32_t kBpMaxRow = 8192;
 constexpr uint64_t kBpWorkGrid = 16384;  // workgroups of a work-list pass
-// One block per vertex: bitonic sort of its row in LDS (ascending ids, the
-// order the local solve sums contact rows in).
-__device__ void bp_row_sort_item(const DeviceState& s, uint32_t* keys, uint32_t item) {
-  const uint32_t vb = s.bp_list[item / kSceneBlock];
+constexpr uint64_t kBpLongGrid = 2048;   // workgroups sorting rows longer than a wavefront's
+constexpr uint32_t kBpWaveRow = 256;      // rows a wavefront sorts in registers
+// Vertex rows sorted ascending in candidate id (the order the local solve
+// sums contact rows in). A row of at most 256 entries is sorted by one
+// wavefront in registers: entry i = 64 r + lane sits in register r of a
+// lane, and a bitonic network exchanges partners i ^ j across lanes
+// (__shfl_xor) for j < 64 and between a lane's registers for j >= 64. Longer
+// rows go to a list that k_bp_row_sort_long sorts one block each in LDS.
+__device__ __forceinline__ void bp_cas(uint32_t& a, uint32_t& b, bool ascending) {
+  if ((a > b) == ascending) {
+    const uint32_t t = a;
+    a = b;
+    b = t;
+  }
+}
+__device__ void bp_row_sort_wave(const DeviceState& s, uint32_t item) {
+  const uint32_t vb = s.bp_list[item / 64u];
   const uint32_t sc = s.vblock_scene[vb];
   if (!bp_building(s, sc)) return;
-  const uint32_t v = vb * kSceneBlock + item % kSceneBlock;
+  const uint32_t lane = threadIdx.x % 64u;
+  const uint32_t v = vb * kSceneBlock + (item % 64u) * 4u + threadIdx.x / 64u;
+  const uint2 row = s.cand_row[v];
+  const uint32_t n = row.y;
+  if (n < 2u) return;  // uniform per wavefront
+  if (n > kBpWaveRow) {
+    if (lane == 0u) s.bp_long[atomicAdd(s.bp_work_n + 4, 1u)] = v;
+    return;
+  }
+  uint32_t m = 64;
+  while (m < n) m <<= 1;
+  uint32_t key[4];
+#pragma unroll
+  for (uint32_t r = 0; r < 4u; ++r) {
+    const uint32_t i = 64u * r + lane;
+    key[r] = i < n ? s.cand_ids[row.x + i] : 0xffffffffu;
+  }
+  for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j == 128u) {
+        bp_cas(key[0], key[2], ((0u + lane) & k) == 0u);
+        bp_cas(key[1], key[3], ((64u + lane) & k) == 0u);
+      } else if (j == 64u) {
+        bp_cas(key[0], key[1], ((0u + lane) & k) == 0u);
+        bp_cas(key[2], key[3], ((128u + lane) & k) == 0u);
+      } else {
+        const bool lower = (lane & j) == 0u;
+#pragma unroll
+        for (uint32_t r = 0; r < 4u; ++r) {
+          const uint32_t o = uint32_t(__shfl_xor(int(key[r]), int(j)));
+          const bool ascending = ((64u * r + lane) & k) == 0u;
+          key[r] = lower == ascending ? min(key[r], o) : max(key[r], o);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < 4u; ++r) {
+    const uint32_t i = 64u * r + lane;
+    if (i < n) s.cand_ids[row.x + i] = key[r];
+  }
+}
+__global__ void __launch_bounds__(kBlock) k_bp_row_sort(DeviceState s) {
+  bp_for_work(s, kBpSegV, 64u, [&](uint32_t item) { bp_row_sort_wave(s, item); });
+}
+// One block per long row: bitonic sort in LDS.
+__device__ void bp_row_sort_item(const DeviceState& s, uint32_t* keys, uint32_t v) {
+  const uint32_t sc = s.vblock_scene[v / kSceneBlock];
   const uint2 row = s.cand_row[v];
   const uint32_t n = row.y;
   if (n < 2u) return;
@@ -4109,9 +4188,13 @@ __device__ void bp_row_sort_item(const DeviceState& s, uint32_t* keys, uint32_t 
   }
   for (uint32_t i = threadIdx.x; i < n; i += kBlock) s.cand_ids[row.x + i] = keys[i];
 }
-__global__ void __launch_bounds__(kBlock) k_bp_row_sort(DeviceState s) {
+__global__ void __launch_bounds__(kBlock) k_bp_row_sort_long(DeviceState s) {
   __shared__ uint32_t keys[kBpMaxRow];
-  bp_for_work(s, kBpSegV, kSceneBlock, [&](uint32_t item) { bp_row_sort_item(s, keys, item); });
+  const uint32_t n = s.bp_work_n[4];
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    bp_row_sort_item(s, keys, s.bp_long[w]);
+    __syncthreads();
+  }
 }
 // The build pose of every rebuilt scene, and its query offsets (the ranges
 // a later rebuild inside a frame searches this list in).
@@ -4284,6 +4367,7 @@ void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t s
   const uint32_t vb = blocks_for(p.vertex_count);
   const bool cl = has_cloth(p), ipc = p.contact != 0u;
   if (p.solve_lanes == 4u || p.solve_lanes == 8u || p.solve_lanes == 16u) {
+    if (ipc) hipLaunchKernelGGL(k_ipc_gradient_lanes, dim3(blocks_for(16u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
     // The convergence gather splits a vertex over at most 8 lanes.
     const uint32_t lanes = p.solve_lanes >= 8u ? 8u : 4u;
     if (lanes == 8u) {
@@ -4389,7 +4473,8 @@ void launch_broad_phase(const DeviceState& s, const PassParams& p, bool check, b
   hipLaunchKernelGGL(k_bp_scan_apply, grid(nv), B, 0, st, s, uint32_t(kBpScanRows));
   hipLaunchKernelGGL(k_bp_row_offsets, grid(nv), B, 0, st, s);
   if (nc) hipLaunchKernelGGL(k_bp_rows<true>, grid(nc), B, 0, st, s, p, cr);
-  hipLaunchKernelGGL(k_bp_row_sort, grid(uint64_t(kSceneBlock) * nv), B, 0, st, s);
+  hipLaunchKernelGGL(k_bp_row_sort, grid(64 * nv), B, 0, st, s);
+  hipLaunchKernelGGL(k_bp_row_sort_long, grid(kBpLongGrid), B, 0, st, s);
   hipLaunchKernelGGL(k_bp_commit, grid(nv + ne), B, 0, st, s, p);
 }
 
