@@ -37,7 +37,7 @@ struct PassParams {
   uint32_t solve_lanes;         // lanes per vertex of the local solve for scenes that use lanes (4, 8, 16)
   uint32_t solve_classes;       // bit 0: scenes solved with lanes resident; bit 1: scenes solved one lane per vertex
   uint32_t cloth_base;          // cloth element records hold this pass's source pose (k_cloth_base ran)
-  uint32_t pad8;
+  uint32_t ipc_local;           // the solve's contact gather terms are in ipc_local (k_ipc_local_lanes ran)
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
@@ -103,6 +103,7 @@ struct DeviceState {
   float4* vertex_energy = nullptr;
   float2* ipc_energy = nullptr;  // per vertex: owned contact energy at the source and the target pose
   float4* ipc_grad = nullptr;    // per vertex: contact gradient at the candidate pose (convergence)
+  float4* ipc_local = nullptr;   // per vertex, 3 float4: contact g and H at a Jacobi sweep's source pose
   float4* conv = nullptr;        // 5 per vertex
   float4* control = nullptr;     // 4
   float4* history = nullptr;     // 64 * 8

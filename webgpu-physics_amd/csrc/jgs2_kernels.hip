@@ -1628,6 +1628,9 @@ template <int LPV> __device__ __forceinline__ f3 lanes_sum3(f3 a) {
 template <int LPV> __device__ __forceinline__ uint32_t lane_first(uint32_t sub, uint32_t prefix) {
   return (sub + LPV - prefix % LPV) % LPV;
 }
+// Active contact rows above which a vertex's gather terms come from
+// k_ipc_local_lanes (16 lanes) instead of the solve's own lanes.
+constexpr uint32_t kIpcLocalRows = 128;
 
 template <bool kCloth, bool kIpc, int LPV>
 __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, uint32_t v, uint32_t sub) {
@@ -1650,9 +1653,20 @@ __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, u
   uint32_t prefix = 0;
   if constexpr (kIpc) {  // S:3193-3195
     const uint32_t begin = s.cand_row[v].x, n = s.active_count[v];
+    if (p.ipc_local && n > kIpcLocalRows) {  // summed over the rows by k_ipc_local_lanes at this source pose
+      if (sub == 0) {
+        const float4 a = s.ipc_local[3 * size_t(v)], b = s.ipc_local[3 * size_t(v) + 1],
+                     c = s.ipc_local[3 * size_t(v) + 2];
+        L.g = L.g + mk(a.x, a.y, a.z);
+        L.H.c0 = L.H.c0 + mk(a.w, b.x, b.y);
+        L.H.c1 = L.H.c1 + mk(b.z, b.w, c.x);
+        L.H.c2 = L.H.c2 + mk(c.y, c.z, c.w);
+      }
+    } else {
 #pragma unroll 1
-    for (uint32_t k = lane_first<LPV>(sub, prefix); k < n; k += LPV)
-      ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], L.g, L.H);
+      for (uint32_t k = lane_first<LPV>(sub, prefix); k < n; k += LPV)
+        ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], L.g, L.H);
+    }
     prefix += n;
   }
 #pragma unroll 1
@@ -2768,6 +2782,36 @@ __global__ void __launch_bounds__(kBlock) k_ipc_gradient_lanes(DeviceState s, Pa
   for (uint32_t k = sub; k < n; k += kLanes) ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], g, h);
   g = lanes_sum3<kLanes>(g);
   if (sub == 0) s.ipc_grad[v] = make_float4(g.x, g.y, g.z, 0.0f);
+}
+
+// The contact rows' gradient and Hessian terms of the local solve's gather
+// (S:3193-3195, ipcLocalContribution S:786-874) at a Jacobi sweep's source
+// pose, for vertices with more than kIpcLocalRows active rows, summed by 16
+// lanes per vertex like k_ipc_gradient_lanes; the lane sweep of the solve
+// then reads one record per such vertex: {g, H c0.x}, {H c0.yz, H c1.xy},
+// {H c1.z, H c2} (columns of H). Vertices with fewer rows keep them in the
+// solve's own lanes, where they overlap the element terms.
+__global__ void __launch_bounds__(kBlock) k_ipc_local_lanes(DeviceState s, PassParams p) {
+  constexpr int kLanes = 16;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t v = gid / kLanes, sub = gid % kLanes;
+  if (v >= p.vertex_count) return;  // uniform per 16-lane group
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || stop_latched(s, p, sc) || s.vtx_info[v].z != 0u) return;
+  const uint32_t begin = s.cand_row[v].x, n = s.active_count[v];
+  if (n <= kIpcLocalRows) return;  // uniform per 16-lane group
+  const float4* pos = s.pos[p.src];
+  f3 g = mk(0, 0, 0);
+  m3 h = mzero();
+#pragma unroll 1
+  for (uint32_t k = sub; k < n; k += kLanes) ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], g, h);
+  g = lanes_sum3<kLanes>(g);
+  h = {lanes_sum3<kLanes>(h.c0), lanes_sum3<kLanes>(h.c1), lanes_sum3<kLanes>(h.c2)};
+  if (sub == 0) {
+    s.ipc_local[3 * size_t(v)] = make_float4(g.x, g.y, g.z, h.c0.x);
+    s.ipc_local[3 * size_t(v) + 1] = make_float4(h.c0.y, h.c0.z, h.c1.x, h.c1.y);
+    s.ipc_local[3 * size_t(v) + 2] = make_float4(h.c1.z, h.c2.x, h.c2.y, h.c2.z);
+  }
 }
 
 // convergenceGradient with LPV lanes per vertex. A block still covers one
@@ -4308,6 +4352,10 @@ void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
   if (!p.corotated && (p.solve_classes & 1u) && p.solve_lanes > 1u) {
     const uint32_t lanes = p.solve_lanes;
     const dim3 grid((n * lanes + block - 1) / block);
+    // The contact rows' gather terms first (Jacobi sweeps with contact): the
+    // local solve is this pass and the lane sweep together.
+    if (p.ipc_local && !p.colored)
+      hipLaunchKernelGGL(k_ipc_local_lanes, dim3(blocks_for(16u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
     if (lanes == 4u) JGS2_LANES_DISPATCH(4, grid, dim3(block));
     else if (lanes == 8u) JGS2_LANES_DISPATCH(8, grid, dim3(block));
     else JGS2_LANES_DISPATCH(16, grid, dim3(block));
