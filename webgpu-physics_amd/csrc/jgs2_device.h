@@ -104,6 +104,7 @@ struct DeviceState {
   float2* ipc_energy = nullptr;  // per vertex: owned contact energy at the source and the target pose
   float4* ipc_grad = nullptr;    // per vertex: contact gradient at the candidate pose (convergence)
   float4* ipc_local = nullptr;   // per vertex, 3 float4: contact g and H at a Jacobi sweep's source pose
+  float4* ipc_erec = nullptr;    // per candidate: owned contact energies at the source and target pose, flags, owner
   float4* conv = nullptr;        // 5 per vertex
   float4* control = nullptr;     // 4
   float4* history = nullptr;     // 64 * 8

@@ -216,7 +216,7 @@ namespace {
 void free_device(DeviceState& d) {
   void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_rec, d.vcub[0], d.vcub[1], d.tet_k, d.adjacency, d.cub, d.objectives,
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
-                  d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.ipc_grad, d.ipc_local, d.conv, d.control,
+                  d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.ipc_grad, d.ipc_local, d.ipc_erec, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.diag_x, d.diag_xp, d.diag_buf, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
                   d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
@@ -740,6 +740,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.ipc_energy, Vc, by, dry));
   if (Cc) TRYS(dalloc(&d.ipc_grad, Vc, by, dry));
   if (Cc) TRYS(dalloc(&d.ipc_local, 3 * Vc, by, dry));
+  if (Cc) TRYS(dalloc(&d.ipc_erec, Cc, by, dry));
   TRYS(dalloc(&d.conv, 5 * Vc, by, dry));
   TRYS(dalloc(&d.tri_row, Vc, by, dry));
   TRYS(dalloc(&d.hinge_row, Vc, by, dry));

@@ -2315,6 +2315,40 @@ __global__ void __launch_bounds__(kBlock) k_vertex_energy(DeviceState s, PassPar
 // so each sum is the sequential one bit for bit; any invalid row or
 // non-finite term gives the f32 sentinel as the sequential loop's early
 // return does.
+// The owned contact energy of every active candidate at the source and the
+// target pose, once per candidate (not once per each of its four rows): a
+// record {energy at the source, energy at the target, flags, owner vertex}.
+// Flags: bit w (w = 0 source, 1 target) the cached distance is invalid (the
+// row makes its vertices' energies invalid), bit 2 + w the row is active
+// there (barrier within reach or lagged friction), the energy then being
+// the owner's term.
+__global__ void __launch_bounds__(kBlock) k_ipc_energy_candidates(DeviceState s, PassParams p) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t sc = s.cblock_scene[blockIdx.x];
+  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
+  if ((p.energy_after_sweep && stop_latched(s, p, sc)) || s.cand_keep[c] == 0u) return;
+  const float2 q = s.src_d[c];
+  const uint4 ix = s.cand[c];
+  const uint32_t owner = ipc_energy_owner(ix, s.cand_meta[c]);
+  const bool use_target = p.tgt != p.src && s.contact_ctl[sc].w != 0.0f;
+  float e[2] = {0.0f, 0.0f};
+  uint32_t flags = 0;
+#pragma unroll
+  for (int which = 0; which < 2; ++which) {
+    const bool target = which == 1 && use_target;
+    const float dq = target ? s.tgt_d[c] : q.x;
+    if (dq < 0.0f) {
+      flags |= 1u << which;
+    } else if (ipc_barrier_active(dq, p.ipc_dhat) || (s.ipc_mu > 0.0f && q.y > 0.0f)) {
+      const Contact k = ipc_cached(s, c, target);
+      e[which] = ipc_energy_from_contact(s, p, c, ix, k, which ? s.pos[p.tgt] : s.pos[p.src], kNoVertex, mk(0, 0, 0));
+      flags |= 4u << which;
+    }
+  }
+  s.ipc_erec[c] = make_float4(e[0], e[1], __uint_as_float(flags), __uint_as_float(owner));
+}
+// Per vertex, 16 lanes: the owned energies of its active rows summed in row
+// order (S:2643, ipcCachedContactAt S:450-460), from the candidate records.
 __global__ void __launch_bounds__(kBlock) k_ipc_energy_lanes(DeviceState s, PassParams p) {
   constexpr uint32_t kLanes = 16;
   const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
@@ -2324,42 +2358,33 @@ __global__ void __launch_bounds__(kBlock) k_ipc_energy_lanes(DeviceState s, Pass
   if (sc == kNoScene || (p.energy_after_sweep && stop_latched(s, p, sc))) return;
   const uint32_t group = (threadIdx.x % 64u) & ~(kLanes - 1u);
   const uint32_t first = s.cand_row[v].x, last = first + s.active_count[v];
-  float out[2];
+  float r[2] = {0.0f, 0.0f};
+  bool bad[2] = {false, false};
+  for (uint32_t base = first; base < last; base += kLanes) {  // same trip count on the group's lanes
+    const uint32_t i = base + sub;
+    float4 rec = make_float4(0, 0, 0, 0);
+    if (i < last) rec = s.ipc_erec[s.active_ids[i]];
+    const uint32_t flags = __float_as_uint(rec.z);
+    const bool mine = i < last && __float_as_uint(rec.w) == v;
 #pragma unroll
-  for (int which = 0; which < 2; ++which) {
-    const float4* pos = which ? s.pos[p.tgt] : s.pos[p.src];
-    const bool use_target = which == 1 && p.tgt != p.src && s.contact_ctl[sc].w != 0.0f;
-    float r = 0.0f;
-    bool bad = false;
-    for (uint32_t base = first; base < last; base += kLanes) {  // same trip count on the group's lanes
-      const uint32_t i = base + sub;
-      bool owned = false;
-      float e = 0.0f;
-      if (i < last) {
-        const uint32_t c = s.active_ids[i];
-        const float2 q = s.src_d[c];  // summaries decide first: validity, barrier, lagged force
-        const float dq = use_target ? s.tgt_d[c] : q.x;
-        if (dq < 0.0f) {
-          bad = true;
-        } else if (ipc_barrier_active(dq, p.ipc_dhat) || (s.ipc_mu > 0.0f && q.y > 0.0f)) {
-          const uint4 ix = s.cand[c];
-          if (ipc_energy_owner(ix, s.cand_meta[c]) == v) {  // the owner alone loads the cached contact
-            const Contact k = ipc_cached(s, c, use_target);
-            e = ipc_energy_from_contact(s, p, c, ix, k, pos, kNoVertex, mk(0, 0, 0));
-            owned = true;
-            bad = bad || !finite_s(e);
-          }
-        }
-      }
+    for (int which = 0; which < 2; ++which) {
+      bad[which] = bad[which] || (flags & (1u << which)) != 0u;
+      const bool owned = mine && (flags & (4u << which)) != 0u;
+      const float e = which ? rec.y : rec.x;
+      bad[which] = bad[which] || (owned && !finite_s(e));
       uint32_t mask = uint32_t(__ballot(owned) >> group) & 0xffffu;
       while (mask) {
         const uint32_t j = uint32_t(__builtin_ctz(mask));
         mask &= mask - 1u;
-        r += __shfl(e, int(group + j));
+        r[which] += __shfl(e, int(group + j));
       }
     }
-    bad = (uint32_t(__ballot(bad) >> group) & 0xffffu) != 0u;
-    out[which] = bad ? kF32Max : r;
+  }
+  float out[2];
+#pragma unroll
+  for (int which = 0; which < 2; ++which) {
+    const bool b = (uint32_t(__ballot(bad[which]) >> group) & 0xffffu) != 0u;
+    out[which] = b ? kF32Max : r[which];
   }
   if (sub == 0) s.ipc_energy[v] = make_float2(out[0], out[1]);
 }
@@ -4388,7 +4413,10 @@ void launch_cache_flush(const DeviceState& s, uint32_t records, hipStream_t st) 
 void launch_vertex_energy(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const dim3 grid(blocks_for(p.vertex_count));
   const bool cl = has_cloth(p), ipc = p.contact != 0u;
-  if (ipc) hipLaunchKernelGGL(k_ipc_energy_lanes, dim3(blocks_for(16u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
+  if (ipc) {
+    hipLaunchKernelGGL(k_ipc_energy_candidates, dim3(blocks_for(s.NC)), dim3(kBlock), 0, st, s, p);
+    hipLaunchKernelGGL(k_ipc_energy_lanes, dim3(blocks_for(16u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
+  }
   if (cl && ipc) hipLaunchKernelGGL((k_vertex_energy<true, true>), grid, dim3(kBlock), 0, st, s, p);
   else if (cl) hipLaunchKernelGGL((k_vertex_energy<true, false>), grid, dim3(kBlock), 0, st, s, p);
   else if (ipc) hipLaunchKernelGGL((k_vertex_energy<false, true>), grid, dim3(kBlock), 0, st, s, p);
