@@ -38,6 +38,9 @@ struct PassParams {
   uint32_t solve_classes;       // bit 0: scenes solved with lanes resident; bit 1: scenes solved one lane per vertex
   uint32_t cloth_base;          // cloth element records hold this pass's source pose (k_cloth_base ran)
   uint32_t ipc_local;           // the solve's contact gather terms are in ipc_local (k_ipc_local_lanes ran)
+  uint32_t cub_local;           // the lane sweep's cubature gather terms are in cub_local (k_cub_local_lanes ran)
+  uint32_t kind_filter;         // lane sweep over: 0 every scene; 1, 2, 3 the scenes of that kind_blocks list
+  uint32_t kind_blocks1, kind_blocks2, kind_blocks3;  // lengths of the three kind_blocks lists
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
@@ -104,6 +107,7 @@ struct DeviceState {
   float2* ipc_energy = nullptr;  // per vertex: owned contact energy at the source and the target pose
   float4* ipc_grad = nullptr;    // per vertex: contact gradient at the candidate pose (convergence)
   float4* ipc_local = nullptr;   // per vertex, 3 float4: contact g and H at a Jacobi sweep's source pose
+  float4* cub_local = nullptr;   // per vertex, 3 float4: cubature g, symmetric H and record count at a sweep's source pose
   float4* ipc_erec = nullptr;    // per candidate: owned contact energies at the source and target pose, flags, owner
   float4* conv = nullptr;        // 5 per vertex
   float4* control = nullptr;     // 4
@@ -229,6 +233,9 @@ struct DeviceState {
   uint4* scene_end = nullptr;   // absolute ends {vertex, tet, triangle, candidate}
   float* scene_scale = nullptr;
   uint32_t* scene_lanes = nullptr;  // lanes per vertex of the scene's local solve (1 = one thread per vertex)
+  uint32_t* scene_kind = nullptr;   // per scene: bit 0 cloth elements, bit 1 contact candidates, bit 2 cubature (kKind*)
+  uint32_t* kind_blocks = nullptr;  // vertex blocks of lane-swept scenes: kind 0 (no cloth or contact), the others,
+                                    // then the others that hold cubature records (counts in PassParams)
   void* scene_conv = nullptr;   // per-scene convergence scale records
   uint4* body_rng = nullptr;    // per body {scene vertex begin, end, 0, 0}
 };
@@ -237,6 +244,10 @@ constexpr uint32_t kNoScene = 0xffffffffu;
 constexpr uint32_t kSceneBlock = 256;  // scene ranges are multiples of this
 
 constexpr uint32_t kHistoryCapacity = 64;
+// Scene kinds (scene_kind): element terms beyond tetrahedra that a scene holds.
+constexpr uint32_t kKindCloth = 1u;
+constexpr uint32_t kKindContact = 2u;
+constexpr uint32_t kKindCubature = 4u;  // a cubature record in use
 
 // Launchers (jgs2_kernels.hip). All enqueue on `stream`.
 void launch_predict(const DeviceState& s, const PassParams& p, hipStream_t stream);

@@ -138,6 +138,7 @@ struct jgs2_solver {
   bool globalization = true;
   bool corotated = false;
   uint32_t V = 0, T = 0, K = 0, NT = 0, NH = 0, NC = 0, NVT = 0, bodies = 1, color_count = 1, free_vertices = 0;
+  uint32_t cub_used = 0;  // cubature records in use (tetrahedron set, weight > 0)
   // Arena spans: vertex/tet/triangle/candidate spans in 256-entry blocks;
   // the others in entries.
   Span vblk, tblk, rblk, cblk, adj, hin, tel, hel, cel, body;
@@ -184,6 +185,7 @@ struct jgs2_batch {
   uint64_t synced_version = ~0ull;
   uint32_t color_count = 1;
   std::vector<uint32_t> color_offsets;
+  uint32_t kind_blocks_n[3] = {0, 0, 0};  // lengths of the dev.kind_blocks lists
   // Launch shape of the local solve. Block size and XCD remap leave every
   // result unchanged (autotuned); lanes per vertex set the summation order
   // of the gathered terms and stay fixed unless a caller sets them.
@@ -216,12 +218,12 @@ namespace {
 void free_device(DeviceState& d) {
   void* ptrs[] = {d.vtx_rest, d.vtx_info, d.tet_rec, d.vcub[0], d.vcub[1], d.tet_k, d.adjacency, d.cub, d.objectives,
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
-                  d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.ipc_grad, d.ipc_local, d.ipc_erec, d.conv, d.control,
+                  d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.ipc_grad, d.ipc_local, d.cub_local, d.ipc_erec, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.diag_x, d.diag_xp, d.diag_buf, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
                   d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
                   d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
-                  d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_conv,
+                  d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_kind, d.kind_blocks, d.scene_conv,
                   d.body_rng, d.bp_list, d.bp_work, d.bp_work_n, d.bp_long, d.bp_size, d.bp_rows_at, d.bp_blk_v, d.bp_blk_e, d.bp_blk_r, d.bp_tri, d.bp_edge, d.bp_tbox,
                   d.bp_ebox, d.bp_tchunk, d.bp_echunk, d.bp_tsuper, d.bp_esuper, d.ring_row, d.ring_ids, d.bp_qv, d.bp_vcnt, d.bp_ecnt, d.bp_vprev, d.bp_eprev, d.bp_x,
                   d.bp_cpose, d.bp_tval, d.cand_key, d.prev_key, d.prev_lag_n, d.prev_lag_w, d.prev_src_c, d.prev_src_w,
@@ -704,6 +706,14 @@ uint32_t scene_lanes(const jgs2_batch* b, const jgs2_solver* s) {
 }
 
 
+// Element terms beyond tetrahedra that a scene holds (scene_kind): the lane
+// sweep of a mixed batch solves scenes of kind 0 with the tetrahedral
+// kernel variant and the others with the batch's full variant.
+uint32_t scene_kind(const jgs2_solver* s) {
+  return ((s->NT > 0 || s->NH > 0) ? kKindCloth : 0u) | ((s->NC > 0 || s->bp) ? kKindContact : 0u) |
+         (s->cub_used > 0 ? kKindCubature : 0u);
+}
+
 // Allocates every arena array for the configured capacities.
 jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   const jgs2_batch_config& c = b->cfg;
@@ -741,6 +751,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   if (Cc) TRYS(dalloc(&d.ipc_grad, Vc, by, dry));
   if (Cc) TRYS(dalloc(&d.ipc_local, 3 * Vc, by, dry));
   if (Cc) TRYS(dalloc(&d.ipc_erec, Cc, by, dry));
+  if (K) TRYS(dalloc(&d.cub_local, 3 * Vc, by, dry));
   TRYS(dalloc(&d.conv, 5 * Vc, by, dry));
   TRYS(dalloc(&d.tri_row, Vc, by, dry));
   TRYS(dalloc(&d.hinge_row, Vc, by, dry));
@@ -777,6 +788,8 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.contact_ctl, S, by, dry));
   TRYS(dalloc(&d.scene_scale, S, by, dry));
   TRYS(dalloc(&d.scene_lanes, S, by, dry));
+  TRYS(dalloc(&d.scene_kind, S, by, dry));
+  TRYS(dalloc(&d.kind_blocks, 2 * (Vc / kSceneBlock), by, dry));
   TRYS(dalloc(reinterpret_cast<char**>(&d.scene_conv), scene_conv_bytes(uint32_t(S)), by, dry));
   TRYS(dalloc(&d.scene_blk0, S, by, dry));
   TRYS(dalloc(&d.scene_blk1, S, by, dry));
@@ -980,6 +993,7 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
       std::memcpy(&wbits, &in.cubature_weights[ro], 4);
       const uint32_t t = in.cubature_tet_ids[ro];
       cub[10 * rn] = make_uint4(t < T ? uint32_t(TB) + tnew[t] : 0xffffffffu, wbits, 0, 0);
+      if (t < T && in.cubature_weights[ro] > 0) ++s->cub_used;
       std::memcpy(&cub[10 * rn + 1], &in.cubature_basis[36 * ro], 36 * sizeof(float));
     }
   // Cloth records, vertex -> element CSR rows {begin, count}.
@@ -1129,6 +1143,8 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
   HIP_CHECK(hipMemcpyAsync(d.scene_scale + sl, &s->scene_scale, sizeof(float), hipMemcpyHostToDevice, q));
   const uint32_t lanes = scene_lanes(b, s);
   HIP_CHECK(hipMemcpyAsync(d.scene_lanes + sl, &lanes, sizeof(uint32_t), hipMemcpyHostToDevice, q));
+  const uint32_t kind = scene_kind(s);
+  HIP_CHECK(hipMemcpyAsync(d.scene_kind + sl, &kind, sizeof(uint32_t), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.contact_ctl + sl, &ctl, sizeof(float4), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.control + 4 * size_t(sl), zero_ctl.data(), 4 * sizeof(float4), hipMemcpyHostToDevice, q));
   HIP_CHECK(hipMemcpyAsync(d.history + size_t(sl) * kHistoryCapacity * 8, zero_hist.data(),
@@ -1223,7 +1239,13 @@ PassParams make_params(jgs2_batch* b, const jgs2_settings& st, bool stop) {
   p.solve_block = b->solve_block;
   p.solve_lanes = b->solve_lanes;
   for (auto* s : b->slots)
-    if (s) p.solve_classes |= scene_lanes(b, s) > 1u ? 1u : 2u;
+    if (s) {
+      p.solve_classes |= scene_lanes(b, s) > 1u ? 1u : 2u;
+      if (scene_lanes(b, s) > 1u) p.solve_classes |= (scene_kind(s) & (kKindCloth | kKindContact)) == 0u ? 4u : 8u;
+    }
+  p.kind_blocks1 = b->kind_blocks_n[0];
+  p.kind_blocks2 = b->kind_blocks_n[1];
+  p.kind_blocks3 = b->kind_blocks_n[2];
   p.xcd_remap = b->xcd_remap;
   // packUniforms (jgs2-solver.ts:604-730): f64 expressions rounded to f32.
   const double inv = 1.0 / st.timestep;
@@ -1272,6 +1294,28 @@ jgs2_status sync_colors(jgs2_batch* b) {
   }
   if (!list.empty()) {
     HIP_CHECK(hipMemcpyAsync(b->dev.color_order, list.data(), 4 * list.size(), hipMemcpyHostToDevice, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+  }
+  // Vertex blocks of the lane-swept scenes by kind, in arena order: the lane
+  // sweep of a mixed batch launches over each list instead of every block.
+  std::vector<uint32_t> kinds[3];
+  for (auto* s : order) {
+    if (scene_lanes(b, s) <= 1u) continue;
+    const uint32_t k = scene_kind(s);
+    const bool plain = (k & (kKindCloth | kKindContact)) == 0u;
+    for (uint64_t i = 0; i < s->vblk.n; ++i) {
+      const uint32_t blk = uint32_t(s->vblk.at + i);
+      kinds[plain ? 0 : 1].push_back(blk);
+      if (!plain && (k & kKindCubature)) kinds[2].push_back(blk);
+    }
+  }
+  std::vector<uint32_t> all;
+  for (int i = 0; i < 3; ++i) {
+    b->kind_blocks_n[i] = uint32_t(kinds[i].size());
+    all.insert(all.end(), kinds[i].begin(), kinds[i].end());
+  }
+  if (!all.empty()) {
+    HIP_CHECK(hipMemcpyAsync(b->dev.kind_blocks, all.data(), 4 * all.size(), hipMemcpyHostToDevice, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
   }
   b->synced_version = b->version;

@@ -49,18 +49,40 @@ enum : uint32_t {
 enum : uint32_t { kForce = 1u, kTarget = 2u };
 
 // ------------------------------------------------------------------ small math
+// Vector sums of products are fused multiply-adds written out by the types
+// themselves: `s * a` is kept unevaluated (sf3) until it meets an addition,
+// which then issues one fmaf per component. The rounding of every
+// expression is therefore fixed by the source, identical in every kernel
+// variant (the device code is compiled with -ffp-contract=on, which fuses
+// only within one scalar expression), and a 3x3 product costs 27 fused
+// operations instead of 27 multiplies and 18 adds.
 struct f3 {
   float x, y, z;
 };
+struct sf3 {  // s * a, not yet rounded
+  float s;
+  f3 a;
+  __device__ __forceinline__ operator f3() const { return f3{s * a.x, s * a.y, s * a.z}; }
+};
 __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ f3 fma3(float s, f3 a, f3 b) {  // s * a + b, one rounding per component
+  return f3{fmaf(s, a.x, b.x), fmaf(s, a.y, b.y), fmaf(s, a.z, b.z)};
+}
 __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
-__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ sf3 operator*(float s, f3 a) { return sf3{s, a}; }
+__device__ __forceinline__ sf3 operator*(f3 a, float s) { return sf3{s, a}; }
+__device__ __forceinline__ sf3 operator-(sf3 m) { return sf3{-m.s, m.a}; }
+__device__ __forceinline__ f3 operator+(f3 b, sf3 m) { return fma3(m.s, m.a, b); }
+__device__ __forceinline__ f3 operator+(sf3 m, f3 b) { return fma3(m.s, m.a, b); }
+__device__ __forceinline__ f3 operator+(sf3 m, sf3 n) { return fma3(n.s, n.a, f3(m)); }
+__device__ __forceinline__ f3 operator-(f3 b, sf3 m) { return fma3(-m.s, m.a, b); }
+__device__ __forceinline__ f3 operator-(sf3 m, f3 b) { return fma3(m.s, m.a, -b); }
+__device__ __forceinline__ f3 operator-(sf3 m, sf3 n) { return fma3(-n.s, n.a, f3(m)); }
 __device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
-__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 __device__ __forceinline__ f3 cross(f3 a, f3 b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
@@ -82,7 +104,13 @@ __device__ __forceinline__ f3 tmul(const m3& m, f3 v) {  // transpose(m) * v
 __device__ __forceinline__ m3 madd(const m3& a, const m3& b) { return {a.c0 + b.c0, a.c1 + b.c1, a.c2 + b.c2}; }
 __device__ __forceinline__ m3 msub(const m3& a, const m3& b) { return {a.c0 - b.c0, a.c1 - b.c1, a.c2 - b.c2}; }
 __device__ __forceinline__ m3 mscale(float s, const m3& a) { return {s * a.c0, s * a.c1, s * a.c2}; }
-__device__ __forceinline__ float frob(const m3& a, const m3& b) { return dot(a.c0, b.c0) + dot(a.c1, b.c1) + dot(a.c2, b.c2); }
+__device__ __forceinline__ m3 mfma(float s, const m3& a, const m3& b) {  // s * a + b
+  return {fma3(s, a.c0, b.c0), fma3(s, a.c1, b.c1), fma3(s, a.c2, b.c2)};
+}
+__device__ __forceinline__ float frob(const m3& a, const m3& b) {
+  return fmaf(a.c2.z, b.c2.z, fmaf(a.c2.y, b.c2.y, fmaf(a.c2.x, b.c2.x, fmaf(a.c1.z, b.c1.z,
+         fmaf(a.c1.y, b.c1.y, fmaf(a.c1.x, b.c1.x, fmaf(a.c0.z, b.c0.z, fmaf(a.c0.y, b.c0.y, a.c0.x * b.c0.x))))))));
+}
 __device__ __forceinline__ float mdet(const m3& f) { return dot(f.c0, cross(f.c1, f.c2)); }
 __device__ __forceinline__ m3 cofactor(const m3& f) { return {cross(f.c1, f.c2), cross(f.c2, f.c0), cross(f.c0, f.c1)}; }
 
@@ -127,6 +155,25 @@ __device__ __forceinline__ const float4* rot_source(const DeviceState& s, const 
 __device__ __forceinline__ bool stop_latched(const DeviceState& s, const PassParams& p, uint32_t sc) {  // S:909-917
   if (!p.stop_enabled) return false;
   return (control_bits(s.control[4 * size_t(sc)].w) & kStopLatch) != 0u;
+}
+// Whether a lane sweep with kind filter `f` solves scene `sc` (0: every
+// scene; 1: scenes without cloth or contact; 2: the others; 3: the others
+// that hold cubature records).
+__device__ __forceinline__ bool kind_selected(const DeviceState& s, uint32_t f, uint32_t sc) {
+  if (f == 0u) return true;
+  const uint32_t k = s.scene_kind[sc];
+  const bool plain = (k & (kKindCloth | kKindContact)) == 0u;
+  return f == 1u ? plain : (!plain && (f == 2u || (k & kKindCubature) != 0u));
+}
+// Thread index of a Jacobi lane sweep over vertex blocks: with a kind
+// filter the launch covers only the blocks of that kind_blocks list
+// (`per` launch blocks of `threads` threads per 256-vertex block).
+__device__ __forceinline__ uint32_t lane_sweep_gid(const DeviceState& s, const PassParams& p, uint32_t lb,
+                                                   uint32_t threads, uint32_t lanes) {
+  if (p.kind_filter == 0u || p.colored) return lb * threads + threadIdx.x;
+  const uint32_t per = kSceneBlock * lanes / threads;
+  const uint32_t first = p.kind_filter == 1u ? 0u : p.kind_filter == 2u ? p.kind_blocks1 : p.kind_blocks1 + p.kind_blocks2;
+  return s.kind_blocks[first + lb / per] * (kSceneBlock * lanes) + (lb % per) * threads + threadIdx.x;
 }
 // Scene of a vertex (kNoScene in free arena blocks).
 __device__ __forceinline__ uint32_t vscene(const DeviceState& s, uint32_t v) { return s.vblock_scene[v / kSceneBlock]; }
@@ -1177,7 +1224,7 @@ __device__ __forceinline__ void cubature_record_snh(const DeviceState& s, const 
     ph.c2.z -= diag;
   }
   L.g = L.g + w * pg;
-  L.H = madd(L.H, mscale(w, ph));
+  L.H = mfma(w, ph, L.H);
 }
 
 // Legacy co-rotated record (S:2368-2398 with S:3226-3259).
@@ -1227,7 +1274,7 @@ __device__ void cubature_record_corot(const DeviceState& s, const PassParams& p,
     ph = msub(ph, madd(mul(mul(R, stiffness_block(s, t, ss, ss)), tr(R)), mscale(w_src, mid())));
   }
   L.g = L.g + w * pg;
-  L.H = madd(L.H, mscale(w, ph));
+  L.H = mfma(w, ph, L.H);
 }
 
 // Gather of jgs2Solve (S:3074-3263): inertia, objectives, the exact
@@ -1281,7 +1328,7 @@ __device__ Local gather_local(const DeviceState& s, const PassParams& p, uint32_
         const float ad = wrapped_angle_difference(d.angle, r.x);
         const float k2 = s.cloth_bend * r.y;
         L.g = L.g + (k2 * ad) * d.grad;
-        L.H = madd(L.H, m3{(k2 * d.grad.x) * d.grad, (k2 * d.grad.y) * d.grad, (k2 * d.grad.z) * d.grad});
+        L.H = m3{fma3(k2 * d.grad.x, d.grad, L.H.c0), fma3(k2 * d.grad.y, d.grad, L.H.c1), fma3(k2 * d.grad.z, d.grad, L.H.c2)};
       }
     }
   }
@@ -1582,7 +1629,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar(DeviceState s, PassPara
   for (uint32_t k = 0; k < inf.y; ++k) {
     const uint32_t t = s.adjacency[inf.x + k];
     const Tet e = load_tet(s, t);
-    avg = madd(avg, mscale(e.vol, deformation(e, s.pos[p.src])));
+    avg = mfma(e.vol, deformation(e, s.pos[p.src]), avg);
     tv += e.vol;
   }
   avg = tv > 0.0f ? mscale(1.0f / tv, avg) : mid();
@@ -1702,26 +1749,102 @@ __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, u
         const float ad = wrapped_angle_difference(d.angle, r.x);
         const float k2 = s.cloth_bend * r.y;
         L.g = L.g + (k2 * ad) * d.grad;
-        L.H = madd(L.H, m3{(k2 * d.grad.x) * d.grad, (k2 * d.grad.y) * d.grad, (k2 * d.grad.z) * d.grad});
+        L.H = m3{fma3(k2 * d.grad.x, d.grad, L.H.c0), fma3(k2 * d.grad.y, d.grad, L.H.c1), fma3(k2 * d.grad.z, d.grad, L.H.c2)};
       }
     }
     prefix += hr.y;
   }
-  const uint32_t k0 = lane_first<LPV>(sub, prefix);
-  if (k0 < p.cubature_k) {  // S:3198-3263
-    const m3 Rs_t = tr(load_vcub(vcub_source(s, p), v).R);
+  // The cubature records (S:3198-3263). The tetrahedral variant deals them
+  // over its lanes behind the incident tetrahedra; the variants that also
+  // carry cloth or contact terms read them from k_cub_local_lanes (run by
+  // launch_solve before the sweep at this source pose), which keeps their
+  // registers within three waves per SIMD without spilling.
+  if constexpr (kCloth || kIpc) {
+    if (p.cub_local && sub == 0 && (s.scene_kind[vscene(s, v)] & kKindCubature)) {
+      const float4 a = s.cub_local[3 * size_t(v)], b = s.cub_local[3 * size_t(v) + 1],
+                   c = s.cub_local[3 * size_t(v) + 2];
+      if (c.w > 0.0f) {  // the vertex has a record in use
+        L.g = L.g + mk(a.x, a.y, a.z);
+        L.H.c0 = L.H.c0 + mk(a.w, b.x, b.y);
+        L.H.c1 = L.H.c1 + mk(b.x, b.z, b.w);
+        L.H.c2 = L.H.c2 + mk(b.y, b.w, c.x);
+      }
+    }
+  } else {
+    const uint32_t k0 = lane_first<LPV>(sub, prefix);
+    if (k0 < p.cubature_k) {
+      const m3 Rs_t = tr(load_vcub(vcub_source(s, p), v).R);
 #pragma unroll 1
-    for (uint32_t k = k0; k < p.cubature_k; k += LPV) {
-      const size_t rec = size_t(v) * p.cubature_k + k;
-      const uint4 head = s.cub[10 * rec];
-      const float w = __uint_as_float(head.y);
-      if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
-      cubature_record_snh(s, p, v, x, head.x, reinterpret_cast<const float*>(s.cub + 10 * rec + 1), Rs_t, w, L);
+      for (uint32_t k = k0; k < p.cubature_k; k += LPV) {
+        const size_t rec = size_t(v) * p.cubature_k + k;
+        const uint4 head = s.cub[10 * rec];
+        const float w = __uint_as_float(head.y);
+        if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
+        cubature_record_snh(s, p, v, x, head.x, reinterpret_cast<const float*>(s.cub + 10 * rec + 1), Rs_t, w, L);
+      }
     }
   }
   L.g = lanes_sum3<LPV>(L.g);
   L.H = {lanes_sum3<LPV>(L.H.c0), lanes_sum3<LPV>(L.H.c1), lanes_sum3<LPV>(L.H.c2)};
   return L;
+}
+
+// The cubature correction's gather terms (S:3198-3263) of the vertices the
+// lane sweep solves next, at its source pose: one record per lane, eight
+// lanes per vertex, the lane sums combined by an xor butterfly (a fixed
+// order). Inside the sweep a vertex's K records were dealt over its four
+// lanes behind the incident tetrahedra, so two lanes evaluated two records
+// each in series; here every record of a vertex with K <= 8 is evaluated at
+// once, and the sweep reads one record per vertex: {g, H00}, {H01, H02,
+// H11, H12}, {H22, 0, 0, records in use} (H is symmetric). Vertices the
+// sweep does not solve (pinned, latched, an infeasible source) are skipped.
+__global__ void __launch_bounds__(kBlock) k_cub_local_lanes(DeviceState s, PassParams p) {
+  constexpr int kLanes = 8;
+  const uint32_t gid = lane_sweep_gid(s, p, logical_block(p.xcd_remap), kBlock, kLanes);
+  const uint32_t lane = gid / kLanes, sub = gid % kLanes;
+  uint32_t v;
+  if (p.colored) {
+    if (p.color_begin + lane >= p.color_end) return;  // uniform per 8-lane group
+    v = s.color_order[p.color_begin + lane];
+  } else {
+    if (lane >= p.vertex_count) return;
+    v = lane;
+  }
+  const uint32_t sc = vscene(s, v);
+  if (sc == kNoScene || s.scene_lanes[sc] != p.solve_lanes || !kind_selected(s, p.kind_filter, sc) ||
+      stop_latched(s, p, sc) || s.vtx_info[v].z != 0u)
+    return;
+  {  // the sweep's feasibility gate (k_solve_lanes)
+    const float4 pre = s.control[4 * size_t(sc)];
+    const uint32_t pv = control_bits(pre.w) & kValidityMask;
+    const bool amv = (pv & kAcceptedMinimumValid) != 0u && finite_s(pre.z);
+    if (!(amv && pre.z > kDeterminantFloor)) return;
+  }
+  const f3 x = load_pos(s.pos[p.src], v);
+  Local L;
+  L.g = mk(0, 0, 0);
+  L.H = mzero();
+  float used = 0.0f;
+  if (sub < p.cubature_k) {
+    const m3 Rs_t = tr(load_vcub(vcub_source(s, p), v).R);
+#pragma unroll 1
+    for (uint32_t k = sub; k < p.cubature_k; k += kLanes) {
+      const size_t rec = size_t(v) * p.cubature_k + k;
+      const uint4 head = s.cub[10 * rec];
+      const float w = __uint_as_float(head.y);
+      if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
+      cubature_record_snh(s, p, v, x, head.x, reinterpret_cast<const float*>(s.cub + 10 * rec + 1), Rs_t, w, L);
+      used += 1.0f;
+    }
+  }
+  L.g = lanes_sum3<kLanes>(L.g);
+  L.H = {lanes_sum3<kLanes>(L.H.c0), lanes_sum3<kLanes>(L.H.c1), lanes_sum3<kLanes>(L.H.c2)};
+  used = lanes_sum<kLanes>(used);
+  if (sub == 0) {
+    s.cub_local[3 * size_t(v)] = make_float4(L.g.x, L.g.y, L.g.z, L.H.c0.x);
+    s.cub_local[3 * size_t(v) + 1] = make_float4(L.H.c1.x, L.H.c2.x, L.H.c1.y, L.H.c2.y);
+    s.cub_local[3 * size_t(v) + 2] = make_float4(L.H.c2.z, 0.0f, 0.0f, used);
+  }
 }
 
 // One lane's share of the restricted trial. `fail` is the canonical index of
@@ -1928,7 +2051,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar_lanes(DeviceState s, Pa
 #pragma unroll 1
   for (uint32_t k = sub; k < inf.y; k += LPV) {
     const Tet e = load_tet(s, s.adjacency[inf.x + k]);
-    avg = madd(avg, mscale(e.vol, deformation_at(e, s.pos[p.src], v, x)));
+    avg = mfma(e.vol, deformation_at(e, s.pos[p.src], v, x), avg);
     tv += e.vol;
   }
   avg = {lanes_sum3<LPV>(avg.c0), lanes_sum3<LPV>(avg.c1), lanes_sum3<LPV>(avg.c2)};
@@ -1950,7 +2073,10 @@ template <bool kCloth, bool kIpc, int LPV>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_solve_lanes(DeviceState s, PassParams p) {
   constexpr bool kEnergyInSolve = !kCloth && !kIpc;
-  const uint32_t gid = logical_block(p.xcd_remap) * blockDim.x + threadIdx.x;
+  // In a mixed batch the vertex-energy pass after the sweep covers every
+  // scene, so the tetrahedral variant leaves the energy to it.
+  const bool energy_here = !p.colored && p.kind_filter == 0u;
+  const uint32_t gid = lane_sweep_gid(s, p, logical_block(p.xcd_remap), blockDim.x, LPV);
   const uint32_t lane = gid / LPV, sub = gid % LPV;
   uint32_t v;
   if (p.colored) {
@@ -1962,7 +2088,7 @@ k_solve_lanes(DeviceState s, PassParams p) {
   }
   // Every branch below is uniform over a vertex's LPV lanes.
   const uint32_t sc = vscene(s, v);
-  if (sc == kNoScene || s.scene_lanes[sc] != uint32_t(LPV)) return;
+  if (sc == kNoScene || s.scene_lanes[sc] != uint32_t(LPV) || !kind_selected(s, p.kind_filter, sc)) return;
   float4* tgt = s.pos[p.tgt];
   if (stop_latched(s, p, sc)) {
     if (sub == 0) tgt[v] = s.pos[p.src][v];
@@ -1977,7 +2103,7 @@ k_solve_lanes(DeviceState s, PassParams p) {
       tgt[v] = make_float4(r.x, r.y, r.z, 1.0f);
       s.final_update[v] = make_float4(0, 0, 0, 1);
       store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, kPinned);
-      if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
+      if (kEnergyInSolve && energy_here) store_vertex_energy<false, false>(s, p, v);
     }
     return;
   }
@@ -1994,7 +2120,7 @@ k_solve_lanes(DeviceState s, PassParams p) {
         s.final_update[v] = make_float4(0, 0, 0, 1);
         store_local_diag(s, v, mk(0, 0, 0), 0, 0, 0, 0, 0, 0, 0, 0, 0, amv ? pre.z : 0.0f, 0, 0,
                          amv ? kLineSearchFailed : kNonfinite);
-        if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
+        if (kEnergyInSolve && energy_here) store_vertex_energy<false, false>(s, p, v);
       }
       return;
     }
@@ -2090,7 +2216,7 @@ k_solve_lanes(DeviceState s, PassParams p) {
     s.final_update[v] = make_float4(scaled_length3(accepted_d), 0, 0, 1);
     store_local_diag(s, v, dr.dir, alpha, dr.min_eig * dr.scale, dr.scale, dr.shift * dr.scale, dr.shift, accepted_de,
                      bound, dr.gdotp, dr.residual, min_trial, backtracks, evaluations, status);
-    if (kEnergyInSolve && !p.colored) store_vertex_energy<false, false>(s, p, v);
+    if (kEnergyInSolve && energy_here) store_vertex_energy<false, false>(s, p, v);
   }
 }
 
@@ -2718,7 +2844,7 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient(DeviceState s, 
           const Tet e = load_tet(s, t);
           const m3 F = deformation(e, pos);
           if (p.compute_next_rot) {
-            avg = madd(avg, mscale(e.vol, F));
+            avg = mfma(e.vol, F, avg);
             tv += e.vol;
           }
           const uint32_t slot = local_slot(e.ix, v);
@@ -2872,7 +2998,7 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient_lanes(DeviceSta
           const Tet e = load_tet(s, s.adjacency[inf.x + k]);
           const m3 F = deformation_at(e, pos, v, x);
           if (p.compute_next_rot) {
-            avg = madd(avg, mscale(e.vol, F));
+            avg = mfma(e.vol, F, avg);
             tv += e.vol;
           }
           const uint32_t slot = local_slot(e.ix, v);
@@ -4355,13 +4481,13 @@ void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t 
     else hipLaunchKernelGGL((KERNEL<false, false, false>), GRID, BLOCK, 0, st, __VA_ARGS__); \
   } while (0)
 
-#define JGS2_LANES_DISPATCH(LPV, GRID, BLOCK)                                                        \
+#define JGS2_LANES_DISPATCH_P(LPV, GRID, BLOCK, P)                                                   \
   do {                                                                                               \
-    const bool cl = has_cloth(p), ipc = p.contact != 0u;                                            \
-    if (cl && ipc) hipLaunchKernelGGL((k_solve_lanes<true, true, LPV>), GRID, BLOCK, 0, st, s, p);  \
-    else if (cl) hipLaunchKernelGGL((k_solve_lanes<true, false, LPV>), GRID, BLOCK, 0, st, s, p);   \
-    else if (ipc) hipLaunchKernelGGL((k_solve_lanes<false, true, LPV>), GRID, BLOCK, 0, st, s, p);  \
-    else hipLaunchKernelGGL((k_solve_lanes<false, false, LPV>), GRID, BLOCK, 0, st, s, p);          \
+    const bool cl = has_cloth(P), ipc = (P).contact != 0u;                                          \
+    if (cl && ipc) hipLaunchKernelGGL((k_solve_lanes<true, true, LPV>), GRID, BLOCK, 0, st, s, P);  \
+    else if (cl) hipLaunchKernelGGL((k_solve_lanes<true, false, LPV>), GRID, BLOCK, 0, st, s, P);   \
+    else if (ipc) hipLaunchKernelGGL((k_solve_lanes<false, true, LPV>), GRID, BLOCK, 0, st, s, P);  \
+    else hipLaunchKernelGGL((k_solve_lanes<false, false, LPV>), GRID, BLOCK, 0, st, s, P);          \
   } while (0)
 
 // The local solve runs with the autotuned block size (64, 128 or 256) and,
@@ -4381,9 +4507,42 @@ void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
     // local solve is this pass and the lane sweep together.
     if (p.ipc_local && !p.colored)
       hipLaunchKernelGGL(k_ipc_local_lanes, dim3(blocks_for(16u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
-    if (lanes == 4u) JGS2_LANES_DISPATCH(4, grid, dim3(block));
-    else if (lanes == 8u) JGS2_LANES_DISPATCH(8, grid, dim3(block));
-    else JGS2_LANES_DISPATCH(16, grid, dim3(block));
+    PassParams q = p;
+    dim3 qgrid = grid;
+    // Launch blocks of a Jacobi sweep over one kind_blocks list.
+    auto list_grid = [&](uint32_t listed, uint32_t threads, uint32_t lpv) {
+      return p.colored ? dim3(uint32_t((uint64_t(n) * lpv + threads - 1) / threads))
+                       : dim3(listed * (kSceneBlock * lpv / threads));
+    };
+    if (has_cloth(p) || p.contact != 0u) {
+      // A mixed batch: scenes without cloth or contact are swept by the
+      // tetrahedral variant first, the others by the full variant, whose
+      // cubature terms come from k_cub_local_lanes (eight lanes per vertex).
+      if ((p.solve_classes & 4u) && (p.solve_classes & 8u)) {
+        PassParams a = p;
+        a.kind_filter = 1u;
+        const dim3 ag = list_grid(p.kind_blocks1, block, lanes);
+        if (ag.x > 0) {
+          if (lanes == 4u) hipLaunchKernelGGL((k_solve_lanes<false, false, 4>), ag, dim3(block), 0, st, s, a);
+          else if (lanes == 8u) hipLaunchKernelGGL((k_solve_lanes<false, false, 8>), ag, dim3(block), 0, st, s, a);
+          else hipLaunchKernelGGL((k_solve_lanes<false, false, 16>), ag, dim3(block), 0, st, s, a);
+        }
+        q.kind_filter = 2u;
+        qgrid = list_grid(p.kind_blocks2, block, lanes);
+      }
+      if (p.cubature_k > 0u && s.cub_local) {
+        PassParams c = q;
+        c.kind_filter = 3u;
+        const dim3 cg = list_grid(p.kind_blocks3, kBlock, 8u);
+        q.cub_local = 1u;
+        if (cg.x > 0) hipLaunchKernelGGL(k_cub_local_lanes, cg, dim3(kBlock), 0, st, s, c);
+      }
+    }
+    if (qgrid.x > 0) {
+      if (lanes == 4u) JGS2_LANES_DISPATCH_P(4, qgrid, dim3(block), q);
+      else if (lanes == 8u) JGS2_LANES_DISPATCH_P(8, qgrid, dim3(block), q);
+      else JGS2_LANES_DISPATCH_P(16, qgrid, dim3(block), q);
+    }
   }
   if (p.corotated || (p.solve_classes & 2u))
     JGS2_FEATURE_DISPATCH_BLOCK(k_solve, dim3((n + block - 1) / block), dim3(block), s, p);
