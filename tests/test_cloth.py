@@ -144,6 +144,44 @@ def test_dihedral_sign_and_energy_of_a_known_fold(theta):
     assert b["bending_energy"] == pytest.approx(0.5 * 2.0 * 1.0 * theta ** 2, rel=1e-13)
 
 
+def test_dihedral_gradient_entries_are_finite_above_the_edge_floor():
+    """The device evaluates a hinge's validity without forming its twelve
+    gradient entries when no corner gradient is wanted and the shared edge
+    is at least 1e-20 long (they are then finite, so validity is decided by
+    the gate and the angle alone). Pinned to the FP32 oracle, which forms
+    all twelve as the reference does (S:1700-1810): over seeded hinges at
+    scales 1e-19 to 1e15, down to nearly degenerate wings, every hinge that
+    passes the gate with a margin is valid."""
+    rng = np.random.default_rng(7)
+    f = np.float32
+    checked = 0
+    for scale in (1e-19, 1e-15, 1e-9, 1e-3, 1.0, 1e3, 1e9, 1e15):
+        for flat in (1.0, 1e-3, 1e-5):
+            for _ in range(40):
+                x = rng.normal(size=(4, 3))
+                x[1] = x[0] + rng.normal(size=3)
+                for w in (2, 3):  # wings pressed toward the edge line
+                    t = rng.uniform(-0.5, 1.5)
+                    x[w] = x[0] + t * (x[1] - x[0]) + flat * (x[w] - x[0] - t * (x[1] - x[0]))
+                x = (x * scale).astype(f)
+                e = x[1] - x[0]
+                el = np.sqrt(np.dot(e, e))
+                if not el >= f(1e-20):
+                    continue
+                rn0, rn1 = np.cross(e, x[2] - x[0]), np.cross(-e, x[3] - x[1])
+                nl0, nl1 = np.sqrt(np.dot(rn0, rn0)), np.sqrt(np.dot(rn1, rn1))
+                es0 = max(el, np.linalg.norm(x[2] - x[0]), np.linalg.norm(x[2] - x[1]))
+                es1 = max(el, np.linalg.norm(x[3] - x[1]), np.linalg.norm(x[3] - x[0]))
+                margin = f(2e-12)
+                if not (np.isfinite([nl0, nl1, es0, es1]).all() and nl0 > margin * es0 * es0 and nl1 > margin * es1 * es1):
+                    continue
+                b = O.cloth_probe(REST_HINGE, x.astype(np.float64).ravel(), 1000.0, 0.3, 0.01, 1.0, precision=32)
+                assert b["hinge_valid"], (scale, flat, x.tolist())
+                assert np.isfinite(b["angle"])
+                checked += 1
+    assert checked > 500
+
+
 def incidence(V, idx, width):
     idx = np.asarray(idx, np.uint32)
     count = idx.size // width

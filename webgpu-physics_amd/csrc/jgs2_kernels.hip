@@ -594,8 +594,16 @@ __device__ Dihedral dihedral_at(const DeviceState& s, uint32_t h, const float4* 
   if (!finite_s(den) || !(den > 0.0f)) return out;
   bool fin = true;
   f3 keep = mk(0, 0, 0);
+  // The twelve entries only decide validity when no corner gradient is
+  // wanted, and then they are finite whenever el >= 1e-20: with unit
+  // directions |dt| <= 2.01 / el, |dn0r| <= |to2| + |edge| <= 2 es0 and
+  // nl0 >= max(1e-12 es0^2, smallest f32) give |dn0| <= 1.1e29 (likewise
+  // dn1, es0 and es1 >= el), and den = sine^2 + cosine^2 of unit normals
+  // is 1 to rounding, so every entry is below 1e30. Shorter edges form
+  // them all, as the reference does.
+  const uint32_t corners = (slot < 4u || store != nullptr || !(el >= 1.0e-20f)) ? 4u : 0u;
 #pragma unroll 1
-  for (uint32_t dv = 0; dv < 4; ++dv) {
+  for (uint32_t dv = 0; dv < corners; ++dv) {
     float gc[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
