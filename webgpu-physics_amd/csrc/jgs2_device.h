@@ -41,6 +41,7 @@ struct PassParams {
   uint32_t cub_local;           // the lane sweep's cubature gather terms are in cub_local (k_cub_local_lanes ran)
   uint32_t kind_filter;         // lane sweep over: 0 every scene; 1, 2, 3 the scenes of that kind_blocks list
   uint32_t kind_blocks1, kind_blocks2, kind_blocks3;  // lengths of the three kind_blocks lists
+  uint32_t fric_records;        // Jacobi sweeps: k_candidate_keep writes the friction records (fric)
 };
 
 // Device buffers of one solver. Layout (16-byte records throughout):
@@ -141,6 +142,11 @@ struct DeviceState {
   float ipc_dmin = 0, ipc_mu = 0, ipc_eps = 0, ipc_safety = 0;
   uint4* cand = nullptr;
   float4 *lag_n = nullptr, *lag_w = nullptr;
+  // Per candidate, 4 float4, written by k_candidate_keep before each Jacobi
+  // sweep for the kept candidates with lagged friction: {x_l - old_l} of its
+  // four vertices at the sweep's source pose, and in the first w the row's
+  // energy at that pose (the trials' source term).
+  float4* fric = nullptr;
   float4 *src_c = nullptr, *src_w = nullptr, *tgt_c = nullptr, *tgt_w = nullptr;
   float2* src_d = nullptr;  // per candidate: cached source distance (-1 invalid), lagged force
   float* tgt_d = nullptr;   // per candidate: cached target distance (-1 invalid)

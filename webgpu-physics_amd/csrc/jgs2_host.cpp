@@ -220,7 +220,7 @@ void free_device(DeviceState& d) {
                   d.color_order, d.colors, d.pos[0], d.pos[1], d.predicted, d.velocity, d.old, d.vrot, d.vrot_next, d.trot,
                   d.body_corr, d.final_update, d.local_diag, d.tet_glob, d.vertex_energy, d.ipc_energy, d.ipc_grad, d.ipc_local, d.cub_local, d.ipc_erec, d.conv, d.control,
                   d.history, d.partials, d.debug_g, d.debug_h, d.diag_x, d.diag_xp, d.diag_buf, d.tri_ix, d.tri_inv, d.tri_area, d.hinge_ix,
-                  d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w,
+                  d.hinge_rest, d.tri_row, d.tri_elems, d.hinge_row, d.hinge_elems, d.tri_glob, d.tri_base, d.hinge_base, d.cand, d.lag_n, d.lag_w, d.fric,
                   d.src_c, d.src_w, d.src_d, d.tgt_d, d.tgt_c, d.tgt_w, d.contact_ctl, d.cand_row, d.cand_ids,
                   d.active_count, d.active_ids, d.cand_meta, d.cand_keep, d.vblock_scene, d.tblock_scene, d.rblock_scene,
                   d.cblock_scene, d.scene_blk0, d.scene_blk1, d.scene_cnt, d.scene_end, d.scene_scale, d.scene_lanes, d.scene_kind, d.kind_blocks, d.scene_conv,
@@ -777,6 +777,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
   TRYS(dalloc(&d.cand_meta, Cc, by, dry));
   TRYS(dalloc(&d.cand_keep, Cc, by, dry));
   for (float4** a : {&d.lag_n, &d.lag_w, &d.src_c, &d.src_w, &d.tgt_c, &d.tgt_w}) TRYS(dalloc(a, Cc, by, dry));
+  if (Cc) TRYS(dalloc(&d.fric, 4 * Cc, by, dry));
   TRYS(dalloc(&d.src_d, Cc, by, dry));
   TRYS(dalloc(&d.tgt_d, Cc, by, dry));
   TRYS(dalloc(&d.cand_ids, 4 * Cc, by, dry));
@@ -1346,6 +1347,9 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
   const bool contact = base.contact != 0u;
   base.src = 0;
   base.tgt = 0;
+  // Jacobi sweeps read friction rows from per-candidate records written by
+  // the active-row pass at the sweep's source pose.
+  base.fric_records = (contact && !gs) ? 1u : 0u;
   if (globalization) launch_clear_frame(d, base, q);
   launch_predict(d, base, q);
   if (contact) launch_broad_phase(d, base, true, false, q);  // lists complete at the frame start

@@ -934,19 +934,37 @@ __device__ __forceinline__ void ipc_store_summary(const DeviceState& s, uint32_t
 __device__ __forceinline__ bool ipc_friction_active(const DeviceState& s, uint32_t c) {
   return s.ipc_mu > 0.0f && s.lag_n[c].w > 0.0f;
 }
-__device__ float ipc_energy_from_contact(const DeviceState& s, const PassParams& p, uint32_t c, uint4 ix,
-                                         const Contact& k, const float4* pos, uint32_t mv, f3 mp) {  // S:707-740
+// A candidate's energy at contact `k` with the lagged record `lag` and the
+// friction displacement `rel` (used only with lagged friction).
+__device__ __forceinline__ float ipc_energy_with_rel(const DeviceState& s, const PassParams& p, const Contact& k,
+                                                     float4 lag, f3 rel) {  // S:707-740
   if (!k.valid || !(k.distance > s.ipc_dmin)) return kF32Max;
   float e = p.ipc_kappa * ipc_barrier(k.distance, p.ipc_dhat);
-  const float4 lag = s.lag_n[c];
   if (s.ipc_mu > 0.0f && lag.w > 0.0f) {
     const f3 n = ipc_safe_unit(xyz(lag), k.normal);
-    const f3 rel = ipc_weighted_displacement(s, ix, s.lag_w[c], pos, mv, mp);
     const f3 tang = rel - n * dot(n, rel);
     const float slip_velocity = sqrtf(dot(tang, tang)) * p.inv_dt;
     e += p.dt * ipc_friction_dissipation(slip_velocity, s.ipc_eps, s.ipc_mu, lag.w);
   }
   return e;
+}
+__device__ float ipc_energy_from_contact(const DeviceState& s, const PassParams& p, uint32_t c, uint4 ix,
+                                         const Contact& k, const float4* pos, uint32_t mv, f3 mp) {  // S:707-740
+  if (!k.valid || !(k.distance > s.ipc_dmin)) return kF32Max;
+  const float4 lag = s.lag_n[c];
+  const bool friction = s.ipc_mu > 0.0f && lag.w > 0.0f;
+  return ipc_energy_with_rel(s, p, k, lag, friction ? ipc_weighted_displacement(s, ix, s.lag_w[c], pos, mv, mp) : mk(0, 0, 0));
+}
+// The friction displacement from a candidate's sweep record (fric): the
+// same terms, in the same order, as ipc_weighted_displacement at the
+// sweep's source pose with vertex mv's term (x_mv' - old_mv) given.
+__device__ __forceinline__ f3 ipc_record_displacement(const float4* fr, uint4 ix, float4 w, uint32_t mv, f3 mv_term) {
+  f3 r = mk(0, 0, 0);
+  r = r + w.x * (ix.x == mv ? mv_term : xyz(fr[0]));
+  r = r + w.y * (ix.y == mv ? mv_term : xyz(fr[1]));
+  r = r + w.z * (ix.z == mv ? mv_term : xyz(fr[2]));
+  r = r + w.w * (ix.w == mv ? mv_term : xyz(fr[3]));
+  return r;
 }
 // The record slot (0-3) of the vertex that owns a candidate's energy
 // (S:742-748: the first unpinned vertex, else the first). Pins are fixed at
@@ -966,8 +984,10 @@ __device__ __forceinline__ uint32_t ipc_energy_owner(uint4 ix, uint32_t meta) {
 // Barrier and lagged-friction terms of one vertex (ipcLocalContribution,
 // S:786-874) over its active candidate row.
 // One active candidate's barrier and friction terms for vertex v.
+// `records`: pos is a Jacobi sweep's source pose and the friction records
+// (fric) hold it.
 __device__ void ipc_row_contribution(const DeviceState& s, const PassParams& p, uint32_t v, const float4* pos,
-                                     uint32_t c, f3& g, m3& H) {
+                                     uint32_t c, f3& g, m3& H, bool records = false) {
   {
     if (!p.colored) {  // the same tests on the eight-byte summary first
       const float2 q = s.src_d[c];
@@ -1001,7 +1021,8 @@ __device__ void ipc_row_contribution(const DeviceState& s, const PassParams& p, 
     const float lw = ipc_weight_for_vertex(ix, s.lag_w[c], v);
     if (fa && lw != 0.0f) {
       const f3 n = ipc_safe_unit(xyz(lag), k.normal);
-      const f3 rel = ipc_weighted_displacement(s, ix, s.lag_w[c], pos, kNoVertex, mk(0, 0, 0));
+      const f3 rel = records ? ipc_record_displacement(s.fric + 4 * size_t(c), ix, s.lag_w[c], kNoVertex, mk(0, 0, 0))
+                             : ipc_weighted_displacement(s, ix, s.lag_w[c], pos, kNoVertex, mk(0, 0, 0));
       const f3 tang = rel - n * dot(n, rel);
       const float slip = sqrtf(dot(tang, tang));
       const float fs = s.ipc_mu * lag.w;
@@ -1718,9 +1739,10 @@ __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, u
         L.H.c2 = L.H.c2 + mk(c.y, c.z, c.w);
       }
     } else {
+      const bool records = p.fric_records && !p.colored;
 #pragma unroll 1
       for (uint32_t k = lane_first<LPV>(sub, prefix); k < n; k += LPV)
-        ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], L.g, L.H);
+        ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], L.g, L.H, records);
     }
     prefix += n;
   }
@@ -1878,13 +1900,53 @@ __device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p
     const f3 tp = x + d;
     const float dl = sqrtf(dot(d, d));
     const uint32_t begin = s.cand_row[v].x, n = s.active_count[v];
+    // Jacobi sweeps read a row's source distance from its summary, and a
+    // friction row's source energy and displacement terms from its sweep
+    // record; the moved vertex's own term is formed here.
+    const bool records = p.fric_records && !p.colored;
+    const f3 own = records ? tp - load_pos(s.old, v) : mk(0, 0, 0);
 #pragma unroll 1
     for (uint32_t k = lane_first<LPV>(sub, prefix); k < n; k += LPV) {
       const uint32_t c = s.active_ids[begin + k];
+      float2 q = make_float2(0.0f, 0.0f);
       if (!p.colored) {  // the same tests on the eight-byte summary first
-        const float2 q = s.src_d[c];
+        q = s.src_d[c];
         if (q.x >= 0.0f && q.x - dl > s.ipc_dmin && !(q.x - dl < p.ipc_dhat) && !(s.ipc_mu > 0.0f && q.y > 0.0f))
           continue;
+      }
+      if (records) {
+        if (!(q.x >= 0.0f)) {  // the cached source contact is invalid
+          tr.fail = 2u * (prefix + k) + 1u;
+          return tr;
+        }
+        const bool near = !(q.x - dl > s.ipc_dmin);
+        const bool may_change = q.x - dl < p.ipc_dhat;
+        const bool fa = s.ipc_mu > 0.0f && q.y > 0.0f;
+        if (!near && !may_change && !fa) continue;
+        Contact tc = Contact{p.ipc_dhat, mk(1, 0, 0), true, make_float4(0, 0, 0, 0)};
+        if (near || may_change) tc = ipc_contact_at(s, c, pos, p.ipc_dhat, v, tp);
+        if (near) {
+          if (!tc.valid) {
+            tr.fail = 2u * (prefix + k) + 1u;
+            return tr;
+          }
+          if (!(tc.distance > s.ipc_dmin)) {
+            tr.fail = 2u * (prefix + k);
+            return tr;
+          }
+        }
+        if (!may_change && !fa) continue;
+        if (fa) {
+          const float4* fr = s.fric + 4 * size_t(c);
+          const f3 rel = ipc_record_displacement(fr, s.cand[c], s.lag_w[c], v, own);
+          tr.energy += ipc_energy_with_rel(s, p, tc, s.lag_n[c], rel) - fr[0].w;
+        } else {
+          // The source contact's energy is its barrier term alone.
+          const Contact sc = Contact{q.x, mk(1, 0, 0), true, make_float4(0, 0, 0, 0)};
+          tr.energy += ipc_energy_with_rel(s, p, tc, make_float4(0, 0, 0, 0), mk(0, 0, 0)) -
+                       ipc_energy_with_rel(s, p, sc, make_float4(0, 0, 0, 0), mk(0, 0, 0));
+        }
+        continue;
       }
       const Contact sc = p.colored ? ipc_contact_at(s, c, pos, p.ipc_dhat) : ipc_cached(s, c, false);
       if (!sc.valid) {
@@ -2962,8 +3024,9 @@ __global__ void __launch_bounds__(kBlock) k_ipc_local_lanes(DeviceState s, PassP
   const float4* pos = s.pos[p.src];
   f3 g = mk(0, 0, 0);
   m3 h = mzero();
+  const bool records = p.fric_records != 0u;
 #pragma unroll 1
-  for (uint32_t k = sub; k < n; k += kLanes) ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], g, h);
+  for (uint32_t k = sub; k < n; k += kLanes) ipc_row_contribution(s, p, v, pos, s.active_ids[begin + k], g, h, records);
   g = lanes_sum3<kLanes>(g);
   h = {lanes_sum3<kLanes>(h.c0), lanes_sum3<kLanes>(h.c1), lanes_sum3<kLanes>(h.c2)};
   if (sub == 0) {
@@ -3412,6 +3475,18 @@ __global__ void __launch_bounds__(kBlock) k_candidate_keep(DeviceState s, PassPa
     }
   }
   s.cand_keep[c] = include ? 1u : 0u;
+  if (p.fric_records && s.ipc_mu > 0.0f && q.y > 0.0f) {  // a friction row: its record for this sweep
+    const float4* src = s.pos[p.src];
+    const uint4 ix = s.cand[c];
+    float4* fr = s.fric + 4 * size_t(c);
+    const f3 d0 = load_pos(src, ix.x) - load_pos(s.old, ix.x), d1 = load_pos(src, ix.y) - load_pos(s.old, ix.y);
+    const f3 d2 = load_pos(src, ix.z) - load_pos(s.old, ix.z), d3 = load_pos(src, ix.w) - load_pos(s.old, ix.w);
+    const float e = ipc_energy_from_contact(s, p, c, ix, ipc_cached(s, c, false), src, kNoVertex, mk(0, 0, 0));
+    fr[0] = make_float4(d0.x, d0.y, d0.z, e);
+    fr[1] = make_float4(d1.x, d1.y, d1.z, 0.0f);
+    fr[2] = make_float4(d2.x, d2.y, d2.z, 0.0f);
+    fr[3] = make_float4(d3.x, d3.y, d3.z, 0.0f);
+  }
 }
 __global__ void __launch_bounds__(kBlock) k_compact_active_rows(DeviceState s, PassParams p) {
   constexpr uint32_t kLanes = 16;
