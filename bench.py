@@ -261,7 +261,7 @@ def main(argv=None):
         "batched_over_plain": value / plain["vertex_updates_per_s"] if plain and plain["vertex_updates_per_s"] > 0 else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_solve (local solve)",
+            "kernel": "k_solve_lanes (local solve, every launch of a sweep)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -552,7 +552,7 @@ def measure_large(cells, iterations, steps):
             solve_ms, launches, _ = solver.profile_end()
             b = scenes.algorithmic_bytes_per_sweep(scene.vertex_count, scene.tet_count, scene.cubature_k)
             achieved = b / (solve_ms * 1e-3) / 1e9
-            leg["roofline"] = {"bound": "hbm", "kernel": "k_solve (local solve)", "achieved": achieved,
+            leg["roofline"] = {"bound": "hbm", "kernel": "k_solve_lanes (local solve, every launch of a sweep)", "achieved": achieved,
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                                "algorithmic_bytes_per_launch": b, "ms_per_launch": solve_ms,
                                "launches_profiled": launches}
