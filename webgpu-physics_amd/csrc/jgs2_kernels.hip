@@ -3531,10 +3531,50 @@ __device__ __forceinline__ void contact_block_partial(const DeviceState& s, bool
   block_reduce(r, sh, contact_min);
   if (threadIdx.x == 0) reinterpret_cast<float4*>(s.partials)[blockIdx.x] = r;
 }
+// The largest step length |x_T - x_S| (as the safe-step pass forms it) of
+// each scene with candidates, in contact_ctl[scene].x: per-block maxima
+// over the scene's vertex blocks, then one block per scene (a non-finite
+// step gives +inf). A candidate's Lipschitz bound l_a + max(l_b..) (VT) or
+// max + max (EE) is at most twice it, rounding included (f32 addition is
+// monotone and 2M is exact), so a candidate whose cached source distance
+// exceeds the minimum distance by twice the bound takes the full step and
+// its record is known without loading its eight positions.
+__global__ void __launch_bounds__(kBlock) k_step_bound_partials(DeviceState s, PassParams p) {
+  __shared__ float sh[kBlock];
+  const uint32_t sc = s.vblock_scene[blockIdx.x];
+  if (sc == kNoScene) return;                 // uniform per block
+  const uint4 b1 = s.scene_blk1[sc];
+  if (b1.z == b1.w) return;                   // no candidates: uniform per block
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  float l = 0.0f;
+  if (v < p.vertex_count) {
+    const f3 d = load_pos(s.pos[p.tgt], v) - load_pos(s.pos[p.src], v);
+    l = sqrtf(dot(d, d));
+    if (!(l <= kF32Max)) l = INFINITY;       // NaN or inf: no candidate skips
+  }
+  block_reduce(l, sh, [](float& a, const float& b) { a = fmaxf(a, b); });
+  if (threadIdx.x == 0) reinterpret_cast<float4*>(s.partials)[blockIdx.x] = make_float4(l, 0, 0, 0);
+}
+__global__ void __launch_bounds__(kBlock) k_step_bound_final(DeviceState s) {
+  __shared__ float sh[kBlock];
+  const uint32_t sc = blockIdx.x;
+  const uint4 b0 = s.scene_blk0[sc], b1 = s.scene_blk1[sc];
+  if (s.scene_cnt[sc].z == 0u || b1.z == b1.w) return;
+  float m = 0.0f;
+  const float4* parts = reinterpret_cast<const float4*>(s.partials);
+  for (uint32_t i = b0.x + threadIdx.x; i < b0.y; i += kBlock) m = fmaxf(m, parts[i].x);
+  block_reduce(m, sh, [](float& a, const float& b) { a = fmaxf(a, b); });
+  if (threadIdx.x == 0) s.contact_ctl[sc].x = m;
+}
 // candidateContactStep (S:2788-2852): per-candidate conservative step cap.
 __device__ __forceinline__ float4 candidate_contact_step(const DeviceState& s, const PassParams& p, uint32_t c,
                                                          uint32_t sc) {
   if (stop_latched(s, p, sc)) return make_float4(1, 0, 0, 1);
+  {  // far from contact: the full step, the record it would compute
+    const float2 q = s.src_d[c];
+    const float m = s.contact_ctl[sc].x;
+    if (q.x >= 0.0f && q.x - (m + m) > s.ipc_dmin) return make_float4(1.0f, q.x, q.x, 1.0f);
+  }
   const uint4 ix = s.cand[c];
   const float4* src = s.pos[p.src];
   const float4* tgt = s.pos[p.tgt];
@@ -4514,7 +4554,8 @@ inline bool has_cloth(const PassParams& p) { return p.tri_count != 0u || p.hinge
 
 uint32_t partial_blocks_needed(uint32_t vertices, uint32_t tets, uint32_t triangles, uint32_t candidates) {
   // Candidate partials use blocks(T) + blocks(triangles) + blocks(V);
-  // convergence 2 blocks(V); the contact step blocks(candidates).
+  // convergence 2 blocks(V); the contact step blocks(candidates) and its
+  // step bound blocks(V).
   const uint32_t a = blocks_for(tets) + blocks_for(triangles) + blocks_for(vertices);
   const uint32_t b = 2 * blocks_for(vertices);
   const uint32_t c = blocks_for(candidates);
@@ -4740,6 +4781,8 @@ void launch_build_active_rows(const DeviceState& s, const PassParams& p, hipStre
 void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t st) {
   const uint32_t cb = blocks_for(s.NC), vb = blocks_for(p.vertex_count);
   if (s.NC == 0) return;
+  hipLaunchKernelGGL(k_step_bound_partials, dim3(vb), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_step_bound_final, dim3(p.scene_count), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_candidate_contact_step, dim3(cb), dim3(kBlock), 0, st, s, p);
   hipLaunchKernelGGL(k_reduce_contact_final, dim3(p.scene_count), dim3(kBlock), 0, st, s);
   hipLaunchKernelGGL(k_apply_contact_step, dim3(vb), dim3(kBlock), 0, st, s, p, 1u);
