@@ -1046,10 +1046,13 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
       cand[c] = make_uint4(uint32_t(VB) + vnew[ix[0]], uint32_t(VB) + vnew[ix[1]], uint32_t(VB) + vnew[ix[2]],
                            uint32_t(VB) + vnew[ix[3]]);
       // Bit 0: edge-edge; bits 1-2: the record slot of the energy owner,
-      // the first unpinned vertex, else the first (S:742-748).
+      // the first unpinned vertex, else the first (S:742-748); bit 3: a
+      // vertex is pinned.
       uint32_t owner = 0;
       while (owner < 4 && in.vertex_info[4 * size_t(ix[owner]) + 2] != 0u) ++owner;
-      cand_meta[c] = (c < s->NVT ? 0u : 1u) | ((owner < 4 ? owner : 0u) << 1);
+      bool pinned = false;
+      for (int l = 0; l < 4; ++l) pinned = pinned || in.vertex_info[4 * size_t(ix[l]) + 2] != 0u;
+      cand_meta[c] = (c < s->NVT ? 0u : 1u) | ((owner < 4 ? owner : 0u) << 1) | (pinned ? 8u : 0u);
     }
     TRYS(rows_of(in.contact->packed_indices, 4, NC, s->cel.at, CB, cand_row, cand_ids));
   }

@@ -977,6 +977,15 @@ __device__ __forceinline__ uint32_t ipc_owner_slot(const DeviceState& s, uint4 i
   if (s.vtx_info[ix.w].z == 0u) return 3u;
   return 0u;
 }
+// cand_meta of a record: bit 0 edge-edge, bits 1-2 the energy owner's
+// slot, bit 3 set when one of its vertices is pinned (the active-row pass
+// checks pinned motion only for those).
+constexpr uint32_t kMetaPinned = 8u;
+__device__ __forceinline__ uint32_t ipc_meta(const DeviceState& s, uint4 ix, bool ee) {
+  const bool pinned = s.vtx_info[ix.x].z != 0u || s.vtx_info[ix.y].z != 0u || s.vtx_info[ix.z].z != 0u ||
+                      s.vtx_info[ix.w].z != 0u;
+  return (ee ? 1u : 0u) | (ipc_owner_slot(s, ix) << 1) | (pinned ? kMetaPinned : 0u);
+}
 __device__ __forceinline__ uint32_t ipc_energy_owner(uint4 ix, uint32_t meta) {
   const uint32_t slot = (meta >> 1) & 3u;
   return slot == 0u ? ix.x : slot == 1u ? ix.y : slot == 2u ? ix.z : ix.w;
@@ -3462,7 +3471,7 @@ __global__ void __launch_bounds__(kBlock) k_candidate_keep(DeviceState s, PassPa
   const float bound = fmaxf(p.max_step, 0.0f);
   const float2 q = s.src_d[c];  // the cached source contact's validity and distance, and the lagged force
   bool include = q.x < 0.0f || !bounded || q.x <= reach || (s.ipc_mu > 0.0f && q.y > 0.0f);
-  if (!include) {  // ipcCandidatePinnedMotionExceedsBound (S:484-507)
+  if (!include && (s.cand_meta[c] & kMetaPinned)) {  // ipcCandidatePinnedMotionExceedsBound (S:484-507)
     const float4* src = s.pos[p.src];
     const uint4 ix = s.cand[c];
 #pragma unroll
@@ -4084,7 +4093,7 @@ __device__ void bp_vt_item(const DeviceState& s, const PassParams& p, uint32_t i
           if (slot < sz.z) {
             const uint4 rec = make_uint4(dv, ix.x, ix.y, ix.z);
             s.cand[cb + slot] = rec;
-            s.cand_meta[cb + slot] = ipc_owner_slot(s, rec) << 1;
+            s.cand_meta[cb + slot] = ipc_meta(s, rec, false);
             s.cand_key[cb + slot] = make_uint2(q, tl);
           }
         }
@@ -4171,7 +4180,7 @@ __device__ void bp_ee_item(const DeviceState& s, const PassParams& p, uint32_t i
           if (slot < sz.z) {
             const uint4 rec = make_uint4(e.x, e.y, o.x, o.y);
             s.cand[cb + slot] = rec;
-            s.cand_meta[cb + slot] = 1u | (ipc_owner_slot(s, rec) << 1);
+            s.cand_meta[cb + slot] = ipc_meta(s, rec, true);
             s.cand_key[cb + slot] = make_uint2(i | 0x80000000u, jl);
           }
         }
