@@ -4037,6 +4037,11 @@ __device__ void bp_vt_item(const DeviceState& s, const PassParams& p, uint32_t i
     if (!kWrite && lane == 0u) s.bp_vcnt[qa] = 0u;
     return;
   }
+  if (kWrite) {  // a query with no match writes nothing: its offset equals the next one's
+    const uint32_t next = q + 1u < s.scene_cnt[sc].x ? s.bp_vcnt[qa + 1u]
+                          : s.bp_size[sc].y ? s.bp_ecnt[s.scene_blk2[sc].z * kSceneBlock] : s.bp_ctl[sc].w;
+    if (next == s.bp_vcnt[qa]) return;  // uniform per wavefront
+  }
   const uint32_t dv = s.bp_qv[qa];
   const f3 x = load_pos(s.pos[p.src], dv);
   const bool wild = bp_wild(s, p, dv, x);
@@ -4120,6 +4125,10 @@ __device__ void bp_ee_item(const DeviceState& s, const PassParams& p, uint32_t i
   if (i >= sz.y) {  // padding: no matches (uniform per wavefront)
     if (!kWrite && lane == 0u) s.bp_ecnt[ea] = 0u;
     return;
+  }
+  if (kWrite) {  // a query with no match writes nothing: its offset equals the next one's
+    const uint32_t next = i + 1u < sz.y ? s.bp_ecnt[ea + 1u] : s.bp_ctl[sc].w;
+    if (next == s.bp_ecnt[ea]) return;  // uniform per wavefront
   }
   const uint2 e = s.bp_edge[ea];
   const float4 elo = s.bp_ebox[2 * size_t(ea)], ehi = s.bp_ebox[2 * size_t(ea) + 1];
