@@ -601,7 +601,30 @@ __device__ Dihedral dihedral_at(const DeviceState& s, uint32_t h, const float4* 
   // dn1, es0 and es1 >= el), and den = sine^2 + cosine^2 of unit normals
   // is 1 to rounding, so every entry is below 1e30. Shorter edges form
   // them all, as the reference does.
-  const uint32_t corners = (slot < 4u || store != nullptr || !(el >= 1.0e-20f)) ? 4u : 0u;
+  uint32_t corners = (slot < 4u || store != nullptr || !(el >= 1.0e-20f)) ? 4u : 0u;
+  if (corners && el >= 1.0e-20f) {
+    // The corner gradients in closed form, equal in exact arithmetic to the
+    // twelve entries below (pinned against the FP64 oracle): with N0 = rn0,
+    // N1 = rn1 and a_k = N_k / |N_k|^2, grad_2 = -|e| a_0, grad_3 = -|e| a_1,
+    // grad_0 = ((x1 - x2).e a_0 + (x1 - x3).e a_1) / |e|,
+    // grad_1 = ((x2 - x0).e a_0 + (x3 - x0).e a_1) / |e|. When all are finite
+    // so are the entries (el >= 1e-20, above), and validity is unchanged;
+    // otherwise the entries are formed as the reference does.
+    const f3 a0 = rn0 / dot(rn0, rn0), a1 = rn1 / dot(rn1, rn1);
+    const f3 g2 = (-el) * a0, g3 = (-el) * a1;
+    const f3 g0 = (dot(x1 - x2, edge) / el) * a0 + (dot(x1 - x3, edge) / el) * a1;
+    const f3 g1 = (dot(x2 - x0, edge) / el) * a0 + (dot(x3 - x0, edge) / el) * a1;
+    if (finite3(g0) && finite3(g1) && finite3(g2) && finite3(g3)) {
+      corners = 0u;
+      keep = slot == 0u ? g0 : slot == 1u ? g1 : slot == 2u ? g2 : slot == 3u ? g3 : mk(0, 0, 0);
+      if (store) {
+        store[0] = make_float4(g0.x, g0.y, g0.z, 0.0f);
+        store[1] = make_float4(g1.x, g1.y, g1.z, 0.0f);
+        store[2] = make_float4(g2.x, g2.y, g2.z, 0.0f);
+        store[3] = make_float4(g3.x, g3.y, g3.z, 0.0f);
+      }
+    }
+  }
 #pragma unroll 1
   for (uint32_t dv = 0; dv < corners; ++dv) {
     float gc[3];
