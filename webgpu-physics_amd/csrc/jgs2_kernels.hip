@@ -3700,7 +3700,9 @@ __global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassP
   if (s.control[4 * size_t(sc) + 1].x == 1.0f && finite_s(alpha) && alpha > 0.0f) {
     s.src_c[c] = s.tgt_c[c];
     s.src_w[c] = s.tgt_w[c];
-    s.src_d[c] = make_float2(s.tgt_d[c], s.lag_n[c].w);  // the summary of the promoted cache
+    // The summary of the promoted cache: its distance; the lagged force
+    // (the summary's second word) is unchanged within a frame.
+    reinterpret_cast<float*>(s.src_d)[2 * size_t(c)] = s.tgt_d[c];
   }
 }
 
