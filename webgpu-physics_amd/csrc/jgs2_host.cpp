@@ -1072,10 +1072,12 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
       const uint4 inf = vtx_info[v];
       const float n = float(inf.y > 1u ? inf.y : 1u);
       const float w = inf.z != 0u ? -1.0f : vtx_rest[v].w * float(1.0 / (s->defaults.timestep * s->defaults.timestep)) / n;
-      vc[4 * v] = make_float4(1, 0, 0, x.x);
-      vc[4 * v + 1] = make_float4(0, 1, 0, x.y);
-      vc[4 * v + 2] = make_float4(0, 0, 1, x.z);
-      vc[4 * v + 3] = make_float4(0, 0, 0, w);
+      // {x, w}, {R.c0, diff.x}, {R.c1, diff.y}, {R.c2, diff.z}; diff is
+      // rewritten by the first polar pass.
+      vc[4 * v] = make_float4(x.x, x.y, x.z, w);
+      vc[4 * v + 1] = make_float4(1, 0, 0, 0);
+      vc[4 * v + 2] = make_float4(0, 1, 0, 0);
+      vc[4 * v + 3] = make_float4(0, 0, 1, 0);
     }
     UP(d.vcub[0], 4 * VB, vc);
     UP(d.vcub[1], 4 * VB, vc);

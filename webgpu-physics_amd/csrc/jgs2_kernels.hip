@@ -250,21 +250,20 @@ struct VCub {
   float w;     // m / (h^2 n); 0 for a pinned vertex
   bool pinned;
 };
+// Layout: {x, w}, {R.c0, diff.x}, {R.c1, diff.y}, {R.c2, diff.z}, so a
+// sample tetrahedron's pose is one aligned load per vertex.
 __device__ __forceinline__ VCub load_vcub(const float4* vc, uint32_t v) {
   const float4* r = vc + 4 * size_t(v);
   const float4 a = r[0], b = r[1], c = r[2], d = r[3];
   VCub o;
-  o.R = {xyz(a), xyz(b), xyz(c)};
-  o.x = mk(a.w, b.w, c.w);
-  o.diff = xyz(d);
-  o.pinned = d.w < 0.0f;
-  o.w = o.pinned ? 0.0f : d.w;
+  o.R = {xyz(b), xyz(c), xyz(d)};
+  o.x = xyz(a);
+  o.diff = mk(b.w, c.w, d.w);
+  o.pinned = a.w < 0.0f;
+  o.w = o.pinned ? 0.0f : a.w;
   return o;
 }
-__device__ __forceinline__ f3 vcub_pose(const float4* vc, uint32_t v) {
-  const float4* r = vc + 4 * size_t(v);
-  return mk(r[0].w, r[1].w, r[2].w);
-}
+__device__ __forceinline__ f3 vcub_pose(const float4* vc, uint32_t v) { return xyz(vc[4 * size_t(v)]); }
 // Inertia weight of a vertex's distributed share (S:2401-2408).
 __device__ __forceinline__ float share_weight(const DeviceState& s, const PassParams& p, uint32_t v, uint4 inf) {
   const float n = float(inf.y > 1u ? inf.y : 1u);
@@ -275,10 +274,10 @@ __device__ __forceinline__ void store_vcub(const DeviceState& s, const PassParam
   const f3 diff = x - xyz(s.predicted[v]);
   const float w = inf.z != 0u ? -1.0f : share_weight(s, p, v, inf);
   float4* r = vc + 4 * size_t(v);
-  r[0] = make_float4(R.c0.x, R.c0.y, R.c0.z, x.x);
-  r[1] = make_float4(R.c1.x, R.c1.y, R.c1.z, x.y);
-  r[2] = make_float4(R.c2.x, R.c2.y, R.c2.z, x.z);
-  r[3] = make_float4(diff.x, diff.y, diff.z, w);
+  r[0] = make_float4(x.x, x.y, x.z, w);
+  r[1] = make_float4(R.c0.x, R.c0.y, R.c0.z, diff.x);
+  r[2] = make_float4(R.c1.x, R.c1.y, R.c1.z, diff.y);
+  r[3] = make_float4(R.c2.x, R.c2.y, R.c2.z, diff.z);
 }
 // The record paired with this sweep's frames (rot_source).
 __device__ __forceinline__ const float4* vcub_source(const DeviceState& s, const PassParams& p) {
