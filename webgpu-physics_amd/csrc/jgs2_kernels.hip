@@ -4107,8 +4107,9 @@ __device__ void bp_vt_item(const DeviceState& s, const PassParams& p, uint32_t i
           const size_t t = t0 + tl;
           const float4 lo = s.bp_tbox[2 * t], hi = s.bp_tbox[2 * t + 1];
           in = wild || lo.w != 0.0f || !(bp_lower_bound(x, x, xyz(lo), xyz(hi)) > reach);
-          ix = s.bp_tri[t];
+          if (in) ix = s.bp_tri[t];
         }
+        if (__ballot(in) == 0ull) continue;  // no triangle of the chunk within reach (uniform per wavefront)
         // isInClosedOneRing (ipc-contact.ts:586-592) against the query vertex.
         bool ringed = ix.x == dv || ix.y == dv || ix.z == dv;
         for (uint32_t j = 0; j < ring.y; ++j) {
@@ -4193,8 +4194,9 @@ __device__ void bp_ee_item(const DeviceState& s, const PassParams& p, uint32_t i
           const size_t j = e0 + jl;
           const float4 lo = s.bp_ebox[2 * j], hi = s.bp_ebox[2 * j + 1];
           in = wild || lo.w != 0.0f || !(bp_lower_bound(xyz(elo), xyz(ehi), xyz(lo), xyz(hi)) > reach);
-          o = s.bp_edge[j];
+          if (in) o = s.bp_edge[j];
         }
+        if (__ballot(in) == 0ull) continue;  // no edge of the chunk within reach (uniform per wavefront)
         // Topologically adjacent edges (ipc-contact.ts:647-651): an endpoint of
         // one in the closed one-ring of an endpoint of the other.
         bool adjacent = o.x == e.x || o.x == e.y || o.y == e.x || o.y == e.y;
