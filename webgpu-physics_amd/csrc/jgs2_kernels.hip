@@ -674,9 +674,12 @@ __device__ __forceinline__ float bending_energy_at(const DeviceState& s, uint32_
 // same functions produce them, so they equal the inline evaluations.
 //   tri_base[3t]  {f0, energy}, {f1, 0}, {stress, 0}
 //   hinge_base[4h] {grad0, angle}, {grad1, valid}, {grad2, bending energy}, {grad3, 0}
+// kRecords: the caller launches only with p.cloth_base set, so the inline
+// evaluation is not compiled into it (and does not raise its register count).
+template <bool kRecords = false>
 __device__ __forceinline__ Membrane membrane_base(const DeviceState& s, const PassParams& p, uint32_t tri,
                                                   const float4* pos) {
-  if (!p.cloth_base) return membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0));
+  if (!kRecords && !p.cloth_base) return membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0));
   const float4* r = s.tri_base + 3 * size_t(tri);
   const float4 a = r[0], b = r[1], c = r[2];
   Membrane m;
@@ -686,9 +689,10 @@ __device__ __forceinline__ Membrane membrane_base(const DeviceState& s, const Pa
   m.stress = xyz(c);
   return m;
 }
+template <bool kRecords = false>
 __device__ __forceinline__ Dihedral dihedral_base(const DeviceState& s, const PassParams& p, uint32_t h,
                                                   const float4* pos, uint32_t slot) {
-  if (!p.cloth_base) return dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
+  if (!kRecords && !p.cloth_base) return dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), slot);
   const float4* r = s.hinge_base + 4 * size_t(h);
   Dihedral d;
   d.angle = r[0].w;
@@ -3071,7 +3075,9 @@ __global__ void __launch_bounds__(kBlock) k_ipc_local_lanes(DeviceState s, PassP
 // 256-vertex arena block (its partial record), in LPV rounds of 256 / LPV
 // vertices; the lanes of a vertex split its incident tetrahedra and combine
 // the material gradient and the rotation average with xor butterflies.
-template <bool kCloth, bool kIpc, int LPV>
+// kRecords: cloth element records at the candidate are present (Jacobi
+// sweeps), so the cloth terms are read from them only.
+template <bool kCloth, bool kIpc, int LPV, bool kRecords = false>
 __global__ void __launch_bounds__(kBlock) k_convergence_gradient_lanes(DeviceState s, PassParams p) {
   __shared__ ConvPartial sh[kBlock];
   ConvPartial r;
@@ -3122,14 +3128,14 @@ __global__ void __launch_bounds__(kBlock) k_convergence_gradient_lanes(DeviceSta
           for (uint32_t k = lane_first<LPV>(sub, 0); k < trr.y; k += LPV) {
             const uint32_t tri = s.tri_elems[trr.x + k];
             const uint32_t slot = tri_slot(s.tri_ix[tri], v);
-            if (slot < 3) cg = cg + membrane_gradient(s, tri, membrane_base(s, p, tri, pos), slot);
+            if (slot < 3) cg = cg + membrane_gradient(s, tri, membrane_base<kRecords>(s, p, tri, pos), slot);
           }
 #pragma unroll 1
           for (uint32_t k = lane_first<LPV>(sub, trr.y); k < hr.y; k += LPV) {
             const uint32_t h = s.hinge_elems[hr.x + k];
             const uint32_t slot = hinge_slot(s.hinge_ix[h], v);
             if (slot >= 4) continue;
-            const Dihedral d = dihedral_base(s, p, h, pos, slot);
+            const Dihedral d = dihedral_base<kRecords>(s, p, h, pos, slot);
             if (!d.valid) {
               cg = cg + mk(kF32Max, kF32Max, kF32Max);
             } else {
@@ -4773,13 +4779,18 @@ void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t s
     if (ipc) hipLaunchKernelGGL(k_ipc_gradient_lanes, dim3(blocks_for(16u * p.vertex_count)), dim3(kBlock), 0, st, s, p);
     // The convergence gather splits a vertex over at most 8 lanes.
     const uint32_t lanes = p.solve_lanes >= 8u ? 8u : 4u;
+    const bool rec = p.cloth_base != 0u;
     if (lanes == 8u) {
-      if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, true, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      if (cl && ipc && rec) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, true, 8, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else if (cl && rec) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, false, 8, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, true, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
       else if (cl) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, false, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
       else if (ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<false, true, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
       else hipLaunchKernelGGL((k_convergence_gradient_lanes<false, false, 8>), dim3(vb), dim3(kBlock), 0, st, s, p);
     } else {
-      if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, true, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      if (cl && ipc && rec) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, true, 4, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else if (cl && rec) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, false, 4, true>), dim3(vb), dim3(kBlock), 0, st, s, p);
+      else if (cl && ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, true, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
       else if (cl) hipLaunchKernelGGL((k_convergence_gradient_lanes<true, false, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
       else if (ipc) hipLaunchKernelGGL((k_convergence_gradient_lanes<false, true, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
       else hipLaunchKernelGGL((k_convergence_gradient_lanes<false, false, 4>), dim3(vb), dim3(kBlock), 0, st, s, p);
