@@ -2938,6 +2938,7 @@ jgs2_status jgs2_batch_autotune(jgs2_batch* b, const jgs2_settings* ov, uint64_t
   if (err == hipSuccess) err = hipStreamSynchronize(b->stream);
   double best = INFINITY;
   uint32_t best_block = keep_block, best_remap = keep_remap;
+  double med[kShapes];
   for (uint32_t c = 0; c < kShapes && err == hipSuccess; ++c) {
     std::vector<float> t(repeats);
     for (uint32_t k = 0; k < repeats && err == hipSuccess; ++k) {
@@ -2945,11 +2946,24 @@ jgs2_status jgs2_batch_autotune(jgs2_batch* b, const jgs2_settings* ov, uint64_t
       err = hipEventElapsedTime(&t[k], e[0], e[1]);
     }
     std::nth_element(t.begin(), t.begin() + repeats / 2, t.end());
-    const double med = t[repeats / 2];
-    if (err == hipSuccess && med < best) {
-      best = med;
+    med[c] = t[repeats / 2];
+    if (err == hipSuccess && med[c] < best) {
+      best = med[c];
       best_block = blocks[c % 3];
       best_remap = c / 3;
+    }
+  }
+  // Within 2 % of the fastest the XCD order wins: it keeps the vertices that
+  // share elements on one L2 and moves about half the L2 misses at the same
+  // time, which leaves the memory system to the other passes of a frame.
+  if (err == hipSuccess && best_remap == 0u) {
+    uint32_t pick = kShapes;
+    for (uint32_t c = 3; c < kShapes; ++c)
+      if (med[c] <= 1.02 * best && (pick == kShapes || med[c] < med[pick])) pick = c;
+    if (pick < kShapes) {
+      best = med[pick];
+      best_block = blocks[pick % 3];
+      best_remap = 1u;
     }
   }
   for (auto& e : ev)
