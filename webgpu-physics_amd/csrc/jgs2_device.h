@@ -39,8 +39,8 @@ struct PassParams {
   uint32_t cloth_base;          // cloth element records hold this pass's source pose (k_cloth_base ran)
   uint32_t ipc_local;           // the solve's contact gather terms are in ipc_local (k_ipc_local_lanes ran)
   uint32_t cub_local;           // the lane sweep's cubature gather terms are in cub_local (k_cub_local_lanes ran)
-  uint32_t kind_filter;         // lane sweep over: 0 every scene; 1, 2, 3 the scenes of that kind_blocks list
-  uint32_t kind_blocks1, kind_blocks2, kind_blocks3;  // lengths of the three kind_blocks lists
+  uint32_t kind_filter;         // lane sweep over: 0 every scene; 1-4 the scenes of that kind_blocks list
+  uint32_t kind_blocks1, kind_blocks2, kind_blocks3, kind_blocks4;  // lengths of the four kind_blocks lists
   uint32_t fric_records;        // Jacobi sweeps: k_candidate_keep writes the friction records (fric)
 };
 
@@ -239,9 +239,11 @@ struct DeviceState {
   uint4* scene_end = nullptr;   // absolute ends {vertex, tet, triangle, candidate}
   float* scene_scale = nullptr;
   uint32_t* scene_lanes = nullptr;  // lanes per vertex of the scene's local solve (1 = one thread per vertex)
-  uint32_t* scene_kind = nullptr;   // per scene: bit 0 cloth elements, bit 1 contact candidates, bit 2 cubature (kKind*)
-  uint32_t* kind_blocks = nullptr;  // vertex blocks of lane-swept scenes: kind 0 (no cloth or contact), the others,
-                                    // then the others that hold cubature records (counts in PassParams)
+  uint32_t* scene_kind = nullptr;   // per scene: bit 0 cloth elements, bit 1 contact candidates, bit 2 cubature,
+                                    // bit 3 tetrahedral terms at an unpinned vertex (kKind*)
+  uint32_t* kind_blocks = nullptr;  // vertex blocks of lane-swept scenes: kind 0 (no cloth or contact), the others
+                                    // with tetrahedral terms, those of them with cubature records, then the
+                                    // others without (counts in PassParams)
   void* scene_conv = nullptr;   // per-scene convergence scale records
   uint4* body_rng = nullptr;    // per body {scene vertex begin, end, 0, 0}
 };
@@ -254,6 +256,7 @@ constexpr uint32_t kHistoryCapacity = 64;
 constexpr uint32_t kKindCloth = 1u;
 constexpr uint32_t kKindContact = 2u;
 constexpr uint32_t kKindCubature = 4u;  // a cubature record in use
+constexpr uint32_t kKindTets = 8u;      // an unpinned vertex with incident tetrahedra or cubature records
 
 // Launchers (jgs2_kernels.hip). All enqueue on `stream`.
 void launch_predict(const DeviceState& s, const PassParams& p, hipStream_t stream);

@@ -139,6 +139,7 @@ struct jgs2_solver {
   bool corotated = false;
   uint32_t V = 0, T = 0, K = 0, NT = 0, NH = 0, NC = 0, NVT = 0, bodies = 1, color_count = 1, free_vertices = 0;
   uint32_t cub_used = 0;  // cubature records in use (tetrahedron set, weight > 0)
+  bool free_tets = false;  // an unpinned vertex has an incident tetrahedron or a cubature record in use
   // Arena spans: vertex/tet/triangle/candidate spans in 256-entry blocks;
   // the others in entries.
   Span vblk, tblk, rblk, cblk, adj, hin, tel, hel, cel, body;
@@ -185,7 +186,7 @@ struct jgs2_batch {
   uint64_t synced_version = ~0ull;
   uint32_t color_count = 1;
   std::vector<uint32_t> color_offsets;
-  uint32_t kind_blocks_n[3] = {0, 0, 0};  // lengths of the dev.kind_blocks lists
+  uint32_t kind_blocks_n[4] = {0, 0, 0, 0};  // lengths of the dev.kind_blocks lists
   // Launch shape of the local solve. Block size and XCD remap leave every
   // result unchanged (autotuned); lanes per vertex set the summation order
   // of the gathered terms and stay fixed unless a caller sets them.
@@ -706,12 +707,14 @@ uint32_t scene_lanes(const jgs2_batch* b, const jgs2_solver* s) {
 }
 
 
-// Element terms beyond tetrahedra that a scene holds (scene_kind): the lane
-// sweep of a mixed batch solves scenes of kind 0 with the tetrahedral
-// kernel variant and the others with the batch's full variant.
+// Element terms that a scene holds (scene_kind): the Jacobi lane sweep of a
+// batch with cloth or contact solves scenes without either with the
+// tetrahedral kernel variant, the others with the batch's full variant, or
+// with its variant without tetrahedral terms when no unpinned vertex has
+// any (a sheet over a pinned tetrahedral collider).
 uint32_t scene_kind(const jgs2_solver* s) {
   return ((s->NT > 0 || s->NH > 0) ? kKindCloth : 0u) | ((s->NC > 0 || s->bp) ? kKindContact : 0u) |
-         (s->cub_used > 0 ? kKindCubature : 0u);
+         (s->cub_used > 0 ? kKindCubature : 0u) | (s->free_tets ? kKindTets : 0u);
 }
 
 // Allocates every arena array for the configured capacities.
@@ -937,6 +940,7 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
       // order); entries are arena tet ids.
       for (uint32_t k = 0; k < i[1]; ++k) adjacency[cursor + k] = uint32_t(TB) + tnew[in.adjacency[i[0] + k]];
       cursor += i[1];
+      if (i[2] == 0u && i[1] > 0u) s->free_tets = true;
       const float* x = &in.positions[4 * size_t(ov)];
       positions[nv] = make_float4(x[0], x[1], x[2], 1.0f);
       if (in.velocities)
@@ -994,7 +998,10 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
       std::memcpy(&wbits, &in.cubature_weights[ro], 4);
       const uint32_t t = in.cubature_tet_ids[ro];
       cub[10 * rn] = make_uint4(t < T ? uint32_t(TB) + tnew[t] : 0xffffffffu, wbits, 0, 0);
-      if (t < T && in.cubature_weights[ro] > 0) ++s->cub_used;
+      if (t < T && in.cubature_weights[ro] > 0) {
+        ++s->cub_used;
+        if (in.vertex_info[4 * size_t(vold[nv]) + 2] == 0u) s->free_tets = true;
+      }
       std::memcpy(&cub[10 * rn + 1], &in.cubature_basis[36 * ro], 36 * sizeof(float));
     }
   // Cloth records, vertex -> element CSR rows {begin, count}.
@@ -1252,6 +1259,7 @@ PassParams make_params(jgs2_batch* b, const jgs2_settings& st, bool stop) {
   p.kind_blocks1 = b->kind_blocks_n[0];
   p.kind_blocks2 = b->kind_blocks_n[1];
   p.kind_blocks3 = b->kind_blocks_n[2];
+  p.kind_blocks4 = b->kind_blocks_n[3];
   p.xcd_remap = b->xcd_remap;
   // packUniforms (jgs2-solver.ts:604-730): f64 expressions rounded to f32.
   const double inv = 1.0 / st.timestep;
@@ -1304,19 +1312,19 @@ jgs2_status sync_colors(jgs2_batch* b) {
   }
   // Vertex blocks of the lane-swept scenes by kind, in arena order: the lane
   // sweep of a mixed batch launches over each list instead of every block.
-  std::vector<uint32_t> kinds[3];
+  std::vector<uint32_t> kinds[4];
   for (auto* s : order) {
     if (scene_lanes(b, s) <= 1u) continue;
     const uint32_t k = scene_kind(s);
     const bool plain = (k & (kKindCloth | kKindContact)) == 0u;
     for (uint64_t i = 0; i < s->vblk.n; ++i) {
       const uint32_t blk = uint32_t(s->vblk.at + i);
-      kinds[plain ? 0 : 1].push_back(blk);
+      kinds[plain ? 0 : (k & kKindTets) ? 1 : 3].push_back(blk);
       if (!plain && (k & kKindCubature)) kinds[2].push_back(blk);
     }
   }
   std::vector<uint32_t> all;
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < 4; ++i) {
     b->kind_blocks_n[i] = uint32_t(kinds[i].size());
     all.insert(all.end(), kinds[i].begin(), kinds[i].end());
   }

@@ -157,13 +157,20 @@ __device__ __forceinline__ bool stop_latched(const DeviceState& s, const PassPar
   return (control_bits(s.control[4 * size_t(sc)].w) & kStopLatch) != 0u;
 }
 // Whether a lane sweep with kind filter `f` solves scene `sc` (0: every
-// scene; 1: scenes without cloth or contact; 2: the others; 3: the others
-// that hold cubature records).
-__device__ __forceinline__ bool kind_selected(const DeviceState& s, uint32_t f, uint32_t sc) {
-  if (f == 0u) return true;
-  const uint32_t k = s.scene_kind[sc];
-  const bool plain = (k & (kKindCloth | kKindContact)) == 0u;
-  return f == 1u ? plain : (!plain && (f == 2u || (k & kKindCubature) != 0u));
+// scene; 1: scenes without cloth or contact; 2: the others with
+// tetrahedral terms at an unpinned vertex; 3: those of them that hold
+// cubature records; 4: the others without; 5: every other, colored sweeps
+// only).
+// Bit k of a filter's mask selects the scenes of kind k (a set of the four
+// kKind bits): without cloth or contact 0x1111, with cubature 0xF0F0, with
+// tetrahedral terms 0xFF00.
+// A Jacobi sweep over a kind_blocks list visits only that kind's blocks.
+__device__ __forceinline__ bool kind_selected(const DeviceState& s, const PassParams& p, uint32_t sc) {
+  const uint32_t f = p.kind_filter;
+  if (f == 0u || !p.colored) return true;
+  static_assert(kKindCloth == 1u && kKindContact == 2u && kKindCubature == 4u && kKindTets == 8u, "kind bits");
+  const uint32_t m = f == 1u ? 0x1111u : f == 2u ? 0xEE00u : f == 3u ? 0xE0E0u : f == 4u ? 0x00EEu : 0xEEEEu;
+  return ((m >> (s.scene_kind[sc] & 15u)) & 1u) != 0u;
 }
 // Thread index of a Jacobi lane sweep over vertex blocks: with a kind
 // filter the launch covers only the blocks of that kind_blocks list
@@ -172,7 +179,8 @@ __device__ __forceinline__ uint32_t lane_sweep_gid(const DeviceState& s, const P
                                                    uint32_t threads, uint32_t lanes) {
   if (p.kind_filter == 0u || p.colored) return lb * threads + threadIdx.x;
   const uint32_t per = kSceneBlock * lanes / threads;
-  const uint32_t first = p.kind_filter == 1u ? 0u : p.kind_filter == 2u ? p.kind_blocks1 : p.kind_blocks1 + p.kind_blocks2;
+  const uint32_t f = p.kind_filter;
+  const uint32_t first = (f > 1u ? p.kind_blocks1 : 0u) + (f > 2u ? p.kind_blocks2 : 0u) + (f > 3u ? p.kind_blocks3 : 0u);
   return s.kind_blocks[first + lb / per] * (kSceneBlock * lanes) + (lb % per) * threads + threadIdx.x;
 }
 // Scene of a vertex (kNoScene in free arena blocks).
@@ -1743,7 +1751,7 @@ template <int LPV> __device__ __forceinline__ uint32_t lane_first(uint32_t sub, 
 // k_ipc_local_lanes (16 lanes) instead of the solve's own lanes.
 constexpr uint32_t kIpcLocalRows = 128;
 
-template <bool kCloth, bool kIpc, int LPV>
+template <bool kCloth, bool kIpc, int LPV, bool kTets = true>
 __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, uint32_t v, uint32_t sub) {
   const float4* pos = s.pos[p.src];
   const f3 x = load_pos(pos, v);
@@ -1781,12 +1789,14 @@ __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, u
     }
     prefix += n;
   }
+  if constexpr (kTets) {
 #pragma unroll 1
-  for (uint32_t k = lane_first<LPV>(sub, prefix); k < inf.y; k += LPV) {  // S:3097-3111
-    const Tet e = load_tet(s, s.adjacency[inf.x + k]);
-    const uint32_t slot = local_slot(e.ix, v);
-    if (slot >= 4) continue;
-    add_incident_block(snh_point(deformation_at(e, pos, v, x), e.lam, e.mu), e, shape_gradient(e, slot), 1.0f, L.g, L.H);
+    for (uint32_t k = lane_first<LPV>(sub, prefix); k < inf.y; k += LPV) {  // S:3097-3111
+      const Tet e = load_tet(s, s.adjacency[inf.x + k]);
+      const uint32_t slot = local_slot(e.ix, v);
+      if (slot >= 4) continue;
+      add_incident_block(snh_point(deformation_at(e, pos, v, x), e.lam, e.mu), e, shape_gradient(e, slot), 1.0f, L.g, L.H);
+    }
   }
   prefix += inf.y;
   if constexpr (kCloth) {
@@ -1824,7 +1834,8 @@ __device__ Local gather_local_lanes(const DeviceState& s, const PassParams& p, u
   // carry cloth or contact terms read them from k_cub_local_lanes (run by
   // launch_solve before the sweep at this source pose), which keeps their
   // registers within three waves per SIMD without spilling.
-  if constexpr (kCloth || kIpc) {
+  if constexpr (!kTets) {
+  } else if constexpr (kCloth || kIpc) {
     if (p.cub_local && sub == 0 && (s.scene_kind[vscene(s, v)] & kKindCubature)) {
       const float4 a = s.cub_local[3 * size_t(v)], b = s.cub_local[3 * size_t(v) + 1],
                    c = s.cub_local[3 * size_t(v) + 2];
@@ -1876,7 +1887,7 @@ __global__ void __launch_bounds__(kBlock) k_cub_local_lanes(DeviceState s, PassP
     v = lane;
   }
   const uint32_t sc = vscene(s, v);
-  if (sc == kNoScene || s.scene_lanes[sc] != p.solve_lanes || !kind_selected(s, p.kind_filter, sc) ||
+  if (sc == kNoScene || s.scene_lanes[sc] != p.solve_lanes || !kind_selected(s, p, sc) ||
       stop_latched(s, p, sc) || s.vtx_info[v].z != 0u)
     return;
   {  // the sweep's feasibility gate (k_solve_lanes)
@@ -1919,7 +1930,7 @@ struct TrialLanes {
   float energy, min_det;
   uint32_t fail;
 };
-template <bool kCloth, bool kIpc, int LPV>
+template <bool kCloth, bool kIpc, int LPV, bool kTets = true>
 __device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p, uint32_t v, uint32_t sub, f3 x, f3 d,
                                        float initial_min) {
   const float4* pos = s.pos[p.src];
@@ -2011,24 +2022,26 @@ __device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p
     }
     prefix += n;
   }
+  if constexpr (kTets) {
 #pragma unroll 1
-  for (uint32_t k = lane_first<LPV>(sub, prefix); k < inf.y; k += LPV) {  // S:2038-2061, S:2120-2139
-    const Tet e = load_tet(s, s.adjacency[inf.x + k]);
-    const f3 gr = shape_gradient(e, local_slot(e.ix, v));
-    const m3 F = deformation_at(e, pos, v, x);
-    const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
-    const float J = dot(F.c0, c12);
-    const f3 b = gr.x * c12 + gr.y * c20 + gr.z * c01;
-    const float jd = dot(d, b);
-    const float det_trial = J + jd;
-    if (!finite_s(det_trial)) {
-      tr.fail = 2u * (prefix + k) + 1u;
-      return tr;
+    for (uint32_t k = lane_first<LPV>(sub, prefix); k < inf.y; k += LPV) {  // S:2038-2061, S:2120-2139
+      const Tet e = load_tet(s, s.adjacency[inf.x + k]);
+      const f3 gr = shape_gradient(e, local_slot(e.ix, v));
+      const m3 F = deformation_at(e, pos, v, x);
+      const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
+      const float J = dot(F.c0, c12);
+      const f3 b = gr.x * c12 + gr.y * c20 + gr.z * c01;
+      const float jd = dot(d, b);
+      const float det_trial = J + jd;
+      if (!finite_s(det_trial)) {
+        tr.fail = 2u * (prefix + k) + 1u;
+        return tr;
+      }
+      tr.min_det = fminf(tr.min_det, det_trial);
+      const float id = 2.0f * dot(d, mul(F, gr)) + dot(d, d) * dot(gr, gr);
+      const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
+      tr.energy += e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
     }
-    tr.min_det = fminf(tr.min_det, det_trial);
-    const float id = 2.0f * dot(d, mul(F, gr)) + dot(d, d) * dot(gr, gr);
-    const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
-    tr.energy += e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
   }
   prefix += inf.y;
   if constexpr (kCloth) {
@@ -2062,73 +2075,75 @@ __device__ TrialLanes trial_lanes_part(const DeviceState& s, const PassParams& p
     }
     prefix += hr.y;
   }
-  const uint32_t k0 = lane_first<LPV>(sub, prefix);
-  if (k0 < p.cubature_k) {  // S:2272-2337
-    const float4* vc = vcub_source(s, p);
-    const f3 ds = tmul(load_vcub(vc, v).R, d);  // R_s^T d
-    const float n_src = float(inf.y > 1u ? inf.y : 1u);
-    const float w_src = s.vtx_rest[v].w * p.inv_dt2 / n_src;
+  if constexpr (kTets) {
+    const uint32_t k0 = lane_first<LPV>(sub, prefix);
+    if (k0 < p.cubature_k) {  // S:2272-2337
+      const float4* vc = vcub_source(s, p);
+      const f3 ds = tmul(load_vcub(vc, v).R, d);  // R_s^T d
+      const float n_src = float(inf.y > 1u ? inf.y : 1u);
+      const float w_src = s.vtx_rest[v].w * p.inv_dt2 / n_src;
 #pragma unroll 1
-    for (uint32_t k = k0; k < p.cubature_k; k += LPV) {
-      const size_t rec = size_t(v) * p.cubature_k + k;
-      const uint4 head = s.cub[10 * rec];
-      const float w = __uint_as_float(head.y);
-      if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
-      const Tet e = load_tet(s, head.x);
-      const float* ub = reinterpret_cast<const float*>(s.cub + 10 * rec + 1);
-      const f3 x0 = vcub_pose(vc, e.ix.x), x1 = vcub_pose(vc, e.ix.y), x2 = vcub_pose(vc, e.ix.z),
-               x3 = vcub_pose(vc, e.ix.w);
-      const m3 F = mul(m3{x1 - x0, x2 - x0, x3 - x0}, e.inv);
-      m3 Dd = mzero();
-      float compl_e = 0.0f;
-      int ss = 4;
+      for (uint32_t k = k0; k < p.cubature_k; k += LPV) {
+        const size_t rec = size_t(v) * p.cubature_k + k;
+        const uint4 head = s.cub[10 * rec];
+        const float w = __uint_as_float(head.y);
+        if (head.x == kEmptyTet || head.x >= p.tet_count || !(w > 0.0f)) continue;
+        const Tet e = load_tet(s, head.x);
+        const float* ub = reinterpret_cast<const float*>(s.cub + 10 * rec + 1);
+        const f3 x0 = vcub_pose(vc, e.ix.x), x1 = vcub_pose(vc, e.ix.y), x2 = vcub_pose(vc, e.ix.z),
+                 x3 = vcub_pose(vc, e.ix.w);
+        const m3 F = mul(m3{x1 - x0, x2 - x0, x3 - x0}, e.inv);
+        m3 Dd = mzero();
+        float compl_e = 0.0f;
+        int ss = 4;
 #pragma unroll 1
-      for (int l = 0; l < 4; ++l) {
-        const uint32_t vl = tet_vertex(e.ix, l);
-        if (vl == v) ss = l;
-        const VCub ql = load_vcub(vc, vl);
-        const f3 bd = mul(ql.R, ubar_times(ub, l, ds));  // B_l d
-        const f3 gl = shape_gradient(e, l);
-        Dd.c0 = Dd.c0 + gl.x * bd;
-        Dd.c1 = Dd.c1 + gl.y * bd;
-        Dd.c2 = Dd.c2 + gl.z * bd;
-        if (!ql.pinned) {
-          compl_e += ql.w * (dot(ql.diff, bd) + 0.5f * dot(bd, bd));
-          if (p.objective_flags != 0u) {
-            const uint4 il = s.vtx_info[vl];
-            const float n = float(il.y > 1u ? il.y : 1u);
-            compl_e += objective_delta(s, p, vl, ql.x, bd) / n;
+        for (int l = 0; l < 4; ++l) {
+          const uint32_t vl = tet_vertex(e.ix, l);
+          if (vl == v) ss = l;
+          const VCub ql = load_vcub(vc, vl);
+          const f3 bd = mul(ql.R, ubar_times(ub, l, ds));  // B_l d
+          const f3 gl = shape_gradient(e, l);
+          Dd.c0 = Dd.c0 + gl.x * bd;
+          Dd.c1 = Dd.c1 + gl.y * bd;
+          Dd.c2 = Dd.c2 + gl.z * bd;
+          if (!ql.pinned) {
+            compl_e += ql.w * (dot(ql.diff, bd) + 0.5f * dot(bd, bd));
+            if (p.objective_flags != 0u) {
+              const uint4 il = s.vtx_info[vl];
+              const float n = float(il.y > 1u ? il.y : 1u);
+              compl_e += objective_delta(s, p, vl, ql.x, bd) / n;
+            }
           }
         }
+        const float det_trial = mdet(madd(F, Dd));
+        if (!finite_s(det_trial)) {
+          tr.fail = 2u * (prefix + k) + 1u;
+          return tr;
+        }
+        tr.min_det = fminf(tr.min_det, det_trial);
+        compl_e += snh_delta_general(F, Dd, e.lam, e.mu) * e.vol;
+        if (ss < 4) {
+          const f3 gs = shape_gradient(e, ss);
+          const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
+          const float J = dot(F.c0, c12);
+          const f3 b = gs.x * c12 + gs.y * c20 + gs.z * c01;
+          const float jd = dot(d, b);
+          const float id = 2.0f * dot(d, mul(F, gs)) + dot(d, d) * dot(gs, gs);
+          const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
+          compl_e -= e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
+          compl_e -= quadratic_delta(x, pred, d, w_src);
+          if (p.objective_flags != 0u) compl_e -= objective_delta(s, p, v, x, d) / n_src;
+        }
+        tr.energy += w * compl_e;
       }
-      const float det_trial = mdet(madd(F, Dd));
-      if (!finite_s(det_trial)) {
-        tr.fail = 2u * (prefix + k) + 1u;
-        return tr;
-      }
-      tr.min_det = fminf(tr.min_det, det_trial);
-      compl_e += snh_delta_general(F, Dd, e.lam, e.mu) * e.vol;
-      if (ss < 4) {
-        const f3 gs = shape_gradient(e, ss);
-        const f3 c12 = cross(F.c1, F.c2), c20 = cross(F.c2, F.c0), c01 = cross(F.c0, F.c1);
-        const float J = dot(F.c0, c12);
-        const f3 b = gs.x * c12 + gs.y * c20 + gs.z * c01;
-        const float jd = dot(d, b);
-        const float id = 2.0f * dot(d, mul(F, gs)) + dot(d, d) * dot(gs, gs);
-        const float lr = log_one_plus(id / (frob(F, F) + 1.0f));
-        compl_e -= e.vol * (0.5f * e.mu * id - 0.75f * e.mu * jd + 0.5f * e.lam * jd * (2.0f * J + jd - 2.0f) - 0.5f * e.mu * lr);
-        compl_e -= quadratic_delta(x, pred, d, w_src);
-        if (p.objective_flags != 0u) compl_e -= objective_delta(s, p, v, x, d) / n_src;
-      }
-      tr.energy += w * compl_e;
     }
   }
   return tr;
 }
-template <bool kCloth, bool kIpc, int LPV>
+template <bool kCloth, bool kIpc, int LPV, bool kTets = true>
 __device__ Trial trial_lanes(const DeviceState& s, const PassParams& p, uint32_t v, uint32_t sub, f3 x, f3 d,
                              float initial_min) {
-  const TrialLanes part = trial_lanes_part<kCloth, kIpc, LPV>(s, p, v, sub, x, d, initial_min);
+  const TrialLanes part = trial_lanes_part<kCloth, kIpc, LPV, kTets>(s, p, v, sub, x, d, initial_min);
   const uint32_t fail = lanes_min_u<LPV>(part.fail);
   const float min_det = lanes_min<LPV>(part.min_det);
   const float energy = lanes_sum<LPV>(part.energy);
@@ -2173,8 +2188,11 @@ __global__ void __launch_bounds__(kBlock) k_vertex_polar_lanes(DeviceState s, Pa
 // Three waves per SIMD: the register budget drops from 177 to 168 VGPRs at the
 // price of a few spilled registers (32 B/lane of scratch), which measures about
 // 15% faster than two waves on both the mixed batch and the layered box; four
-// waves spill over 200 B/lane.
-template <bool kCloth, bool kIpc, int LPV>
+// waves spill over 200 B/lane. kTets false: the launch covers only scenes
+// whose unpinned vertices have no incident tetrahedra and no cubature
+// records, so those terms (zero there) are not compiled; the cloth and
+// contact form then fits 168 VGPRs without spilling.
+template <bool kCloth, bool kIpc, int LPV, bool kTets = true>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_solve_lanes(DeviceState s, PassParams p) {
   constexpr bool kEnergyInSolve = !kCloth && !kIpc;
@@ -2193,7 +2211,7 @@ k_solve_lanes(DeviceState s, PassParams p) {
   }
   // Every branch below is uniform over a vertex's LPV lanes.
   const uint32_t sc = vscene(s, v);
-  if (sc == kNoScene || s.scene_lanes[sc] != uint32_t(LPV) || !kind_selected(s, p.kind_filter, sc)) return;
+  if (sc == kNoScene || s.scene_lanes[sc] != uint32_t(LPV) || !kind_selected(s, p, sc)) return;
   float4* tgt = s.pos[p.tgt];
   if (stop_latched(s, p, sc)) {
     if (sub == 0) tgt[v] = s.pos[p.src][v];
@@ -2230,7 +2248,7 @@ k_solve_lanes(DeviceState s, PassParams p) {
       return;
     }
   }
-  const Local L = gather_local_lanes<kCloth, kIpc, LPV>(s, p, v, sub);
+  const Local L = gather_local_lanes<kCloth, kIpc, LPV, kTets>(s, p, v, sub);
   const float inertia = s.vtx_rest[v].w * p.inv_dt2;
   // Globalized Newton direction and line search (S:3265-3471).
   const SolveOut dr = globalization_solve(L.H, L.g, inertia);
@@ -2285,7 +2303,7 @@ k_solve_lanes(DeviceState s, PassParams p) {
         backtracks = b;
         break;
       }
-      const Trial trial = trial_lanes<kCloth, kIpc, LPV>(s, p, v, sub, x, td, initial_min);
+      const Trial trial = trial_lanes<kCloth, kIpc, LPV, kTets>(s, p, v, sub, x, td, initial_min);
       if (!trial.geometry_valid) {
         status = kNonfinite;
         min_trial = 0.0f;
@@ -4663,6 +4681,15 @@ void launch_vertex_polar(const DeviceState& s, const PassParams& p, hipStream_t 
     else if (ipc) hipLaunchKernelGGL((k_solve_lanes<false, true, LPV>), GRID, BLOCK, 0, st, s, P);  \
     else hipLaunchKernelGGL((k_solve_lanes<false, false, LPV>), GRID, BLOCK, 0, st, s, P);          \
   } while (0)
+// The same for scenes without tetrahedral terms at unpinned vertices, four
+// lanes per vertex.
+#define JGS2_SHEET_DISPATCH_P(GRID, BLOCK, P)                                                           \
+  do {                                                                                                  \
+    const bool cl = has_cloth(P), ipc = (P).contact != 0u;                                             \
+    if (cl && ipc) hipLaunchKernelGGL((k_solve_lanes<true, true, 4, false>), GRID, BLOCK, 0, st, s, P); \
+    else if (cl) hipLaunchKernelGGL((k_solve_lanes<true, false, 4, false>), GRID, BLOCK, 0, st, s, P);  \
+    else hipLaunchKernelGGL((k_solve_lanes<false, true, 4, false>), GRID, BLOCK, 0, st, s, P);          \
+  } while (0)
 
 // The local solve runs with the autotuned block size (64, 128 or 256) and,
 // on the stable Neo-Hookean path, the autotuned lanes per vertex (1, 4, 8
@@ -4689,10 +4716,15 @@ void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
                        : dim3(listed * (kSceneBlock * lpv / threads));
     };
     if (has_cloth(p) || p.contact != 0u) {
-      // A mixed batch: scenes without cloth or contact are swept by the
-      // tetrahedral variant first, the others by the full variant, whose
+      // A batch with cloth or contact: scenes without either are swept by
+      // the tetrahedral variant first, the others by the full variant, whose
       // cubature terms come from k_cub_local_lanes (eight lanes per vertex).
-      if ((p.solve_classes & 4u) && (p.solve_classes & 8u)) {
+      // Jacobi sweeps launch over per-kind block lists and give the scenes
+      // whose unpinned vertices have no tetrahedral or cubature terms (a
+      // sheet over a pinned collider) a form of the full variant compiled
+      // without those terms.
+      const bool mixed = (p.solve_classes & 4u) && (p.solve_classes & 8u);
+      if (mixed) {
         PassParams a = p;
         a.kind_filter = 1u;
         const dim3 ag = list_grid(p.kind_blocks1, block, lanes);
@@ -4701,8 +4733,20 @@ void launch_solve(const DeviceState& s, const PassParams& p, hipStream_t st) {
           else if (lanes == 8u) hipLaunchKernelGGL((k_solve_lanes<false, false, 8>), ag, dim3(block), 0, st, s, a);
           else hipLaunchKernelGGL((k_solve_lanes<false, false, 16>), ag, dim3(block), 0, st, s, a);
         }
+      }
+      if (!p.colored) {
+        PassParams w = p;
+        w.kind_filter = 4u;
+        const dim3 wg = list_grid(p.kind_blocks4, block, lanes);
+        if (wg.x > 0) {
+          if (lanes == 4u) JGS2_SHEET_DISPATCH_P(wg, dim3(block), w);
+          else if (lanes == 8u) JGS2_LANES_DISPATCH_P(8, wg, dim3(block), w);
+          else JGS2_LANES_DISPATCH_P(16, wg, dim3(block), w);
+        }
         q.kind_filter = 2u;
         qgrid = list_grid(p.kind_blocks2, block, lanes);
+      } else if (mixed) {
+        q.kind_filter = 5u;
       }
       if (p.cubature_k > 0u && s.cub_local) {
         PassParams c = q;
