@@ -37,6 +37,7 @@ struct PassParams {
   uint32_t solve_lanes;         // lanes per vertex of the local solve for scenes that use lanes (4, 8, 16)
   uint32_t solve_classes;       // bit 0: scenes solved with lanes resident; bit 1: scenes solved one lane per vertex
   uint32_t cloth_base;          // cloth element records hold this pass's source pose (k_cloth_base ran)
+  uint32_t target_records;      // energy pass: the records hold the target pose, the source energies in spare words
   uint32_t ipc_local;           // the solve's contact gather terms are in ipc_local (k_ipc_local_lanes ran)
   uint32_t cub_local;           // the lane sweep's cubature gather terms are in cub_local (k_cub_local_lanes ran)
   uint32_t kind_filter;         // lane sweep over: 0 every scene; 1-4 the scenes of that kind_blocks list
@@ -273,7 +274,10 @@ void launch_apply_candidate(const DeviceState& s, const PassParams& p, hipStream
 void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t stream);
 // Cloth element records at positions `which` (0 = posA, 1 = posB).
-void launch_cloth_base(const DeviceState& s, const PassParams& p, uint32_t which, hipStream_t stream);
+// mode 0: element records at positions `which`; 1: the same, first keeping
+// each element's energy (at the pose the records held) in a spare word;
+// 2: only the elements of scenes whose candidate was rejected.
+void launch_cloth_base(const DeviceState& s, const PassParams& p, uint32_t which, hipStream_t stream, uint32_t mode = 0);
 void launch_body_correction(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_finalize(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_clear_frame(const DeviceState& s, const PassParams& p, hipStream_t stream);

@@ -1401,9 +1401,13 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
       ps.ipc_local = contact ? 1u : 0u;
       record_solve(b, ps);
       if (globalization && cloth && !contact) {
+        // Element records move to the candidate (keeping the source
+        // energies), so the energy pair reads both from them.
+        launch_cloth_base(d, p, p.tgt, q, 1u);
         PassParams pe = p;
         pe.energy_after_sweep = 1;
         pe.cloth_base = 1;
+        pe.target_records = 1;
         launch_vertex_energy(d, pe, q);
       }
     }
@@ -1411,7 +1415,11 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
       launch_contact_step(d, p, q);
       if (globalization) {
         PassParams pe = p;
-        pe.cloth_base = (!gs && cloth) ? 1u : 0u;
+        if (!gs && cloth) {  // as above, at the candidate after the contact step
+          launch_cloth_base(d, p, p.tgt, q, 1u);
+          pe.cloth_base = 1;
+          pe.target_records = 1;
+        }
         launch_vertex_energy(d, pe, q);
       }
     }
@@ -1421,8 +1429,11 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
       if (contact) launch_promote_contact(d, p, q);
       PassParams pc = p;
       pc.compute_next_rot = (!gs && it + 1 < iterations) ? 1u : 0u;
-      if (!gs && cloth) {  // element records at the candidate: this pass and the next sweep read them
-        launch_cloth_base(d, p, p.tgt, q);
+      if (!gs && cloth) {
+        // The records hold the candidate; scenes that rejected it are back
+        // at the source pose and recompute theirs. This pass and the next
+        // sweep read them.
+        launch_cloth_base(d, p, p.tgt, q, 2u);
         pc.cloth_base = 1;
       }
       launch_convergence(d, pc, q);

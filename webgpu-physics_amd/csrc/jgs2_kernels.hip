@@ -680,8 +680,10 @@ __device__ __forceinline__ float bending_energy_at(const DeviceState& s, uint32_
 // Element records at a sweep's source pose, computed once per element by
 // k_cloth_base (p.cloth_base set) instead of once per incident vertex; the
 // same functions produce them, so they equal the inline evaluations.
-//   tri_base[3t]  {f0, energy}, {f1, 0}, {stress, 0}
-//   hinge_base[4h] {grad0, angle}, {grad1, valid}, {grad2, bending energy}, {grad3, 0}
+//   tri_base[3t]  {f0, energy}, {f1, source energy}, {stress, 0}
+//   hinge_base[4h] {grad0, angle}, {grad1, valid}, {grad2, bending energy}, {grad3, source energy}
+// The source energies are the energies the records held before a pass
+// moved them to the target pose (k_cloth_base mode 1), for the energy pass.
 // kRecords: the caller launches only with p.cloth_base set, so the inline
 // evaluation is not compiled into it (and does not raise its register count).
 template <bool kRecords = false>
@@ -1123,13 +1125,16 @@ __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const Pa
       const float tra = 2.0f * s.tri_area[tri];
       const float stretch = len3(tri_raw_normal(s, tri, pos, kNoVertex, mk(0, 0, 0))) / tra;
       if (!(tra > 0.0f) || !finite_s(stretch) || !(stretch > kDeterminantFloor)) return kF32Max;
-      // At the source pose the element records hold these energies.
-      e += is_target ? membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy : membrane_base(s, p, tri, pos).energy;
+      // At the source pose the element records hold these energies; after
+      // k_cloth_base mode 1 they hold the target's, the source's in a spare word.
+      if (p.target_records) e += s.tri_base[3 * size_t(tri) + (is_target ? 0 : 1)].w;
+      else e += is_target ? membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy : membrane_base(s, p, tri, pos).energy;
     }
     for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {
       const uint32_t h = s.hinge_elems[k];
       if (__float_as_uint(s.hinge_rest[h].z) != v) continue;
-      e += is_target ? bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0)) : bending_base(s, p, h, pos);
+      if (p.target_records) e += s.hinge_base[4 * size_t(h) + (is_target ? 2 : 3)].w;
+      else e += is_target ? bending_energy_at(s, h, pos, kNoVertex, mk(0, 0, 0)) : bending_base(s, p, h, pos);
     }
   }
   if constexpr (kIpc) {  // S:2643 with ipcCachedContactAt (S:450-460); summed by k_ipc_energy_lanes
@@ -2346,31 +2351,66 @@ k_solve_lanes(DeviceState s, PassParams p) {
 // Cloth element records (membrane state; dihedral angle, validity, four
 // corner gradients and bending energy) at positions `which`, once per
 // element, for the gathers and trials of the vertices that share it.
-__global__ void __launch_bounds__(kBlock) k_cloth_base(DeviceState s, PassParams p, uint32_t which, uint32_t tri_blocks) {
-  const float4* pos = s.pos[which];
-  if (blockIdx.x < tri_blocks) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= p.tri_count) return;
-    const uint32_t sc = s.rblock_scene[t / kSceneBlock];
-    if (sc == kNoScene || t >= s.scene_end[sc].z || stop_latched(s, p, sc)) return;
-    const Membrane m = membrane_at(s, t, pos, kNoVertex, mk(0, 0, 0));
-    float4* r = s.tri_base + 3 * size_t(t);
-    r[0] = make_float4(m.f0.x, m.f0.y, m.f0.z, m.energy);
-    r[1] = make_float4(m.f1.x, m.f1.y, m.f1.z, 0.0f);
-    r[2] = make_float4(m.stress.x, m.stress.y, m.stress.z, 0.0f);
-    return;
-  }
-  const uint32_t h = (blockIdx.x - tri_blocks) * kBlock + threadIdx.x;
-  if (h >= p.hinge_count) return;
+// Modes: 0 the records at `which`; 1 the same, each element's previous
+// energy first kept in its spare word (the records held the sweep's source
+// pose; latched scenes included, whose target equals their source); 2 the
+// elements of scenes whose candidate k_apply_candidate rejected (their
+// target was restored to the source pose the records held before mode 1).
+__device__ __forceinline__ void cloth_tri_record(const DeviceState& s, const PassParams& p, const float4* pos,
+                                                 uint32_t t, uint32_t mode) {
+  const uint32_t sc = s.rblock_scene[t / kSceneBlock];
+  if (sc == kNoScene || t >= s.scene_end[sc].z || (mode != 1u && stop_latched(s, p, sc))) return;
+  if (mode == 2u && s.control[4 * size_t(sc) + 1].x == 1.0f) return;
+  float4* r = s.tri_base + 3 * size_t(t);
+  const float se = mode == 1u ? r[0].w : 0.0f;
+  const Membrane m = membrane_at(s, t, pos, kNoVertex, mk(0, 0, 0));
+  r[0] = make_float4(m.f0.x, m.f0.y, m.f0.z, m.energy);
+  r[1] = make_float4(m.f1.x, m.f1.y, m.f1.z, se);
+  r[2] = make_float4(m.stress.x, m.stress.y, m.stress.z, 0.0f);
+}
+__device__ __forceinline__ void cloth_hinge_record(const DeviceState& s, const PassParams& p, const float4* pos,
+                                                   uint32_t h, uint32_t mode) {
   const uint4 ix = s.hinge_ix[h];
   if (ix.x >= p.vertex_count) return;  // an entry no resident scene owns
   const uint32_t sc = vscene(s, ix.x);
-  if (sc == kNoScene || stop_latched(s, p, sc)) return;
+  if (sc == kNoScene || (mode != 1u && stop_latched(s, p, sc))) return;
+  if (mode == 2u && s.control[4 * size_t(sc) + 1].x == 1.0f) return;
   float4* r = s.hinge_base + 4 * size_t(h);
+  const float se = mode == 1u ? r[2].w : 0.0f;
   const Dihedral d = dihedral_at(s, h, pos, kNoVertex, mk(0, 0, 0), 4u, r);
   r[0].w = d.angle;
   r[1].w = d.valid ? 1.0f : 0.0f;
   r[2].w = bending_energy_of(s, h, d);
+  r[3].w = se;
+}
+__global__ void __launch_bounds__(kBlock) k_cloth_base(DeviceState s, PassParams p, uint32_t which, uint32_t tri_blocks,
+                                                       uint32_t mode) {
+  const float4* pos = s.pos[which];
+  if (blockIdx.x < tri_blocks) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < p.tri_count) cloth_tri_record(s, p, pos, t, mode);
+    return;
+  }
+  const uint32_t h = (blockIdx.x - tri_blocks) * kBlock + threadIdx.x;
+  if (h < p.hinge_count) cloth_hinge_record(s, p, pos, h, mode);
+}
+// Mode 2 over a small grid: a block first scans the scenes' accept flags
+// and leaves when every scene accepted its candidate (the common case), so
+// a sweep without rejections pays one launch of kClothRejectBlocks blocks.
+constexpr uint32_t kClothRejectBlocks = 1024;
+__global__ void __launch_bounds__(kBlock) k_cloth_base_rejected(DeviceState s, PassParams p, uint32_t which) {
+  __shared__ uint32_t any;
+  if (threadIdx.x == 0) any = 0u;
+  __syncthreads();
+  for (uint32_t sc = threadIdx.x; sc < p.scene_count; sc += kBlock)  // resident scenes have vertices
+    if (s.scene_cnt[sc].x != 0u && s.control[4 * size_t(sc) + 1].x != 1.0f && !stop_latched(s, p, sc)) any = 1u;
+  __syncthreads();
+  if (!any) return;  // uniform per block
+  const float4* pos = s.pos[which];
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < p.tri_count; t += gridDim.x * kBlock)
+    cloth_tri_record(s, p, pos, t, 2u);
+  for (uint32_t h = blockIdx.x * kBlock + threadIdx.x; h < p.hinge_count; h += gridDim.x * kBlock)
+    cloth_hinge_record(s, p, pos, h, 2u);
 }
 
 // jgs2Solve (S:2992-3487), one thread per vertex (Jacobi) or per vertex of
@@ -4847,10 +4887,15 @@ void launch_convergence(const DeviceState& s, const PassParams& p, hipStream_t s
   hipLaunchKernelGGL(k_convergence_sums, dim3(vb), dim3(kBlock), 0, st, s, p, vb);
   hipLaunchKernelGGL(k_convergence_final, dim3(p.scene_count), dim3(kBlock), 0, st, s, p, vb);
 }
-void launch_cloth_base(const DeviceState& s, const PassParams& p, uint32_t which, hipStream_t st) {
+void launch_cloth_base(const DeviceState& s, const PassParams& p, uint32_t which, hipStream_t st, uint32_t mode) {
   const uint32_t tb = blocks_for(p.tri_count), hb = blocks_for(p.hinge_count);
   if (tb + hb == 0) return;
-  hipLaunchKernelGGL(k_cloth_base, dim3(tb + hb), dim3(kBlock), 0, st, s, p, which, tb);
+  if (mode == 2u) {
+    hipLaunchKernelGGL(k_cloth_base_rejected, dim3(std::min(tb + hb, kClothRejectBlocks)), dim3(kBlock), 0, st, s, p,
+                       which);
+    return;
+  }
+  hipLaunchKernelGGL(k_cloth_base, dim3(tb + hb), dim3(kBlock), 0, st, s, p, which, tb, mode);
 }
 void launch_copy_position(const DeviceState& s, const PassParams& p, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_position, dim3(blocks_for(p.vertex_count)), dim3(kBlock), 0, st, s, p);
