@@ -183,7 +183,8 @@ struct DeviceState {
   //   bp_rowcnt, bp_rowfill uint32[V]: candidate CSR construction
   //   bp_ctl    uint4[S]  {flags, VT count, total, needed total}
   //   bp_reach  float4[S] {reach of the last build, margin, builds, 0}
-  //   bp_motion uint4[2S] per-axis max |x - x_build| and |x - x_frame| (float bits)
+  //   bp_motion uint4[3S] per-axis max of x - x_build (order keys), max |x - x_frame|
+  //                        (float bits), min of x - x_build (order keys)
   //   bp_size   uint4[S]  {surface triangles, surface edges, candidate capacity,
   //                        candidates before the current build}
   //   bp_blk_v, bp_blk_e, bp_blk_r uint32[blocks]: block sums, then block

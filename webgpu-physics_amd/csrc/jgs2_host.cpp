@@ -834,7 +834,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
     TRYS(dalloc(&d.bp_reach, S, by, dry));
     TRYS(dalloc(&d.bp_size, S, by, dry));
     TRYS(dalloc(&d.bp_rows_at, S, by, dry));
-    TRYS(dalloc(&d.bp_motion, 2 * S, by, dry));
+    TRYS(dalloc(&d.bp_motion, 3 * S, by, dry));
     TRYS(dalloc(&d.bp_bodies, S, by, dry));
     TRYS(dalloc(&d.bp_bref, Bc, by, dry));
     TRYS(dalloc(&d.bp_bdev, Bc, by, dry));
@@ -866,7 +866,7 @@ jgs2_status batch_allocate(jgs2_batch* b, bool dry = false) {
     if (Sc) HIP_CHECK(hipMemcpyAsync(d.sblock_scene, no_scene.data(), 4 * (Sc / kSceneBlock), hipMemcpyHostToDevice, q));
     if (Ec) HIP_CHECK(hipMemcpyAsync(d.eblock_scene, no_scene.data(), 4 * (Ec / kSceneBlock), hipMemcpyHostToDevice, q));
     HIP_CHECK(hipMemsetAsync(d.bp_ctl, 0, S * sizeof(uint4), q));
-    HIP_CHECK(hipMemsetAsync(d.bp_motion, 0, 2 * S * sizeof(uint4), q));
+    HIP_CHECK(hipMemsetAsync(d.bp_motion, 0, 3 * S * sizeof(uint4), q));
   }
   HIP_CHECK(hipStreamSynchronize(q));
   b->va.init(Vc / kSceneBlock);
@@ -1169,13 +1169,15 @@ jgs2_status pack_into_arena(jgs2_batch* b, jgs2_solver* s, const jgs2_scene_inpu
     const uint4 size = make_uint4(s->NS, s->NE, s->bp ? NC : 0u, 0);
     const uint4 blk2 = make_uint4(uint32_t(s->sblk.at), uint32_t(s->sblk.at + s->sblk.n), uint32_t(s->eblk.at),
                                   uint32_t(s->eblk.at + s->eblk.n));
-    const uint4 zero2[2] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+    // Motion accumulators: maxima start at key 0, minima at the largest key.
+    const uint4 zero2[3] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0),
+                            make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu)};
     HIP_CHECK(hipMemcpyAsync(d.bp_ctl + sl, &ctl_bp, sizeof(uint4), hipMemcpyHostToDevice, q));
     HIP_CHECK(hipMemcpyAsync(d.bp_reach + sl, &reach, sizeof(float4), hipMemcpyHostToDevice, q));
     HIP_CHECK(hipMemcpyAsync(d.bp_size + sl, &size, sizeof(uint4), hipMemcpyHostToDevice, q));
     const uint32_t rows_at = uint32_t(s->cel.at);
     HIP_CHECK(hipMemcpyAsync(d.bp_rows_at + sl, &rows_at, sizeof(uint32_t), hipMemcpyHostToDevice, q));
-    HIP_CHECK(hipMemcpyAsync(d.bp_motion + 2 * size_t(sl), zero2, sizeof(zero2), hipMemcpyHostToDevice, q));
+    HIP_CHECK(hipMemcpyAsync(d.bp_motion + 3 * size_t(sl), zero2, sizeof(zero2), hipMemcpyHostToDevice, q));
     HIP_CHECK(hipMemcpyAsync(d.scene_blk2 + sl, &blk2, sizeof(uint4), hipMemcpyHostToDevice, q));
     // The body rule: up to 64 bodies, every surface primitive within one body.
     std::vector<uint32_t> bref(s->body.n, 0xffffffffu);  // no vertex

@@ -3875,10 +3875,16 @@ __global__ void __launch_bounds__(kBlock) k_bp_motion(DeviceState s, PassParams 
   const uint32_t v = vb * kSceneBlock + threadIdx.x;
   const bool real = v - s.scene_blk0[sc].x * kSceneBlock < s.scene_cnt[sc].x;
   const bool bodies = s.bp_bodies[sc].z != 0u;
-  uint32_t m[6] = {0, 0, 0, 0, 0, 0}, dev[3] = {0, 0, 0}, body = 0;
+  // m[0..2]: largest x - x_build per axis (order keys), m[3..5]: largest
+  // |x - x_frame| (float bits), m[6..8]: smallest x - x_build, kept as the
+  // complement of its key so that every entry reduces by a maximum.
+  uint32_t m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dev[3] = {0, 0, 0}, body = 0;
   if (real) {
     const f3 x = load_pos(s.pos[p.src], v), b = xyz(s.bp_x[v]), o = load_pos(s.old, v);
-    m[0] = bp_bits(x.x - b.x), m[1] = bp_bits(x.y - b.y), m[2] = bp_bits(x.z - b.z);
+    const f3 d = x - b;
+    const bool fin = finite3(d);  // a non-finite motion spans every axis
+    m[0] = f2key(fin ? d.x : INFINITY), m[1] = f2key(fin ? d.y : INFINITY), m[2] = f2key(fin ? d.z : INFINITY);
+    m[6] = ~f2key(fin ? d.x : -INFINITY), m[7] = ~f2key(fin ? d.y : -INFINITY), m[8] = ~f2key(fin ? d.z : -INFINITY);
     m[3] = bp_bits(x.x - o.x), m[4] = bp_bits(x.y - o.y), m[5] = bp_bits(x.z - o.z);
     if (bodies) {
       body = s.vtx_info[v].w;
@@ -3889,11 +3895,12 @@ __global__ void __launch_bounds__(kBlock) k_bp_motion(DeviceState s, PassParams 
     }
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) m[k] = wave_max_u(m[k]);
+  for (int k = 0; k < 9; ++k) m[k] = wave_max_u(m[k]);
   if (threadIdx.x % 64u == 0u) {
-    uint32_t* acc = reinterpret_cast<uint32_t*>(s.bp_motion + 2 * size_t(sc));
+    uint32_t* acc = reinterpret_cast<uint32_t*>(s.bp_motion + 3 * size_t(sc));
     for (int k = 0; k < 3; ++k) atomicMax(acc + k, m[k]);
     for (int k = 0; k < 3; ++k) atomicMax(acc + 4 + k, m[3 + k]);
+    for (int k = 0; k < 3; ++k) atomicMin(acc + 8 + k, ~m[6 + k]);
   }
   if (bodies) {  // uniform per block
     uint32_t* bd = reinterpret_cast<uint32_t*>(s.bp_bdev);
@@ -3940,11 +3947,15 @@ __device__ bool bp_bodies_keep(const DeviceState& s, const PassParams& p, uint32
   }
   return true;
 }
-// Per scene: rebuild now? Motion m since the build moves an AABB lower bound
-// by at most 2|m| (m the per-axis maxima), so the list stays complete while
-// 2|m| < reach - activation reach. A rebuild's reach keeps the margin past
-// the activation reach and covers every pair that was inside the barrier at
-// the frame start (the pairs with lagged friction).
+// Per scene: rebuild now? AABB lower bounds are invariant under a common
+// translation, so with every vertex's motion since the build inside
+// [lo_a, hi_a] per axis a, the gap of two boxes along a shrinks by at most
+// hi_a - lo_a and a lower bound by at most |hi - lo|: the list stays complete
+// while |hi - lo| < reach - activation reach (at most 2|m| with m the
+// per-axis largest |motion|; a sheet that falls or swings about its pins
+// spans far less than twice its largest motion). A rebuild's reach keeps the
+// margin past the activation reach and covers every pair that was inside
+// the barrier at the frame start (the pairs with lagged friction).
 template <bool kCheck>
 __global__ void __launch_bounds__(kBlock) k_bp_decide(DeviceState s, PassParams p, uint32_t carry) {
   // Work-list counts (k_bp_worklist) and the long-row count (k_bp_row_sort).
@@ -3953,17 +3964,18 @@ __global__ void __launch_bounds__(kBlock) k_bp_decide(DeviceState s, PassParams 
   if (sc >= p.scene_count) return;
   uint4 ctl = s.bp_ctl[sc];
   if (!(ctl.x & kBpScene) || s.scene_cnt[sc].z == 0u) return;
-  uint4* mot = s.bp_motion + 2 * size_t(sc);
-  const uint4 mb = mot[0], mf = mot[1];
+  uint4* mot = s.bp_motion + 3 * size_t(sc);
+  const uint4 mb = mot[0], mf = mot[1], ml = mot[2];
   mot[0] = make_uint4(0, 0, 0, 0);
   mot[1] = make_uint4(0, 0, 0, 0);
+  mot[2] = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
   const bool bounded = p.max_step > 0.0f;
   const float act = bounded ? p.ipc_dhat + 2.0f * (p.max_step + ipc_rounding_reach(p, s.scene_scale[sc])) : INFINITY;
   float4 r = s.bp_reach[sc];
   bool need = (ctl.x & kBpForce) != 0u;
   if (kCheck && !need && r.x != INFINITY) {
-    const float bx = __uint_as_float(mb.x), by = __uint_as_float(mb.y), bz = __uint_as_float(mb.z);
-    const float grow = 2.0f * sqrtf(bx * bx + by * by + bz * bz);
+    const float bx = key2f(mb.x) - key2f(ml.x), by = key2f(mb.y) - key2f(ml.y), bz = key2f(mb.z) - key2f(ml.z);
+    const float grow = sqrtf(bx * bx + by * by + bz * bz);
     need = !(grow < r.x - act - 1.0e-4f * act);
     if (need && s.bp_bodies[sc].z != 0u) need = !bp_bodies_keep(s, p, sc, r.x, act);
   }
