@@ -3894,9 +3894,18 @@ __global__ void __launch_bounds__(kBlock) k_bp_motion(DeviceState s, PassParams 
       dev[0] = bp_bits(e.x), dev[1] = bp_bits(e.y), dev[2] = bp_bits(e.z);
     }
   }
+  // Wavefront maxima, then the block's in LDS: one lane per block updates
+  // the scene's accumulators (every block of a large sheet hits the same
+  // nine words).
+  __shared__ uint32_t wm[kBlock / 64][9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) m[k] = wave_max_u(m[k]);
-  if (threadIdx.x % 64u == 0u) {
+  if (threadIdx.x % 64u == 0u)
+    for (int k = 0; k < 9; ++k) wm[threadIdx.x / 64u][k] = m[k];
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    for (uint32_t w = 1; w < kBlock / 64; ++w)
+      for (int k = 0; k < 9; ++k) m[k] = max(m[k], wm[w][k]);
     uint32_t* acc = reinterpret_cast<uint32_t*>(s.bp_motion + 3 * size_t(sc));
     for (int k = 0; k < 3; ++k) atomicMax(acc + k, m[k]);
     for (int k = 0; k < 3; ++k) atomicMax(acc + 4 + k, m[3 + k]);
