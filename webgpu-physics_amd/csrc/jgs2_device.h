@@ -150,7 +150,7 @@ struct DeviceState {
   float4* fric = nullptr;
   float4 *src_c = nullptr, *src_w = nullptr, *tgt_c = nullptr, *tgt_w = nullptr;
   float2* src_d = nullptr;  // per candidate: cached source distance (-1 invalid), lagged force
-  float* tgt_d = nullptr;   // per candidate: cached target distance (-1 invalid)
+  float2* tgt_d = nullptr;  // per candidate: cached target distance (-1 invalid), lagged force
   float4* contact_ctl = nullptr;
   uint2* cand_row = nullptr;
   uint32_t *cand_ids = nullptr, *active_count = nullptr, *active_ids = nullptr, *cand_meta = nullptr;
@@ -306,7 +306,10 @@ hipError_t diagnostics_scratch(int device, double** ptr);
 void launch_lag_contact(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_build_active_rows(const DeviceState& s, const PassParams& p, hipStream_t stream);
 void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t stream);
+// Call with the source and target cache arrays swapped (see k_promote_contact).
 void launch_promote_contact(const DeviceState& s, const PassParams& p, hipStream_t stream);
+// Source cache copied into the target array before the host swaps them back.
+void launch_contact_cache_home(const DeviceState& s, hipStream_t stream);
 uint32_t partial_blocks_needed(uint32_t vertices, uint32_t tets, uint32_t triangles, uint32_t candidates);
 size_t scene_conv_bytes(uint32_t scenes);
 // Broad-phase contact (jgs2_broad_phase.hip). `check` decides per scene

@@ -1350,6 +1350,14 @@ void record_solve(jgs2_batch* b, const PassParams& p) {
   launch_solve(b->dev, p, b->stream);
 }
 
+// The source and target contact cache arrays and summaries trade roles at a
+// promotion.
+void swap_contact_caches(DeviceState& d) {
+  std::swap(d.src_c, d.tgt_c);
+  std::swap(d.src_w, d.tgt_w);
+  std::swap(d.src_d, d.tgt_d);
+}
+
 // encodeFrame (jgs2-solver.ts:2207-2494), one launch per pass over every
 // resident scene.
 void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, bool stop) {
@@ -1365,6 +1373,7 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
   // Jacobi sweeps read friction rows from per-candidate records written by
   // the active-row pass at the sweep's source pose.
   base.fric_records = (contact && !gs) ? 1u : 0u;
+  uint32_t cache_swaps = 0;
   if (globalization) launch_clear_frame(d, base, q);
   launch_predict(d, base, q);
   if (contact) launch_broad_phase(d, base, true, false, q);  // lists complete at the frame start
@@ -1428,7 +1437,13 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
     if (globalization) {
       launch_reduce_candidate(d, p, q);
       launch_apply_candidate(d, p, q);
-      if (contact) launch_promote_contact(d, p, q);
+      if (contact) {
+        // Promotion by renaming: the validated target caches become the
+        // source arrays for every later pass of the frame.
+        swap_contact_caches(b->dev);
+        ++cache_swaps;
+        launch_promote_contact(d, p, q);
+      }
       PassParams pc = p;
       pc.compute_next_rot = (!gs && it + 1 < iterations) ? 1u : 0u;
       if (!gs && cloth) {
@@ -1440,6 +1455,10 @@ void enqueue_frame(jgs2_batch* b, const jgs2_settings& st, uint32_t iterations, 
       }
       launch_convergence(d, pc, q);
     }
+  }
+  if (cache_swaps % 2 == 1) {  // the frame ends with the cache arrays in their roles at its start
+    launch_contact_cache_home(d, q);
+    swap_contact_caches(b->dev);
   }
   if (iterations % 2 == 0) {
     PassParams p = base;

@@ -2625,7 +2625,7 @@ __global__ void __launch_bounds__(kBlock) k_ipc_energy_candidates(DeviceState s,
 #pragma unroll
   for (int which = 0; which < 2; ++which) {
     const bool target = which == 1 && use_target;
-    const float dq = target ? s.tgt_d[c] : q.x;
+    const float dq = target ? s.tgt_d[c].x : q.x;
     if (dq < 0.0f) {
       flags |= 1u << which;
     } else if (ipc_barrier_active(dq, p.ipc_dhat) || (s.ipc_mu > 0.0f && q.y > 0.0f)) {
@@ -3707,7 +3707,7 @@ __device__ __forceinline__ float4 validate_contact_step(const DeviceState& s, co
   // The target cache's distance as ipc_cached reads it (-1 when invalid):
   // ipc_store_cached keeps the contact exactly when this holds.
   const bool cached = k.valid && finite_s(k.distance) && k.distance >= 0.0f && finite3(k.normal) && finite4(k.weights);
-  s.tgt_d[c] = cached ? k.distance : -1.0f;
+  s.tgt_d[c] = make_float2(cached ? k.distance : -1.0f, s.src_d[c].y);
   const bool valid = k.valid && finite_s(k.distance) && k.distance > s.ipc_dmin;
   const float d = valid ? k.distance : 0.0f;
   return make_float4(1.0f, d, d, valid ? 1.0f : 0.0f);
@@ -3754,19 +3754,50 @@ __global__ void __launch_bounds__(kBlock) k_apply_contact_step(DeviceState s, Pa
   if (keep) s.bp_tval[v] = make_float4(x.x, x.y, x.z, 1.0f);
 }
 // promoteAcceptedContactCache (S:2695-2710).
-__global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassParams p) {
-  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= s.NC) return;
-  const uint32_t sc = cscene(s, c);
-  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
+// Launched after the host swapped the source and target cache arrays and
+// summaries (the validated target cache becomes the source without a copy;
+// the validation wrote each target summary with the lagged force, which is
+// unchanged within a frame). A scene that does not promote copies its
+// previous source cache back from what are now the target arrays (nothing
+// reads the target cache before the next validation rewrites it).
+// A bounded grid: each workgroup first scans the scenes and leaves when
+// every scene with candidates promoted (the common case), else strides over
+// the candidate blocks of the scenes that did not.
+constexpr uint32_t kPromoteGrid = 1024;
+__device__ __forceinline__ bool contact_promotes(const DeviceState& s, uint32_t sc) {
   const float alpha = s.contact_ctl[sc].w;
-  if (s.control[4 * size_t(sc) + 1].x == 1.0f && finite_s(alpha) && alpha > 0.0f) {
+  return s.control[4 * size_t(sc) + 1].x == 1.0f && finite_s(alpha) && alpha > 0.0f;
+}
+__global__ void __launch_bounds__(kBlock) k_promote_contact(DeviceState s, PassParams p) {
+  __shared__ uint32_t any;
+  if (threadIdx.x == 0) any = 0u;
+  __syncthreads();
+  for (uint32_t sc = threadIdx.x; sc < p.scene_count; sc += kBlock) {
+    const uint4 b1 = s.scene_blk1[sc];
+    if (b1.z != b1.w && s.scene_cnt[sc].x != 0u && !contact_promotes(s, sc)) any = 1u;
+  }
+  __syncthreads();
+  if (!any) return;  // uniform per block
+  for (uint32_t blk = blockIdx.x; blk < s.NC / kBlock; blk += gridDim.x) {
+    const uint32_t sc = s.cblock_scene[blk];
+    if (sc == kNoScene || contact_promotes(s, sc)) continue;  // uniform per candidate block
+    const uint32_t c = blk * kBlock + threadIdx.x;
+    if (c >= s.scene_end[sc].w) continue;
     s.src_c[c] = s.tgt_c[c];
     s.src_w[c] = s.tgt_w[c];
-    // The summary of the promoted cache: its distance; the lagged force
-    // (the summary's second word) is unchanged within a frame.
-    reinterpret_cast<float*>(s.src_d)[2 * size_t(c)] = s.tgt_d[c];
+    s.src_d[c] = s.tgt_d[c];
   }
+}
+// Copies every live candidate's source cache into the target array, which
+// the host then names the source again (a frame with an odd number of
+// promotions ends with the arrays in their original roles).
+__global__ void __launch_bounds__(kBlock) k_contact_cache_home(DeviceState s) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t sc = s.cblock_scene[blockIdx.x];
+  if (sc == kNoScene || c >= s.scene_end[sc].w) return;
+  s.tgt_c[c] = s.src_c[c];
+  s.tgt_w[c] = s.src_w[c];
+  s.tgt_d[c] = s.src_d[c];
 }
 
 // ------------------------------------------------------------------ broad-phase contact
@@ -4957,7 +4988,11 @@ void launch_contact_step(const DeviceState& s, const PassParams& p, hipStream_t 
 }
 void launch_promote_contact(const DeviceState& s, const PassParams& p, hipStream_t st) {
   if (s.NC == 0) return;
-  hipLaunchKernelGGL(k_promote_contact, dim3(blocks_for(s.NC)), dim3(kBlock), 0, st, s, p);
+  hipLaunchKernelGGL(k_promote_contact, dim3(std::min(blocks_for(s.NC), kPromoteGrid)), dim3(kBlock), 0, st, s, p);
+}
+void launch_contact_cache_home(const DeviceState& s, hipStream_t st) {
+  if (s.NC == 0) return;
+  hipLaunchKernelGGL(k_contact_cache_home, dim3(blocks_for(s.NC)), dim3(kBlock), 0, st, s);
 }
 // Broad-phase check point: decide per scene, then (for the scenes that
 // rebuild) AABBs, counts, offsets, records and vertex rows. Every kernel
