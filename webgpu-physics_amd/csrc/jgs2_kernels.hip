@@ -680,7 +680,7 @@ __device__ __forceinline__ float bending_energy_at(const DeviceState& s, uint32_
 // Element records at a sweep's source pose, computed once per element by
 // k_cloth_base (p.cloth_base set) instead of once per incident vertex; the
 // same functions produce them, so they equal the inline evaluations.
-//   tri_base[3t]  {f0, energy}, {f1, source energy}, {stress, 0}
+//   tri_base[3t]  {f0, energy}, {f1, source energy}, {stress, stretch test bits}
 //   hinge_base[4h] {grad0, angle}, {grad1, valid}, {grad2, bending energy}, {grad3, source energy}
 // The source energies are the energies the records held before a pass
 // moved them to the target pose (k_cloth_base mode 1), for the energy pass.
@@ -1092,6 +1092,14 @@ __device__ void ipc_local_contribution(const DeviceState& s, const PassParams& p
 #pragma unroll 1
   for (uint32_t i = begin; i < end; ++i) ipc_row_contribution(s, p, v, pos, s.active_ids[i], g, H);
 }
+// The signed relative area stretch test of an assembled energy
+// (S:2585-2600): the triangle's rest area is positive and its stretch at
+// `pos` finite and above the floor.
+__device__ __forceinline__ bool tri_stretch_ok(const DeviceState& s, uint32_t tri, const float4* pos) {
+  const float tra = 2.0f * s.tri_area[tri];
+  const float stretch = len3(tri_raw_normal(s, tri, pos, kNoVertex, mk(0, 0, 0))) / tra;
+  return (tra > 0.0f) && finite_s(stretch) && (stretch > kDeterminantFloor);
+}
 template <bool kCloth, bool kIpc>
 __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const PassParams& p, const float4* pos, uint32_t v,
                                                   bool is_target) {  // S:2557-2645
@@ -1122,13 +1130,17 @@ __device__ __forceinline__ float vertex_energy_at(const DeviceState& s, const Pa
     for (uint32_t k = s.tri_row[v].x, ke = k + s.tri_row[v].y; k < ke; ++k) {
       const uint32_t tri = s.tri_elems[k];
       if (s.tri_ix[tri].w != v) continue;
-      const float tra = 2.0f * s.tri_area[tri];
-      const float stretch = len3(tri_raw_normal(s, tri, pos, kNoVertex, mk(0, 0, 0))) / tra;
-      if (!(tra > 0.0f) || !finite_s(stretch) || !(stretch > kDeterminantFloor)) return kF32Max;
-      // At the source pose the element records hold these energies; after
-      // k_cloth_base mode 1 they hold the target's, the source's in a spare word.
-      if (p.target_records) e += s.tri_base[3 * size_t(tri) + (is_target ? 0 : 1)].w;
-      else e += is_target ? membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy : membrane_base(s, p, tri, pos).energy;
+      if (p.target_records) {
+        // After k_cloth_base mode 1 the records hold the target's energy and
+        // stretch test, the source's in spare words.
+        const float* r = reinterpret_cast<const float*>(s.tri_base + 3 * size_t(tri));
+        if (!(__float_as_uint(r[11]) & (is_target ? 1u : 2u))) return kF32Max;
+        e += r[is_target ? 3 : 7];
+        continue;
+      }
+      if (!tri_stretch_ok(s, tri, pos)) return kF32Max;
+      // At the source pose the element records hold these energies.
+      e += is_target ? membrane_at(s, tri, pos, kNoVertex, mk(0, 0, 0)).energy : membrane_base(s, p, tri, pos).energy;
     }
     for (uint32_t k = s.hinge_row[v].x, ke = k + s.hinge_row[v].y; k < ke; ++k) {
       const uint32_t h = s.hinge_elems[k];
@@ -2362,11 +2374,15 @@ __device__ __forceinline__ void cloth_tri_record(const DeviceState& s, const Pas
   if (sc == kNoScene || t >= s.scene_end[sc].z || (mode != 1u && stop_latched(s, p, sc))) return;
   if (mode == 2u && s.control[4 * size_t(sc) + 1].x == 1.0f) return;
   float4* r = s.tri_base + 3 * size_t(t);
+  // Word 7: the energy at the pose the records held; word 11: bit 0 the
+  // stretch test here, bit 1 the one at that pose.
   const float se = mode == 1u ? r[0].w : 0.0f;
+  const uint32_t sf = mode == 1u ? (__float_as_uint(r[2].w) & 1u) << 1 : 0u;
   const Membrane m = membrane_at(s, t, pos, kNoVertex, mk(0, 0, 0));
+  const uint32_t fl = sf | (tri_stretch_ok(s, t, pos) ? 1u : 0u);
   r[0] = make_float4(m.f0.x, m.f0.y, m.f0.z, m.energy);
   r[1] = make_float4(m.f1.x, m.f1.y, m.f1.z, se);
-  r[2] = make_float4(m.stress.x, m.stress.y, m.stress.z, 0.0f);
+  r[2] = make_float4(m.stress.x, m.stress.y, m.stress.z, __uint_as_float(fl));
 }
 __device__ __forceinline__ void cloth_hinge_record(const DeviceState& s, const PassParams& p, const float4* pos,
                                                    uint32_t h, uint32_t mode) {
