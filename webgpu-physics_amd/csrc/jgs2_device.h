@@ -154,7 +154,7 @@ struct DeviceState {
   float4* contact_ctl = nullptr;
   uint2* cand_row = nullptr;
   uint32_t *cand_ids = nullptr, *active_count = nullptr, *active_ids = nullptr, *cand_meta = nullptr;
-  uint32_t* cand_keep = nullptr;  // per candidate: kept in its vertices' active rows this iteration
+  uint8_t* cand_keep = nullptr;   // per candidate: kept in its vertices' active rows this iteration
   // Broad-phase contact: scenes whose candidates are the static superset's
   // pairs within AABB reach, rebuilt on the device when motion since the
   // last build could bring an unlisted pair into reach.

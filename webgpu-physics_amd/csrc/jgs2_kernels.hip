@@ -3585,7 +3585,7 @@ __global__ void __launch_bounds__(kBlock) k_candidate_keep(DeviceState s, PassPa
       include = !finite3(d) || !finite_s(dl) || dl > bound;
     }
   }
-  s.cand_keep[c] = include ? 1u : 0u;
+  s.cand_keep[c] = include ? uint8_t(1) : uint8_t(0);
   if (p.fric_records && s.ipc_mu > 0.0f && q.y > 0.0f) {  // a friction row: its record for this sweep
     const float4* src = s.pos[p.src];
     const uint4 ix = s.cand[c];
